@@ -1,0 +1,78 @@
+/*
+ * jxs_oracle -- CPU restatement of the SVT-JPEG-XS encode/decode hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker: only tests/, the
+ * smoke() entry point and bench.py's cpu_baseline leg may load it.  The
+ * product (svt-jpeg-xs_amd) never links, loads or calls it.
+ *
+ * Scope: the reference's default coding tools -- rate-control mode 1 (CBR per
+ * precinct, move padding), significance coding on/off, no vertical
+ * prediction, no separate sign coding, deadzone or uniform quantisation,
+ * Bw = 20, Fq = 8, one precinct column (Cw = 0).  Each function in the .c file
+ * cites the reference file:line it restates.
+ */
+#ifndef JXS_ORACLE_H
+#define JXS_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Colour formats use the reference's ColourFormat_t values (SvtJpegxs.h). */
+typedef struct jxso_params {
+    uint32_t width, height, depth, format;
+    uint32_t bpp_num, bpp_den;
+    uint32_t decomp_v, decomp_h;
+    uint32_t quantization; /* 0 deadzone, 1 uniform */
+    uint32_t slice_height;
+    uint32_t significance; /* coding_significance flag */
+} jxso_params_t;
+
+/* Fills defaults exactly as svt_jpeg_xs_encoder_load_default_parameters
+ * (EncHandle.c:490-532) for the fields above. */
+void jxso_default_params(jxso_params_t* p);
+
+/* Bytes of one codestream (Lcod) or 0 on invalid parameters. */
+uint32_t jxso_codestream_bytes(const jxso_params_t* p);
+
+/* Encodes one frame.  planes[c] / strides[c] (in samples) follow the
+ * reference's svt_jpeg_xs_image_buffer_t convention: for the packed format
+ * only planes[0]/strides[0] are used.  Returns 0 on success, the reference's
+ * SvtJxsErrorType_t value otherwise.  out must hold jxso_codestream_bytes. */
+int32_t jxso_encode(const jxso_params_t* p, const void* const planes[4], const uint32_t strides[4], uint8_t* out,
+                    uint32_t out_cap, uint32_t* out_len);
+
+/* Decodes one codestream into planar components (stride = component width).
+ * info receives width, height, depth, components and component sizes. */
+typedef struct jxso_image_info {
+    uint32_t width, height, depth, comps;
+    uint32_t comp_w[4], comp_h[4];
+} jxso_image_info_t;
+int32_t jxso_probe(const uint8_t* cs, uint32_t len, jxso_image_info_t* info);
+int32_t jxso_decode(const uint8_t* cs, uint32_t len, void* const planes[4], jxso_image_info_t* info);
+
+/* Stage dumps used by the tests to localise a mismatch (same encoder run):
+ *  coeff: all bands of all components, band-major, u16 sign-magnitude,
+ *         before quantisation (layout: comp, band, row, column);
+ *  gcli:  GCLI per group, same order with gcli_width columns;
+ *  qr:    per precinct (Q, R) pairs;
+ *  sizes: number of elements written to each array. */
+int32_t jxso_encode_dump(const jxso_params_t* p, const void* const planes[4], const uint32_t strides[4], uint16_t* coeff,
+                         uint8_t* gcli, uint8_t* qr, uint32_t sizes[3]);
+
+/* Binary search used by the rate control (BinarySearch.c:9-78), exported so
+ * the tests can check it against a linear search. */
+typedef struct jxso_bsearch {
+    int32_t beg, end, step, best, last;
+    uint8_t below;
+} jxso_bsearch_t;
+void jxso_bsearch_init(jxso_bsearch_t* b, uint32_t beg, uint32_t end, uint8_t below, uint32_t step);
+/* result: 0 begin, 1 too small, 2 too big, 3 out of range.
+ * returns 0 continue, 1 found, 2 error. */
+int32_t jxso_bsearch_next(jxso_bsearch_t* b, int32_t result, uint32_t* next);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

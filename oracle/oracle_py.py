@@ -1,0 +1,128 @@
+"""ctypes front end of the C oracle (oracle/jxs_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: loaded by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, never by the product.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "libjxs_oracle.so")
+REF_ENC = os.path.join(HERE, "_ref", "ref_enc")
+REF_DEC = os.path.join(HERE, "_ref", "ref_dec")
+
+
+class Params(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_uint32) for n in (
+        "width", "height", "depth", "format", "bpp_num", "bpp_den", "decomp_v", "decomp_h",
+        "quantization", "slice_height", "significance")]
+
+
+class ImageInfo(ctypes.Structure):
+    _fields_ = [("width", ctypes.c_uint32), ("height", ctypes.c_uint32), ("depth", ctypes.c_uint32),
+                ("comps", ctypes.c_uint32), ("comp_w", ctypes.c_uint32 * 4), ("comp_h", ctypes.c_uint32 * 4)]
+
+
+_lib = None
+
+
+def build():
+    """Compile the oracle with gcc into oracle/_build (always rebuilds)."""
+    os.makedirs(os.path.dirname(LIB_PATH), exist_ok=True)
+    src = os.path.join(HERE, "jxs_oracle.c")
+    tmp = LIB_PATH + ".tmp"
+    cmd = "gcc -O2 -fPIC -shared -o '%s' '%s'" % (tmp, src)
+    if os.system(cmd) != 0:
+        raise RuntimeError("oracle build failed: " + cmd)
+    os.replace(tmp, LIB_PATH)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = ctypes.CDLL(LIB_PATH)
+        L.jxso_default_params.argtypes = [ctypes.POINTER(Params)]
+        L.jxso_codestream_bytes.argtypes = [ctypes.POINTER(Params)]
+        L.jxso_codestream_bytes.restype = ctypes.c_uint32
+        L.jxso_encode.argtypes = [ctypes.POINTER(Params), ctypes.c_void_p * 4, ctypes.c_uint32 * 4, ctypes.c_void_p,
+                                  ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32)]
+        L.jxso_encode.restype = ctypes.c_int32
+        L.jxso_encode_dump.argtypes = [ctypes.POINTER(Params), ctypes.c_void_p * 4, ctypes.c_uint32 * 4, ctypes.c_void_p,
+                                       ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32 * 3]
+        L.jxso_encode_dump.restype = ctypes.c_int32
+        L.jxso_probe.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.POINTER(ImageInfo)]
+        L.jxso_probe.restype = ctypes.c_int32
+        L.jxso_decode.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p * 4, ctypes.POINTER(ImageInfo)]
+        L.jxso_decode.restype = ctypes.c_int32
+        _lib = L
+    return _lib
+
+
+def make_params(width, height, depth, fmt, bpp_num, bpp_den=1, decomp_v=2, decomp_h=5, quantization=0,
+                slice_height=16, significance=1):
+    p = Params()
+    lib().jxso_default_params(ctypes.byref(p))
+    p.width, p.height, p.depth, p.format = width, height, depth, fmt
+    p.bpp_num, p.bpp_den, p.decomp_v, p.decomp_h = bpp_num, bpp_den, decomp_v, decomp_h
+    p.quantization, p.slice_height, p.significance = quantization, slice_height, significance
+    return p
+
+
+def _plane_args(planes, fmt):
+    ptrs = (ctypes.c_void_p * 4)()
+    strides = (ctypes.c_uint32 * 4)()
+    for i, pl in enumerate(planes):
+        assert pl.flags["C_CONTIGUOUS"]
+        ptrs[i] = pl.ctypes.data
+        strides[i] = pl.shape[1]
+    return ptrs, strides
+
+
+def codestream_bytes(p):
+    return lib().jxso_codestream_bytes(ctypes.byref(p))
+
+
+def encode(p, planes):
+    """Returns (error_code, bytes)."""
+    n = codestream_bytes(p)
+    if n == 0:
+        return (int(np.int32(np.uint32(0x80001005))), b"")
+    out = np.zeros(n, dtype=np.uint8)
+    ptrs, strides = _plane_args(planes, p.format)
+    used = ctypes.c_uint32(0)
+    e = lib().jxso_encode(ctypes.byref(p), ptrs, strides, out.ctypes.data, n, ctypes.byref(used))
+    return e, out.tobytes()
+
+
+def encode_dump(p, planes, total_coeffs, total_gcli, rows):
+    coeff = np.zeros(total_coeffs, dtype=np.uint16)
+    gcli = np.zeros(total_gcli, dtype=np.uint8)
+    qr = np.zeros(2 * rows, dtype=np.uint8)
+    ptrs, strides = _plane_args(planes, p.format)
+    sizes = (ctypes.c_uint32 * 3)()
+    e = lib().jxso_encode_dump(ctypes.byref(p), ptrs, strides, coeff.ctypes.data, gcli.ctypes.data, qr.ctypes.data, sizes)
+    return e, coeff[:sizes[0]], gcli[:sizes[1]], qr[:sizes[2]]
+
+
+def decode(cs):
+    """Returns (error_code, [planes])."""
+    buf = np.frombuffer(cs, dtype=np.uint8).copy()
+    info = ImageInfo()
+    e = lib().jxso_probe(buf.ctypes.data, len(buf), ctypes.byref(info))
+    if e:
+        return e, []
+    dt = np.uint8 if info.depth <= 8 else np.uint16
+    planes = [np.zeros((info.comp_h[c], info.comp_w[c]), dtype=dt) for c in range(info.comps)]
+    ptrs = (ctypes.c_void_p * 4)()
+    for c, pl in enumerate(planes):
+        ptrs[c] = pl.ctypes.data
+    e = lib().jxso_decode(buf.ctypes.data, len(buf), ptrs, ctypes.byref(info))
+    return e, planes
+
+
+def have_reference():
+    return os.path.exists(REF_ENC) and os.path.exists(REF_DEC)

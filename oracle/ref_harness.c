@@ -1,0 +1,252 @@
+/*
+ * Test-infrastructure driver for the reference SVT-JPEG-XS library built from
+ * its own sources into oracle/_ref (see oracle/Makefile.ref).  It only calls
+ * the reference's public API (Source/API/SvtJpegxsEnc.h:100-142,
+ * Source/API/SvtJpegxsDec.h:59-148), always with use_cpu_flags = CPU_FLAGS_C.
+ *
+ * Encoder (HARNESS_ENCODER):
+ *   ref_enc in.raw out.jxs W H depth format bpp_num bpp_den decomp_v decomp_h frames threads [slice_height [quant [significance]]]
+ *   in.raw holds `frames` frames back to back; each frame is the planes of the
+ *   format (planar) or one interleaved plane (packed RGB), u8 for depth 8 and
+ *   u16 little-endian above.  Writes the concatenated codestreams and prints one
+ *   JSON line with fps and send->get latency percentiles (ms).
+ *
+ * Decoder (HARNESS_DECODER):
+ *   ref_dec in.jxs out.raw frames threads
+ *   Decodes `frames` concatenated codestreams and writes the decoded planes.
+ */
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+#include <pthread.h>
+#include "SvtJpegxsEnc.h"
+#include "SvtJpegxsDec.h"
+
+static double now_ms(void) {
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return ts.tv_sec * 1e3 + ts.tv_nsec * 1e-6;
+}
+
+static int cmp_d(const void* a, const void* b) {
+    double x = *(const double*)a, y = *(const double*)b;
+    return (x > y) - (x < y);
+}
+
+#ifdef HARNESS_ENCODER
+typedef struct {
+    svt_jpeg_xs_encoder_api_t* enc;
+    svt_jpeg_xs_frame_t* frames;
+    double* t_send;
+    uint32_t n;
+    int err;
+} sender_t;
+
+static void* sender(void* p) {
+    sender_t* s = (sender_t*)p;
+    for (uint32_t i = 0; i < s->n; i++) {
+        s->t_send[i] = now_ms();
+        SvtJxsErrorType_t e = svt_jpeg_xs_encoder_send_picture(s->enc, &s->frames[i], 1);
+        if (e) {
+            s->err = (int)e;
+            return NULL;
+        }
+    }
+    return NULL;
+}
+
+int main(int argc, char** argv) {
+    if (argc < 13) {
+        fprintf(stderr, "usage: ref_enc in out W H depth format bpp_num bpp_den V H frames threads [slice_h]\n");
+        return 2;
+    }
+    const char* in_name = argv[1];
+    const char* out_name = argv[2];
+    svt_jpeg_xs_encoder_api_t enc;
+    if (svt_jpeg_xs_encoder_load_default_parameters(SVT_JPEGXS_API_VER_MAJOR, SVT_JPEGXS_API_VER_MINOR, &enc))
+        return 3;
+    enc.source_width = (uint32_t)atoi(argv[3]);
+    enc.source_height = (uint32_t)atoi(argv[4]);
+    enc.input_bit_depth = (uint8_t)atoi(argv[5]);
+    enc.colour_format = (ColourFormat_t)atoi(argv[6]);
+    enc.bpp_numerator = (uint32_t)atoi(argv[7]);
+    enc.bpp_denominator = (uint32_t)atoi(argv[8]);
+    enc.ndecomp_v = (uint32_t)atoi(argv[9]);
+    enc.ndecomp_h = (uint32_t)atoi(argv[10]);
+    uint32_t nframes = (uint32_t)atoi(argv[11]);
+    enc.threads_num = (uint32_t)atoi(argv[12]);
+    if (argc > 13)
+        enc.slice_height = (uint32_t)atoi(argv[13]);
+    if (argc > 14)
+        enc.quantization = (uint32_t)atoi(argv[14]);
+    if (argc > 15)
+        enc.coding_significance = (uint32_t)atoi(argv[15]);
+    enc.use_cpu_flags = CPU_FLAGS_C;
+    enc.verbose = VERBOSE_ERRORS;
+
+    svt_jpeg_xs_image_config_t cfg;
+    uint32_t bytes_per_frame = 0;
+    if (svt_jpeg_xs_encoder_get_image_config(SVT_JPEGXS_API_VER_MAJOR, SVT_JPEGXS_API_VER_MINOR, &enc, &cfg, &bytes_per_frame))
+        return 4;
+    SvtJxsErrorType_t e = svt_jpeg_xs_encoder_init(SVT_JPEGXS_API_VER_MAJOR, SVT_JPEGXS_API_VER_MINOR, &enc);
+    if (e) {
+        fprintf(stderr, "init error 0x%x\n", (unsigned)e);
+        return 5;
+    }
+    uint32_t pix = enc.input_bit_depth <= 8 ? 1 : 2;
+    size_t frame_bytes = 0;
+    for (int c = 0; c < cfg.components_num; c++)
+        frame_bytes += cfg.components[c].byte_size;
+    FILE* fi = fopen(in_name, "rb");
+    if (!fi)
+        return 6;
+    uint8_t* raw = (uint8_t*)malloc(frame_bytes * nframes);
+    if (fread(raw, 1, frame_bytes * nframes, fi) != frame_bytes * nframes) {
+        fprintf(stderr, "short input\n");
+        return 7;
+    }
+    fclose(fi);
+    svt_jpeg_xs_frame_t* frames = (svt_jpeg_xs_frame_t*)calloc(nframes, sizeof(*frames));
+    uint8_t* bs = (uint8_t*)malloc((size_t)bytes_per_frame * nframes);
+    for (uint32_t f = 0; f < nframes; f++) {
+        size_t off = f * frame_bytes;
+        for (int c = 0; c < cfg.components_num; c++) {
+            frames[f].image.data_yuv[c] = raw + off;
+            frames[f].image.stride[c] = cfg.components[c].width * (cfg.format == COLOUR_FORMAT_PACKED_YUV444_OR_RGB ? 3 : 1);
+            frames[f].image.alloc_size[c] = cfg.components[c].byte_size;
+            off += cfg.components[c].byte_size;
+        }
+        if (cfg.format == COLOUR_FORMAT_PACKED_YUV444_OR_RGB) {
+            /* send_picture validates three planes (EncHandle.c:974-990); the packed
+             * frame is one interleaved plane, so all three name it */
+            for (int c = 1; c < 3; c++) {
+                frames[f].image.data_yuv[c] = frames[f].image.data_yuv[0];
+                frames[f].image.stride[c] = frames[f].image.stride[0];
+                frames[f].image.alloc_size[c] = frames[f].image.alloc_size[0];
+            }
+        }
+        frames[f].bitstream.buffer = bs + (size_t)f * bytes_per_frame;
+        frames[f].bitstream.allocation_size = bytes_per_frame;
+        frames[f].bitstream.used_size = 0;
+        frames[f].user_prv_ctx_ptr = (void*)(uintptr_t)(f + 1);
+    }
+    double* t_send = (double*)calloc(nframes, sizeof(double));
+    double* lat = (double*)calloc(nframes, sizeof(double));
+    sender_t s = {&enc, frames, t_send, nframes, 0};
+    double t0 = now_ms();
+    pthread_t th;
+    pthread_create(&th, NULL, sender, &s);
+    FILE* fo = fopen(out_name, "wb");
+    int rc = 0;
+    for (uint32_t f = 0; f < nframes; f++) {
+        svt_jpeg_xs_frame_t out;
+        e = svt_jpeg_xs_encoder_get_packet(&enc, &out, 1);
+        double t = now_ms();
+        if (e) {
+            fprintf(stderr, "encode error 0x%x on frame %u\n", (unsigned)e, f);
+            rc = 8;
+            break;
+        }
+        uint32_t idx = (uint32_t)(uintptr_t)out.user_prv_ctx_ptr - 1;
+        lat[f] = t - t_send[idx];
+        fwrite(out.bitstream.buffer, 1, out.bitstream.used_size, fo);
+    }
+    double t1 = now_ms();
+    pthread_join(th, NULL);
+    fclose(fo);
+    svt_jpeg_xs_encoder_close(&enc);
+    if (rc || s.err)
+        return rc ? rc : 9;
+    qsort(lat, nframes, sizeof(double), cmp_d);
+    uint32_t i50 = (nframes - 1) * 50 / 100, i99 = (nframes - 1) * 99 / 100;
+    printf("{\"frames\": %u, \"bytes_per_frame\": %u, \"fps\": %.3f, \"p50_ms\": %.3f, \"p99_ms\": %.3f, \"threads\": %u}\n",
+           nframes, bytes_per_frame, nframes * 1e3 / (t1 - t0), lat[i50], lat[i99], enc.threads_num);
+    free(raw);
+    free(bs);
+    free(frames);
+    free(t_send);
+    free(lat);
+    return 0;
+}
+#endif /*HARNESS_ENCODER*/
+
+#ifdef HARNESS_DECODER
+int main(int argc, char** argv) {
+    if (argc < 5) {
+        fprintf(stderr, "usage: ref_dec in.jxs out.raw frames threads\n");
+        return 2;
+    }
+    FILE* fi = fopen(argv[1], "rb");
+    if (!fi)
+        return 3;
+    fseek(fi, 0, SEEK_END);
+    size_t total = (size_t)ftell(fi);
+    fseek(fi, 0, SEEK_SET);
+    uint8_t* data = (uint8_t*)malloc(total);
+    if (fread(data, 1, total, fi) != total)
+        return 4;
+    fclose(fi);
+    uint32_t nframes = (uint32_t)atoi(argv[3]);
+    svt_jpeg_xs_decoder_api_t dec;
+    memset(&dec, 0, sizeof(dec));
+    dec.verbose = VERBOSE_ERRORS;
+    dec.threads_num = (uint32_t)atoi(argv[4]);
+    dec.use_cpu_flags = CPU_FLAGS_C;
+    dec.proxy_mode = proxy_mode_full;
+    uint32_t frame_size = 0;
+    SvtJxsErrorType_t e = svt_jpeg_xs_decoder_get_single_frame_size_with_proxy(data, total, NULL, &frame_size, 1, proxy_mode_full);
+    if (e || frame_size == 0)
+        return 5;
+    svt_jpeg_xs_image_config_t cfg;
+    e = svt_jpeg_xs_decoder_init(SVT_JPEGXS_API_VER_MAJOR, SVT_JPEGXS_API_VER_MINOR, &dec, data, frame_size, &cfg);
+    if (e) {
+        fprintf(stderr, "decoder init error 0x%x\n", (unsigned)e);
+        return 6;
+    }
+    uint32_t pix = cfg.bit_depth <= 8 ? 1 : 2;
+    svt_jpeg_xs_image_buffer_t img;
+    memset(&img, 0, sizeof(img));
+    for (int c = 0; c < cfg.components_num; c++) {
+        img.stride[c] = cfg.components[c].width;
+        img.alloc_size[c] = cfg.components[c].width * cfg.components[c].height * pix;
+        img.data_yuv[c] = malloc(img.alloc_size[c]);
+    }
+    FILE* fo = fopen(argv[2], "wb");
+    size_t off = 0;
+    double t0 = now_ms();
+    for (uint32_t f = 0; f < nframes; f++) {
+        uint32_t fs = 0;
+        e = svt_jpeg_xs_decoder_get_single_frame_size_with_proxy(data + off, total - off, NULL, &fs, 1, proxy_mode_full);
+        if (e || fs == 0 || off + fs > total) {
+            fprintf(stderr, "frame size error on frame %u\n", f);
+            return 7;
+        }
+        svt_jpeg_xs_frame_t in;
+        memset(&in, 0, sizeof(in));
+        in.bitstream.buffer = data + off;
+        in.bitstream.allocation_size = fs;
+        in.bitstream.used_size = fs;
+        in.image = img;
+        e = svt_jpeg_xs_decoder_send_frame(&dec, &in, 1);
+        if (e)
+            return 8;
+        svt_jpeg_xs_frame_t out;
+        e = svt_jpeg_xs_decoder_get_frame(&dec, &out, 1);
+        if (e) {
+            fprintf(stderr, "decode error 0x%x on frame %u\n", (unsigned)e, f);
+            return 9;
+        }
+        for (int c = 0; c < cfg.components_num; c++)
+            fwrite(out.image.data_yuv[c], 1, cfg.components[c].byte_size, fo);
+        off += fs;
+    }
+    double t1 = now_ms();
+    fclose(fo);
+    svt_jpeg_xs_decoder_close(&dec);
+    printf("{\"frames\": %u, \"fps\": %.3f, \"components\": %u, \"depth\": %u}\n", nframes, nframes * 1e3 / (t1 - t0),
+           cfg.components_num, cfg.bit_depth);
+    return 0;
+}
+#endif /*HARNESS_DECODER*/

@@ -1,0 +1,92 @@
+"""Seeded synthetic test frames for the JPEG XS hot path.
+
+Each plane is a sum of three seeded low-frequency 2D sinusoids (periods
+64..512 px, amplitude (2^d-1)/8 around mid-grey), overwritten by 20 seeded
+hard-edged rectangles, plus seeded Gaussian noise, rounded half away from
+zero and clipped to [0, 2^d-1].  Seeds are splitmix64(base ^ frame<<8 ^ plane).
+
+Arithmetic is float64 numpy on one machine image; tests pin the output by
+SHA-256 so any drift between hosts is caught before parity is claimed.
+"""
+import hashlib
+
+import numpy as np
+
+BASE_SEED = 0x4A5853
+MASK64 = (1 << 64) - 1
+
+# ColourFormat_t values of the reference API (Source/API/SvtJpegxs.h:47-61).
+FORMAT_YUV420 = 2
+FORMAT_YUV422 = 3
+FORMAT_YUV444 = 4
+FORMAT_PACKED444 = 21
+
+
+def splitmix64(x):
+    x = (x + 0x9E3779B97F4A7C15) & MASK64
+    z = x
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & MASK64
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & MASK64
+    return z ^ (z >> 31)
+
+
+def plane_seed(frame, plane, base=BASE_SEED):
+    return splitmix64(base ^ (frame << 8) ^ plane)
+
+
+def component_sizes(width, height, fmt):
+    """(w, h) of each component, as svt_jpeg_xs_encoder_get_image_config."""
+    if fmt == FORMAT_YUV422:
+        return [(width, height), (width // 2, height), (width // 2, height)]
+    if fmt == FORMAT_YUV420:
+        return [(width, height), (width // 2, height // 2), (width // 2, height // 2)]
+    if fmt in (FORMAT_YUV444, FORMAT_PACKED444):
+        return [(width, height)] * 3
+    raise ValueError("unsupported format %r" % fmt)
+
+
+def make_plane(w, h, depth, seed, sigma=4.0):
+    rng = np.random.Generator(np.random.PCG64(seed))
+    vmax = (1 << depth) - 1
+    amp = vmax / 8.0
+    mid = float(1 << (depth - 1))
+    xs = np.arange(w, dtype=np.float64)
+    ys = np.arange(h, dtype=np.float64)
+    v = np.full((h, w), mid, dtype=np.float64)
+    for _ in range(3):
+        px = int(rng.integers(64, 513))
+        py = int(rng.integers(64, 513))
+        phi = float(rng.uniform(0.0, 2.0 * np.pi))
+        ax = 2.0 * np.pi * xs / px
+        by = 2.0 * np.pi * ys / py + phi
+        # sin(ax + by) = sin(ax)cos(by) + cos(ax)sin(by): separable, two outer products
+        v += amp * (np.outer(np.cos(by), np.sin(ax)) + np.outer(np.sin(by), np.cos(ax)))
+    for _ in range(20):
+        rw = int(rng.integers(8, max(9, w // 4 + 1)))
+        rh = int(rng.integers(8, max(9, h // 4 + 1)))
+        x0 = int(rng.integers(0, max(1, w - rw + 1)))
+        y0 = int(rng.integers(0, max(1, h - rh + 1)))
+        val = float(rng.integers(0, vmax + 1))
+        v[y0:y0 + rh, x0:x0 + rw] = val
+    v += rng.standard_normal((h, w)) * sigma
+    v = np.sign(v) * np.floor(np.abs(v) + 0.5)
+    np.clip(v, 0, vmax, out=v)
+    return v.astype(np.uint8 if depth <= 8 else np.uint16)
+
+
+def make_frame(width, height, depth, fmt, frame=0, base=BASE_SEED, sigma=4.0):
+    """Planes of one frame: a list of 2D arrays (planar formats) or a single
+    (h, 3w) interleaved array for the packed format."""
+    sizes = component_sizes(width, height, fmt)
+    planes = [make_plane(w, h, depth, plane_seed(frame, c, base), sigma) for c, (w, h) in enumerate(sizes)]
+    if fmt == FORMAT_PACKED444:
+        return [np.ascontiguousarray(np.stack(planes, axis=-1).reshape(height, width * 3))]
+    return planes
+
+
+def frame_bytes(planes):
+    return b"".join(np.ascontiguousarray(p).tobytes() for p in planes)
+
+
+def frame_sha256(planes):
+    return hashlib.sha256(frame_bytes(planes)).hexdigest()
