@@ -1,0 +1,73 @@
+/*
+ * svt_jpegxs_amd.h -- entry points of the MI355X library beyond the
+ * reference API.  The reference has no counterpart for these: they serve
+ * callers that keep frames resident in HBM (batch encoders, the serving loop,
+ * benchmarks) and want one launch sequence for many frames.
+ *
+ * Device pointers are HIP device addresses; `stream` is a hipStream_t (NULL
+ * for the default stream).
+ */
+#ifndef SVT_JPEGXS_AMD_EXT_H
+#define SVT_JPEGXS_AMD_EXT_H
+
+#include <stdint.h>
+#include "SvtJpegxsEnc.h"
+#include "SvtJpegxsDec.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* One frame of a device-resident batch.  `encoder` is an initialised
+ * encoder whose parameters define the coding configuration of this frame
+ * (frames of one batch may use different encoders: resolution, format,
+ * depth, bpp and decomposition may differ).  Planes follow the reference's
+ * image-buffer convention (stride in samples; one interleaved plane for the
+ * packed format). */
+typedef struct svt_jpeg_xs_amd_frame_spec {
+    svt_jpeg_xs_encoder_api_t* encoder;
+    const void* d_planes[MAX_COMPONENTS_NUM];
+    uint32_t strides[MAX_COMPONENTS_NUM];
+    uint8_t* d_out; /* >= bytes_per_frame of the encoder */
+} svt_jpeg_xs_amd_frame_spec_t;
+
+typedef struct svt_jpeg_xs_amd_plan svt_jpeg_xs_amd_plan_t;
+
+/* Builds the job lists, frame descriptors and work buffers of a batch of
+ * `count` frames and keeps them in HBM.  Returns NULL on error. */
+PREFIX_API svt_jpeg_xs_amd_plan_t* svt_jpeg_xs_amd_plan_create(const svt_jpeg_xs_amd_frame_spec_t* frames, uint32_t count);
+
+/* Enqueues the encode of every frame of the plan on `stream`: one launch per
+ * stage for the whole batch.  When `use_graph` is set the launch sequence is
+ * captured into a HIP graph on the first call and replayed afterwards. */
+PREFIX_API SvtJxsErrorType_t svt_jpeg_xs_amd_plan_run(svt_jpeg_xs_amd_plan_t* plan, void* stream, uint32_t use_graph);
+
+/* Same as plan_run without a graph, recording a HIP event around every
+ * kernel on `stream`; waits and returns the milliseconds of each stage in
+ * stage_ms[0..6] (vertical 1, rows 1, vertical 2, rows 2, rate table,
+ * select, pack). */
+PREFIX_API SvtJxsErrorType_t svt_jpeg_xs_amd_plan_run_timed(svt_jpeg_xs_amd_plan_t* plan, void* stream, float* stage_ms);
+
+/* After the stream has completed: per-frame error flags (0 = ok), count
+ * entries.  Returns SvtJxsErrorEncodeFrameError if any frame failed. */
+PREFIX_API SvtJxsErrorType_t svt_jpeg_xs_amd_plan_status(svt_jpeg_xs_amd_plan_t* plan, uint32_t* frame_errors);
+
+PREFIX_API void svt_jpeg_xs_amd_plan_destroy(svt_jpeg_xs_amd_plan_t* plan);
+
+/* Device bytes a plan of these frames allocates (inputs and outputs excluded). */
+PREFIX_API uint64_t svt_jpeg_xs_amd_plan_footprint(const svt_jpeg_xs_amd_frame_spec_t* frames, uint32_t count);
+
+/* Diagnostics: copies the work buffers of frame `frame` of a completed plan
+ * to host memory (any pointer may be NULL): coefficients (u16, band-major as
+ * laid out by the plan), GCLI (u8), and per precinct {q, r, total, pad,
+ * offset, err} (6 x u32). */
+PREFIX_API SvtJxsErrorType_t svt_jpeg_xs_amd_plan_debug_copy(svt_jpeg_xs_amd_plan_t* plan, uint32_t frame, uint16_t* coeff,
+                                                             uint8_t* gcli, uint32_t* prec);
+
+/* Version / build string of the library. */
+PREFIX_API const char* svt_jpeg_xs_amd_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,346 @@
+"""Python front end of the MI355X JPEG XS library (libsvtjpegxs_amd.so).
+
+Mirrors the reference C API (Source/API/SvtJpegxsEnc.h, SvtJpegxsDec.h):
+the ctypes structures below have the reference's layout, and the Encoder /
+Decoder classes call svt_jpeg_xs_encoder_* / svt_jpeg_xs_decoder_* exactly as
+the reference's own tools do (load defaults, init, send, get, close).  The
+Plan class drives the device-resident batch entry points of svt_jpegxs_amd.h.
+
+The package has no CPU fallback: importing it on a machine without the built
+library raises, and the GPU entry points fail loudly without a GPU.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libsvtjpegxs_amd.so")
+
+API_MAJOR, API_MINOR = 0, 10
+
+# SvtJxsErrorType_t values (Source/API/SvtJpegxs.h:130-159)
+ERR_NONE = 0
+ERR_BAD_PARAMETER = -0x7FFFEFFB          # 0x80001005
+ERR_INSUFFICIENT_RESOURCES = -0x7FFFF000  # 0x80001000
+ERR_EMPTY_QUEUE = -0x7FFFDFCD            # 0x80002033
+ERR_ENCODE_FRAME = -0x7FFFDFCB           # 0x80002035
+ERR_DEC_INVALID_POINTER = -0x7FFFD000    # 0x80003000
+ERR_DEC_INVALID_BITSTREAM = -0x7FFFCFFF  # 0x80003001
+ERR_DEC_TOO_SHORT = -0x7FFFCFFD          # 0x80003003
+
+FORMAT_YUV420, FORMAT_YUV422, FORMAT_YUV444, FORMAT_PACKED444 = 2, 3, 4, 21
+
+_CB = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_void_p)
+
+
+class EncoderApi(ctypes.Structure):
+    """svt_jpeg_xs_encoder_api_t (Source/API/SvtJpegxsEnc.h:18-93)."""
+    _fields_ = [
+        ("source_width", ctypes.c_uint32), ("source_height", ctypes.c_uint32),
+        ("input_bit_depth", ctypes.c_uint8), ("colour_format", ctypes.c_int32),
+        ("bpp_numerator", ctypes.c_uint32), ("bpp_denominator", ctypes.c_uint32),
+        ("ndecomp_v", ctypes.c_uint32), ("ndecomp_h", ctypes.c_uint32),
+        ("quantization", ctypes.c_uint32), ("slice_height", ctypes.c_uint32),
+        ("use_cpu_flags", ctypes.c_uint64), ("threads_num", ctypes.c_uint32),
+        ("cpu_profile", ctypes.c_uint8), ("print_bands_info", ctypes.c_uint32),
+        ("coding_signs_handling", ctypes.c_uint32), ("coding_significance", ctypes.c_uint32),
+        ("rate_control_mode", ctypes.c_uint32), ("coding_vertical_prediction_mode", ctypes.c_uint32),
+        ("verbose", ctypes.c_uint32),
+        ("callback_send_data_available", _CB), ("callback_send_data_available_context", ctypes.c_void_p),
+        ("callback_get_data_available", _CB), ("callback_get_data_available_context", ctypes.c_void_p),
+        ("slice_packetization_mode", ctypes.c_uint8), ("private_ptr", ctypes.c_void_p),
+        ("padding", ctypes.c_uint8 * 64)]
+
+
+class DecoderApi(ctypes.Structure):
+    """svt_jpeg_xs_decoder_api_t (Source/API/SvtJpegxsDec.h:16-57)."""
+    _fields_ = [
+        ("use_cpu_flags", ctypes.c_uint64), ("verbose", ctypes.c_uint32), ("threads_num", ctypes.c_uint32),
+        ("packetization_mode", ctypes.c_uint8), ("proxy_mode", ctypes.c_int32),
+        ("callback_send_data_available", _CB), ("callback_send_data_available_context", ctypes.c_void_p),
+        ("callback_get_data_available", _CB), ("callback_get_data_available_context", ctypes.c_void_p),
+        ("private_ptr", ctypes.c_void_p), ("padding", ctypes.c_uint8 * 64)]
+
+
+class _CompCfg(ctypes.Structure):
+    _fields_ = [("width", ctypes.c_uint32), ("height", ctypes.c_uint32), ("byte_size", ctypes.c_uint32)]
+
+
+class ImageConfig(ctypes.Structure):
+    """svt_jpeg_xs_image_config_t (Source/API/SvtJpegxs.h:73-84)."""
+    _fields_ = [("width", ctypes.c_uint32), ("height", ctypes.c_uint32), ("bit_depth", ctypes.c_uint8),
+                ("format", ctypes.c_int32), ("components_num", ctypes.c_uint8), ("components", _CompCfg * 4)]
+
+
+class ImageBuffer(ctypes.Structure):
+    _fields_ = [("data_yuv", ctypes.c_void_p * 4), ("stride", ctypes.c_uint32 * 4), ("alloc_size", ctypes.c_uint32 * 4),
+                ("release_ctx_ptr", ctypes.c_void_p), ("ready_to_release", ctypes.c_uint8)]
+
+
+class BitstreamBuffer(ctypes.Structure):
+    _fields_ = [("buffer", ctypes.c_void_p), ("allocation_size", ctypes.c_uint32), ("used_size", ctypes.c_uint32),
+                ("release_ctx_ptr", ctypes.c_void_p), ("ready_to_release", ctypes.c_uint8),
+                ("last_packet_in_frame", ctypes.c_uint8)]
+
+
+class Frame(ctypes.Structure):
+    """svt_jpeg_xs_frame_t (Source/API/SvtJpegxs.h:121-128)."""
+    _fields_ = [("image", ImageBuffer), ("bitstream", BitstreamBuffer), ("user_prv_ctx_ptr", ctypes.c_void_p)]
+
+
+class FrameSpec(ctypes.Structure):
+    """svt_jpeg_xs_amd_frame_spec_t (include/svt_jpegxs_amd.h)."""
+    _fields_ = [("encoder", ctypes.POINTER(EncoderApi)), ("d_planes", ctypes.c_void_p * 4),
+                ("strides", ctypes.c_uint32 * 4), ("d_out", ctypes.c_void_p)]
+
+
+# Every C-ABI symbol the library exports (include/*.h).
+EXPORTED_SYMBOLS = [
+    "svt_jpeg_xs_encoder_load_default_parameters", "svt_jpeg_xs_encoder_get_image_config",
+    "svt_jpeg_xs_encoder_init", "svt_jpeg_xs_encoder_close", "svt_jpeg_xs_encoder_send_picture",
+    "svt_jpeg_xs_encoder_get_packet",
+    "svt_jpeg_xs_decoder_init", "svt_jpeg_xs_decoder_close", "svt_jpeg_xs_decoder_get_single_frame_size",
+    "svt_jpeg_xs_decoder_get_single_frame_size_with_proxy", "svt_jpeg_xs_decoder_send_frame",
+    "svt_jpeg_xs_decoder_send_packet", "svt_jpeg_xs_decoder_get_frame", "svt_jpeg_xs_decoder_send_eoc",
+    "svt_jpeg_xs_image_buffer_alloc", "svt_jpeg_xs_image_buffer_free", "svt_jpeg_xs_bitstream_alloc",
+    "svt_jpeg_xs_bitstream_free", "svt_jpeg_xs_frame_pool_alloc", "svt_jpeg_xs_frame_pool_free",
+    "svt_jpeg_xs_frame_pool_get", "svt_jpeg_xs_frame_pool_release",
+    "svt_jpeg_xs_amd_plan_create", "svt_jpeg_xs_amd_plan_run", "svt_jpeg_xs_amd_plan_run_timed",
+    "svt_jpeg_xs_amd_plan_status", "svt_jpeg_xs_amd_plan_destroy", "svt_jpeg_xs_amd_plan_footprint",
+    "svt_jpeg_xs_amd_version",
+    "svt_jpeg_xs_amd_decode_plan_create", "svt_jpeg_xs_amd_decode_plan_run", "svt_jpeg_xs_amd_decode_plan_status",
+    "svt_jpeg_xs_amd_decode_plan_destroy",
+]
+
+_lib = None
+
+
+def lib():
+    """Loads the library; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libsvtjpegxs_amd.so is not built: run svt-jpeg-xs_amd/build.py")
+        L = ctypes.CDLL(LIB_PATH)
+        E = ctypes.POINTER(EncoderApi)
+        D = ctypes.POINTER(DecoderApi)
+        L.svt_jpeg_xs_encoder_load_default_parameters.argtypes = [ctypes.c_uint64, ctypes.c_uint64, E]
+        L.svt_jpeg_xs_encoder_get_image_config.argtypes = [ctypes.c_uint64, ctypes.c_uint64, E,
+                                                           ctypes.POINTER(ImageConfig), ctypes.POINTER(ctypes.c_uint32)]
+        L.svt_jpeg_xs_encoder_init.argtypes = [ctypes.c_uint64, ctypes.c_uint64, E]
+        L.svt_jpeg_xs_encoder_close.argtypes = [E]
+        L.svt_jpeg_xs_encoder_close.restype = None
+        L.svt_jpeg_xs_encoder_send_picture.argtypes = [E, ctypes.POINTER(Frame), ctypes.c_uint8]
+        L.svt_jpeg_xs_encoder_get_packet.argtypes = [E, ctypes.POINTER(Frame), ctypes.c_uint8]
+        if hasattr(L, "svt_jpeg_xs_decoder_init"):
+            L.svt_jpeg_xs_decoder_init.argtypes = [ctypes.c_uint64, ctypes.c_uint64, D, ctypes.c_void_p,
+                                                   ctypes.c_size_t, ctypes.POINTER(ImageConfig)]
+            L.svt_jpeg_xs_decoder_close.argtypes = [D]
+            L.svt_jpeg_xs_decoder_close.restype = None
+            L.svt_jpeg_xs_decoder_get_single_frame_size_with_proxy.argtypes = [
+                ctypes.c_void_p, ctypes.c_size_t, ctypes.POINTER(ImageConfig), ctypes.POINTER(ctypes.c_uint32),
+                ctypes.c_uint32, ctypes.c_int32]
+            L.svt_jpeg_xs_decoder_send_frame.argtypes = [D, ctypes.POINTER(Frame), ctypes.c_uint8]
+            L.svt_jpeg_xs_decoder_get_frame.argtypes = [D, ctypes.POINTER(Frame), ctypes.c_uint8]
+        L.svt_jpeg_xs_amd_plan_create.argtypes = [ctypes.POINTER(FrameSpec), ctypes.c_uint32]
+        L.svt_jpeg_xs_amd_plan_create.restype = ctypes.c_void_p
+        L.svt_jpeg_xs_amd_plan_run.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32]
+        L.svt_jpeg_xs_amd_plan_run_timed.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_float)]
+        L.svt_jpeg_xs_amd_plan_status.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint32)]
+        L.svt_jpeg_xs_amd_plan_destroy.argtypes = [ctypes.c_void_p]
+        L.svt_jpeg_xs_amd_plan_destroy.restype = None
+        L.svt_jpeg_xs_amd_plan_footprint.argtypes = [ctypes.POINTER(FrameSpec), ctypes.c_uint32]
+        L.svt_jpeg_xs_amd_plan_footprint.restype = ctypes.c_uint64
+        L.svt_jpeg_xs_amd_version.restype = ctypes.c_char_p
+        for name in ("svt_jpeg_xs_encoder_load_default_parameters", "svt_jpeg_xs_encoder_get_image_config",
+                     "svt_jpeg_xs_encoder_init", "svt_jpeg_xs_encoder_send_picture", "svt_jpeg_xs_encoder_get_packet",
+                     "svt_jpeg_xs_decoder_init", "svt_jpeg_xs_decoder_get_single_frame_size_with_proxy",
+                     "svt_jpeg_xs_decoder_send_frame", "svt_jpeg_xs_decoder_get_frame", "svt_jpeg_xs_amd_plan_run",
+                     "svt_jpeg_xs_amd_plan_run_timed", "svt_jpeg_xs_amd_plan_status"):
+            if hasattr(L, name):
+                getattr(L, name).restype = ctypes.c_int32
+        _lib = L
+    return _lib
+
+
+def missing_symbols():
+    L = lib()
+    return [s for s in EXPORTED_SYMBOLS if not hasattr(L, s)]
+
+
+class JxsError(RuntimeError):
+    def __init__(self, code, what):
+        super().__init__("%s failed with 0x%08x" % (what, code & 0xFFFFFFFF))
+        self.code = code
+
+
+class Encoder:
+    """One encoder handle: svt_jpeg_xs_encoder_load_default_parameters, then
+    the given fields, then svt_jpeg_xs_encoder_init."""
+
+    def __init__(self, width, height, depth, fmt, bpp_num, bpp_den=1, decomp_v=2, decomp_h=5, quantization=0,
+                 slice_height=16, significance=1, verbose=1):
+        L = lib()
+        self.api = EncoderApi()
+        rc = L.svt_jpeg_xs_encoder_load_default_parameters(API_MAJOR, API_MINOR, ctypes.byref(self.api))
+        if rc:
+            raise JxsError(rc, "load_default_parameters")
+        a = self.api
+        a.source_width, a.source_height, a.input_bit_depth, a.colour_format = width, height, depth, fmt
+        a.bpp_numerator, a.bpp_denominator, a.ndecomp_v, a.ndecomp_h = bpp_num, bpp_den, decomp_v, decomp_h
+        a.quantization, a.slice_height, a.coding_significance, a.verbose = quantization, slice_height, significance, verbose
+        self.config = ImageConfig()
+        nb = ctypes.c_uint32(0)
+        rc = L.svt_jpeg_xs_encoder_get_image_config(API_MAJOR, API_MINOR, ctypes.byref(a), ctypes.byref(self.config),
+                                                    ctypes.byref(nb))
+        if rc:
+            raise JxsError(rc, "get_image_config")
+        self.bytes_per_frame = nb.value
+        rc = L.svt_jpeg_xs_encoder_init(API_MAJOR, API_MINOR, ctypes.byref(a))
+        if rc:
+            raise JxsError(rc, "encoder_init")
+        self.fmt = fmt
+        self.depth = depth
+
+    def close(self):
+        if self.api.private_ptr:
+            lib().svt_jpeg_xs_encoder_close(ctypes.byref(self.api))
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _frame(self, planes, out):
+        f = Frame()
+        pix = 1 if self.depth <= 8 else 2
+        for c, p in enumerate(planes):
+            assert p.flags["C_CONTIGUOUS"]
+            f.image.data_yuv[c] = p.ctypes.data
+            f.image.stride[c] = p.shape[1]
+            f.image.alloc_size[c] = p.nbytes
+        if self.fmt == FORMAT_PACKED444:
+            for c in (1, 2):
+                f.image.data_yuv[c] = f.image.data_yuv[0]
+                f.image.stride[c] = f.image.stride[0]
+                f.image.alloc_size[c] = f.image.alloc_size[0]
+        f.bitstream.buffer = out.ctypes.data
+        f.bitstream.allocation_size = out.nbytes
+        del pix
+        return f
+
+    def send(self, planes, out, ctx=None, blocking=1):
+        f = self._frame(planes, out)
+        f.user_prv_ctx_ptr = ctx
+        return lib().svt_jpeg_xs_encoder_send_picture(ctypes.byref(self.api), ctypes.byref(f), blocking)
+
+    def get(self, blocking=1):
+        f = Frame()
+        rc = lib().svt_jpeg_xs_encoder_get_packet(ctypes.byref(self.api), ctypes.byref(f), blocking)
+        return rc, f
+
+    def encode(self, planes):
+        """One frame through send_picture / get_packet; returns (rc, bytes)."""
+        out = np.zeros(self.bytes_per_frame, dtype=np.uint8)
+        rc = self.send(planes, out)
+        if rc:
+            return rc, b""
+        rc, f = self.get()
+        n = f.bitstream.used_size
+        return rc, out[:n].tobytes()
+
+
+class Decoder:
+    """svt_jpeg_xs_decoder_init on the first codestream, then send/get."""
+
+    def __init__(self, codestream, verbose=1):
+        L = lib()
+        self.api = DecoderApi()
+        self.api.verbose = verbose
+        self.config = ImageConfig()
+        buf = np.frombuffer(codestream, dtype=np.uint8)
+        rc = L.svt_jpeg_xs_decoder_init(API_MAJOR, API_MINOR, ctypes.byref(self.api), buf.ctypes.data, len(buf),
+                                        ctypes.byref(self.config))
+        if rc:
+            raise JxsError(rc, "decoder_init")
+
+    def close(self):
+        if self.api.private_ptr:
+            lib().svt_jpeg_xs_decoder_close(ctypes.byref(self.api))
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def decode(self, codestream):
+        cfg = self.config
+        dt = np.uint8 if cfg.bit_depth <= 8 else np.uint16
+        planes = [np.zeros((cfg.components[c].height, cfg.components[c].width), dtype=dt)
+                  for c in range(cfg.components_num)]
+        buf = np.frombuffer(codestream, dtype=np.uint8).copy()
+        f = Frame()
+        for c, p in enumerate(planes):
+            f.image.data_yuv[c] = p.ctypes.data
+            f.image.stride[c] = p.shape[1]
+            f.image.alloc_size[c] = p.nbytes
+        f.bitstream.buffer = buf.ctypes.data
+        f.bitstream.allocation_size = len(buf)
+        f.bitstream.used_size = len(buf)
+        rc = lib().svt_jpeg_xs_decoder_send_frame(ctypes.byref(self.api), ctypes.byref(f), 1)
+        if rc:
+            return rc, []
+        out = Frame()
+        rc = lib().svt_jpeg_xs_decoder_get_frame(ctypes.byref(self.api), ctypes.byref(out), 1)
+        return rc, planes
+
+
+class Plan:
+    """A device-resident batch (svt_jpeg_xs_amd_plan_*).  `frames` is a list
+    of (encoder, [device plane pointers], [strides], device output pointer)."""
+
+    def __init__(self, frames):
+        self.frames = frames
+        specs = (FrameSpec * len(frames))()
+        for i, (enc, planes, strides, out) in enumerate(frames):
+            specs[i].encoder = ctypes.pointer(enc.api)
+            for c, (p, s) in enumerate(zip(planes, strides)):
+                specs[i].d_planes[c] = p
+                specs[i].strides[c] = s
+            specs[i].d_out = out
+        self._specs = specs
+        self.footprint = lib().svt_jpeg_xs_amd_plan_footprint(specs, len(frames))
+        self.handle = lib().svt_jpeg_xs_amd_plan_create(specs, len(frames))
+        if not self.handle:
+            raise RuntimeError("svt_jpeg_xs_amd_plan_create failed")
+
+    def run(self, stream=0, use_graph=False):
+        rc = lib().svt_jpeg_xs_amd_plan_run(self.handle, stream, 1 if use_graph else 0)
+        if rc:
+            raise JxsError(rc, "plan_run")
+
+    def run_timed(self, stream=0):
+        ms = (ctypes.c_float * 7)()
+        rc = lib().svt_jpeg_xs_amd_plan_run_timed(self.handle, stream, ms)
+        if rc:
+            raise JxsError(rc, "plan_run_timed")
+        return list(ms)
+
+    def status(self):
+        errs = (ctypes.c_uint32 * len(self.frames))()
+        rc = lib().svt_jpeg_xs_amd_plan_status(self.handle, errs)
+        return rc, list(errs)
+
+    def destroy(self):
+        if self.handle:
+            lib().svt_jpeg_xs_amd_plan_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass

@@ -1,0 +1,711 @@
+/*
+ * jxs_encoder.cpp -- encoder runtime of the MI355X library: launch plans
+ * (job lists and frame descriptors resident in HBM), the plain
+ * send_picture / get_packet path, and the C-ABI entry points of
+ * SvtJpegxsEnc.h and svt_jpegxs_amd.h.
+ */
+#include <hip/hip_runtime.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <condition_variable>
+#include <deque>
+#include <mutex>
+#include <new>
+#include <vector>
+
+#include "SvtJpegxsEnc.h"
+#include "svt_jpegxs_amd.h"
+#include "jxs_host.h"
+
+extern "C" {
+hipError_t jxs_launch_vert(const JxsVertJob*, const uint32_t*, int, uint32_t, hipStream_t);
+hipError_t jxs_launch_horz(const JxsHorzJob*, const uint32_t*, int, uint32_t, hipStream_t);
+hipError_t jxs_launch_rc_table(const JxsFrameDesc*, int, uint32_t, hipStream_t);
+hipError_t jxs_launch_select(const JxsFrameDesc*, int, uint32_t, hipStream_t);
+hipError_t jxs_launch_pack(const JxsFrameDesc*, int, uint32_t, uint32_t, hipStream_t);
+hipError_t jxs_launch_zero(uint32_t*, uint32_t, hipStream_t);
+}
+
+namespace jxs {
+
+#define CHECK_HIP(x)                                                                        \
+    do {                                                                                    \
+        hipError_t e_ = (x);                                                                \
+        if (e_ != hipSuccess) {                                                             \
+            fprintf(stderr, "svt-jpeg-xs-amd: %s failed: %s\n", #x, hipGetErrorString(e_)); \
+            return SvtJxsErrorInsufficientResources;                                        \
+        }                                                                                   \
+    } while (0)
+
+static inline uint64_t align256(uint64_t x) { return (x + 255) & ~(uint64_t)255; }
+
+/* Geometry of one encoder, host copy plus resident device copy. */
+struct DevGeom {
+    HostGeom host;
+    JxsGeom* d_geom = nullptr;
+    uint32_t* d_slice_off = nullptr;
+    uint32_t* d_slice_bud = nullptr;
+    void release() {
+        if (d_geom)
+            (void)hipFree(d_geom);
+        if (d_slice_off)
+            (void)hipFree(d_slice_off);
+        if (d_slice_bud)
+            (void)hipFree(d_slice_bud);
+        d_geom = nullptr;
+        d_slice_off = d_slice_bud = nullptr;
+    }
+    SvtJxsErrorType_t upload() {
+        CHECK_HIP(hipMalloc(&d_geom, sizeof(JxsGeom)));
+        CHECK_HIP(hipMemcpy(d_geom, &host.g, sizeof(JxsGeom), hipMemcpyHostToDevice));
+        const size_t n = host.g.slices * sizeof(uint32_t);
+        CHECK_HIP(hipMalloc(&d_slice_off, n));
+        CHECK_HIP(hipMalloc(&d_slice_bud, n));
+        CHECK_HIP(hipMemcpy(d_slice_off, host.slice_off.data(), n, hipMemcpyHostToDevice));
+        CHECK_HIP(hipMemcpy(d_slice_bud, host.slice_bud.data(), n, hipMemcpyHostToDevice));
+        return SvtJxsErrorNone;
+    }
+};
+
+/* Work bytes of one frame of a geometry (inputs and outputs excluded). */
+static uint64_t frame_work_bytes(const JxsGeom& g) {
+    uint64_t b = 0;
+    b += align256(g.n_coeff * 2);
+    b += align256(g.n_gcli);
+    b += align256(g.n_smax);
+    b += align256(g.n_i32 * 4);
+    b += align256((uint64_t)g.rows * g.n_bl * JXS_LUT * 4);
+    b += align256((uint64_t)g.rows * JXS_NQ * JXS_NR * 4);
+    b += align256((uint64_t)g.rows * sizeof(JxsPrec));
+    return b;
+}
+
+struct EncoderPriv;
+
+}  // namespace jxs
+
+/* A launch plan: everything one encode launch sequence needs, in HBM. */
+struct svt_jpeg_xs_amd_plan {
+    uint32_t nfr = 0;
+    std::vector<const jxs::DevGeom*> geoms;
+    void* d_work = nullptr;
+    uint32_t* d_status = nullptr;
+    JxsFrameDesc* d_frames = nullptr;
+    JxsVertJob* d_vert[2] = {nullptr, nullptr};
+    uint32_t* d_vbase[2] = {nullptr, nullptr};
+    JxsHorzJob* d_horz[2] = {nullptr, nullptr};
+    uint32_t* d_hbase[2] = {nullptr, nullptr};
+    int nvert[2] = {0, 0}, nhorz[2] = {0, 0};
+    uint32_t vblocks[2] = {0, 0}, hblocks[2] = {0, 0};
+    uint32_t total_prec = 0, total_slices = 0, max_npk = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    hipStream_t graph_stream = nullptr;
+};
+
+namespace jxs {
+
+struct EncoderPriv {
+    DevGeom geom;
+    uint32_t verbose = 0;
+    /* plain path */
+    struct Slot {
+        void* d_in = nullptr;
+        uint8_t* d_out = nullptr;
+        svt_jpeg_xs_amd_plan_t* plan = nullptr;
+        hipEvent_t done = nullptr;
+        uint32_t* h_status = nullptr; /* pinned */
+        svt_jpeg_xs_frame_t frame;
+        bool busy = false;
+    };
+    std::vector<Slot> slots;
+    std::deque<int> pending; /* in send order */
+    std::mutex mu;
+    std::condition_variable cv_free, cv_pending;
+    hipStream_t stream = nullptr;
+    uint64_t in_bytes = 0;
+    size_t plane_off[4] = {0, 0, 0, 0};
+    uint32_t plane_stride[4] = {0, 0, 0, 0}; /* samples */
+    uint32_t plane_rows[4] = {0, 0, 0, 0};
+    uint32_t plane_row_bytes[4] = {0, 0, 0, 0};
+    uint32_t nplanes = 0;
+};
+
+static void plan_free(svt_jpeg_xs_amd_plan_t* p) {
+    if (!p)
+        return;
+    if (p->graph_exec)
+        (void)hipGraphExecDestroy(p->graph_exec);
+    if (p->graph)
+        (void)hipGraphDestroy(p->graph);
+    (void)hipFree(p->d_work);
+    (void)hipFree(p->d_status);
+    (void)hipFree(p->d_frames);
+    for (int s = 0; s < 2; s++) {
+        (void)hipFree(p->d_vert[s]);
+        (void)hipFree(p->d_vbase[s]);
+        (void)hipFree(p->d_horz[s]);
+        (void)hipFree(p->d_hbase[s]);
+    }
+    delete p;
+}
+
+template <typename T>
+static hipError_t upload_vec(const std::vector<T>& v, T** dst) {
+    *dst = nullptr;
+    if (v.empty())
+        return hipSuccess;
+    hipError_t e = hipMalloc(dst, v.size() * sizeof(T));
+    if (e != hipSuccess)
+        return e;
+    return hipMemcpy(*dst, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice);
+}
+
+static svt_jpeg_xs_amd_plan_t* plan_build(const svt_jpeg_xs_amd_frame_spec_t* specs, uint32_t count) {
+    if (!specs || count == 0)
+        return nullptr;
+    auto* p = new (std::nothrow) svt_jpeg_xs_amd_plan;
+    if (!p)
+        return nullptr;
+    p->nfr = count;
+    uint64_t work = 0;
+    for (uint32_t i = 0; i < count; i++) {
+        if (!specs[i].encoder || !specs[i].encoder->private_ptr || !specs[i].d_out) {
+            plan_free(p);
+            return nullptr;
+        }
+        const DevGeom* dg = &((EncoderPriv*)specs[i].encoder->private_ptr)->geom;
+        p->geoms.push_back(dg);
+        work += frame_work_bytes(dg->host.g);
+    }
+    if (hipMalloc(&p->d_work, work) != hipSuccess || hipMalloc(&p->d_status, count * sizeof(uint32_t)) != hipSuccess) {
+        plan_free(p);
+        return nullptr;
+    }
+    std::vector<JxsFrameDesc> frames(count);
+    std::vector<JxsVertJob> vj[2];
+    std::vector<JxsHorzJob> hj[2];
+    uint8_t* cur = (uint8_t*)p->d_work;
+    uint32_t prec_base = 0, slice_base = 0;
+    for (uint32_t i = 0; i < count; i++) {
+        const DevGeom* dg = p->geoms[i];
+        const JxsGeom& g = dg->host.g;
+        JxsFrameDesc& F = frames[i];
+        memset(&F, 0, sizeof(F));
+        F.g = dg->d_geom;
+        F.slice_off = dg->d_slice_off;
+        F.slice_bud = dg->d_slice_bud;
+        F.coeff = (uint16_t*)cur;
+        cur += align256(g.n_coeff * 2);
+        F.gcli = cur;
+        cur += align256(g.n_gcli);
+        F.smax = cur;
+        cur += align256(g.n_smax);
+        int32_t* scratch = (int32_t*)cur;
+        cur += align256(g.n_i32 * 4);
+        F.lut = (uint32_t*)cur;
+        cur += align256((uint64_t)g.rows * g.n_bl * JXS_LUT * 4);
+        F.table = (uint32_t*)cur;
+        cur += align256((uint64_t)g.rows * JXS_NQ * JXS_NR * 4);
+        F.prec = (JxsPrec*)cur;
+        cur += align256((uint64_t)g.rows * sizeof(JxsPrec));
+        F.out = specs[i].d_out;
+        F.status = p->d_status + i;
+        F.rows = g.rows;
+        F.npk = g.npk;
+        F.prec_base = prec_base;
+        F.slice_base = slice_base;
+        prec_base += g.rows;
+        slice_base += g.slices;
+        p->max_npk = g.npk > p->max_npk ? g.npk : p->max_npk;
+        FwdJobs jobs;
+        fwd_jobs(g, specs[i].d_planes, specs[i].strides, F.coeff, F.gcli, F.smax, scratch, jobs);
+        for (int s = 0; s < 2; s++) {
+            vj[s].insert(vj[s].end(), jobs.vert[s].begin(), jobs.vert[s].end());
+            hj[s].insert(hj[s].end(), jobs.horz[s].begin(), jobs.horz[s].end());
+        }
+    }
+    p->total_prec = prec_base;
+    p->total_slices = slice_base;
+    bool ok = upload_vec(frames, &p->d_frames) == hipSuccess;
+    for (int s = 0; s < 2 && ok; s++) {
+        std::vector<uint32_t> vb, hb;
+        uint32_t blk = 0;
+        for (auto& v : vj[s]) {
+            const uint32_t nl = (v.h + 1) / 2;
+            v.tiles_x = (v.w + 255) / 256;
+            v.tiles = v.tiles_x * ((nl + 15) / 16);
+            v.base = blk;
+            vb.push_back(blk);
+            blk += v.tiles;
+        }
+        p->vblocks[s] = blk;
+        blk = 0;
+        for (auto& h : hj[s]) {
+            h.base = blk;
+            hb.push_back(blk);
+            blk += h.rows;
+        }
+        p->hblocks[s] = blk;
+        p->nvert[s] = (int)vj[s].size();
+        p->nhorz[s] = (int)hj[s].size();
+        ok = upload_vec(vj[s], &p->d_vert[s]) == hipSuccess && upload_vec(vb, &p->d_vbase[s]) == hipSuccess &&
+            upload_vec(hj[s], &p->d_horz[s]) == hipSuccess && upload_vec(hb, &p->d_hbase[s]) == hipSuccess;
+    }
+    if (!ok) {
+        plan_free(p);
+        return nullptr;
+    }
+    return p;
+}
+
+/* The launch sequence of a plan.  ev (optional) gets 8 events recorded
+ * around the 7 stages. */
+static hipError_t plan_launch(svt_jpeg_xs_amd_plan_t* p, hipStream_t st, hipEvent_t* ev) {
+    hipError_t e;
+    const int nf = (int)p->nfr;
+#define STEP(k, call)                       \
+    do {                                    \
+        if (ev)                             \
+            (void)hipEventRecord(ev[k], st); \
+        e = (call);                         \
+        if (e != hipSuccess)                \
+            return e;                       \
+    } while (0)
+    if ((e = jxs_launch_zero(p->d_status, p->nfr, st)) != hipSuccess)
+        return e;
+    STEP(0, jxs_launch_vert(p->d_vert[0], p->d_vbase[0], p->nvert[0], p->vblocks[0], st));
+    STEP(1, jxs_launch_horz(p->d_horz[0], p->d_hbase[0], p->nhorz[0], p->hblocks[0], st));
+    STEP(2, jxs_launch_vert(p->d_vert[1], p->d_vbase[1], p->nvert[1], p->vblocks[1], st));
+    STEP(3, jxs_launch_horz(p->d_horz[1], p->d_hbase[1], p->nhorz[1], p->hblocks[1], st));
+    STEP(4, jxs_launch_rc_table(p->d_frames, nf, p->total_prec, st));
+    STEP(5, jxs_launch_select(p->d_frames, nf, p->total_slices, st));
+    STEP(6, jxs_launch_pack(p->d_frames, nf, p->max_npk, p->total_prec, st));
+    if (ev)
+        (void)hipEventRecord(ev[7], st);
+#undef STEP
+    return hipSuccess;
+}
+
+static EncoderPriv* priv_of(svt_jpeg_xs_encoder_api_t* api) {
+    return api ? (EncoderPriv*)api->private_ptr : nullptr;
+}
+
+static bool version_ok(uint64_t maj, uint64_t min) {
+    return !(maj > SVT_JPEGXS_API_VER_MAJOR || (maj == SVT_JPEGXS_API_VER_MAJOR && min > SVT_JPEGXS_API_VER_MINOR));
+}
+
+static void encoder_destroy(EncoderPriv* e) {
+    if (!e)
+        return;
+    if (e->stream)
+        (void)hipStreamSynchronize(e->stream);
+    for (auto& s : e->slots) {
+        plan_free(s.plan);
+        (void)hipFree(s.d_in);
+        (void)hipFree(s.d_out);
+        if (s.done)
+            (void)hipEventDestroy(s.done);
+        if (s.h_status)
+            (void)hipHostFree(s.h_status);
+    }
+    if (e->stream)
+        (void)hipStreamDestroy(e->stream);
+    e->geom.release();
+    delete e;
+}
+
+static const int kSlots = 3;
+
+static SvtJxsErrorType_t encoder_create(svt_jpeg_xs_encoder_api_t* api) {
+    EncParams p;
+    p.width = api->source_width;
+    p.height = api->source_height;
+    p.depth = api->input_bit_depth;
+    p.format = (uint32_t)api->colour_format;
+    p.bpp_num = api->bpp_numerator;
+    p.bpp_den = api->bpp_denominator;
+    p.decomp_v = api->ndecomp_v;
+    p.decomp_h = api->ndecomp_h;
+    p.quantization = api->quantization;
+    p.slice_height = api->slice_height;
+    p.significance = api->coding_significance;
+    /* Coding tools outside the default set have no GPU implementation yet. */
+    if (api->rate_control_mode != 1 || api->coding_vertical_prediction_mode != 0 || api->coding_signs_handling != 0) {
+        if (api->verbose >= VERBOSE_ERRORS)
+            fprintf(stderr,
+                    "svt-jpeg-xs-amd: only rate_control_mode 1 without vertical prediction or sign handling is "
+                    "implemented\n");
+        return SvtJxsErrorBadParameter;
+    }
+    if (api->slice_packetization_mode > 1 || api->cpu_profile > 1)
+        return SvtJxsErrorBadParameter;
+    auto* e = new (std::nothrow) EncoderPriv;
+    if (!e)
+        return SvtJxsErrorInsufficientResources;
+    e->verbose = api->verbose;
+    int32_t err = build_encoder_geometry(p, e->geom.host, api->verbose >= VERBOSE_ERRORS);
+    if (err) {
+        delete e;
+        return (SvtJxsErrorType_t)err;
+    }
+    if (api->slice_height >= api->source_height)
+        api->slice_height = api->source_height;
+    SvtJxsErrorType_t r = e->geom.upload();
+    if (r) {
+        encoder_destroy(e);
+        return r;
+    }
+    const JxsGeom& g = e->geom.host.g;
+    /* device input layout: tightly packed planes */
+    const uint32_t elem = g.depth <= 8 ? 1 : 2;
+    e->nplanes = g.packed ? 1 : g.nc;
+    uint64_t off = 0;
+    for (uint32_t c = 0; c < e->nplanes; c++) {
+        const uint32_t w = g.packed ? 3 * g.w : g.c[c].w, h = g.packed ? g.h : g.c[c].h;
+        e->plane_off[c] = off;
+        e->plane_stride[c] = w;
+        e->plane_rows[c] = h;
+        e->plane_row_bytes[c] = w * elem;
+        off = align256(off + (uint64_t)w * h * elem);
+    }
+    e->in_bytes = off;
+    if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess) {
+        encoder_destroy(e);
+        return SvtJxsErrorInsufficientResources;
+    }
+    api->private_ptr = e;
+    e->slots.resize(kSlots);
+    for (auto& s : e->slots) {
+        if (hipMalloc(&s.d_in, e->in_bytes) != hipSuccess || hipMalloc(&s.d_out, g.lcod) != hipSuccess ||
+            hipEventCreateWithFlags(&s.done, hipEventDisableTiming) != hipSuccess ||
+            hipHostMalloc((void**)&s.h_status, sizeof(uint32_t), 0) != hipSuccess) {
+            api->private_ptr = nullptr;
+            encoder_destroy(e);
+            return SvtJxsErrorInsufficientResources;
+        }
+        svt_jpeg_xs_amd_frame_spec_t spec;
+        memset(&spec, 0, sizeof(spec));
+        spec.encoder = api;
+        for (uint32_t c = 0; c < e->nplanes; c++) {
+            spec.d_planes[c] = (const uint8_t*)s.d_in + e->plane_off[c];
+            spec.strides[c] = e->plane_stride[c];
+        }
+        spec.d_out = s.d_out;
+        s.plan = plan_build(&spec, 1);
+        if (!s.plan) {
+            api->private_ptr = nullptr;
+            encoder_destroy(e);
+            return SvtJxsErrorInsufficientResources;
+        }
+    }
+    return SvtJxsErrorNone;
+}
+
+}  // namespace jxs
+
+using namespace jxs;
+
+extern "C" {
+
+PREFIX_API SvtJxsErrorType_t svt_jpeg_xs_encoder_load_default_parameters(uint64_t version_api_major,
+                                                                         uint64_t version_api_minor,
+                                                                         svt_jpeg_xs_encoder_api_t* enc_api) {
+    if (!version_ok(version_api_major, version_api_minor))
+        return SvtJxsErrorInvalidApiVersion;
+    if (!enc_api)
+        return SvtJxsErrorBadParameter;
+    memset(enc_api, 0, sizeof(*enc_api));
+    enc_api->colour_format = COLOUR_FORMAT_INVALID;
+    enc_api->bpp_denominator = 1;
+    enc_api->ndecomp_v = 2;
+    enc_api->ndecomp_h = 5;
+    enc_api->quantization = 0;
+    enc_api->slice_height = 16;
+    enc_api->use_cpu_flags = CPU_FLAGS_ALL;
+    enc_api->verbose = VERBOSE_SYSTEM_INFO;
+    enc_api->rate_control_mode = 1;
+    enc_api->coding_significance = 1;
+    return SvtJxsErrorNone;
+}
+
+PREFIX_API SvtJxsErrorType_t svt_jpeg_xs_encoder_get_image_config(uint64_t version_api_major, uint64_t version_api_minor,
+                                                                  svt_jpeg_xs_encoder_api_t* enc_api,
+                                                                  svt_jpeg_xs_image_config_t* cfg, uint32_t* out_bytes) {
+    if (!version_ok(version_api_major, version_api_minor))
+        return SvtJxsErrorInvalidApiVersion;
+    if (!enc_api || !cfg)
+        return SvtJxsErrorBadParameter;
+    cfg->width = enc_api->source_width;
+    cfg->height = enc_api->source_height;
+    cfg->bit_depth = enc_api->input_bit_depth;
+    cfg->format = enc_api->colour_format;
+    if (cfg->bit_depth < 8 || cfg->bit_depth > 14)
+        return SvtJxsErrorBadParameter;
+    uint32_t nc = 0, sx[4] = {1, 1, 1, 1}, sy[4] = {1, 1, 1, 1};
+    switch (cfg->format) {
+    case COLOUR_FORMAT_PLANAR_YUV422: nc = 3; sx[1] = sx[2] = 2; break;
+    case COLOUR_FORMAT_PLANAR_YUV420: nc = 3; sx[1] = sx[2] = sy[1] = sy[2] = 2; break;
+    case COLOUR_FORMAT_PLANAR_YUV444_OR_RGB:
+    case COLOUR_FORMAT_PACKED_YUV444_OR_RGB: nc = 3; break;
+    case COLOUR_FORMAT_PLANAR_4_COMPONENTS: nc = 4; break;
+    case COLOUR_FORMAT_GRAY: nc = 1; break;
+    default: return SvtJxsErrorBadParameter;
+    }
+    const uint32_t pix = cfg->bit_depth <= 8 ? 1 : 2;
+    if (cfg->format == COLOUR_FORMAT_PACKED_YUV444_OR_RGB) {
+        const uint64_t b = (uint64_t)enc_api->source_width * enc_api->source_height * 3 * pix;
+        if (b > UINT32_MAX)
+            return SvtJxsErrorBadParameter;
+        cfg->components_num = 1;
+        cfg->components[0].width = enc_api->source_width;
+        cfg->components[0].height = enc_api->source_height;
+        cfg->components[0].byte_size = (uint32_t)b;
+    } else {
+        cfg->components_num = (uint8_t)nc;
+        for (uint32_t c = 0; c < nc; c++) {
+            cfg->components[c].width = cfg->width >> (sx[c] - 1);
+            cfg->components[c].height = cfg->height >> (sy[c] - 1);
+            const uint64_t b = (uint64_t)cfg->components[c].width * cfg->components[c].height * pix;
+            if (b > UINT32_MAX)
+                return SvtJxsErrorBadParameter;
+            cfg->components[c].byte_size = (uint32_t)b;
+        }
+    }
+    if (out_bytes) {
+        if (enc_api->bpp_denominator == 0)
+            return SvtJxsErrorBadParameter;
+        *out_bytes = (uint32_t)(((uint64_t)enc_api->source_width * enc_api->source_height * enc_api->bpp_numerator /
+                                     enc_api->bpp_denominator +
+                                 7) /
+                                8);
+    }
+    return SvtJxsErrorNone;
+}
+
+PREFIX_API SvtJxsErrorType_t svt_jpeg_xs_encoder_init(uint64_t version_api_major, uint64_t version_api_minor,
+                                                      svt_jpeg_xs_encoder_api_t* enc_api) {
+    if (!version_ok(version_api_major, version_api_minor))
+        return SvtJxsErrorInvalidApiVersion;
+    if (!enc_api)
+        return SvtJxsErrorBadParameter;
+    enc_api->private_ptr = nullptr;
+    SvtJxsErrorType_t r = encoder_create(enc_api);
+    if (r)
+        return r;
+    EncoderPriv* e = priv_of(enc_api);
+    if (enc_api->callback_send_data_available)
+        enc_api->callback_send_data_available(enc_api, enc_api->callback_send_data_available_context);
+    (void)e;
+    return SvtJxsErrorNone;
+}
+
+PREFIX_API void svt_jpeg_xs_encoder_close(svt_jpeg_xs_encoder_api_t* enc_api) {
+    if (!enc_api || !enc_api->private_ptr)
+        return;
+    encoder_destroy(priv_of(enc_api));
+    enc_api->private_ptr = nullptr;
+}
+
+PREFIX_API SvtJxsErrorType_t svt_jpeg_xs_encoder_send_picture(svt_jpeg_xs_encoder_api_t* enc_api,
+                                                              svt_jpeg_xs_frame_t* in, uint8_t blocking_flag) {
+    EncoderPriv* e = priv_of(enc_api);
+    if (!e || !in)
+        return SvtJxsErrorBadParameter;
+    const JxsGeom& g = e->geom.host.g;
+    if (in->bitstream.allocation_size < g.lcod || !in->bitstream.buffer)
+        return SvtJxsErrorBadParameter;
+    const uint32_t pix = g.depth <= 8 ? 1 : 2;
+    /* plane checks as EncHandle.c:969-990 (all components are validated,
+     * also for the packed format) */
+    for (uint32_t c = 0; c < g.nc; c++) {
+        if (!in->image.data_yuv[c])
+            return SvtJxsErrorBadParameter;
+        const uint64_t min_size =
+            (uint64_t)in->image.stride[c] * pix * (g.c[c].h - 1) + (uint64_t)g.c[c].w * pix;
+        if (in->image.alloc_size[c] < min_size)
+            return SvtJxsErrorBadParameter;
+    }
+    int slot = -1;
+    {
+        std::unique_lock<std::mutex> lk(e->mu);
+        for (;;) {
+            for (int i = 0; i < (int)e->slots.size(); i++)
+                if (!e->slots[i].busy) {
+                    slot = i;
+                    break;
+                }
+            if (slot >= 0 || !blocking_flag)
+                break;
+            e->cv_free.wait(lk);
+        }
+        if (slot < 0)
+            return SvtJxsErrorNoErrorEmptyQueue;
+        e->slots[slot].busy = true;
+    }
+    EncoderPriv::Slot& s = e->slots[slot];
+    s.frame = *in;
+    hipError_t he = hipSuccess;
+    for (uint32_t c = 0; c < e->nplanes && he == hipSuccess; c++) {
+        const uint32_t src_pitch = in->image.stride[c] * pix;
+        he = hipMemcpy2DAsync((uint8_t*)s.d_in + e->plane_off[c], e->plane_row_bytes[c], in->image.data_yuv[c], src_pitch,
+                              e->plane_row_bytes[c], e->plane_rows[c], hipMemcpyHostToDevice, e->stream);
+    }
+    if (he == hipSuccess)
+        he = plan_launch(s.plan, e->stream, nullptr);
+    if (he == hipSuccess)
+        he = hipMemcpyAsync(in->bitstream.buffer, s.d_out, g.lcod, hipMemcpyDeviceToHost, e->stream);
+    if (he == hipSuccess)
+        he = hipMemcpyAsync(s.h_status, s.plan->d_status, sizeof(uint32_t), hipMemcpyDeviceToHost, e->stream);
+    if (he == hipSuccess)
+        he = hipEventRecord(s.done, e->stream);
+    std::lock_guard<std::mutex> lk(e->mu);
+    if (he != hipSuccess) {
+        s.busy = false;
+        e->cv_free.notify_one();
+        return SvtJxsErrorUndefined;
+    }
+    e->pending.push_back(slot);
+    e->cv_pending.notify_one();
+    return SvtJxsErrorNone;
+}
+
+PREFIX_API SvtJxsErrorType_t svt_jpeg_xs_encoder_get_packet(svt_jpeg_xs_encoder_api_t* enc_api,
+                                                            svt_jpeg_xs_frame_t* out, uint8_t blocking_flag) {
+    EncoderPriv* e = priv_of(enc_api);
+    if (!e || !out)
+        return SvtJxsErrorBadParameter;
+    int slot;
+    {
+        std::unique_lock<std::mutex> lk(e->mu);
+        if (e->pending.empty()) {
+            if (!blocking_flag) {
+                out->user_prv_ctx_ptr = nullptr;
+                return SvtJxsErrorNoErrorEmptyQueue;
+            }
+            e->cv_pending.wait(lk, [&] { return !e->pending.empty(); });
+        }
+        slot = e->pending.front();
+        if (!blocking_flag && hipEventQuery(e->slots[slot].done) == hipErrorNotReady) {
+            out->user_prv_ctx_ptr = nullptr;
+            return SvtJxsErrorNoErrorEmptyQueue;
+        }
+        e->pending.pop_front();
+    }
+    EncoderPriv::Slot& s = e->slots[slot];
+    hipError_t he = hipEventSynchronize(s.done);
+    *out = s.frame;
+    out->bitstream.used_size = e->geom.host.g.lcod;
+    out->bitstream.ready_to_release = 1;
+    out->bitstream.last_packet_in_frame = 1;
+    out->image.ready_to_release = 1;
+    const bool failed = he != hipSuccess || *s.h_status != 0;
+    {
+        std::lock_guard<std::mutex> lk(e->mu);
+        s.busy = false;
+    }
+    e->cv_free.notify_one();
+    if (enc_api->callback_send_data_available)
+        enc_api->callback_send_data_available(enc_api, enc_api->callback_send_data_available_context);
+    return failed ? SvtJxsErrorEncodeFrameError : SvtJxsErrorNone;
+}
+
+/* ---------------- extension entry points ---------------- */
+
+PREFIX_API svt_jpeg_xs_amd_plan_t* svt_jpeg_xs_amd_plan_create(const svt_jpeg_xs_amd_frame_spec_t* frames, uint32_t count) {
+    return plan_build(frames, count);
+}
+
+PREFIX_API uint64_t svt_jpeg_xs_amd_plan_footprint(const svt_jpeg_xs_amd_frame_spec_t* frames, uint32_t count) {
+    uint64_t b = 0;
+    for (uint32_t i = 0; i < count; i++) {
+        EncoderPriv* e = frames[i].encoder ? priv_of(frames[i].encoder) : nullptr;
+        if (e)
+            b += frame_work_bytes(e->geom.host.g);
+    }
+    return b;
+}
+
+PREFIX_API SvtJxsErrorType_t svt_jpeg_xs_amd_plan_run(svt_jpeg_xs_amd_plan_t* plan, void* stream, uint32_t use_graph) {
+    if (!plan)
+        return SvtJxsErrorBadParameter;
+    hipStream_t st = (hipStream_t)stream;
+    if (use_graph && st) {
+        if (plan->graph_exec && plan->graph_stream != st) {
+            (void)hipGraphExecDestroy(plan->graph_exec);
+            (void)hipGraphDestroy(plan->graph);
+            plan->graph_exec = nullptr;
+            plan->graph = nullptr;
+        }
+        if (!plan->graph_exec) {
+            if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess)
+                return SvtJxsErrorUndefined;
+            hipError_t le = plan_launch(plan, st, nullptr);
+            hipError_t ee = hipStreamEndCapture(st, &plan->graph);
+            if (le != hipSuccess || ee != hipSuccess)
+                return SvtJxsErrorUndefined;
+            if (hipGraphInstantiate(&plan->graph_exec, plan->graph, nullptr, nullptr, 0) != hipSuccess)
+                return SvtJxsErrorUndefined;
+            plan->graph_stream = st;
+        }
+        return hipGraphLaunch(plan->graph_exec, st) == hipSuccess ? SvtJxsErrorNone : SvtJxsErrorUndefined;
+    }
+    return plan_launch(plan, st, nullptr) == hipSuccess ? SvtJxsErrorNone : SvtJxsErrorUndefined;
+}
+
+PREFIX_API SvtJxsErrorType_t svt_jpeg_xs_amd_plan_run_timed(svt_jpeg_xs_amd_plan_t* plan, void* stream, float* stage_ms) {
+    if (!plan || !stage_ms)
+        return SvtJxsErrorBadParameter;
+    hipStream_t st = (hipStream_t)stream;
+    hipEvent_t ev[8];
+    for (int i = 0; i < 8; i++)
+        if (hipEventCreate(&ev[i]) != hipSuccess)
+            return SvtJxsErrorInsufficientResources;
+    hipError_t e = plan_launch(plan, st, ev);
+    if (e == hipSuccess)
+        e = hipEventSynchronize(ev[7]);
+    for (int i = 0; i < 7 && e == hipSuccess; i++)
+        (void)hipEventElapsedTime(&stage_ms[i], ev[i], ev[i + 1]);
+    for (int i = 0; i < 8; i++)
+        (void)hipEventDestroy(ev[i]);
+    return e == hipSuccess ? SvtJxsErrorNone : SvtJxsErrorUndefined;
+}
+
+PREFIX_API SvtJxsErrorType_t svt_jpeg_xs_amd_plan_status(svt_jpeg_xs_amd_plan_t* plan, uint32_t* frame_errors) {
+    if (!plan)
+        return SvtJxsErrorBadParameter;
+    std::vector<uint32_t> st(plan->nfr);
+    if (hipMemcpy(st.data(), plan->d_status, plan->nfr * sizeof(uint32_t), hipMemcpyDeviceToHost) != hipSuccess)
+        return SvtJxsErrorUndefined;
+    bool any = false;
+    for (uint32_t i = 0; i < plan->nfr; i++) {
+        if (frame_errors)
+            frame_errors[i] = st[i];
+        any |= st[i] != 0;
+    }
+    return any ? SvtJxsErrorEncodeFrameError : SvtJxsErrorNone;
+}
+
+PREFIX_API void svt_jpeg_xs_amd_plan_destroy(svt_jpeg_xs_amd_plan_t* plan) { plan_free(plan); }
+
+PREFIX_API SvtJxsErrorType_t svt_jpeg_xs_amd_plan_debug_copy(svt_jpeg_xs_amd_plan_t* plan, uint32_t frame, uint16_t* coeff,
+                                                             uint8_t* gcli, uint32_t* prec) {
+    if (!plan || frame >= plan->nfr)
+        return SvtJxsErrorBadParameter;
+    JxsFrameDesc F;
+    if (hipMemcpy(&F, plan->d_frames + frame, sizeof(F), hipMemcpyDeviceToHost) != hipSuccess)
+        return SvtJxsErrorUndefined;
+    const JxsGeom& g = plan->geoms[frame]->host.g;
+    if (coeff && hipMemcpy(coeff, F.coeff, g.n_coeff * 2, hipMemcpyDeviceToHost) != hipSuccess)
+        return SvtJxsErrorUndefined;
+    if (gcli && hipMemcpy(gcli, F.gcli, g.n_gcli, hipMemcpyDeviceToHost) != hipSuccess)
+        return SvtJxsErrorUndefined;
+    if (prec && hipMemcpy(prec, F.prec, g.rows * sizeof(JxsPrec), hipMemcpyDeviceToHost) != hipSuccess)
+        return SvtJxsErrorUndefined;
+    return SvtJxsErrorNone;
+}
+
+PREFIX_API const char* svt_jpeg_xs_amd_version(void) { return "svt-jpeg-xs-amd 0.1 (gfx950, API 0.10)"; }
+
+}  // extern "C"

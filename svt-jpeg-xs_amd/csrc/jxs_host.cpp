@@ -1,0 +1,768 @@
+/*
+ * jxs_host.cpp -- host-side coding configuration for the MI355X path.
+ *
+ * Restates, for the default coding tools (RC mode 1, no vertical prediction,
+ * no separate signs, one precinct column), the reference's init-time
+ * computations: parameter checks (EncHandle.c:128-454), band layout and
+ * packet inclusion (Pi.c:21-406), weight tables (WeightTable.c), Q/R bounds
+ * (PiEnc.c:11-53), frame header (PackHeaders.c:11-140) and slice budgets
+ * (EncHandle.c:760-809).  The result is a JxsGeom the kernels read from HBM.
+ */
+#include "jxs_host.h"
+
+#include <stdio.h>
+#include <string.h>
+
+namespace jxs {
+
+enum : int32_t {
+    E_NONE = 0,
+    E_BAD_PARAM = (int32_t)0x80001005,
+    E_INVALID_BS = (int32_t)0x80003001,
+    E_TOO_SHORT = (int32_t)0x80003003,
+};
+
+enum { F_YUV400 = 1, F_YUV420 = 2, F_YUV422 = 3, F_YUV444 = 4, F_4COMP = 5, F_GRAY = 6, F_PACKED444 = 21 };
+static const uint8_t NE = 0xFF;
+
+static int sampling(uint32_t fmt, uint32_t& nc, uint32_t sx[4], uint32_t sy[4]) {
+    for (int i = 0; i < 4; i++)
+        sx[i] = sy[i] = 1;
+    switch (fmt) {
+    case F_YUV422: nc = 3; sx[1] = sx[2] = 2; return 0;
+    case F_YUV420: nc = 3; sx[1] = sx[2] = sy[1] = sy[2] = 2; return 0;
+    case F_YUV444:
+    case F_PACKED444: nc = 3; return 0;
+    case F_4COMP: nc = 4; return 0;
+    case F_GRAY: nc = 1; return 0;
+    default: return -1;
+    }
+}
+
+static inline uint32_t div_up(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
+
+int build_layout(JxsGeom& g, uint32_t nc, const uint32_t sx[4], const uint32_t sy[4], uint32_t W, uint32_t H, uint32_t dh,
+                 uint32_t dv, uint32_t slice_h) {
+    if (dv > dh || dh > 5 || dv > 2 || nc == 0 || nc > JXS_MAXC || slice_h == 0)
+        return -1;
+    g.nc = nc;
+    g.w = W;
+    g.h = H;
+    g.dh = dh;
+    g.dv = dv;
+    const uint32_t ph = 1u << dv;
+    if (slice_h < H && (slice_h % ph))
+        return -1;
+    g.pps = div_up(slice_h, ph);
+    g.slices = div_up(H, slice_h);
+    g.nb_exist = 0;
+    for (uint32_t c = 0; c < nc; c++) {
+        JxsComp& cp = g.c[c];
+        cp.sx = sx[c];
+        cp.sy = sy[c];
+        cp.w = W / sx[c];
+        cp.h = H / sy[c];
+        if (ph / sy[c] == 0 || dv < sy[c] / 2)
+            return -1;
+        cp.dh = dh;
+        cp.dv = dv - sy[c] / 2;
+        cp.nb = 2 * cp.dv + cp.dh + 1;
+        if (cp.nb > JXS_NBC)
+            return -1;
+        g.nb_exist += cp.nb;
+    }
+    g.nb_all = g.c[0].nb * nc;
+    /* global band order with gaps for components with fewer vertical levels
+     * (Pi.c:268-303) */
+    const uint32_t nb_global = 2 * dv + dh + 1;
+    memset(g.gc, NE, sizeof(g.gc));
+    memset(g.gb, NE, sizeof(g.gb));
+    for (uint32_t c = 0; c < nc; c++) {
+        JxsComp& cp = g.c[c];
+        const uint32_t skip = nb_global - cp.nb, tail = 3 * cp.dv;
+        uint32_t gid = c;
+        for (uint32_t i = 0; i < cp.nb; i++) {
+            cp.b[i].gid = gid;
+            g.gc[gid] = (uint8_t)c;
+            g.gb[gid] = (uint8_t)i;
+            gid += nc;
+            if (skip > 0 && i == cp.nb - tail + 1 - skip)
+                gid += nc * skip;
+        }
+    }
+    /* band sizes (Pi.c:318-368) */
+    for (uint32_t c = 0; c < nc; c++) {
+        JxsComp& cp = g.c[c];
+        uint32_t lvl = cp.nb - 1, w = cp.w, h = cp.h;
+        for (uint32_t d = 0; d < cp.dv; d++) {
+            const uint32_t w2 = w / 2, w1 = w - w2, h2 = h / 2, h1 = h - h2;
+            cp.b[lvl].w = w2;
+            cp.b[lvl].h = h2;
+            cp.b[lvl - 1].w = w1;
+            cp.b[lvl - 1].h = h2;
+            cp.b[lvl - 2].w = w2;
+            cp.b[lvl - 2].h = h1;
+            lvl -= 3;
+            w = w1;
+            h = h1;
+        }
+        for (uint32_t d = cp.dv; d < cp.dh; d++) {
+            const uint32_t w2 = w / 2;
+            cp.b[lvl].w = w2;
+            cp.b[lvl].h = h;
+            lvl -= 1;
+            w -= w2;
+        }
+        cp.b[0].w = w;
+        cp.b[0].h = h;
+        for (uint32_t b = 0; b < cp.nb; b++) {
+            JxsBand& bd = cp.b[b];
+            if (bd.w == 0 || bd.h == 0)
+                return -1;
+            bd.hl = (cp.dv == 2 && b >= cp.nb - 3) ? 2 : 1;
+            bd.gw = div_up(bd.w, 4);
+            bd.sw = div_up(bd.gw, 8);
+        }
+    }
+    /* packets (Pi.c:131-187) */
+    uint32_t n = 0;
+    const uint32_t first = (dh > dv ? dh - dv : dv - dh) + 1;
+    g.pk_line[n] = 0;
+    g.pk_bs[n] = 0;
+    g.pk_be[n] = first * nc;
+    n++;
+    for (uint32_t lv = first; lv < g.c[0].nb; lv++, n++) {
+        g.pk_line[n] = 0;
+        g.pk_bs[n] = g.pk_be[n] = lv * nc;
+        for (uint32_t i = lv * nc; i < (lv + 1) * nc; i++)
+            if (g.gb[i] != NE)
+                g.pk_be[n]++;
+    }
+    for (uint32_t lv = first; lv < g.c[0].nb; lv++) {
+        if (g.c[0].b[lv].hl == 1)
+            continue;
+        g.pk_line[n] = 1;
+        g.pk_bs[n] = g.pk_be[n] = lv * nc;
+        for (uint32_t i = lv * nc; i < (lv + 1) * nc; i++)
+            if (g.gb[i] != NE && g.c[g.gc[i]].b[g.gb[i]].hl > 1)
+                g.pk_be[n]++;
+        n++;
+    }
+    g.npk = n;
+    g.short_hdr = (W * nc) < 32752;
+    g.pkt_hdr_bytes = g.short_hdr ? 5 : 7;
+    g.prec_hdr_bytes = div_up(40 + 2 * g.nb_exist, 8);
+    /* precinct rows (Pi.c:21-39, one precinct column) */
+    g.rows = div_up(H, ph);
+    uint32_t bl = 0;
+    for (uint32_t c = 0; c < nc; c++)
+        for (uint32_t b = 0; b < g.c[c].nb; b++) {
+            JxsBand& bd = g.c[c].b[b];
+            bd.lines[0] = bd.hl;
+            bd.lines[1] = bd.h - (g.rows - 1) * bd.hl;
+            bd.bl = bl;
+            bl += bd.hl;
+        }
+    g.n_bl = bl;
+    for (int t = 0; t < 2; t++) {
+        uint32_t exist = 0;
+        for (uint32_t p = 0; p < g.npk; p++) {
+            uint32_t raw_bits = 0, any = 0;
+            for (uint32_t gi = g.pk_bs[p]; gi < g.pk_be[p]; gi++) {
+                const JxsBand& bd = g.c[g.gc[gi]].b[g.gb[gi]];
+                if (g.pk_line[p] < bd.lines[t]) {
+                    any = 1;
+                    raw_bits += bd.gw * 4;
+                }
+            }
+            exist += any;
+            g.raw_bytes[t][p] = div_up(raw_bits, 8);
+        }
+        g.pk_exist[t] = t == 0 ? g.npk : exist;
+        g.rc_hdr_bytes[t] = g.prec_hdr_bytes + div_up(g.pkt_hdr_bytes * 8 * g.pk_exist[t], 8);
+    }
+    return 0;
+}
+
+/* ---------------- weight tables ----------------
+ * ISO/IEC 21122 default gain/priority tables, as carried by
+ * WeightTable.c:18-74 (values must be identical), indexed by global band. */
+struct WTab {
+    uint8_t g[JXS_NGB], p[JXS_NGB];
+    uint32_t n;
+};
+
+#define X NE
+static const uint8_t kG444[3][30] = {
+    {3, 2, 2, 2, 1, 1, 2, 1, 1, 1, 0, 0, 1, 0, 0, 1, 0, 0},
+    {4, 2, 2, 3, 2, 2, 2, 1, 1, 2, 1, 1, 1, 0, 0, 1, 0, 0, 1, 0, 0, 1, 0, 0},
+    {4, 3, 3, 3, 2, 2, 3, 2, 2, 2, 1, 1, 2, 1, 1, 2, 1, 1, 1, 0, 0, 1, 0, 0, 1, 0, 0, 1, 0, 0}};
+static const uint8_t kP444[3][30] = {
+    {6, 8, 7, 1, 4, 5, 12, 14, 15, 0, 2, 3, 9, 11, 10, 13, 16, 17},
+    {21, 1, 0, 15, 19, 18, 5, 9, 8, 14, 17, 16, 2, 4, 3, 7, 13, 11, 6, 12, 10, 20, 23, 22},
+    {12, 15, 14, 3, 11, 10, 24, 26, 27, 0, 4, 5, 18, 21, 20, 19, 23, 22, 13, 16, 17, 2, 9, 6, 1, 7, 8, 25, 28, 29}};
+static const uint8_t kG422[3][30] = {
+    {2, 2, 2, 1, 1, 1, 1, 1, 1, 0, 0, 0, 0, 0, 0, 0, 0, 0},
+    {2, 2, 2, 2, 2, 2, 1, 1, 1, 1, 1, 1, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0},
+    {3, 3, 3, 2, 2, 2, 2, 2, 2, 1, 1, 1, 1, 1, 1, 1, 1, 1, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0}};
+static const uint8_t kP422[3][30] = {
+    {8, 7, 6, 5, 3, 4, 13, 12, 14, 1, 0, 2, 9, 11, 10, 16, 15, 17},
+    {0, 1, 2, 18, 19, 20, 6, 7, 8, 15, 16, 17, 3, 4, 5, 10, 12, 14, 9, 11, 13, 21, 22, 23},
+    {14, 13, 12, 9, 11, 10, 25, 24, 26, 0, 1, 2, 19, 20, 18, 23, 22, 21, 17, 15, 16, 4, 8, 5, 3, 6, 7, 28, 27, 29}};
+static const uint32_t kN44x[3] = {18, 24, 30};
+/* 4:2:0, [V-1][H-1] */
+static const uint8_t kG420[2][5][30] = {
+    {{1, 1, 1, 1, 0, 0, 1, X, X, 0, X, X},
+     {1, 1, 1, 1, 0, 0, 1, 0, 0, 1, X, X, 0, X, X},
+     {2, 1, 1, 1, 1, 1, 1, 0, 0, 1, 0, 0, 1, X, X, 0, X, X},
+     {2, 2, 2, 2, 1, 1, 1, 1, 1, 1, 0, 0, 1, 0, 0, 1, X, X, 0, X, X},
+     {3, 3, 3, 2, 2, 2, 2, 1, 1, 1, 1, 1, 1, 0, 0, 1, 0, 0, 1, X, X, 0, X, X}},
+    {{0},
+     {2, 1, 1, 1, 1, 1, 1, X, X, 0, X, X, 1, 1, 1, 1, 1, 1, 0, 0, 0},
+     {2, 2, 2, 2, 1, 1, 1, 1, 1, 1, X, X, 0, X, X, 1, 1, 1, 1, 1, 1, 0, 0, 0},
+     {3, 2, 2, 2, 2, 2, 2, 1, 1, 1, 1, 1, 1, X, X, 0, X, X, 1, 1, 1, 1, 1, 1, 0, 0, 0},
+     {3, 3, 3, 3, 2, 2, 2, 2, 2, 2, 1, 1, 1, 1, 1, 1, X, X, 0, X, X, 1, 1, 1, 1, 1, 1, 0, 0, 0}}};
+static const uint8_t kP420[2][5][30] = {
+    {{2, 5, 4, 6, 1, 0, 7, X, X, 3, X, X},
+     {4, 7, 6, 8, 1, 0, 10, 3, 2, 9, X, X, 5, X, X},
+     {11, 3, 2, 6, 9, 8, 10, 1, 0, 13, 5, 4, 12, X, X, 7, X, X},
+     {7, 12, 11, 14, 3, 2, 6, 10, 9, 13, 1, 0, 16, 5, 4, 15, X, X, 8, X, X},
+     {9, 16, 15, 7, 13, 12, 17, 3, 2, 6, 11, 10, 14, 1, 0, 19, 5, 4, 18, X, X, 8, X, X}},
+    {{0},
+     {10, 4, 3, 2, 9, 8, 1, X, X, 0, X, X, 16, 14, 12, 15, 13, 11, 7, 6, 5},
+     {5, 12, 11, 13, 4, 3, 2, 10, 9, 1, X, X, 0, X, X, 19, 17, 15, 18, 16, 14, 8, 7, 6},
+     {22, 6, 5, 7, 14, 13, 15, 4, 3, 2, 12, 11, 1, X, X, 0, X, X, 21, 19, 17, 20, 18, 16, 10, 9, 8},
+     {0, 13, 12, 25, 7, 6, 8, 17, 16, 18, 5, 4, 3, 15, 14, 2, X, X, 1, X, X, 24, 22, 20, 23, 21, 19, 11, 10, 9}}};
+static const uint32_t kN420[2][5] = {{12, 15, 18, 21, 24}, {0, 21, 24, 27, 30}};
+#undef X
+
+static int wtab_default(uint32_t fmt, uint32_t v, uint32_t h, WTab& t) {
+    if (h == 5 && (fmt == F_YUV444 || fmt == F_PACKED444 || fmt == F_YUV422)) {
+        const bool is422 = fmt == F_YUV422;
+        t.n = kN44x[v];
+        memcpy(t.g, is422 ? kG422[v] : kG444[v], t.n);
+        memcpy(t.p, is422 ? kP422[v] : kP444[v], t.n);
+        return 0;
+    }
+    if (fmt == F_YUV420 && v >= 1 && h >= 1 && h >= v && kN420[v - 1][h - 1]) {
+        t.n = kN420[v - 1][h - 1];
+        memcpy(t.g, kG420[v - 1][h - 1], t.n);
+        memcpy(t.p, kP420[v - 1][h - 1], t.n);
+        return 0;
+    }
+    return -1;
+}
+
+/* Tables for 4:2:2 / 4:4:4 at other levels are derived from the V2/H5 one
+ * (WeightTable.c:128-334): remove the entries of the dropped levels, then
+ * close the gaps of the priority order. */
+static void wtab_drop(WTab& t, uint32_t grp) {
+    for (uint32_t i = 3 * grp; i + 3 < t.n; i++) {
+        t.g[i] = t.g[i + 3];
+        t.p[i] = t.p[i + 3];
+    }
+    t.n -= 3;
+}
+
+static int wtab_derived(uint32_t fmt, uint32_t dv, uint32_t dh, WTab& t) {
+    if (fmt != F_YUV422 && fmt != F_YUV444 && fmt != F_PACKED444)
+        return -1;
+    if (wtab_default(fmt, 2, 5, t) || dh < dv)
+        return -1;
+    for (int v = 2; v > (int)dv; v--) {
+        wtab_drop(t, 4);
+        wtab_drop(t, 0);
+    }
+    for (int h = 5; h > (int)dh;) {
+        h--;
+        if (h != 1)
+            wtab_drop(t, 0);
+        else
+            t.n -= 3;
+    }
+    for (uint32_t i = 0; i < t.n; i++) {
+        for (;;) {
+            bool found = false;
+            for (uint32_t j = 0; j < t.n && !found; j++)
+                found = t.p[j] == i;
+            if (found)
+                break;
+            for (uint32_t j = 0; j < t.n; j++)
+                if (t.p[j] > i) {
+                    t.p[j]--;
+                    break;
+                }
+        }
+    }
+    return 0;
+}
+
+static inline uint32_t truncation(uint32_t gain, uint32_t prio, uint32_t q, uint32_t r) {
+    const uint32_t pump = prio < r ? 1 : 0;
+    if (q < gain + pump)
+        return 0;
+    const uint32_t t = q - gain - pump;
+    return t > 15 ? 15 : t;
+}
+
+static bool all_truncated(const JxsGeom& g, uint32_t q, uint32_t r) {
+    for (uint32_t c = 0; c < g.nc; c++)
+        for (uint32_t b = 0; b < g.c[c].nb; b++)
+            if (truncation(g.c[c].b[b].gain, g.c[c].b[b].prio, q, r) != 15)
+                return false;
+    return true;
+}
+
+/* ---------------- frame header ---------------- */
+struct BitOut {
+    uint8_t* m;
+    uint32_t bit;
+    void put(uint32_t v, uint32_t n) {
+        for (int i = (int)n - 1; i >= 0; i--, bit++)
+            if ((v >> i) & 1)
+                m[bit >> 3] |= (uint8_t)(0x80u >> (bit & 7));
+    }
+    void align() { bit = (bit + 7) & ~7u; }
+};
+
+static uint32_t write_header(JxsGeom& g) {
+    memset(g.hdr, 0, sizeof(g.hdr));
+    BitOut o{g.hdr, 0};
+    o.put(0xFF10, 16); /* SOC */
+    o.put(0xFF50, 16); /* CAP */
+    o.put(4, 16);
+    for (int i = 0; i < 9; i++)
+        o.put(i == 4 ? (g.format == F_YUV420) : (i == 8 ? 1u : 0u), 1);
+    o.align();
+    o.put(0xFF12, 16); /* PIH */
+    o.put(26, 16);
+    o.put(g.lcod, 32);
+    o.put(0, 16);
+    o.put(0, 16);
+    o.put(g.w, 16);
+    o.put(g.h, 16);
+    o.put(0, 16); /* Cw */
+    o.put(g.pps, 16);
+    o.put(g.nc, 8);
+    o.put(4, 8);
+    o.put(8, 8);
+    o.put(20, 8); /* Bw */
+    o.put(8, 4);  /* Fq */
+    o.put(4, 4);  /* Br */
+    o.put(0, 8);
+    o.put(g.dh, 4);
+    o.put(g.dv, 4);
+    o.put(!g.short_hdr, 1);
+    o.put(1, 1);
+    o.put(g.qpih, 2);
+    o.put(0, 2);
+    o.put(0, 2);
+    o.put(0xFF13, 16); /* CDT */
+    o.put(2 * g.nc + 2, 16);
+    for (uint32_t c = 0; c < g.nc; c++) {
+        o.put(g.depth, 8);
+        o.put(g.c[c].sx, 4);
+        o.put(g.c[c].sy, 4);
+    }
+    o.put(0xFF14, 16); /* WGT */
+    o.put(2 * g.nb_exist + 2, 16);
+    for (uint32_t gi = 0; gi < g.nb_all; gi++) {
+        if (g.gc[gi] == NE)
+            continue;
+        const JxsBand& b = g.c[g.gc[gi]].b[g.gb[gi]];
+        o.put(b.gain, 8);
+        o.put(b.prio, 8);
+    }
+    o.align();
+    return o.bit >> 3;
+}
+
+void assign_buffers(JxsGeom& g) {
+    uint64_t co = 0, go = 0, so = 0, scr = 0;
+    for (uint32_t c = 0; c < g.nc; c++) {
+        JxsComp& cp = g.c[c];
+        for (uint32_t b = 0; b < cp.nb; b++) {
+            JxsBand& bd = cp.b[b];
+            bd.coeff_off = co;
+            bd.gcli_off = go;
+            bd.smax_off = so;
+            co += (uint64_t)bd.w * bd.h;
+            go += (uint64_t)bd.gw * bd.h;
+            so += (uint64_t)bd.sw * bd.h;
+        }
+        if (cp.dv > 0)
+            scr += (uint64_t)cp.w * cp.h + (uint64_t)div_up(cp.w, 2) * div_up(cp.h, 2);
+        co = (co + 63) & ~(uint64_t)63;
+        go = (go + 255) & ~(uint64_t)255;
+        so = (so + 255) & ~(uint64_t)255;
+    }
+    g.n_coeff = co;
+    g.n_gcli = go;
+    g.n_smax = so;
+    g.n_i32 = scr;
+}
+
+int32_t build_encoder_geometry(const EncParams& p, HostGeom& out, int verbose) {
+    JxsGeom& g = out.g;
+    memset(&g, 0, sizeof(g));
+    uint32_t nc, sx[4], sy[4];
+    auto bad = [&](const char* why) {
+        if (verbose)
+            fprintf(stderr, "Error: %s\n", why);
+        return E_BAD_PARAM;
+    };
+    if (p.depth < 8 || p.depth > 14)
+        return bad("bit depth must be 8..14");
+    if (sampling(p.format, nc, sx, sy))
+        return bad("unsupported colour format");
+    if (p.decomp_v == 0 && p.format == F_YUV420)
+        return bad("YUV420 requires vertical decomposition 1 or 2");
+    uint32_t slice_h = p.slice_height;
+    if (slice_h == 0)
+        return bad("slice_height cannot be 0");
+    if (slice_h >= p.height)
+        slice_h = p.height;
+    else if (slice_h % (1u << p.decomp_v))
+        return bad("slice_height must be a multiple of 2^decomp_v");
+    if (p.quantization > 1)
+        return bad("quantization must be 0 or 1");
+    if (p.bpp_den == 0 || p.bpp_num == 0)
+        return bad("bpp numerator and denominator must be non-zero");
+    const uint64_t bpf = ((uint64_t)p.width * p.height * p.bpp_num / p.bpp_den + 7) / 8;
+    if (bpf >= ((uint64_t)1 << 32) || bpf == 0)
+        return bad("impossible compression");
+    if (p.width < 4)
+        return bad("minimum width is 4");
+    if ((p.format == F_YUV422 || p.format == F_YUV420) && (p.width % 2))
+        return bad("format requires an even width");
+    if (p.format == F_YUV420 && (p.height % 2))
+        return bad("YUV420 requires an even height");
+    if (p.decomp_v > 2 || p.decomp_h > 5 || p.decomp_h < p.decomp_v || (p.decomp_h == 0 && p.decomp_v == 0))
+        return bad("decomposition levels out of range");
+    uint32_t mw = p.width, mh = p.height, mv = p.decomp_v;
+    if (p.format == F_YUV420) {
+        mw >>= 1;
+        mh >>= 1;
+        mv -= 1;
+    }
+    if (p.format == F_YUV422)
+        mw >>= 1;
+    for (uint32_t i = 0; i < p.decomp_h && mw; i++)
+        mw = mw > 1 ? mw - mw / 2 : 0;
+    for (uint32_t i = 0; i < mv && mh; i++)
+        mh = mh > 1 ? mh - mh / 2 : 0;
+    if (mw == 0 || mh == 0)
+        return bad("frame too small for the decomposition");
+    if (build_layout(g, nc, sx, sy, p.width, p.height, p.decomp_h, p.decomp_v, slice_h))
+        return bad("band layout");
+    for (uint32_t c = 0; c < nc; c++)
+        if (g.c[c].w > JXS_MAX_ROW)
+            return bad("component wider than the row kernel supports (7680 samples)");
+    g.depth = p.depth;
+    g.format = p.format;
+    g.packed = p.format == F_PACKED444;
+    g.lcod = (uint32_t)bpf;
+    g.qpih = p.quantization;
+    g.sig = p.significance ? 1 : 0;
+    WTab t;
+    bool ok = nc == 3 && wtab_default(p.format, p.decomp_v, p.decomp_h, t) == 0;
+    if (!ok && (nc != 3 || wtab_derived(p.format, p.decomp_v, p.decomp_h, t)))
+        return bad("no weight table for this configuration");
+    for (uint32_t c = 0; c < nc; c++)
+        for (uint32_t b = 0; b < g.c[c].nb; b++) {
+            JxsBand& bd = g.c[c].b[b];
+            if (bd.gid >= t.n)
+                return bad("weight table");
+            bd.gain = t.g[bd.gid];
+            bd.prio = t.p[bd.gid];
+        }
+    uint32_t mr = 0;
+    for (uint32_t c = 0; c < nc; c++)
+        for (uint32_t b = 0; b < g.c[c].nb; b++)
+            mr = g.c[c].b[b].prio > mr ? g.c[c].b[b].prio : mr;
+    if (mr < 255)
+        mr++;
+    if (mr > g.nb_all - 1)
+        mr = g.nb_all - 1;
+    g.maxr = mr;
+    uint32_t mq = 31;
+    while (mq > 0 && all_truncated(g, mq, mr))
+        mq--;
+    g.maxq = mq;
+    if (g.maxr >= JXS_NR)
+        return bad("refinement range");
+    for (uint32_t q = 0; q < JXS_NQ; q++) {
+        g.empty_mask[q] = 0;
+        for (uint32_t r = 0; r < JXS_NR; r++)
+            if (all_truncated(g, q, r))
+                g.empty_mask[q] |= 1u << r;
+    }
+    g.hdr_len = write_header(g);
+    const uint32_t tags = g.hdr_len + 2 + 6 * g.slices;
+    if (tags >= g.lcod)
+        return bad("impossible compression, use a bigger bpp");
+    /* slice sizes (EncHandle.c:760-809) */
+    out.slice_size.assign(g.slices, 0);
+    out.slice_off.assign(g.slices, 0);
+    out.slice_bud.assign(g.slices, 0);
+    {
+        const uint32_t all = g.lcod - tags;
+        const uint32_t last_prec = g.pps - (g.slices * g.pps - g.rows);
+        const uint32_t per_prec = all / g.rows;
+        uint32_t per_slice = per_prec * g.pps;
+        uint32_t last_slice = per_prec * last_prec;
+        uint32_t left = all - last_slice - per_slice * (g.slices - 1);
+        if (g.slices > 1)
+            per_slice += (left * (g.slices - 1) * g.pps / g.rows) / (g.slices - 1);
+        last_slice += left * last_prec / g.rows;
+        left = all - last_slice - per_slice * (g.slices - 1);
+        uint32_t remain = g.lcod - g.hdr_len, off = g.hdr_len;
+        for (uint32_t i = 0; i < g.slices; i++) {
+            const uint32_t s = (i != g.slices - 1) ? per_slice + (i < left ? 1 : 0) + 6 : remain;
+            out.slice_size[i] = s;
+            out.slice_off[i] = off;
+            out.slice_bud[i] = s - 6 - (i == g.slices - 1 ? 2 : 0);
+            off += s;
+            remain -= s;
+        }
+    }
+    assign_buffers(g);
+    return E_NONE;
+}
+
+/* ---------------- decoder header ---------------- */
+struct BitIn {
+    const uint8_t* m;
+    uint64_t bit, end;
+    bool err = false;
+    uint32_t get(uint32_t n) {
+        uint32_t v = 0;
+        for (uint32_t i = 0; i < n; i++, bit++) {
+            if (bit >= end) {
+                err = true;
+                return 0;
+            }
+            v = (v << 1) | ((m[bit >> 3] >> (7 - (bit & 7))) & 1);
+        }
+        return v;
+    }
+};
+
+int32_t parse_codestream_header(const uint8_t* cs, size_t len, HostGeom& out, uint32_t* header_end) {
+    JxsGeom& g = out.g;
+    memset(&g, 0, sizeof(g));
+    BitIn r{cs, 0, (uint64_t)len * 8};
+    if (len < 4)
+        return E_TOO_SHORT;
+    if (r.get(16) != 0xFF10)
+        return E_INVALID_BS;
+    bool pih = false, cdt = false, wgt = false;
+    uint32_t W = 0, H = 0, nc = 0, dh = 0, dv = 0, hsl = 0, cw = 0, lh = 0, bw = 0, fq = 0, nw = 0;
+    uint32_t sx[4] = {1, 1, 1, 1}, sy[4] = {1, 1, 1, 1};
+    uint8_t gains[JXS_NGB], prios[JXS_NGB];
+    for (;;) {
+        if (r.bit + 16 > r.end)
+            return E_TOO_SHORT;
+        const uint32_t marker = r.get(16);
+        if (marker == 0xFF20) {
+            r.bit -= 16;
+            break;
+        }
+        const uint32_t seglen = r.get(16);
+        const uint64_t start = r.bit - 16;
+        switch (marker) {
+        case 0xFF12:
+            g.lcod = r.get(32);
+            r.get(32);
+            W = r.get(16);
+            H = r.get(16);
+            cw = r.get(16);
+            hsl = r.get(16);
+            nc = r.get(8);
+            if (r.get(8) != 4 || r.get(8) != 8)
+                return E_INVALID_BS;
+            bw = r.get(8);
+            fq = r.get(4);
+            if (r.get(4) != 4 || r.get(8) != 0)
+                return E_INVALID_BS;
+            dh = r.get(4);
+            dv = r.get(4);
+            lh = r.get(1);
+            r.get(1);
+            g.qpih = r.get(2);
+            if (r.get(2) != 0 || r.get(2) != 0)
+                return E_INVALID_BS;
+            pih = true;
+            break;
+        case 0xFF13:
+            for (uint32_t c = 0; c < (seglen - 2) / 2 && c < 4; c++) {
+                g.depth = r.get(8);
+                sx[c] = r.get(4);
+                sy[c] = r.get(4);
+            }
+            cdt = true;
+            break;
+        case 0xFF14:
+            nw = (seglen - 2) / 2;
+            if (nw > JXS_NGB)
+                return E_INVALID_BS;
+            for (uint32_t i = 0; i < nw; i++) {
+                gains[i] = (uint8_t)r.get(8);
+                prios[i] = (uint8_t)r.get(8);
+            }
+            wgt = true;
+            break;
+        case 0xFF50:
+        case 0xFF15:
+            break;
+        default:
+            return E_INVALID_BS;
+        }
+        r.bit = start + 8ull * seglen;
+        if (r.err || r.bit > r.end)
+            return E_TOO_SHORT;
+    }
+    if (!pih || !cdt || !wgt || cw != 0 || nc == 0 || nc > 4 || hsl == 0 || bw != 20 || fq != 8)
+        return E_INVALID_BS;
+    if (build_layout(g, nc, sx, sy, W, H, dh, dv, hsl << dv))
+        return E_INVALID_BS;
+    if ((uint32_t)g.short_hdr != (lh ? 0u : 1u) || nw != g.nb_exist)
+        return E_INVALID_BS;
+    uint32_t k = 0;
+    for (uint32_t gi = 0; gi < g.nb_all; gi++) {
+        if (g.gc[gi] == NE)
+            continue;
+        g.c[g.gc[gi]].b[g.gb[gi]].gain = gains[k];
+        g.c[g.gc[gi]].b[g.gb[gi]].prio = prios[k];
+        k++;
+    }
+    assign_buffers(g);
+    *header_end = (uint32_t)(r.bit >> 3);
+    return E_NONE;
+}
+
+/* ---------------- forward-transform jobs ---------------- */
+static JxsBandOut band_out(const JxsGeom& g, uint32_t c, uint32_t b, uint16_t* coeff, uint8_t* gcli, uint8_t* smax) {
+    const JxsBand& bd = g.c[c].b[b];
+    JxsBandOut o;
+    o.coeff = coeff + bd.coeff_off;
+    o.gcli = gcli + bd.gcli_off;
+    o.smax = smax + bd.smax_off;
+    o.w = bd.w;
+    o.gw = bd.gw;
+    o.sw = bd.sw;
+    o.pad_ = 0;
+    return o;
+}
+
+void fwd_jobs(const JxsGeom& g, const void* const planes[4], const uint32_t strides[4], uint16_t* coeff, uint8_t* gcli,
+              uint8_t* smax, int32_t* scratch, FwdJobs& jobs) {
+    int32_t* scr = scratch;
+    for (uint32_t c = 0; c < g.nc; c++) {
+        const JxsComp& cp = g.c[c];
+        const uint32_t kind = g.depth <= 8 ? JXS_SRC_U8 : JXS_SRC_U16;
+        const uint32_t elem = g.depth <= 8 ? 1 : 2;
+        const void* src = g.packed ? (const void*)((const uint8_t*)planes[0] + (size_t)c * elem) : planes[c];
+        const uint32_t stride = g.packed ? strides[0] : strides[c];
+        const uint32_t step = g.packed ? 3 : 1;
+        const uint32_t shift = 20 - g.depth, mask = (1u << g.depth) - 1;
+        const int32_t offset = 1 << 19;
+        const uint32_t nb = cp.nb;
+        auto horz = [&](const void* s, uint32_t k, uint32_t st, uint32_t stp, uint32_t w, uint32_t rows) {
+            JxsHorzJob j;
+            memset(&j, 0, sizeof(j));
+            j.src = s;
+            j.kind = k;
+            j.stride = st;
+            j.step = stp;
+            j.w = w;
+            j.rows = rows;
+            j.shift = shift;
+            j.mask = mask;
+            j.offset = offset;
+            return j;
+        };
+        if (cp.dv == 0) {
+            JxsHorzJob j = horz(src, kind, stride, step, cp.w, cp.h);
+            j.levels = cp.dh;
+            for (uint32_t k = 0; k < cp.dh; k++)
+                j.hf[k] = band_out(g, c, nb - 1 - k, coeff, gcli, smax);
+            j.lf_is_band = 1;
+            j.lf = band_out(g, c, 0, coeff, gcli, smax);
+            jobs.horz[0].push_back(j);
+            continue;
+        }
+        const uint32_t w = cp.w, h = cp.h, w1 = (w + 1) / 2, h1 = (h + 1) / 2, h2 = h / 2;
+        int32_t* A = scr;
+        int32_t* B = scr + (size_t)w * h;
+        scr += (size_t)w * h + (size_t)w1 * h1;
+        JxsVertJob v;
+        memset(&v, 0, sizeof(v));
+        v.src = src;
+        v.dst = A;
+        v.kind = kind;
+        v.stride = stride;
+        v.step = step;
+        v.w = w;
+        v.h = h;
+        v.shift = shift;
+        v.mask = mask;
+        v.offset = offset;
+        jobs.vert[0].push_back(v);
+        /* low rows of level 1 */
+        JxsHorzJob jl = horz(A, JXS_SRC_I32, w, 1, w, h1);
+        jl.hf[0] = band_out(g, c, nb - 3, coeff, gcli, smax);
+        if (cp.dv == 1) {
+            jl.levels = cp.dh;
+            for (uint32_t k = 1; k < cp.dh; k++)
+                jl.hf[k] = band_out(g, c, cp.dh - k, coeff, gcli, smax);
+            jl.lf_is_band = 1;
+            jl.lf = band_out(g, c, 0, coeff, gcli, smax);
+        } else {
+            jl.levels = 1;
+            jl.lf_is_band = 0;
+            jl.lf_i32 = B;
+            jl.lf_stride = w1;
+        }
+        jobs.horz[0].push_back(jl);
+        /* high rows of level 1 */
+        if (h2 > 0) {
+            JxsHorzJob jh = horz(A + (size_t)h1 * w, JXS_SRC_I32, w, 1, w, h2);
+            jh.levels = 1;
+            jh.hf[0] = band_out(g, c, nb - 1, coeff, gcli, smax);
+            jh.lf_is_band = 1;
+            jh.lf = band_out(g, c, nb - 2, coeff, gcli, smax);
+            jobs.horz[0].push_back(jh);
+        }
+        if (cp.dv == 2) {
+            const uint32_t g1 = (h1 + 1) / 2, g2 = h1 / 2;
+            JxsVertJob v2 = v;
+            v2.src = B;
+            v2.dst = A;
+            v2.kind = JXS_SRC_I32;
+            v2.stride = w1;
+            v2.step = 1;
+            v2.w = w1;
+            v2.h = h1;
+            jobs.vert[1].push_back(v2);
+            JxsHorzJob kl = horz(A, JXS_SRC_I32, w1, 1, w1, g1);
+            kl.levels = 1 + (cp.dh - 2);
+            kl.hf[0] = band_out(g, c, nb - 6, coeff, gcli, smax);
+            for (uint32_t k = 1; k < kl.levels; k++)
+                kl.hf[k] = band_out(g, c, cp.dh - 2 - (k - 1), coeff, gcli, smax);
+            kl.lf_is_band = 1;
+            kl.lf = band_out(g, c, 0, coeff, gcli, smax);
+            jobs.horz[1].push_back(kl);
+            if (g2 > 0) {
+                JxsHorzJob kh = horz(A + (size_t)g1 * w1, JXS_SRC_I32, w1, 1, w1, g2);
+                kh.levels = 1;
+                kh.hf[0] = band_out(g, c, nb - 4, coeff, gcli, smax);
+                kh.lf_is_band = 1;
+                kh.lf = band_out(g, c, nb - 5, coeff, gcli, smax);
+                jobs.horz[1].push_back(kh);
+            }
+        }
+    }
+}
+
+}  // namespace jxs

@@ -1,0 +1,63 @@
+/*
+ * jxs_host.h -- host-side coding configuration: parameter validation, band
+ * layout, weight tables, frame header, slice budgets, and the job lists the
+ * kernels run.
+ */
+#ifndef JXS_HOST_H
+#define JXS_HOST_H
+
+#include <stdint.h>
+#include <vector>
+
+#include "jxs_geom.h"
+
+namespace jxs {
+
+struct EncParams {
+    uint32_t width, height, depth, format;
+    uint32_t bpp_num, bpp_den, decomp_v, decomp_h;
+    uint32_t quantization, slice_height, significance;
+};
+
+/* Host geometry = device geometry + slice tables. */
+struct HostGeom {
+    JxsGeom g;
+    std::vector<uint32_t> slice_size; /* bytes of each slice incl. its header (EOC in the last) */
+    std::vector<uint32_t> slice_off;  /* codestream offset of each slice header */
+    std::vector<uint32_t> slice_bud;  /* precinct bytes of each slice */
+};
+
+/* Encoder configuration: EncHandle.c:128-454 (validation), Pi.c:189-406
+ * (layout), WeightTable.c:336-413, PiEnc.c:11-53, PackHeaders.c:11-140,
+ * EncHandle.c:760-809.  Returns 0 or an SvtJxsErrorType_t value. */
+int32_t build_encoder_geometry(const EncParams& p, HostGeom& out, int verbose);
+
+/* Decoder configuration from a codestream header (ParseHeader.c). */
+int32_t parse_codestream_header(const uint8_t* cs, size_t len, HostGeom& out, uint32_t* header_end);
+
+/* Band layout only (shared by the encoder and decoder builders). */
+int build_layout(JxsGeom& g, uint32_t nc, const uint32_t sx[4], const uint32_t sy[4], uint32_t W, uint32_t H, uint32_t dh,
+                 uint32_t dv, uint32_t slice_h);
+
+/* Assigns buffer offsets and sizes (coefficients, GCLI, significance max,
+ * transform scratch) for one frame of this geometry. */
+void assign_buffers(JxsGeom& g);
+
+/* Forward-transform job lists for one frame.  The base pointers are device
+ * addresses of this frame's buffers; the input planes are device addresses.
+ * stage 0: first vertical level, 1: its rows (or all rows for V0
+ * components), 2: second vertical level, 3: its rows. */
+struct FwdJobs {
+    std::vector<JxsVertJob> vert[2];
+    std::vector<JxsHorzJob> horz[2];
+};
+void fwd_jobs(const JxsGeom& g, const void* const planes[4], const uint32_t strides[4], uint16_t* coeff, uint8_t* gcli,
+              uint8_t* smax, int32_t* scratch, FwdJobs& jobs);
+
+/* Inverse-transform job lists (decoder). */
+struct InvVertJob;
+struct InvHorzJob;
+
+}  // namespace jxs
+
+#endif
