@@ -64,6 +64,37 @@ PREFIX_API uint64_t svt_jpeg_xs_amd_plan_footprint(const svt_jpeg_xs_amd_frame_s
 PREFIX_API SvtJxsErrorType_t svt_jpeg_xs_amd_plan_debug_copy(svt_jpeg_xs_amd_plan_t* plan, uint32_t frame, uint16_t* coeff,
                                                              uint8_t* gcli, uint32_t* prec);
 
+/* Device address of the precinct-offset table the encode plan wrote for
+ * frame `frame` (byte offset of every precinct header in its codestream,
+ * one u32 every *stride u32).  Valid after the plan has run; a decode plan of
+ * the same codestream can take it instead of walking the precinct chain. */
+PREFIX_API SvtJxsErrorType_t svt_jpeg_xs_amd_plan_precinct_offsets(svt_jpeg_xs_amd_plan_t* plan, uint32_t frame,
+                                                                   const uint32_t** d_offsets, uint32_t* stride);
+
+/* One codestream of a device-resident decode batch.  `decoder` is a decoder
+ * initialised on a codestream of the same configuration; planes receive the
+ * decoded components (stride in samples).  d_precinct_offsets may be NULL,
+ * then the precinct chain is walked on the device. */
+typedef struct svt_jpeg_xs_amd_decode_spec {
+    svt_jpeg_xs_decoder_api_t* decoder;
+    const uint8_t* d_codestream;
+    uint32_t codestream_size;
+    void* d_planes[MAX_COMPONENTS_NUM];
+    uint32_t strides[MAX_COMPONENTS_NUM];
+    const uint32_t* d_precinct_offsets;
+    uint32_t precinct_offsets_stride;
+} svt_jpeg_xs_amd_decode_spec_t;
+
+typedef struct svt_jpeg_xs_amd_decode_plan svt_jpeg_xs_amd_decode_plan_t;
+
+PREFIX_API svt_jpeg_xs_amd_decode_plan_t* svt_jpeg_xs_amd_decode_plan_create(const svt_jpeg_xs_amd_decode_spec_t* frames,
+                                                                           uint32_t count);
+/* Enqueues unpack, inverse transform and output scaling of every codestream. */
+PREFIX_API SvtJxsErrorType_t svt_jpeg_xs_amd_decode_plan_run(svt_jpeg_xs_amd_decode_plan_t* plan, void* stream);
+/* After completion: per-codestream error flags; SvtJxsErrorDecoderInvalidBitstream if any. */
+PREFIX_API SvtJxsErrorType_t svt_jpeg_xs_amd_decode_plan_status(svt_jpeg_xs_amd_decode_plan_t* plan, uint32_t* errors);
+PREFIX_API void svt_jpeg_xs_amd_decode_plan_destroy(svt_jpeg_xs_amd_decode_plan_t* plan);
+
 /* Version / build string of the library. */
 PREFIX_API const char* svt_jpeg_xs_amd_version(void);
 

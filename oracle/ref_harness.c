@@ -83,7 +83,12 @@ int main(int argc, char** argv) {
         enc.quantization = (uint32_t)atoi(argv[14]);
     if (argc > 15)
         enc.coding_significance = (uint32_t)atoi(argv[15]);
-    enc.use_cpu_flags = CPU_FLAGS_C;
+    /* JXS_REF_SIMD=1 enables the reference's AVX2 / AVX-512 intrinsic kernels
+     * (the SSE2 bit stays clear: the only SSE2 entries are the unassembled NASM
+     * routines, whose C twins are then selected).  Output is identical either
+     * way (EncoderTest.sh:734-769); the default is the plain C path. */
+    const char* simd = getenv("JXS_REF_SIMD");
+    enc.use_cpu_flags = (simd && simd[0] == '1') ? (CPU_FLAGS_ALL & ~(CPU_FLAGS)CPU_FLAGS_SSE2) : CPU_FLAGS_C;
     enc.verbose = VERBOSE_ERRORS;
 
     svt_jpeg_xs_image_config_t cfg;

@@ -95,6 +95,14 @@ class FrameSpec(ctypes.Structure):
                 ("strides", ctypes.c_uint32 * 4), ("d_out", ctypes.c_void_p)]
 
 
+class DecodeSpec(ctypes.Structure):
+    """svt_jpeg_xs_amd_decode_spec_t (include/svt_jpegxs_amd.h)."""
+    _fields_ = [("decoder", ctypes.POINTER(DecoderApi)), ("d_codestream", ctypes.c_void_p),
+                ("codestream_size", ctypes.c_uint32), ("d_planes", ctypes.c_void_p * 4),
+                ("strides", ctypes.c_uint32 * 4), ("d_precinct_offsets", ctypes.c_void_p),
+                ("precinct_offsets_stride", ctypes.c_uint32)]
+
+
 # Every C-ABI symbol the library exports (include/*.h).
 EXPORTED_SYMBOLS = [
     "svt_jpeg_xs_encoder_load_default_parameters", "svt_jpeg_xs_encoder_get_image_config",
@@ -110,7 +118,7 @@ EXPORTED_SYMBOLS = [
     "svt_jpeg_xs_amd_plan_status", "svt_jpeg_xs_amd_plan_destroy", "svt_jpeg_xs_amd_plan_footprint",
     "svt_jpeg_xs_amd_version",
     "svt_jpeg_xs_amd_decode_plan_create", "svt_jpeg_xs_amd_decode_plan_run", "svt_jpeg_xs_amd_decode_plan_status",
-    "svt_jpeg_xs_amd_decode_plan_destroy",
+    "svt_jpeg_xs_amd_decode_plan_destroy", "svt_jpeg_xs_amd_plan_precinct_offsets", "svt_jpeg_xs_amd_plan_debug_copy",
 ]
 
 _lib = None
@@ -133,16 +141,31 @@ def lib():
         L.svt_jpeg_xs_encoder_close.restype = None
         L.svt_jpeg_xs_encoder_send_picture.argtypes = [E, ctypes.POINTER(Frame), ctypes.c_uint8]
         L.svt_jpeg_xs_encoder_get_packet.argtypes = [E, ctypes.POINTER(Frame), ctypes.c_uint8]
-        if hasattr(L, "svt_jpeg_xs_decoder_init"):
-            L.svt_jpeg_xs_decoder_init.argtypes = [ctypes.c_uint64, ctypes.c_uint64, D, ctypes.c_void_p,
-                                                   ctypes.c_size_t, ctypes.POINTER(ImageConfig)]
-            L.svt_jpeg_xs_decoder_close.argtypes = [D]
-            L.svt_jpeg_xs_decoder_close.restype = None
-            L.svt_jpeg_xs_decoder_get_single_frame_size_with_proxy.argtypes = [
-                ctypes.c_void_p, ctypes.c_size_t, ctypes.POINTER(ImageConfig), ctypes.POINTER(ctypes.c_uint32),
-                ctypes.c_uint32, ctypes.c_int32]
-            L.svt_jpeg_xs_decoder_send_frame.argtypes = [D, ctypes.POINTER(Frame), ctypes.c_uint8]
-            L.svt_jpeg_xs_decoder_get_frame.argtypes = [D, ctypes.POINTER(Frame), ctypes.c_uint8]
+        L.svt_jpeg_xs_decoder_init.argtypes = [ctypes.c_uint64, ctypes.c_uint64, D, ctypes.c_void_p, ctypes.c_size_t,
+                                               ctypes.POINTER(ImageConfig)]
+        L.svt_jpeg_xs_decoder_close.argtypes = [D]
+        L.svt_jpeg_xs_decoder_close.restype = None
+        L.svt_jpeg_xs_decoder_get_single_frame_size_with_proxy.argtypes = [
+            ctypes.c_void_p, ctypes.c_size_t, ctypes.POINTER(ImageConfig), ctypes.POINTER(ctypes.c_uint32),
+            ctypes.c_uint32, ctypes.c_int32]
+        L.svt_jpeg_xs_decoder_get_single_frame_size.argtypes = [
+            ctypes.c_void_p, ctypes.c_size_t, ctypes.POINTER(ImageConfig), ctypes.POINTER(ctypes.c_uint32),
+            ctypes.c_uint32]
+        L.svt_jpeg_xs_decoder_send_frame.argtypes = [D, ctypes.POINTER(Frame), ctypes.c_uint8]
+        L.svt_jpeg_xs_decoder_send_packet.argtypes = [D, ctypes.POINTER(Frame), ctypes.POINTER(ctypes.c_uint32)]
+        L.svt_jpeg_xs_decoder_get_frame.argtypes = [D, ctypes.POINTER(Frame), ctypes.c_uint8]
+        L.svt_jpeg_xs_decoder_send_eoc.argtypes = [D]
+        L.svt_jpeg_xs_amd_decode_plan_create.argtypes = [ctypes.POINTER(DecodeSpec), ctypes.c_uint32]
+        L.svt_jpeg_xs_amd_decode_plan_create.restype = ctypes.c_void_p
+        L.svt_jpeg_xs_amd_decode_plan_run.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        L.svt_jpeg_xs_amd_decode_plan_status.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint32)]
+        L.svt_jpeg_xs_amd_decode_plan_destroy.argtypes = [ctypes.c_void_p]
+        L.svt_jpeg_xs_amd_decode_plan_destroy.restype = None
+        L.svt_jpeg_xs_amd_plan_precinct_offsets.argtypes = [ctypes.c_void_p, ctypes.c_uint32,
+                                                            ctypes.POINTER(ctypes.c_void_p),
+                                                            ctypes.POINTER(ctypes.c_uint32)]
+        L.svt_jpeg_xs_amd_plan_debug_copy.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p,
+                                                      ctypes.c_void_p, ctypes.c_void_p]
         L.svt_jpeg_xs_amd_plan_create.argtypes = [ctypes.POINTER(FrameSpec), ctypes.c_uint32]
         L.svt_jpeg_xs_amd_plan_create.restype = ctypes.c_void_p
         L.svt_jpeg_xs_amd_plan_run.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32]
@@ -156,10 +179,12 @@ def lib():
         for name in ("svt_jpeg_xs_encoder_load_default_parameters", "svt_jpeg_xs_encoder_get_image_config",
                      "svt_jpeg_xs_encoder_init", "svt_jpeg_xs_encoder_send_picture", "svt_jpeg_xs_encoder_get_packet",
                      "svt_jpeg_xs_decoder_init", "svt_jpeg_xs_decoder_get_single_frame_size_with_proxy",
-                     "svt_jpeg_xs_decoder_send_frame", "svt_jpeg_xs_decoder_get_frame", "svt_jpeg_xs_amd_plan_run",
-                     "svt_jpeg_xs_amd_plan_run_timed", "svt_jpeg_xs_amd_plan_status"):
-            if hasattr(L, name):
-                getattr(L, name).restype = ctypes.c_int32
+                     "svt_jpeg_xs_decoder_get_single_frame_size", "svt_jpeg_xs_decoder_send_frame",
+                     "svt_jpeg_xs_decoder_send_packet", "svt_jpeg_xs_decoder_get_frame", "svt_jpeg_xs_decoder_send_eoc",
+                     "svt_jpeg_xs_amd_plan_run", "svt_jpeg_xs_amd_plan_run_timed", "svt_jpeg_xs_amd_plan_status",
+                     "svt_jpeg_xs_amd_decode_plan_run", "svt_jpeg_xs_amd_decode_plan_status",
+                     "svt_jpeg_xs_amd_plan_precinct_offsets", "svt_jpeg_xs_amd_plan_debug_copy"):
+            getattr(L, name).restype = ctypes.c_int32
         _lib = L
     return _lib
 
@@ -334,9 +359,63 @@ class Plan:
         rc = lib().svt_jpeg_xs_amd_plan_status(self.handle, errs)
         return rc, list(errs)
 
+    def precinct_offsets(self, frame):
+        ptr = ctypes.c_void_p()
+        stride = ctypes.c_uint32()
+        rc = lib().svt_jpeg_xs_amd_plan_precinct_offsets(self.handle, frame, ctypes.byref(ptr), ctypes.byref(stride))
+        if rc:
+            raise JxsError(rc, "plan_precinct_offsets")
+        return ptr.value, stride.value
+
     def destroy(self):
         if self.handle:
             lib().svt_jpeg_xs_amd_plan_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+class DecodePlan:
+    """A device-resident decode batch (svt_jpeg_xs_amd_decode_plan_*).
+    `frames` is a list of (decoder, codestream device pointer, size,
+    [plane device pointers], [strides], precinct table pointer or None,
+    table stride)."""
+
+    def __init__(self, frames):
+        self.n = len(frames)
+        specs = (DecodeSpec * self.n)()
+        for i, (dec, cs, size, planes, strides, prec, pstride) in enumerate(frames):
+            specs[i].decoder = ctypes.pointer(dec.api)
+            specs[i].d_codestream = cs
+            specs[i].codestream_size = size
+            for c, (p, s) in enumerate(zip(planes, strides)):
+                specs[i].d_planes[c] = p
+                specs[i].strides[c] = s
+            specs[i].d_precinct_offsets = prec
+            specs[i].precinct_offsets_stride = pstride or 1
+        self._specs = specs
+        self._decoders = [f[0] for f in frames]
+        self.handle = lib().svt_jpeg_xs_amd_decode_plan_create(specs, self.n)
+        if not self.handle:
+            raise RuntimeError("svt_jpeg_xs_amd_decode_plan_create failed")
+
+    def run(self, stream=0):
+        rc = lib().svt_jpeg_xs_amd_decode_plan_run(self.handle, stream)
+        if rc:
+            raise JxsError(rc, "decode_plan_run")
+
+    def status(self):
+        errs = (ctypes.c_uint32 * self.n)()
+        rc = lib().svt_jpeg_xs_amd_decode_plan_status(self.handle, errs)
+        return rc, list(errs)
+
+    def destroy(self):
+        if self.handle:
+            lib().svt_jpeg_xs_amd_decode_plan_destroy(self.handle)
             self.handle = None
 
     def __del__(self):

@@ -1,0 +1,52 @@
+/*
+ * jxs_dec.h -- decoder descriptors shared by the host code and the gfx950
+ * decode kernels.
+ */
+#ifndef JXS_DEC_H
+#define JXS_DEC_H
+
+#include <stdint.h>
+
+#include "jxs_geom.h"
+
+/* One codestream of a decode launch. */
+struct JxsDecFrame {
+    const JxsGeom* g;
+    const uint8_t* cs; /* codestream in HBM */
+    uint32_t cs_len;
+    uint32_t hdr_end; /* first slice header */
+    uint32_t* prec_off; /* rows: byte offset of each precinct header */
+    int16_t* coef;      /* band planes (JxsBand::coeff_off), two's complement */
+    uint32_t* status;   /* non-zero: invalid codestream */
+    uint32_t prec_base; /* first precinct of this frame in the launch */
+    const uint32_t* prec_in; /* known precinct offsets (stride prec_in_stride), or NULL: walk the codestream */
+    uint32_t prec_in_stride;
+    uint32_t pad_;
+};
+
+/* Inverse horizontal job: one block per row.  Inputs are the final low band
+ * (int16 band or int32 plane) and the high bands of `levels` levels (finest
+ * first); the output is an int32 row or an output-plane row. */
+struct JxsInvRowJob {
+    const int16_t* lf16; /* low band (if lf32 == NULL), stride lf_stride */
+    const int32_t* lf32;
+    uint32_t lf_stride;
+    uint32_t levels, w, rows, base;
+    const int16_t* hf[5];
+    uint32_t hf_stride[5];
+    int32_t* out32; /* int32 output rows (stride out_stride), or NULL */
+    void* out;      /* output plane (u8 / u16), when out32 == NULL */
+    uint32_t out_stride, depth;
+};
+
+/* Inverse vertical job: int32 low rows then high rows (stride w) -> int32
+ * plane or output plane. */
+struct JxsInvVertJob {
+    const int32_t* src;
+    int32_t* out32;
+    void* out;
+    uint32_t w, h, out_stride, depth;
+    uint32_t tiles_x, tiles, base, pad_;
+};
+
+#endif

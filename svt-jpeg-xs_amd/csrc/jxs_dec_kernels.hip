@@ -1,0 +1,536 @@
+/*
+ * jxs_dec_kernels.hip -- gfx950 kernels of the JPEG XS decode path.
+ *
+ *   k_dec_scan    one lane per codestream: walks slice headers and the
+ *                 precinct length chain (Lprc) to find every precinct; only
+ *                 used when the caller has no precinct table.
+ *   k_dec_unpack  one block per (packet, precinct): packet headers, the
+ *                 significance flags, the unary bit-plane counts decoded in
+ *                 parallel (code j ends at the j-th zero bit: zero counts per
+ *                 32-bit word + block prefix sum), then the data nibbles at
+ *                 prefix-summed bit offsets, dequantised and turned into
+ *                 two's complement (Packing.c:513-868, Dequant.c:9-57,
+ *                 Precinct.c:9-29).
+ *   k_inv_rows    one block per row: all inverse horizontal levels of a row in
+ *                 LDS (Idwt.c:11-53 semantics), into an int32 row or, for
+ *                 components without vertical levels, straight to the output
+ *                 plane with the linear output scaling (NltDec.c:242-265).
+ *   k_inv_vert    inverse vertical 5/3 lifting, one lane per column, with the
+ *                 output scaling fused when it produces the final plane.
+ *
+ * Every codestream read is bounds-checked: a corrupt or truncated
+ * codestream sets the frame's status word and never reads outside it.
+ */
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "jxs_dec.h"
+
+#define WAVE 64
+
+__device__ __forceinline__ uint32_t dtrunc(uint32_t gain, uint32_t prio, uint32_t q, uint32_t r) {
+    const uint32_t pump = prio < r ? 1u : 0u;
+    if (q < gain + pump)
+        return 0;
+    const uint32_t t = q - gain - pump;
+    return t > 15u ? 15u : t;
+}
+
+__device__ __forceinline__ uint32_t cs_byte(const JxsDecFrame& F, uint64_t pos) { return pos < F.cs_len ? F.cs[pos] : 0u; }
+
+/* n (1..32) bits starting at absolute bit position bp, MSB first. */
+__device__ __forceinline__ uint32_t cs_bits(const JxsDecFrame& F, uint64_t bp, uint32_t n) {
+    const uint64_t b = bp >> 3;
+    const uint32_t sh = (uint32_t)(bp & 7);
+    uint64_t acc = 0;
+#pragma unroll
+    for (int k = 0; k < 5; k++)
+        acc = (acc << 8) | cs_byte(F, b + k);
+    return (uint32_t)((acc >> (40 - sh - n)) & (n == 32 ? 0xFFFFFFFFull : ((1ull << n) - 1)));
+}
+
+__device__ __forceinline__ uint32_t dblock_scan(uint32_t v, uint32_t* tmp, uint32_t* total) {
+    const uint32_t lane = threadIdx.x & (WAVE - 1), wid = threadIdx.x / WAVE, nw = blockDim.x / WAVE;
+    uint32_t x = v;
+#pragma unroll
+    for (uint32_t o = 1; o < WAVE; o <<= 1) {
+        uint32_t y = __shfl_up(x, o, WAVE);
+        if (lane >= o)
+            x += y;
+    }
+    if (lane == WAVE - 1)
+        tmp[wid] = x;
+    __syncthreads();
+    uint32_t before = 0, tot = 0;
+    for (uint32_t w = 0; w < nw; w++) {
+        const uint32_t t = tmp[w];
+        before += w < wid ? t : 0u;
+        tot += t;
+    }
+    __syncthreads();
+    *total = tot;
+    return before + x - v;
+}
+
+__device__ __forceinline__ int find_dec_frame(const JxsDecFrame* fr, int nfr, uint32_t blk) {
+    int lo = 0, hi = nfr - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (fr[mid].prec_base <= blk)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+/* One block per codestream: copies a known precinct table, or walks the
+ * slice headers and the precinct length chain (lane 0). */
+__global__ void k_dec_scan(const JxsDecFrame* __restrict__ frames, int nfr) {
+    const JxsDecFrame& F = frames[blockIdx.x];
+    const JxsGeom& g = *F.g;
+    if (F.prec_in) {
+        for (uint32_t r = threadIdx.x; r < g.rows; r += blockDim.x)
+            F.prec_off[r] = F.prec_in[(size_t)r * F.prec_in_stride];
+        return;
+    }
+    if (threadIdx.x != 0)
+        return;
+    uint64_t pos = F.hdr_end;
+    for (uint32_t s = 0; s < g.slices; s++) {
+        if (cs_byte(F, pos) != 0xFF || cs_byte(F, pos + 1) != 0x20 || cs_byte(F, pos + 2) != 0 || cs_byte(F, pos + 3) != 4 ||
+            ((cs_byte(F, pos + 4) << 8) | cs_byte(F, pos + 5)) != s || pos + 6 > F.cs_len) {
+            atomicOr(F.status, 1u);
+            for (uint32_t r = s * g.pps; r < g.rows; r++)
+                F.prec_off[r] = F.cs_len;
+            return;
+        }
+        pos += 6;
+        const uint32_t first = s * g.pps;
+        const uint32_t n = (s == g.slices - 1) ? g.rows - first : g.pps;
+        for (uint32_t i = 0; i < n; i++) {
+            F.prec_off[first + i] = (uint32_t)pos;
+            const uint32_t lprc = (cs_byte(F, pos) << 16) | (cs_byte(F, pos + 1) << 8) | cs_byte(F, pos + 2);
+            pos += g.prec_hdr_bytes + (uint64_t)lprc;
+            if (pos > F.cs_len) {
+                atomicOr(F.status, 1u);
+                for (uint32_t r = first + i + 1; r < g.rows; r++)
+                    F.prec_off[r] = F.cs_len;
+                return;
+            }
+        }
+    }
+}
+
+#define DMAXG 1024 /* groups of one band line (7680 / 2 / 4 = 960) */
+
+struct UnpackCtx {
+    uint8_t dmode[JXS_MAXC][JXS_NBC];
+    uint8_t gtli[JXS_MAXC][JXS_NBC];
+    uint32_t s_bit, g_bit, d_bit, g_end, d_end, raw, ok;
+    uint32_t zpos[DMAXG];
+    uint8_t gl[DMAXG];
+    uint8_t sflag[DMAXG / 8];
+    uint32_t sbase[DMAXG / 8];
+    uint32_t scan[8];
+};
+
+__global__ void __launch_bounds__(256) k_dec_unpack(const JxsDecFrame* __restrict__ frames, int nfr) {
+    __shared__ UnpackCtx C;
+    const int f = find_dec_frame(frames, nfr, blockIdx.y);
+    const JxsDecFrame& F = frames[f];
+    const JxsGeom& g = *F.g;
+    const uint32_t row = blockIdx.y - F.prec_base;
+    const uint32_t p = blockIdx.x;
+    if (p >= g.npk || row >= g.rows)
+        return;
+    const uint32_t type = row + 1 >= g.rows ? 1 : 0;
+    const uint32_t l = g.pk_line[p];
+    if (threadIdx.x == 0) {
+        C.ok = 0;
+        const uint64_t off = F.prec_off[row];
+        const uint32_t lprc = cs_bits(F, off * 8, 24);
+        const uint32_t q = cs_byte(F, off + 3), r = cs_byte(F, off + 4);
+        uint64_t bit = off * 8 + 40;
+        bool bad = false;
+        for (uint32_t gi = 0; gi < g.nb_all; gi++) {
+            if (g.gb[gi] == 0xFF)
+                continue;
+            const uint32_t c = g.gc[gi], b = g.gb[gi];
+            const uint32_t d = cs_bits(F, bit, 2);
+            bit += 2;
+            C.dmode[c][b] = (uint8_t)d;
+            C.gtli[c][b] = (uint8_t)dtrunc(g.c[c].b[b].gain, g.c[c].b[b].prio, q, r);
+            if (d & 1)
+                bad = true; /* vertical prediction: not produced by this library's encoder */
+        }
+        const uint64_t pend = off + g.prec_hdr_bytes + (uint64_t)lprc;
+        uint64_t pos = off + g.prec_hdr_bytes;
+        bool exists = false;
+        for (uint32_t pp = 0; pp <= p && !bad; pp++) {
+            bool any = false;
+            uint32_t sig_bits = 0;
+            for (uint32_t gi = g.pk_bs[pp]; gi < g.pk_be[pp]; gi++) {
+                const JxsBand& bd = g.c[g.gc[gi]].b[g.gb[gi]];
+                if (g.pk_line[pp] < bd.lines[type]) {
+                    any = true;
+                    if (C.dmode[g.gc[gi]][g.gb[gi]] & 2)
+                        sig_bits += bd.sw;
+                }
+            }
+            if (!any)
+                continue;
+            uint32_t raw, ld, lg;
+            if (g.short_hdr) {
+                raw = cs_bits(F, pos * 8, 1);
+                ld = cs_bits(F, pos * 8 + 1, 15);
+                lg = cs_bits(F, pos * 8 + 16, 13);
+            } else {
+                raw = cs_bits(F, pos * 8, 1);
+                ld = cs_bits(F, pos * 8 + 1, 20);
+                lg = cs_bits(F, pos * 8 + 21, 20);
+            }
+            const uint32_t sb = raw ? 0u : (sig_bits + 7) / 8;
+            const uint64_t s_off = pos + g.pkt_hdr_bytes, g_off = s_off + sb, d_off = g_off + lg, nxt = d_off + ld;
+            if (pp == p) {
+                exists = true;
+                C.raw = raw;
+                C.s_bit = (uint32_t)(s_off * 8);
+                C.g_bit = (uint32_t)(g_off * 8);
+                C.g_end = (uint32_t)(d_off * 8);
+                C.d_bit = (uint32_t)(d_off * 8);
+                C.d_end = (uint32_t)(nxt * 8);
+            }
+            pos = nxt;
+        }
+        if (pend > F.cs_len || pos > pend || (uint64_t)pend * 8 >= 0xFFFFFFFFull)
+            bad = true;
+        if (bad)
+            atomicOr(F.status, 1u);
+        C.ok = (!bad && exists) ? 1u : 0u;
+    }
+    __syncthreads();
+    if (!C.ok)
+        return;
+    for (uint32_t gi = g.pk_bs[p]; gi < g.pk_be[p]; gi++) {
+        const uint32_t c = g.gc[gi], b = g.gb[gi];
+        const JxsBand& bd = g.c[c].b[b];
+        if (l >= bd.lines[type])
+            continue;
+        const uint32_t gt = C.gtli[c][b], gw = bd.gw, sw = bd.sw;
+        const bool use_sig = (C.dmode[c][b] & 2) != 0;
+        const uint32_t y = row * bd.hl + l;
+        bool err = false;
+        if (C.raw) {
+            const uint32_t g0 = C.g_bit;
+            for (uint32_t i = threadIdx.x; i < gw; i += blockDim.x)
+                C.gl[i] = (uint8_t)cs_bits(F, (uint64_t)g0 + 4 * i, 4);
+            __syncthreads();
+            if (threadIdx.x == 0)
+                C.g_bit = g0 + 4 * gw;
+        } else {
+            /* significance flags: 1 = insignificant group of 8 GCLIs */
+            uint32_t ncodes = gw;
+            if (use_sig) {
+                const uint32_t s0 = C.s_bit;
+                uint32_t tot = 0;
+                for (uint32_t base = 0; base < sw; base += blockDim.x) {
+                    const uint32_t s = base + threadIdx.x;
+                    uint32_t cnt = 0;
+                    if (s < sw) {
+                        const uint32_t fl = cs_bits(F, (uint64_t)s0 + s, 1);
+                        C.sflag[s] = (uint8_t)fl;
+                        const uint32_t size = min(8u, gw - 8 * s);
+                        cnt = fl ? 0u : size;
+                    }
+                    uint32_t t;
+                    const uint32_t ex = dblock_scan(cnt, C.scan, &t);
+                    if (s < sw)
+                        C.sbase[s] = tot + ex;
+                    tot += t;
+                }
+                ncodes = tot;
+                __syncthreads();
+                if (threadIdx.x == 0)
+                    C.s_bit = s0 + sw;
+            }
+            /* unary codes: the j-th zero bit from the start ends code j */
+            const uint32_t g0 = C.g_bit, gend = C.g_end;
+            uint32_t found = 0;
+            for (uint32_t w = g0 >> 5; found < ncodes; w += blockDim.x) {
+                if ((uint64_t)w * 32 >= gend) {
+                    err = true;
+                    break;
+                }
+                const uint32_t wi = w + threadIdx.x;
+                uint32_t word = 0xFFFFFFFFu;
+                if ((uint64_t)wi * 32 < gend) {
+                    word = (cs_byte(F, 4ull * wi) << 24) | (cs_byte(F, 4ull * wi + 1) << 16) | (cs_byte(F, 4ull * wi + 2) << 8) |
+                        cs_byte(F, 4ull * wi + 3);
+                    const uint32_t lo = wi * 32;
+                    if (lo < g0)
+                        word |= ~(0xFFFFFFFFu >> (g0 - lo)); /* bits before the line start */
+                    if (lo + 32 > gend)
+                        word |= (0xFFFFFFFFu >> (gend - lo)); /* bits after the sub-packet end */
+                }
+                uint32_t z = ~word;
+                uint32_t t;
+                const uint32_t ex = dblock_scan((uint32_t)__popc(z), C.scan, &t);
+                uint32_t k = found + ex;
+                while (z) {
+                    const uint32_t lead = __clz(z);
+                    if (k < ncodes && k < DMAXG)
+                        C.zpos[k] = wi * 32 + lead;
+                    k++;
+                    z &= ~(0x80000000u >> lead);
+                }
+                found += t;
+                __syncthreads();
+            }
+            if (!err) {
+                for (uint32_t i = threadIdx.x; i < gw; i += blockDim.x) {
+                    uint32_t j = i;
+                    bool coded = true;
+                    if (use_sig) {
+                        const uint32_t s = i / 8;
+                        coded = !C.sflag[s];
+                        j = C.sbase[s] + (i - 8 * s);
+                    }
+                    uint32_t gc = gt;
+                    if (coded) {
+                        const uint32_t start = j == 0 ? g0 : C.zpos[j - 1] + 1;
+                        gc = gt + (C.zpos[j] - start);
+                    }
+                    if (gc > 15) {
+                        gc = 15;
+                        atomicOr(F.status, 1u);
+                    }
+                    C.gl[i] = (uint8_t)gc;
+                }
+                __syncthreads();
+                if (threadIdx.x == 0)
+                    C.g_bit = ncodes ? C.zpos[ncodes - 1] + 1 : g0;
+            }
+        }
+        if (err) {
+            if (threadIdx.x == 0)
+                atomicOr(F.status, 1u);
+            return;
+        }
+        __syncthreads();
+        /* data: 4 sign bits then (gcli - gtli) bit planes of 4 bits per group */
+        const uint32_t d0 = C.d_bit;
+        int16_t* crow = F.coef + bd.coeff_off + (size_t)y * bd.w;
+        uint32_t pos = 0;
+        for (uint32_t base = 0; base < gw; base += blockDim.x) {
+            const uint32_t i = base + threadIdx.x;
+            uint32_t gc = 0, nb = 0;
+            if (i < gw) {
+                gc = C.gl[i];
+                nb = gc > gt ? 4 * (gc - gt + 1) : 0;
+            }
+            uint32_t t;
+            const uint32_t ex = dblock_scan(nb, C.scan, &t);
+            if (i < gw) {
+                uint32_t mag[4] = {0, 0, 0, 0}, sg = 0;
+                if (nb) {
+                    const uint64_t bp = (uint64_t)d0 + pos + ex;
+                    if (bp + nb > C.d_end) {
+                        atomicOr(F.status, 1u);
+                    } else {
+                        sg = cs_bits(F, bp, 4);
+                        uint32_t k = 0;
+                        for (int bit = (int)gc - 1; bit >= (int)gt; bit--, k++) {
+                            const uint32_t nib = cs_bits(F, bp + 4 + 4 * k, 4);
+#pragma unroll
+                            for (int t4 = 0; t4 < 4; t4++)
+                                mag[t4] |= ((nib >> (3 - t4)) & 1u) << bit;
+                        }
+                    }
+                }
+#pragma unroll
+                for (uint32_t t4 = 0; t4 < 4; t4++) {
+                    const uint32_t ci = 4 * i + t4;
+                    if (ci >= bd.w)
+                        continue;
+                    uint32_t m = mag[t4];
+                    if (gt > 0 && m) {
+                        if (g.qpih == 0) {
+                            m |= 1u << (gt - 1);
+                        } else {
+                            const uint32_t sc = gc - gt + 1;
+                            uint32_t acc = 0;
+                            for (uint32_t x = m; x > 0; x >>= sc)
+                                acc += x;
+                            m = acc & 0xFFFFu;
+                        }
+                    }
+                    const bool neg = (sg >> (3 - t4)) & 1u;
+                    crow[ci] = (int16_t)(neg ? -(int32_t)(m & 0x7FFF) : (int32_t)(m & 0x7FFF));
+                }
+            }
+            pos += t;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0)
+            C.d_bit = d0 + pos;
+        __syncthreads();
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* inverse transform                                                   */
+/* ------------------------------------------------------------------ */
+__device__ __forceinline__ uint32_t nlt_out(int32_t v, uint32_t depth) {
+    const int32_t dz = 20 - (int32_t)depth;
+    int64_t x = (int64_t)v + (1 << 19);
+    x = (x + ((1ll << dz) >> 1)) >> dz;
+    const int64_t m = (1ll << depth) - 1;
+    return (uint32_t)(x < 0 ? 0 : (x > m ? m : x));
+}
+
+__device__ __forceinline__ void store_out(void* out, size_t idx, int32_t v, uint32_t depth) {
+    const uint32_t o = nlt_out(v, depth);
+    if (depth <= 8)
+        ((uint8_t*)out)[idx] = (uint8_t)o;
+    else
+        ((uint16_t*)out)[idx] = (uint16_t)o;
+}
+
+__device__ __forceinline__ int find_base(const uint32_t* bases, int n, uint32_t blk) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (bases[mid] <= blk)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+__global__ void __launch_bounds__(256) k_inv_rows(const JxsInvRowJob* __restrict__ jobs, const uint32_t* __restrict__ bases,
+                                                  int njobs) {
+    __shared__ int32_t buf[JXS_MAX_ROW];
+    const JxsInvRowJob& J = jobs[find_base(bases, njobs, blockIdx.x)];
+    const uint32_t y = blockIdx.x - J.base;
+    uint32_t n[6];
+    n[0] = J.w;
+    for (uint32_t k = 0; k < J.levels; k++)
+        n[k + 1] = (n[k] + 1) / 2;
+    const uint32_t L = J.levels;
+    for (uint32_t i = threadIdx.x; i < n[L]; i += blockDim.x)
+        buf[i << L] = J.lf32 ? J.lf32[(size_t)y * J.lf_stride + i] : (int32_t)J.lf16[(size_t)y * J.lf_stride + i] * 256;
+    for (uint32_t k = 0; k < L; k++) {
+        const uint32_t nh = n[k] / 2;
+        const int16_t* h = J.hf[k] + (size_t)y * J.hf_stride[k];
+        for (uint32_t i = threadIdx.x; i < nh; i += blockDim.x)
+            buf[(2 * i + 1) << k] = (int32_t)h[i] * 256;
+    }
+    __syncthreads();
+    for (int k = (int)L - 1; k >= 0; k--) {
+        const uint32_t s = 1u << k, nk = n[k], nh = nk / 2, nl = nk - nh;
+        for (uint32_t i = threadIdx.x; i < nl; i += blockDim.x) {
+            if (i == 0)
+                buf[0] -= (buf[s] + 1) >> 1;
+            else if (i < nh)
+                buf[2 * i * s] -= (buf[(2 * i - 1) * s] + buf[(2 * i + 1) * s] + 2) >> 2;
+            else
+                buf[2 * i * s] -= (buf[(2 * i - 1) * s] + 1) >> 1;
+        }
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < nh; i += blockDim.x) {
+            if (2 * i + 2 < nk)
+                buf[(2 * i + 1) * s] += (buf[2 * i * s] + buf[(2 * i + 2) * s]) >> 1;
+            else
+                buf[(2 * i + 1) * s] += buf[2 * i * s];
+        }
+        __syncthreads();
+    }
+    if (J.out32) {
+        int32_t* o = J.out32 + (size_t)y * J.out_stride;
+        for (uint32_t i = threadIdx.x; i < J.w; i += blockDim.x)
+            o[i] = buf[i];
+    } else {
+        for (uint32_t i = threadIdx.x; i < J.w; i += blockDim.x)
+            store_out(J.out, (size_t)y * J.out_stride + i, buf[i], J.depth);
+    }
+}
+
+#define IVSEG 16
+
+__global__ void __launch_bounds__(256) k_inv_vert(const JxsInvVertJob* __restrict__ jobs, const uint32_t* __restrict__ bases,
+                                                  int njobs) {
+    const JxsInvVertJob J = jobs[find_base(bases, njobs, blockIdx.x)];
+    const uint32_t t = blockIdx.x - J.base;
+    const uint32_t tx = t % J.tiles_x, ty = t / J.tiles_x;
+    const uint32_t x = tx * blockDim.x + threadIdx.x;
+    if (x >= J.w)
+        return;
+    const uint32_t n = J.h, nh = n / 2, nl = n - nh;
+    const int32_t* Lp = J.src;
+    const int32_t* Hp = J.src + (size_t)nl * J.w;
+    auto Lv = [&](uint32_t i) { return Lp[(size_t)i * J.w + x]; };
+    auto Hv = [&](uint32_t i) { return Hp[(size_t)i * J.w + x]; };
+    auto even = [&](uint32_t i) -> int32_t {
+        if (i == 0)
+            return Lv(0) - ((Hv(0) + 1) >> 1);
+        if (i < nh)
+            return Lv(i) - ((Hv(i - 1) + Hv(i) + 2) >> 2);
+        return Lv(i) - ((Hv(i - 1) + 1) >> 1);
+    };
+    const uint32_t i0 = ty * IVSEG, i1 = min(i0 + IVSEG, nl);
+    int32_t e = even(i0);
+    for (uint32_t i = i0; i < i1; i++) {
+        int32_t e_next = 0, odd = 0;
+        const bool has_odd = i < nh;
+        if (has_odd) {
+            if (2 * i + 2 < n) {
+                e_next = even(i + 1);
+                odd = Hv(i) + ((e + e_next) >> 1);
+            } else {
+                odd = Hv(i) + e;
+            }
+        }
+        if (J.out32) {
+            J.out32[(size_t)(2 * i) * J.w + x] = e;
+            if (has_odd)
+                J.out32[(size_t)(2 * i + 1) * J.w + x] = odd;
+        } else {
+            store_out(J.out, (size_t)(2 * i) * J.out_stride + x, e, J.depth);
+            if (has_odd)
+                store_out(J.out, (size_t)(2 * i + 1) * J.out_stride + x, odd, J.depth);
+        }
+        e = e_next;
+    }
+}
+
+extern "C" {
+
+hipError_t jxs_launch_dec_scan(const JxsDecFrame* frames, int nfr, hipStream_t st) {
+    hipLaunchKernelGGL(k_dec_scan, dim3(nfr), dim3(256), 0, st, frames, nfr);
+    return hipGetLastError();
+}
+
+hipError_t jxs_launch_dec_unpack(const JxsDecFrame* frames, int nfr, uint32_t max_npk, uint32_t precincts, hipStream_t st) {
+    if (!precincts)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_dec_unpack, dim3(max_npk, precincts), dim3(256), 0, st, frames, nfr);
+    return hipGetLastError();
+}
+
+hipError_t jxs_launch_inv_rows(const JxsInvRowJob* jobs, const uint32_t* bases, int njobs, uint32_t blocks, hipStream_t st) {
+    if (!njobs || !blocks)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_inv_rows, dim3(blocks), dim3(256), 0, st, jobs, bases, njobs);
+    return hipGetLastError();
+}
+
+hipError_t jxs_launch_inv_vert(const JxsInvVertJob* jobs, const uint32_t* bases, int njobs, uint32_t blocks, hipStream_t st) {
+    if (!njobs || !blocks)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_inv_vert, dim3(blocks), dim3(256), 0, st, jobs, bases, njobs);
+    return hipGetLastError();
+}
+
+}  // extern "C"

@@ -706,6 +706,18 @@ PREFIX_API SvtJxsErrorType_t svt_jpeg_xs_amd_plan_debug_copy(svt_jpeg_xs_amd_pla
     return SvtJxsErrorNone;
 }
 
+PREFIX_API SvtJxsErrorType_t svt_jpeg_xs_amd_plan_precinct_offsets(svt_jpeg_xs_amd_plan_t* plan, uint32_t frame,
+                                                                   const uint32_t** d_offsets, uint32_t* stride) {
+    if (!plan || frame >= plan->nfr || !d_offsets || !stride)
+        return SvtJxsErrorBadParameter;
+    JxsFrameDesc F;
+    if (hipMemcpy(&F, plan->d_frames + frame, sizeof(F), hipMemcpyDeviceToHost) != hipSuccess)
+        return SvtJxsErrorUndefined;
+    *d_offsets = &F.prec->offset;
+    *stride = sizeof(JxsPrec) / sizeof(uint32_t);
+    return SvtJxsErrorNone;
+}
+
 PREFIX_API const char* svt_jpeg_xs_amd_version(void) { return "svt-jpeg-xs-amd 0.1 (gfx950, API 0.10)"; }
 
 }  // extern "C"

@@ -52,15 +52,23 @@ def make_plane(w, h, depth, seed, sigma=4.0):
     mid = float(1 << (depth - 1))
     xs = np.arange(w, dtype=np.float64)
     ys = np.arange(h, dtype=np.float64)
-    v = np.full((h, w), mid, dtype=np.float64)
-    for _ in range(3):
+    left = np.empty((h, 6), dtype=np.float64)
+    right = np.empty((6, w), dtype=np.float64)
+    for k in range(3):
         px = int(rng.integers(64, 513))
         py = int(rng.integers(64, 513))
         phi = float(rng.uniform(0.0, 2.0 * np.pi))
         ax = 2.0 * np.pi * xs / px
         by = 2.0 * np.pi * ys / py + phi
-        # sin(ax + by) = sin(ax)cos(by) + cos(ax)sin(by): separable, two outer products
-        v += amp * (np.outer(np.cos(by), np.sin(ax)) + np.outer(np.sin(by), np.cos(ax)))
+        # sin(ax + by) = sin(ax)cos(by) + cos(ax)sin(by): the three sinusoids
+        # are one rank-6 product
+        left[:, 2 * k] = np.cos(by)
+        left[:, 2 * k + 1] = np.sin(by)
+        right[2 * k] = np.sin(ax)
+        right[2 * k + 1] = np.cos(ax)
+    v = left @ right
+    v *= amp
+    v += mid
     for _ in range(20):
         rw = int(rng.integers(8, max(9, w // 4 + 1)))
         rh = int(rng.integers(8, max(9, h // 4 + 1)))
@@ -68,10 +76,31 @@ def make_plane(w, h, depth, seed, sigma=4.0):
         y0 = int(rng.integers(0, max(1, h - rh + 1)))
         val = float(rng.integers(0, vmax + 1))
         v[y0:y0 + rh, x0:x0 + rw] = val
-    v += rng.standard_normal((h, w)) * sigma
-    v = np.sign(v) * np.floor(np.abs(v) + 0.5)
+    noise = rng.standard_normal((h, w))
+    noise *= sigma
+    v += noise
+    # round half away from zero, then clip: every value below -0.5 clips to 0
+    # either way, so floor(v + 0.5) gives the same result after the clip
+    v += 0.5
+    np.floor(v, out=v)
     np.clip(v, 0, vmax, out=v)
     return v.astype(np.uint8 if depth <= 8 else np.uint16)
+
+
+def _frame_job(args):
+    return make_frame(*args)
+
+
+def make_frames(width, height, depth, fmt, count, base=BASE_SEED, sigma=4.0, workers=None):
+    """`count` frames with per-frame seeds, generated by a process pool."""
+    import multiprocessing as mp
+    import os
+    jobs = [(width, height, depth, fmt, f, base, sigma) for f in range(count)]
+    n = workers or min(16, os.cpu_count() or 1, count)
+    if n <= 1:
+        return [_frame_job(j) for j in jobs]
+    with mp.get_context("fork").Pool(n) as pool:
+        return pool.map(_frame_job, jobs)
 
 
 def make_frame(width, height, depth, fmt, frame=0, base=BASE_SEED, sigma=4.0):

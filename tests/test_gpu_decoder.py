@@ -1,0 +1,109 @@
+"""GPU decoder parity: decoded planes against the oracle's decoder, through
+the SvtJpegxsDec C-ABI and through device-resident decode plans."""
+import numpy as np
+import pytest
+
+from jxs_util import load_oracle, load_package, load_synth
+
+CASES = [
+    (64, 32, 8, 3, 5, 2, 5, 0, 1),
+    (256, 64, 10, 3, 3, 2, 5, 0, 1),
+    (200, 200, 8, 4, 3, 1, 5, 0, 1),
+    (201, 203, 10, 4, 2, 2, 5, 0, 1),
+    (256, 128, 8, 2, 2, 2, 5, 0, 1),
+    (128, 96, 8, 2, 2, 1, 3, 0, 1),
+    (160, 90, 12, 4, 4, 2, 5, 1, 1),
+    (160, 90, 10, 3, 5, 2, 5, 0, 0),
+    (96, 64, 14, 21, 2, 1, 5, 0, 1),
+    (128, 64, 11, 4, 3, 0, 3, 0, 1),
+    (1024, 768, 10, 3, 20, 2, 5, 0, 1),
+]
+
+
+def _codestream(cfg, frame=0, sigma=4.0):
+    W, H, D, F, bpp, V, Hh, q, sig = cfg
+    S, O = load_synth(), load_oracle()
+    planes = S.make_frame(W, H, D, F, frame=frame, sigma=sigma)
+    e, cs = O.encode(O.make_params(W, H, D, F, bpp, 1, V, Hh, q, 16, sig), planes)
+    assert e == 0
+    return planes, cs
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("cfg", CASES, ids=lambda c: "x".join(str(v) for v in c))
+def test_decode_matches_oracle(cfg):
+    print("case", cfg, flush=True)
+    P, O = load_package(), load_oracle()
+    _, cs = _codestream(cfg)
+    e, ref = O.decode(cs)
+    assert e == 0
+    dec = P.Decoder(cs)
+    rc, got = dec.decode(cs)
+    dec.close()
+    assert rc == 0
+    assert len(got) == len(ref)
+    for c, (a, b) in enumerate(zip(got, ref)):
+        assert a.shape == b.shape
+        assert np.array_equal(a, b), "component %d differs at %s" % (c, np.argwhere(a != b)[:3].tolist())
+
+
+@pytest.mark.gpu
+def test_decode_corrupt_codestream_returns_error_or_frame():
+    """Bit flips in the entropy-coded data must never read outside the
+    codestream: the decoder returns (an error or a frame), the process lives."""
+    P = load_package()
+    _, cs = _codestream((256, 64, 10, 3, 3, 2, 5, 0, 1))
+    rng = np.random.default_rng(5)
+    dec = P.Decoder(cs)
+    for k in range(8):
+        bad = bytearray(cs)
+        for pos in rng.integers(200, len(cs) - 2, size=8):
+            bad[int(pos)] ^= int(rng.integers(1, 256))
+        rc, _ = dec.decode(bytes(bad))
+        assert rc in (0, P.ERR_DEC_INVALID_BITSTREAM, P.ERR_DEC_TOO_SHORT)
+    rc, _ = dec.decode(cs[: len(cs) // 2])
+    assert rc != 0
+    dec.close()
+
+
+@pytest.mark.gpu
+def test_device_roundtrip_packed_rgb14_v1():
+    """Config-5 shape at reduced size: packed RGB 14-bit, bpp 2, V1 H5, sigma
+    16; GPU encode plan -> GPU decode plan fed with the encoder's precinct
+    table; codestream and planes bit-exact with the oracle."""
+    import torch
+    P, O, S = load_package(), load_oracle(), load_synth()
+    W, H, D, F = 1024, 512, 14, 21
+    frames = [S.make_frame(W, H, D, F, frame=i, sigma=16.0) for i in range(3)]
+    enc = P.Encoder(W, H, D, F, 2, 1, 1, 5)
+    dev = torch.device("cuda:0")
+    d_in = [torch.from_numpy(f[0].view(np.int16)).to(dev) for f in frames]
+    d_cs = torch.zeros((3, enc.bytes_per_frame), dtype=torch.uint8, device=dev)
+    plan = P.Plan([(enc, [d_in[i].data_ptr()], [W * 3], d_cs[i].data_ptr()) for i in range(3)])
+    plan.run()
+    torch.cuda.synchronize()
+    assert plan.status()[0] == 0
+    cs0 = d_cs[0].cpu().numpy().tobytes()
+    dec = P.Decoder(cs0)
+    d_out = [[torch.zeros((H, W), dtype=torch.int16, device=dev) for _ in range(3)] for _ in range(3)]
+    specs = []
+    for i in range(3):
+        prec, stride = plan.precinct_offsets(i)
+        specs.append((dec, d_cs[i].data_ptr(), enc.bytes_per_frame, [t.data_ptr() for t in d_out[i]], [W] * 3,
+                      prec if i != 2 else None, stride))
+    dplan = P.DecodePlan(specs)
+    dplan.run()
+    torch.cuda.synchronize()
+    assert dplan.status()[0] == 0
+    prm = O.make_params(W, H, D, F, 2, 1, 1, 5)
+    for i in range(3):
+        e, ref_cs = O.encode(prm, frames[i])
+        got_cs = d_cs[i].cpu().numpy().tobytes()
+        assert e == 0 and got_cs == ref_cs, "codestream %d differs" % i
+        e, ref = O.decode(ref_cs)
+        assert e == 0
+        for c in range(3):
+            got = d_out[i][c].cpu().numpy().view(np.uint16)
+            assert np.array_equal(got, ref[c]), "frame %d component %d differs" % (i, c)
+    dplan.destroy()
+    plan.destroy()
