@@ -1396,8 +1396,9 @@ static int32_t parse_header(const uint8_t* cs, uint32_t len, dec_hdr* d) {
             lh = br_get(&r, 1);
             br_get(&r, 1); /* Rl */
             d->qpih = br_get(&r, 2);
-            if (br_get(&r, 2) != 0 || br_get(&r, 2) != 0) /* Fs, Rm */
+            if (br_get(&r, 2) != 0) /* Fs: separate sign coding is not decoded here */
                 return E_INVALID_BS;
+            br_get(&r, 2); /* Rm: only matters for vertically predicted bands, rejected per precinct */
             have_pih = 1;
             break;
         case 0xFF13: /* CDT */
@@ -1683,3 +1684,31 @@ int32_t jxso_decode(const uint8_t* cs, uint32_t len, void* const planes[4], jxso
         }
     return e;
 }
+
+/* ----------------------------------------------------------------------
+ * Test hooks: one line of quantization() (Quant.c:9-60, no gtli == 0 skip)
+ * and the 1-D lifting steps, for the known-answer and transform tests.
+ * ---------------------------------------------------------------------- */
+void jxso_quantize(uint16_t* buf, uint32_t n, const uint8_t* gclis, uint8_t gtli, uint32_t uniform) {
+    for (uint32_t i = 0; i < n; i++) {
+        const uint8_t gcli = gclis[i / 4];
+        if (gcli <= gtli) {
+            buf[i] = 0;
+            continue;
+        }
+        const uint16_t sign = buf[i] & 0x8000u, m = buf[i] & 0x7FFFu;
+        uint16_t o;
+        if (!uniform) {
+            o = (uint16_t)((m >> gtli) << gtli);
+        } else {
+            const uint8_t sc = gcli - gtli + 1;
+            const uint16_t d = (uint16_t)((((uint32_t)m << sc) - m + (1u << gcli)) >> (gcli + 1));
+            o = (uint16_t)(d << gtli);
+        }
+        buf[i] = o ? (uint16_t)(o | sign) : 0;
+    }
+}
+
+void jxso_fwd53(const int32_t* x, int32_t* lf, int32_t* hf, uint32_t n) { fwd53(x, 1, lf, 1, hf, 1, n); }
+
+void jxso_inv53(const int32_t* lf, const int32_t* hf, int32_t* x, uint32_t n) { inv53(lf, 1, hf, 1, x, 1, n); }

@@ -72,6 +72,12 @@ void jxso_bsearch_init(jxso_bsearch_t* b, uint32_t beg, uint32_t end, uint8_t be
  * returns 0 continue, 1 found, 2 error. */
 int32_t jxso_bsearch_next(jxso_bsearch_t* b, int32_t result, uint32_t* next);
 
+/* Test hooks: quantization() of one line (Quant.c semantics) and the 1-D
+ * forward / inverse 5/3 lifting with symmetric extension. */
+void jxso_quantize(uint16_t* buf, uint32_t n, const uint8_t* gclis, uint8_t gtli, uint32_t uniform);
+void jxso_fwd53(const int32_t* x, int32_t* lf, int32_t* hf, uint32_t n);
+void jxso_inv53(const int32_t* lf, const int32_t* hf, int32_t* x, uint32_t n);
+
 #ifdef __cplusplus
 }
 #endif

@@ -25,7 +25,52 @@ class ImageInfo(ctypes.Structure):
                 ("comps", ctypes.c_uint32), ("comp_w", ctypes.c_uint32 * 4), ("comp_h", ctypes.c_uint32 * 4)]
 
 
+class BSearch(ctypes.Structure):
+    _fields_ = [("beg", ctypes.c_int32), ("end", ctypes.c_int32), ("step", ctypes.c_int32), ("best", ctypes.c_int32),
+                ("last", ctypes.c_int32), ("below", ctypes.c_uint8)]
+
+
 _lib = None
+
+
+def quantize(values, gclis, gtli, uniform):
+    buf = np.array(values, dtype=np.uint16)
+    g = np.array(gclis, dtype=np.uint8)
+    lib().jxso_quantize(buf.ctypes.data, len(buf), g.ctypes.data, gtli, 1 if uniform else 0)
+    return buf
+
+
+def fwd53(x):
+    x = np.ascontiguousarray(x, dtype=np.int32)
+    n = len(x)
+    lf = np.zeros((n + 1) // 2, dtype=np.int32)
+    hf = np.zeros(n // 2, dtype=np.int32)
+    lib().jxso_fwd53(x.ctypes.data, lf.ctypes.data, hf.ctypes.data, n)
+    return lf, hf
+
+
+def inv53(lf, hf):
+    lf = np.ascontiguousarray(lf, dtype=np.int32)
+    hf = np.ascontiguousarray(hf, dtype=np.int32)
+    x = np.zeros(len(lf) + len(hf), dtype=np.int32)
+    lib().jxso_inv53(lf.ctypes.data, hf.ctypes.data, x.ctypes.data, len(x))
+    return x
+
+
+def binary_search(begin, end, below, step, verdict):
+    """Runs the rate control's binary search; verdict(i) -> 1 too small,
+    2 too big, 3 out of range.  Returns the found index or None."""
+    b = BSearch()
+    lib().jxso_bsearch_init(ctypes.byref(b), begin, end, below, step)
+    nxt = ctypes.c_uint32(0)
+    res = 0
+    while True:
+        r = lib().jxso_bsearch_next(ctypes.byref(b), res, ctypes.byref(nxt))
+        if r == 1:
+            return nxt.value
+        if r == 2:
+            return None
+        res = verdict(nxt.value)
 
 
 def build():
@@ -42,7 +87,8 @@ def build():
 def lib():
     global _lib
     if _lib is None:
-        if not os.path.exists(LIB_PATH):
+        src = os.path.join(HERE, "jxs_oracle.c")
+        if not os.path.exists(LIB_PATH) or (os.path.exists(src) and os.path.getmtime(src) > os.path.getmtime(LIB_PATH)):
             build()
         L = ctypes.CDLL(LIB_PATH)
         L.jxso_default_params.argtypes = [ctypes.POINTER(Params)]
@@ -58,6 +104,17 @@ def lib():
         L.jxso_probe.restype = ctypes.c_int32
         L.jxso_decode.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p * 4, ctypes.POINTER(ImageInfo)]
         L.jxso_decode.restype = ctypes.c_int32
+        L.jxso_quantize.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_uint8, ctypes.c_uint32]
+        L.jxso_quantize.restype = None
+        L.jxso_fwd53.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32]
+        L.jxso_fwd53.restype = None
+        L.jxso_inv53.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32]
+        L.jxso_inv53.restype = None
+        L.jxso_bsearch_init.argtypes = [ctypes.POINTER(BSearch), ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint8,
+                                        ctypes.c_uint32]
+        L.jxso_bsearch_init.restype = None
+        L.jxso_bsearch_next.argtypes = [ctypes.POINTER(BSearch), ctypes.c_int32, ctypes.POINTER(ctypes.c_uint32)]
+        L.jxso_bsearch_next.restype = ctypes.c_int32
         _lib = L
     return _lib
 

@@ -1,0 +1,199 @@
+"""Writes the golden fixtures of tests/golden from the reference SVT-JPEG-XS
+library built from its own sources (oracle/Makefile.ref -> oracle/_ref).
+
+    make -f oracle/Makefile.ref && python tests/golden/make_golden.py
+
+Fixtures (data only):
+  golden.json         per case: generator parameters, SHA-256 of the input
+                      planes, of the reference codestream and of the reference
+                      decoder's planes; struct sizes of the reference API;
+                      the quantisation known-answer vectors.
+  <case>.jxs          reference codestreams of the small cases.
+  sample_16x16_422.jxs  the 16x16 4:2:2 codestream the reference's unit tests
+                      decode (tests/UnitTests/SampleFramesData.cc:8-33).
+"""
+import hashlib
+import json
+import os
+import re
+import subprocess
+import sys
+import tempfile
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+REF = "/root/reference"
+
+from jxs_util import load_oracle, load_synth  # noqa: E402
+
+# name: (W, H, depth, format, bpp_num, bpp_den, decomp_v, decomp_h, quant, slice_h, significance, sigma, frame)
+SMALL = {
+    "s_64x32_422_8b_bpp5_v2": (64, 32, 8, 3, 5, 1, 2, 5, 0, 16, 1, 4.0, 0),
+    "s_256x64_422_10b_bpp3_v2": (256, 64, 10, 3, 3, 1, 2, 5, 0, 16, 1, 4.0, 1),
+    "s_200x200_444_8b_bpp3_v1": (200, 200, 8, 4, 3, 1, 1, 5, 0, 16, 1, 4.0, 2),
+    "s_201x203_444_10b_bpp2_v2": (201, 203, 10, 4, 2, 1, 2, 5, 0, 16, 1, 4.0, 3),
+    "s_256x128_420_8b_bpp2_v2": (256, 128, 8, 2, 2, 1, 2, 5, 0, 16, 1, 4.0, 4),
+    "s_128x96_420_8b_bpp2_v1h3": (128, 96, 8, 2, 2, 1, 1, 3, 0, 16, 1, 4.0, 5),
+    "s_160x90_444_12b_bpp4_uniform": (160, 90, 12, 4, 4, 1, 2, 5, 1, 16, 1, 4.0, 6),
+    "s_160x90_422_10b_bpp5_nosig": (160, 90, 10, 3, 5, 1, 2, 5, 0, 16, 0, 4.0, 7),
+    "s_96x64_rgbpacked_14b_bpp2_v1": (96, 64, 14, 21, 2, 1, 1, 5, 0, 16, 1, 16.0, 8),
+    "s_128x64_444_11b_bpp3_v0h3": (128, 64, 11, 4, 3, 1, 0, 3, 0, 16, 1, 4.0, 9),
+    "s_130x66_422_9b_bpp4_v1h2": (130, 66, 9, 3, 4, 1, 1, 2, 0, 16, 1, 4.0, 10),
+    "s_320x180_422_10b_bpp7_2_slice32": (320, 180, 10, 3, 7, 2, 2, 5, 0, 32, 1, 4.0, 11),
+    "s_256x192_422_10b_bpp20": (256, 192, 10, 3, 20, 1, 2, 5, 0, 16, 1, 4.0, 12),
+    "s_30x60_444_8b_bpp4_error": (30, 60, 8, 4, 4, 1, 2, 5, 0, 16, 1, 4.0, 13),
+}
+
+# The configurations of the benchmark workloads, first frame of each.
+FULL = {
+    "c1_1920x1080_422_8b_bpp5_v2": (1920, 1080, 8, 3, 5, 1, 2, 5, 0, 16, 1, 4.0, 0),
+    "c2_3840x2160_422_10b_bpp3_v2": (3840, 2160, 10, 3, 3, 1, 2, 5, 0, 16, 1, 4.0, 0),
+    "c3a_1280x720_420_8b_bpp1_v2": (1280, 720, 8, 2, 1, 1, 2, 5, 0, 16, 1, 4.0, 0),
+    "c3b_1920x1080_444_12b_bpp4_v2": (1920, 1080, 12, 4, 4, 1, 2, 5, 0, 16, 1, 4.0, 0),
+    "c3c_3840x2160_rgb_10b_bpp3_v2": (3840, 2160, 10, 4, 3, 1, 2, 5, 0, 16, 1, 4.0, 0),
+    "c3d_1920x540_422_10b_bpp5_v1": (1920, 540, 10, 3, 5, 1, 1, 5, 0, 16, 1, 4.0, 0),
+    "c5_7680x4320_rgbpacked_14b_bpp2_v1": (7680, 4320, 14, 21, 2, 1, 1, 5, 0, 16, 1, 16.0, 0),
+}
+
+# Quantisation known answers restated from tests/UnitTests/TestQuant.cc:27-136
+# are read from the reference test source as data.
+
+
+def sha(b):
+    return hashlib.sha256(b).hexdigest()
+
+
+def ref_encode(cfg, planes, td):
+    W, H, D, F, bn, bd, V, Hh, q, sh, sig, _, _ = cfg
+    O = load_oracle()
+    raw = os.path.join(td, "in.raw")
+    out = os.path.join(td, "out.jxs")
+    S = load_synth()
+    with open(raw, "wb") as f:
+        f.write(S.frame_bytes(planes))
+    cmd = [O.REF_ENC, raw, out, str(W), str(H), str(D), str(F), str(bn), str(bd), str(V), str(Hh), "1", "1", str(sh),
+           str(q), str(sig)]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        return None, r.returncode
+    return open(out, "rb").read(), 0
+
+
+def ref_decode(cs, td):
+    O = load_oracle()
+    p = os.path.join(td, "d.jxs")
+    o = os.path.join(td, "d.raw")
+    open(p, "wb").write(cs)
+    r = subprocess.run([O.REF_DEC, p, o, "1", "1"], capture_output=True, text=True)
+    if r.returncode != 0:
+        return None
+    return open(o, "rb").read()
+
+
+def case_entry(name, cfg, td, keep):
+    S = load_synth()
+    W, H, D, F, bn, bd, V, Hh, q, sh, sig, sigma, frame = cfg
+    planes = S.make_frame(W, H, D, F, frame=frame, sigma=sigma)
+    cs, rc = ref_encode(cfg, planes, td)
+    e = {"params": dict(zip(["width", "height", "depth", "format", "bpp_num", "bpp_den", "decomp_v", "decomp_h",
+                             "quantization", "slice_height", "significance", "sigma", "frame"], cfg)),
+         "input_sha256": S.frame_sha256(planes)}
+    if cs is None:
+        e["reference_error"] = True
+        return e
+    e["codestream_sha256"] = sha(cs)
+    e["codestream_bytes"] = len(cs)
+    dec = ref_decode(cs, td)
+    e["decoded_sha256"] = sha(dec) if dec is not None else None
+    if keep:
+        open(os.path.join(HERE, name + ".jxs"), "wb").write(cs)
+    return e
+
+
+def api_struct_sizes():
+    """sizeof / offsetof of the reference API structs, from the reference
+    headers (compiled here with gcc)."""
+    prog = r'''
+#include <stdio.h>
+#include <stddef.h>
+#include "SvtJpegxsEnc.h"
+#include "SvtJpegxsDec.h"
+int main(void) {
+  printf("{\"svt_jpeg_xs_encoder_api_t\": %zu, \"svt_jpeg_xs_decoder_api_t\": %zu, \"svt_jpeg_xs_frame_t\": %zu,"
+         " \"svt_jpeg_xs_image_config_t\": %zu, \"svt_jpeg_xs_image_buffer_t\": %zu,"
+         " \"svt_jpeg_xs_bitstream_buffer_t\": %zu, \"enc.private_ptr\": %zu, \"enc.padding\": %zu,"
+         " \"enc.slice_packetization_mode\": %zu, \"enc.use_cpu_flags\": %zu, \"dec.private_ptr\": %zu,"
+         " \"dec.proxy_mode\": %zu, \"frame.bitstream\": %zu, \"frame.user_prv_ctx_ptr\": %zu}\n",
+    sizeof(svt_jpeg_xs_encoder_api_t), sizeof(svt_jpeg_xs_decoder_api_t), sizeof(svt_jpeg_xs_frame_t),
+    sizeof(svt_jpeg_xs_image_config_t), sizeof(svt_jpeg_xs_image_buffer_t), sizeof(svt_jpeg_xs_bitstream_buffer_t),
+    offsetof(svt_jpeg_xs_encoder_api_t, private_ptr), offsetof(svt_jpeg_xs_encoder_api_t, padding),
+    offsetof(svt_jpeg_xs_encoder_api_t, slice_packetization_mode), offsetof(svt_jpeg_xs_encoder_api_t, use_cpu_flags),
+    offsetof(svt_jpeg_xs_decoder_api_t, private_ptr), offsetof(svt_jpeg_xs_decoder_api_t, proxy_mode),
+    offsetof(svt_jpeg_xs_frame_t, bitstream), offsetof(svt_jpeg_xs_frame_t, user_prv_ctx_ptr));
+  return 0;
+}
+'''
+    with tempfile.TemporaryDirectory() as td:
+        c = os.path.join(td, "s.c")
+        open(c, "w").write(prog)
+        exe = os.path.join(td, "s")
+        subprocess.check_call(["gcc", "-I" + REF + "/Source/API", c, "-o", exe])
+        return json.loads(subprocess.check_output([exe]).decode())
+
+
+def quant_kats():
+    src = open(os.path.join(REF, "tests/UnitTests/TestQuant.cc")).read()
+
+    def arr(name):
+        body = src[src.index(name):]
+        body = body[body.index("{") + 1:body.index("};")]
+        return [int(x, 16) if x.startswith("0x") else int(x) for x in re.findall(r"0x[0-9a-fA-F]+|\b\d+\b", body)]
+
+    inp = arr("quant_input[QUANT_MAX_SIZE]")
+    gclis = arr("gclis_input[GCLI_SIZE]")
+    vec = src[src.index("quant_test_vector[5]"):]
+    vec = vec[:vec.index("};\n\nclass")]
+    tests = []
+    for block in vec.split("/* Test")[1:]:
+        nums = re.findall(r"0x[0-9a-fA-F]+|\b\d+\b|QUANT_TYPE_DEADZONE|QUANT_TYPE_UNIFORM", block)
+        nums = nums[1:]  # drop the test number of the comment
+        buf_len, gtli, qt = int(nums[0]), int(nums[1]), nums[2]
+        out = [int(x, 16) if x.startswith("0x") else int(x) for x in nums[3:]]
+        tests.append({"buf_len": buf_len, "gtli": gtli, "uniform": qt == "QUANT_TYPE_UNIFORM", "expected": out})
+    return {"input": inp, "gclis": gclis, "tests": tests}
+
+
+def sample_codestream():
+    src = open(os.path.join(REF, "tests/UnitTests/SampleFramesData.cc")).read()
+    body = src[src.index("Frame_Sample_1_16x16_8bit_422_bitstream[] = {"):]
+    body = body[body.index("{") + 1:body.index("};")]
+    return bytes(int(x, 16) for x in re.findall(r"0x[0-9A-Fa-f]+", body))
+
+
+def main():
+    O = load_oracle()
+    if not O.have_reference():
+        sys.exit("build the reference first: make -f oracle/Makefile.ref")
+    out = {"small": {}, "full": {}}
+    with tempfile.TemporaryDirectory() as td:
+        for name, cfg in SMALL.items():
+            out["small"][name] = case_entry(name, cfg, td, keep=True)
+            print(name, "ok", flush=True)
+        for name, cfg in FULL.items():
+            out["full"][name] = case_entry(name, cfg, td, keep=False)
+            print(name, "ok", flush=True)
+        cs = sample_codestream()
+        open(os.path.join(HERE, "sample_16x16_422.jxs"), "wb").write(cs)
+        dec = ref_decode(cs, td)
+        out["sample_16x16_422"] = {"codestream_sha256": sha(cs), "codestream_bytes": len(cs),
+                                   "decoded_sha256": sha(dec), "decoded_bytes": len(dec),
+                                   "uses_vertical_prediction": True}
+    out["api_struct_sizes"] = api_struct_sizes()
+    out["quant_kat"] = quant_kats()
+    with open(os.path.join(HERE, "golden.json"), "w") as f:
+        json.dump(out, f, indent=1, sort_keys=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -55,3 +55,45 @@ def test_1080p_422_8bit_bpp5_bit_exact():
     e_ref, ref, rc, got = _encode_both((1920, 1080, 8, 3, 5, 2, 5, 0, 1))
     assert e_ref == 0 and rc == 0
     assert got == ref
+
+
+# ---- against the reference's own codestreams (tests/golden) ----
+import hashlib  # noqa: E402
+import json  # noqa: E402
+import os  # noqa: E402
+
+from jxs_util import ROOT  # noqa: E402
+
+_GOLDEN = os.path.join(ROOT, "tests", "golden")
+with open(os.path.join(_GOLDEN, "golden.json")) as _f:
+    _G = json.load(_f)
+
+
+def _gpu_encode(p):
+    P, S = load_package(), load_synth()
+    planes = S.make_frame(p["width"], p["height"], p["depth"], p["format"], frame=p["frame"], sigma=p["sigma"])
+    enc = P.Encoder(p["width"], p["height"], p["depth"], p["format"], p["bpp_num"], p["bpp_den"], p["decomp_v"],
+                    p["decomp_h"], p["quantization"], p["slice_height"], p["significance"])
+    rc, cs = enc.encode(planes)
+    enc.close()
+    return rc, cs
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", sorted(n for n in _G["small"] if not _G["small"][n].get("reference_error")))
+def test_gpu_codestream_equals_reference_golden(name):
+    print("golden", name, flush=True)
+    rc, cs = _gpu_encode(_G["small"][name]["params"])
+    assert rc == 0
+    assert cs == open(os.path.join(_GOLDEN, name + ".jxs"), "rb").read()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", sorted(_G["full"]))
+def test_gpu_full_size_workloads_equal_reference(name):
+    """First frame of every benchmark workload at full size (1080p .. 8K):
+    the GPU codestream hashes to the reference encoder's."""
+    print("full", name, flush=True)
+    rc, cs = _gpu_encode(_G["full"][name]["params"])
+    assert rc == 0
+    assert hashlib.sha256(cs).hexdigest() == _G["full"][name]["codestream_sha256"]
