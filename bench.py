@@ -25,9 +25,9 @@ sys.path.insert(0, os.path.join(ROOT, "tests"))
 W, H, DEPTH, FMT, BPP, DV, DH = 3840, 2160, 10, 3, 3, 2, 5
 NFRAMES = 64
 HBM_PEAK_GBS = 8000.0
-STAGES = ["vert1", "rows1", "vert2", "rows2", "rc_table", "select", "pack"]
-KERNELS = {"vert1": "k_vert", "rows1": "k_horz", "vert2": "k_vert", "rows2": "k_horz", "rc_table": "k_rc_table",
-           "select": "k_select", "pack": "k_pack"}
+STAGES = ["dwt1", "dwt2", "rc_table", "select", "plan", "pack"]
+KERNELS = {"dwt1": "k_dwt", "dwt2": "k_dwt", "rc_table": "k_rc_table", "select": "k_select", "plan": "k_plan",
+           "pack": "k_pack"}
 
 
 def log(*a):
@@ -87,6 +87,9 @@ def main():
     ap.add_argument("--frames", type=int, default=NFRAMES)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-check", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="run one device-to-device copy of MIB MiB first: the reference point for the "
+                         "FETCH_SIZE / WRITE_SIZE counters of a profiled run")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (every box here has one MI355X); "
@@ -110,8 +113,9 @@ def main():
 
     log("uploading frames to HBM")
     d_in = [[torch.from_numpy(p.view(np.int16) if p.dtype == np.uint16 else p).to(dev) for p in fr] for fr in frames]
-    d_out_plain = torch.zeros((nf, lcod), dtype=torch.uint8, device=dev)
-    d_out_batch = torch.zeros((nf, lcod), dtype=torch.uint8, device=dev)
+    ostride = (lcod + 3) & ~3  # codestream buffers are 4-byte aligned
+    d_out_plain = torch.zeros((nf, ostride), dtype=torch.uint8, device=dev)
+    d_out_batch = torch.zeros((nf, ostride), dtype=torch.uint8, device=dev)
 
     def spec(i, out):
         return (enc, [t.data_ptr() for t in d_in[i]], [t.shape[1] for t in d_in[i]], out[i].data_ptr())
@@ -120,6 +124,15 @@ def main():
     batch = P.Plan([spec(i, d_out_batch) for i in range(nf)])
     stream = torch.cuda.current_stream(dev)
     sh = stream.cuda_stream
+
+    if args.copy_calibration:
+        n = args.copy_calibration << 20
+        log("calibration copy of %d MiB" % args.copy_calibration)
+        a = torch.ones(n // 4, dtype=torch.int32, device=dev)
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
 
     log("warmup")
     for _ in range(args.warmup):
@@ -182,8 +195,8 @@ def main():
         errs += rc != 0
     rc, _ = batch.status()
     errs += rc != 0
-    plain_host = d_out_plain.cpu().numpy()
-    batch_host = d_out_batch.cpu().numpy()
+    plain_host = d_out_plain[:, :lcod].cpu().numpy()
+    batch_host = d_out_batch[:, :lcod].cpu().numpy()
     same = bool(np.array_equal(plain_host, batch_host))
     exact = None
     if not args.no_check:

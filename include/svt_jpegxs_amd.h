@@ -28,7 +28,7 @@ typedef struct svt_jpeg_xs_amd_frame_spec {
     svt_jpeg_xs_encoder_api_t* encoder;
     const void* d_planes[MAX_COMPONENTS_NUM];
     uint32_t strides[MAX_COMPONENTS_NUM];
-    uint8_t* d_out; /* >= bytes_per_frame of the encoder */
+    uint8_t* d_out; /* 4-byte aligned, >= bytes_per_frame of the encoder rounded up to a multiple of 4 */
 } svt_jpeg_xs_amd_frame_spec_t;
 
 typedef struct svt_jpeg_xs_amd_plan svt_jpeg_xs_amd_plan_t;
@@ -44,8 +44,9 @@ PREFIX_API SvtJxsErrorType_t svt_jpeg_xs_amd_plan_run(svt_jpeg_xs_amd_plan_t* pl
 
 /* Same as plan_run without a graph, recording a HIP event around every
  * kernel on `stream`; waits and returns the milliseconds of each stage in
- * stage_ms[0..6] (vertical 1, rows 1, vertical 2, rows 2, rate table,
- * select, pack). */
+ * stage_ms[0 .. SVT_JPEG_XS_AMD_STAGES - 1] (transform level 1, transform
+ * level 2, rate table, select, packet plan, pack). */
+#define SVT_JPEG_XS_AMD_STAGES 6
 PREFIX_API SvtJxsErrorType_t svt_jpeg_xs_amd_plan_run_timed(svt_jpeg_xs_amd_plan_t* plan, void* stream, float* stage_ms);
 
 /* After the stream has completed: per-frame error flags (0 = ok), count

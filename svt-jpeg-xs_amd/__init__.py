@@ -19,6 +19,10 @@ LIB_PATH = os.path.join(HERE, "libsvtjpegxs_amd.so")
 
 API_MAJOR, API_MINOR = 0, 10
 
+# Stages timed by Plan.run_timed (SVT_JPEG_XS_AMD_STAGES, include/svt_jpegxs_amd.h)
+STAGE_NAMES = ["dwt1", "dwt2", "rc_table", "select", "plan", "pack"]
+STAGES = len(STAGE_NAMES)
+
 # SvtJxsErrorType_t values (Source/API/SvtJpegxs.h:130-159)
 ERR_NONE = 0
 ERR_BAD_PARAMETER = -0x7FFFEFFB          # 0x80001005
@@ -348,7 +352,7 @@ class Plan:
             raise JxsError(rc, "plan_run")
 
     def run_timed(self, stream=0):
-        ms = (ctypes.c_float * 7)()
+        ms = (ctypes.c_float * STAGES)()
         rc = lib().svt_jpeg_xs_amd_plan_run_timed(self.handle, stream, ms)
         if rc:
             raise JxsError(rc, "plan_run_timed")

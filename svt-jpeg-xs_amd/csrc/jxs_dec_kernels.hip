@@ -320,7 +320,7 @@ __global__ void __launch_bounds__(256) k_dec_unpack(const JxsDecFrame* __restric
         __syncthreads();
         /* data: 4 sign bits then (gcli - gtli) bit planes of 4 bits per group */
         const uint32_t d0 = C.d_bit;
-        int16_t* crow = F.coef + bd.coeff_off + (size_t)y * bd.w;
+        int16_t* crow = F.coef + bd.coeff_off + (size_t)y * bd.cs;
         uint32_t pos = 0;
         for (uint32_t base = 0; base < gw; base += blockDim.x) {
             const uint32_t i = base + threadIdx.x;

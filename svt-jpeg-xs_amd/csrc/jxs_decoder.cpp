@@ -97,11 +97,11 @@ static void inv_jobs(const JxsGeom& g, int16_t* coef, int32_t* scratch, void* co
         if (cp.dv == 0) {
             JxsInvRowJob j = rowjob(cp.w, cp.h);
             j.lf16 = band(0);
-            j.lf_stride = cp.b[0].w;
+            j.lf_stride = cp.b[0].cs;
             j.levels = cp.dh;
             for (uint32_t k = 0; k < cp.dh; k++) {
                 j.hf[k] = band(nb - 1 - k);
-                j.hf_stride[k] = cp.b[nb - 1 - k].w;
+                j.hf_stride[k] = cp.b[nb - 1 - k].cs;
             }
             j.out = planes[c];
             j.out_stride = strides[c];
@@ -114,26 +114,26 @@ static void inv_jobs(const JxsGeom& g, int16_t* coef, int32_t* scratch, void* co
         scr += (size_t)w * h + (size_t)w1 * h1;
         JxsInvRowJob lo = rowjob(w, h1);
         lo.hf[0] = band(nb - 3);
-        lo.hf_stride[0] = cp.b[nb - 3].w;
+        lo.hf_stride[0] = cp.b[nb - 3].cs;
         if (cp.dv == 1) {
             lo.lf16 = band(0);
-            lo.lf_stride = cp.b[0].w;
+            lo.lf_stride = cp.b[0].cs;
             lo.levels = cp.dh;
             for (uint32_t k = 1; k < cp.dh; k++) {
                 lo.hf[k] = band(cp.dh - k);
-                lo.hf_stride[k] = cp.b[cp.dh - k].w;
+                lo.hf_stride[k] = cp.b[cp.dh - k].cs;
             }
         } else {
             const uint32_t g1 = (h1 + 1) / 2, g2 = h1 / 2;
             JxsInvRowJob l2 = rowjob(w1, g1);
             l2.lf16 = band(0);
-            l2.lf_stride = cp.b[0].w;
+            l2.lf_stride = cp.b[0].cs;
             l2.levels = 1 + (cp.dh - 2);
             l2.hf[0] = band(nb - 6);
-            l2.hf_stride[0] = cp.b[nb - 6].w;
+            l2.hf_stride[0] = cp.b[nb - 6].cs;
             for (uint32_t k = 1; k < l2.levels; k++) {
                 l2.hf[k] = band(cp.dh - 2 - (k - 1));
-                l2.hf_stride[k] = cp.b[cp.dh - 2 - (k - 1)].w;
+                l2.hf_stride[k] = cp.b[cp.dh - 2 - (k - 1)].cs;
             }
             l2.out32 = A;
             l2.out_stride = w1;
@@ -141,10 +141,10 @@ static void inv_jobs(const JxsGeom& g, int16_t* coef, int32_t* scratch, void* co
             if (g2 > 0) {
                 JxsInvRowJob h2j = rowjob(w1, g2);
                 h2j.lf16 = band(nb - 5);
-                h2j.lf_stride = cp.b[nb - 5].w;
+                h2j.lf_stride = cp.b[nb - 5].cs;
                 h2j.levels = 1;
                 h2j.hf[0] = band(nb - 4);
-                h2j.hf_stride[0] = cp.b[nb - 4].w;
+                h2j.hf_stride[0] = cp.b[nb - 4].cs;
                 h2j.out32 = A + (size_t)g1 * w1;
                 h2j.out_stride = w1;
                 J.rows[0].push_back(h2j);
@@ -168,10 +168,10 @@ static void inv_jobs(const JxsGeom& g, int16_t* coef, int32_t* scratch, void* co
         if (h2 > 0) {
             JxsInvRowJob hi = rowjob(w, h2);
             hi.lf16 = band(nb - 2);
-            hi.lf_stride = cp.b[nb - 2].w;
+            hi.lf_stride = cp.b[nb - 2].cs;
             hi.levels = 1;
             hi.hf[0] = band(nb - 1);
-            hi.hf_stride[0] = cp.b[nb - 1].w;
+            hi.hf_stride[0] = cp.b[nb - 1].cs;
             hi.out32 = A + (size_t)h1 * w;
             hi.out_stride = w;
             J.rows[1].push_back(hi);

@@ -19,11 +19,11 @@
 #include "jxs_host.h"
 
 extern "C" {
-hipError_t jxs_launch_vert(const JxsVertJob*, const uint32_t*, int, uint32_t, hipStream_t);
-hipError_t jxs_launch_horz(const JxsHorzJob*, const uint32_t*, int, uint32_t, hipStream_t);
+hipError_t jxs_launch_dwt(const JxsDwtJob*, const uint32_t*, int, uint32_t, uint32_t, hipStream_t);
 hipError_t jxs_launch_rc_table(const JxsFrameDesc*, int, uint32_t, hipStream_t);
 hipError_t jxs_launch_select(const JxsFrameDesc*, int, uint32_t, hipStream_t);
-hipError_t jxs_launch_pack(const JxsFrameDesc*, int, uint32_t, uint32_t, hipStream_t);
+hipError_t jxs_launch_plan(const JxsFrameDesc*, int, uint32_t, hipStream_t);
+hipError_t jxs_launch_pack(const JxsFrameDesc*, int, uint32_t, hipStream_t);
 hipError_t jxs_launch_zero(uint32_t*, uint32_t, hipStream_t);
 }
 
@@ -75,9 +75,10 @@ static uint64_t frame_work_bytes(const JxsGeom& g) {
     b += align256(g.n_gcli);
     b += align256(g.n_smax);
     b += align256(g.n_i32 * 4);
-    b += align256((uint64_t)g.rows * g.n_bl * JXS_LUT * 4);
+    b += align256((uint64_t)g.rows * g.n_bl * JXS_HREC * 2);
     b += align256((uint64_t)g.rows * JXS_NQ * JXS_NR * 4);
     b += align256((uint64_t)g.rows * sizeof(JxsPrec));
+    b += align256((uint64_t)g.rows * sizeof(JxsPlan));
     return b;
 }
 
@@ -92,13 +93,12 @@ struct svt_jpeg_xs_amd_plan {
     void* d_work = nullptr;
     uint32_t* d_status = nullptr;
     JxsFrameDesc* d_frames = nullptr;
-    JxsVertJob* d_vert[2] = {nullptr, nullptr};
-    uint32_t* d_vbase[2] = {nullptr, nullptr};
-    JxsHorzJob* d_horz[2] = {nullptr, nullptr};
-    uint32_t* d_hbase[2] = {nullptr, nullptr};
-    int nvert[2] = {0, 0}, nhorz[2] = {0, 0};
-    uint32_t vblocks[2] = {0, 0}, hblocks[2] = {0, 0};
-    uint32_t total_prec = 0, total_slices = 0, max_npk = 0;
+    JxsDwtJob* d_dwt[2] = {nullptr, nullptr};
+    uint32_t* d_dbase[2] = {nullptr, nullptr};
+    int ndwt[2] = {0, 0};
+    uint32_t dblocks[2] = {0, 0};
+    uint32_t dwmax[2] = {0, 0}; /* widest row of each transform stage */
+    uint32_t total_prec = 0, total_slices = 0, total_units = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
     hipStream_t graph_stream = nullptr;
@@ -143,10 +143,8 @@ static void plan_free(svt_jpeg_xs_amd_plan_t* p) {
     (void)hipFree(p->d_status);
     (void)hipFree(p->d_frames);
     for (int s = 0; s < 2; s++) {
-        (void)hipFree(p->d_vert[s]);
-        (void)hipFree(p->d_vbase[s]);
-        (void)hipFree(p->d_horz[s]);
-        (void)hipFree(p->d_hbase[s]);
+        (void)hipFree(p->d_dwt[s]);
+        (void)hipFree(p->d_dbase[s]);
     }
     delete p;
 }
@@ -171,7 +169,9 @@ static svt_jpeg_xs_amd_plan_t* plan_build(const svt_jpeg_xs_amd_frame_spec_t* sp
     p->nfr = count;
     uint64_t work = 0;
     for (uint32_t i = 0; i < count; i++) {
-        if (!specs[i].encoder || !specs[i].encoder->private_ptr || !specs[i].d_out) {
+        /* the packing kernel writes the codestream as aligned dwords */
+        if (!specs[i].encoder || !specs[i].encoder->private_ptr || !specs[i].d_out ||
+            ((uintptr_t)specs[i].d_out & 3) != 0) {
             plan_free(p);
             return nullptr;
         }
@@ -184,10 +184,9 @@ static svt_jpeg_xs_amd_plan_t* plan_build(const svt_jpeg_xs_amd_frame_spec_t* sp
         return nullptr;
     }
     std::vector<JxsFrameDesc> frames(count);
-    std::vector<JxsVertJob> vj[2];
-    std::vector<JxsHorzJob> hj[2];
+    std::vector<JxsDwtJob> dj[2];
     uint8_t* cur = (uint8_t*)p->d_work;
-    uint32_t prec_base = 0, slice_base = 0;
+    uint32_t prec_base = 0, slice_base = 0, unit_base = 0;
     for (uint32_t i = 0; i < count; i++) {
         const DevGeom* dg = p->geoms[i];
         const JxsGeom& g = dg->host.g;
@@ -204,54 +203,47 @@ static svt_jpeg_xs_amd_plan_t* plan_build(const svt_jpeg_xs_amd_frame_spec_t* sp
         cur += align256(g.n_smax);
         int32_t* scratch = (int32_t*)cur;
         cur += align256(g.n_i32 * 4);
-        F.lut = (uint32_t*)cur;
-        cur += align256((uint64_t)g.rows * g.n_bl * JXS_LUT * 4);
+        F.hist = (uint16_t*)cur;
+        cur += align256((uint64_t)g.rows * g.n_bl * JXS_HREC * 2);
         F.table = (uint32_t*)cur;
         cur += align256((uint64_t)g.rows * JXS_NQ * JXS_NR * 4);
         F.prec = (JxsPrec*)cur;
         cur += align256((uint64_t)g.rows * sizeof(JxsPrec));
+        F.plan = (JxsPlan*)cur;
+        cur += align256((uint64_t)g.rows * sizeof(JxsPlan));
         F.out = specs[i].d_out;
         F.status = p->d_status + i;
         F.rows = g.rows;
         F.npk = g.npk;
         F.prec_base = prec_base;
         F.slice_base = slice_base;
+        F.unit_base = unit_base;
+        F.n_units = g.n_units;
         prec_base += g.rows;
         slice_base += g.slices;
-        p->max_npk = g.npk > p->max_npk ? g.npk : p->max_npk;
+        unit_base += g.rows * g.n_units;
         FwdJobs jobs;
-        fwd_jobs(g, specs[i].d_planes, specs[i].strides, F.coeff, F.gcli, F.smax, scratch, jobs);
+        fwd_jobs(g, specs[i].d_planes, specs[i].strides, F.coeff, F.gcli, F.smax, F.hist, scratch, jobs);
         for (int s = 0; s < 2; s++) {
-            vj[s].insert(vj[s].end(), jobs.vert[s].begin(), jobs.vert[s].end());
-            hj[s].insert(hj[s].end(), jobs.horz[s].begin(), jobs.horz[s].end());
+            dj[s].insert(dj[s].end(), jobs.dwt[s].begin(), jobs.dwt[s].end());
         }
     }
     p->total_prec = prec_base;
     p->total_slices = slice_base;
+    p->total_units = unit_base;
     bool ok = upload_vec(frames, &p->d_frames) == hipSuccess;
     for (int s = 0; s < 2 && ok; s++) {
-        std::vector<uint32_t> vb, hb;
+        std::vector<uint32_t> bases;
         uint32_t blk = 0;
-        for (auto& v : vj[s]) {
-            const uint32_t nl = (v.h + 1) / 2;
-            v.tiles_x = (v.w + 255) / 256;
-            v.tiles = v.tiles_x * ((nl + 15) / 16);
-            v.base = blk;
-            vb.push_back(blk);
-            blk += v.tiles;
+        for (auto& j : dj[s]) {
+            j.base = blk;
+            bases.push_back(blk);
+            blk += j.rows;
+            p->dwmax[s] = j.w > p->dwmax[s] ? j.w : p->dwmax[s];
         }
-        p->vblocks[s] = blk;
-        blk = 0;
-        for (auto& h : hj[s]) {
-            h.base = blk;
-            hb.push_back(blk);
-            blk += h.rows;
-        }
-        p->hblocks[s] = blk;
-        p->nvert[s] = (int)vj[s].size();
-        p->nhorz[s] = (int)hj[s].size();
-        ok = upload_vec(vj[s], &p->d_vert[s]) == hipSuccess && upload_vec(vb, &p->d_vbase[s]) == hipSuccess &&
-            upload_vec(hj[s], &p->d_horz[s]) == hipSuccess && upload_vec(hb, &p->d_hbase[s]) == hipSuccess;
+        p->dblocks[s] = blk;
+        p->ndwt[s] = (int)dj[s].size();
+        ok = upload_vec(dj[s], &p->d_dwt[s]) == hipSuccess && upload_vec(bases, &p->d_dbase[s]) == hipSuccess;
     }
     if (!ok) {
         plan_free(p);
@@ -260,8 +252,8 @@ static svt_jpeg_xs_amd_plan_t* plan_build(const svt_jpeg_xs_amd_frame_spec_t* sp
     return p;
 }
 
-/* The launch sequence of a plan.  ev (optional) gets 8 events recorded
- * around the 7 stages. */
+/* The launch sequence of a plan.  ev (optional) gets an event recorded
+ * before every stage and after the last. */
 static hipError_t plan_launch(svt_jpeg_xs_amd_plan_t* p, hipStream_t st, hipEvent_t* ev) {
     hipError_t e;
     const int nf = (int)p->nfr;
@@ -275,15 +267,14 @@ static hipError_t plan_launch(svt_jpeg_xs_amd_plan_t* p, hipStream_t st, hipEven
     } while (0)
     if ((e = jxs_launch_zero(p->d_status, p->nfr, st)) != hipSuccess)
         return e;
-    STEP(0, jxs_launch_vert(p->d_vert[0], p->d_vbase[0], p->nvert[0], p->vblocks[0], st));
-    STEP(1, jxs_launch_horz(p->d_horz[0], p->d_hbase[0], p->nhorz[0], p->hblocks[0], st));
-    STEP(2, jxs_launch_vert(p->d_vert[1], p->d_vbase[1], p->nvert[1], p->vblocks[1], st));
-    STEP(3, jxs_launch_horz(p->d_horz[1], p->d_hbase[1], p->nhorz[1], p->hblocks[1], st));
-    STEP(4, jxs_launch_rc_table(p->d_frames, nf, p->total_prec, st));
-    STEP(5, jxs_launch_select(p->d_frames, nf, p->total_slices, st));
-    STEP(6, jxs_launch_pack(p->d_frames, nf, p->max_npk, p->total_prec, st));
+    STEP(0, jxs_launch_dwt(p->d_dwt[0], p->d_dbase[0], p->ndwt[0], p->dblocks[0], p->dwmax[0], st));
+    STEP(1, jxs_launch_dwt(p->d_dwt[1], p->d_dbase[1], p->ndwt[1], p->dblocks[1], p->dwmax[1], st));
+    STEP(2, jxs_launch_rc_table(p->d_frames, nf, p->total_prec, st));
+    STEP(3, jxs_launch_select(p->d_frames, nf, p->total_slices, st));
+    STEP(4, jxs_launch_plan(p->d_frames, nf, p->total_prec, st));
+    STEP(5, jxs_launch_pack(p->d_frames, nf, p->total_units, st));
     if (ev)
-        (void)hipEventRecord(ev[7], st);
+        (void)hipEventRecord(ev[SVT_JPEG_XS_AMD_STAGES], st);
 #undef STEP
     return hipSuccess;
 }
@@ -378,7 +369,7 @@ static SvtJxsErrorType_t encoder_create(svt_jpeg_xs_encoder_api_t* api) {
     api->private_ptr = e;
     e->slots.resize(kSlots);
     for (auto& s : e->slots) {
-        if (hipMalloc(&s.d_in, e->in_bytes) != hipSuccess || hipMalloc(&s.d_out, g.lcod) != hipSuccess ||
+        if (hipMalloc(&s.d_in, e->in_bytes) != hipSuccess || hipMalloc(&s.d_out, ((size_t)g.lcod + 3) & ~(size_t)3) != hipSuccess ||
             hipEventCreateWithFlags(&s.done, hipEventDisableTiming) != hipSuccess ||
             hipHostMalloc((void**)&s.h_status, sizeof(uint32_t), 0) != hipSuccess) {
             api->private_ptr = nullptr;
@@ -658,16 +649,16 @@ PREFIX_API SvtJxsErrorType_t svt_jpeg_xs_amd_plan_run_timed(svt_jpeg_xs_amd_plan
     if (!plan || !stage_ms)
         return SvtJxsErrorBadParameter;
     hipStream_t st = (hipStream_t)stream;
-    hipEvent_t ev[8];
-    for (int i = 0; i < 8; i++)
+    hipEvent_t ev[SVT_JPEG_XS_AMD_STAGES + 1];
+    for (int i = 0; i <= SVT_JPEG_XS_AMD_STAGES; i++)
         if (hipEventCreate(&ev[i]) != hipSuccess)
             return SvtJxsErrorInsufficientResources;
     hipError_t e = plan_launch(plan, st, ev);
     if (e == hipSuccess)
-        e = hipEventSynchronize(ev[7]);
-    for (int i = 0; i < 7 && e == hipSuccess; i++)
+        e = hipEventSynchronize(ev[SVT_JPEG_XS_AMD_STAGES]);
+    for (int i = 0; i < SVT_JPEG_XS_AMD_STAGES && e == hipSuccess; i++)
         (void)hipEventElapsedTime(&stage_ms[i], ev[i], ev[i + 1]);
-    for (int i = 0; i < 8; i++)
+    for (int i = 0; i <= SVT_JPEG_XS_AMD_STAGES; i++)
         (void)hipEventDestroy(ev[i]);
     return e == hipSuccess ? SvtJxsErrorNone : SvtJxsErrorUndefined;
 }

@@ -21,15 +21,20 @@
 #define JXS_NBL 64  /* band lines per precinct (<= 4 components x 13) */
 #define JXS_NQ 32   /* quantisation values 0..31 */
 #define JXS_NR 32   /* refinement values 0..31 (<= bands - 1) */
-#define JXS_LUT 52  /* u32 per band-line rate-control table */
 #define JXS_HDR_MAX 256
+#define JXS_NU 256 /* packing units per precinct: packet headers + band lines of each sub-packet */
+
+/* kinds of packing units (the sub-packets of a packet, in codestream order) */
+enum { UNIT_HDR = 0, UNIT_SIG = 1, UNIT_GCLI = 2, UNIT_DATA = 3 };
 #define JXS_MAX_ROW 7680 /* widest component row the row kernels stage in LDS */
 
-/* Offsets inside one band-line rate-control table (u32 units). */
-#define LUT_CNT 0   /* [16] groups with gcli <= g */
-#define LUT_DATA 16 /* [16] sum over gcli > g of (gcli - g + 1) */
-#define LUT_SIG 32  /* [16] full significance groups with max <= g */
-#define LUT_TAIL 48 /* max of the partial significance group, or -1 */
+/* Band-line histogram record (u16 units), written by the transform for
+ * every band line of every precinct:
+ *   [0, 16)   groups with GCLI == v
+ *   [16, 31)  full significance groups (8 GCLIs) with maximum == v, v < 15
+ *   31        maximum of the trailing partial significance group, or 0xFFFF */
+#define JXS_HREC 32
+#define HREC_TAIL 31
 
 enum { JXS_SRC_U8 = 0, JXS_SRC_U16 = 1, JXS_SRC_I32 = 2 };
 
@@ -39,7 +44,7 @@ struct JxsBand {
     uint32_t gain, prio;
     uint32_t lines[2]; /* band lines in a normal / the last precinct row */
     uint32_t bl;       /* first band-line index of this band inside a precinct */
-    uint32_t pad_;
+    uint32_t cs;       /* coefficient row stride in elements (4 * gw: every group 8-byte aligned) */
     uint64_t coeff_off; /* u16 elements into the frame's coefficient buffer */
     uint64_t gcli_off;  /* bytes into the frame's GCLI buffer */
     uint64_t smax_off;  /* bytes into the frame's significance-max buffer */
@@ -59,53 +64,71 @@ struct JxsGeom {
     uint32_t pkt_hdr_bytes;  /* 5 (short) or 7 (long) */
     uint32_t rc_hdr_bytes[2];
     uint32_t n_bl; /* band lines per precinct */
+    uint8_t ex_gid[JXS_NGB]; /* global band index of the e-th existing band */
     uint32_t pk_exist[2];
     uint32_t empty_mask[JXS_NQ]; /* bit r set when (q, r) truncates every band to 15 */
     JxsComp c[JXS_MAXC];
     uint8_t gc[JXS_NGB], gb[JXS_NGB];
     uint32_t pk_bs[JXS_NPK], pk_be[JXS_NPK], pk_line[JXS_NPK];
     uint32_t raw_bytes[2][JXS_NPK];
+    /* packing units of a precinct in codestream order: for every packet its
+     * header, then per sub-packet (significance, bit-plane counts, data) one
+     * unit per band of the packet (u_gi = global band, 0xFF for a header) */
+    uint32_t n_units, pad_u[3];
+    uint8_t u_p[JXS_NU], u_kind[JXS_NU], u_gi[JXS_NU];
     uint64_t n_coeff, n_gcli, n_smax; /* buffer sizes per frame */
     uint64_t n_i32;                   /* int32 transform scratch per frame */
     uint8_t hdr[JXS_HDR_MAX];
 };
+
+/* kernels stage the geometry in LDS with 16-byte copies */
+static_assert(sizeof(JxsGeom) % 16 == 0, "JxsGeom size must be a multiple of 16 bytes");
 
 /* Per-precinct rate-control result. */
 struct JxsPrec {
     uint32_t q, r, total, pad, offset, err;
 };
 
-/* Vertical 5/3 lifting job: columns of a w x h plane -> low rows then high
- * rows (int32, stride w) at dst. */
-struct JxsVertJob {
-    const void* src;
-    int32_t* dst;
-    uint32_t kind, stride, step, w, h;
-    uint32_t shift, mask;
-    int32_t offset;
-    uint32_t tiles_x, tiles, base; /* blocks of this job: [base, base + tiles) */
-    uint32_t pad_;
+/* Packet plan of one precinct (after rate control): byte offset of every
+ * packet in the codestream and the byte sizes of its sub-packets. */
+enum { PKT_ANY = 1u, PKT_RAW = 2u };
+struct JxsPkt {
+    uint32_t off, s, g, d, flags;
+};
+struct JxsPlan {
+    JxsPkt pk[JXS_NPK];
+    uint8_t band[JXS_NGB]; /* per global band: truncation | (significance coding << 4) */
+    uint32_t ubit[JXS_NU]; /* first bit of each packing unit, from the precinct's first byte */
+    uint32_t ulen[JXS_NU]; /* bits of each packing unit */
 };
 
 struct JxsBandOut {
     uint16_t* coeff;
     uint8_t* gcli;
     uint8_t* smax;
-    uint32_t w, gw, sw, pad_;
+    uint16_t* hist; /* frame's histogram records, or null */
+    uint32_t w, gw, sw, cs;
+    uint32_t hl, bl, n_bl, pad_; /* record of band row y: (y / hl) * n_bl + bl + y % hl */
 };
 
-/* Horizontal multi-level 5/3 lifting job: one block per row. */
-struct JxsHorzJob {
+/* One forward-transform step of one component plane (w x h samples at src):
+ * with `vert` set, one vertical 5/3 level followed by horizontal levels of
+ * the low rows (lev_l) and one horizontal level of the high rows; without,
+ * horizontal levels of every row.  One block per output row (pair):
+ * blocks [base, base + rows). */
+struct JxsDwtJob {
     const void* src;
-    uint32_t kind, stride, step, w, rows, levels;
+    uint32_t kind, stride, step, w, h;
     uint32_t shift, mask;
     int32_t offset;
-    uint32_t base; /* first block of this job */
-    uint32_t lf_is_band;
+    uint32_t vert, lev_l;
+    uint32_t base, rows;
+    uint32_t lf_is_band; /* low rows' low-pass output: band, or int32 plane */
     uint32_t lf_stride;
+    uint32_t vec; /* rows readable 4 columns per load (step 1, aligned, width and stride multiples of 4) */
     int32_t* lf_i32;
-    JxsBandOut lf;
-    JxsBandOut hf[5];
+    JxsBandOut lf, hf[5]; /* low rows: final low-pass and high-pass of each level */
+    JxsBandOut lh, hh;    /* high rows: low-pass and high-pass of their level */
 };
 
 /* One frame of a (possibly heterogeneous) launch. */
@@ -116,14 +139,17 @@ struct JxsFrameDesc {
     uint16_t* coeff;
     uint8_t* gcli;
     uint8_t* smax;
-    uint32_t* lut;   /* rows x n_bl x JXS_LUT */
+    uint16_t* hist;  /* rows x n_bl x JXS_HREC */
     uint32_t* table; /* rows x JXS_NQ x JXS_NR */
     JxsPrec* prec;   /* rows */
+    JxsPlan* plan;   /* rows */
     uint8_t* out;    /* codestream */
     uint32_t* status;
     uint32_t rows, npk;
     uint32_t prec_base; /* first precinct of this frame in the launch */
     uint32_t slice_base;
+    uint32_t unit_base; /* first packing unit (rows x n_units numbering) of this frame in the launch */
+    uint32_t n_units;
 };
 
 #endif

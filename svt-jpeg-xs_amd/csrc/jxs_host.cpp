@@ -149,6 +149,25 @@ int build_layout(JxsGeom& g, uint32_t nc, const uint32_t sx[4], const uint32_t s
         n++;
     }
     g.npk = n;
+    /* packing units (one per packet header, one per band of each sub-packet) */
+    g.n_units = 0;
+    for (uint32_t p = 0; p < g.npk; p++) {
+        if (g.n_units + 1 + 3 * (g.pk_be[p] - g.pk_bs[p]) > JXS_NU)
+            return -1;
+        g.u_p[g.n_units] = (uint8_t)p;
+        g.u_kind[g.n_units] = UNIT_HDR;
+        g.u_gi[g.n_units++] = NE;
+        for (uint32_t kind = UNIT_SIG; kind <= UNIT_DATA; kind++)
+            for (uint32_t gi = g.pk_bs[p]; gi < g.pk_be[p]; gi++) {
+                g.u_p[g.n_units] = (uint8_t)p;
+                g.u_kind[g.n_units] = (uint8_t)kind;
+                g.u_gi[g.n_units++] = (uint8_t)gi;
+            }
+    }
+    memset(g.ex_gid, NE, sizeof(g.ex_gid));
+    for (uint32_t gi = 0, e = 0; gi < g.nb_all; gi++)
+        if (g.gb[gi] != NE)
+            g.ex_gid[e++] = (uint8_t)gi;
     g.short_hdr = (W * nc) < 32752;
     g.pkt_hdr_bytes = g.short_hdr ? 5 : 7;
     g.prec_hdr_bytes = div_up(40 + 2 * g.nb_exist, 8);
@@ -386,7 +405,8 @@ void assign_buffers(JxsGeom& g) {
             bd.coeff_off = co;
             bd.gcli_off = go;
             bd.smax_off = so;
-            co += (uint64_t)bd.w * bd.h;
+            bd.cs = 4 * bd.gw;
+            co += (uint64_t)bd.cs * bd.h;
             go += (uint64_t)bd.gw * bd.h;
             so += (uint64_t)bd.sw * bd.h;
         }
@@ -643,126 +663,107 @@ int32_t parse_codestream_header(const uint8_t* cs, size_t len, HostGeom& out, ui
 }
 
 /* ---------------- forward-transform jobs ---------------- */
-static JxsBandOut band_out(const JxsGeom& g, uint32_t c, uint32_t b, uint16_t* coeff, uint8_t* gcli, uint8_t* smax) {
+static JxsBandOut band_out(const JxsGeom& g, uint32_t c, uint32_t b, uint16_t* coeff, uint8_t* gcli, uint8_t* smax,
+                           uint16_t* hist) {
     const JxsBand& bd = g.c[c].b[b];
     JxsBandOut o;
     o.coeff = coeff + bd.coeff_off;
     o.gcli = gcli + bd.gcli_off;
     o.smax = smax + bd.smax_off;
+    o.hist = hist;
     o.w = bd.w;
     o.gw = bd.gw;
     o.sw = bd.sw;
+    o.cs = bd.cs;
+    o.hl = bd.hl;
+    o.bl = bd.bl;
+    o.n_bl = g.n_bl;
     o.pad_ = 0;
     return o;
 }
 
+/* Rows can be read four columns per load when they are contiguous, 16-byte
+ * aligned at the base and every row starts on a multiple of four samples. */
+static JxsDwtJob with_vec(JxsDwtJob j) {
+    j.vec = (j.step == 1 && j.w % 4 == 0 && j.stride % 4 == 0 && ((uintptr_t)j.src & 15) == 0) ? 1u : 0u;
+    return j;
+}
+
 void fwd_jobs(const JxsGeom& g, const void* const planes[4], const uint32_t strides[4], uint16_t* coeff, uint8_t* gcli,
-              uint8_t* smax, int32_t* scratch, FwdJobs& jobs) {
+              uint8_t* smax, uint16_t* hist, int32_t* scratch, FwdJobs& jobs) {
     int32_t* scr = scratch;
     for (uint32_t c = 0; c < g.nc; c++) {
         const JxsComp& cp = g.c[c];
-        const uint32_t kind = g.depth <= 8 ? JXS_SRC_U8 : JXS_SRC_U16;
-        const uint32_t elem = g.depth <= 8 ? 1 : 2;
-        const void* src = g.packed ? (const void*)((const uint8_t*)planes[0] + (size_t)c * elem) : planes[c];
-        const uint32_t stride = g.packed ? strides[0] : strides[c];
-        const uint32_t step = g.packed ? 3 : 1;
-        const uint32_t shift = 20 - g.depth, mask = (1u << g.depth) - 1;
-        const int32_t offset = 1 << 19;
         const uint32_t nb = cp.nb;
-        auto horz = [&](const void* s, uint32_t k, uint32_t st, uint32_t stp, uint32_t w, uint32_t rows) {
-            JxsHorzJob j;
-            memset(&j, 0, sizeof(j));
-            j.src = s;
-            j.kind = k;
-            j.stride = st;
-            j.step = stp;
-            j.w = w;
-            j.rows = rows;
-            j.shift = shift;
-            j.mask = mask;
-            j.offset = offset;
-            return j;
-        };
+        const uint32_t elem = g.depth <= 8 ? 1 : 2;
+        JxsDwtJob j;
+        memset(&j, 0, sizeof(j));
+        j.src = g.packed ? (const void*)((const uint8_t*)planes[0] + (size_t)c * elem) : planes[c];
+        j.kind = g.depth <= 8 ? JXS_SRC_U8 : JXS_SRC_U16;
+        j.stride = g.packed ? strides[0] : strides[c];
+        j.step = g.packed ? 3 : 1;
+        j.w = cp.w;
+        j.h = cp.h;
+        j.shift = 20 - g.depth;
+        j.mask = (1u << g.depth) - 1;
+        j.offset = 1 << 19;
+        auto band = [&](uint32_t b) { return band_out(g, c, b, coeff, gcli, smax, hist); };
         if (cp.dv == 0) {
-            JxsHorzJob j = horz(src, kind, stride, step, cp.w, cp.h);
-            j.levels = cp.dh;
+            /* horizontal levels of every row */
+            j.rows = cp.h;
+            j.lev_l = cp.dh;
             for (uint32_t k = 0; k < cp.dh; k++)
-                j.hf[k] = band_out(g, c, nb - 1 - k, coeff, gcli, smax);
+                j.hf[k] = band(nb - 1 - k);
             j.lf_is_band = 1;
-            j.lf = band_out(g, c, 0, coeff, gcli, smax);
-            jobs.horz[0].push_back(j);
+            j.lf = band(0);
+            jobs.dwt[0].push_back(with_vec(j));
             continue;
         }
-        const uint32_t w = cp.w, h = cp.h, w1 = (w + 1) / 2, h1 = (h + 1) / 2, h2 = h / 2;
-        int32_t* A = scr;
-        int32_t* B = scr + (size_t)w * h;
-        scr += (size_t)w * h + (size_t)w1 * h1;
-        JxsVertJob v;
-        memset(&v, 0, sizeof(v));
-        v.src = src;
-        v.dst = A;
-        v.kind = kind;
-        v.stride = stride;
-        v.step = step;
-        v.w = w;
-        v.h = h;
-        v.shift = shift;
-        v.mask = mask;
-        v.offset = offset;
-        jobs.vert[0].push_back(v);
-        /* low rows of level 1 */
-        JxsHorzJob jl = horz(A, JXS_SRC_I32, w, 1, w, h1);
-        jl.hf[0] = band_out(g, c, nb - 3, coeff, gcli, smax);
+        const uint32_t w1 = (cp.w + 1) / 2, h1 = (cp.h + 1) / 2;
+        /* first vertical level: high rows -> LH1 / HH1, low rows -> HL1 and
+         * (one vertical level) the remaining horizontal levels or (two) the
+         * low-pass plane of the second level */
+        j.vert = 1;
+        j.rows = h1;
+        j.hf[0] = band(nb - 3);
+        j.lh = band(nb - 2);
+        j.hh = band(nb - 1);
         if (cp.dv == 1) {
-            jl.levels = cp.dh;
+            j.lev_l = cp.dh;
             for (uint32_t k = 1; k < cp.dh; k++)
-                jl.hf[k] = band_out(g, c, cp.dh - k, coeff, gcli, smax);
-            jl.lf_is_band = 1;
-            jl.lf = band_out(g, c, 0, coeff, gcli, smax);
-        } else {
-            jl.levels = 1;
-            jl.lf_is_band = 0;
-            jl.lf_i32 = B;
-            jl.lf_stride = w1;
+                j.hf[k] = band(cp.dh - k);
+            j.lf_is_band = 1;
+            j.lf = band(0);
+            jobs.dwt[0].push_back(with_vec(j));
+            continue;
         }
-        jobs.horz[0].push_back(jl);
-        /* high rows of level 1 */
-        if (h2 > 0) {
-            JxsHorzJob jh = horz(A + (size_t)h1 * w, JXS_SRC_I32, w, 1, w, h2);
-            jh.levels = 1;
-            jh.hf[0] = band_out(g, c, nb - 1, coeff, gcli, smax);
-            jh.lf_is_band = 1;
-            jh.lf = band_out(g, c, nb - 2, coeff, gcli, smax);
-            jobs.horz[0].push_back(jh);
-        }
-        if (cp.dv == 2) {
-            const uint32_t g1 = (h1 + 1) / 2, g2 = h1 / 2;
-            JxsVertJob v2 = v;
-            v2.src = B;
-            v2.dst = A;
-            v2.kind = JXS_SRC_I32;
-            v2.stride = w1;
-            v2.step = 1;
-            v2.w = w1;
-            v2.h = h1;
-            jobs.vert[1].push_back(v2);
-            JxsHorzJob kl = horz(A, JXS_SRC_I32, w1, 1, w1, g1);
-            kl.levels = 1 + (cp.dh - 2);
-            kl.hf[0] = band_out(g, c, nb - 6, coeff, gcli, smax);
-            for (uint32_t k = 1; k < kl.levels; k++)
-                kl.hf[k] = band_out(g, c, cp.dh - 2 - (k - 1), coeff, gcli, smax);
-            kl.lf_is_band = 1;
-            kl.lf = band_out(g, c, 0, coeff, gcli, smax);
-            jobs.horz[1].push_back(kl);
-            if (g2 > 0) {
-                JxsHorzJob kh = horz(A + (size_t)g1 * w1, JXS_SRC_I32, w1, 1, w1, g2);
-                kh.levels = 1;
-                kh.hf[0] = band_out(g, c, nb - 4, coeff, gcli, smax);
-                kh.lf_is_band = 1;
-                kh.lf = band_out(g, c, nb - 5, coeff, gcli, smax);
-                jobs.horz[1].push_back(kh);
-            }
-        }
+        int32_t* B = scr;
+        scr += (size_t)w1 * h1;
+        j.lev_l = 1;
+        j.lf_is_band = 0;
+        j.lf_i32 = B;
+        j.lf_stride = w1;
+        jobs.dwt[0].push_back(with_vec(j));
+        /* second vertical level on the w1 x h1 low-pass plane */
+        JxsDwtJob k2;
+        memset(&k2, 0, sizeof(k2));
+        k2.src = B;
+        k2.kind = JXS_SRC_I32;
+        k2.stride = w1;
+        k2.step = 1;
+        k2.w = w1;
+        k2.h = h1;
+        k2.vert = 1;
+        k2.rows = (h1 + 1) / 2;
+        k2.lev_l = 1 + (cp.dh - 2);
+        k2.hf[0] = band(nb - 6);
+        for (uint32_t k = 1; k < k2.lev_l; k++)
+            k2.hf[k] = band(cp.dh - 2 - (k - 1));
+        k2.lf_is_band = 1;
+        k2.lf = band(0);
+        k2.lh = band(nb - 5);
+        k2.hh = band(nb - 4);
+        jobs.dwt[1].push_back(with_vec(k2));
     }
 }
 

@@ -44,15 +44,15 @@ int build_layout(JxsGeom& g, uint32_t nc, const uint32_t sx[4], const uint32_t s
 void assign_buffers(JxsGeom& g);
 
 /* Forward-transform job lists for one frame.  The base pointers are device
- * addresses of this frame's buffers; the input planes are device addresses.
- * stage 0: first vertical level, 1: its rows (or all rows for V0
- * components), 2: second vertical level, 3: its rows. */
+ * addresses of this frame's buffers (hist: band-line histogram records, may
+ * be null; scratch: the int32 low-pass planes between the two vertical
+ * levels); the input planes are device addresses.  dwt[0]: first vertical
+ * level (or horizontal-only components), dwt[1]: second vertical level. */
 struct FwdJobs {
-    std::vector<JxsVertJob> vert[2];
-    std::vector<JxsHorzJob> horz[2];
+    std::vector<JxsDwtJob> dwt[2];
 };
 void fwd_jobs(const JxsGeom& g, const void* const planes[4], const uint32_t strides[4], uint16_t* coeff, uint8_t* gcli,
-              uint8_t* smax, int32_t* scratch, FwdJobs& jobs);
+              uint8_t* smax, uint16_t* hist, int32_t* scratch, FwdJobs& jobs);
 
 /* Inverse-transform job lists (decoder). */
 struct InvVertJob;

@@ -1,28 +1,33 @@
 /*
  * jxs_kernels.hip -- gfx950 kernels of the JPEG XS encode path.
  *
- *   k_vert     vertical 5/3 lifting of one decomposition level; the first
- *              level reads the input planes and applies the input scaling
- *              (NltEnc.c:33-71) on the fly.  One lane per column, a segment of
- *              rows per block, coalesced row loads.
- *   k_horz     one block per row: the row is staged in LDS and all horizontal
- *              levels of that row are lifted in place; every band produced is
+ *   k_dwt      one block per output row pair of one vertical 5/3 level: the
+ *              low and the high row are lifted vertically from the input
+ *              (with the input scaling of NltEnc.c:33-71 on the fly for the
+ *              first level) straight into LDS, then through their horizontal
+ *              levels in place; every band produced is
  *              written as 16-bit sign-magnitude coefficients (image_shift,
  *              NltEnc.c:12-31) together with its GCLI per 4 coefficients and
  *              the maximum per 8 GCLIs (GcStageProcess.c:22-97), using wave
- *              shuffles for the maxima.
- *   k_rc_table one block per precinct: GCLI / significance histograms per band
- *              line (RateControl.c:18-154), then the packet-level byte count
- *              of the precinct for every (Q, R) (RateControl.c:457-781).
- *   k_select   one lane per slice: precinct budgets with the +10 % first
+ *              shuffles for the maxima, and the histogram record of the band
+ *              line (GCLI and significance-maximum counts, from wave ballots)
+ *              the rate control needs (RateControl.c:18-154).
+ *   k_rc_table one block per precinct: from the histogram records, the
+ *              packet-level byte count of the precinct for every (Q, R)
+ *              (RateControl.c:457-781).
+ *   k_select   one wave per slice: precinct budgets with the +10 % first
  *              precinct and the padding carry (PackStageProcess.c:600-684),
- *              the reference's binary searches replayed on the table
- *              (RateControl.c:889-1083, BinarySearch.c), precinct offsets.
- *   k_pack     one block per (packet, precinct): precinct / packet headers,
- *              significance, bit-plane-count and data sub-packets
- *              (PackPrecinct.c:235-572) with quantisation (Quant.c:9-45)
- *              fused; bit offsets from block prefix sums, bits assembled in an
- *              LDS ring with atomicOr and flushed as bytes.
+ *              the reference's binary searches replayed on the LDS-staged
+ *              tables (RateControl.c:889-1083, BinarySearch.c), precinct
+ *              offsets.
+ *   k_plan     one wave per precinct: per-band truncation and coding method,
+ *              packet sizes and offsets, slice / precinct headers, padding,
+ *              end marker and frame header.
+ *   k_pack     one wave per (precinct, packet): packet header, significance,
+ *              bit-plane-count and data sub-packets (PackPrecinct.c:235-572)
+ *              with quantisation (Quant.c:9-45) fused; bit offsets from wave
+ *              prefix sums, bits assembled in a per-wave LDS ring and written
+ *              out as dwords.
  *
  * All arithmetic is integer.  Every kernel takes a frame-descriptor array so
  * one launch covers a batch of heterogeneous frames.
@@ -37,11 +42,35 @@
 /* ------------------------------------------------------------------ */
 /* helpers                                                             */
 /* ------------------------------------------------------------------ */
+/* Global-memory access through address-space-1 pointers: the buffers are
+ * reached through pointers held in descriptors, which the compiler would
+ * otherwise address with flat instructions (slower, and counted against the
+ * LDS wait counter). */
+#define GAS __attribute__((address_space(1)))
+/* Read-only descriptors (geometry, plans) written before the kernel starts:
+ * constant address space, so the compiler keeps them in scalar loads even
+ * after the kernel's own global stores. */
+#define CAS __attribute__((address_space(4)))
+template <typename T>
+__device__ __forceinline__ T gld(const T* p) {
+    return *(const GAS T*)p;
+}
+template <typename T>
+__device__ __forceinline__ void gst(T* p, T v) {
+    *(GAS T*)p = v;
+}
+/* 8-byte aligned pairs of words */
+__device__ __forceinline__ uint2 gld2(const void* p) {
+    const uint64_t v = *(const GAS uint64_t*)p;
+    return make_uint2((uint32_t)v, (uint32_t)(v >> 32));
+}
+__device__ __forceinline__ void gst2(void* p, uint2 v) { *(GAS uint64_t*)p = ((uint64_t)v.y << 32) | v.x; }
+
 __device__ __forceinline__ int32_t load_src(const void* src, uint32_t kind, size_t idx, uint32_t shift, uint32_t mask,
                                             int32_t offset) {
     if (kind == JXS_SRC_I32)
-        return ((const int32_t*)src)[idx];
-    uint32_t v = kind == JXS_SRC_U8 ? (uint32_t)((const uint8_t*)src)[idx] : (uint32_t)((const uint16_t*)src)[idx];
+        return gld((const int32_t*)src + idx);
+    uint32_t v = kind == JXS_SRC_U8 ? (uint32_t)gld((const uint8_t*)src + idx) : (uint32_t)gld((const uint16_t*)src + idx);
     return (int32_t)((v & mask) << shift) - offset;
 }
 
@@ -89,8 +118,6 @@ __device__ __forceinline__ uint32_t block_scan(uint32_t v, uint32_t* tmp, uint32
 /* ------------------------------------------------------------------ */
 /* forward transform                                                   */
 /* ------------------------------------------------------------------ */
-#define VSEG 16 /* output row pairs per block row */
-
 __device__ __forceinline__ int find_job_u32(const uint32_t* bases, int n, uint32_t blk) {
     int lo = 0, hi = n - 1;
     while (lo < hi) {
@@ -103,100 +130,106 @@ __device__ __forceinline__ int find_job_u32(const uint32_t* bases, int n, uint32
     return lo;
 }
 
-__global__ void __launch_bounds__(256) k_vert(const JxsVertJob* __restrict__ jobs, const uint32_t* __restrict__ bases,
-                                              int njobs) {
-    const int j = find_job_u32(bases, njobs, blockIdx.x);
-    const JxsVertJob J = jobs[j];
-    const uint32_t t = blockIdx.x - J.base;
-    const uint32_t tx = t % J.tiles_x, ty = t / J.tiles_x;
-    const uint32_t x = tx * blockDim.x + threadIdx.x;
-    if (x >= J.w)
-        return;
-    const uint32_t n = J.h, nh = n / 2, nl = n - nh;
-    const uint32_t i0 = ty * VSEG, i1 = min(i0 + VSEG, nl);
-    const size_t col = (size_t)x * J.step;
-    auto X = [&](uint32_t row) { return load_src(J.src, J.kind, (size_t)row * J.stride + col, J.shift, J.mask, J.offset); };
-    int32_t* L = J.dst;
-    int32_t* H = J.dst + (size_t)nl * J.w;
-    /* hf(k) = x(2k+1) - ((x(2k) + x(2k+2)) >> 1), or x(2k+1) - x(2k) at an even end */
-    int32_t hprev = 0;
-    int32_t xe = X(2 * i0); /* x(2i) */
-    if (i0 > 0) {
-        const int32_t xp = X(2 * i0 - 2), xo = X(2 * i0 - 1);
-        hprev = xo - ((xp + xe) >> 1);
-    }
-    for (uint32_t i = i0; i < i1; i++) {
-        int32_t hcur = 0, xn = 0;
-        if (i < nh) {
-            const int32_t xo = X(2 * i + 1);
-            if (2 * i + 2 < n) {
-                xn = X(2 * i + 2);
-                hcur = xo - ((xe + xn) >> 1);
-            } else {
-                hcur = xo - xe;
-            }
-        }
-        int32_t lf;
-        if (i == 0)
-            lf = xe + ((hcur + 1) >> 1);
-        else if (i < nh)
-            lf = xe + ((hprev + hcur + 2) >> 2);
-        else
-            lf = xe + ((hprev + 1) >> 1);
-        L[(size_t)i * J.w + x] = lf;
-        if (i < nh)
-            H[(size_t)i * J.w + x] = hcur;
-        hprev = hcur;
-        xe = xn;
-    }
+/* Workgroups are dealt to the 8 XCDs round robin.  This maps the dispatch
+ * index to a logical block so that each XCD runs one contiguous range of
+ * rows: neighbouring row pairs, which read overlapping input rows, then
+ * share that XCD's L2. */
+#define NXCD 8
+__device__ __forceinline__ uint32_t xcd_remap(uint32_t b, uint32_t n) {
+    const uint32_t q = n / NXCD, r = n % NXCD, x = b % NXCD;
+    return x * q + min(x, r) + b / NXCD;
 }
 
 #define HMAX JXS_MAX_ROW
 
-/* Writes `count` values buf[start + k * step] as one band row with GCLI and
- * significance maxima.  Called by all threads of the block. */
-__device__ void emit_band_row(const JxsBandOut& o, uint32_t y, const int32_t* buf, uint32_t start, uint32_t step,
-                              uint32_t count) {
-    uint16_t* crow = o.coeff + (size_t)y * o.w;
+/* Count of the lanes whose 4-bit value equals this lane's `want`, among the
+ * lanes of `valid`; b0..b3 are the ballots of the value's bits. */
+__device__ __forceinline__ uint32_t ballot_count(uint64_t valid, uint64_t b0, uint64_t b1, uint64_t b2, uint64_t b3,
+                                                 uint32_t want) {
+    uint64_t m = valid;
+    m &= (want & 1u) ? b0 : ~b0;
+    m &= (want & 2u) ? b1 : ~b1;
+    m &= (want & 4u) ? b2 : ~b2;
+    m &= (want & 8u) ? b3 : ~b3;
+    return (uint32_t)__popcll(m);
+}
+
+/* Writes `count` values buf[start + k * step] as one band row (row stride
+ * o.cs, the padding of the last group zero) with its GCLIs and significance
+ * maxima, and the row's histogram record when o.hist is set.  hcnt is a
+ * zeroed LDS array of 32 words; it is zero again on return.  Called by all
+ * threads of the block. */
+__device__ __forceinline__ void emit_band_row(const JxsBandOut o, uint32_t y, const int32_t* buf, uint32_t start,
+                                              uint32_t step,
+                              uint32_t count, uint32_t* hcnt) {
+    uint16_t* crow = o.coeff + (size_t)y * o.cs;
     uint8_t* grow = o.gcli + (size_t)y * o.gw;
     uint8_t* srow = o.smax + (size_t)y * o.sw;
+    const uint32_t lane = threadIdx.x & (WAVE - 1);
+    const uint32_t full = o.gw / 8;
+    /* lanes 0..15 count GCLI value `lane`, lanes 16..30 full significance
+     * groups of maximum `lane - 16` */
+    const bool count_gcli = lane < 16;
+    const uint32_t want = lane & 15u;
+    uint32_t mycnt = 0;
     for (uint32_t base = 0; base < o.gw; base += blockDim.x) {
         const uint32_t gidx = base + threadIdx.x;
         uint32_t orv = 0;
-        if (gidx < o.gw) {
+        const bool valid = gidx < o.gw;
+        if (valid) {
+            uint32_t v[4];
 #pragma unroll
             for (uint32_t k = 0; k < 4; k++) {
                 const uint32_t i = 4 * gidx + k;
-                if (i < count) {
-                    const uint16_t v = to_sign_mag(buf[start + i * step]);
-                    crow[i] = v;
-                    orv |= v & 0x7FFFu;
-                }
+                v[k] = i < count ? to_sign_mag(buf[start + i * step]) : 0u;
+                orv |= v[k] & 0x7FFFu;
             }
-            grow[gidx] = (uint8_t)bitlen(orv);
+            gst2(crow + 4 * gidx, make_uint2(v[0] | (v[1] << 16), v[2] | (v[3] << 16)));
+            gst(grow + gidx, (uint8_t)bitlen(orv));
         }
-        uint32_t m = bitlen(orv);
+        const uint32_t gc = bitlen(orv);
+        uint32_t m = gc;
         m = max(m, (uint32_t)__shfl_xor((int)m, 1, WAVE));
         m = max(m, (uint32_t)__shfl_xor((int)m, 2, WAVE));
         m = max(m, (uint32_t)__shfl_xor((int)m, 4, WAVE));
-        if ((gidx & 7) == 0 && gidx / 8 < o.sw)
-            srow[gidx / 8] = (uint8_t)m;
+        const bool head = (gidx & 7) == 0 && gidx / 8 < o.sw;
+        if (head)
+            gst(srow + gidx / 8, (uint8_t)m);
+        if (o.hist) {
+            const bool sfull = head && gidx / 8 < full;
+            const uint64_t vg = __ballot(valid), vs = __ballot(sfull);
+            const uint64_t g0 = __ballot(valid && (gc & 1u)), g1 = __ballot(valid && (gc & 2u)),
+                           g2 = __ballot(valid && (gc & 4u)), g3 = __ballot(valid && (gc & 8u));
+            const uint64_t s0 = __ballot(sfull && (m & 1u)), s1 = __ballot(sfull && (m & 2u)),
+                           s2 = __ballot(sfull && (m & 4u)), s3 = __ballot(sfull && (m & 8u));
+            mycnt += count_gcli ? ballot_count(vg, g0, g1, g2, g3, want) : ballot_count(vs, s0, s1, s2, s3, want);
+            if (head && gidx / 8 == full)
+                hcnt[HREC_TAIL] = m + 1; /* the partial significance group */
+        }
     }
+    if (!o.hist)
+        return;
+    if (lane < HREC_TAIL && mycnt)
+        atomicAdd(&hcnt[lane], mycnt);
+    __syncthreads();
+    if (threadIdx.x < JXS_HREC) {
+        const uint32_t t = threadIdx.x, c = hcnt[t];
+        uint16_t* rec = o.hist + ((size_t)(y / o.hl) * o.n_bl + o.bl + y % o.hl) * JXS_HREC;
+        gst(rec + t, (uint16_t)(t == HREC_TAIL ? (c ? c - 1 : 0xFFFFu) : c));
+        hcnt[t] = 0;
+    }
+    __syncthreads();
 }
 
-__global__ void __launch_bounds__(256) k_horz(const JxsHorzJob* __restrict__ jobs, const uint32_t* __restrict__ bases,
-                                              int njobs) {
-    __shared__ int32_t buf[HMAX];
-    const int j = find_job_u32(bases, njobs, blockIdx.x);
-    const JxsHorzJob& J = jobs[j];
-    const uint32_t y = blockIdx.x - J.base;
-    const uint32_t w = J.w;
-    const size_t rbase = (size_t)y * J.stride;
-    for (uint32_t i = threadIdx.x; i < w; i += blockDim.x)
-        buf[i] = load_src(J.src, J.kind, (rbase + i) * (size_t)1 + (size_t)i * (J.step - 1), J.shift, J.mask, J.offset);
-    __syncthreads();
+/* Lifts the row of w samples in buf through `levels` horizontal 5/3 levels
+ * in place (the low-pass samples of level k at stride 2^k) and writes the
+ * high band of every level and the final low-pass output (a band, or int32
+ * samples at lf_i32).  buf is loaded and synchronised by the caller. */
+__device__ __forceinline__ void horz_emit(int32_t* buf, uint32_t w, uint32_t levels, const JxsBandOut* hf,
+                                          uint32_t lf_is_band,
+                          const JxsBandOut& lf, int32_t* lf_i32, uint32_t y, uint32_t* hcnt) {
     uint32_t n = w, s = 1;
-    for (uint32_t lvl = 0; lvl < J.levels; lvl++) {
+    for (uint32_t lvl = 0; lvl < levels; lvl++) {
         const uint32_t nh = n / 2, nl = n - nh;
         for (uint32_t i = threadIdx.x; i < nh; i += blockDim.x) {
             const int32_t a = buf[2 * i * s];
@@ -215,17 +248,222 @@ __global__ void __launch_bounds__(256) k_horz(const JxsHorzJob* __restrict__ job
                 buf[2 * i * s] += (buf[(2 * i - 1) * s] + 1) >> 1;
         }
         __syncthreads();
-        emit_band_row(J.hf[lvl], y, buf, s, 2 * s, nh);
+        emit_band_row(hf[lvl], y, buf, s, 2 * s, nh, hcnt);
         s *= 2;
         n = nl;
     }
-    if (J.lf_is_band) {
-        emit_band_row(J.lf, y, buf, 0, s, n);
+    if (lf_is_band) {
+        emit_band_row(lf, y, buf, 0, s, n, hcnt);
     } else {
-        int32_t* dst = J.lf_i32 + (size_t)y * J.lf_stride;
         for (uint32_t i = threadIdx.x; i < n; i += blockDim.x)
-            dst[i] = buf[i * s];
+            gst(lf_i32 + i, buf[i * s]);
     }
+}
+
+/* A component plane read through load_src (input scaling applied). */
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+struct PlaneSrc {
+    const void* src;
+    uint32_t kind, stride, step, shift, mask;
+    int32_t offset;
+    __device__ __forceinline__ int32_t at(uint32_t row, uint32_t x) const {
+        return load_src(src, kind, (size_t)row * stride + (size_t)x * step, shift, mask, offset);
+    }
+    __device__ __forceinline__ int32_t scale(uint32_t v) const { return (int32_t)((v & mask) << shift) - offset; }
+    /* columns x .. x+3 of a row with one load; needs step 1, x and the
+     * stride multiples of 4 and an aligned base (checked by the host). */
+    __device__ __forceinline__ int4 at4(uint32_t row, uint32_t x) const {
+        const size_t idx = (size_t)row * stride + x;
+        if (kind == JXS_SRC_U16) {
+            const u32x2 v = *(const GAS u32x2*)((const uint16_t*)src + idx);
+            return make_int4(scale(v.x & 0xFFFFu), scale(v.x >> 16), scale(v.y & 0xFFFFu), scale(v.y >> 16));
+        }
+        if (kind == JXS_SRC_U8) {
+            const uint32_t v = *(const GAS uint32_t*)((const uint8_t*)src + idx);
+            return make_int4(scale(v & 0xFFu), scale((v >> 8) & 0xFFu), scale((v >> 16) & 0xFFu), scale(v >> 24));
+        }
+        const u32x4 v = *(const GAS u32x4*)((const int32_t*)src + idx);
+        return make_int4((int32_t)v.x, (int32_t)v.y, (int32_t)v.z, (int32_t)v.w);
+    }
+};
+
+/* The vertical 5/3 step of one column from its input samples x(2i-2) ..
+ * x(2i+2): low sample and high sample of row i. */
+struct VertFlags {
+    bool prev, h, next; /* i > 0, i < n/2, 2i+2 < n */
+};
+__device__ __forceinline__ void vert53(const VertFlags& f, int32_t xm2, int32_t xm1, int32_t xc, int32_t xp1, int32_t xp2,
+                                       int32_t& lf, int32_t& hf) {
+    const int32_t hcur = f.h ? (f.next ? xp1 - ((xc + xp2) >> 1) : xp1 - xc) : 0;
+    const int32_t hprev = f.prev ? xm1 - ((xm2 + xc) >> 1) : 0;
+    if (!f.prev)
+        lf = xc + ((hcur + 1) >> 1);
+    else if (f.h)
+        lf = xc + ((hprev + hcur + 2) >> 2);
+    else
+        lf = xc + ((hprev + 1) >> 1);
+    hf = hcur;
+}
+
+#define QUAD_BATCH 4 /* column quads per thread whose loads are issued together */
+
+/* vert_fill for planes whose rows allow 4-column loads: each thread takes
+ * QUAD_BATCH quads of columns, the five input rows of all of them loaded
+ * before any is computed. */
+__device__ __forceinline__ void vert_fill4(const PlaneSrc& S, uint32_t i, uint32_t n, uint32_t w, int32_t* lo,
+                                           int32_t* hi) {
+    const VertFlags f = {i > 0, i < n / 2, 2 * i + 2 < n};
+    const uint32_t nq = w / 4;
+    const int4 z = make_int4(0, 0, 0, 0);
+    for (uint32_t q0 = 0; q0 < nq; q0 += QUAD_BATCH * blockDim.x) {
+        int4 xm2[QUAD_BATCH], xm1[QUAD_BATCH], xc[QUAD_BATCH], xp1[QUAD_BATCH], xp2[QUAD_BATCH];
+#pragma unroll
+        for (uint32_t k = 0; k < QUAD_BATCH; k++) {
+            const uint32_t x = 4 * min(q0 + k * blockDim.x + threadIdx.x, nq - 1);
+            xc[k] = S.at4(2 * i, x);
+            xm2[k] = f.prev ? S.at4(2 * i - 2, x) : z;
+            xm1[k] = f.prev ? S.at4(2 * i - 1, x) : z;
+            xp1[k] = f.h ? S.at4(2 * i + 1, x) : z;
+            xp2[k] = f.next ? S.at4(2 * i + 2, x) : z;
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < QUAD_BATCH; k++) {
+            const uint32_t q = q0 + k * blockDim.x + threadIdx.x;
+            if (q >= nq)
+                break;
+            int4 l, h;
+            vert53(f, xm2[k].x, xm1[k].x, xc[k].x, xp1[k].x, xp2[k].x, l.x, h.x);
+            vert53(f, xm2[k].y, xm1[k].y, xc[k].y, xp1[k].y, xp2[k].y, l.y, h.y);
+            vert53(f, xm2[k].z, xm1[k].z, xc[k].z, xp1[k].z, xp2[k].z, l.z, h.z);
+            vert53(f, xm2[k].w, xm1[k].w, xc[k].w, xp1[k].w, xp2[k].w, l.w, h.w);
+            *(int4*)(lo + 4 * q) = l;
+            if (f.h)
+                *(int4*)(hi + 4 * q) = h;
+        }
+    }
+}
+
+#define FILL_BATCH 4 /* column batches of blockDim.x whose loads are issued together */
+
+/* Low row i into lo[] and, when it exists, high row i into hi[] for the w
+ * columns, from input rows 2i-2 .. 2i+2.  Loads use clamped column indexes
+ * (always in bounds); FILL_BATCH coalesced column batches are loaded before
+ * any is computed. */
+__device__ __forceinline__ void vert_fill(const PlaneSrc& S, uint32_t i, uint32_t n, uint32_t w, int32_t* lo,
+                                          int32_t* hi) {
+    const bool has_prev = i > 0, has_h = i < n / 2, has_next = 2 * i + 2 < n;
+    for (uint32_t x0 = 0; x0 < w; x0 += FILL_BATCH * blockDim.x) {
+        int32_t xm2[FILL_BATCH], xm1[FILL_BATCH], xc[FILL_BATCH], xp1[FILL_BATCH], xp2[FILL_BATCH];
+#pragma unroll
+        for (uint32_t k = 0; k < FILL_BATCH; k++) {
+            const uint32_t x = min(x0 + k * blockDim.x + threadIdx.x, w - 1);
+            xc[k] = S.at(2 * i, x);
+            xm2[k] = has_prev ? S.at(2 * i - 2, x) : 0;
+            xm1[k] = has_prev ? S.at(2 * i - 1, x) : 0;
+            xp1[k] = has_h ? S.at(2 * i + 1, x) : 0;
+            xp2[k] = has_next ? S.at(2 * i + 2, x) : 0;
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < FILL_BATCH; k++) {
+            const uint32_t x = x0 + k * blockDim.x + threadIdx.x;
+            if (x >= w)
+                break;
+            const int32_t hcur = has_h ? (has_next ? xp1[k] - ((xc[k] + xp2[k]) >> 1) : xp1[k] - xc[k]) : 0;
+            const int32_t hprev = has_prev ? xm1[k] - ((xm2[k] + xc[k]) >> 1) : 0;
+            int32_t lf;
+            if (!has_prev)
+                lf = xc[k] + ((hcur + 1) >> 1);
+            else if (has_h)
+                lf = xc[k] + ((hprev + hcur + 2) >> 2);
+            else
+                lf = xc[k] + ((hprev + 1) >> 1);
+            lo[x] = lf;
+            if (has_h)
+                hi[x] = hcur;
+        }
+    }
+}
+
+/* One block per output row (pair).  With a vertical level, block i computes
+ * low row i and high row i from input rows 2i-2 .. 2i+2 straight into LDS
+ * (vertical 5/3: hf(k) = x(2k+1) - ((x(2k) + x(2k+2)) >> 1), lf(k) = x(2k) +
+ * ((hf(k-1) + hf(k) + 2) >> 2), one-sided at the ends), lifts the low row
+ * through the horizontal levels of the low rows and writes its bands (or the
+ * int32 low-pass plane of the next vertical level), then the high row
+ * through one horizontal level into the LH and HH bands.  The intermediate
+ * rows never leave LDS.  Without a vertical level, block i lifts input row
+ * i.  Dynamic LDS: two rows of the widest job. */
+__global__ void __launch_bounds__(256) k_dwt(const JxsDwtJob* __restrict__ jobs, const uint32_t* __restrict__ bases,
+                                             int njobs, uint32_t nblocks, uint32_t wmax) {
+    extern __shared__ __align__(16) int32_t rows_lds[];
+    __shared__ uint32_t hcnt[JXS_HREC];
+    int32_t* lo = rows_lds;
+    int32_t* hi = rows_lds + ((wmax + 3) & ~3u); /* 16-byte aligned */
+    const uint32_t blk = xcd_remap(blockIdx.x, nblocks);
+    const int j = find_job_u32(bases, njobs, blk);
+    const JxsDwtJob& J = jobs[j];
+    const uint32_t i = blk - J.base;
+    const PlaneSrc S = {J.src, J.kind, J.stride, J.step, J.shift, J.mask, J.offset};
+    const uint32_t w = J.w, n = J.h;
+    if (threadIdx.x < JXS_HREC)
+        hcnt[threadIdx.x] = 0;
+    if (J.vert) {
+        if (J.vec)
+            vert_fill4(S, i, n, w, lo, hi);
+        else
+            vert_fill(S, i, n, w, lo, hi);
+    } else {
+        for (uint32_t x0 = 0; x0 < w; x0 += FILL_BATCH * blockDim.x) {
+            int32_t v[FILL_BATCH];
+#pragma unroll
+            for (uint32_t k = 0; k < FILL_BATCH; k++)
+                v[k] = S.at(i, min(x0 + k * blockDim.x + threadIdx.x, w - 1));
+#pragma unroll
+            for (uint32_t k = 0; k < FILL_BATCH; k++) {
+                const uint32_t x = x0 + k * blockDim.x + threadIdx.x;
+                if (x < w)
+                    lo[x] = v[k];
+            }
+        }
+    }
+    __syncthreads();
+    horz_emit(lo, w, J.lev_l, J.hf, J.lf_is_band, J.lf, J.lf_is_band ? nullptr : J.lf_i32 + (size_t)i * J.lf_stride,
+              i, hcnt);
+    if (J.vert && i < n / 2) {
+        __syncthreads();
+        horz_emit(hi, w, 1, &J.hh, 1, J.lh, nullptr, i, hcnt);
+    }
+}
+
+/* Copies a frame's geometry into LDS (all threads of the block take part;
+ * the caller synchronises).  The rate-control kernels index it with
+ * data-dependent band and packet numbers. */
+__device__ __forceinline__ void stage_geom(JxsGeom* dst, const JxsGeom* src) {
+    const uint4* s = (const uint4*)src;
+    uint4* d = (uint4*)dst;
+    constexpr uint32_t n = sizeof(JxsGeom) / 16;
+    /* blocks of 64 threads or more: at most 5 copies per thread, loaded
+     * together */
+    const uint32_t t = threadIdx.x, bd = blockDim.x;
+    /* indexes clamped into the struct: every load is in bounds, only the
+     * stores are predicated */
+    const uint4 v0 = s[min(t, n - 1)];
+    const uint4 v1 = s[min(t + bd, n - 1)];
+    const uint4 v2 = s[min(t + 2 * bd, n - 1)];
+    const uint4 v3 = s[min(t + 3 * bd, n - 1)];
+    const uint4 v4 = s[min(t + 4 * bd, n - 1)];
+    if (t < n)
+        d[t] = v0;
+    if (t + bd < n)
+        d[t + bd] = v1;
+    if (t + 2 * bd < n)
+        d[t + 2 * bd] = v2;
+    if (t + 3 * bd < n)
+        d[t + 3 * bd] = v3;
+    if (t + 4 * bd < n)
+        d[t + 4 * bd] = v4;
 }
 
 /* ------------------------------------------------------------------ */
@@ -243,113 +481,137 @@ __device__ __forceinline__ int find_frame(const JxsFrameDesc* fr, int nfr, uint3
     return lo;
 }
 
-/* Packet-level bytes of one precinct for (q, r), reading band-line tables
- * from lut (n_bl x JXS_LUT).  Mirrors precinct_get_budget_bytes with the
- * per-band method choice of rate_control_calculate_band_best_method. */
-__device__ uint32_t precinct_bytes(const JxsGeom& g, uint32_t type, const uint32_t* lut, uint32_t q, uint32_t r) {
-    uint32_t total = 0;
-    for (uint32_t p = 0; p < g.npk; p++) {
-        uint32_t d = 0, gb = 0, sb = 0;
-        const uint32_t l = g.pk_line[p];
-        for (uint32_t gi = g.pk_bs[p]; gi < g.pk_be[p]; gi++) {
-            const JxsBand& bd = g.c[g.gc[gi]].b[g.gb[gi]];
-            const uint32_t nl = bd.lines[type];
-            if (l >= nl)
-                continue;
-            const uint32_t gt = truncation(bd.gain, bd.prio, q, r);
-            uint32_t nosig_sum = 0, sig_sum = bd.sw * nl, mine_nosig = 0, mine_sig = 0;
-            for (uint32_t k = 0; k < nl; k++) {
-                const uint32_t* L = lut + (size_t)(bd.bl + k) * JXS_LUT;
-                const uint32_t ns = L[LUT_DATA + gt] + L[LUT_CNT + gt];
-                uint32_t zeroed = L[LUT_SIG + gt] * 8;
-                const int32_t tail = (int32_t)L[LUT_TAIL];
-                if (tail >= 0 && (uint32_t)tail <= gt)
-                    zeroed += bd.gw - (bd.gw / 8) * 8;
-                nosig_sum += ns;
-                sig_sum += ns - zeroed;
-                if (k == l) {
-                    mine_nosig = ns;
-                    mine_sig = ns - zeroed;
-                    d += 4 * L[LUT_DATA + gt];
-                }
-            }
-            if (g.sig && nosig_sum > sig_sum) {
-                gb += mine_sig;
-                sb += bd.sw;
-            } else {
-                gb += mine_nosig;
-            }
-        }
-        const uint32_t db = (d + 7) >> 3, gbb = (gb + 7) >> 3, sbb = (sb + 7) >> 3;
-        total += db;
-        total += (sbb + gbb > g.raw_bytes[type][p]) ? g.raw_bytes[type][p] : sbb + gbb;
+/* Bit counts of one band line at truncation t from its histogram record:
+ * the GCLI sub-packet without significance (one unary code per group,
+ * RateControl.c:18-63), the groups a significance flag removes
+ * (RateControl.c:65-154) and the data bit-planes with signs
+ * (RateControl.c:457-520). */
+struct LineBits {
+    uint32_t ns, zeroed, data;
+};
+__device__ __forceinline__ LineBits line_bits(const uint16_t* rec, uint32_t t, uint32_t gw) {
+    uint32_t cnt = 0, data = 0, sig = 0;
+#pragma unroll
+    for (uint32_t v = 0; v < 16; v++) {
+        const uint32_t h = rec[v];
+        cnt += v <= t ? h : 0u;
+        data += v > t ? (v - t + 1) * h : 0u;
     }
-    return total;
+    const uint32_t full = gw / 8;
+    if (t >= 15) {
+        sig = full;
+    } else {
+#pragma unroll
+        for (uint32_t v = 0; v < 15; v++)
+            sig += v <= t ? (uint32_t)rec[16 + v] : 0u;
+    }
+    const uint32_t tail = rec[HREC_TAIL];
+    LineBits b;
+    b.ns = data + cnt;
+    b.zeroed = sig * 8 + ((tail != 0xFFFFu && tail <= t) ? gw - full * 8 : 0u);
+    b.data = 4 * data;
+    return b;
 }
 
+/* One block per precinct: the packet-level byte count of the precinct for
+ * every (Q, R) (precinct_get_budget_bytes with the per-band method choice of
+ * rate_control_calculate_band_best_method, RateControl.c:457-781), from the
+ * band-line histograms the transform wrote.  Per band line and truncation the
+ * bit counts are tabulated once in LDS; each (Q, R) entry then sums the
+ * packets' bands. */
 __global__ void __launch_bounds__(256) k_rc_table(const JxsFrameDesc* __restrict__ frames, int nfr) {
-    __shared__ uint32_t hist[JXS_NBL][16];
-    __shared__ uint32_t shist[JXS_NBL][16];
-    __shared__ int32_t tailv[JXS_NBL];
-    __shared__ uint32_t lut[JXS_NBL * JXS_LUT];
+    __shared__ uint16_t rec[JXS_NBL][JXS_HREC];
+    __shared__ uint32_t t_ns[JXS_NBL][16], t_zr[JXS_NBL][16], t_d[JXS_NBL][16], t_g[JXS_NBL][16], t_s[JXS_NBL][16];
+    __shared__ uint8_t bl_band[JXS_NBL];
+    __shared__ uint32_t b_gain[JXS_NGB], b_prio[JXS_NGB], b_bl[JXS_NGB], b_nl[JXS_NGB], b_gw[JXS_NGB], b_sw[JXS_NGB];
     const int f = find_frame(frames, nfr, blockIdx.x);
     const JxsFrameDesc& F = frames[f];
-    const JxsGeom& g = *F.g;
+    __shared__ __align__(16) JxsGeom sg;
+    stage_geom(&sg, F.g);
+    __syncthreads();
+    const JxsGeom& g = sg;
     const uint32_t row = blockIdx.x - F.prec_base;
     const uint32_t type = row + 1 >= g.rows ? 1 : 0;
-    for (uint32_t i = threadIdx.x; i < JXS_NBL * 16; i += blockDim.x) {
-        (&hist[0][0])[i] = 0;
-        (&shist[0][0])[i] = 0;
+    const uint32_t nbl = g.n_bl;
+    for (uint32_t gi = threadIdx.x; gi < g.nb_all; gi += blockDim.x) {
+        uint32_t nl = 0;
+        if (g.gb[gi] != 0xFF) {
+            const JxsBand& bd = g.c[g.gc[gi]].b[g.gb[gi]];
+            nl = bd.lines[type];
+            b_gain[gi] = bd.gain;
+            b_prio[gi] = bd.prio;
+            b_bl[gi] = bd.bl;
+            b_gw[gi] = bd.gw;
+            b_sw[gi] = bd.sw;
+            for (uint32_t k = 0; k < bd.hl; k++)
+                bl_band[bd.bl + k] = (uint8_t)gi;
+        }
+        b_nl[gi] = nl;
     }
-    for (uint32_t i = threadIdx.x; i < JXS_NBL; i += blockDim.x)
-        tailv[i] = -1;
+    const uint32_t* src = (const uint32_t*)(F.hist + (size_t)row * nbl * JXS_HREC);
+    {
+        /* nbl * 16 <= 1024 words: four loads per thread, issued together */
+        uint32_t v[4];
+        const uint32_t words = nbl * JXS_HREC / 2;
+#pragma unroll
+        for (uint32_t j = 0; j < 4; j++) {
+            const uint32_t i = j * blockDim.x + threadIdx.x;
+            v[j] = gld(src + min(i, words - 1)); /* clamped: every load in bounds */
+        }
+#pragma unroll
+        for (uint32_t j = 0; j < 4; j++) {
+            const uint32_t i = j * blockDim.x + threadIdx.x;
+            if (i < words)
+                ((uint32_t*)&rec[0][0])[i] = v[j];
+        }
+    }
     __syncthreads();
-    for (uint32_t c = 0; c < g.nc; c++)
-        for (uint32_t b = 0; b < g.c[c].nb; b++) {
-            const JxsBand& bd = g.c[c].b[b];
-            for (uint32_t l = 0; l < bd.lines[type]; l++) {
-                const uint32_t y = row * bd.hl + l, bl = bd.bl + l;
-                const uint8_t* gl = F.gcli + bd.gcli_off + (size_t)y * bd.gw;
-                const uint8_t* sl = F.smax + bd.smax_off + (size_t)y * bd.sw;
-                for (uint32_t i = threadIdx.x; i < bd.gw; i += blockDim.x)
-                    atomicAdd(&hist[bl][gl[i] & 15], 1u);
-                const uint32_t full = bd.gw / 8;
-                for (uint32_t i = threadIdx.x; i < full; i += blockDim.x)
-                    atomicAdd(&shist[bl][sl[i] & 15], 1u);
-                if (threadIdx.x == 0 && full < bd.sw)
-                    tailv[bl] = sl[full];
-            }
-        }
+    for (uint32_t i = threadIdx.x; i < nbl * 16; i += blockDim.x) {
+        const uint32_t bl = i >> 4, t = i & 15;
+        const LineBits b = line_bits(rec[bl], t, b_gw[bl_band[bl]]);
+        t_ns[bl][t] = b.ns;
+        t_zr[bl][t] = b.zeroed;
+        t_d[bl][t] = b.data;
+    }
     __syncthreads();
-    /* cumulative tables per band line */
-    uint32_t* glut = F.lut + (size_t)row * g.n_bl * JXS_LUT;
-    for (uint32_t bl = threadIdx.x; bl < g.n_bl; bl += blockDim.x) {
-        uint32_t* L = lut + bl * JXS_LUT;
-        uint32_t acc = 0, sacc = 0;
-        for (int t = 0; t < 16; t++) {
-            acc += hist[bl][t];
-            sacc += shist[bl][t];
-            L[LUT_CNT + t] = acc;
-            L[LUT_SIG + t] = sacc;
+    for (uint32_t i = threadIdx.x; i < g.nb_all * 16; i += blockDim.x) {
+        const uint32_t gi = i >> 4, t = i & 15, nl = b_nl[gi];
+        if (nl == 0)
+            continue;
+        const uint32_t bl = b_bl[gi], sw = b_sw[gi];
+        uint32_t nosig = 0, sig = sw * nl;
+        for (uint32_t k = 0; k < nl; k++) {
+            nosig += t_ns[bl + k][t];
+            sig += t_ns[bl + k][t] - t_zr[bl + k][t];
         }
-        /* data[t] = sum_{i > t} (i - t + 1) * hist[i], built from the top */
-        uint32_t above = 0, d = 0;
-        L[LUT_DATA + 15] = 0;
-        for (int t = 14; t >= 0; t--) {
-            above += hist[bl][t + 1];
-            d += above + hist[bl][t + 1];
-            L[LUT_DATA + t] = d;
+        const bool use = g.sig && nosig > sig;
+        for (uint32_t k = 0; k < nl; k++) {
+            t_g[bl + k][t] = use ? t_ns[bl + k][t] - t_zr[bl + k][t] : t_ns[bl + k][t];
+            t_s[bl + k][t] = use ? sw : 0u;
         }
-        L[LUT_TAIL] = (uint32_t)tailv[bl];
-        for (int k = 0; k < JXS_LUT; k++)
-            glut[bl * JXS_LUT + k] = L[k];
     }
     __syncthreads();
     uint32_t* tab = F.table + (size_t)row * JXS_NQ * JXS_NR;
-    const uint32_t nq = g.maxq + 1, nr = g.maxr + 1;
+    const uint32_t nq = g.maxq + 1, nr = g.maxr + 1, npk = g.npk;
     for (uint32_t idx = threadIdx.x; idx < nq * nr; idx += blockDim.x) {
         const uint32_t q = idx / nr, r = idx % nr;
-        tab[q * JXS_NR + r] = precinct_bytes(g, type, lut, q, r);
+        uint32_t total = 0;
+        for (uint32_t p = 0; p < npk; p++) {
+            const uint32_t l = g.pk_line[p];
+            uint32_t d = 0, gb = 0, sb = 0;
+            for (uint32_t gi = g.pk_bs[p]; gi < g.pk_be[p]; gi++) {
+                if (l >= b_nl[gi])
+                    continue;
+                const uint32_t gt = truncation(b_gain[gi], b_prio[gi], q, r), bl = b_bl[gi] + l;
+                d += t_d[bl][gt];
+                gb += t_g[bl][gt];
+                sb += t_s[bl][gt];
+            }
+            const uint32_t db = (d + 7) >> 3, gbb = (gb + 7) >> 3, sbb = (sb + 7) >> 3;
+            const uint32_t raw = g.raw_bytes[type][p];
+            total += db + (sbb + gbb > raw ? raw : sbb + gbb);
+        }
+        tab[q * JXS_NR + r] = total;
     }
 }
 
@@ -399,8 +661,18 @@ struct BSearch {
     }
 };
 
-__global__ void __launch_bounds__(64) k_select(const JxsFrameDesc* __restrict__ frames, int nfr, uint32_t total_slices) {
-    const uint32_t gs = blockIdx.x * blockDim.x + threadIdx.x;
+#define SEL_LDS 8192 /* u32 of staged rate tables per slice block */
+
+/* One block per slice: the slice's rate tables are staged in LDS (as many
+ * precincts at a time as fit), then thread 0 walks the precincts in order:
+ * budgets with the +10 % first precinct and the padding carry
+ * (PackStageProcess.c:600-684), the binary searches of Q then R
+ * (RateControl.c:889-1083) and the precinct offsets. */
+#define SEL_THREADS 256
+__global__ void __launch_bounds__(SEL_THREADS) k_select(const JxsFrameDesc* __restrict__ frames, int nfr,
+                                                        uint32_t total_slices) {
+    __shared__ uint32_t tabs[SEL_LDS];
+    const uint32_t gs = blockIdx.x;
     if (gs >= total_slices)
         return;
     int lo = 0, hi = nfr - 1;
@@ -412,7 +684,10 @@ __global__ void __launch_bounds__(64) k_select(const JxsFrameDesc* __restrict__ 
             hi = mid - 1;
     }
     const JxsFrameDesc& F = frames[lo];
-    const JxsGeom& g = *F.g;
+    __shared__ __align__(16) JxsGeom sg;
+    stage_geom(&sg, F.g);
+    __syncthreads();
+    const JxsGeom& g = sg;
     const uint32_t s = gs - F.slice_base;
     const uint32_t first_row = s * g.pps;
     const uint32_t nprec = (s == g.slices - 1) ? g.rows - first_row : g.pps;
@@ -426,385 +701,571 @@ __global__ void __launch_bounds__(64) k_select(const JxsFrameDesc* __restrict__ 
         minb = (sbud - first) / (nprec - 1);
         left = sbud - minb * (nprec - 1) - first;
     }
+    const uint32_t per = (g.maxq + 1) * JXS_NR; /* staged words per precinct */
+    const uint32_t chunk = SEL_LDS / per;       /* >= 8 */
     uint32_t carry = 0, off = F.slice_off[s] + 6;
     bool failed = false;
-    for (uint32_t i = 0; i < nprec; i++) {
-        const uint32_t row = first_row + i, type = row + 1 >= g.rows ? 1 : 0;
-        JxsPrec P;
-        P.q = P.r = P.total = P.pad = 0;
-        P.offset = off;
-        P.err = 1;
-        if (failed) {
-            F.prec[row] = P;
-            continue;
-        }
-        const uint32_t budget = (i == 0 ? first : minb) + (i < left ? 1 : 0) + carry;
-        const uint32_t hdr = g.rc_hdr_bytes[type];
-        const uint32_t* tab = F.table + (size_t)row * JXS_NQ * JXS_NR;
-        bool ok = budget <= 1048575u && budget > hdr;
-        uint32_t q = 0, r = 0, data = 0;
-        if (ok) {
-            const uint32_t avail = budget - hdr;
-            BSearch bs;
-            bs.init(0, (int32_t)g.maxq, 0, 6 > g.maxq ? 0 : 6);
-            int res, nx = 0;
-            while ((res = bs.next(nx, &q)) == 0) {
-                if ((g.empty_mask[q] & 1u) != 0) {
-                    nx = 3;
-                    continue;
-                }
-                nx = tab[q * JXS_NR] > avail ? 1 : 2;
+    for (uint32_t i0 = 0; i0 < nprec; i0 += chunk) {
+        const uint32_t n = min(chunk, nprec - i0);
+        const uint32_t words = n * per;
+        const uint32_t* src = F.table + (size_t)(first_row + i0) * JXS_NQ * JXS_NR;
+        __syncthreads();
+        /* consecutive precincts' tables are JXS_NQ * JXS_NR words apart */
+        for (uint32_t b0 = 0; b0 < words; b0 += blockDim.x * 8) {
+            uint32_t v[8];
+#pragma unroll
+            for (uint32_t j = 0; j < 8; j++) {
+                const uint32_t idx = b0 + j * blockDim.x + threadIdx.x;
+                const uint32_t ic = min(idx, words - 1); /* clamped: every load in bounds */
+                v[j] = gld(src + (size_t)(ic / per) * JXS_NQ * JXS_NR + ic % per);
             }
-            ok = res == 1;
+#pragma unroll
+            for (uint32_t j = 0; j < 8; j++) {
+                const uint32_t idx = b0 + j * blockDim.x + threadIdx.x;
+                if (idx < words)
+                    tabs[idx] = v[j];
+            }
+        }
+        __syncthreads();
+        for (uint32_t i = i0; i < i0 + n && threadIdx.x == 0; i++) {
+            const uint32_t row = first_row + i, type = row + 1 >= g.rows ? 1 : 0;
+            JxsPrec P;
+            P.q = P.r = P.total = P.pad = 0;
+            P.offset = off;
+            P.err = 1;
+            if (failed) {
+                F.prec[row] = P;
+                continue;
+            }
+            const uint32_t budget = (i == 0 ? first : minb) + (i < left ? 1 : 0) + carry;
+            const uint32_t hdr = g.rc_hdr_bytes[type];
+            const uint32_t* tab = tabs + (i - i0) * per;
+            bool ok = budget <= 1048575u && budget > hdr;
+            uint32_t q = 0, r = 0, data = 0;
             if (ok) {
-                bs.init(0, (int32_t)g.maxr, 1, 6 >= g.maxr ? 0 : 6);
-                nx = 0;
-                while ((res = bs.next(nx, &r)) == 0) {
-                    if ((g.empty_mask[q] >> r) & 1u) {
+                const uint32_t avail = budget - hdr;
+                BSearch bs;
+                bs.init(0, (int32_t)g.maxq, 0, 6 > g.maxq ? 0 : 6);
+                int res, nx = 0;
+                while ((res = bs.next(nx, &q)) == 0) {
+                    if ((g.empty_mask[q] & 1u) != 0) {
                         nx = 3;
                         continue;
                     }
-                    nx = tab[q * JXS_NR + r] <= avail ? 1 : 2;
+                    nx = tab[q * JXS_NR] > avail ? 1 : 2;
                 }
                 ok = res == 1;
                 if (ok) {
-                    data = tab[q * JXS_NR + r];
-                    uint32_t pad = avail - data, total = budget;
-                    if (i + 1 < nprec) {
-                        total -= pad;
-                        carry = pad;
-                        pad = 0;
+                    bs.init(0, (int32_t)g.maxr, 1, 6 >= g.maxr ? 0 : 6);
+                    nx = 0;
+                    while ((res = bs.next(nx, &r)) == 0) {
+                        if ((g.empty_mask[q] >> r) & 1u) {
+                            nx = 3;
+                            continue;
+                        }
+                        nx = tab[q * JXS_NR + r] <= avail ? 1 : 2;
                     }
-                    P.q = q;
-                    P.r = r;
-                    P.total = total;
-                    P.pad = pad;
-                    P.err = 0;
-                    off += total;
+                    ok = res == 1;
+                    if (ok) {
+                        data = tab[q * JXS_NR + r];
+                        uint32_t pad = avail - data, total = budget;
+                        if (i + 1 < nprec) {
+                            total -= pad;
+                            carry = pad;
+                            pad = 0;
+                        }
+                        P.q = q;
+                        P.r = r;
+                        P.total = total;
+                        P.pad = pad;
+                        P.err = 0;
+                        off += total;
+                    }
                 }
             }
+            if (!ok) {
+                failed = true;
+                atomicOr(F.status, 1u);
+            }
+            F.prec[row] = P;
         }
-        if (!ok) {
-            failed = true;
-            atomicOr(F.status, 1u);
-        }
-        F.prec[row] = P;
     }
+}
+
+/* ------------------------------------------------------------------ */
+/* packet plan                                                         */
+/* ------------------------------------------------------------------ */
+/* Inclusive sum over the 64 lanes of a wave with DPP moves: shifts by 1, 2,
+ * 4, 8 inside each row of 16 lanes, then the row totals broadcast from lane
+ * 15 into rows 1 and 3 and from lane 31 into rows 2 and 3.  Every lane of
+ * the wave must be active. */
+#define DPP_ROW_SHR(n) (0x110 + (n))
+#define DPP_ROW_BCAST15 0x142
+#define DPP_ROW_BCAST31 0x143
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v, uint32_t /*lane*/) {
+    int x = (int)v;
+    x += __builtin_amdgcn_update_dpp(0, x, DPP_ROW_SHR(1), 0xF, 0xF, true);
+    x += __builtin_amdgcn_update_dpp(0, x, DPP_ROW_SHR(2), 0xF, 0xF, true);
+    x += __builtin_amdgcn_update_dpp(0, x, DPP_ROW_SHR(4), 0xF, 0xF, true);
+    x += __builtin_amdgcn_update_dpp(0, x, DPP_ROW_SHR(8), 0xF, 0xF, true);
+    x += __builtin_amdgcn_update_dpp(0, x, DPP_ROW_BCAST15, 0xA, 0xF, false);
+    x += __builtin_amdgcn_update_dpp(0, x, DPP_ROW_BCAST31, 0xC, 0xF, false);
+    return (uint32_t)x;
+}
+
+__device__ __forceinline__ uint32_t wave_last(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, WAVE - 1);
+}
+
+/* One single-wave block per precinct, after rate control: truncation and
+ * coding method of every band for the chosen (Q, R), the sub-packet sizes,
+ * RAW choice and codestream offset of every packet (PackPrecinct.c:235-572,
+ * RateControl.c:457-781), and the bytes around the packets: slice header
+ * (PackHeaders.c:142-156), precinct header (PackHeaders.c:158-196),
+ * padding, end-of-codestream marker and, for the first precinct, the frame
+ * header.  A precinct whose packets would not add up to the size the rate
+ * control chose is flagged and nothing of it is written. */
+__global__ void __launch_bounds__(64) k_plan(const JxsFrameDesc* __restrict__ frames, int nfr) {
+    __shared__ uint32_t bd_d[JXS_NBL], bd_g[JXS_NBL], bd_s[JXS_NBL];
+    __shared__ uint8_t b_ms[JXS_NGB];
+    const int f = find_frame(frames, nfr, blockIdx.x);
+    const JxsFrameDesc& F = frames[f];
+    __shared__ __align__(16) JxsGeom sg;
+    stage_geom(&sg, F.g);
+    __syncthreads();
+    const JxsGeom& g = sg;
+    const uint32_t row = blockIdx.x - F.prec_base;
+    const uint32_t type = row + 1 >= g.rows ? 1 : 0;
+    const uint32_t lane = threadIdx.x;
+    const JxsPrec P = F.prec[row];
+    JxsPlan& PL = F.plan[row];
+    if (P.err) {
+        /* nothing of a failed precinct is packed */
+        if (lane < g.npk)
+            PL.pk[lane].flags = 0;
+        for (uint32_t u = lane; u < g.n_units; u += WAVE)
+            PL.ulen[u] = 0;
+        return;
+    }
+    const uint16_t* recs = F.hist + (size_t)row * g.n_bl * JXS_HREC;
+    if (lane < g.nb_all) {
+        const uint32_t gi = lane;
+        uint32_t ms = 0, gt = 0;
+        if (g.gb[gi] != 0xFF) {
+            const JxsBand& bd = g.c[g.gc[gi]].b[g.gb[gi]];
+            gt = truncation(bd.gain, bd.prio, P.q, P.r);
+            const uint32_t nl = bd.lines[type]; /* 0, 1 or 2 */
+            LineBits l0 = {0, 0, 0}, l1 = {0, 0, 0};
+            if (nl > 0)
+                l0 = line_bits(recs + (size_t)bd.bl * JXS_HREC, gt, bd.gw);
+            if (nl > 1)
+                l1 = line_bits(recs + (size_t)(bd.bl + 1) * JXS_HREC, gt, bd.gw);
+            const uint32_t nosig = l0.ns + l1.ns;
+            const uint32_t sig = bd.sw * nl + (l0.ns - l0.zeroed) + (l1.ns - l1.zeroed);
+            ms = (g.sig && nl > 0 && nosig > sig) ? 1u : 0u;
+            if (nl > 0) {
+                bd_d[bd.bl] = l0.data;
+                bd_g[bd.bl] = ms ? l0.ns - l0.zeroed : l0.ns;
+                bd_s[bd.bl] = ms ? bd.sw : 0u;
+            }
+            if (nl > 1) {
+                bd_d[bd.bl + 1] = l1.data;
+                bd_g[bd.bl + 1] = ms ? l1.ns - l1.zeroed : l1.ns;
+                bd_s[bd.bl + 1] = ms ? bd.sw : 0u;
+            }
+        }
+        b_ms[gi] = (uint8_t)ms;
+        PL.band[gi] = (uint8_t)(gt | (ms << 4));
+    }
+    __syncthreads();
+    uint32_t size = 0, sb = 0, gbb = 0, db = 0, flags = 0;
+    if (lane < g.npk) {
+        const uint32_t p = lane, l = g.pk_line[p];
+        uint32_t d = 0, gb = 0, s = 0, any = 0;
+        for (uint32_t gi = g.pk_bs[p]; gi < g.pk_be[p]; gi++) {
+            const JxsBand& bd = g.c[g.gc[gi]].b[g.gb[gi]];
+            if (l >= bd.lines[type])
+                continue;
+            any = 1;
+            d += bd_d[bd.bl + l];
+            gb += bd_g[bd.bl + l];
+            s += bd_s[bd.bl + l];
+        }
+        db = (d + 7) >> 3;
+        gbb = (gb + 7) >> 3;
+        sb = (s + 7) >> 3;
+        const uint32_t raw = g.raw_bytes[type][p];
+        if (sb + gbb > raw) {
+            flags = PKT_RAW;
+            sb = 0;
+            gbb = raw;
+        }
+        if (any) {
+            flags |= PKT_ANY;
+            size = g.pkt_hdr_bytes + sb + gbb + db;
+        }
+    }
+    const uint32_t incl = wave_incl_scan(size, lane);
+    const uint32_t sum = wave_last(incl);
+    const uint32_t eoc = row + 1 == g.rows ? 2u : 0u;
+    const bool valid = g.prec_hdr_bytes + sum + P.pad == P.total && P.offset >= 6 &&
+        (uint64_t)P.offset + P.total + eoc <= g.lcod;
+    if (!valid) {
+        if (lane < g.npk)
+            PL.pk[lane].flags = 0;
+        for (uint32_t u = lane; u < g.n_units; u += WAVE)
+            PL.ulen[u] = 0;
+        if (lane == 0)
+            atomicOr(F.status, 2u);
+        return;
+    }
+    const uint32_t rel = g.prec_hdr_bytes + incl - size; /* packet offset from the precinct's first byte */
+    if (lane < g.npk) {
+        JxsPkt K;
+        K.off = P.offset + rel;
+        K.s = sb;
+        K.g = gbb;
+        K.d = db;
+        K.flags = flags;
+        PL.pk[lane] = K;
+    }
+    /* packing units: lane p walks the units of packet p (header, then one
+     * per band of each sub-packet) and places them bit by bit */
+    const uint32_t nu_p = lane < g.npk ? 1 + 3 * (g.pk_be[lane] - g.pk_bs[lane]) : 0u;
+    uint32_t u = wave_incl_scan(nu_p, lane) - nu_p;
+    if (lane < g.npk) {
+        const uint32_t p = lane, l = g.pk_line[p];
+        const bool raw = (flags & PKT_RAW) != 0, any = (flags & PKT_ANY) != 0;
+        PL.ubit[u] = 8 * rel;
+        PL.ulen[u] = any ? 8 * g.pkt_hdr_bytes : 0u;
+        u++;
+        for (uint32_t kind = UNIT_SIG; kind <= UNIT_DATA; kind++) {
+            uint32_t cur = 8 * (rel + g.pkt_hdr_bytes + (kind >= UNIT_GCLI ? sb : 0u) + (kind == UNIT_DATA ? gbb : 0u));
+            for (uint32_t gi = g.pk_bs[p]; gi < g.pk_be[p]; gi++) {
+                const JxsBand& bd = g.c[g.gc[gi]].b[g.gb[gi]];
+                uint32_t bits = 0;
+                if (l < bd.lines[type]) {
+                    const uint32_t bl = bd.bl + l;
+                    if (kind == UNIT_SIG)
+                        bits = raw ? 0u : bd_s[bl];
+                    else if (kind == UNIT_GCLI)
+                        bits = raw ? 4 * bd.gw : bd_g[bl];
+                    else
+                        bits = bd_d[bl];
+                }
+                PL.ubit[u] = cur;
+                PL.ulen[u] = bits;
+                cur += bits;
+                u++;
+            }
+        }
+    }
+    uint8_t* o = F.out;
+    /* packet bodies and padding start as zeros: the packing units OR their
+     * first and last words in */
+    {
+        const uint32_t z0 = P.offset + g.prec_hdr_bytes, z1 = P.offset + P.total;
+        const uint32_t a0 = (z0 + 3) & ~3u, a1 = z1 & ~3u;
+        if (a0 < a1) {
+            for (uint32_t wd = a0 / 4 + lane; wd < a1 / 4; wd += WAVE)
+                gst((uint32_t*)o + wd, 0u);
+            if (z0 + lane < a0)
+                gst(o + z0 + lane, (uint8_t)0);
+            if (a1 + lane < z1)
+                gst(o + a1 + lane, (uint8_t)0);
+        } else {
+            for (uint32_t k = z0 + lane; k < z1; k += WAVE)
+                gst(o + k, (uint8_t)0);
+        }
+    }
+    /* slice header */
+    if (row % g.pps == 0 && lane < 6) {
+        const uint32_t s = row / g.pps;
+        /* FF20 (SLH marker), Lslh = 4, slice index */
+        const uint32_t v = lane == 0 ? 0xFFu : lane == 1 ? 0x20u : lane == 3 ? 0x04u : lane == 4 ? (s >> 8) : lane == 5 ? s : 0u;
+        o[P.offset - 6 + lane] = (uint8_t)v;
+    }
+    /* precinct header: Lprc, Q, R, then 2 bits per existing band */
+    const uint32_t lprc = P.total - g.prec_hdr_bytes;
+    for (uint32_t k = lane; k < g.prec_hdr_bytes; k += WAVE) {
+        uint32_t v;
+        if (k < 3)
+            v = lprc >> (16 - 8 * k);
+        else if (k == 3)
+            v = P.q;
+        else if (k == 4)
+            v = P.r;
+        else {
+            v = 0;
+            for (uint32_t j = 0; j < 4; j++) {
+                const uint32_t e = 4 * (k - 5) + j;
+                if (e < g.nb_exist && b_ms[g.ex_gid[e]])
+                    v |= 2u << (6 - 2 * j);
+            }
+        }
+        o[P.offset + k] = (uint8_t)v;
+    }
+    if (eoc && lane < 2)
+        o[P.offset + P.total + lane] = lane ? 0x11 : 0xFF;
+    if (row == 0)
+        for (uint32_t k = lane; k < g.hdr_len; k += WAVE)
+            o[k] = g.hdr[k];
 }
 
 /* ------------------------------------------------------------------ */
 /* packing                                                             */
 /* ------------------------------------------------------------------ */
-#define RING 2048 /* 32-bit words of the LDS bit ring (8 KiB) */
+#define PK_WAVES 4
+#define RINGW 1024       /* 32-bit words of one wave's LDS bit ring (4 KiB): one step adds at most 512 */
+#define RING_FLUSH 256   /* flush once this many complete words are pending */
 
-__device__ __forceinline__ void ring_put(uint32_t* ring, uint64_t pos, uint64_t v, uint32_t n) {
-    while (n) {
-        const uint32_t sh = (uint32_t)(pos & 31), room = 32 - sh, k = n < room ? n : room;
-        const uint32_t part = (uint32_t)((v >> (n - k)) & ((k == 32) ? 0xFFFFFFFFull : ((1ull << k) - 1)));
-        atomicOr(&ring[(pos >> 5) % RING], part << (room - k));
-        pos += k;
-        n -= k;
-    }
+/* Orders this wave's LDS accesses around cross-lane reads of the ring. */
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-/* Writes words [from, to) of the ring as bytes of the sub-packet (only bytes
- * below nbytes) and clears them for reuse. */
-__device__ void ring_flush(uint32_t* ring, uint8_t* out, uint32_t nbytes, uint32_t from, uint32_t to) {
-    for (uint32_t w = from + threadIdx.x; w < to; w += blockDim.x) {
-        const uint32_t v = ring[w % RING];
-        ring[w % RING] = 0;
-        const uint32_t b0 = 4 * w;
-#pragma unroll
-        for (uint32_t k = 0; k < 4; k++)
-            if (b0 + k < nbytes)
-                out[b0 + k] = (uint8_t)(v >> (24 - 8 * k));
-    }
-    __syncthreads();
+/* ORs the n (1..64) low bits of v, MSB first, into the ring at bit pos: the
+ * bits are MSB-aligned in 64 bits and cut at the word boundaries; words
+ * that receive only zeros are skipped. */
+__device__ __forceinline__ void ring_put(uint32_t* ring, uint32_t pos, uint64_t v, uint32_t n) {
+    const uint32_t sh = pos & 31, wd = pos >> 5;
+    const uint64_t a = v << (64 - n);
+    const uint32_t w0 = (uint32_t)(a >> (32 + sh));
+    const uint64_t rest = a << (32 - sh);
+    const uint32_t w1 = (uint32_t)(rest >> 32), w2 = (uint32_t)rest;
+    if (w0)
+        atomicOr(&ring[wd & (RINGW - 1)], w0);
+    if (w1)
+        atomicOr(&ring[(wd + 1) & (RINGW - 1)], w1);
+    if (w2)
+        atomicOr(&ring[(wd + 2) & (RINGW - 1)], w2);
 }
-
-struct PackCtx {
-    uint8_t gtli[JXS_MAXC][JXS_NBC];
-    uint8_t msig[JXS_MAXC][JXS_NBC];
-    uint32_t bits_d[JXS_NBL], bits_g[JXS_NBL], bits_s[JXS_NBL];
-    uint32_t pk_d[JXS_NPK], pk_g[JXS_NPK], pk_s[JXS_NPK], pk_raw[JXS_NPK], pk_off[JXS_NPK], pk_any[JXS_NPK];
-    uint32_t scan[8];
-};
 
 /* kinds of item streams */
 enum { ITEM_RAW = 0, ITEM_SIG = 1, ITEM_VLC = 2, ITEM_DATA = 3 };
 
-/* Emits the items of one band line into the ring at bit `pos0` (relative to
- * the sub-packet start); returns the number of bits written. */
-__device__ uint32_t emit_line(uint32_t* ring, PackCtx& C, uint8_t* out, uint32_t nbytes, uint32_t& flushed, uint32_t pos0,
-                              int kind, const uint8_t* gl, const uint8_t* sl, const uint16_t* cl, uint32_t w, uint32_t gw,
-                              uint32_t sw, uint32_t gt, uint32_t use_sig, uint32_t qpih) {
-    const uint32_t nitems = kind == ITEM_SIG ? sw : gw;
-    uint32_t pos = pos0;
-    for (uint32_t base = 0; base < nitems; base += blockDim.x) {
-        /* everything below pos is final: flush the completed words */
-        const uint32_t upto = pos >> 5;
-        if (upto > flushed) {
-            ring_flush(ring, out, nbytes, flushed, upto);
-            flushed = upto;
-        }
-        const uint32_t i = base + threadIdx.x;
-        uint32_t nb = 0;
-        uint64_t v = 0;
-        if (i < nitems) {
-            if (kind == ITEM_RAW) {
-                nb = 4;
-                v = gl[i];
-            } else if (kind == ITEM_SIG) {
-                nb = 1;
-                v = sl[i] <= gt ? 1u : 0u;
-            } else if (kind == ITEM_VLC) {
-                if (!(use_sig && sl[i / 8] <= gt)) {
-                    const uint32_t gc = gl[i], m = gc > gt ? gc - gt : 0;
-                    nb = m + 1;
-                    v = ((1ull << m) - 1) << 1;
-                }
-            } else {
-                const uint32_t gc = gl[i];
-                if (gc > gt) {
-                    uint32_t qv[4];
-#pragma unroll
-                    for (uint32_t k = 0; k < 4; k++) {
-                        const uint32_t ci = 4 * i + k;
-                        uint32_t val = 0;
-                        if (ci < w) {
-                            const uint32_t c = cl[ci], sign = c & 0x8000u, m = c & 0x7FFFu;
-                            uint32_t o;
-                            if (gt == 0) {
-                                o = m;
-                            } else if (qpih == 0) {
-                                o = (m >> gt) << gt;
-                            } else {
-                                const uint32_t sc = gc - gt + 1;
-                                const uint32_t dq = (((m << sc) - m + (1u << gc)) >> (gc + 1)) & 0xFFFFu;
-                                o = (dq << gt) & 0xFFFFu;
-                            }
-                            val = o ? (o | sign) : 0u;
-                        }
-                        qv[k] = val;
-                    }
-                    v = ((qv[0] >> 15) << 3) | ((qv[1] >> 15) << 2) | ((qv[2] >> 15) << 1) | (qv[3] >> 15);
-                    for (int bit = (int)gc - 1; bit >= (int)gt; bit--) {
-                        const uint32_t nib = (((qv[0] >> bit) & 1u) << 3) | (((qv[1] >> bit) & 1u) << 2) |
-                            (((qv[2] >> bit) & 1u) << 1) | ((qv[3] >> bit) & 1u);
-                        v = (v << 4) | nib;
-                    }
-                    nb = 4 * (gc - gt + 1);
-                }
-            }
-        }
-        uint32_t tot;
-        const uint32_t off = block_scan(nb, C.scan, &tot);
-        if (nb)
-            ring_put(ring, (uint64_t)pos + off, v, nb);
-        __syncthreads();
-        pos += tot;
-    }
-    return pos - pos0;
+/* bit i of x (x < 2^8) -> bit 4i */
+__device__ __forceinline__ uint32_t spread8(uint32_t x) {
+    x = (x | (x << 12)) & 0x000F000Fu;
+    x = (x | (x << 6)) & 0x03030303u;
+    x = (x | (x << 3)) & 0x11111111u;
+    return x;
 }
 
-__global__ void __launch_bounds__(256) k_pack(const JxsFrameDesc* __restrict__ frames, int nfr) {
-    __shared__ uint32_t ring[RING];
-    __shared__ PackCtx C;
-    const int f = find_frame(frames, nfr, blockIdx.y);
-    const JxsFrameDesc& F = frames[f];
-    const JxsGeom& g = *F.g;
-    const uint32_t row = blockIdx.y - F.prec_base;
-    const uint32_t p = blockIdx.x;
-    if (p >= g.npk)
-        return;
-    const JxsPrec P = F.prec[row];
-    if (P.err)
-        return;
-    const uint32_t type = row + 1 >= g.rows ? 1 : 0;
-    const uint32_t* lut = F.lut + (size_t)row * g.n_bl * JXS_LUT;
-    /* per band: truncation and coding method for (Q, R) */
-    for (uint32_t gi = threadIdx.x; gi < g.nb_all; gi += blockDim.x) {
-        if (g.gb[gi] == 0xFF)
-            continue;
-        const uint32_t c = g.gc[gi], b = g.gb[gi];
-        const JxsBand& bd = g.c[c].b[b];
-        const uint32_t gt = truncation(bd.gain, bd.prio, P.q, P.r);
-        C.gtli[c][b] = (uint8_t)gt;
-        const uint32_t nl = bd.lines[type];
-        uint32_t nosig = 0, sig = bd.sw * nl;
-        for (uint32_t k = 0; k < nl; k++) {
-            const uint32_t* L = lut + (size_t)(bd.bl + k) * JXS_LUT;
-            const uint32_t ns = L[LUT_DATA + gt] + L[LUT_CNT + gt];
-            uint32_t zeroed = L[LUT_SIG + gt] * 8;
-            const int32_t tail = (int32_t)L[LUT_TAIL];
-            if (tail >= 0 && (uint32_t)tail <= gt)
-                zeroed += bd.gw - (bd.gw / 8) * 8;
-            nosig += ns;
-            sig += ns - zeroed;
-            C.bits_d[bd.bl + k] = 4 * L[LUT_DATA + gt];
-            C.bits_g[bd.bl + k] = ns;
-            C.bits_s[bd.bl + k] = zeroed; /* temporarily: bits removed by significance */
-        }
-        const uint32_t use = (g.sig && nl > 0 && nosig > sig) ? 1u : 0u;
-        C.msig[c][b] = (uint8_t)use;
-        for (uint32_t k = 0; k < nl; k++) {
-            if (use) {
-                C.bits_g[bd.bl + k] -= C.bits_s[bd.bl + k];
-                C.bits_s[bd.bl + k] = bd.sw;
-            } else {
-                C.bits_s[bd.bl + k] = 0;
-            }
-        }
+/* Bit-plane nibbles of four magnitudes (< 2^15): nibble j holds bit j of
+ * q0, q1, q2, q3 from its MSB down.  The high word (planes 8..14) is built
+ * only when some lane of the wave needs it. */
+__device__ __forceinline__ uint64_t interleave4(uint32_t q0, uint32_t q1, uint32_t q2, uint32_t q3, bool any_high) {
+    const uint32_t lo = (spread8(q0 & 0xFFu) << 3) | (spread8(q1 & 0xFFu) << 2) | (spread8(q2 & 0xFFu) << 1) |
+        spread8(q3 & 0xFFu);
+    uint32_t hi = 0;
+    if (any_high)
+        hi = (spread8(q0 >> 8) << 3) | (spread8(q1 >> 8) << 2) | (spread8(q2 >> 8) << 1) | spread8(q3 >> 8);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+/* Magnitude of one quantised coefficient shifted down by gt
+ * (Quant.c:9-45: deadzone keeps the bit-planes above gt, uniform rounds). */
+__device__ __forceinline__ uint32_t quant_mag(uint32_t c, uint32_t gc, uint32_t gt, uint32_t qpih) {
+    const uint32_t m = c & 0x7FFFu;
+    if (gt == 0)
+        return m;
+    if (qpih == 0)
+        return m >> gt;
+    const uint32_t sc = gc - gt + 1;
+    return (((m << sc) - m + (1u << gc)) >> (gc + 1)) & 0xFFFFu;
+}
+
+/* The output of one packing unit: bits [lo, hi) counted from the dword at
+ * ob (the dword holding the unit's first bit), assembled in the wave's LDS
+ * ring; ring word w is output dword ob[w]. */
+struct WaveOut {
+    uint32_t* ring;
+    uint32_t* ob;
+    uint32_t lo, hi, pos, flushed, lane;
+};
+
+/* Writes ring words [from, to) to the output and clears them.  A word that
+ * lies wholly inside the unit is stored; the unit's first and last words,
+ * which it may share with its neighbours, are ORed into the zeroed output
+ * (device-scope atomics: the neighbours may run on other XCDs). */
+__device__ __forceinline__ void ring_flush(WaveOut& W, uint32_t from, uint32_t to) {
+    for (uint32_t w = from + W.lane; w < to; w += WAVE) {
+        uint32_t* slot = &W.ring[w & (RINGW - 1)];
+        const uint32_t v = *slot;
+        *slot = 0;
+        const uint32_t b0 = 32 * w;
+        if (b0 >= W.lo && b0 + 32 <= W.hi)
+            gst(W.ob + w, __builtin_bswap32(v));
+        else if (v)
+            atomicOr(W.ob + w, __builtin_bswap32(v));
     }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        uint32_t off = P.offset + g.prec_hdr_bytes;
-        for (uint32_t pp = 0; pp < g.npk; pp++) {
-            uint32_t d = 0, gb = 0, sb = 0, any = 0;
-            const uint32_t l = g.pk_line[pp];
-            for (uint32_t gi = g.pk_bs[pp]; gi < g.pk_be[pp]; gi++) {
-                const JxsBand& bd = g.c[g.gc[gi]].b[g.gb[gi]];
-                if (l >= bd.lines[type])
-                    continue;
-                any = 1;
-                d += C.bits_d[bd.bl + l];
-                gb += C.bits_g[bd.bl + l];
-                sb += C.bits_s[bd.bl + l];
-            }
-            const uint32_t db = (d + 7) >> 3, gbb = (gb + 7) >> 3, sbb = (sb + 7) >> 3;
-            C.pk_any[pp] = any;
-            C.pk_d[pp] = db;
-            if (sbb + gbb > g.raw_bytes[type][pp]) {
-                C.pk_raw[pp] = 1;
-                C.pk_s[pp] = 0;
-                C.pk_g[pp] = g.raw_bytes[type][pp];
-            } else {
-                C.pk_raw[pp] = 0;
-                C.pk_s[pp] = sbb;
-                C.pk_g[pp] = gbb;
-            }
-            C.pk_off[pp] = off;
-            if (any)
-                off += g.pkt_hdr_bytes + C.pk_s[pp] + C.pk_g[pp] + C.pk_d[pp];
+}
+
+
+#define IPL 4 /* consecutive items coded by one lane per step */
+
+/* Appends the items of one band line, 256 per step: lane L codes items
+ * 4L .. 4L+3 of the step (consecutive, so its bits are contiguous), one wave
+ * prefix sum over the lanes' bit counts places them.  Bit-plane-count,
+ * significance and raw codes of a lane are concatenated into one value
+ * (at most 4 x 16 bits); data groups are written one by one. */
+__device__ __forceinline__ void emit_line(WaveOut& W, int kind, const uint8_t* gl, const uint8_t* sl,
+                                          const uint16_t* cl, uint32_t w, uint32_t gw, uint32_t sw, uint32_t gt,
+                                          uint32_t use_sig, uint32_t qpih) {
+    const uint32_t nitems = kind == ITEM_SIG ? sw : gw;
+    const uint32_t lane = W.lane;
+    for (uint32_t base = 0; base < nitems; base += IPL * WAVE) {
+        const uint32_t i0 = base + IPL * lane;
+        uint32_t a[IPL], s8[IPL];
+        uint2 cv[IPL];
+#pragma unroll
+        for (uint32_t k = 0; k < IPL; k++) {
+            const uint32_t idx = min(i0 + k, nitems - 1); /* clamped: every load in bounds */
+            a[k] = kind == ITEM_SIG ? gld(sl + idx) : gld(gl + idx);
+            s8[k] = (kind == ITEM_VLC && use_sig) ? gld(sl + idx / 8) : 0u;
+            cv[k] = kind == ITEM_DATA ? gld2(cl + 4 * idx) : make_uint2(0u, 0u);
         }
-        /* precinct-level bytes, written by the first packet's block */
-        if (p == 0 && (uint64_t)P.offset + P.total + (row + 1 == g.rows ? 2u : 0u) <= g.lcod && P.offset >= 6) {
-            uint8_t* o = F.out;
-            const uint32_t s = row / g.pps;
-            if (row % g.pps == 0) {
-                const uint32_t so = P.offset - 6;
-                o[so] = 0xFF;
-                o[so + 1] = 0x20;
-                o[so + 2] = 0x00;
-                o[so + 3] = 0x04;
-                o[so + 4] = (uint8_t)(s >> 8);
-                o[so + 5] = (uint8_t)s;
+        uint32_t nb = 0;
+        if (kind != ITEM_DATA) {
+            uint64_t v = 0;
+#pragma unroll
+            for (uint32_t k = 0; k < IPL; k++) {
+                if (i0 + k >= nitems)
+                    break;
+                uint32_t len, code;
+                if (kind == ITEM_RAW) {
+                    len = 4;
+                    code = a[k];
+                } else if (kind == ITEM_SIG) {
+                    len = 1;
+                    code = a[k] <= gt ? 1u : 0u;
+                } else if (use_sig && s8[k] <= gt) {
+                    len = 0; /* group inside an insignificant significance group */
+                    code = 0;
+                } else {
+                    const uint32_t m = a[k] > gt ? a[k] - gt : 0u; /* unary: m ones and a zero */
+                    len = m + 1;
+                    code = ((1u << m) - 1) << 1;
+                }
+                v = (v << len) | code;
+                nb += len;
             }
-            const uint32_t lprc = P.total - g.prec_hdr_bytes;
-            uint8_t* h = o + P.offset;
-            for (uint32_t k = 0; k < g.prec_hdr_bytes; k++)
-                h[k] = 0;
-            h[0] = (uint8_t)(lprc >> 16);
-            h[1] = (uint8_t)(lprc >> 8);
-            h[2] = (uint8_t)lprc;
-            h[3] = (uint8_t)P.q;
-            h[4] = (uint8_t)P.r;
-            uint32_t bit = 40;
-            for (uint32_t gi = 0; gi < g.nb_all; gi++) {
-                if (g.gb[gi] == 0xFF)
-                    continue;
-                const uint32_t c = g.gc[gi], b = g.gb[gi];
-                const uint32_t dmode = (g.c[c].b[b].lines[type] > 0 && C.msig[c][b]) ? 2u : 0u;
-                h[bit >> 3] |= (uint8_t)(dmode << (6 - (bit & 7)));
-                bit += 2;
-            }
-            /* zero padding closing the precinct */
-            for (uint32_t k = 0; k < P.pad; k++)
-                o[P.offset + P.total - P.pad + k] = 0;
-            if (row + 1 == g.rows) {
-                const uint32_t e = P.offset + P.total;
-                o[e] = 0xFF;
-                o[e + 1] = 0x11;
-            }
-        }
-    }
-    if (p == 0 && row == 0) {
-        for (uint32_t k = threadIdx.x; k < g.hdr_len; k += blockDim.x)
-            F.out[k] = g.hdr[k];
-    }
-    for (uint32_t k = threadIdx.x; k < RING; k += blockDim.x)
-        ring[k] = 0;
-    __syncthreads();
-    if (!C.pk_any[p])
-        return;
-    /* the sizes above reproduce the rate-control table exactly; a packet that
-     * would not fit its precinct is an internal error, never written */
-    if ((uint64_t)C.pk_off[p] + g.pkt_hdr_bytes + C.pk_s[p] + C.pk_g[p] + C.pk_d[p] > (uint64_t)P.offset + P.total ||
-        (uint64_t)P.offset + P.total > g.lcod) {
-        if (threadIdx.x == 0)
-            atomicOr(F.status, 2u);
-        return;
-    }
-    uint8_t* o = F.out + C.pk_off[p];
-    const uint32_t l = g.pk_line[p];
-    if (threadIdx.x == 0) {
-        /* packet header (PackHeaders.c:98-127), sign size 0 */
-        const uint32_t raw = C.pk_raw[p], d = C.pk_d[p], gb = C.pk_g[p];
-        if (g.short_hdr) {
-            o[0] = (uint8_t)((raw << 7) | ((d >> 8) & 0x7F));
-            o[1] = (uint8_t)d;
-            o[2] = (uint8_t)(gb >> 5);
-            o[3] = (uint8_t)((gb & 0x1F) << 3);
-            o[4] = 0;
+            const uint32_t incl = wave_incl_scan(nb, lane);
+            if (nb && v)
+                ring_put(W.ring, W.pos + incl - nb, v, nb);
+            W.pos += wave_last(incl);
         } else {
-            o[0] = (uint8_t)((raw << 7) | ((d >> 13) & 0x7F));
-            o[1] = (uint8_t)(d >> 5);
-            o[2] = (uint8_t)(((d & 0x1F) << 3) | ((gb >> 17) & 7));
-            o[3] = (uint8_t)(gb >> 9);
-            o[4] = (uint8_t)(gb >> 1);
-            o[5] = (uint8_t)((gb & 1) << 7);
-            o[6] = 0;
-        }
-    }
-    o += g.pkt_hdr_bytes;
-    /* sub-packets in order: significance, bit-plane counts, data */
-    for (int sub = 0; sub < 3; sub++) {
-        uint32_t nbytes;
-        if (sub == 0) {
-            if (C.pk_raw[p])
-                continue;
-            nbytes = C.pk_s[p];
-        } else if (sub == 1) {
-            nbytes = C.pk_g[p];
-        } else {
-            nbytes = C.pk_d[p];
-        }
-        uint32_t flushed = 0, pos = 0;
-        for (uint32_t gi = g.pk_bs[p]; gi < g.pk_be[p]; gi++) {
-            const uint32_t c = g.gc[gi], b = g.gb[gi];
-            const JxsBand& bd = g.c[c].b[b];
-            if (l >= bd.lines[type])
-                continue;
-            const uint32_t y = row * bd.hl + l;
-            const uint8_t* gl = F.gcli + bd.gcli_off + (size_t)y * bd.gw;
-            const uint8_t* sl = F.smax + bd.smax_off + (size_t)y * bd.sw;
-            const uint16_t* cl = F.coeff + bd.coeff_off + (size_t)y * bd.w;
-            const uint32_t gt = C.gtli[c][b], ms = C.msig[c][b];
-            int kind;
-            if (sub == 0) {
-                if (!ms)
-                    continue;
-                kind = ITEM_SIG;
-            } else if (sub == 1) {
-                kind = C.pk_raw[p] ? ITEM_RAW : ITEM_VLC;
-            } else {
-                kind = ITEM_DATA;
+            uint64_t v[IPL];
+            uint32_t len[IPL];
+#pragma unroll
+            for (uint32_t k = 0; k < IPL; k++) {
+                const uint32_t i = i0 + k, gc = a[k];
+                const bool coded = i < nitems && gc > gt;
+                const uint32_t np = coded ? gc - gt : 0u;
+                const uint32_t c[4] = {cv[k].x & 0xFFFFu, cv[k].x >> 16, cv[k].y & 0xFFFFu, cv[k].y >> 16};
+                uint32_t q[4], signs = 0;
+#pragma unroll
+                for (uint32_t e = 0; e < 4; e++) {
+                    q[e] = (coded && 4 * i + e < w) ? quant_mag(c[e], gc, gt, qpih) : 0u;
+                    signs |= (q[e] && (c[e] & 0x8000u)) ? (8u >> e) : 0u;
+                }
+                const bool any_high = __ballot(np > 8) != 0;
+                const uint64_t planes = interleave4(q[0], q[1], q[2], q[3], any_high);
+                len[k] = coded ? 4 * np + 4 : 0u;
+                v[k] = coded ? ((uint64_t)signs << (4 * np)) | planes : 0u;
+                nb += len[k];
             }
-            pos += emit_line(ring, C, o, nbytes, flushed, pos, kind, gl, sl, cl, bd.w, bd.gw, bd.sw, gt, ms, g.qpih);
+            const uint32_t incl = wave_incl_scan(nb, lane);
+            uint32_t at = W.pos + incl - nb;
+#pragma unroll
+            for (uint32_t k = 0; k < IPL; k++) {
+                if (len[k] && v[k])
+                    ring_put(W.ring, at, v[k], len[k]);
+                at += len[k];
+            }
+            W.pos += wave_last(incl);
         }
-        ring_flush(ring, o, nbytes, flushed, (nbytes + 3) / 4);
-        o += nbytes;
+        if ((W.pos >> 5) - W.flushed >= RING_FLUSH) {
+            wave_sync();
+            ring_flush(W, W.flushed, W.pos >> 5);
+            W.flushed = W.pos >> 5;
+            wave_sync();
+        }
     }
+}
+
+/* One wave per packing unit of a precinct: a packet header
+ * (PackHeaders.c:98-127, sign size 0) or the items of one band line in one
+ * sub-packet -- significance flags, bit-plane counts (unary or raw) or data
+ * bit-planes with the quantisation (Quant.c:9-45) fused
+ * (PackPrecinct.c:235-572).  The packet plan gives every unit its first bit
+ * and length, so units are independent: each is assembled in its wave's LDS
+ * ring and written out as it completes; waves never wait on each other. */
+__global__ void __launch_bounds__(64 * PK_WAVES) k_pack(const JxsFrameDesc* __restrict__ frames, int nfr,
+                                                        uint32_t total_units) {
+    __shared__ uint32_t rings[PK_WAVES][RINGW];
+    const uint32_t wid = threadIdx.x / WAVE, lane = threadIdx.x & (WAVE - 1);
+    const uint32_t gu = blockIdx.x * PK_WAVES + wid;
+    if (gu >= total_units)
+        return;
+    int lo = 0, hi = nfr - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (frames[mid].unit_base <= gu)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    /* descriptors, geometry and plan are read-only here: read them through
+     * the constant address space (scalar loads, not re-read after stores) */
+    const CAS JxsFrameDesc& F = ((const CAS JxsFrameDesc*)frames)[lo];
+    const CAS JxsGeom& g = *(const CAS JxsGeom*)F.g;
+    const uint32_t row = (gu - F.unit_base) / F.n_units, u = (gu - F.unit_base) % F.n_units;
+    const CAS JxsPlan& PL = ((const CAS JxsPlan*)F.plan)[row];
+    const uint32_t len = PL.ulen[u];
+    if (len == 0)
+        return;
+    const uint32_t p = g.u_p[u], kind = g.u_kind[u];
+    const uint64_t first = 8 * (uint64_t)((const CAS JxsPrec*)F.prec)[row].offset + PL.ubit[u];
+    WaveOut W;
+    W.ring = rings[wid];
+    W.ob = (uint32_t*)F.out + (first >> 5);
+    W.lo = (uint32_t)(first & 31);
+    W.hi = W.lo + len;
+    W.pos = W.lo;
+    W.flushed = 0;
+    W.lane = lane;
+    const uint32_t words = min((W.hi + 31) / 32, (uint32_t)RINGW);
+    for (uint32_t k = lane; k < words; k += WAVE)
+        W.ring[k] = 0;
+    wave_sync();
+    const uint32_t raw = (PL.pk[p].flags & PKT_RAW) ? 1u : 0u;
+    if (kind == UNIT_HDR) {
+        const uint32_t d = PL.pk[p].d, gb = PL.pk[p].g;
+        if (lane == 0) {
+            if (g.short_hdr)
+                ring_put(W.ring, W.pos, ((uint64_t)raw << 39) | ((uint64_t)d << 24) | ((uint64_t)gb << 11), 40);
+            else
+                ring_put(W.ring, W.pos, ((uint64_t)raw << 55) | ((uint64_t)d << 35) | ((uint64_t)gb << 15), 56);
+        }
+        W.pos += 8 * g.pkt_hdr_bytes;
+    } else {
+        const uint32_t gi = g.u_gi[u];
+        const CAS JxsBand& bd = g.c[g.gc[gi]].b[g.gb[gi]];
+        const uint32_t gt = PL.band[gi] & 15u, ms = PL.band[gi] >> 4;
+        const int item = kind == UNIT_SIG ? ITEM_SIG : kind == UNIT_DATA ? ITEM_DATA : raw ? ITEM_RAW : ITEM_VLC;
+        const uint32_t y = row * bd.hl + g.pk_line[p];
+        emit_line(W, item, F.gcli + bd.gcli_off + (size_t)y * bd.gw, F.smax + bd.smax_off + (size_t)y * bd.sw,
+                  F.coeff + bd.coeff_off + (size_t)y * bd.cs, bd.w, bd.gw, bd.sw, gt, ms, g.qpih);
+    }
+    wave_sync();
+    ring_flush(W, W.flushed, (W.hi + 31) / 32);
+    /* the plan's length is the unit's exact bit count */
+    if (W.pos != W.hi && lane == 0)
+        atomicOr(F.status, 2u);
 }
 
 /* ------------------------------------------------------------------ */
@@ -812,17 +1273,14 @@ __global__ void __launch_bounds__(256) k_pack(const JxsFrameDesc* __restrict__ f
 /* ------------------------------------------------------------------ */
 extern "C" {
 
-hipError_t jxs_launch_vert(const JxsVertJob* jobs, const uint32_t* bases, int njobs, uint32_t blocks, hipStream_t st) {
+hipError_t jxs_launch_dwt(const JxsDwtJob* jobs, const uint32_t* bases, int njobs, uint32_t blocks, uint32_t wmax,
+                          hipStream_t st) {
     if (!njobs || !blocks)
         return hipSuccess;
-    hipLaunchKernelGGL(k_vert, dim3(blocks), dim3(256), 0, st, jobs, bases, njobs);
-    return hipGetLastError();
-}
-
-hipError_t jxs_launch_horz(const JxsHorzJob* jobs, const uint32_t* bases, int njobs, uint32_t blocks, hipStream_t st) {
-    if (!njobs || !blocks)
-        return hipSuccess;
-    hipLaunchKernelGGL(k_horz, dim3(blocks), dim3(256), 0, st, jobs, bases, njobs);
+    if (wmax == 0 || wmax > HMAX)
+        return hipErrorInvalidValue;
+    const size_t lds = 2 * (size_t)((wmax + 3) & ~3u) * sizeof(int32_t);
+    hipLaunchKernelGGL(k_dwt, dim3(blocks), dim3(256), lds, st, jobs, bases, njobs, blocks, wmax);
     return hipGetLastError();
 }
 
@@ -832,12 +1290,19 @@ hipError_t jxs_launch_rc_table(const JxsFrameDesc* frames, int nfr, uint32_t pre
 }
 
 hipError_t jxs_launch_select(const JxsFrameDesc* frames, int nfr, uint32_t slices, hipStream_t st) {
-    hipLaunchKernelGGL(k_select, dim3((slices + 63) / 64), dim3(64), 0, st, frames, nfr, slices);
+    hipLaunchKernelGGL(k_select, dim3(slices), dim3(SEL_THREADS), 0, st, frames, nfr, slices);
     return hipGetLastError();
 }
 
-hipError_t jxs_launch_pack(const JxsFrameDesc* frames, int nfr, uint32_t max_npk, uint32_t precincts, hipStream_t st) {
-    hipLaunchKernelGGL(k_pack, dim3(max_npk, precincts), dim3(256), 0, st, frames, nfr);
+hipError_t jxs_launch_plan(const JxsFrameDesc* frames, int nfr, uint32_t precincts, hipStream_t st) {
+    hipLaunchKernelGGL(k_plan, dim3(precincts), dim3(64), 0, st, frames, nfr);
+    return hipGetLastError();
+}
+
+hipError_t jxs_launch_pack(const JxsFrameDesc* frames, int nfr, uint32_t units, hipStream_t st) {
+    if (!units)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_pack, dim3((units + PK_WAVES - 1) / PK_WAVES), dim3(64 * PK_WAVES), 0, st, frames, nfr, units);
     return hipGetLastError();
 }
 

@@ -78,7 +78,7 @@ def test_device_roundtrip_packed_rgb14_v1():
     enc = P.Encoder(W, H, D, F, 2, 1, 1, 5)
     dev = torch.device("cuda:0")
     d_in = [torch.from_numpy(f[0].view(np.int16)).to(dev) for f in frames]
-    d_cs = torch.zeros((3, enc.bytes_per_frame), dtype=torch.uint8, device=dev)
+    d_cs = torch.zeros((3, (enc.bytes_per_frame + 3) & ~3), dtype=torch.uint8, device=dev)
     plan = P.Plan([(enc, [d_in[i].data_ptr()], [W * 3], d_cs[i].data_ptr()) for i in range(3)])
     plan.run()
     torch.cuda.synchronize()
