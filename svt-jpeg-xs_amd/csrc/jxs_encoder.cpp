@@ -23,7 +23,7 @@ hipError_t jxs_launch_dwt(const JxsDwtJob*, const uint32_t*, int, uint32_t, uint
 hipError_t jxs_launch_rc_table(const JxsFrameDesc*, int, uint32_t, hipStream_t);
 hipError_t jxs_launch_select(const JxsFrameDesc*, int, uint32_t, hipStream_t);
 hipError_t jxs_launch_plan(const JxsFrameDesc*, int, uint32_t, hipStream_t);
-hipError_t jxs_launch_pack(const JxsFrameDesc*, int, uint32_t, hipStream_t);
+hipError_t jxs_launch_pack(const JxsUnit*, uint32_t, hipStream_t);
 hipError_t jxs_launch_zero(uint32_t*, uint32_t, hipStream_t);
 }
 
@@ -93,6 +93,7 @@ struct svt_jpeg_xs_amd_plan {
     void* d_work = nullptr;
     uint32_t* d_status = nullptr;
     JxsFrameDesc* d_frames = nullptr;
+    JxsUnit* d_units = nullptr; /* packing units of every frame, frame after frame */
     JxsDwtJob* d_dwt[2] = {nullptr, nullptr};
     uint32_t* d_dbase[2] = {nullptr, nullptr};
     int ndwt[2] = {0, 0};
@@ -142,6 +143,7 @@ static void plan_free(svt_jpeg_xs_amd_plan_t* p) {
     (void)hipFree(p->d_work);
     (void)hipFree(p->d_status);
     (void)hipFree(p->d_frames);
+    (void)hipFree(p->d_units);
     for (int s = 0; s < 2; s++) {
         (void)hipFree(p->d_dwt[s]);
         (void)hipFree(p->d_dbase[s]);
@@ -231,6 +233,12 @@ static svt_jpeg_xs_amd_plan_t* plan_build(const svt_jpeg_xs_amd_frame_spec_t* sp
     p->total_prec = prec_base;
     p->total_slices = slice_base;
     p->total_units = unit_base;
+    if (hipMalloc(&p->d_units, (size_t)(unit_base ? unit_base : 1) * sizeof(JxsUnit)) != hipSuccess) {
+        plan_free(p);
+        return nullptr;
+    }
+    for (uint32_t i = 0; i < count; i++)
+        frames[i].units = p->d_units + frames[i].unit_base;
     bool ok = upload_vec(frames, &p->d_frames) == hipSuccess;
     for (int s = 0; s < 2 && ok; s++) {
         std::vector<uint32_t> bases;
@@ -272,7 +280,7 @@ static hipError_t plan_launch(svt_jpeg_xs_amd_plan_t* p, hipStream_t st, hipEven
     STEP(2, jxs_launch_rc_table(p->d_frames, nf, p->total_prec, st));
     STEP(3, jxs_launch_select(p->d_frames, nf, p->total_slices, st));
     STEP(4, jxs_launch_plan(p->d_frames, nf, p->total_prec, st));
-    STEP(5, jxs_launch_pack(p->d_frames, nf, p->total_units, st));
+    STEP(5, jxs_launch_pack(p->d_units, p->total_units, st));
     if (ev)
         (void)hipEventRecord(ev[SVT_JPEG_XS_AMD_STAGES], st);
 #undef STEP
@@ -613,7 +621,8 @@ PREFIX_API uint64_t svt_jpeg_xs_amd_plan_footprint(const svt_jpeg_xs_amd_frame_s
     for (uint32_t i = 0; i < count; i++) {
         EncoderPriv* e = frames[i].encoder ? priv_of(frames[i].encoder) : nullptr;
         if (e)
-            b += frame_work_bytes(e->geom.host.g);
+            b += frame_work_bytes(e->geom.host.g) +
+                (uint64_t)e->geom.host.g.rows * e->geom.host.g.n_units * sizeof(JxsUnit);
     }
     return b;
 }

@@ -98,8 +98,26 @@ struct JxsPkt {
 struct JxsPlan {
     JxsPkt pk[JXS_NPK];
     uint8_t band[JXS_NGB]; /* per global band: truncation | (significance coding << 4) */
-    uint32_t ubit[JXS_NU]; /* first bit of each packing unit, from the precinct's first byte */
-    uint32_t ulen[JXS_NU]; /* bits of each packing unit */
+};
+
+/* item streams of the packing units */
+enum { ITEM_RAW = 0, ITEM_SIG = 1, ITEM_VLC = 2, ITEM_DATA = 3, ITEM_HDR = 4 };
+
+/* One packing unit, self-contained (written by the packet plan, read by the
+ * packing kernel with a single load): where its bits go and what produces
+ * them. */
+struct JxsUnit {
+    const uint8_t* gl;  /* GCLI row of the band line */
+    const uint8_t* sl;  /* significance-maximum row */
+    const uint16_t* cl; /* coefficient row */
+    uint32_t* ob;       /* output dword holding the unit's first bit */
+    uint32_t* status;   /* the frame's error flags */
+    uint64_t hdr;       /* packet header bits (ITEM_HDR) */
+    uint32_t lo, len;   /* first bit inside *ob (0..31), number of bits (0: nothing) */
+    uint32_t item;      /* ITEM_* */
+    uint32_t gt, ms, qpih;
+    uint32_t w, gw, sw;
+    uint32_t pad_[3];
 };
 
 struct JxsBandOut {
@@ -143,6 +161,7 @@ struct JxsFrameDesc {
     uint32_t* table; /* rows x JXS_NQ x JXS_NR */
     JxsPrec* prec;   /* rows */
     JxsPlan* plan;   /* rows */
+    JxsUnit* units;  /* rows x n_units packing units */
     uint8_t* out;    /* codestream */
     uint32_t* status;
     uint32_t rows, npk;

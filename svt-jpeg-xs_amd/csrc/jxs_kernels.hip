@@ -827,7 +827,9 @@ __device__ __forceinline__ uint32_t wave_last(uint32_t v) {
  * control chose is flagged and nothing of it is written. */
 __global__ void __launch_bounds__(64) k_plan(const JxsFrameDesc* __restrict__ frames, int nfr) {
     __shared__ uint32_t bd_d[JXS_NBL], bd_g[JXS_NBL], bd_s[JXS_NBL];
-    __shared__ uint8_t b_ms[JXS_NGB];
+    __shared__ uint8_t b_ms[JXS_NGB], b_gt[JXS_NGB];
+    __shared__ uint32_t k_rel[JXS_NPK], k_sb[JXS_NPK], k_gb[JXS_NPK], k_db[JXS_NPK], k_fl[JXS_NPK];
+    __shared__ uint32_t u_bits[JXS_NU];
     const int f = find_frame(frames, nfr, blockIdx.x);
     const JxsFrameDesc& F = frames[f];
     __shared__ __align__(16) JxsGeom sg;
@@ -844,7 +846,7 @@ __global__ void __launch_bounds__(64) k_plan(const JxsFrameDesc* __restrict__ fr
         if (lane < g.npk)
             PL.pk[lane].flags = 0;
         for (uint32_t u = lane; u < g.n_units; u += WAVE)
-            PL.ulen[u] = 0;
+            F.units[(size_t)row * g.n_units + u].len = 0;
         return;
     }
     const uint16_t* recs = F.hist + (size_t)row * g.n_bl * JXS_HREC;
@@ -875,6 +877,7 @@ __global__ void __launch_bounds__(64) k_plan(const JxsFrameDesc* __restrict__ fr
             }
         }
         b_ms[gi] = (uint8_t)ms;
+        b_gt[gi] = (uint8_t)gt;
         PL.band[gi] = (uint8_t)(gt | (ms << 4));
     }
     __syncthreads();
@@ -914,7 +917,7 @@ __global__ void __launch_bounds__(64) k_plan(const JxsFrameDesc* __restrict__ fr
         if (lane < g.npk)
             PL.pk[lane].flags = 0;
         for (uint32_t u = lane; u < g.n_units; u += WAVE)
-            PL.ulen[u] = 0;
+            F.units[(size_t)row * g.n_units + u].len = 0;
         if (lane == 0)
             atomicOr(F.status, 2u);
         return;
@@ -929,35 +932,78 @@ __global__ void __launch_bounds__(64) k_plan(const JxsFrameDesc* __restrict__ fr
         K.flags = flags;
         PL.pk[lane] = K;
     }
-    /* packing units: lane p walks the units of packet p (header, then one
-     * per band of each sub-packet) and places them bit by bit */
-    const uint32_t nu_p = lane < g.npk ? 1 + 3 * (g.pk_be[lane] - g.pk_bs[lane]) : 0u;
-    uint32_t u = wave_incl_scan(nu_p, lane) - nu_p;
+    /* packing units, one lane each: first the bits of every unit, then its
+     * place after the earlier units of the same sub-packet */
     if (lane < g.npk) {
-        const uint32_t p = lane, l = g.pk_line[p];
-        const bool raw = (flags & PKT_RAW) != 0, any = (flags & PKT_ANY) != 0;
-        PL.ubit[u] = 8 * rel;
-        PL.ulen[u] = any ? 8 * g.pkt_hdr_bytes : 0u;
-        u++;
-        for (uint32_t kind = UNIT_SIG; kind <= UNIT_DATA; kind++) {
-            uint32_t cur = 8 * (rel + g.pkt_hdr_bytes + (kind >= UNIT_GCLI ? sb : 0u) + (kind == UNIT_DATA ? gbb : 0u));
-            for (uint32_t gi = g.pk_bs[p]; gi < g.pk_be[p]; gi++) {
-                const JxsBand& bd = g.c[g.gc[gi]].b[g.gb[gi]];
-                uint32_t bits = 0;
-                if (l < bd.lines[type]) {
-                    const uint32_t bl = bd.bl + l;
-                    if (kind == UNIT_SIG)
-                        bits = raw ? 0u : bd_s[bl];
-                    else if (kind == UNIT_GCLI)
-                        bits = raw ? 4 * bd.gw : bd_g[bl];
-                    else
-                        bits = bd_d[bl];
-                }
-                PL.ubit[u] = cur;
-                PL.ulen[u] = bits;
-                cur += bits;
-                u++;
+        k_rel[lane] = rel;
+        k_sb[lane] = sb;
+        k_gb[lane] = gbb;
+        k_db[lane] = db;
+        k_fl[lane] = flags;
+    }
+    __syncthreads();
+    for (uint32_t u = lane; u < g.n_units; u += WAVE) {
+        const uint32_t p = g.u_p[u], kind = g.u_kind[u];
+        uint32_t bits = 0;
+        if (kind == UNIT_HDR) {
+            bits = (k_fl[p] & PKT_ANY) ? 8 * g.pkt_hdr_bytes : 0u;
+        } else {
+            const uint32_t gi = g.u_gi[u], l = g.pk_line[p];
+            const JxsBand& bd = g.c[g.gc[gi]].b[g.gb[gi]];
+            const bool raw = (k_fl[p] & PKT_RAW) != 0;
+            if (l < bd.lines[type]) {
+                const uint32_t bl = bd.bl + l;
+                if (kind == UNIT_SIG)
+                    bits = raw ? 0u : bd_s[bl];
+                else if (kind == UNIT_GCLI)
+                    bits = raw ? 4 * bd.gw : bd_g[bl];
+                else
+                    bits = bd_d[bl];
             }
+        }
+        u_bits[u] = bits;
+    }
+    __syncthreads();
+    {
+        const uint64_t pbit = 8 * (uint64_t)P.offset; /* precinct's first bit in the codestream */
+        JxsUnit* UR = F.units + (size_t)row * g.n_units;
+        for (uint32_t u = lane; u < g.n_units; u += WAVE) {
+            const uint32_t p = g.u_p[u], kind = g.u_kind[u];
+            const bool raw = (k_fl[p] & PKT_RAW) != 0;
+            JxsUnit U = {};
+            uint32_t bit = 8 * k_rel[p];
+            if (kind == UNIT_HDR) {
+                /* packet header (PackHeaders.c:98-127): RAW flag, data
+                 * bytes, bit-plane-count bytes, sign bytes (0) */
+                U.item = ITEM_HDR;
+                U.hdr = g.short_hdr ? ((uint64_t)raw << 39) | ((uint64_t)k_db[p] << 24) | ((uint64_t)k_gb[p] << 11)
+                                    : ((uint64_t)raw << 55) | ((uint64_t)k_db[p] << 35) | ((uint64_t)k_gb[p] << 15);
+            } else {
+                const uint32_t gi = g.u_gi[u], l = g.pk_line[p], j = gi - g.pk_bs[p];
+                const JxsBand& bd = g.c[g.gc[gi]].b[g.gb[gi]];
+                bit += 8 * (g.pkt_hdr_bytes + (kind >= UNIT_GCLI ? k_sb[p] : 0u) + (kind == UNIT_DATA ? k_gb[p] : 0u));
+                for (uint32_t k = 0; k < j; k++)
+                    bit += u_bits[u - j + k];
+                if (u_bits[u]) {
+                    const uint32_t y = row * bd.hl + l;
+                    U.gl = F.gcli + bd.gcli_off + (size_t)y * bd.gw;
+                    U.sl = F.smax + bd.smax_off + (size_t)y * bd.sw;
+                    U.cl = F.coeff + bd.coeff_off + (size_t)y * bd.cs;
+                    U.item = kind == UNIT_SIG ? ITEM_SIG : kind == UNIT_DATA ? ITEM_DATA : raw ? ITEM_RAW : ITEM_VLC;
+                    U.gt = b_gt[gi];
+                    U.ms = b_ms[gi];
+                    U.qpih = g.qpih;
+                    U.w = bd.w;
+                    U.gw = bd.gw;
+                    U.sw = bd.sw;
+                }
+            }
+            const uint64_t first = pbit + bit;
+            U.ob = (uint32_t*)F.out + (first >> 5);
+            U.lo = (uint32_t)(first & 31);
+            U.len = u_bits[u];
+            U.status = F.status;
+            UR[u] = U;
         }
     }
     uint8_t* o = F.out;
@@ -1043,8 +1089,6 @@ __device__ __forceinline__ void ring_put(uint32_t* ring, uint32_t pos, uint64_t 
         atomicOr(&ring[(wd + 2) & (RINGW - 1)], w2);
 }
 
-/* kinds of item streams */
-enum { ITEM_RAW = 0, ITEM_SIG = 1, ITEM_VLC = 2, ITEM_DATA = 3 };
 
 /* bit i of x (x < 2^8) -> bit 4i */
 __device__ __forceinline__ uint32_t spread8(uint32_t x) {
@@ -1197,43 +1241,27 @@ __device__ __forceinline__ void emit_line(WaveOut& W, int kind, const uint8_t* g
     }
 }
 
-/* One wave per packing unit of a precinct: a packet header
- * (PackHeaders.c:98-127, sign size 0) or the items of one band line in one
- * sub-packet -- significance flags, bit-plane counts (unary or raw) or data
- * bit-planes with the quantisation (Quant.c:9-45) fused
- * (PackPrecinct.c:235-572).  The packet plan gives every unit its first bit
- * and length, so units are independent: each is assembled in its wave's LDS
- * ring and written out as it completes; waves never wait on each other. */
-__global__ void __launch_bounds__(64 * PK_WAVES) k_pack(const JxsFrameDesc* __restrict__ frames, int nfr,
-                                                        uint32_t total_units) {
+/* One wave per packing unit: a packet header (PackHeaders.c:98-127, sign
+ * size 0) or the items of one band line in one sub-packet -- significance
+ * flags, bit-plane counts (unary or raw) or data bit-planes with the
+ * quantisation (Quant.c:9-45) fused (PackPrecinct.c:235-572).  Every unit is
+ * described by one self-contained record of the packet plan, so a wave makes
+ * one descriptor load, assembles its bits in its LDS ring and writes them out
+ * as they complete; waves never wait on each other. */
+__global__ void __launch_bounds__(64 * PK_WAVES) k_pack(const JxsUnit* __restrict__ units, uint32_t total_units) {
     __shared__ uint32_t rings[PK_WAVES][RINGW];
     const uint32_t wid = threadIdx.x / WAVE, lane = threadIdx.x & (WAVE - 1);
     const uint32_t gu = blockIdx.x * PK_WAVES + wid;
     if (gu >= total_units)
         return;
-    int lo = 0, hi = nfr - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (frames[mid].unit_base <= gu)
-            lo = mid;
-        else
-            hi = mid - 1;
-    }
-    /* descriptors, geometry and plan are read-only here: read them through
-     * the constant address space (scalar loads, not re-read after stores) */
-    const CAS JxsFrameDesc& F = ((const CAS JxsFrameDesc*)frames)[lo];
-    const CAS JxsGeom& g = *(const CAS JxsGeom*)F.g;
-    const uint32_t row = (gu - F.unit_base) / F.n_units, u = (gu - F.unit_base) % F.n_units;
-    const CAS JxsPlan& PL = ((const CAS JxsPlan*)F.plan)[row];
-    const uint32_t len = PL.ulen[u];
+    const CAS JxsUnit& U = ((const CAS JxsUnit*)units)[gu];
+    const uint32_t len = U.len;
     if (len == 0)
         return;
-    const uint32_t p = g.u_p[u], kind = g.u_kind[u];
-    const uint64_t first = 8 * (uint64_t)((const CAS JxsPrec*)F.prec)[row].offset + PL.ubit[u];
     WaveOut W;
     W.ring = rings[wid];
-    W.ob = (uint32_t*)F.out + (first >> 5);
-    W.lo = (uint32_t)(first & 31);
+    W.ob = U.ob;
+    W.lo = U.lo;
     W.hi = W.lo + len;
     W.pos = W.lo;
     W.flushed = 0;
@@ -1242,30 +1270,19 @@ __global__ void __launch_bounds__(64 * PK_WAVES) k_pack(const JxsFrameDesc* __re
     for (uint32_t k = lane; k < words; k += WAVE)
         W.ring[k] = 0;
     wave_sync();
-    const uint32_t raw = (PL.pk[p].flags & PKT_RAW) ? 1u : 0u;
-    if (kind == UNIT_HDR) {
-        const uint32_t d = PL.pk[p].d, gb = PL.pk[p].g;
-        if (lane == 0) {
-            if (g.short_hdr)
-                ring_put(W.ring, W.pos, ((uint64_t)raw << 39) | ((uint64_t)d << 24) | ((uint64_t)gb << 11), 40);
-            else
-                ring_put(W.ring, W.pos, ((uint64_t)raw << 55) | ((uint64_t)d << 35) | ((uint64_t)gb << 15), 56);
-        }
-        W.pos += 8 * g.pkt_hdr_bytes;
+    const uint32_t item = U.item;
+    if (item == ITEM_HDR) {
+        if (lane == 0)
+            ring_put(W.ring, W.pos, U.hdr, len);
+        W.pos += len;
     } else {
-        const uint32_t gi = g.u_gi[u];
-        const CAS JxsBand& bd = g.c[g.gc[gi]].b[g.gb[gi]];
-        const uint32_t gt = PL.band[gi] & 15u, ms = PL.band[gi] >> 4;
-        const int item = kind == UNIT_SIG ? ITEM_SIG : kind == UNIT_DATA ? ITEM_DATA : raw ? ITEM_RAW : ITEM_VLC;
-        const uint32_t y = row * bd.hl + g.pk_line[p];
-        emit_line(W, item, F.gcli + bd.gcli_off + (size_t)y * bd.gw, F.smax + bd.smax_off + (size_t)y * bd.sw,
-                  F.coeff + bd.coeff_off + (size_t)y * bd.cs, bd.w, bd.gw, bd.sw, gt, ms, g.qpih);
+        emit_line(W, (int)item, U.gl, U.sl, U.cl, U.w, U.gw, U.sw, U.gt, U.ms, U.qpih);
     }
     wave_sync();
     ring_flush(W, W.flushed, (W.hi + 31) / 32);
     /* the plan's length is the unit's exact bit count */
     if (W.pos != W.hi && lane == 0)
-        atomicOr(F.status, 2u);
+        atomicOr(U.status, 2u);
 }
 
 /* ------------------------------------------------------------------ */
@@ -1299,10 +1316,10 @@ hipError_t jxs_launch_plan(const JxsFrameDesc* frames, int nfr, uint32_t precinc
     return hipGetLastError();
 }
 
-hipError_t jxs_launch_pack(const JxsFrameDesc* frames, int nfr, uint32_t units, hipStream_t st) {
-    if (!units)
+hipError_t jxs_launch_pack(const JxsUnit* units, uint32_t n, hipStream_t st) {
+    if (!n)
         return hipSuccess;
-    hipLaunchKernelGGL(k_pack, dim3((units + PK_WAVES - 1) / PK_WAVES), dim3(64 * PK_WAVES), 0, st, frames, nfr, units);
+    hipLaunchKernelGGL(k_pack, dim3((n + PK_WAVES - 1) / PK_WAVES), dim3(64 * PK_WAVES), 0, st, units, n);
     return hipGetLastError();
 }
 
