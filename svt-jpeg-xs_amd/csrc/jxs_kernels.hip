@@ -74,13 +74,6 @@ __device__ __forceinline__ int32_t load_src(const void* src, uint32_t kind, size
     return (int32_t)((v & mask) << shift) - offset;
 }
 
-__device__ __forceinline__ uint16_t to_sign_mag(int32_t v) {
-    if (v >= 0)
-        return (uint16_t)((v + 128) >> 8);
-    const uint32_t m = (uint32_t)((-v + 128) >> 8);
-    return (uint16_t)(m ? (m | 0x8000u) : 0u);
-}
-
 __device__ __forceinline__ uint32_t bitlen(uint32_t v) { return v ? 32u - __clz(v) : 0u; }
 
 __device__ __forceinline__ uint32_t truncation(uint32_t gain, uint32_t prio, uint32_t q, uint32_t r) {
@@ -142,74 +135,72 @@ __device__ __forceinline__ uint32_t xcd_remap(uint32_t b, uint32_t n) {
 
 #define HMAX JXS_MAX_ROW
 
-/* Count of the lanes whose 4-bit value equals this lane's `want`, among the
- * lanes of `valid`; b0..b3 are the ballots of the value's bits. */
-__device__ __forceinline__ uint32_t ballot_count(uint64_t valid, uint64_t b0, uint64_t b1, uint64_t b2, uint64_t b3,
-                                                 uint32_t want) {
-    uint64_t m = valid;
-    m &= (want & 1u) ? b0 : ~b0;
-    m &= (want & 2u) ? b1 : ~b1;
-    m &= (want & 4u) ? b2 : ~b2;
-    m &= (want & 8u) ? b3 : ~b3;
-    return (uint32_t)__popcll(m);
-}
-
 /* Writes `count` values buf[start + k * step] as one band row (row stride
  * o.cs, the padding of the last group zero) with its GCLIs and significance
  * maxima, and the row's histogram record when o.hist is set.  hcnt is a
  * zeroed LDS array of 32 words; it is zero again on return.  Called by all
- * threads of the block. */
+ * threads of the block.  Straight-line per group: the four LDS reads use
+ * clamped indexes and are issued together, the sign-magnitude conversion
+ * is branch-free, GCLI counts come from wave ballots (lanes 0..15 count one
+ * value each) and significance-maximum counts from LDS atomics (one lane in
+ * eight holds a maximum). */
 __device__ __forceinline__ void emit_band_row(const JxsBandOut o, uint32_t y, const int32_t* buf, uint32_t start,
-                                              uint32_t step,
-                              uint32_t count, uint32_t* hcnt) {
+                                              uint32_t step, uint32_t count, uint32_t* hcnt) {
     uint16_t* crow = o.coeff + (size_t)y * o.cs;
     uint8_t* grow = o.gcli + (size_t)y * o.gw;
     uint8_t* srow = o.smax + (size_t)y * o.sw;
     const uint32_t lane = threadIdx.x & (WAVE - 1);
     const uint32_t full = o.gw / 8;
-    /* lanes 0..15 count GCLI value `lane`, lanes 16..30 full significance
-     * groups of maximum `lane - 16` */
-    const bool count_gcli = lane < 16;
-    const uint32_t want = lane & 15u;
+    const bool hist = o.hist != nullptr;
+    /* per-lane masks: ~(ballot of GCLI bit k XOR mk) has a one for every
+     * lane whose GCLI bit k equals bit k of `lane` */
+    const uint64_t m0 = (lane & 1u) ? ~0ull : 0ull, m1 = (lane & 2u) ? ~0ull : 0ull;
+    const uint64_t m2 = (lane & 4u) ? ~0ull : 0ull, m3 = (lane & 8u) ? ~0ull : 0ull;
     uint32_t mycnt = 0;
     for (uint32_t base = 0; base < o.gw; base += blockDim.x) {
         const uint32_t gidx = base + threadIdx.x;
-        uint32_t orv = 0;
         const bool valid = gidx < o.gw;
-        if (valid) {
-            uint32_t v[4];
+        const uint32_t i0 = 4 * min(gidx, o.gw - 1);
+        int32_t x[4];
 #pragma unroll
-            for (uint32_t k = 0; k < 4; k++) {
-                const uint32_t i = 4 * gidx + k;
-                v[k] = i < count ? to_sign_mag(buf[start + i * step]) : 0u;
-                orv |= v[k] & 0x7FFFu;
-            }
-            gst2(crow + 4 * gidx, make_uint2(v[0] | (v[1] << 16), v[2] | (v[3] << 16)));
-            gst(grow + gidx, (uint8_t)bitlen(orv));
+        for (uint32_t k = 0; k < 4; k++)
+            x[k] = buf[start + min(i0 + k, count - 1) * step];
+        uint32_t v[4], orv = 0;
+#pragma unroll
+        for (uint32_t k = 0; k < 4; k++) {
+            const uint32_t a = (uint32_t)(x[k] < 0 ? -x[k] : x[k]);
+            const uint32_t mg = (a + 128) >> 8;
+            v[k] = (i0 + k < count && mg) ? (mg | (((uint32_t)x[k] >> 31) << 15)) : 0u;
+            orv |= v[k];
         }
-        const uint32_t gc = bitlen(orv);
-        uint32_t m = gc;
+        const uint32_t gc = bitlen(orv & 0x7FFFu);
+        if (valid) {
+            gst2(crow + 4 * gidx, make_uint2(v[0] | (v[1] << 16), v[2] | (v[3] << 16)));
+            gst(grow + gidx, (uint8_t)gc);
+        }
+        uint32_t m = valid ? gc : 0u;
         m = max(m, (uint32_t)__shfl_xor((int)m, 1, WAVE));
         m = max(m, (uint32_t)__shfl_xor((int)m, 2, WAVE));
         m = max(m, (uint32_t)__shfl_xor((int)m, 4, WAVE));
-        const bool head = (gidx & 7) == 0 && gidx / 8 < o.sw;
+        const bool head = valid && (gidx & 7) == 0;
         if (head)
             gst(srow + gidx / 8, (uint8_t)m);
-        if (o.hist) {
-            const bool sfull = head && gidx / 8 < full;
-            const uint64_t vg = __ballot(valid), vs = __ballot(sfull);
-            const uint64_t g0 = __ballot(valid && (gc & 1u)), g1 = __ballot(valid && (gc & 2u)),
-                           g2 = __ballot(valid && (gc & 4u)), g3 = __ballot(valid && (gc & 8u));
-            const uint64_t s0 = __ballot(sfull && (m & 1u)), s1 = __ballot(sfull && (m & 2u)),
-                           s2 = __ballot(sfull && (m & 4u)), s3 = __ballot(sfull && (m & 8u));
-            mycnt += count_gcli ? ballot_count(vg, g0, g1, g2, g3, want) : ballot_count(vs, s0, s1, s2, s3, want);
-            if (head && gidx / 8 == full)
-                hcnt[HREC_TAIL] = m + 1; /* the partial significance group */
+        if (hist) {
+            const uint64_t vg = __ballot(valid);
+            const uint64_t g0 = __ballot(gc & 1u), g1 = __ballot(gc & 2u), g2 = __ballot(gc & 4u),
+                           g3 = __ballot(gc & 8u);
+            mycnt += (uint32_t)__popcll(vg & ~(g0 ^ m0) & ~(g1 ^ m1) & ~(g2 ^ m2) & ~(g3 ^ m3));
+            if (head) {
+                if (gidx / 8 >= full)
+                    hcnt[HREC_TAIL] = m + 1; /* the partial significance group */
+                else if (m < 15)
+                    atomicAdd(&hcnt[16 + m], 1u); /* maximum 15: implied by the group count */
+            }
         }
     }
-    if (!o.hist)
+    if (!hist)
         return;
-    if (lane < HREC_TAIL && mycnt)
+    if (lane < 16 && mycnt)
         atomicAdd(&hcnt[lane], mycnt);
     __syncthreads();
     if (threadIdx.x < JXS_HREC) {
@@ -221,76 +212,103 @@ __device__ __forceinline__ void emit_band_row(const JxsBandOut o, uint32_t y, co
     __syncthreads();
 }
 
+/* Field-wise copy of a band output descriptor out of constant memory. */
+__device__ __forceinline__ JxsBandOut band_out(const CAS JxsBandOut& b) {
+    JxsBandOut o;
+    o.coeff = b.coeff;
+    o.gcli = b.gcli;
+    o.smax = b.smax;
+    o.hist = b.hist;
+    o.w = b.w;
+    o.gw = b.gw;
+    o.sw = b.sw;
+    o.cs = b.cs;
+    o.hl = b.hl;
+    o.bl = b.bl;
+    o.n_bl = b.n_bl;
+    o.pad_ = 0;
+    return o;
+}
+
 /* Lifts the row of w samples in buf through `levels` horizontal 5/3 levels
  * in place (the low-pass samples of level k at stride 2^k) and writes the
  * high band of every level and the final low-pass output (a band, or int32
  * samples at lf_i32).  buf is loaded and synchronised by the caller. */
-__device__ __forceinline__ void horz_emit(int32_t* buf, uint32_t w, uint32_t levels, const JxsBandOut* hf,
-                                          uint32_t lf_is_band,
-                          const JxsBandOut& lf, int32_t* lf_i32, uint32_t y, uint32_t* hcnt) {
+__device__ __forceinline__ void horz_emit(int32_t* buf, uint32_t w, uint32_t levels, const CAS JxsBandOut* hf,
+                                          uint32_t lf_is_band, const CAS JxsBandOut& lf, int32_t* lf_i32, uint32_t y,
+                                          uint32_t* hcnt) {
     uint32_t n = w, s = 1;
     for (uint32_t lvl = 0; lvl < levels; lvl++) {
         const uint32_t nh = n / 2, nl = n - nh;
+        /* predict: hf(i) = x(2i+1) - ((x(2i) + x(2i+2)) >> 1); at an even
+         * end x(2i+2) is replaced by x(2i), which gives x(2i+1) - x(2i) */
         for (uint32_t i = threadIdx.x; i < nh; i += blockDim.x) {
-            const int32_t a = buf[2 * i * s];
-            if (2 * i + 2 < n)
-                buf[(2 * i + 1) * s] -= (a + buf[(2 * i + 2) * s]) >> 1;
-            else
-                buf[(2 * i + 1) * s] -= a;
+            const uint32_t e = 2 * i + 2 < n ? 2 * i + 2 : 2 * i;
+            buf[(2 * i + 1) * s] -= (buf[2 * i * s] + buf[e * s]) >> 1;
         }
         __syncthreads();
+        /* update: lf(i) = x(2i) + ((hf(i-1) + hf(i) + 2) >> 2); a missing
+         * neighbour is replaced by the other one, which gives the one-sided
+         * (hf + 1) >> 1 at both ends (n >= 2 at every level) */
         for (uint32_t i = threadIdx.x; i < nl; i += blockDim.x) {
-            if (i == 0)
-                buf[0] += (buf[s] + 1) >> 1;
-            else if (i < nh)
-                buf[2 * i * s] += (buf[(2 * i - 1) * s] + buf[(2 * i + 1) * s] + 2) >> 2;
-            else
-                buf[2 * i * s] += (buf[(2 * i - 1) * s] + 1) >> 1;
+            const uint32_t l = i > 0 ? 2 * i - 1 : 2 * i + 1, r = i < nh ? 2 * i + 1 : 2 * i - 1;
+            buf[2 * i * s] += (buf[l * s] + buf[r * s] + 2) >> 2;
         }
         __syncthreads();
-        emit_band_row(hf[lvl], y, buf, s, 2 * s, nh, hcnt);
+        emit_band_row(band_out(hf[lvl]), y, buf, s, 2 * s, nh, hcnt);
         s *= 2;
         n = nl;
     }
     if (lf_is_band) {
-        emit_band_row(lf, y, buf, 0, s, n, hcnt);
+        emit_band_row(band_out(lf), y, buf, 0, s, n, hcnt);
     } else {
         for (uint32_t i = threadIdx.x; i < n; i += blockDim.x)
             gst(lf_i32 + i, buf[i * s]);
     }
 }
 
-/* A component plane read through load_src (input scaling applied). */
+/* Sample access for the transform, specialised per input kind so that the
+ * batched loads of a block are straight-line code.  Input samples are
+ * scaled as NltEnc.c:33-71 does: x = (v & mask) << shift - offset; the
+ * int32 low-pass planes of the second level are read as they are. */
+struct Scale {
+    uint32_t shift, mask;
+    int32_t offset;
+    __device__ __forceinline__ int32_t operator()(uint32_t v) const { return (int32_t)((v & mask) << shift) - offset; }
+};
+
 typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
-struct PlaneSrc {
-    const void* src;
-    uint32_t kind, stride, step, shift, mask;
-    int32_t offset;
-    __device__ __forceinline__ int32_t at(uint32_t row, uint32_t x) const {
-        return load_src(src, kind, (size_t)row * stride + (size_t)x * step, shift, mask, offset);
+template <int KIND>
+__device__ __forceinline__ int32_t ld1(const void* row, uint32_t x, uint32_t step, const Scale& sc) {
+    if (KIND == JXS_SRC_U16)
+        return sc(gld((const uint16_t*)row + (size_t)x * step));
+    if (KIND == JXS_SRC_U8)
+        return sc(gld((const uint8_t*)row + (size_t)x * step));
+    return gld((const int32_t*)row + (size_t)x * step);
+}
+
+/* columns x .. x+3 of a row with one load (contiguous rows, x a multiple
+ * of 4, 16-byte aligned plane and stride: checked by the host) */
+template <int KIND>
+__device__ __forceinline__ int4 ld4(const void* row, uint32_t x, const Scale& sc) {
+    if (KIND == JXS_SRC_U16) {
+        const u32x2 v = *(const GAS u32x2*)((const uint16_t*)row + x);
+        return make_int4(sc(v.x & 0xFFFFu), sc(v.x >> 16), sc(v.y & 0xFFFFu), sc(v.y >> 16));
     }
-    __device__ __forceinline__ int32_t scale(uint32_t v) const { return (int32_t)((v & mask) << shift) - offset; }
-    /* columns x .. x+3 of a row with one load; needs step 1, x and the
-     * stride multiples of 4 and an aligned base (checked by the host). */
-    __device__ __forceinline__ int4 at4(uint32_t row, uint32_t x) const {
-        const size_t idx = (size_t)row * stride + x;
-        if (kind == JXS_SRC_U16) {
-            const u32x2 v = *(const GAS u32x2*)((const uint16_t*)src + idx);
-            return make_int4(scale(v.x & 0xFFFFu), scale(v.x >> 16), scale(v.y & 0xFFFFu), scale(v.y >> 16));
-        }
-        if (kind == JXS_SRC_U8) {
-            const uint32_t v = *(const GAS uint32_t*)((const uint8_t*)src + idx);
-            return make_int4(scale(v & 0xFFu), scale((v >> 8) & 0xFFu), scale((v >> 16) & 0xFFu), scale(v >> 24));
-        }
-        const u32x4 v = *(const GAS u32x4*)((const int32_t*)src + idx);
-        return make_int4((int32_t)v.x, (int32_t)v.y, (int32_t)v.z, (int32_t)v.w);
+    if (KIND == JXS_SRC_U8) {
+        const uint32_t v = *(const GAS uint32_t*)((const uint8_t*)row + x);
+        return make_int4(sc(v & 0xFFu), sc((v >> 8) & 0xFFu), sc((v >> 16) & 0xFFu), sc(v >> 24));
     }
-};
+    const u32x4 v = *(const GAS u32x4*)((const int32_t*)row + x);
+    return make_int4((int32_t)v.x, (int32_t)v.y, (int32_t)v.z, (int32_t)v.w);
+}
 
 /* The vertical 5/3 step of one column from its input samples x(2i-2) ..
- * x(2i+2): low sample and high sample of row i. */
+ * x(2i+2) (DWT of CodeDeprecated.cc:84-127 / Dwt.c:337-398 semantics):
+ *   hf(k) = x(2k+1) - ((x(2k) + x(2k+2)) >> 1), or x(2k+1) - x(2k) at an even end
+ *   lf(k) = x(2k) + ((hf(k-1) + hf(k) + 2) >> 2), one-sided at the ends */
 struct VertFlags {
     bool prev, h, next; /* i > 0, i < n/2, 2i+2 < n */
 };
@@ -308,118 +326,89 @@ __device__ __forceinline__ void vert53(const VertFlags& f, int32_t xm2, int32_t 
 }
 
 #define QUAD_BATCH 4 /* column quads per thread whose loads are issued together */
-
-/* vert_fill for planes whose rows allow 4-column loads: each thread takes
- * QUAD_BATCH quads of columns, the five input rows of all of them loaded
- * before any is computed. */
-__device__ __forceinline__ void vert_fill4(const PlaneSrc& S, uint32_t i, uint32_t n, uint32_t w, int32_t* lo,
-                                           int32_t* hi) {
-    const VertFlags f = {i > 0, i < n / 2, 2 * i + 2 < n};
-    const uint32_t nq = w / 4;
-    const int4 z = make_int4(0, 0, 0, 0);
-    for (uint32_t q0 = 0; q0 < nq; q0 += QUAD_BATCH * blockDim.x) {
-        int4 xm2[QUAD_BATCH], xm1[QUAD_BATCH], xc[QUAD_BATCH], xp1[QUAD_BATCH], xp2[QUAD_BATCH];
-#pragma unroll
-        for (uint32_t k = 0; k < QUAD_BATCH; k++) {
-            const uint32_t x = 4 * min(q0 + k * blockDim.x + threadIdx.x, nq - 1);
-            xc[k] = S.at4(2 * i, x);
-            xm2[k] = f.prev ? S.at4(2 * i - 2, x) : z;
-            xm1[k] = f.prev ? S.at4(2 * i - 1, x) : z;
-            xp1[k] = f.h ? S.at4(2 * i + 1, x) : z;
-            xp2[k] = f.next ? S.at4(2 * i + 2, x) : z;
-        }
-#pragma unroll
-        for (uint32_t k = 0; k < QUAD_BATCH; k++) {
-            const uint32_t q = q0 + k * blockDim.x + threadIdx.x;
-            if (q >= nq)
-                break;
-            int4 l, h;
-            vert53(f, xm2[k].x, xm1[k].x, xc[k].x, xp1[k].x, xp2[k].x, l.x, h.x);
-            vert53(f, xm2[k].y, xm1[k].y, xc[k].y, xp1[k].y, xp2[k].y, l.y, h.y);
-            vert53(f, xm2[k].z, xm1[k].z, xc[k].z, xp1[k].z, xp2[k].z, l.z, h.z);
-            vert53(f, xm2[k].w, xm1[k].w, xc[k].w, xp1[k].w, xp2[k].w, l.w, h.w);
-            *(int4*)(lo + 4 * q) = l;
-            if (f.h)
-                *(int4*)(hi + 4 * q) = h;
-        }
-    }
-}
-
-#define FILL_BATCH 4 /* column batches of blockDim.x whose loads are issued together */
+#define FILL_BATCH 4 /* columns per thread whose loads are issued together */
 
 /* Low row i into lo[] and, when it exists, high row i into hi[] for the w
- * columns, from input rows 2i-2 .. 2i+2.  Loads use clamped column indexes
- * (always in bounds); FILL_BATCH coalesced column batches are loaded before
- * any is computed. */
-__device__ __forceinline__ void vert_fill(const PlaneSrc& S, uint32_t i, uint32_t n, uint32_t w, int32_t* lo,
-                                          int32_t* hi) {
-    const bool has_prev = i > 0, has_h = i < n / 2, has_next = 2 * i + 2 < n;
-    for (uint32_t x0 = 0; x0 < w; x0 += FILL_BATCH * blockDim.x) {
-        int32_t xm2[FILL_BATCH], xm1[FILL_BATCH], xc[FILL_BATCH], xp1[FILL_BATCH], xp2[FILL_BATCH];
+ * columns, from the five input rows r[0..4] = 2i-2 .. 2i+2 (clamped into
+ * the plane, so every load is in bounds; the flags drop what lies
+ * outside).  The loads of a whole batch are issued before any is used. */
+template <int KIND, bool VEC>
+__device__ __forceinline__ void vert_fill(const void* const r[5], const VertFlags& f, uint32_t w, uint32_t step,
+                                          const Scale& sc, int32_t* lo, int32_t* hi) {
+    if (VEC) {
+        const uint32_t nq = w / 4;
+        for (uint32_t q0 = 0; q0 < nq; q0 += QUAD_BATCH * blockDim.x) {
+            int4 v[5][QUAD_BATCH];
 #pragma unroll
-        for (uint32_t k = 0; k < FILL_BATCH; k++) {
-            const uint32_t x = min(x0 + k * blockDim.x + threadIdx.x, w - 1);
-            xc[k] = S.at(2 * i, x);
-            xm2[k] = has_prev ? S.at(2 * i - 2, x) : 0;
-            xm1[k] = has_prev ? S.at(2 * i - 1, x) : 0;
-            xp1[k] = has_h ? S.at(2 * i + 1, x) : 0;
-            xp2[k] = has_next ? S.at(2 * i + 2, x) : 0;
+            for (uint32_t k = 0; k < QUAD_BATCH; k++) {
+                const uint32_t x = 4 * min(q0 + k * blockDim.x + threadIdx.x, nq - 1);
+#pragma unroll
+                for (uint32_t j = 0; j < 5; j++)
+                    v[j][k] = ld4<KIND>(r[j], x, sc);
+            }
+#pragma unroll
+            for (uint32_t k = 0; k < QUAD_BATCH; k++) {
+                const uint32_t q = q0 + k * blockDim.x + threadIdx.x;
+                if (q >= nq)
+                    break;
+                int4 l, h;
+                vert53(f, v[0][k].x, v[1][k].x, v[2][k].x, v[3][k].x, v[4][k].x, l.x, h.x);
+                vert53(f, v[0][k].y, v[1][k].y, v[2][k].y, v[3][k].y, v[4][k].y, l.y, h.y);
+                vert53(f, v[0][k].z, v[1][k].z, v[2][k].z, v[3][k].z, v[4][k].z, l.z, h.z);
+                vert53(f, v[0][k].w, v[1][k].w, v[2][k].w, v[3][k].w, v[4][k].w, l.w, h.w);
+                *(int4*)(lo + 4 * q) = l;
+                if (f.h)
+                    *(int4*)(hi + 4 * q) = h;
+            }
         }
+    } else {
+        for (uint32_t x0 = 0; x0 < w; x0 += FILL_BATCH * blockDim.x) {
+            int32_t v[5][FILL_BATCH];
 #pragma unroll
-        for (uint32_t k = 0; k < FILL_BATCH; k++) {
-            const uint32_t x = x0 + k * blockDim.x + threadIdx.x;
-            if (x >= w)
-                break;
-            const int32_t hcur = has_h ? (has_next ? xp1[k] - ((xc[k] + xp2[k]) >> 1) : xp1[k] - xc[k]) : 0;
-            const int32_t hprev = has_prev ? xm1[k] - ((xm2[k] + xc[k]) >> 1) : 0;
-            int32_t lf;
-            if (!has_prev)
-                lf = xc[k] + ((hcur + 1) >> 1);
-            else if (has_h)
-                lf = xc[k] + ((hprev + hcur + 2) >> 2);
-            else
-                lf = xc[k] + ((hprev + 1) >> 1);
-            lo[x] = lf;
-            if (has_h)
-                hi[x] = hcur;
+            for (uint32_t k = 0; k < FILL_BATCH; k++) {
+                const uint32_t x = min(x0 + k * blockDim.x + threadIdx.x, w - 1);
+#pragma unroll
+                for (uint32_t j = 0; j < 5; j++)
+                    v[j][k] = ld1<KIND>(r[j], x, step, sc);
+            }
+#pragma unroll
+            for (uint32_t k = 0; k < FILL_BATCH; k++) {
+                const uint32_t x = x0 + k * blockDim.x + threadIdx.x;
+                if (x >= w)
+                    break;
+                int32_t l, h;
+                vert53(f, v[0][k], v[1][k], v[2][k], v[3][k], v[4][k], l, h);
+                lo[x] = l;
+                if (f.h)
+                    hi[x] = h;
+            }
         }
     }
 }
 
-/* One block per output row (pair).  With a vertical level, block i computes
- * low row i and high row i from input rows 2i-2 .. 2i+2 straight into LDS
- * (vertical 5/3: hf(k) = x(2k+1) - ((x(2k) + x(2k+2)) >> 1), lf(k) = x(2k) +
- * ((hf(k-1) + hf(k) + 2) >> 2), one-sided at the ends), lifts the low row
- * through the horizontal levels of the low rows and writes its bands (or the
- * int32 low-pass plane of the next vertical level), then the high row
- * through one horizontal level into the LH and HH bands.  The intermediate
- * rows never leave LDS.  Without a vertical level, block i lifts input row
- * i.  Dynamic LDS: two rows of the widest job. */
-__global__ void __launch_bounds__(256) k_dwt(const JxsDwtJob* __restrict__ jobs, const uint32_t* __restrict__ bases,
-                                             int njobs, uint32_t nblocks, uint32_t wmax) {
-    extern __shared__ __align__(16) int32_t rows_lds[];
-    __shared__ uint32_t hcnt[JXS_HREC];
-    int32_t* lo = rows_lds;
-    int32_t* hi = rows_lds + ((wmax + 3) & ~3u); /* 16-byte aligned */
-    const uint32_t blk = xcd_remap(blockIdx.x, nblocks);
-    const int j = find_job_u32(bases, njobs, blk);
-    const JxsDwtJob& J = jobs[j];
-    const uint32_t i = blk - J.base;
-    const PlaneSrc S = {J.src, J.kind, J.stride, J.step, J.shift, J.mask, J.offset};
-    const uint32_t w = J.w, n = J.h;
-    if (threadIdx.x < JXS_HREC)
-        hcnt[threadIdx.x] = 0;
-    if (J.vert) {
-        if (J.vec)
-            vert_fill4(S, i, n, w, lo, hi);
-        else
-            vert_fill(S, i, n, w, lo, hi);
+/* Input row i into lo[] (components without vertical levels). */
+template <int KIND, bool VEC>
+__device__ __forceinline__ void row_fill(const void* row, uint32_t w, uint32_t step, const Scale& sc, int32_t* lo) {
+    if (VEC) {
+        const uint32_t nq = w / 4;
+        for (uint32_t q0 = 0; q0 < nq; q0 += QUAD_BATCH * blockDim.x) {
+            int4 v[QUAD_BATCH];
+#pragma unroll
+            for (uint32_t k = 0; k < QUAD_BATCH; k++)
+                v[k] = ld4<KIND>(row, 4 * min(q0 + k * blockDim.x + threadIdx.x, nq - 1), sc);
+#pragma unroll
+            for (uint32_t k = 0; k < QUAD_BATCH; k++) {
+                const uint32_t q = q0 + k * blockDim.x + threadIdx.x;
+                if (q < nq)
+                    *(int4*)(lo + 4 * q) = v[k];
+            }
+        }
     } else {
         for (uint32_t x0 = 0; x0 < w; x0 += FILL_BATCH * blockDim.x) {
             int32_t v[FILL_BATCH];
 #pragma unroll
             for (uint32_t k = 0; k < FILL_BATCH; k++)
-                v[k] = S.at(i, min(x0 + k * blockDim.x + threadIdx.x, w - 1));
+                v[k] = ld1<KIND>(row, min(x0 + k * blockDim.x + threadIdx.x, w - 1), step, sc);
 #pragma unroll
             for (uint32_t k = 0; k < FILL_BATCH; k++) {
                 const uint32_t x = x0 + k * blockDim.x + threadIdx.x;
@@ -428,12 +417,77 @@ __global__ void __launch_bounds__(256) k_dwt(const JxsDwtJob* __restrict__ jobs,
             }
         }
     }
+}
+
+/* Element size of an input kind. */
+__device__ __forceinline__ uint32_t kind_bytes(uint32_t kind) {
+    return kind == JXS_SRC_U8 ? 1u : kind == JXS_SRC_U16 ? 2u : 4u;
+}
+
+/* The work of one block of k_dwt for one input kind. */
+template <int KIND, bool VEC>
+__device__ __forceinline__ void dwt_block(const CAS JxsDwtJob& J, uint32_t i, int32_t* lo, int32_t* hi, uint32_t* hcnt) {
+    const uint32_t w = J.w, n = J.h, step = J.step;
+    const Scale sc = {J.shift, J.mask, J.offset};
+    const uint8_t* base = (const uint8_t*)J.src;
+    const size_t pitch = (size_t)J.stride * kind_bytes(KIND);
+    if (J.vert) {
+        const VertFlags f = {i > 0, i < n / 2, 2 * i + 2 < n};
+        const void* r[5];
+#pragma unroll
+        for (int j = 0; j < 5; j++) {
+            const int row = min(max((int)(2 * i) + j - 2, 0), (int)n - 1);
+            r[j] = base + (size_t)row * pitch;
+        }
+        vert_fill<KIND, VEC>(r, f, w, step, sc, lo, hi);
+    } else {
+        row_fill<KIND, VEC>(base + (size_t)i * pitch, w, step, sc, lo);
+    }
     __syncthreads();
     horz_emit(lo, w, J.lev_l, J.hf, J.lf_is_band, J.lf, J.lf_is_band ? nullptr : J.lf_i32 + (size_t)i * J.lf_stride,
               i, hcnt);
     if (J.vert && i < n / 2) {
         __syncthreads();
         horz_emit(hi, w, 1, &J.hh, 1, J.lh, nullptr, i, hcnt);
+    }
+}
+
+/* One block per output row (pair).  With a vertical level, block i computes
+ * low row i and high row i from input rows 2i-2 .. 2i+2 straight into LDS,
+ * lifts the low row through the horizontal levels of the low rows and
+ * writes its bands (or the int32 low-pass plane of the next vertical level),
+ * then the high row through one horizontal level into the LH and HH bands.
+ * The intermediate rows never leave LDS.  Without a vertical level, block i
+ * lifts input row i.  Dynamic LDS: two rows of the widest job. */
+__global__ void __launch_bounds__(256) k_dwt(const JxsDwtJob* __restrict__ jobs, const uint32_t* __restrict__ bases,
+                                             int njobs, uint32_t nblocks, uint32_t wmax) {
+    extern __shared__ __align__(16) int32_t rows_lds[];
+    __shared__ uint32_t hcnt[JXS_HREC];
+    int32_t* lo = rows_lds;
+    int32_t* hi = rows_lds + ((wmax + 3) & ~3u); /* 16-byte aligned */
+    const uint32_t blk = xcd_remap(blockIdx.x, nblocks);
+    const int j = find_job_u32(bases, njobs, blk);
+    const CAS JxsDwtJob& J = ((const CAS JxsDwtJob*)jobs)[j];
+    const uint32_t i = blk - J.base;
+    if (threadIdx.x < JXS_HREC)
+        hcnt[threadIdx.x] = 0;
+    const uint32_t kind = J.kind;
+    const bool vec = J.vec != 0;
+    if (kind == JXS_SRC_U16) {
+        if (vec)
+            dwt_block<JXS_SRC_U16, true>(J, i, lo, hi, hcnt);
+        else
+            dwt_block<JXS_SRC_U16, false>(J, i, lo, hi, hcnt);
+    } else if (kind == JXS_SRC_U8) {
+        if (vec)
+            dwt_block<JXS_SRC_U8, true>(J, i, lo, hi, hcnt);
+        else
+            dwt_block<JXS_SRC_U8, false>(J, i, lo, hi, hcnt);
+    } else {
+        if (vec)
+            dwt_block<JXS_SRC_I32, true>(J, i, lo, hi, hcnt);
+        else
+            dwt_block<JXS_SRC_I32, false>(J, i, lo, hi, hcnt);
     }
 }
 
