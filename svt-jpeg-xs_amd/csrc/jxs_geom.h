@@ -22,10 +22,7 @@
 #define JXS_NQ 32   /* quantisation values 0..31 */
 #define JXS_NR 32   /* refinement values 0..31 (<= bands - 1) */
 #define JXS_HDR_MAX 256
-#define JXS_NU 256 /* packing units per precinct: packet headers + band lines of each sub-packet */
-
-/* kinds of packing units (the sub-packets of a packet, in codestream order) */
-enum { UNIT_HDR = 0, UNIT_SIG = 1, UNIT_GCLI = 2, UNIT_DATA = 3 };
+#define JXS_NU 256 /* packing units (band lines) per precinct */
 #define JXS_MAX_ROW 7680 /* widest component row the row kernels stage in LDS */
 
 /* Band-line histogram record (u16 units), written by the transform for
@@ -71,11 +68,10 @@ struct JxsGeom {
     uint8_t gc[JXS_NGB], gb[JXS_NGB];
     uint32_t pk_bs[JXS_NPK], pk_be[JXS_NPK], pk_line[JXS_NPK];
     uint32_t raw_bytes[2][JXS_NPK];
-    /* packing units of a precinct in codestream order: for every packet its
-     * header, then per sub-packet (significance, bit-plane counts, data) one
-     * unit per band of the packet (u_gi = global band, 0xFF for a header) */
+    /* packing units of a precinct: the band lines in packet order (u_p
+     * packet, u_gi global band) */
     uint32_t n_units, pad_u[3];
-    uint8_t u_p[JXS_NU], u_kind[JXS_NU], u_gi[JXS_NU];
+    uint8_t u_p[JXS_NU], u_gi[JXS_NU], pad_v[JXS_NU];
     uint64_t n_coeff, n_gcli, n_smax; /* buffer sizes per frame */
     uint64_t n_i32;                   /* int32 transform scratch per frame */
     uint8_t hdr[JXS_HDR_MAX];
@@ -100,22 +96,23 @@ struct JxsPlan {
     uint8_t band[JXS_NGB]; /* per global band: truncation | (significance coding << 4) */
 };
 
-/* item streams of the packing units */
-enum { ITEM_RAW = 0, ITEM_SIG = 1, ITEM_VLC = 2, ITEM_DATA = 3, ITEM_HDR = 4 };
 
-/* One packing unit, self-contained (written by the packet plan, read by the
- * packing kernel with a single load): where its bits go and what produces
- * them. */
+/* One packing unit = one band line of one precinct, self-contained
+ * (written by the packet plan, read by the packing kernel with a single
+ * load): where the line's significance, bit-plane-count and data segments
+ * go and what produces them; the first band line of a packet also carries
+ * the packet header. */
 struct JxsUnit {
     const uint8_t* gl;  /* GCLI row of the band line */
     const uint8_t* sl;  /* significance-maximum row */
     const uint16_t* cl; /* coefficient row */
-    uint32_t* ob;       /* output dword holding the unit's first bit */
     uint32_t* status;   /* the frame's error flags */
-    uint64_t hdr;       /* packet header bits (ITEM_HDR) */
-    uint32_t lo, len;   /* first bit inside *ob (0..31), number of bits (0: nothing) */
-    uint32_t item;      /* ITEM_* */
-    uint32_t gt, ms, qpih;
+    uint32_t* ob[3];    /* output dword holding the first bit of each segment */
+    uint32_t* hob;      /* output dword holding the packet header's first bit */
+    uint64_t hdr;       /* packet header bits */
+    uint32_t lo[3], len[3]; /* first bit inside *ob (0..31) and bits of each segment */
+    uint32_t hlo, hlen;     /* the same for the packet header (hlen 0: none) */
+    uint32_t gt, ms, raw, qpih;
     uint32_t w, gw, sw;
     uint32_t pad_[3];
 };

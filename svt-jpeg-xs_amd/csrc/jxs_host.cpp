@@ -149,21 +149,15 @@ int build_layout(JxsGeom& g, uint32_t nc, const uint32_t sx[4], const uint32_t s
         n++;
     }
     g.npk = n;
-    /* packing units (one per packet header, one per band of each sub-packet) */
+    /* packing units: the band lines of a precinct, packet by packet */
     g.n_units = 0;
-    for (uint32_t p = 0; p < g.npk; p++) {
-        if (g.n_units + 1 + 3 * (g.pk_be[p] - g.pk_bs[p]) > JXS_NU)
-            return -1;
-        g.u_p[g.n_units] = (uint8_t)p;
-        g.u_kind[g.n_units] = UNIT_HDR;
-        g.u_gi[g.n_units++] = NE;
-        for (uint32_t kind = UNIT_SIG; kind <= UNIT_DATA; kind++)
-            for (uint32_t gi = g.pk_bs[p]; gi < g.pk_be[p]; gi++) {
-                g.u_p[g.n_units] = (uint8_t)p;
-                g.u_kind[g.n_units] = (uint8_t)kind;
-                g.u_gi[g.n_units++] = (uint8_t)gi;
-            }
-    }
+    for (uint32_t p = 0; p < g.npk; p++)
+        for (uint32_t gi = g.pk_bs[p]; gi < g.pk_be[p]; gi++) {
+            if (g.n_units >= JXS_NU)
+                return -1;
+            g.u_p[g.n_units] = (uint8_t)p;
+            g.u_gi[g.n_units++] = (uint8_t)gi;
+        }
     memset(g.ex_gid, NE, sizeof(g.ex_gid));
     for (uint32_t gi = 0, e = 0; gi < g.nb_all; gi++)
         if (g.gb[gi] != NE)

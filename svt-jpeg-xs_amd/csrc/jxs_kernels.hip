@@ -23,11 +23,13 @@
  *   k_plan     one wave per precinct: per-band truncation and coding method,
  *              packet sizes and offsets, slice / precinct headers, padding,
  *              end marker and frame header.
- *   k_pack     one wave per (precinct, packet): packet header, significance,
- *              bit-plane-count and data sub-packets (PackPrecinct.c:235-572)
- *              with quantisation (Quant.c:9-45) fused; bit offsets from wave
- *              prefix sums, bits assembled in a per-wave LDS ring and written
- *              out as dwords.
+ *   k_pack     one wave per band line of a precinct: its significance,
+ *              bit-plane-count and data segments (PackPrecinct.c:235-572)
+ *              with quantisation (Quant.c:9-45) fused, each at the bit
+ *              offset the plan gives it, plus the packet header for a
+ *              packet's first band line; bit offsets inside a segment from
+ *              wave prefix sums, bits assembled in per-wave LDS rings and
+ *              written out as dwords.
  *
  * All arithmetic is integer.  Every kernel takes a frame-descriptor array so
  * one launch covers a batch of heterogeneous frames.
@@ -883,7 +885,7 @@ __global__ void __launch_bounds__(64) k_plan(const JxsFrameDesc* __restrict__ fr
     __shared__ uint32_t bd_d[JXS_NBL], bd_g[JXS_NBL], bd_s[JXS_NBL];
     __shared__ uint8_t b_ms[JXS_NGB], b_gt[JXS_NGB];
     __shared__ uint32_t k_rel[JXS_NPK], k_sb[JXS_NPK], k_gb[JXS_NPK], k_db[JXS_NPK], k_fl[JXS_NPK];
-    __shared__ uint32_t u_bits[JXS_NU];
+    __shared__ uint32_t ub_s[JXS_NU], ub_g[JXS_NU], ub_d[JXS_NU];
     const int f = find_frame(frames, nfr, blockIdx.x);
     const JxsFrameDesc& F = frames[f];
     __shared__ __align__(16) JxsGeom sg;
@@ -900,7 +902,11 @@ __global__ void __launch_bounds__(64) k_plan(const JxsFrameDesc* __restrict__ fr
         if (lane < g.npk)
             PL.pk[lane].flags = 0;
         for (uint32_t u = lane; u < g.n_units; u += WAVE)
-            F.units[(size_t)row * g.n_units + u].len = 0;
+            {
+                JxsUnit& U = F.units[(size_t)row * g.n_units + u];
+                U.len[0] = U.len[1] = U.len[2] = 0;
+                U.hlen = 0;
+            }
         return;
     }
     const uint16_t* recs = F.hist + (size_t)row * g.n_bl * JXS_HREC;
@@ -971,7 +977,11 @@ __global__ void __launch_bounds__(64) k_plan(const JxsFrameDesc* __restrict__ fr
         if (lane < g.npk)
             PL.pk[lane].flags = 0;
         for (uint32_t u = lane; u < g.n_units; u += WAVE)
-            F.units[(size_t)row * g.n_units + u].len = 0;
+            {
+                JxsUnit& U = F.units[(size_t)row * g.n_units + u];
+                U.len[0] = U.len[1] = U.len[2] = 0;
+                U.hlen = 0;
+            }
         if (lane == 0)
             atomicOr(F.status, 2u);
         return;
@@ -986,8 +996,9 @@ __global__ void __launch_bounds__(64) k_plan(const JxsFrameDesc* __restrict__ fr
         K.flags = flags;
         PL.pk[lane] = K;
     }
-    /* packing units, one lane each: first the bits of every unit, then its
-     * place after the earlier units of the same sub-packet */
+    /* packing units, one lane each: the bits of the band line's three
+     * segments (significance, bit-plane counts, data), then their places
+     * after the earlier band lines of the same packet */
     if (lane < g.npk) {
         k_rel[lane] = rel;
         k_sb[lane] = sb;
@@ -997,65 +1008,68 @@ __global__ void __launch_bounds__(64) k_plan(const JxsFrameDesc* __restrict__ fr
     }
     __syncthreads();
     for (uint32_t u = lane; u < g.n_units; u += WAVE) {
-        const uint32_t p = g.u_p[u], kind = g.u_kind[u];
-        uint32_t bits = 0;
-        if (kind == UNIT_HDR) {
-            bits = (k_fl[p] & PKT_ANY) ? 8 * g.pkt_hdr_bytes : 0u;
-        } else {
-            const uint32_t gi = g.u_gi[u], l = g.pk_line[p];
-            const JxsBand& bd = g.c[g.gc[gi]].b[g.gb[gi]];
-            const bool raw = (k_fl[p] & PKT_RAW) != 0;
-            if (l < bd.lines[type]) {
-                const uint32_t bl = bd.bl + l;
-                if (kind == UNIT_SIG)
-                    bits = raw ? 0u : bd_s[bl];
-                else if (kind == UNIT_GCLI)
-                    bits = raw ? 4 * bd.gw : bd_g[bl];
-                else
-                    bits = bd_d[bl];
-            }
+        const uint32_t p = g.u_p[u], gi = g.u_gi[u], l = g.pk_line[p];
+        const JxsBand& bd = g.c[g.gc[gi]].b[g.gb[gi]];
+        const bool raw = (k_fl[p] & PKT_RAW) != 0;
+        uint32_t s = 0, gb = 0, d = 0;
+        if (l < bd.lines[type]) {
+            const uint32_t bl = bd.bl + l;
+            s = raw ? 0u : bd_s[bl];
+            gb = raw ? 4 * bd.gw : bd_g[bl];
+            d = bd_d[bl];
         }
-        u_bits[u] = bits;
+        ub_s[u] = s;
+        ub_g[u] = gb;
+        ub_d[u] = d;
     }
     __syncthreads();
     {
         const uint64_t pbit = 8 * (uint64_t)P.offset; /* precinct's first bit in the codestream */
         JxsUnit* UR = F.units + (size_t)row * g.n_units;
         for (uint32_t u = lane; u < g.n_units; u += WAVE) {
-            const uint32_t p = g.u_p[u], kind = g.u_kind[u];
+            const uint32_t p = g.u_p[u], gi = g.u_gi[u], l = g.pk_line[p], j = gi - g.pk_bs[p];
+            const JxsBand& bd = g.c[g.gc[gi]].b[g.gb[gi]];
             const bool raw = (k_fl[p] & PKT_RAW) != 0;
+            uint32_t a[3];
+            a[0] = 8 * (k_rel[p] + g.pkt_hdr_bytes);
+            a[1] = a[0] + 8 * k_sb[p];
+            a[2] = a[1] + 8 * k_gb[p];
+            for (uint32_t k = 0; k < j; k++) {
+                a[0] += ub_s[u - j + k];
+                a[1] += ub_g[u - j + k];
+                a[2] += ub_d[u - j + k];
+            }
+            const uint32_t len[3] = {ub_s[u], ub_g[u], ub_d[u]};
             JxsUnit U = {};
-            uint32_t bit = 8 * k_rel[p];
-            if (kind == UNIT_HDR) {
-                /* packet header (PackHeaders.c:98-127): RAW flag, data
-                 * bytes, bit-plane-count bytes, sign bytes (0) */
-                U.item = ITEM_HDR;
+            if (l < bd.lines[type]) {
+                const uint32_t y = row * bd.hl + l;
+                U.gl = F.gcli + bd.gcli_off + (size_t)y * bd.gw;
+                U.sl = F.smax + bd.smax_off + (size_t)y * bd.sw;
+                U.cl = F.coeff + bd.coeff_off + (size_t)y * bd.cs;
+                U.gt = b_gt[gi];
+                U.ms = b_ms[gi];
+                U.raw = raw ? 1u : 0u;
+                U.qpih = g.qpih;
+                U.w = bd.w;
+                U.gw = bd.gw;
+                U.sw = bd.sw;
+            }
+            for (uint32_t k = 0; k < 3; k++) {
+                const uint64_t first = pbit + a[k];
+                U.ob[k] = (uint32_t*)F.out + (first >> 5);
+                U.lo[k] = (uint32_t)(first & 31);
+                U.len[k] = len[k];
+            }
+            if (j == 0 && (k_fl[p] & PKT_ANY)) {
+                /* packet header (PackHeaders.c:98-127): RAW flag, data bytes,
+                 * bit-plane-count bytes, sign bytes (0) */
+                const uint64_t first = pbit + 8 * k_rel[p];
+                U.hob = (uint32_t*)F.out + (first >> 5);
+                U.hlo = (uint32_t)(first & 31);
+                U.hlen = 8 * g.pkt_hdr_bytes;
                 U.hdr = g.short_hdr ? ((uint64_t)raw << 39) | ((uint64_t)k_db[p] << 24) | ((uint64_t)k_gb[p] << 11)
                                     : ((uint64_t)raw << 55) | ((uint64_t)k_db[p] << 35) | ((uint64_t)k_gb[p] << 15);
-            } else {
-                const uint32_t gi = g.u_gi[u], l = g.pk_line[p], j = gi - g.pk_bs[p];
-                const JxsBand& bd = g.c[g.gc[gi]].b[g.gb[gi]];
-                bit += 8 * (g.pkt_hdr_bytes + (kind >= UNIT_GCLI ? k_sb[p] : 0u) + (kind == UNIT_DATA ? k_gb[p] : 0u));
-                for (uint32_t k = 0; k < j; k++)
-                    bit += u_bits[u - j + k];
-                if (u_bits[u]) {
-                    const uint32_t y = row * bd.hl + l;
-                    U.gl = F.gcli + bd.gcli_off + (size_t)y * bd.gw;
-                    U.sl = F.smax + bd.smax_off + (size_t)y * bd.sw;
-                    U.cl = F.coeff + bd.coeff_off + (size_t)y * bd.cs;
-                    U.item = kind == UNIT_SIG ? ITEM_SIG : kind == UNIT_DATA ? ITEM_DATA : raw ? ITEM_RAW : ITEM_VLC;
-                    U.gt = b_gt[gi];
-                    U.ms = b_ms[gi];
-                    U.qpih = g.qpih;
-                    U.w = bd.w;
-                    U.gw = bd.gw;
-                    U.sw = bd.sw;
-                }
             }
-            const uint64_t first = pbit + bit;
-            U.ob = (uint32_t*)F.out + (first >> 5);
-            U.lo = (uint32_t)(first & 31);
-            U.len = u_bits[u];
             U.status = F.status;
             UR[u] = U;
         }
@@ -1116,33 +1130,105 @@ __global__ void __launch_bounds__(64) k_plan(const JxsFrameDesc* __restrict__ fr
 /* packing                                                             */
 /* ------------------------------------------------------------------ */
 #define PK_WAVES 4
-#define RINGW 1024       /* 32-bit words of one wave's LDS bit ring (4 KiB): one step adds at most 512 */
-#define RING_FLUSH 256   /* flush once this many complete words are pending */
+#define RINGB 512      /* words of a wave's significance / bit-plane-count ring (2 KiB): a step adds <= 128 */
+#define RINGC 1024     /* words of a wave's data ring (4 KiB): a step adds <= 512 */
+#define RING_FLUSH 256 /* flush once this many complete words are pending */
 
-/* Orders this wave's LDS accesses around cross-lane reads of the ring. */
+/* Orders this wave's LDS accesses around cross-lane reads of a ring. */
 __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-/* ORs the n (1..64) low bits of v, MSB first, into the ring at bit pos: the
- * bits are MSB-aligned in 64 bits and cut at the word boundaries; words
- * that receive only zeros are skipped. */
-__device__ __forceinline__ void ring_put(uint32_t* ring, uint32_t pos, uint64_t v, uint32_t n) {
-    const uint32_t sh = pos & 31, wd = pos >> 5;
+/* The n (1..64) low bits of v, MSB first, placed at bit offset sh (0..31)
+ * of three consecutive 32-bit words (MSB-first bit order). */
+__device__ __forceinline__ void split_bits(uint64_t v, uint32_t n, uint32_t sh, uint32_t& w0, uint32_t& w1,
+                                           uint32_t& w2) {
     const uint64_t a = v << (64 - n);
-    const uint32_t w0 = (uint32_t)(a >> (32 + sh));
+    w0 = (uint32_t)(a >> (32 + sh));
     const uint64_t rest = a << (32 - sh);
-    const uint32_t w1 = (uint32_t)(rest >> 32), w2 = (uint32_t)rest;
-    if (w0)
-        atomicOr(&ring[wd & (RINGW - 1)], w0);
-    if (w1)
-        atomicOr(&ring[(wd + 1) & (RINGW - 1)], w1);
-    if (w2)
-        atomicOr(&ring[(wd + 2) & (RINGW - 1)], w2);
+    w1 = (uint32_t)(rest >> 32);
+    w2 = (uint32_t)rest;
 }
 
+/* The output of one segment: bits [lo, hi) counted from the dword at ob
+ * (the dword holding the segment's first bit), assembled in an LDS ring of
+ * mask + 1 words; ring word w is output dword ob[w]. */
+struct WaveOut {
+    uint32_t* ring;
+    uint32_t mask;
+    uint32_t* ob;
+    uint32_t lo, hi, pos, flushed, lane;
+    __device__ __forceinline__ void start(uint32_t* r, uint32_t m, uint32_t* o, uint32_t first, uint32_t len,
+                                          uint32_t ln) {
+        ring = r;
+        mask = m;
+        ob = o;
+        lo = first;
+        hi = first + len;
+        pos = first;
+        flushed = 0;
+        lane = ln;
+        const uint32_t words = min((hi + 31) / 32, mask + 1);
+        for (uint32_t k = lane; k < words; k += WAVE)
+            ring[k] = 0;
+    }
+    /* ORs the n (1..64) low bits of v into the ring at bit p (words that
+     * receive only zeros are skipped) */
+    __device__ __forceinline__ void put(uint32_t p, uint64_t v, uint32_t n) {
+        uint32_t w0, w1, w2;
+        split_bits(v, n, p & 31, w0, w1, w2);
+        const uint32_t wd = p >> 5;
+        if (w0)
+            atomicOr(&ring[wd & mask], w0);
+        if (w1)
+            atomicOr(&ring[(wd + 1) & mask], w1);
+        if (w2)
+            atomicOr(&ring[(wd + 2) & mask], w2);
+    }
+    /* Writes ring words [from, to) out and clears them.  A word wholly inside
+     * the segment is stored; the segment's first and last words, which it
+     * may share with its neighbours, are ORed into the zeroed output
+     * (device-scope atomics: the neighbours may run on other XCDs). */
+    __device__ __forceinline__ void flush(uint32_t from, uint32_t to) {
+        for (uint32_t w = from + lane; w < to; w += WAVE) {
+            uint32_t* slot = &ring[w & mask];
+            const uint32_t v = *slot;
+            *slot = 0;
+            const uint32_t b0 = 32 * w;
+            if (b0 >= lo && b0 + 32 <= hi)
+                gst(ob + w, __builtin_bswap32(v));
+            else if (v)
+                atomicOr(ob + w, __builtin_bswap32(v));
+        }
+    }
+    __device__ __forceinline__ void maybe_flush() {
+        if ((pos >> 5) - flushed >= RING_FLUSH) {
+            wave_sync();
+            flush(flushed, pos >> 5);
+            flushed = pos >> 5;
+            wave_sync();
+        }
+    }
+    __device__ __forceinline__ void finish() {
+        wave_sync();
+        flush(flushed, (hi + 31) / 32);
+    }
+};
+
+/* Magnitude of one quantised coefficient shifted down by gt (Quant.c:9-45):
+ * deadzone keeps the bit-planes above gt (m >> gt), uniform rounds to the
+ * nearest of the 2^(gc-gt) levels (identity at gt = 0). */
+template <bool UNIFORM>
+__device__ __forceinline__ uint32_t quant_mag(uint32_t c, uint32_t gc, uint32_t gt) {
+    const uint32_t m = c & 0x7FFFu;
+    if (!UNIFORM)
+        return m >> gt;
+    const uint32_t sc = gc - gt + 1;
+    const uint32_t u = (((m << sc) - m + (1u << gc)) >> (gc + 1)) & 0xFFFFu;
+    return gt == 0 ? m : u;
+}
 
 /* bit i of x (x < 2^8) -> bit 4i */
 __device__ __forceinline__ uint32_t spread8(uint32_t x) {
@@ -1164,178 +1250,179 @@ __device__ __forceinline__ uint64_t interleave4(uint32_t q0, uint32_t q1, uint32
     return ((uint64_t)hi << 32) | lo;
 }
 
-/* Magnitude of one quantised coefficient shifted down by gt
- * (Quant.c:9-45: deadzone keeps the bit-planes above gt, uniform rounds). */
-__device__ __forceinline__ uint32_t quant_mag(uint32_t c, uint32_t gc, uint32_t gt, uint32_t qpih) {
-    const uint32_t m = c & 0x7FFFu;
-    if (gt == 0)
-        return m;
-    if (qpih == 0)
-        return m >> gt;
-    const uint32_t sc = gc - gt + 1;
-    return (((m << sc) - m + (1u << gc)) >> (gc + 1)) & 0xFFFFu;
-}
-
-/* The output of one packing unit: bits [lo, hi) counted from the dword at
- * ob (the dword holding the unit's first bit), assembled in the wave's LDS
- * ring; ring word w is output dword ob[w]. */
-struct WaveOut {
-    uint32_t* ring;
-    uint32_t* ob;
-    uint32_t lo, hi, pos, flushed, lane;
-};
-
-/* Writes ring words [from, to) to the output and clears them.  A word that
- * lies wholly inside the unit is stored; the unit's first and last words,
- * which it may share with its neighbours, are ORed into the zeroed output
- * (device-scope atomics: the neighbours may run on other XCDs). */
-__device__ __forceinline__ void ring_flush(WaveOut& W, uint32_t from, uint32_t to) {
-    for (uint32_t w = from + W.lane; w < to; w += WAVE) {
-        uint32_t* slot = &W.ring[w & (RINGW - 1)];
-        const uint32_t v = *slot;
-        *slot = 0;
-        const uint32_t b0 = 32 * w;
-        if (b0 >= W.lo && b0 + 32 <= W.hi)
-            gst(W.ob + w, __builtin_bswap32(v));
-        else if (v)
-            atomicOr(W.ob + w, __builtin_bswap32(v));
-    }
-}
-
-
 #define IPL 4 /* consecutive items coded by one lane per step */
 
-/* Appends the items of one band line, 256 per step: lane L codes items
- * 4L .. 4L+3 of the step (consecutive, so its bits are contiguous), one wave
- * prefix sum over the lanes' bit counts places them.  Bit-plane-count,
- * significance and raw codes of a lane are concatenated into one value
- * (at most 4 x 16 bits); data groups are written one by one. */
-__device__ __forceinline__ void emit_line(WaveOut& W, int kind, const uint8_t* gl, const uint8_t* sl,
-                                          const uint16_t* cl, uint32_t w, uint32_t gw, uint32_t sw, uint32_t gt,
-                                          uint32_t use_sig, uint32_t qpih) {
-    const uint32_t nitems = kind == ITEM_SIG ? sw : gw;
-    const uint32_t lane = W.lane;
-    for (uint32_t base = 0; base < nitems; base += IPL * WAVE) {
-        const uint32_t i0 = base + IPL * lane;
-        uint32_t a[IPL], s8[IPL];
-        uint2 cv[IPL];
+/* Significance sub-packet segment of one band line: one bit per group of
+ * 8 GCLIs, 1 when its maximum is <= gt (PackPrecinct.c, significance
+ * flags).  Lane L codes flags 4L .. 4L+3 of a step of 256. */
+__device__ __forceinline__ void emit_sig(WaveOut& W, const uint8_t* sl, uint32_t sw, uint32_t gt) {
+    for (uint32_t base = 0; base < sw; base += IPL * WAVE) {
+        const uint32_t i0 = base + IPL * W.lane;
+        uint32_t a[IPL];
 #pragma unroll
-        for (uint32_t k = 0; k < IPL; k++) {
-            const uint32_t idx = min(i0 + k, nitems - 1); /* clamped: every load in bounds */
-            a[k] = kind == ITEM_SIG ? gld(sl + idx) : gld(gl + idx);
-            s8[k] = (kind == ITEM_VLC && use_sig) ? gld(sl + idx / 8) : 0u;
-            cv[k] = kind == ITEM_DATA ? gld2(cl + 4 * idx) : make_uint2(0u, 0u);
-        }
+        for (uint32_t k = 0; k < IPL; k++)
+            a[k] = gld(sl + min(i0 + k, sw - 1));
         uint32_t nb = 0;
-        if (kind != ITEM_DATA) {
-            uint64_t v = 0;
+        uint64_t v = 0;
 #pragma unroll
-            for (uint32_t k = 0; k < IPL; k++) {
-                if (i0 + k >= nitems)
-                    break;
-                uint32_t len, code;
-                if (kind == ITEM_RAW) {
-                    len = 4;
-                    code = a[k];
-                } else if (kind == ITEM_SIG) {
-                    len = 1;
-                    code = a[k] <= gt ? 1u : 0u;
-                } else if (use_sig && s8[k] <= gt) {
-                    len = 0; /* group inside an insignificant significance group */
-                    code = 0;
-                } else {
-                    const uint32_t m = a[k] > gt ? a[k] - gt : 0u; /* unary: m ones and a zero */
-                    len = m + 1;
-                    code = ((1u << m) - 1) << 1;
-                }
-                v = (v << len) | code;
-                nb += len;
+        for (uint32_t k = 0; k < IPL; k++)
+            if (i0 + k < sw) {
+                v = (v << 1) | (a[k] <= gt ? 1u : 0u);
+                nb++;
             }
-            const uint32_t incl = wave_incl_scan(nb, lane);
-            if (nb && v)
-                ring_put(W.ring, W.pos + incl - nb, v, nb);
-            W.pos += wave_last(incl);
-        } else {
-            uint64_t v[IPL];
-            uint32_t len[IPL];
-#pragma unroll
-            for (uint32_t k = 0; k < IPL; k++) {
-                const uint32_t i = i0 + k, gc = a[k];
-                const bool coded = i < nitems && gc > gt;
-                const uint32_t np = coded ? gc - gt : 0u;
-                const uint32_t c[4] = {cv[k].x & 0xFFFFu, cv[k].x >> 16, cv[k].y & 0xFFFFu, cv[k].y >> 16};
-                uint32_t q[4], signs = 0;
-#pragma unroll
-                for (uint32_t e = 0; e < 4; e++) {
-                    q[e] = (coded && 4 * i + e < w) ? quant_mag(c[e], gc, gt, qpih) : 0u;
-                    signs |= (q[e] && (c[e] & 0x8000u)) ? (8u >> e) : 0u;
-                }
-                const bool any_high = __ballot(np > 8) != 0;
-                const uint64_t planes = interleave4(q[0], q[1], q[2], q[3], any_high);
-                len[k] = coded ? 4 * np + 4 : 0u;
-                v[k] = coded ? ((uint64_t)signs << (4 * np)) | planes : 0u;
-                nb += len[k];
-            }
-            const uint32_t incl = wave_incl_scan(nb, lane);
-            uint32_t at = W.pos + incl - nb;
-#pragma unroll
-            for (uint32_t k = 0; k < IPL; k++) {
-                if (len[k] && v[k])
-                    ring_put(W.ring, at, v[k], len[k]);
-                at += len[k];
-            }
-            W.pos += wave_last(incl);
-        }
-        if ((W.pos >> 5) - W.flushed >= RING_FLUSH) {
-            wave_sync();
-            ring_flush(W, W.flushed, W.pos >> 5);
-            W.flushed = W.pos >> 5;
-            wave_sync();
-        }
+        const uint32_t incl = wave_incl_scan(nb, W.lane);
+        if (nb && v)
+            W.put(W.pos + incl - nb, v, nb);
+        W.pos += wave_last(incl);
+        W.maybe_flush();
     }
 }
 
-/* One wave per packing unit: a packet header (PackHeaders.c:98-127, sign
- * size 0) or the items of one band line in one sub-packet -- significance
- * flags, bit-plane counts (unary or raw) or data bit-planes with the
- * quantisation (Quant.c:9-45) fused (PackPrecinct.c:235-572).  Every unit is
- * described by one self-contained record of the packet plan, so a wave makes
- * one descriptor load, assembles its bits in its LDS ring and writes them out
- * as they complete; waves never wait on each other. */
+/* Bit-plane-count and data segments of one band line together, 64 * N
+ * groups per step: lane L codes groups N*L .. N*L+N-1 of the step (N = 4
+ * for wide lines, 1 for lines of at most 64 groups).  Bit-plane counts are
+ * unary (gcli - gt ones and a zero, none for a group inside an
+ * insignificant significance group) or raw 4-bit GCLIs; a lane's codes are
+ * concatenated (at most 64 bits).  Data: per coded group a sign nibble,
+ * then bit-planes gcli-1 .. gt of the quantised magnitudes, 4 bits each;
+ * groups no lane codes are skipped.  The inputs of a step are loaded before
+ * any of them is coded. */
+template <uint32_t N, bool UNIFORM>
+__device__ __forceinline__ void emit_gd(WaveOut& B, WaveOut& C, const uint8_t* gl, const uint8_t* sl,
+                                        const uint16_t* cl, uint32_t w, uint32_t gw, uint32_t gt, uint32_t ms,
+                                        uint32_t raw) {
+    const uint32_t lane = B.lane;
+    for (uint32_t base = 0; base < gw; base += N * WAVE) {
+        const uint32_t i0 = base + N * lane;
+        uint32_t a[N], s8[N];
+        uint2 cv[N];
+#pragma unroll
+        for (uint32_t k = 0; k < N; k++) {
+            const uint32_t idx = min(i0 + k, gw - 1); /* clamped: every load in bounds */
+            a[k] = gld(gl + idx);
+            s8[k] = ms ? gld(sl + idx / 8) : 0u;
+            cv[k] = gld2(cl + 4 * idx);
+        }
+        /* bit-plane counts */
+        uint32_t nb = 0;
+        uint64_t v = 0;
+#pragma unroll
+        for (uint32_t k = 0; k < N; k++) {
+            if (i0 + k >= gw)
+                break;
+            uint32_t len, code;
+            if (raw) {
+                len = 4;
+                code = a[k];
+            } else if (ms && s8[k] <= gt) {
+                len = 0;
+                code = 0;
+            } else {
+                const uint32_t m = a[k] > gt ? a[k] - gt : 0u;
+                len = m + 1;
+                code = ((1u << m) - 1) << 1;
+            }
+            v = (v << len) | code;
+            nb += len;
+        }
+        const uint32_t incl = wave_incl_scan(nb, lane);
+        if (nb && v)
+            B.put(B.pos + incl - nb, v, nb);
+        B.pos += wave_last(incl);
+        /* data */
+        uint64_t dv[N];
+        uint32_t dl[N], dn = 0;
+#pragma unroll
+        for (uint32_t k = 0; k < N; k++) {
+            const uint32_t i = i0 + k, gc = a[k];
+            const bool coded = i < gw && gc > gt;
+            dl[k] = 0;
+            dv[k] = 0;
+            if (__ballot(coded) == 0)
+                continue;
+            const uint32_t np = coded ? gc - gt : 0u;
+            const uint32_t c[4] = {cv[k].x & 0xFFFFu, cv[k].x >> 16, cv[k].y & 0xFFFFu, cv[k].y >> 16};
+            uint32_t q[4], signs = 0;
+#pragma unroll
+            for (uint32_t e = 0; e < 4; e++) {
+                q[e] = (coded && 4 * i + e < w) ? quant_mag<UNIFORM>(c[e], gc, gt) : 0u;
+                signs |= (q[e] && (c[e] & 0x8000u)) ? (8u >> e) : 0u;
+            }
+            const bool any_high = __ballot(np > 8) != 0;
+            const uint64_t planes = interleave4(q[0], q[1], q[2], q[3], any_high);
+            dl[k] = coded ? 4 * np + 4 : 0u;
+            dv[k] = coded ? ((uint64_t)signs << (4 * np)) | planes : 0u;
+            dn += dl[k];
+        }
+        const uint32_t dincl = wave_incl_scan(dn, lane);
+        uint32_t at = C.pos + dincl - dn;
+#pragma unroll
+        for (uint32_t k = 0; k < N; k++) {
+            if (dl[k] && dv[k])
+                C.put(at, dv[k], dl[k]);
+            at += dl[k];
+        }
+        C.pos += wave_last(dincl);
+        B.maybe_flush();
+        C.maybe_flush();
+    }
+}
+
+/* One wave per band line of a precinct (PackPrecinct.c:235-572): its
+ * significance flags, bit-plane counts (unary or raw) and data bit-planes
+ * with the quantisation (Quant.c:9-45) fused, each segment into its own
+ * sub-packet; the first band line of a packet also writes the packet
+ * header (PackHeaders.c:98-127, sign size 0).  Every unit is described by
+ * one self-contained record of the packet plan, so a wave makes one
+ * descriptor load; the GCLIs and coefficients of the line are read once.
+ * Waves never wait on each other. */
 __global__ void __launch_bounds__(64 * PK_WAVES) k_pack(const JxsUnit* __restrict__ units, uint32_t total_units) {
-    __shared__ uint32_t rings[PK_WAVES][RINGW];
+    __shared__ uint32_t ringb[PK_WAVES][RINGB];
+    __shared__ uint32_t ringc[PK_WAVES][RINGC];
     const uint32_t wid = threadIdx.x / WAVE, lane = threadIdx.x & (WAVE - 1);
     const uint32_t gu = blockIdx.x * PK_WAVES + wid;
     if (gu >= total_units)
         return;
     const CAS JxsUnit& U = ((const CAS JxsUnit*)units)[gu];
-    const uint32_t len = U.len;
-    if (len == 0)
-        return;
-    WaveOut W;
-    W.ring = rings[wid];
-    W.ob = U.ob;
-    W.lo = U.lo;
-    W.hi = W.lo + len;
-    W.pos = W.lo;
-    W.flushed = 0;
-    W.lane = lane;
-    const uint32_t words = min((W.hi + 31) / 32, (uint32_t)RINGW);
-    for (uint32_t k = lane; k < words; k += WAVE)
-        W.ring[k] = 0;
-    wave_sync();
-    const uint32_t item = U.item;
-    if (item == ITEM_HDR) {
-        if (lane == 0)
-            ring_put(W.ring, W.pos, U.hdr, len);
-        W.pos += len;
-    } else {
-        emit_line(W, (int)item, U.gl, U.sl, U.cl, U.w, U.gw, U.sw, U.gt, U.ms, U.qpih);
+    bool bad = false;
+    if (U.hlen) {
+        uint32_t w0, w1, w2;
+        split_bits(U.hdr, U.hlen, U.hlo, w0, w1, w2);
+        const uint32_t piece = lane == 0 ? w0 : lane == 1 ? w1 : w2;
+        if (lane < 3 && piece)
+            atomicOr(U.hob + lane, __builtin_bswap32(piece));
     }
-    wave_sync();
-    ring_flush(W, W.flushed, (W.hi + 31) / 32);
-    /* the plan's length is the unit's exact bit count */
-    if (W.pos != W.hi && lane == 0)
+    WaveOut B, C;
+    if (U.len[0]) {
+        B.start(ringb[wid], RINGB - 1, U.ob[0], U.lo[0], U.len[0], lane);
+        wave_sync();
+        emit_sig(B, U.sl, U.sw, U.gt);
+        B.finish();
+        bad |= B.pos != B.hi;
+    }
+    if (U.len[1] | U.len[2]) {
+        wave_sync();
+        B.start(ringb[wid], RINGB - 1, U.ob[1], U.lo[1], U.len[1], lane);
+        C.start(ringc[wid], RINGC - 1, U.ob[2], U.lo[2], U.len[2], lane);
+        wave_sync();
+        const bool narrow = U.gw <= WAVE;
+        if (U.qpih) {
+            if (narrow)
+                emit_gd<1, true>(B, C, U.gl, U.sl, U.cl, U.w, U.gw, U.gt, U.ms, U.raw);
+            else
+                emit_gd<IPL, true>(B, C, U.gl, U.sl, U.cl, U.w, U.gw, U.gt, U.ms, U.raw);
+        } else {
+            if (narrow)
+                emit_gd<1, false>(B, C, U.gl, U.sl, U.cl, U.w, U.gw, U.gt, U.ms, U.raw);
+            else
+                emit_gd<IPL, false>(B, C, U.gl, U.sl, U.cl, U.w, U.gw, U.gt, U.ms, U.raw);
+        }
+        B.finish();
+        C.finish();
+        bad |= B.pos != B.hi || C.pos != C.hi;
+    }
+    /* the plan's lengths are the segments' exact bit counts */
+    if (bad && lane == 0)
         atomicOr(U.status, 2u);
 }
 
