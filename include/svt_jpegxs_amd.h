@@ -45,8 +45,8 @@ PREFIX_API SvtJxsErrorType_t svt_jpeg_xs_amd_plan_run(svt_jpeg_xs_amd_plan_t* pl
 /* Same as plan_run without a graph, recording a HIP event around every
  * kernel on `stream`; waits and returns the milliseconds of each stage in
  * stage_ms[0 .. SVT_JPEG_XS_AMD_STAGES - 1] (transform level 1, transform
- * level 2, rate table, select, packet plan, pack). */
-#define SVT_JPEG_XS_AMD_STAGES 6
+ * level 2, rate control and packet plan, pack). */
+#define SVT_JPEG_XS_AMD_STAGES 4
 PREFIX_API SvtJxsErrorType_t svt_jpeg_xs_amd_plan_run_timed(svt_jpeg_xs_amd_plan_t* plan, void* stream, float* stage_ms);
 
 /* After the stream has completed: per-frame error flags (0 = ok), count

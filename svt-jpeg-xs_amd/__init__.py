@@ -20,7 +20,7 @@ LIB_PATH = os.path.join(HERE, "libsvtjpegxs_amd.so")
 API_MAJOR, API_MINOR = 0, 10
 
 # Stages timed by Plan.run_timed (SVT_JPEG_XS_AMD_STAGES, include/svt_jpegxs_amd.h)
-STAGE_NAMES = ["dwt1", "dwt2", "rc_table", "select", "plan", "pack"]
+STAGE_NAMES = ["dwt1", "dwt2", "rate", "pack"]
 STAGES = len(STAGE_NAMES)
 
 # SvtJxsErrorType_t values (Source/API/SvtJpegxs.h:130-159)

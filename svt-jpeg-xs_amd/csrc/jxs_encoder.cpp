@@ -20,9 +20,7 @@
 
 extern "C" {
 hipError_t jxs_launch_dwt(const JxsDwtJob*, const uint32_t*, int, uint32_t, uint32_t, hipStream_t);
-hipError_t jxs_launch_rc_table(const JxsFrameDesc*, int, uint32_t, hipStream_t);
-hipError_t jxs_launch_select(const JxsFrameDesc*, int, uint32_t, hipStream_t);
-hipError_t jxs_launch_plan(const JxsFrameDesc*, int, uint32_t, hipStream_t);
+hipError_t jxs_launch_rate(const JxsFrameDesc*, int, uint32_t, hipStream_t);
 hipError_t jxs_launch_pack(const JxsUnit*, uint32_t, hipStream_t);
 hipError_t jxs_launch_zero(uint32_t*, uint32_t, hipStream_t);
 }
@@ -76,9 +74,7 @@ static uint64_t frame_work_bytes(const JxsGeom& g) {
     b += align256(g.n_smax);
     b += align256(g.n_i32 * 4);
     b += align256((uint64_t)g.rows * g.n_bl * JXS_HREC * 2);
-    b += align256((uint64_t)g.rows * JXS_NQ * JXS_NR * 4);
     b += align256((uint64_t)g.rows * sizeof(JxsPrec));
-    b += align256((uint64_t)g.rows * sizeof(JxsPlan));
     return b;
 }
 
@@ -207,12 +203,8 @@ static svt_jpeg_xs_amd_plan_t* plan_build(const svt_jpeg_xs_amd_frame_spec_t* sp
         cur += align256(g.n_i32 * 4);
         F.hist = (uint16_t*)cur;
         cur += align256((uint64_t)g.rows * g.n_bl * JXS_HREC * 2);
-        F.table = (uint32_t*)cur;
-        cur += align256((uint64_t)g.rows * JXS_NQ * JXS_NR * 4);
         F.prec = (JxsPrec*)cur;
         cur += align256((uint64_t)g.rows * sizeof(JxsPrec));
-        F.plan = (JxsPlan*)cur;
-        cur += align256((uint64_t)g.rows * sizeof(JxsPlan));
         F.out = specs[i].d_out;
         F.status = p->d_status + i;
         F.rows = g.rows;
@@ -221,6 +213,9 @@ static svt_jpeg_xs_amd_plan_t* plan_build(const svt_jpeg_xs_amd_frame_spec_t* sp
         F.slice_base = slice_base;
         F.unit_base = unit_base;
         F.n_units = g.n_units;
+        F.pps = g.pps;
+        F.slices = g.slices;
+        F.n_bl = g.n_bl;
         prec_base += g.rows;
         slice_base += g.slices;
         unit_base += g.rows * g.n_units;
@@ -277,10 +272,8 @@ static hipError_t plan_launch(svt_jpeg_xs_amd_plan_t* p, hipStream_t st, hipEven
         return e;
     STEP(0, jxs_launch_dwt(p->d_dwt[0], p->d_dbase[0], p->ndwt[0], p->dblocks[0], p->dwmax[0], st));
     STEP(1, jxs_launch_dwt(p->d_dwt[1], p->d_dbase[1], p->ndwt[1], p->dblocks[1], p->dwmax[1], st));
-    STEP(2, jxs_launch_rc_table(p->d_frames, nf, p->total_prec, st));
-    STEP(3, jxs_launch_select(p->d_frames, nf, p->total_slices, st));
-    STEP(4, jxs_launch_plan(p->d_frames, nf, p->total_prec, st));
-    STEP(5, jxs_launch_pack(p->d_units, p->total_units, st));
+    STEP(2, jxs_launch_rate(p->d_frames, nf, p->total_slices, st));
+    STEP(3, jxs_launch_pack(p->d_units, p->total_units, st));
     if (ev)
         (void)hipEventRecord(ev[SVT_JPEG_XS_AMD_STAGES], st);
 #undef STEP

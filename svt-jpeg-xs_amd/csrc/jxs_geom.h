@@ -22,7 +22,7 @@
 #define JXS_NQ 32   /* quantisation values 0..31 */
 #define JXS_NR 32   /* refinement values 0..31 (<= bands - 1) */
 #define JXS_HDR_MAX 256
-#define JXS_NU 256 /* packing units (band lines) per precinct */
+#define JXS_NU 64  /* packing units (band lines) per precinct: one per lane of a wave */
 #define JXS_MAX_ROW 7680 /* widest component row the row kernels stage in LDS */
 
 /* Band-line histogram record (u16 units), written by the transform for
@@ -69,13 +69,40 @@ struct JxsGeom {
     uint32_t pk_bs[JXS_NPK], pk_be[JXS_NPK], pk_line[JXS_NPK];
     uint32_t raw_bytes[2][JXS_NPK];
     /* packing units of a precinct: the band lines in packet order (u_p
-     * packet, u_gi global band) */
+     * packet, u_gi global band, u_info the unit word below) */
     uint32_t n_units, pad_u[3];
-    uint8_t u_p[JXS_NU], u_gi[JXS_NU], pad_v[JXS_NU];
+    uint8_t u_p[JXS_NU], u_gi[JXS_NU];
+    uint32_t u_info[JXS_NU];
+    uint32_t ln_info[JXS_NBL]; /* per band line of a precinct, the line word below */
+    /* the rate control's walk over the band lines present in a normal / the
+     * last precinct row, in packet order: walk words (below) and the RAW
+     * bytes of each line's packet; zero past the end */
+    uint32_t rq[2][JXS_NU], rq_raw[2][JXS_NU];
+    uint32_t rq_n[2], pad_rq[2];
     uint64_t n_coeff, n_gcli, n_smax; /* buffer sizes per frame */
     uint64_t n_i32;                   /* int32 transform scratch per frame */
     uint8_t hdr[JXS_HDR_MAX];
 };
+
+/* Unit word (u_info): band line inside the precinct (bits 0-7), band gain
+ * (8-15) and priority (16-23), line present in a normal / in the last
+ * precinct row (24, 25), last band line of its packet (26), packet (27-31). */
+#define UI_NORMAL 24
+#define UI_LAST 26
+#define UI_PKT 27
+/* Walk word (rq): the line's offset in a rate table, line * JXS_RT_STRIDE
+ * (bits 0-10), band gain (11-18) and priority (19-26), last present line of
+ * its packet (27). */
+#define JXS_RT_STRIDE 17 /* words per band line of a rate table: truncations 0..15, then a zero */
+#define RQ_GAIN 11
+#define RQ_PRIO 19
+#define RQ_END 27
+/* Line word (ln_info): groups of 4 coefficients (bits 0-11), second line of
+ * its band (12), lines of its band in a normal / the last precinct row
+ * (13-14, 15-16), significance groups (17-24). */
+#define LI_SECOND 12
+#define LI_NL 13
+#define LI_SW 17
 
 /* kernels stage the geometry in LDS with 16-byte copies */
 static_assert(sizeof(JxsGeom) % 16 == 0, "JxsGeom size must be a multiple of 16 bytes");
@@ -85,17 +112,9 @@ struct JxsPrec {
     uint32_t q, r, total, pad, offset, err;
 };
 
-/* Packet plan of one precinct (after rate control): byte offset of every
- * packet in the codestream and the byte sizes of its sub-packets. */
+/* Packet flags of the packet plan: the packet has a band line, and its
+ * significance / bit-plane-count sub-packets are sent RAW. */
 enum { PKT_ANY = 1u, PKT_RAW = 2u };
-struct JxsPkt {
-    uint32_t off, s, g, d, flags;
-};
-struct JxsPlan {
-    JxsPkt pk[JXS_NPK];
-    uint8_t band[JXS_NGB]; /* per global band: truncation | (significance coding << 4) */
-};
-
 
 /* One packing unit = one band line of one precinct, self-contained
  * (written by the packet plan, read by the packing kernel with a single
@@ -155,9 +174,7 @@ struct JxsFrameDesc {
     uint8_t* gcli;
     uint8_t* smax;
     uint16_t* hist;  /* rows x n_bl x JXS_HREC */
-    uint32_t* table; /* rows x JXS_NQ x JXS_NR */
     JxsPrec* prec;   /* rows */
-    JxsPlan* plan;   /* rows */
     JxsUnit* units;  /* rows x n_units packing units */
     uint8_t* out;    /* codestream */
     uint32_t* status;
@@ -166,6 +183,7 @@ struct JxsFrameDesc {
     uint32_t slice_base;
     uint32_t unit_base; /* first packing unit (rows x n_units numbering) of this frame in the launch */
     uint32_t n_units;
+    uint32_t pps, slices, n_bl, pad_; /* copies of the geometry's, readable before it is staged */
 };
 
 #endif

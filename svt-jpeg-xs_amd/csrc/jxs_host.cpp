@@ -194,6 +194,56 @@ int build_layout(JxsGeom& g, uint32_t nc, const uint32_t sx[4], const uint32_t s
         g.pk_exist[t] = t == 0 ? g.npk : exist;
         g.rc_hdr_bytes[t] = g.prec_hdr_bytes + div_up(g.pkt_hdr_bytes * 8 * g.pk_exist[t], 8);
     }
+    if (g.n_bl > JXS_NBL)
+        return -1;
+    return 0;
+}
+
+/* The rate control's unit and line words (once gains and priorities are
+ * known): every band line of a precinct carries its band's parameters so a
+ * wave can walk the lines in packet order from registers. */
+static int unit_words(JxsGeom& g) {
+    memset(g.u_info, 0, sizeof(g.u_info));
+    memset(g.ln_info, 0, sizeof(g.ln_info));
+    for (uint32_t u = 0; u < g.n_units; u++) {
+        const uint32_t p = g.u_p[u], gi = g.u_gi[u], l = g.pk_line[p];
+        if (g.gb[gi] == NE)
+            return -1;
+        const JxsBand& bd = g.c[g.gc[gi]].b[g.gb[gi]];
+        const uint32_t line = bd.bl + (l < bd.hl ? l : 0);
+        const uint32_t last = u + 1 == g.n_units || g.u_p[u + 1] != p ? 1u : 0u;
+        if (bd.gain > 255 || bd.prio > 255 || line >= g.n_bl)
+            return -1;
+        g.u_info[u] = line | (bd.gain << 8) | (bd.prio << 16) | ((l < bd.lines[0] ? 1u : 0u) << UI_NORMAL) |
+                      ((l < bd.lines[1] ? 1u : 0u) << (UI_NORMAL + 1)) | (last << UI_LAST) | (p << UI_PKT);
+    }
+    memset(g.rq, 0, sizeof(g.rq));
+    memset(g.rq_raw, 0, sizeof(g.rq_raw));
+    for (uint32_t t = 0; t < 2; t++) {
+        uint32_t n = 0, prev_p = ~0u;
+        for (uint32_t u = 0; u < g.n_units; u++) {
+            const uint32_t x = g.u_info[u];
+            if (!((x >> (UI_NORMAL + t)) & 1u))
+                continue;
+            const uint32_t p = x >> UI_PKT, line = x & 0xFFu, gain = (x >> 8) & 0xFFu, prio = (x >> 16) & 0xFFu;
+            if (n > 0 && prev_p == p)
+                g.rq[t][n - 1] &= ~(1u << RQ_END);
+            g.rq[t][n] = line * JXS_RT_STRIDE | (gain << RQ_GAIN) | (prio << RQ_PRIO) | (1u << RQ_END);
+            g.rq_raw[t][n] = g.raw_bytes[t][p];
+            prev_p = p;
+            n++;
+        }
+        g.rq_n[t] = n;
+    }
+    for (uint32_t c = 0; c < g.nc; c++)
+        for (uint32_t b = 0; b < g.c[c].nb; b++) {
+            const JxsBand& bd = g.c[c].b[b];
+            if (bd.gw >= 4096 || bd.sw >= 256 || bd.lines[0] > 2 || bd.lines[1] > 2)
+                return -1;
+            for (uint32_t k = 0; k < bd.hl; k++)
+                g.ln_info[bd.bl + k] = bd.gw | (k << LI_SECOND) | (bd.lines[0] << LI_NL) | (bd.lines[1] << (LI_NL + 2)) |
+                                       (bd.sw << LI_SW);
+        }
     return 0;
 }
 
@@ -490,6 +540,8 @@ int32_t build_encoder_geometry(const EncParams& p, HostGeom& out, int verbose) {
             bd.gain = t.g[bd.gid];
             bd.prio = t.p[bd.gid];
         }
+    if (unit_words(g) != 0)
+        return bad("band layout exceeds the rate control's unit words");
     uint32_t mr = 0;
     for (uint32_t c = 0; c < nc; c++)
         for (uint32_t b = 0; b < g.c[c].nb; b++)

@@ -12,17 +12,17 @@
  *              shuffles for the maxima, and the histogram record of the band
  *              line (GCLI and significance-maximum counts, from wave ballots)
  *              the rate control needs (RateControl.c:18-154).
- *   k_rc_table one block per precinct: from the histogram records, the
- *              packet-level byte count of the precinct for every (Q, R)
- *              (RateControl.c:457-781).
- *   k_select   one wave per slice: precinct budgets with the +10 % first
- *              precinct and the padding carry (PackStageProcess.c:600-684),
- *              the reference's binary searches replayed on the LDS-staged
- *              tables (RateControl.c:889-1083, BinarySearch.c), precinct
- *              offsets.
- *   k_plan     one wave per precinct: per-band truncation and coding method,
- *              packet sizes and offsets, slice / precinct headers, padding,
- *              end marker and frame header.
+ *   k_rate     one block per slice, one wave per precinct: the precinct's
+ *              rate table (bits per band line and truncation) from the
+ *              histogram records (RateControl.c:457-781), then, precinct
+ *              after precinct in one wave, the budgets with the +10 % first
+ *              precinct and the padding carry (PackStageProcess.c:600-684)
+ *              and the reference's binary searches of Q and R
+ *              (RateControl.c:889-1083, BinarySearch.c) on the table
+ *              entries they visit, then per precinct the band truncations
+ *              and coding methods, packet sizes and offsets, the packing
+ *              units, slice / precinct headers, padding, end marker and
+ *              frame header.
  *   k_pack     one wave per band line of a precinct: its significance,
  *              bit-plane-count and data segments (PackPrecinct.c:235-572)
  *              with quantisation (Quant.c:9-45) fused, each at the bit
@@ -67,6 +67,45 @@ __device__ __forceinline__ uint2 gld2(const void* p) {
     return make_uint2((uint32_t)v, (uint32_t)(v >> 32));
 }
 __device__ __forceinline__ void gst2(void* p, uint2 v) { *(GAS uint64_t*)p = ((uint64_t)v.y << 32) | v.x; }
+/* A struct stored to global memory as 16-byte (or 8-byte) words: assigning
+ * the struct through a global pointer would go through a generic-address
+ * copy, i.e. flat stores. */
+template <typename T>
+__device__ __forceinline__ void gst_words(T* p, const T& v) {
+    if constexpr (sizeof(T) % 16 == 0 && alignof(T) >= 8) {
+        typedef uint32_t w4 __attribute__((ext_vector_type(4)));
+        const w4* s = (const w4*)&v;
+        GAS w4* d = (GAS w4*)p;
+#pragma unroll
+        for (uint32_t i = 0; i < sizeof(T) / 16; i++)
+            d[i] = s[i];
+    } else {
+        static_assert(sizeof(T) % 8 == 0, "8-byte multiple");
+        const uint64_t* s = (const uint64_t*)&v;
+        GAS uint64_t* d = (GAS uint64_t*)p;
+#pragma unroll
+        for (uint32_t i = 0; i < sizeof(T) / 8; i++)
+            d[i] = s[i];
+    }
+}
+/* A read-only struct loaded through the constant address space (scalar
+ * loads), word by word. */
+template <typename T>
+__device__ __forceinline__ T cld_words(const T* p) {
+    static_assert(sizeof(T) % 4 == 0, "4-byte multiple");
+    T v;
+    const CAS uint32_t* s = (const CAS uint32_t*)p;
+    uint32_t* d = (uint32_t*)&v;
+#pragma unroll
+    for (uint32_t i = 0; i < sizeof(T) / 4; i++)
+        d[i] = s[i];
+    return v;
+}
+/* Device-scope atomic OR into global memory (a global, not a flat, atomic:
+ * it does not hold up the wave's later LDS accesses). */
+__device__ __forceinline__ void gor(uint32_t* p, uint32_t v) {
+    (void)__hip_atomic_fetch_or((GAS uint32_t*)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
 
 __device__ __forceinline__ int32_t load_src(const void* src, uint32_t kind, size_t idx, uint32_t shift, uint32_t mask,
                                             int32_t offset) {
@@ -493,364 +532,65 @@ __global__ void __launch_bounds__(256) k_dwt(const JxsDwtJob* __restrict__ jobs,
     }
 }
 
-/* Copies a frame's geometry into LDS (all threads of the block take part;
- * the caller synchronises).  The rate-control kernels index it with
- * data-dependent band and packet numbers. */
+/* Copies a frame's geometry into LDS (all GEOM_THREADS threads of the block
+ * take part; the caller synchronises).  The rate-control kernel indexes it
+ * with data-dependent band and packet numbers. */
+#define GEOM_THREADS 256 /* block size of the kernels that stage the geometry */
 __device__ __forceinline__ void stage_geom(JxsGeom* dst, const JxsGeom* src) {
-    const uint4* s = (const uint4*)src;
-    uint4* d = (uint4*)dst;
+    const u32x4* s = (const u32x4*)src; /* read through global, not flat, loads */
+    u32x4* d = (u32x4*)dst;
     constexpr uint32_t n = sizeof(JxsGeom) / 16;
-    /* blocks of 64 threads or more: at most 5 copies per thread, loaded
-     * together */
-    const uint32_t t = threadIdx.x, bd = blockDim.x;
+    constexpr uint32_t per = (n + GEOM_THREADS - 1) / GEOM_THREADS; /* copies per thread, loaded together */
+    const uint32_t t = threadIdx.x;
     /* indexes clamped into the struct: every load is in bounds, only the
      * stores are predicated */
-    const uint4 v0 = s[min(t, n - 1)];
-    const uint4 v1 = s[min(t + bd, n - 1)];
-    const uint4 v2 = s[min(t + 2 * bd, n - 1)];
-    const uint4 v3 = s[min(t + 3 * bd, n - 1)];
-    const uint4 v4 = s[min(t + 4 * bd, n - 1)];
-    if (t < n)
-        d[t] = v0;
-    if (t + bd < n)
-        d[t + bd] = v1;
-    if (t + 2 * bd < n)
-        d[t + 2 * bd] = v2;
-    if (t + 3 * bd < n)
-        d[t + 3 * bd] = v3;
-    if (t + 4 * bd < n)
-        d[t + 4 * bd] = v4;
+    u32x4 v[per];
+#pragma unroll
+    for (uint32_t j = 0; j < per; j++)
+        v[j] = gld(s + min(t + j * GEOM_THREADS, n - 1));
+#pragma unroll
+    for (uint32_t j = 0; j < per; j++)
+        if (t + j * GEOM_THREADS < n)
+            d[t + j * GEOM_THREADS] = v[j];
 }
 
 /* ------------------------------------------------------------------ */
 /* rate control                                                        */
 /* ------------------------------------------------------------------ */
-__device__ __forceinline__ int find_frame(const JxsFrameDesc* fr, int nfr, uint32_t blk) {
-    int lo = 0, hi = nfr - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (fr[mid].prec_base <= blk)
-            lo = mid;
+/* The reference's binary search (BinarySearch.c:9-78) over [0, end], replayed
+ * on the outcome of every probe given at once as bit masks: probing v
+ * answers "too small" when bit v of m1 is set, "out of range" when bit v of
+ * m3 is set, "too big" otherwise.  `below` keeps the largest "too small"
+ * probe as the answer, otherwise the smallest "too big" one; the first probe
+ * is at step0 (or end / 2).  Uniform values only: the loop runs on the scalar
+ * unit. */
+__device__ __forceinline__ bool bsearch_masks(uint32_t m1, uint32_t m3, int32_t end, bool below, int32_t step0,
+                                              uint32_t& out) {
+    int32_t beg = 0, best = -1;
+    int32_t step = step0 ? step0 : end / 2;
+    if (step > end - beg)
+        step = (end - beg + 1) / 2;
+    int32_t last = step + beg; /* beg <= last <= end <= 31 while the search runs */
+    for (;;) {
+        const bool oor = (m3 >> last) & 1u;
+        const bool small = !oor && ((m1 >> last) & 1u);
+        const bool big = !oor && !small;
+        if ((small && below && (best == -1 || best < last)) || (big && !below && (best == -1 || best > last)))
+            best = last;
+        if (small ? end < last + 1 : beg > last - 1)
+            break;
+        if (small)
+            beg = last + 1;
         else
-            hi = mid - 1;
-    }
-    return lo;
-}
-
-/* Bit counts of one band line at truncation t from its histogram record:
- * the GCLI sub-packet without significance (one unary code per group,
- * RateControl.c:18-63), the groups a significance flag removes
- * (RateControl.c:65-154) and the data bit-planes with signs
- * (RateControl.c:457-520). */
-struct LineBits {
-    uint32_t ns, zeroed, data;
-};
-__device__ __forceinline__ LineBits line_bits(const uint16_t* rec, uint32_t t, uint32_t gw) {
-    uint32_t cnt = 0, data = 0, sig = 0;
-#pragma unroll
-    for (uint32_t v = 0; v < 16; v++) {
-        const uint32_t h = rec[v];
-        cnt += v <= t ? h : 0u;
-        data += v > t ? (v - t + 1) * h : 0u;
-    }
-    const uint32_t full = gw / 8;
-    if (t >= 15) {
-        sig = full;
-    } else {
-#pragma unroll
-        for (uint32_t v = 0; v < 15; v++)
-            sig += v <= t ? (uint32_t)rec[16 + v] : 0u;
-    }
-    const uint32_t tail = rec[HREC_TAIL];
-    LineBits b;
-    b.ns = data + cnt;
-    b.zeroed = sig * 8 + ((tail != 0xFFFFu && tail <= t) ? gw - full * 8 : 0u);
-    b.data = 4 * data;
-    return b;
-}
-
-/* One block per precinct: the packet-level byte count of the precinct for
- * every (Q, R) (precinct_get_budget_bytes with the per-band method choice of
- * rate_control_calculate_band_best_method, RateControl.c:457-781), from the
- * band-line histograms the transform wrote.  Per band line and truncation the
- * bit counts are tabulated once in LDS; each (Q, R) entry then sums the
- * packets' bands. */
-__global__ void __launch_bounds__(256) k_rc_table(const JxsFrameDesc* __restrict__ frames, int nfr) {
-    __shared__ uint16_t rec[JXS_NBL][JXS_HREC];
-    __shared__ uint32_t t_ns[JXS_NBL][16], t_zr[JXS_NBL][16], t_d[JXS_NBL][16], t_g[JXS_NBL][16], t_s[JXS_NBL][16];
-    __shared__ uint8_t bl_band[JXS_NBL];
-    __shared__ uint32_t b_gain[JXS_NGB], b_prio[JXS_NGB], b_bl[JXS_NGB], b_nl[JXS_NGB], b_gw[JXS_NGB], b_sw[JXS_NGB];
-    const int f = find_frame(frames, nfr, blockIdx.x);
-    const JxsFrameDesc& F = frames[f];
-    __shared__ __align__(16) JxsGeom sg;
-    stage_geom(&sg, F.g);
-    __syncthreads();
-    const JxsGeom& g = sg;
-    const uint32_t row = blockIdx.x - F.prec_base;
-    const uint32_t type = row + 1 >= g.rows ? 1 : 0;
-    const uint32_t nbl = g.n_bl;
-    for (uint32_t gi = threadIdx.x; gi < g.nb_all; gi += blockDim.x) {
-        uint32_t nl = 0;
-        if (g.gb[gi] != 0xFF) {
-            const JxsBand& bd = g.c[g.gc[gi]].b[g.gb[gi]];
-            nl = bd.lines[type];
-            b_gain[gi] = bd.gain;
-            b_prio[gi] = bd.prio;
-            b_bl[gi] = bd.bl;
-            b_gw[gi] = bd.gw;
-            b_sw[gi] = bd.sw;
-            for (uint32_t k = 0; k < bd.hl; k++)
-                bl_band[bd.bl + k] = (uint8_t)gi;
-        }
-        b_nl[gi] = nl;
-    }
-    const uint32_t* src = (const uint32_t*)(F.hist + (size_t)row * nbl * JXS_HREC);
-    {
-        /* nbl * 16 <= 1024 words: four loads per thread, issued together */
-        uint32_t v[4];
-        const uint32_t words = nbl * JXS_HREC / 2;
-#pragma unroll
-        for (uint32_t j = 0; j < 4; j++) {
-            const uint32_t i = j * blockDim.x + threadIdx.x;
-            v[j] = gld(src + min(i, words - 1)); /* clamped: every load in bounds */
-        }
-#pragma unroll
-        for (uint32_t j = 0; j < 4; j++) {
-            const uint32_t i = j * blockDim.x + threadIdx.x;
-            if (i < words)
-                ((uint32_t*)&rec[0][0])[i] = v[j];
-        }
-    }
-    __syncthreads();
-    for (uint32_t i = threadIdx.x; i < nbl * 16; i += blockDim.x) {
-        const uint32_t bl = i >> 4, t = i & 15;
-        const LineBits b = line_bits(rec[bl], t, b_gw[bl_band[bl]]);
-        t_ns[bl][t] = b.ns;
-        t_zr[bl][t] = b.zeroed;
-        t_d[bl][t] = b.data;
-    }
-    __syncthreads();
-    for (uint32_t i = threadIdx.x; i < g.nb_all * 16; i += blockDim.x) {
-        const uint32_t gi = i >> 4, t = i & 15, nl = b_nl[gi];
-        if (nl == 0)
-            continue;
-        const uint32_t bl = b_bl[gi], sw = b_sw[gi];
-        uint32_t nosig = 0, sig = sw * nl;
-        for (uint32_t k = 0; k < nl; k++) {
-            nosig += t_ns[bl + k][t];
-            sig += t_ns[bl + k][t] - t_zr[bl + k][t];
-        }
-        const bool use = g.sig && nosig > sig;
-        for (uint32_t k = 0; k < nl; k++) {
-            t_g[bl + k][t] = use ? t_ns[bl + k][t] - t_zr[bl + k][t] : t_ns[bl + k][t];
-            t_s[bl + k][t] = use ? sw : 0u;
-        }
-    }
-    __syncthreads();
-    uint32_t* tab = F.table + (size_t)row * JXS_NQ * JXS_NR;
-    const uint32_t nq = g.maxq + 1, nr = g.maxr + 1, npk = g.npk;
-    for (uint32_t idx = threadIdx.x; idx < nq * nr; idx += blockDim.x) {
-        const uint32_t q = idx / nr, r = idx % nr;
-        uint32_t total = 0;
-        for (uint32_t p = 0; p < npk; p++) {
-            const uint32_t l = g.pk_line[p];
-            uint32_t d = 0, gb = 0, sb = 0;
-            for (uint32_t gi = g.pk_bs[p]; gi < g.pk_be[p]; gi++) {
-                if (l >= b_nl[gi])
-                    continue;
-                const uint32_t gt = truncation(b_gain[gi], b_prio[gi], q, r), bl = b_bl[gi] + l;
-                d += t_d[bl][gt];
-                gb += t_g[bl][gt];
-                sb += t_s[bl][gt];
-            }
-            const uint32_t db = (d + 7) >> 3, gbb = (gb + 7) >> 3, sbb = (sb + 7) >> 3;
-            const uint32_t raw = g.raw_bytes[type][p];
-            total += db + (sbb + gbb > raw ? raw : sbb + gbb);
-        }
-        tab[q * JXS_NR + r] = total;
-    }
-}
-
-/* BinarySearch.c:9-78 */
-struct BSearch {
-    int32_t beg, end, step, best, last;
-    int below;
-    __device__ void init(int32_t b, int32_t e, int bl, int32_t st) {
-        beg = b;
-        end = e;
-        below = bl;
-        best = -1;
-        last = -1;
-        step = st ? st : (end - beg) / 2;
-    }
-    /* res: 0 begin, 1 too small, 2 too big, 3 out of range.
-     * returns 0 continue, 1 found, 2 error */
-    __device__ int next(int res, uint32_t* out) {
-        if (res == 1) {
-            if (below && (best == -1 || best < last))
-                best = last;
-            if (end >= last + 1)
-                beg = last + 1;
-            else if (best == -1)
-                return 2;
-            else {
-                *out = (uint32_t)best;
-                return 1;
-            }
-        } else if (res == 2 || res == 3) {
-            if (res == 2 && !below && (best == -1 || best > last))
-                best = last;
-            if (beg <= last - 1)
-                end = last - 1;
-            else if (best == -1)
-                return 2;
-            else {
-                *out = (uint32_t)best;
-                return 1;
-            }
-        }
+            end = last - 1;
         if (step > end - beg)
             step = (end - beg + 1) / 2;
         last = step + beg;
-        *out = (uint32_t)last;
-        return 0;
     }
-};
-
-#define SEL_LDS 8192 /* u32 of staged rate tables per slice block */
-
-/* One block per slice: the slice's rate tables are staged in LDS (as many
- * precincts at a time as fit), then thread 0 walks the precincts in order:
- * budgets with the +10 % first precinct and the padding carry
- * (PackStageProcess.c:600-684), the binary searches of Q then R
- * (RateControl.c:889-1083) and the precinct offsets. */
-#define SEL_THREADS 256
-__global__ void __launch_bounds__(SEL_THREADS) k_select(const JxsFrameDesc* __restrict__ frames, int nfr,
-                                                        uint32_t total_slices) {
-    __shared__ uint32_t tabs[SEL_LDS];
-    const uint32_t gs = blockIdx.x;
-    if (gs >= total_slices)
-        return;
-    int lo = 0, hi = nfr - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (frames[mid].slice_base <= gs)
-            lo = mid;
-        else
-            hi = mid - 1;
-    }
-    const JxsFrameDesc& F = frames[lo];
-    __shared__ __align__(16) JxsGeom sg;
-    stage_geom(&sg, F.g);
-    __syncthreads();
-    const JxsGeom& g = sg;
-    const uint32_t s = gs - F.slice_base;
-    const uint32_t first_row = s * g.pps;
-    const uint32_t nprec = (s == g.slices - 1) ? g.rows - first_row : g.pps;
-    const uint32_t sbud = F.slice_bud[s];
-    uint32_t minb = sbud / nprec;
-    uint32_t first = minb * 110 / 100;
-    if (first > sbud)
-        first = sbud;
-    uint32_t left = 0;
-    if (nprec > 1) {
-        minb = (sbud - first) / (nprec - 1);
-        left = sbud - minb * (nprec - 1) - first;
-    }
-    const uint32_t per = (g.maxq + 1) * JXS_NR; /* staged words per precinct */
-    const uint32_t chunk = SEL_LDS / per;       /* >= 8 */
-    uint32_t carry = 0, off = F.slice_off[s] + 6;
-    bool failed = false;
-    for (uint32_t i0 = 0; i0 < nprec; i0 += chunk) {
-        const uint32_t n = min(chunk, nprec - i0);
-        const uint32_t words = n * per;
-        const uint32_t* src = F.table + (size_t)(first_row + i0) * JXS_NQ * JXS_NR;
-        __syncthreads();
-        /* consecutive precincts' tables are JXS_NQ * JXS_NR words apart */
-        for (uint32_t b0 = 0; b0 < words; b0 += blockDim.x * 8) {
-            uint32_t v[8];
-#pragma unroll
-            for (uint32_t j = 0; j < 8; j++) {
-                const uint32_t idx = b0 + j * blockDim.x + threadIdx.x;
-                const uint32_t ic = min(idx, words - 1); /* clamped: every load in bounds */
-                v[j] = gld(src + (size_t)(ic / per) * JXS_NQ * JXS_NR + ic % per);
-            }
-#pragma unroll
-            for (uint32_t j = 0; j < 8; j++) {
-                const uint32_t idx = b0 + j * blockDim.x + threadIdx.x;
-                if (idx < words)
-                    tabs[idx] = v[j];
-            }
-        }
-        __syncthreads();
-        for (uint32_t i = i0; i < i0 + n && threadIdx.x == 0; i++) {
-            const uint32_t row = first_row + i, type = row + 1 >= g.rows ? 1 : 0;
-            JxsPrec P;
-            P.q = P.r = P.total = P.pad = 0;
-            P.offset = off;
-            P.err = 1;
-            if (failed) {
-                F.prec[row] = P;
-                continue;
-            }
-            const uint32_t budget = (i == 0 ? first : minb) + (i < left ? 1 : 0) + carry;
-            const uint32_t hdr = g.rc_hdr_bytes[type];
-            const uint32_t* tab = tabs + (i - i0) * per;
-            bool ok = budget <= 1048575u && budget > hdr;
-            uint32_t q = 0, r = 0, data = 0;
-            if (ok) {
-                const uint32_t avail = budget - hdr;
-                BSearch bs;
-                bs.init(0, (int32_t)g.maxq, 0, 6 > g.maxq ? 0 : 6);
-                int res, nx = 0;
-                while ((res = bs.next(nx, &q)) == 0) {
-                    if ((g.empty_mask[q] & 1u) != 0) {
-                        nx = 3;
-                        continue;
-                    }
-                    nx = tab[q * JXS_NR] > avail ? 1 : 2;
-                }
-                ok = res == 1;
-                if (ok) {
-                    bs.init(0, (int32_t)g.maxr, 1, 6 >= g.maxr ? 0 : 6);
-                    nx = 0;
-                    while ((res = bs.next(nx, &r)) == 0) {
-                        if ((g.empty_mask[q] >> r) & 1u) {
-                            nx = 3;
-                            continue;
-                        }
-                        nx = tab[q * JXS_NR + r] <= avail ? 1 : 2;
-                    }
-                    ok = res == 1;
-                    if (ok) {
-                        data = tab[q * JXS_NR + r];
-                        uint32_t pad = avail - data, total = budget;
-                        if (i + 1 < nprec) {
-                            total -= pad;
-                            carry = pad;
-                            pad = 0;
-                        }
-                        P.q = q;
-                        P.r = r;
-                        P.total = total;
-                        P.pad = pad;
-                        P.err = 0;
-                        off += total;
-                    }
-                }
-            }
-            if (!ok) {
-                failed = true;
-                atomicOr(F.status, 1u);
-            }
-            F.prec[row] = P;
-        }
-    }
+    out = (uint32_t)best;
+    return best != -1;
 }
 
-/* ------------------------------------------------------------------ */
-/* packet plan                                                         */
-/* ------------------------------------------------------------------ */
 /* Inclusive sum over the 64 lanes of a wave with DPP moves: shifts by 1, 2,
  * 4, 8 inside each row of 16 lanes, then the row totals broadcast from lane
  * 15 into rows 1 and 3 and from lane 31 into rows 2 and 3.  Every lane of
@@ -873,223 +613,326 @@ __device__ __forceinline__ uint32_t wave_last(uint32_t v) {
     return (uint32_t)__builtin_amdgcn_readlane((int)v, WAVE - 1);
 }
 
-/* One single-wave block per precinct, after rate control: truncation and
- * coding method of every band for the chosen (Q, R), the sub-packet sizes,
- * RAW choice and codestream offset of every packet (PackPrecinct.c:235-572,
- * RateControl.c:457-781), and the bytes around the packets: slice header
- * (PackHeaders.c:142-156), precinct header (PackHeaders.c:158-196),
- * padding, end-of-codestream marker and, for the first precinct, the frame
- * header.  A precinct whose packets would not add up to the size the rate
- * control chose is flagged and nothing of it is written. */
-__global__ void __launch_bounds__(64) k_plan(const JxsFrameDesc* __restrict__ frames, int nfr) {
-    __shared__ uint32_t bd_d[JXS_NBL], bd_g[JXS_NBL], bd_s[JXS_NBL];
-    __shared__ uint8_t b_ms[JXS_NGB], b_gt[JXS_NGB];
-    __shared__ uint32_t k_rel[JXS_NPK], k_sb[JXS_NPK], k_gb[JXS_NPK], k_db[JXS_NPK], k_fl[JXS_NPK];
-    __shared__ uint32_t ub_s[JXS_NU], ub_g[JXS_NU], ub_d[JXS_NU];
-    const int f = find_frame(frames, nfr, blockIdx.x);
-    const JxsFrameDesc& F = frames[f];
-    __shared__ __align__(16) JxsGeom sg;
-    stage_geom(&sg, F.g);
-    __syncthreads();
-    const JxsGeom& g = sg;
-    const uint32_t row = blockIdx.x - F.prec_base;
-    const uint32_t type = row + 1 >= g.rows ? 1 : 0;
-    const uint32_t lane = threadIdx.x;
-    const JxsPrec P = F.prec[row];
-    JxsPlan& PL = F.plan[row];
+/* Orders this wave's LDS accesses around reads of what other lanes of the
+ * same wave wrote. */
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ uint32_t lane_of(uint32_t v, uint32_t k) {
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)k);
+}
+/* A value the same in every lane, declared so: the compiler then keeps what
+ * depends on it, loops and branches included, on the scalar unit. */
+__device__ __forceinline__ uint32_t uni(uint32_t v) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)v); }
+
+/* ------------------------------------------------------------------ */
+/* rate control and packet plan                                        */
+/* ------------------------------------------------------------------ */
+#define RT_STRIDE JXS_RT_STRIDE
+#define RT_ZERO 16   /* the zero entry, read for band lines absent from the precinct */
+#define RT_WAVES 4   /* waves of a k_rate block = precincts whose rate tables are held at once */
+
+/* Rate table of one precinct, built by one wave from the transform's
+ * band-line histogram records: for every band line and truncation t, the
+ * data bits (bit-planes above t and the sign bits), the bit-plane-count bits
+ * and the significance bits of the band's coding method at t.  These are the
+ * quantities RateControl.c:18-154 counts coefficient group by group and
+ * RateControl.c:457-520 compares to choose the method. */
+struct RateTab {
+    uint32_t d[JXS_NBL * RT_STRIDE];
+    uint32_t g[JXS_NBL * RT_STRIDE];
+    uint32_t s[JXS_NBL * RT_STRIDE];
+};
+
+/* The histogram record of band line `line` of a precinct (64 bytes). */
+struct Rec {
+    u32x4 a, b, c, e;
+};
+__device__ __forceinline__ Rec load_rec(const uint16_t* recs, uint32_t line) {
+    const u32x4* src = (const u32x4*)(recs + (size_t)line * JXS_HREC);
+    Rec r;
+    r.a = gld(src);
+    r.b = gld(src + 1);
+    r.c = gld(src + 2);
+    r.e = gld(src + 3);
+    return r;
+}
+
+/* Rate-table entries of band line `lane` (one lane per band line; every lane
+ * of the wave takes part, lanes past the last line store nothing).  With h(v)
+ * the groups whose GCLI is v, at truncation t the unary bit-plane counts
+ * take sum_{v<=t} h(v) + sum_{v>t} (v - t) h(v) bits, the data
+ * sum_{v>t} 4 (v - t + 1) h(v) bits (v - t magnitude bit-planes and a sign
+ * per coefficient), and significance coding removes the groups of the
+ * significance groups of 8 whose maximum GCLI is at most t (and of the
+ * trailing partial group) at the cost of one flag per significance group;
+ * all are running sums over t.  The coding method compares the two counts
+ * over the band's lines in this precinct (the other line of a two-line band
+ * is the neighbouring lane). */
+__device__ __forceinline__ void rate_lines(RateTab& T, const Rec& rc, uint32_t n_bl, uint32_t li, uint32_t type,
+                                           uint32_t sig_on, uint32_t lane) {
+    const uint32_t w[16] = {rc.a.x, rc.a.y, rc.a.z, rc.a.w, rc.b.x, rc.b.y, rc.b.z, rc.b.w,
+                            rc.c.x, rc.c.y, rc.c.z, rc.c.w, rc.e.x, rc.e.y, rc.e.z, rc.e.w};
+    uint32_t tot = 0, vtot = 0;
+#pragma unroll
+    for (uint32_t v = 0; v < 16; v++) {
+        const uint32_t h = (w[v >> 1] >> (16 * (v & 1))) & 0xFFFFu;
+        tot += h;
+        vtot += v * h;
+    }
+    const uint32_t gw = li & 0xFFFu, full = gw >> 3, part = gw & 7, sw = (li >> LI_SW) & 0xFFu;
+    const uint32_t nl = (li >> (LI_NL + 2 * type)) & 3u;
+    const bool two = nl == 2;
+    const int partner = (int)(two ? (((li >> LI_SECOND) & 1u) ? lane - 1 : lane + 1) : lane);
+    const uint32_t tail = w[15] >> 16; /* 0xFFFF: no partial group */
+    uint32_t data[16], ns[16], zr[16];
+    uint32_t cnt = 0, vcnt = 0, sig = 0;
+#pragma unroll
+    for (uint32_t t = 0; t < 16; t++) {
+        const uint32_t h = (w[t >> 1] >> (16 * (t & 1))) & 0xFFFFu;
+        cnt += h;
+        vcnt += t * h;
+        const uint32_t above = tot - cnt, vabove = vtot - vcnt;
+        data[t] = vabove + above - t * above; /* sum_{v>t} (v - t + 1) h(v) */
+        if (t < 15)
+            sig += (w[8 + (t >> 1)] >> (16 * (t & 1))) & 0xFFFFu;
+        else
+            sig = full;
+        ns[t] = data[t] + cnt;
+        zr[t] = 8 * sig + (tail <= t ? part : 0u);
+    }
+    /* the other line's counts, all exchanged before any is used */
+    uint32_t pn[16], pz[16];
+#pragma unroll
+    for (uint32_t t = 0; t < 16; t++) {
+        pn[t] = (uint32_t)__shfl((int)ns[t], partner);
+        pz[t] = (uint32_t)__shfl((int)zr[t], partner);
+    }
+    if (lane < n_bl) {
+        const uint32_t base = lane * RT_STRIDE;
+#pragma unroll
+        for (uint32_t t = 0; t < 16; t++) {
+            const uint32_t nosig = ns[t] + (two ? pn[t] : 0u);
+            const uint32_t withsig = sw * nl + (ns[t] - zr[t]) + (two ? pn[t] - pz[t] : 0u);
+            const bool use = sig_on && nosig > withsig;
+            T.d[base + t] = 4 * data[t];
+            T.g[base + t] = use ? ns[t] - zr[t] : ns[t];
+            T.s[base + t] = use ? sw : 0u;
+        }
+        T.d[base + RT_ZERO] = 0;
+        T.g[base + RT_ZERO] = 0;
+        T.s[base + RT_ZERO] = 0;
+    }
+}
+
+/* Bytes of a precinct's packets at (q, r) (precinct_get_budget_bytes,
+ * RateControl.c:700-781): per packet the data bytes plus the smaller of the
+ * significance + bit-plane-count bytes and the packet's RAW size.  Each lane
+ * answers its own (q, r).  The band lines present in the precinct are
+ * walked in packet order from registers: lane k of rq holds the k-th one's
+ * walk word, of rw its packet's RAW bytes.  The walk takes 8 lines per step,
+ * their 24 table reads issued together; the words past the last line carry
+ * no packet end, so what they add is never counted. */
+__device__ __forceinline__ uint32_t rate_query(uint32_t rq, uint32_t rw, uint32_t n, const RateTab& T, uint32_t q,
+                                               uint32_t r) {
+    uint32_t total = 0, d = 0, gb = 0, s = 0;
+    for (uint32_t k0 = 0; k0 < n; k0 += 8) { /* k0 + 7 < 64: n <= JXS_NU */
+        uint32_t vd[8], vg[8], vs[8];
+#pragma unroll
+        for (uint32_t kk = 0; kk < 8; kk++) {
+            const uint32_t x = lane_of(rq, k0 + kk);
+            const int32_t t = (int32_t)q - (int32_t)((x >> RQ_GAIN) & 0xFFu) - (((x >> RQ_PRIO) & 0xFFu) < r ? 1 : 0);
+            const uint32_t o = (x & 0x7FFu) + (uint32_t)min(max(t, 0), 15);
+            vd[kk] = T.d[o];
+            vg[kk] = T.g[o];
+            vs[kk] = T.s[o];
+        }
+#pragma unroll
+        for (uint32_t kk = 0; kk < 8; kk++) {
+            const uint32_t x = lane_of(rq, k0 + kk);
+            d += vd[kk];
+            gb += vg[kk];
+            s += vs[kk];
+            if ((x >> RQ_END) & 1u) {
+                total += ((d + 7) >> 3) + min(((s + 7) >> 3) + ((gb + 7) >> 3), lane_of(rw, k0 + kk));
+                d = gb = s = 0;
+            }
+        }
+    }
+    return total;
+}
+
+/* The binary search of Q (RateControl.c:889-1083): lane q of cq holds the
+ * precinct's bytes at (q, 0), bit q of eq is set when (q, 0) truncates
+ * every band away; too small a Q is one whose bytes exceed avail. */
+__device__ __forceinline__ bool search_q(uint32_t maxq, uint32_t eq, uint32_t cq, uint32_t avail, uint32_t& q) {
+    const uint32_t m1 = (uint32_t)__ballot(cq > avail); /* lanes 0..31 */
+    return bsearch_masks(m1, eq, (int32_t)maxq, false, 6 > maxq ? 0 : 6, q);
+}
+
+/* The binary search of R at Q = q: lane base + r of cr holds the bytes at
+ * (q, r), bit r of er is set when (q, r) truncates every band away; an R
+ * whose bytes fit avail is kept as the answer. */
+__device__ __forceinline__ bool search_r(uint32_t maxr, uint32_t er, uint32_t cr, uint32_t base, uint32_t avail,
+                                         uint32_t& r) {
+    const uint32_t m1 = (uint32_t)(__ballot(cr <= avail) >> base);
+    return bsearch_masks(m1, er, (int32_t)maxr, true, 6 >= maxr ? 0 : 6, r);
+}
+
+/* A packing unit that writes nothing. */
+__device__ __forceinline__ void unit_empty(JxsUnit* u) {
+    GAS JxsUnit* g = (GAS JxsUnit*)u;
+    g->len[0] = g->len[1] = g->len[2] = 0;
+    g->hlen = 0;
+}
+
+/* Sum of v over lanes [first, last] of the wave, given incl = wave_incl_scan(v). */
+__device__ __forceinline__ uint32_t seg_sum(uint32_t incl, uint32_t v, uint32_t first, uint32_t last) {
+    return (uint32_t)__shfl((int)incl, (int)last) - (uint32_t)__shfl((int)(incl - v), (int)first);
+}
+/* Sum of v over lanes [first, lane) of the wave. */
+__device__ __forceinline__ uint32_t seg_before(uint32_t incl, uint32_t v, uint32_t first) {
+    return (incl - v) - (uint32_t)__shfl((int)(incl - v), (int)first);
+}
+
+/* Packet plan of one precinct (one wave, one lane per packing unit), once
+ * its (Q, R) is chosen: truncation and coding method of every band, the
+ * sub-packet sizes, RAW choice and codestream offset of every packet
+ * (PackPrecinct.c:235-572, RateControl.c:457-781), the packing units, and the
+ * bytes around the packets: slice header (PackHeaders.c:142-156), precinct
+ * header (PackHeaders.c:158-196), padding, end-of-codestream marker and, for
+ * the first precinct, the frame header.  Sums over a packet's band lines and
+ * over the packets come from wave prefix sums.  A precinct whose packets
+ * would not add up to the size the rate control chose is flagged and nothing
+ * of it is written. */
+__device__ __forceinline__ void plan_precinct(const JxsFrameDesc& F, const JxsGeom& g, uint32_t row, uint32_t type,
+                                              const JxsPrec& P, const RateTab& T, uint32_t* ms_band, uint32_t ui,
+                                              uint32_t lane) {
+    const uint32_t nu = g.n_units;
+    const bool unit = lane < nu;
+    JxsUnit* UR = F.units + (size_t)row * nu;
     if (P.err) {
         /* nothing of a failed precinct is packed */
-        if (lane < g.npk)
-            PL.pk[lane].flags = 0;
-        for (uint32_t u = lane; u < g.n_units; u += WAVE)
-            {
-                JxsUnit& U = F.units[(size_t)row * g.n_units + u];
-                U.len[0] = U.len[1] = U.len[2] = 0;
-                U.hlen = 0;
-            }
+        if (unit)
+            unit_empty(UR + lane);
         return;
     }
-    const uint16_t* recs = F.hist + (size_t)row * g.n_bl * JXS_HREC;
-    if (lane < g.nb_all) {
-        const uint32_t gi = lane;
-        uint32_t ms = 0, gt = 0;
-        if (g.gb[gi] != 0xFF) {
-            const JxsBand& bd = g.c[g.gc[gi]].b[g.gb[gi]];
-            gt = truncation(bd.gain, bd.prio, P.q, P.r);
-            const uint32_t nl = bd.lines[type]; /* 0, 1 or 2 */
-            LineBits l0 = {0, 0, 0}, l1 = {0, 0, 0};
-            if (nl > 0)
-                l0 = line_bits(recs + (size_t)bd.bl * JXS_HREC, gt, bd.gw);
-            if (nl > 1)
-                l1 = line_bits(recs + (size_t)(bd.bl + 1) * JXS_HREC, gt, bd.gw);
-            const uint32_t nosig = l0.ns + l1.ns;
-            const uint32_t sig = bd.sw * nl + (l0.ns - l0.zeroed) + (l1.ns - l1.zeroed);
-            ms = (g.sig && nl > 0 && nosig > sig) ? 1u : 0u;
-            if (nl > 0) {
-                bd_d[bd.bl] = l0.data;
-                bd_g[bd.bl] = ms ? l0.ns - l0.zeroed : l0.ns;
-                bd_s[bd.bl] = ms ? bd.sw : 0u;
-            }
-            if (nl > 1) {
-                bd_d[bd.bl + 1] = l1.data;
-                bd_g[bd.bl + 1] = ms ? l1.ns - l1.zeroed : l1.ns;
-                bd_s[bd.bl + 1] = ms ? bd.sw : 0u;
-            }
-        }
-        b_ms[gi] = (uint8_t)ms;
-        b_gt[gi] = (uint8_t)gt;
-        PL.band[gi] = (uint8_t)(gt | (ms << 4));
+    const uint32_t p = unit ? g.u_p[lane] : 0u, gi = unit ? g.u_gi[lane] : 0u, l = g.pk_line[p];
+    const uint32_t j = unit ? gi - g.pk_bs[p] : 0u; /* band line's place in its packet */
+    const uint32_t first = lane - j, last = unit ? first + (g.pk_be[p] - g.pk_bs[p]) - 1 : lane;
+    const bool pres = unit && ((ui >> (UI_NORMAL + type)) & 1u);
+    const uint32_t gt = truncation((ui >> 8) & 0xFFu, (ui >> 16) & 0xFFu, P.q, P.r);
+    const uint32_t o = (ui & 0xFFu) * RT_STRIDE + (pres ? gt : RT_ZERO);
+    const uint32_t vs = T.s[o], vg = T.g[o], vd = T.d[o];
+    const uint32_t ms = vs != 0 ? 1u : 0u;
+    if (unit && l == 0)
+        ms_band[gi] = ms;
+    /* packet sizes: every lane of a packet computes them */
+    const uint32_t ps_ = seg_sum(wave_incl_scan(vs, lane), vs, first, last);
+    const uint32_t pg = seg_sum(wave_incl_scan(vg, lane), vg, first, last);
+    const uint32_t pd = seg_sum(wave_incl_scan(vd, lane), vd, first, last);
+    const uint64_t pm = __ballot(pres);
+    const uint32_t n_in = last - first + 1;
+    const bool any = unit && ((pm >> first) & (n_in >= 64 ? ~0ull : ((1ull << n_in) - 1))) != 0;
+    uint32_t sb = (ps_ + 7) >> 3, gbb = (pg + 7) >> 3;
+    const uint32_t db = (pd + 7) >> 3, raw_p = g.raw_bytes[type][p];
+    const bool raw = sb + gbb > raw_p;
+    if (raw) {
+        sb = 0;
+        gbb = raw_p;
     }
-    __syncthreads();
-    uint32_t size = 0, sb = 0, gbb = 0, db = 0, flags = 0;
-    if (lane < g.npk) {
-        const uint32_t p = lane, l = g.pk_line[p];
-        uint32_t d = 0, gb = 0, s = 0, any = 0;
-        for (uint32_t gi = g.pk_bs[p]; gi < g.pk_be[p]; gi++) {
-            const JxsBand& bd = g.c[g.gc[gi]].b[g.gb[gi]];
-            if (l >= bd.lines[type])
-                continue;
-            any = 1;
-            d += bd_d[bd.bl + l];
-            gb += bd_g[bd.bl + l];
-            s += bd_s[bd.bl + l];
-        }
-        db = (d + 7) >> 3;
-        gbb = (gb + 7) >> 3;
-        sb = (s + 7) >> 3;
-        const uint32_t raw = g.raw_bytes[type][p];
-        if (sb + gbb > raw) {
-            flags = PKT_RAW;
-            sb = 0;
-            gbb = raw;
-        }
-        if (any) {
-            flags |= PKT_ANY;
-            size = g.pkt_hdr_bytes + sb + gbb + db;
-        }
-    }
-    const uint32_t incl = wave_incl_scan(size, lane);
-    const uint32_t sum = wave_last(incl);
+    const uint32_t size = any ? g.pkt_hdr_bytes + sb + gbb + db : 0u;
+    const uint32_t psz = (unit && j == 0) ? size : 0u;
+    const uint32_t pincl = wave_incl_scan(psz, lane);
+    const uint32_t sum = wave_last(pincl);
     const uint32_t eoc = row + 1 == g.rows ? 2u : 0u;
     const bool valid = g.prec_hdr_bytes + sum + P.pad == P.total && P.offset >= 6 &&
         (uint64_t)P.offset + P.total + eoc <= g.lcod;
     if (!valid) {
-        if (lane < g.npk)
-            PL.pk[lane].flags = 0;
-        for (uint32_t u = lane; u < g.n_units; u += WAVE)
-            {
-                JxsUnit& U = F.units[(size_t)row * g.n_units + u];
-                U.len[0] = U.len[1] = U.len[2] = 0;
-                U.hlen = 0;
-            }
+        if (unit)
+            unit_empty(UR + lane);
         if (lane == 0)
-            atomicOr(F.status, 2u);
+            gor(F.status, 2u);
         return;
     }
-    const uint32_t rel = g.prec_hdr_bytes + incl - size; /* packet offset from the precinct's first byte */
-    if (lane < g.npk) {
-        JxsPkt K;
-        K.off = P.offset + rel;
-        K.s = sb;
-        K.g = gbb;
-        K.d = db;
-        K.flags = flags;
-        PL.pk[lane] = K;
-    }
-    /* packing units, one lane each: the bits of the band line's three
-     * segments (significance, bit-plane counts, data), then their places
-     * after the earlier band lines of the same packet */
-    if (lane < g.npk) {
-        k_rel[lane] = rel;
-        k_sb[lane] = sb;
-        k_gb[lane] = gbb;
-        k_db[lane] = db;
-        k_fl[lane] = flags;
-    }
-    __syncthreads();
-    for (uint32_t u = lane; u < g.n_units; u += WAVE) {
-        const uint32_t p = g.u_p[u], gi = g.u_gi[u], l = g.pk_line[p];
-        const JxsBand& bd = g.c[g.gc[gi]].b[g.gb[gi]];
-        const bool raw = (k_fl[p] & PKT_RAW) != 0;
-        uint32_t s = 0, gb = 0, d = 0;
-        if (l < bd.lines[type]) {
-            const uint32_t bl = bd.bl + l;
-            s = raw ? 0u : bd_s[bl];
-            gb = raw ? 4 * bd.gw : bd_g[bl];
-            d = bd_d[bl];
-        }
-        ub_s[u] = s;
-        ub_g[u] = gb;
-        ub_d[u] = d;
-    }
-    __syncthreads();
-    {
+    /* the packet's offset from the precinct's first byte */
+    const uint32_t rel = g.prec_hdr_bytes + (uint32_t)__shfl((int)(pincl - psz), (int)first);
+    /* the band line's three segments (RAW packets: no significance, 4 bits
+     * per group for the bit-plane counts) and their places after the earlier
+     * band lines of the packet */
+    const JxsBand& bd = g.c[g.gc[gi]].b[g.gb[gi]];
+    const uint32_t us = raw ? 0u : vs, ug = raw ? (pres ? 4 * bd.gw : 0u) : vg, ud = vd;
+    const uint32_t bs = seg_before(wave_incl_scan(us, lane), us, first);
+    const uint32_t bg = seg_before(wave_incl_scan(ug, lane), ug, first);
+    const uint32_t bdd = seg_before(wave_incl_scan(ud, lane), ud, first);
+    wave_sync(); /* ms_band complete */
+    if (unit) {
         const uint64_t pbit = 8 * (uint64_t)P.offset; /* precinct's first bit in the codestream */
-        JxsUnit* UR = F.units + (size_t)row * g.n_units;
-        for (uint32_t u = lane; u < g.n_units; u += WAVE) {
-            const uint32_t p = g.u_p[u], gi = g.u_gi[u], l = g.pk_line[p], j = gi - g.pk_bs[p];
-            const JxsBand& bd = g.c[g.gc[gi]].b[g.gb[gi]];
-            const bool raw = (k_fl[p] & PKT_RAW) != 0;
-            uint32_t a[3];
-            a[0] = 8 * (k_rel[p] + g.pkt_hdr_bytes);
-            a[1] = a[0] + 8 * k_sb[p];
-            a[2] = a[1] + 8 * k_gb[p];
-            for (uint32_t k = 0; k < j; k++) {
-                a[0] += ub_s[u - j + k];
-                a[1] += ub_g[u - j + k];
-                a[2] += ub_d[u - j + k];
-            }
-            const uint32_t len[3] = {ub_s[u], ub_g[u], ub_d[u]};
-            JxsUnit U = {};
-            if (l < bd.lines[type]) {
-                const uint32_t y = row * bd.hl + l;
-                U.gl = F.gcli + bd.gcli_off + (size_t)y * bd.gw;
-                U.sl = F.smax + bd.smax_off + (size_t)y * bd.sw;
-                U.cl = F.coeff + bd.coeff_off + (size_t)y * bd.cs;
-                U.gt = b_gt[gi];
-                U.ms = b_ms[gi];
-                U.raw = raw ? 1u : 0u;
-                U.qpih = g.qpih;
-                U.w = bd.w;
-                U.gw = bd.gw;
-                U.sw = bd.sw;
-            }
-            for (uint32_t k = 0; k < 3; k++) {
-                const uint64_t first = pbit + a[k];
-                U.ob[k] = (uint32_t*)F.out + (first >> 5);
-                U.lo[k] = (uint32_t)(first & 31);
-                U.len[k] = len[k];
-            }
-            if (j == 0 && (k_fl[p] & PKT_ANY)) {
-                /* packet header (PackHeaders.c:98-127): RAW flag, data bytes,
-                 * bit-plane-count bytes, sign bytes (0) */
-                const uint64_t first = pbit + 8 * k_rel[p];
-                U.hob = (uint32_t*)F.out + (first >> 5);
-                U.hlo = (uint32_t)(first & 31);
-                U.hlen = 8 * g.pkt_hdr_bytes;
-                U.hdr = g.short_hdr ? ((uint64_t)raw << 39) | ((uint64_t)k_db[p] << 24) | ((uint64_t)k_gb[p] << 11)
-                                    : ((uint64_t)raw << 55) | ((uint64_t)k_db[p] << 35) | ((uint64_t)k_gb[p] << 15);
-            }
-            U.status = F.status;
-            UR[u] = U;
+        const uint32_t a0 = 8 * (rel + g.pkt_hdr_bytes) + bs;
+        const uint32_t a1 = 8 * (rel + g.pkt_hdr_bytes + sb) + bg;
+        const uint32_t a2 = 8 * (rel + g.pkt_hdr_bytes + sb + gbb) + bdd;
+        JxsUnit U = {};
+        if (pres) {
+            const uint32_t y = row * bd.hl + l;
+            U.gl = F.gcli + bd.gcli_off + (size_t)y * bd.gw;
+            U.sl = F.smax + bd.smax_off + (size_t)y * bd.sw;
+            U.cl = F.coeff + bd.coeff_off + (size_t)y * bd.cs;
+            U.gt = gt;
+            U.ms = ms;
+            U.raw = raw ? 1u : 0u;
+            U.qpih = g.qpih;
+            U.w = bd.w;
+            U.gw = bd.gw;
+            U.sw = bd.sw;
         }
+        const uint64_t f0 = pbit + a0, f1 = pbit + a1, f2 = pbit + a2;
+        U.ob[0] = (uint32_t*)F.out + (f0 >> 5);
+        U.ob[1] = (uint32_t*)F.out + (f1 >> 5);
+        U.ob[2] = (uint32_t*)F.out + (f2 >> 5);
+        U.lo[0] = (uint32_t)(f0 & 31);
+        U.lo[1] = (uint32_t)(f1 & 31);
+        U.lo[2] = (uint32_t)(f2 & 31);
+        U.len[0] = us;
+        U.len[1] = ug;
+        U.len[2] = ud;
+        if (j == 0 && any) {
+            /* packet header (PackHeaders.c:98-127): RAW flag, data bytes,
+             * bit-plane-count bytes, sign bytes (0) */
+            const uint64_t hb = pbit + 8 * rel;
+            U.hob = (uint32_t*)F.out + (hb >> 5);
+            U.hlo = (uint32_t)(hb & 31);
+            U.hlen = 8 * g.pkt_hdr_bytes;
+            U.hdr = g.short_hdr ? ((uint64_t)raw << 39) | ((uint64_t)db << 24) | ((uint64_t)gbb << 11)
+                                : ((uint64_t)raw << 55) | ((uint64_t)db << 35) | ((uint64_t)gbb << 15);
+        }
+        U.status = F.status;
+        gst_words(UR + lane, U); /* units are 144-byte records of a 256-byte aligned array */
     }
-    uint8_t* o = F.out;
+    uint8_t* o8 = F.out;
     /* packet bodies and padding start as zeros: the packing units OR their
      * first and last words in */
     {
-        const uint32_t z0 = P.offset + g.prec_hdr_bytes, z1 = P.offset + P.total;
-        const uint32_t a0 = (z0 + 3) & ~3u, a1 = z1 & ~3u;
+        /* 16-byte stores between the 16-byte aligned addresses inside
+         * [z0, z1), single bytes before and after */
+        uint8_t* z0 = o8 + P.offset + g.prec_hdr_bytes;
+        uint8_t* z1 = o8 + P.offset + P.total;
+        uint8_t* a0 = (uint8_t*)(((uintptr_t)z0 + 15) & ~(uintptr_t)15);
+        uint8_t* a1 = (uint8_t*)((uintptr_t)z1 & ~(uintptr_t)15);
         if (a0 < a1) {
-            for (uint32_t wd = a0 / 4 + lane; wd < a1 / 4; wd += WAVE)
-                gst((uint32_t*)o + wd, 0u);
+            const u32x4 zero = {0u, 0u, 0u, 0u};
+            const uint32_t nq = (uint32_t)((a1 - a0) / 16);
+            for (uint32_t q4 = lane; q4 < nq; q4 += WAVE)
+                gst((u32x4*)a0 + q4, zero);
             if (z0 + lane < a0)
-                gst(o + z0 + lane, (uint8_t)0);
+                gst(z0 + lane, (uint8_t)0);
             if (a1 + lane < z1)
-                gst(o + a1 + lane, (uint8_t)0);
+                gst(a1 + lane, (uint8_t)0);
         } else {
-            for (uint32_t k = z0 + lane; k < z1; k += WAVE)
-                gst(o + k, (uint8_t)0);
+            for (uint8_t* b = z0 + lane; b < z1; b += WAVE)
+                gst(b, (uint8_t)0);
         }
     }
     /* slice header */
@@ -1097,7 +940,7 @@ __global__ void __launch_bounds__(64) k_plan(const JxsFrameDesc* __restrict__ fr
         const uint32_t s = row / g.pps;
         /* FF20 (SLH marker), Lslh = 4, slice index */
         const uint32_t v = lane == 0 ? 0xFFu : lane == 1 ? 0x20u : lane == 3 ? 0x04u : lane == 4 ? (s >> 8) : lane == 5 ? s : 0u;
-        o[P.offset - 6 + lane] = (uint8_t)v;
+        gst(o8 + P.offset - 6 + lane, (uint8_t)v);
     }
     /* precinct header: Lprc, Q, R, then 2 bits per existing band */
     const uint32_t lprc = P.total - g.prec_hdr_bytes;
@@ -1111,19 +954,176 @@ __global__ void __launch_bounds__(64) k_plan(const JxsFrameDesc* __restrict__ fr
             v = P.r;
         else {
             v = 0;
-            for (uint32_t j = 0; j < 4; j++) {
-                const uint32_t e = 4 * (k - 5) + j;
-                if (e < g.nb_exist && b_ms[g.ex_gid[e]])
-                    v |= 2u << (6 - 2 * j);
+            for (uint32_t e4 = 0; e4 < 4; e4++) {
+                const uint32_t e = 4 * (k - 5) + e4;
+                if (e < g.nb_exist && ms_band[g.ex_gid[e]])
+                    v |= 2u << (6 - 2 * e4);
             }
         }
-        o[P.offset + k] = (uint8_t)v;
+        gst(o8 + P.offset + k, (uint8_t)v);
     }
     if (eoc && lane < 2)
-        o[P.offset + P.total + lane] = lane ? 0x11 : 0xFF;
+        gst(o8 + P.offset + P.total + lane, (uint8_t)(lane ? 0x11 : 0xFF));
     if (row == 0)
         for (uint32_t k = lane; k < g.hdr_len; k += WAVE)
-            o[k] = g.hdr[k];
+            gst(o8 + k, g.hdr[k]);
+}
+
+/* One block per slice (RT_WAVES waves), the rate control and packet plan of
+ * its precincts, RT_WAVES precincts at a time:
+ *   1. each wave builds one precinct's rate table from the histogram records,
+ *      the precinct's bytes at (q, 0) for every q, and the (q, r) rows at the
+ *      two Q its budget most likely gets;
+ *   2. wave 0 walks the precincts in order: budgets with the +10 % first
+ *      precinct and the padding carry (PackStageProcess.c:600-684), the
+ *      reference's binary search of Q on the (q, 0) column, then of R on the
+ *      (Q, r) row, computed now if the carry moved Q elsewhere
+ *      (RateControl.c:889-1083, BinarySearch.c); only these entries of the
+ *      (Q, R) table are ever read;
+ *   3. each wave plans its precinct. */
+__global__ void __launch_bounds__(WAVE * RT_WAVES) k_rate(const JxsFrameDesc* __restrict__ frames, int nfr,
+                                                         uint32_t total_slices) {
+    __shared__ __align__(16) JxsGeom sg;
+    __shared__ RateTab tabs[RT_WAVES];
+    __shared__ uint32_t col[RT_WAVES][WAVE], rowq[RT_WAVES][WAVE], spec[RT_WAVES];
+    __shared__ JxsPrec ps[RT_WAVES];
+    __shared__ uint32_t ms_band[RT_WAVES][JXS_NGB];
+    const uint32_t gs = blockIdx.x;
+    if (gs >= total_slices)
+        return;
+    int fl = 0, fh = nfr - 1;
+    while (fl < fh) {
+        const int mid = (fl + fh + 1) >> 1;
+        if (frames[mid].slice_base <= gs)
+            fl = mid;
+        else
+            fh = mid - 1;
+    }
+    /* the frame descriptor in scalar registers */
+    const JxsFrameDesc F = cld_words(frames + fl);
+    /* the wave index is made visibly uniform: everything a wave derives from
+     * it (its precinct, the searches, the plan) then stays in scalar
+     * registers and scalar branches */
+    const uint32_t tid = threadIdx.x, lane = tid & (WAVE - 1);
+    const uint32_t wid = (uint32_t)__builtin_amdgcn_readfirstlane((int)(tid / WAVE));
+    /* the first precincts' records are loaded while the geometry is staged */
+    const uint32_t pps = F.pps, rows = F.rows, n_bl = F.n_bl;
+    const uint32_t s = gs - F.slice_base;
+    const uint32_t first_row = s * pps;
+    const uint32_t nprec = (s == F.slices - 1) ? rows - first_row : pps;
+    const uint32_t rec_line = min(lane, n_bl - 1);
+    Rec rc = load_rec(F.hist + (size_t)min(first_row + wid, rows - 1) * n_bl * JXS_HREC, rec_line);
+    stage_geom(&sg, F.g);
+    __syncthreads();
+    const JxsGeom& g = sg;
+    const uint32_t ui = g.u_info[lane], li = g.ln_info[lane];
+    const uint32_t rq0 = g.rq[0][lane], rq1 = g.rq[1][lane], rw0 = g.rq_raw[0][lane], rw1 = g.rq_raw[1][lane];
+    const uint32_t em = g.empty_mask[lane & (JXS_NQ - 1)], maxq = uni(g.maxq), maxr = uni(g.maxr);
+    const uint32_t eq = (uint32_t)__ballot(em & 1u); /* bit q: (q, 0) truncates every band away */
+    const uint32_t sbud = gld(F.slice_bud + s); /* global, not flat: a uniform value */
+    uint32_t minb = sbud / nprec;
+    uint32_t first = minb * 110 / 100;
+    if (first > sbud)
+        first = sbud;
+    uint32_t left = 0;
+    if (nprec > 1) {
+        minb = (sbud - first) / (nprec - 1);
+        left = sbud - minb * (nprec - 1) - first;
+    }
+    uint32_t carry = 0, off = gld(F.slice_off + s) + 6;
+    bool failed = false;
+    for (uint32_t i0 = 0; i0 < nprec; i0 += RT_WAVES) {
+        const uint32_t n = min((uint32_t)RT_WAVES, nprec - i0);
+        if (wid < n) {
+            const uint32_t i = i0 + wid, row = first_row + i, type = row + 1 >= rows ? 1 : 0;
+            if (i0 > 0)
+                rc = load_rec(F.hist + (size_t)row * n_bl * JXS_HREC, rec_line);
+            RateTab& T = tabs[wid];
+            rate_lines(T, rc, n_bl, li, type, g.sig, lane);
+            wave_sync();
+            const uint32_t rq = type ? rq1 : rq0, rw = type ? rw1 : rw0, nl = g.rq_n[type];
+            /* lane q (and q + 32): bytes at (q, 0) */
+            const uint32_t cq = rate_query(rq, rw, nl, T, lane & (JXS_NQ - 1), 0);
+            /* the (q, r) rows at the Q this precinct gets without a padding
+             * carry (lanes 0..31: lane r) and at that Q - 1 (lanes 32..63):
+             * a carry only ever adds bytes, and rarely more than one step of
+             * Q's worth */
+            const uint32_t hdr = g.rc_hdr_bytes[type];
+            const uint32_t b0 = (i == 0 ? first : minb) + (i < left ? 1 : 0);
+            uint32_t q0 = 0;
+            if (!(b0 <= 1048575u && b0 > hdr && search_q(maxq, eq, cq, b0 - hdr, q0)))
+                q0 = 0;
+            const uint32_t qc = lane < JXS_NR ? q0 : (q0 > 0 ? q0 - 1 : 0);
+            const uint32_t cr = rate_query(rq, rw, nl, T, qc, lane & (JXS_NR - 1));
+            col[wid][lane] = cq;
+            rowq[wid][lane] = cr;
+            if (lane == 0)
+                spec[wid] = q0;
+        }
+        __syncthreads();
+        if (wid == 0) {
+            for (uint32_t k = 0; k < n; k++) {
+                const uint32_t i = i0 + k, row = first_row + i, type = row + 1 >= rows ? 1 : 0;
+                const uint32_t cq = col[k][lane], crs = rowq[k][lane], q0 = spec[k];
+                JxsPrec P;
+                P.q = P.r = P.total = P.pad = 0;
+                P.offset = off;
+                P.err = 1;
+                if (!failed) {
+                    const uint32_t budget = uni((i == 0 ? first : minb) + (i < left ? 1 : 0) + carry);
+                    const uint32_t hdr = uni(g.rc_hdr_bytes[type]);
+                    bool ok = budget <= 1048575u && budget > hdr;
+                    uint32_t q = 0, r = 0;
+                    if (ok) {
+                        const uint32_t avail = budget - hdr;
+                        ok = search_q(maxq, eq, cq, avail, q);
+                        if (ok) {
+                            /* the bytes at (q, r) are lane base + r of cr */
+                            uint32_t cr = crs, base = 0;
+                            if (q0 > 0 && q + 1 == q0) {
+                                base = JXS_NR;
+                            } else if (q != q0) {
+                                cr = rate_query(type ? rq1 : rq0, type ? rw1 : rw0, g.rq_n[type], tabs[k], q,
+                                                lane & (JXS_NR - 1));
+                            }
+                            ok = search_r(maxr, lane_of(em, q), cr, base, avail, r);
+                            if (ok) {
+                                const uint32_t data = lane_of(cr, base + r);
+                                uint32_t pad = avail - data, total = budget;
+                                if (i + 1 < nprec) {
+                                    total -= pad;
+                                    carry = pad;
+                                    pad = 0;
+                                }
+                                P.q = q;
+                                P.r = r;
+                                P.total = total;
+                                P.pad = pad;
+                                P.err = 0;
+                                off += total;
+                            }
+                        }
+                    }
+                    if (!ok) {
+                        failed = true;
+                        if (lane == 0)
+                            gor(F.status, 1u);
+                    }
+                }
+                if (lane == 0) {
+                    ps[k] = P;
+                    gst_words(F.prec + row, P);
+                }
+            }
+        }
+        __syncthreads();
+        if (wid < n) {
+            const uint32_t row = first_row + i0 + wid, type = row + 1 >= rows ? 1 : 0;
+            const JxsPrec P = ps[wid];
+            plan_precinct(F, g, row, type, P, tabs[wid], ms_band[wid], ui, lane);
+        }
+        __syncthreads();
+    }
 }
 
 /* ------------------------------------------------------------------ */
@@ -1133,13 +1133,6 @@ __global__ void __launch_bounds__(64) k_plan(const JxsFrameDesc* __restrict__ fr
 #define RINGB 512      /* words of a wave's significance / bit-plane-count ring (2 KiB): a step adds <= 128 */
 #define RINGC 1024     /* words of a wave's data ring (4 KiB): a step adds <= 512 */
 #define RING_FLUSH 256 /* flush once this many complete words are pending */
-
-/* Orders this wave's LDS accesses around cross-lane reads of a ring. */
-__device__ __forceinline__ void wave_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
 
 /* The n (1..64) low bits of v, MSB first, placed at bit offset sh (0..31)
  * of three consecutive 32-bit words (MSB-first bit order). */
@@ -1442,18 +1435,10 @@ hipError_t jxs_launch_dwt(const JxsDwtJob* jobs, const uint32_t* bases, int njob
     return hipGetLastError();
 }
 
-hipError_t jxs_launch_rc_table(const JxsFrameDesc* frames, int nfr, uint32_t precincts, hipStream_t st) {
-    hipLaunchKernelGGL(k_rc_table, dim3(precincts), dim3(256), 0, st, frames, nfr);
-    return hipGetLastError();
-}
-
-hipError_t jxs_launch_select(const JxsFrameDesc* frames, int nfr, uint32_t slices, hipStream_t st) {
-    hipLaunchKernelGGL(k_select, dim3(slices), dim3(SEL_THREADS), 0, st, frames, nfr, slices);
-    return hipGetLastError();
-}
-
-hipError_t jxs_launch_plan(const JxsFrameDesc* frames, int nfr, uint32_t precincts, hipStream_t st) {
-    hipLaunchKernelGGL(k_plan, dim3(precincts), dim3(64), 0, st, frames, nfr);
+hipError_t jxs_launch_rate(const JxsFrameDesc* frames, int nfr, uint32_t slices, hipStream_t st) {
+    if (!slices)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_rate, dim3(slices), dim3(WAVE * RT_WAVES), 0, st, frames, nfr, slices);
     return hipGetLastError();
 }
 
