@@ -19,10 +19,9 @@
 #include "jxs_host.h"
 
 extern "C" {
-hipError_t jxs_launch_dwt(const JxsDwtJob*, const uint32_t*, int, uint32_t, uint32_t, hipStream_t);
+hipError_t jxs_launch_dwt(const JxsDwtJob*, const uint32_t*, int, uint32_t, uint32_t, uint32_t*, uint32_t, hipStream_t);
 hipError_t jxs_launch_rate(const JxsFrameDesc*, int, uint32_t, hipStream_t);
 hipError_t jxs_launch_pack(const JxsUnit*, uint32_t, hipStream_t);
-hipError_t jxs_launch_zero(uint32_t*, uint32_t, hipStream_t);
 }
 
 namespace jxs {
@@ -90,11 +89,12 @@ struct svt_jpeg_xs_amd_plan {
     uint32_t* d_status = nullptr;
     JxsFrameDesc* d_frames = nullptr;
     JxsUnit* d_units = nullptr; /* packing units of every frame, frame after frame */
-    JxsDwtJob* d_dwt[2] = {nullptr, nullptr};
-    uint32_t* d_dbase[2] = {nullptr, nullptr};
-    int ndwt[2] = {0, 0};
-    uint32_t dblocks[2] = {0, 0};
-    uint32_t dwmax[2] = {0, 0}; /* widest row of each transform stage */
+    /* transform job lists per stage (first / second vertical level) and
+     * chunk class (16 / 32 columns per thread), one launch each */
+    JxsDwtJob* d_dwt[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
+    uint32_t* d_dbase[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
+    int ndwt[2][2] = {{0, 0}, {0, 0}};
+    uint32_t dblocks[2][2] = {{0, 0}, {0, 0}};
     uint32_t total_prec = 0, total_slices = 0, total_units = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
@@ -141,8 +141,10 @@ static void plan_free(svt_jpeg_xs_amd_plan_t* p) {
     (void)hipFree(p->d_frames);
     (void)hipFree(p->d_units);
     for (int s = 0; s < 2; s++) {
-        (void)hipFree(p->d_dwt[s]);
-        (void)hipFree(p->d_dbase[s]);
+        for (int c = 0; c < 2; c++) {
+            (void)hipFree(p->d_dwt[s][c]);
+            (void)hipFree(p->d_dbase[s][c]);
+        }
     }
     delete p;
 }
@@ -235,19 +237,27 @@ static svt_jpeg_xs_amd_plan_t* plan_build(const svt_jpeg_xs_amd_frame_spec_t* sp
     for (uint32_t i = 0; i < count; i++)
         frames[i].units = p->d_units + frames[i].unit_base;
     bool ok = upload_vec(frames, &p->d_frames) == hipSuccess;
-    for (int s = 0; s < 2 && ok; s++) {
-        std::vector<uint32_t> bases;
-        uint32_t blk = 0;
-        for (auto& j : dj[s]) {
-            j.base = blk;
-            bases.push_back(blk);
-            blk += j.rows;
-            p->dwmax[s] = j.w > p->dwmax[s] ? j.w : p->dwmax[s];
+    for (int s = 0; s < 2 && ok; s++)
+        for (int c = 0; c < 2 && ok; c++) {
+            const uint32_t chunk = c == 0 ? 16u : 32u;
+            std::vector<JxsDwtJob> list;
+            std::vector<uint32_t> bases;
+            uint32_t blk = 0;
+            for (auto j : dj[s]) {
+                if (j.chunk != chunk)
+                    continue;
+                if (j.lev_l < 1 || (1u << j.lev_l) > j.chunk || j.tpr > 256 || j.rpb == 0)
+                    ok = false; /* outside what the transform kernel's layout covers */
+                j.base = blk;
+                bases.push_back(blk);
+                blk += j.nblk;
+                list.push_back(j);
+            }
+            p->dblocks[s][c] = blk;
+            p->ndwt[s][c] = (int)list.size();
+            ok = ok && upload_vec(list, &p->d_dwt[s][c]) == hipSuccess &&
+                 upload_vec(bases, &p->d_dbase[s][c]) == hipSuccess;
         }
-        p->dblocks[s] = blk;
-        p->ndwt[s] = (int)dj[s].size();
-        ok = upload_vec(dj[s], &p->d_dwt[s]) == hipSuccess && upload_vec(bases, &p->d_dbase[s]) == hipSuccess;
-    }
     if (!ok) {
         plan_free(p);
         return nullptr;
@@ -268,10 +278,17 @@ static hipError_t plan_launch(svt_jpeg_xs_amd_plan_t* p, hipStream_t st, hipEven
         if (e != hipSuccess)                \
             return e;                       \
     } while (0)
-    if ((e = jxs_launch_zero(p->d_status, p->nfr, st)) != hipSuccess)
+    /* the first transform launch also clears the frames' error flags */
+    const int c0 = p->ndwt[0][0] ? 0 : 1;
+    STEP(0, jxs_launch_dwt(p->d_dwt[0][c0], p->d_dbase[0][c0], p->ndwt[0][c0], p->dblocks[0][c0], 16u << c0,
+                           p->d_status, p->nfr, st));
+    if ((e = jxs_launch_dwt(p->d_dwt[0][1 - c0], p->d_dbase[0][1 - c0], p->ndwt[0][1 - c0], p->dblocks[0][1 - c0],
+                            16u << (1 - c0), nullptr, 0, st)) != hipSuccess)
         return e;
-    STEP(0, jxs_launch_dwt(p->d_dwt[0], p->d_dbase[0], p->ndwt[0], p->dblocks[0], p->dwmax[0], st));
-    STEP(1, jxs_launch_dwt(p->d_dwt[1], p->d_dbase[1], p->ndwt[1], p->dblocks[1], p->dwmax[1], st));
+    STEP(1, jxs_launch_dwt(p->d_dwt[1][0], p->d_dbase[1][0], p->ndwt[1][0], p->dblocks[1][0], 16u, nullptr, 0, st));
+    if ((e = jxs_launch_dwt(p->d_dwt[1][1], p->d_dbase[1][1], p->ndwt[1][1], p->dblocks[1][1], 32u, nullptr, 0, st)) !=
+        hipSuccess)
+        return e;
     STEP(2, jxs_launch_rate(p->d_frames, nf, p->total_slices, st));
     STEP(3, jxs_launch_pack(p->d_units, p->total_units, st));
     if (ev)

@@ -148,18 +148,22 @@ struct JxsBandOut {
 /* One forward-transform step of one component plane (w x h samples at src):
  * with `vert` set, one vertical 5/3 level followed by horizontal levels of
  * the low rows (lev_l) and one horizontal level of the high rows; without,
- * horizontal levels of every row.  One block per output row (pair):
- * blocks [base, base + rows). */
+ * horizontal levels of every row.  Blocks [base, base + nblk), rpb output
+ * rows (pairs) per block, one thread per `chunk` columns. */
 struct JxsDwtJob {
     const void* src;
     uint32_t kind, stride, step, w, h;
     uint32_t shift, mask;
     int32_t offset;
     uint32_t vert, lev_l;
-    uint32_t base, rows;
+    uint32_t base, rows; /* first block; output rows (row pairs with `vert`) */
+    uint32_t chunk;      /* columns per thread: 16 or 32 (>= 2^lev_l) */
+    uint32_t tpr;        /* threads per row: a multiple of 64, >= w / chunk */
+    uint32_t rpb;        /* rows per block: 256 / tpr */
+    uint32_t nblk;       /* blocks of the job */
     uint32_t lf_is_band; /* low rows' low-pass output: band, or int32 plane */
     uint32_t lf_stride;
-    uint32_t vec; /* rows readable 4 columns per load (step 1, aligned, width and stride multiples of 4) */
+    uint32_t vec; /* rows readable 16 bytes per load (step 1, plane and row pitch 16-byte aligned) */
     int32_t* lf_i32;
     JxsBandOut lf, hf[5]; /* low rows: final low-pass and high-pass of each level */
     JxsBandOut lh, hh;    /* high rows: low-pass and high-pass of their level */

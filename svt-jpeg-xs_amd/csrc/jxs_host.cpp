@@ -728,10 +728,17 @@ static JxsBandOut band_out(const JxsGeom& g, uint32_t c, uint32_t b, uint16_t* c
     return o;
 }
 
-/* Rows can be read four columns per load when they are contiguous, 16-byte
- * aligned at the base and every row starts on a multiple of four samples. */
-static JxsDwtJob with_vec(JxsDwtJob j) {
-    j.vec = (j.step == 1 && j.w % 4 == 0 && j.stride % 4 == 0 && ((uintptr_t)j.src & 15) == 0) ? 1u : 0u;
+/* Launch shape of a transform job: columns per thread (16, or 32 for five
+ * horizontal levels or rows wider than 256 x 16), threads per row (whole
+ * waves), rows per 256-thread block; rows are read 16 bytes per load when
+ * they are contiguous and every row starts 16-byte aligned. */
+static JxsDwtJob finish_job(JxsDwtJob j) {
+    const uint32_t elem = j.kind == JXS_SRC_U8 ? 1u : j.kind == JXS_SRC_U16 ? 2u : 4u;
+    j.vec = (j.step == 1 && (j.stride * elem) % 16 == 0 && ((uintptr_t)j.src & 15) == 0) ? 1u : 0u;
+    j.chunk = (j.lev_l >= 5 || (j.w + 15) / 16 > 256) ? 32u : 16u;
+    j.tpr = ((j.w + j.chunk - 1) / j.chunk + 63) / 64 * 64;
+    j.rpb = 256 / j.tpr;
+    j.nblk = (j.rows + j.rpb - 1) / j.rpb;
     return j;
 }
 
@@ -762,7 +769,7 @@ void fwd_jobs(const JxsGeom& g, const void* const planes[4], const uint32_t stri
                 j.hf[k] = band(nb - 1 - k);
             j.lf_is_band = 1;
             j.lf = band(0);
-            jobs.dwt[0].push_back(with_vec(j));
+            jobs.dwt[0].push_back(finish_job(j));
             continue;
         }
         const uint32_t w1 = (cp.w + 1) / 2, h1 = (cp.h + 1) / 2;
@@ -780,7 +787,7 @@ void fwd_jobs(const JxsGeom& g, const void* const planes[4], const uint32_t stri
                 j.hf[k] = band(cp.dh - k);
             j.lf_is_band = 1;
             j.lf = band(0);
-            jobs.dwt[0].push_back(with_vec(j));
+            jobs.dwt[0].push_back(finish_job(j));
             continue;
         }
         int32_t* B = scr;
@@ -789,7 +796,7 @@ void fwd_jobs(const JxsGeom& g, const void* const planes[4], const uint32_t stri
         j.lf_is_band = 0;
         j.lf_i32 = B;
         j.lf_stride = w1;
-        jobs.dwt[0].push_back(with_vec(j));
+        jobs.dwt[0].push_back(finish_job(j));
         /* second vertical level on the w1 x h1 low-pass plane */
         JxsDwtJob k2;
         memset(&k2, 0, sizeof(k2));
@@ -809,7 +816,7 @@ void fwd_jobs(const JxsGeom& g, const void* const planes[4], const uint32_t stri
         k2.lf = band(0);
         k2.lh = band(nb - 5);
         k2.hh = band(nb - 4);
-        jobs.dwt[1].push_back(with_vec(k2));
+        jobs.dwt[1].push_back(finish_job(k2));
     }
 }
 

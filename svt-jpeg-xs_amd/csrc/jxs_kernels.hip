@@ -174,144 +174,22 @@ __device__ __forceinline__ uint32_t xcd_remap(uint32_t b, uint32_t n) {
     return x * q + min(x, r) + b / NXCD;
 }
 
-#define HMAX JXS_MAX_ROW
+/* Transform layout: every output row (pair) is a row slot of whole waves;
+ * thread t of a slot owns the CH consecutive columns [t * CH, t * CH + CH)
+ * of its row(s) in registers, through the vertical step, every horizontal
+ * level and the band output.  Neighbouring threads exchange one sample per
+ * lifting step through LDS. */
+#define DW_THREADS 256
+#define DW_SLOTS 4  /* row slots per block (at most one per wave) */
+#define DW_NBAND 8  /* band rows of a slot: horizontal levels 0..4, low-pass, LH, HH */
+#define DW_LF 5
+#define DW_LH 6
+#define DW_HH 7
+#define DW_STAGED 3 /* band rows too short for whole groups per thread, staged in LDS */
 
-/* Writes `count` values buf[start + k * step] as one band row (row stride
- * o.cs, the padding of the last group zero) with its GCLIs and significance
- * maxima, and the row's histogram record when o.hist is set.  hcnt is a
- * zeroed LDS array of 32 words; it is zero again on return.  Called by all
- * threads of the block.  Straight-line per group: the four LDS reads use
- * clamped indexes and are issued together, the sign-magnitude conversion
- * is branch-free, GCLI counts come from wave ballots (lanes 0..15 count one
- * value each) and significance-maximum counts from LDS atomics (one lane in
- * eight holds a maximum). */
-__device__ __forceinline__ void emit_band_row(const JxsBandOut o, uint32_t y, const int32_t* buf, uint32_t start,
-                                              uint32_t step, uint32_t count, uint32_t* hcnt) {
-    uint16_t* crow = o.coeff + (size_t)y * o.cs;
-    uint8_t* grow = o.gcli + (size_t)y * o.gw;
-    uint8_t* srow = o.smax + (size_t)y * o.sw;
-    const uint32_t lane = threadIdx.x & (WAVE - 1);
-    const uint32_t full = o.gw / 8;
-    const bool hist = o.hist != nullptr;
-    /* per-lane masks: ~(ballot of GCLI bit k XOR mk) has a one for every
-     * lane whose GCLI bit k equals bit k of `lane` */
-    const uint64_t m0 = (lane & 1u) ? ~0ull : 0ull, m1 = (lane & 2u) ? ~0ull : 0ull;
-    const uint64_t m2 = (lane & 4u) ? ~0ull : 0ull, m3 = (lane & 8u) ? ~0ull : 0ull;
-    uint32_t mycnt = 0;
-    for (uint32_t base = 0; base < o.gw; base += blockDim.x) {
-        const uint32_t gidx = base + threadIdx.x;
-        const bool valid = gidx < o.gw;
-        const uint32_t i0 = 4 * min(gidx, o.gw - 1);
-        int32_t x[4];
-#pragma unroll
-        for (uint32_t k = 0; k < 4; k++)
-            x[k] = buf[start + min(i0 + k, count - 1) * step];
-        uint32_t v[4], orv = 0;
-#pragma unroll
-        for (uint32_t k = 0; k < 4; k++) {
-            const uint32_t a = (uint32_t)(x[k] < 0 ? -x[k] : x[k]);
-            const uint32_t mg = (a + 128) >> 8;
-            v[k] = (i0 + k < count && mg) ? (mg | (((uint32_t)x[k] >> 31) << 15)) : 0u;
-            orv |= v[k];
-        }
-        const uint32_t gc = bitlen(orv & 0x7FFFu);
-        if (valid) {
-            gst2(crow + 4 * gidx, make_uint2(v[0] | (v[1] << 16), v[2] | (v[3] << 16)));
-            gst(grow + gidx, (uint8_t)gc);
-        }
-        uint32_t m = valid ? gc : 0u;
-        m = max(m, (uint32_t)__shfl_xor((int)m, 1, WAVE));
-        m = max(m, (uint32_t)__shfl_xor((int)m, 2, WAVE));
-        m = max(m, (uint32_t)__shfl_xor((int)m, 4, WAVE));
-        const bool head = valid && (gidx & 7) == 0;
-        if (head)
-            gst(srow + gidx / 8, (uint8_t)m);
-        if (hist) {
-            const uint64_t vg = __ballot(valid);
-            const uint64_t g0 = __ballot(gc & 1u), g1 = __ballot(gc & 2u), g2 = __ballot(gc & 4u),
-                           g3 = __ballot(gc & 8u);
-            mycnt += (uint32_t)__popcll(vg & ~(g0 ^ m0) & ~(g1 ^ m1) & ~(g2 ^ m2) & ~(g3 ^ m3));
-            if (head) {
-                if (gidx / 8 >= full)
-                    hcnt[HREC_TAIL] = m + 1; /* the partial significance group */
-                else if (m < 15)
-                    atomicAdd(&hcnt[16 + m], 1u); /* maximum 15: implied by the group count */
-            }
-        }
-    }
-    if (!hist)
-        return;
-    if (lane < 16 && mycnt)
-        atomicAdd(&hcnt[lane], mycnt);
-    __syncthreads();
-    if (threadIdx.x < JXS_HREC) {
-        const uint32_t t = threadIdx.x, c = hcnt[t];
-        uint16_t* rec = o.hist + ((size_t)(y / o.hl) * o.n_bl + o.bl + y % o.hl) * JXS_HREC;
-        gst(rec + t, (uint16_t)(t == HREC_TAIL ? (c ? c - 1 : 0xFFFFu) : c));
-        hcnt[t] = 0;
-    }
-    __syncthreads();
-}
-
-/* Field-wise copy of a band output descriptor out of constant memory. */
-__device__ __forceinline__ JxsBandOut band_out(const CAS JxsBandOut& b) {
-    JxsBandOut o;
-    o.coeff = b.coeff;
-    o.gcli = b.gcli;
-    o.smax = b.smax;
-    o.hist = b.hist;
-    o.w = b.w;
-    o.gw = b.gw;
-    o.sw = b.sw;
-    o.cs = b.cs;
-    o.hl = b.hl;
-    o.bl = b.bl;
-    o.n_bl = b.n_bl;
-    o.pad_ = 0;
-    return o;
-}
-
-/* Lifts the row of w samples in buf through `levels` horizontal 5/3 levels
- * in place (the low-pass samples of level k at stride 2^k) and writes the
- * high band of every level and the final low-pass output (a band, or int32
- * samples at lf_i32).  buf is loaded and synchronised by the caller. */
-__device__ __forceinline__ void horz_emit(int32_t* buf, uint32_t w, uint32_t levels, const CAS JxsBandOut* hf,
-                                          uint32_t lf_is_band, const CAS JxsBandOut& lf, int32_t* lf_i32, uint32_t y,
-                                          uint32_t* hcnt) {
-    uint32_t n = w, s = 1;
-    for (uint32_t lvl = 0; lvl < levels; lvl++) {
-        const uint32_t nh = n / 2, nl = n - nh;
-        /* predict: hf(i) = x(2i+1) - ((x(2i) + x(2i+2)) >> 1); at an even
-         * end x(2i+2) is replaced by x(2i), which gives x(2i+1) - x(2i) */
-        for (uint32_t i = threadIdx.x; i < nh; i += blockDim.x) {
-            const uint32_t e = 2 * i + 2 < n ? 2 * i + 2 : 2 * i;
-            buf[(2 * i + 1) * s] -= (buf[2 * i * s] + buf[e * s]) >> 1;
-        }
-        __syncthreads();
-        /* update: lf(i) = x(2i) + ((hf(i-1) + hf(i) + 2) >> 2); a missing
-         * neighbour is replaced by the other one, which gives the one-sided
-         * (hf + 1) >> 1 at both ends (n >= 2 at every level) */
-        for (uint32_t i = threadIdx.x; i < nl; i += blockDim.x) {
-            const uint32_t l = i > 0 ? 2 * i - 1 : 2 * i + 1, r = i < nh ? 2 * i + 1 : 2 * i - 1;
-            buf[2 * i * s] += (buf[l * s] + buf[r * s] + 2) >> 2;
-        }
-        __syncthreads();
-        emit_band_row(band_out(hf[lvl]), y, buf, s, 2 * s, nh, hcnt);
-        s *= 2;
-        n = nl;
-    }
-    if (lf_is_band) {
-        emit_band_row(band_out(lf), y, buf, 0, s, n, hcnt);
-    } else {
-        for (uint32_t i = threadIdx.x; i < n; i += blockDim.x)
-            gst(lf_i32 + i, buf[i * s]);
-    }
-}
-
-/* Sample access for the transform, specialised per input kind so that the
- * batched loads of a block are straight-line code.  Input samples are
- * scaled as NltEnc.c:33-71 does: x = (v & mask) << shift - offset; the
- * int32 low-pass planes of the second level are read as they are. */
+/* Sample access for the transform.  Input samples are scaled as
+ * NltEnc.c:33-71 does: x = (v & mask) << shift - offset; the int32 low-pass
+ * planes of the second level are read as they are. */
 struct Scale {
     uint32_t shift, mask;
     int32_t offset;
@@ -330,20 +208,56 @@ __device__ __forceinline__ int32_t ld1(const void* row, uint32_t x, uint32_t ste
     return gld((const int32_t*)row + (size_t)x * step);
 }
 
-/* columns x .. x+3 of a row with one load (contiguous rows, x a multiple
- * of 4, 16-byte aligned plane and stride: checked by the host) */
-template <int KIND>
-__device__ __forceinline__ int4 ld4(const void* row, uint32_t x, const Scale& sc) {
-    if (KIND == JXS_SRC_U16) {
-        const u32x2 v = *(const GAS u32x2*)((const uint16_t*)row + x);
-        return make_int4(sc(v.x & 0xFFFFu), sc(v.x >> 16), sc(v.y & 0xFFFFu), sc(v.y >> 16));
+/* CH samples of a row from column x0: 16-byte loads when the row allows
+ * them (contiguous, aligned: checked by the host) and the chunk lies inside
+ * the row, else one load per sample with the column clamped into the row. */
+template <int KIND, int CH>
+__device__ __forceinline__ void load_chunk(const void* row, uint32_t x0, uint32_t w, uint32_t step, bool vec,
+                                           const Scale& sc, int32_t (&out)[CH]) {
+    if (vec && x0 + CH <= w) {
+        if (KIND == JXS_SRC_U8 && CH == 8) {
+            const u32x2 v = gld((const u32x2*)((const uint8_t*)row + x0));
+#pragma unroll
+            for (int k = 0; k < 8; k++)
+                out[k] = sc(((k < 4 ? v.x : v.y) >> (8 * (k & 3))) & 0xFFu);
+        } else if (KIND == JXS_SRC_U16) {
+            const u32x4* p = (const u32x4*)((const uint16_t*)row + x0);
+#pragma unroll
+            for (int q = 0; q < CH / 8; q++) {
+                const u32x4 v = gld(p + q);
+                const uint32_t wd[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    out[8 * q + 2 * k] = sc(wd[k] & 0xFFFFu);
+                    out[8 * q + 2 * k + 1] = sc(wd[k] >> 16);
+                }
+            }
+        } else if (KIND == JXS_SRC_U8) {
+            const u32x4* p = (const u32x4*)((const uint8_t*)row + x0);
+#pragma unroll
+            for (int q = 0; q < CH / 16; q++) {
+                const u32x4 v = gld(p + q);
+                const uint32_t wd[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+                for (int k = 0; k < 16; k++)
+                    out[16 * q + k] = sc((wd[k >> 2] >> (8 * (k & 3))) & 0xFFu);
+            }
+        } else {
+            const u32x4* p = (const u32x4*)((const int32_t*)row + x0);
+#pragma unroll
+            for (int q = 0; q < CH / 4; q++) {
+                const u32x4 v = gld(p + q);
+                out[4 * q] = (int32_t)v.x;
+                out[4 * q + 1] = (int32_t)v.y;
+                out[4 * q + 2] = (int32_t)v.z;
+                out[4 * q + 3] = (int32_t)v.w;
+            }
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < CH; k++)
+            out[k] = ld1<KIND>(row, min(x0 + k, w - 1), step, sc);
     }
-    if (KIND == JXS_SRC_U8) {
-        const uint32_t v = *(const GAS uint32_t*)((const uint8_t*)row + x);
-        return make_int4(sc(v & 0xFFu), sc((v >> 8) & 0xFFu), sc((v >> 16) & 0xFFu), sc(v >> 24));
-    }
-    const u32x4 v = *(const GAS u32x4*)((const int32_t*)row + x);
-    return make_int4((int32_t)v.x, (int32_t)v.y, (int32_t)v.z, (int32_t)v.w);
 }
 
 /* The vertical 5/3 step of one column from its input samples x(2i-2) ..
@@ -366,170 +280,394 @@ __device__ __forceinline__ void vert53(const VertFlags& f, int32_t xm2, int32_t 
     hf = hcur;
 }
 
-#define QUAD_BATCH 4 /* column quads per thread whose loads are issued together */
-#define FILL_BATCH 4 /* columns per thread whose loads are issued together */
+/* Horizontal 5/3 lifting of one level on the CH / S samples v[k * S] a
+ * thread holds (global sample index t * (CH / S) + k of a level of n
+ * samples; n >= 2).  Predict, at odd positions:
+ *   x(2i+1) -= (x(2i) + x(2i+2)) >> 1, x(2i+2) := x(2i) past the end;
+ * update, at even positions:
+ *   x(2i) += (x(2i-1) + x(2i+1) + 2) >> 2, a missing neighbour being
+ * replaced by the other one.  The sample after the thread's last one (first
+ * of the next thread) and the one before its first (the previous thread's
+ * last, after the predict) come from the neighbours. */
+template <int CH, int S>
+__device__ __forceinline__ void lift_predict(int32_t (&v)[CH], uint32_t t, uint32_t n, int32_t right_first) {
+    constexpr int M = CH / S;
+#pragma unroll
+    for (int k = 1; k < M; k += 2) {
+        const uint32_t p = t * M + k;
+        const int32_t xl = v[(k - 1) * S];
+        const int32_t xr = (k + 1 < M) ? v[(k + 1) * S] : right_first;
+        const int32_t r = (p + 1 < n) ? xr : xl;
+        const int32_t nv = v[k * S] - ((xl + r) >> 1);
+        v[k * S] = (p < n) ? nv : v[k * S];
+    }
+}
+template <int CH, int S>
+__device__ __forceinline__ void lift_update(int32_t (&v)[CH], uint32_t t, uint32_t n, int32_t left_last) {
+    constexpr int M = CH / S;
+#pragma unroll
+    for (int k = 0; k < M; k += 2) {
+        const uint32_t p = t * M + k;
+        const int32_t hr = v[(k + 1) * S];
+        const int32_t hl0 = (k > 0) ? v[(k - 1) * S] : left_last;
+        const int32_t hl = (p > 0) ? hl0 : hr;
+        const int32_t hr2 = (p + 1 < n) ? hr : hl;
+        const int32_t nv = v[k * S] + ((hl + hr2 + 2) >> 2);
+        v[k * S] = (p < n) ? nv : v[k * S];
+    }
+}
 
-/* Low row i into lo[] and, when it exists, high row i into hi[] for the w
- * columns, from the five input rows r[0..4] = 2i-2 .. 2i+2 (clamped into
- * the plane, so every load is in bounds; the flags drop what lies
- * outside).  The loads of a whole batch are issued before any is used. */
-template <int KIND, bool VEC>
-__device__ __forceinline__ void vert_fill(const void* const r[5], const VertFlags& f, uint32_t w, uint32_t step,
-                                          const Scale& sc, int32_t* lo, int32_t* hi) {
-    if (VEC) {
-        const uint32_t nq = w / 4;
-        for (uint32_t q0 = 0; q0 < nq; q0 += QUAD_BATCH * blockDim.x) {
-            int4 v[5][QUAD_BATCH];
+/* Field-wise copy of a band output descriptor out of constant memory. */
+__device__ __forceinline__ JxsBandOut band_out(const CAS JxsBandOut& b) {
+    JxsBandOut o;
+    o.coeff = b.coeff;
+    o.gcli = b.gcli;
+    o.smax = b.smax;
+    o.hist = b.hist;
+    o.w = b.w;
+    o.gw = b.gw;
+    o.sw = b.sw;
+    o.cs = b.cs;
+    o.hl = b.hl;
+    o.bl = b.bl;
+    o.n_bl = b.n_bl;
+    o.pad_ = 0;
+    return o;
+}
+
+/* Lane masks of the GCLI counting: ~(ballot of GCLI bit k XOR mask k) has a
+ * one for every lane whose GCLI bit k equals bit k of this lane's index. */
+struct LaneMasks {
+    uint64_t m0, m1, m2, m3;
+};
+
+/* Band row output of G groups of 4 coefficients per thread (groups g0 ..
+ * g0 + G - 1 of band row y, count coefficients in the row): 16-bit
+ * sign-magnitude coefficients (image_shift, NltEnc.c:12-31; zero past the
+ * end of the row), the GCLI of every group and the maximum GCLI of every 8
+ * groups (GcStageProcess.c:22-97, across 8 / G neighbouring lanes), and,
+ * into hc, the counts of the band-line histogram record: GCLI values from
+ * wave ballots (lanes 0..15 count one value each), maxima of full
+ * significance groups below 15, and the trailing partial group's maximum
+ * + 1 at HREC_TAIL.  Called by whole waves (act: this lane's row exists). */
+template <int G>
+__device__ __forceinline__ void emit_groups(const JxsBandOut& o, uint32_t y, const int32_t (&val)[4 * G], uint32_t g0,
+                                            uint32_t count, uint32_t* hc, bool act, uint32_t lane,
+                                            const LaneMasks& lm) {
+    uint16_t* crow = o.coeff + (size_t)y * o.cs;
+    uint8_t* grow = o.gcli + (size_t)y * o.gw;
+    uint8_t* srow = o.smax + (size_t)y * o.sw;
+    uint32_t mx = 0, mycnt = 0;
 #pragma unroll
-            for (uint32_t k = 0; k < QUAD_BATCH; k++) {
-                const uint32_t x = 4 * min(q0 + k * blockDim.x + threadIdx.x, nq - 1);
+    for (int gi = 0; gi < G; gi++) {
+        const uint32_t g = g0 + gi;
+        const bool gv = act && g < o.gw;
+        uint32_t v[4], orv = 0;
 #pragma unroll
-                for (uint32_t j = 0; j < 5; j++)
-                    v[j][k] = ld4<KIND>(r[j], x, sc);
-            }
-#pragma unroll
-            for (uint32_t k = 0; k < QUAD_BATCH; k++) {
-                const uint32_t q = q0 + k * blockDim.x + threadIdx.x;
-                if (q >= nq)
-                    break;
-                int4 l, h;
-                vert53(f, v[0][k].x, v[1][k].x, v[2][k].x, v[3][k].x, v[4][k].x, l.x, h.x);
-                vert53(f, v[0][k].y, v[1][k].y, v[2][k].y, v[3][k].y, v[4][k].y, l.y, h.y);
-                vert53(f, v[0][k].z, v[1][k].z, v[2][k].z, v[3][k].z, v[4][k].z, l.z, h.z);
-                vert53(f, v[0][k].w, v[1][k].w, v[2][k].w, v[3][k].w, v[4][k].w, l.w, h.w);
-                *(int4*)(lo + 4 * q) = l;
-                if (f.h)
-                    *(int4*)(hi + 4 * q) = h;
-            }
+        for (int k = 0; k < 4; k++) {
+            const int32_t x = val[4 * gi + k];
+            const uint32_t a = (uint32_t)(x < 0 ? -x : x);
+            const uint32_t mg = (a + 128) >> 8;
+            v[k] = (4 * g + k < count && mg) ? (mg | (((uint32_t)x >> 31) << 15)) : 0u;
+            orv |= v[k];
         }
-    } else {
-        for (uint32_t x0 = 0; x0 < w; x0 += FILL_BATCH * blockDim.x) {
-            int32_t v[5][FILL_BATCH];
+        const uint32_t gc = gv ? bitlen(orv & 0x7FFFu) : 0u;
+        if (gv) {
+            gst2(crow + 4 * g, make_uint2(v[0] | (v[1] << 16), v[2] | (v[3] << 16)));
+            gst(grow + g, (uint8_t)gc);
+        }
+        mx = max(mx, gc);
+        if (hc) {
+            const uint64_t vg = __ballot(gv);
+            const uint64_t b0 = __ballot(gc & 1u), b1 = __ballot(gc & 2u), b2 = __ballot(gc & 4u), b3 = __ballot(gc & 8u);
+            mycnt += (uint32_t)__popcll(vg & ~(b0 ^ lm.m0) & ~(b1 ^ lm.m1) & ~(b2 ^ lm.m2) & ~(b3 ^ lm.m3));
+        }
+    }
+    /* maximum over the 8 groups of a significance group: 8 / G lanes */
+    if (G < 2)
+        mx = max(mx, (uint32_t)__shfl_xor((int)mx, 4, WAVE));
+    if (G < 4)
+        mx = max(mx, (uint32_t)__shfl_xor((int)mx, 2, WAVE));
+    if (G < 8)
+        mx = max(mx, (uint32_t)__shfl_xor((int)mx, 1, WAVE));
+    const bool head = act && (g0 & 7) == 0 && g0 < o.gw;
+    if (head)
+        gst(srow + g0 / 8, (uint8_t)mx);
+    if (hc) {
+        if (head) {
+            if (g0 / 8 >= o.gw / 8)
+                hc[HREC_TAIL] = mx + 1; /* the partial significance group */
+            else if (mx < 15)
+                atomicAdd(&hc[16 + mx], 1u); /* maximum 15: implied by the group count */
+        }
+        if (lane < 16 && mycnt)
+            atomicAdd(&hc[lane], mycnt);
+    }
+}
+
+/* A band row staged in LDS (values st[0 .. count)), one group per thread of
+ * the slot. */
+__device__ __forceinline__ void emit_staged(const JxsBandOut& o, uint32_t y, const int32_t* st, uint32_t t,
+                                            uint32_t count, uint32_t* hc, bool act, uint32_t lane,
+                                            const LaneMasks& lm) {
+    int32_t val[4];
+    const uint32_t last = count > 0 ? count - 1 : 0;
 #pragma unroll
-            for (uint32_t k = 0; k < FILL_BATCH; k++) {
-                const uint32_t x = min(x0 + k * blockDim.x + threadIdx.x, w - 1);
+    for (int k = 0; k < 4; k++)
+        val[k] = st[min(4 * t + k, last)];
+    emit_groups<1>(o, y, val, t, count, hc, act, lane, lm);
+}
+
+/* Per-block state of the transform (LDS). */
+struct DwLds {
+    int32_t exr[2][DW_THREADS]; /* first sample of every thread (low row, high row) */
+    int32_t exl[2][DW_THREADS]; /* last sample of every thread after the predict */
+    int32_t stage[DW_STAGED][2 * DW_THREADS]; /* short band rows: at most 2 samples per thread */
+    uint32_t hc[DW_SLOTS][DW_NBAND][JXS_HREC];
+};
+
+/* What a slot's thread knows of its row. */
+struct DwRow {
+    uint32_t tid, lane, slot, t, tpr, ir;
+    bool act, right, left;
+    LaneMasks lm;
+    uint32_t st_count[DW_STAGED]; /* staged band rows: coefficient count (0: none) */
+    uint32_t st_band[DW_STAGED];
+};
+
+/* The low-pass samples lo[k * S] (k < CH / S) after the last horizontal level
+ * of the low row: the LL band, or the int32 plane of the next vertical
+ * level. */
+template <int CH, int S>
+__device__ __forceinline__ void dw_lowpass(const CAS JxsDwtJob& J, DwLds& L, DwRow& R, const int32_t (&lo)[CH],
+                                           uint32_t nlev) {
+    constexpr int M = CH / S;
+    if (J.lf_is_band) {
+        const JxsBandOut ob = band_out(J.lf);
+        if constexpr (M >= 4) {
+            int32_t v[M];
 #pragma unroll
-                for (uint32_t j = 0; j < 5; j++)
-                    v[j][k] = ld1<KIND>(r[j], x, step, sc);
-            }
+            for (int k = 0; k < M; k++)
+                v[k] = lo[k * S];
+            emit_groups<M / 4>(ob, R.ir, v, R.t * (M / 4), nlev, ob.hist ? L.hc[R.slot][DW_LF] : nullptr, R.act,
+                               R.lane, R.lm);
+        } else {
 #pragma unroll
-            for (uint32_t k = 0; k < FILL_BATCH; k++) {
-                const uint32_t x = x0 + k * blockDim.x + threadIdx.x;
-                if (x >= w)
-                    break;
-                int32_t l, h;
-                vert53(f, v[0][k], v[1][k], v[2][k], v[3][k], v[4][k], l, h);
-                lo[x] = l;
-                if (f.h)
-                    hi[x] = h;
-            }
+            for (int k = 0; k < M; k++)
+                L.stage[DW_STAGED - 1][R.slot * R.tpr * 2 + R.t * M + k] = lo[k * S];
+            R.st_count[DW_STAGED - 1] = nlev;
+            R.st_band[DW_STAGED - 1] = DW_LF;
+        }
+    } else if (R.act) {
+        int32_t* dst = J.lf_i32 + (size_t)R.ir * J.lf_stride;
+#pragma unroll
+        for (int k = 0; k < M; k++) {
+            const uint32_t x = R.t * M + k;
+            if (x < nlev)
+                gst(dst + x, lo[k * S]);
         }
     }
 }
 
-/* Input row i into lo[] (components without vertical levels). */
-template <int KIND, bool VEC>
-__device__ __forceinline__ void row_fill(const void* row, uint32_t w, uint32_t step, const Scale& sc, int32_t* lo) {
-    if (VEC) {
-        const uint32_t nq = w / 4;
-        for (uint32_t q0 = 0; q0 < nq; q0 += QUAD_BATCH * blockDim.x) {
-            int4 v[QUAD_BATCH];
+/* Horizontal level LVL of the low row (stride S = 2^LVL between the samples
+ * of this level in lo[]), the high row's single level with level 0, the
+ * output of the level's high band, then the next level or the low-pass. */
+template <int CH, int LVL>
+__device__ __forceinline__ void dw_level(const CAS JxsDwtJob& J, DwLds& L, DwRow& R, int32_t (&lo)[CH],
+                                         int32_t (&hi)[CH], bool has_hi, uint32_t w, uint32_t nlev) {
+    constexpr int S = 1 << LVL, M = CH / S;
+    const uint32_t tid = R.tid;
+    L.exr[0][tid] = lo[0];
+    if constexpr (LVL == 0)
+        L.exr[1][tid] = hi[0];
+    __syncthreads();
+    lift_predict<CH, S>(lo, R.t, nlev, R.right ? L.exr[0][tid + 1] : 0);
+    if constexpr (LVL == 0)
+        if (has_hi)
+            lift_predict<CH, 1>(hi, R.t, w, R.right ? L.exr[1][tid + 1] : 0);
+    L.exl[0][tid] = lo[(M - 1) * S];
+    if constexpr (LVL == 0)
+        L.exl[1][tid] = hi[CH - 1];
+    __syncthreads();
+    lift_update<CH, S>(lo, R.t, nlev, R.left ? L.exl[0][tid - 1] : 0);
+    if constexpr (LVL == 0)
+        if (has_hi)
+            lift_update<CH, 1>(hi, R.t, w, R.left ? L.exl[1][tid - 1] : 0);
+    const uint32_t nh = nlev / 2;
+    const JxsBandOut ob = band_out(J.hf[LVL]);
+    if constexpr (M >= 8) {
+        int32_t v[M / 2];
 #pragma unroll
-            for (uint32_t k = 0; k < QUAD_BATCH; k++)
-                v[k] = ld4<KIND>(row, 4 * min(q0 + k * blockDim.x + threadIdx.x, nq - 1), sc);
-#pragma unroll
-            for (uint32_t k = 0; k < QUAD_BATCH; k++) {
-                const uint32_t q = q0 + k * blockDim.x + threadIdx.x;
-                if (q < nq)
-                    *(int4*)(lo + 4 * q) = v[k];
-            }
-        }
+        for (int k = 0; k < M / 2; k++)
+            v[k] = lo[(2 * k + 1) * S];
+        emit_groups<M / 8>(ob, R.ir, v, R.t * (M / 8), nh, ob.hist ? L.hc[R.slot][LVL] : nullptr, R.act, R.lane,
+                           R.lm);
     } else {
-        for (uint32_t x0 = 0; x0 < w; x0 += FILL_BATCH * blockDim.x) {
-            int32_t v[FILL_BATCH];
+        /* M = 4 or 2: the first or second short level */
+        constexpr int SI = M == 4 ? 0 : 1;
 #pragma unroll
-            for (uint32_t k = 0; k < FILL_BATCH; k++)
-                v[k] = ld1<KIND>(row, min(x0 + k * blockDim.x + threadIdx.x, w - 1), step, sc);
+        for (int k = 0; k < M / 2; k++)
+            L.stage[SI][R.slot * R.tpr * 2 + R.t * (M / 2) + k] = lo[(2 * k + 1) * S];
+        R.st_count[SI] = nh;
+        R.st_band[SI] = LVL;
+    }
+    if constexpr (LVL == 0) {
+        if (has_hi) {
+            /* the high row: LH from its low samples, HH from its high ones */
+            int32_t vl[CH / 2], vh[CH / 2];
 #pragma unroll
-            for (uint32_t k = 0; k < FILL_BATCH; k++) {
-                const uint32_t x = x0 + k * blockDim.x + threadIdx.x;
-                if (x < w)
-                    lo[x] = v[k];
+            for (int k = 0; k < CH / 2; k++) {
+                vl[k] = hi[2 * k];
+                vh[k] = hi[2 * k + 1];
             }
+            const JxsBandOut olh = band_out(J.lh), ohh = band_out(J.hh);
+            emit_groups<CH / 8>(olh, R.ir, vl, R.t * (CH / 8), w - w / 2, olh.hist ? L.hc[R.slot][DW_LH] : nullptr,
+                                R.act, R.lane, R.lm);
+            emit_groups<CH / 8>(ohh, R.ir, vh, R.t * (CH / 8), w / 2, ohh.hist ? L.hc[R.slot][DW_HH] : nullptr,
+                                R.act, R.lane, R.lm);
         }
     }
+    const uint32_t nl = nlev - nh;
+    if constexpr (M > 2) {
+        if ((uint32_t)LVL + 1 < J.lev_l) {
+            dw_level<CH, LVL + 1>(J, L, R, lo, hi, false, w, nl);
+            return;
+        }
+    }
+    dw_lowpass<CH, 2 * S>(J, L, R, lo, nl);
 }
 
-/* Element size of an input kind. */
-__device__ __forceinline__ uint32_t kind_bytes(uint32_t kind) {
-    return kind == JXS_SRC_U8 ? 1u : kind == JXS_SRC_U16 ? 2u : 4u;
-}
-
-/* The work of one block of k_dwt for one input kind. */
-template <int KIND, bool VEC>
-__device__ __forceinline__ void dwt_block(const CAS JxsDwtJob& J, uint32_t i, int32_t* lo, int32_t* hi, uint32_t* hcnt) {
-    const uint32_t w = J.w, n = J.h, step = J.step;
+/* One block of the transform for one input kind and chunk width CH: the
+ * rows of J.rpb slots, each of J.tpr threads.  The number of barriers is the
+ * same for every thread of the block (the job fixes it). */
+template <int KIND, int CH>
+__device__ __forceinline__ void dwt_slots(const CAS JxsDwtJob& J, uint32_t bi, DwLds& L) {
+    DwRow R;
+    R.tid = threadIdx.x;
+    R.lane = R.tid & (WAVE - 1);
+    R.tpr = J.tpr;
+    R.slot = R.tid / R.tpr;
+    R.t = R.tid - R.slot * R.tpr;
+    const uint32_t i = bi * J.rpb + R.slot; /* output row (pair) */
+    R.act = R.slot < J.rpb && i < J.rows;
+    R.ir = R.act ? i : 0; /* inactive slots work on row 0 and write nothing */
+    R.right = R.t + 1 < R.tpr;
+    R.left = R.t > 0;
+    R.lm.m0 = (R.lane & 1u) ? ~0ull : 0ull;
+    R.lm.m1 = (R.lane & 2u) ? ~0ull : 0ull;
+    R.lm.m2 = (R.lane & 4u) ? ~0ull : 0ull;
+    R.lm.m3 = (R.lane & 8u) ? ~0ull : 0ull;
+#pragma unroll
+    for (int k = 0; k < DW_STAGED; k++)
+        R.st_count[k] = R.st_band[k] = 0;
+    const uint32_t w = J.w, n = J.h, step = J.step, x0 = R.t * CH;
+    const bool vert = J.vert != 0, vec = J.vec != 0;
     const Scale sc = {J.shift, J.mask, J.offset};
     const uint8_t* base = (const uint8_t*)J.src;
-    const size_t pitch = (size_t)J.stride * kind_bytes(KIND);
-    if (J.vert) {
-        const VertFlags f = {i > 0, i < n / 2, 2 * i + 2 < n};
-        const void* r[5];
+    const size_t pitch = (size_t)J.stride * (KIND == JXS_SRC_U8 ? 1u : KIND == JXS_SRC_U16 ? 2u : 4u);
+    const uint32_t ir = R.ir;
+    const VertFlags f = {ir > 0, ir < n / 2, 2 * ir + 2 < n};
+    const bool has_hi = vert && f.h;
+    int32_t lo[CH], hi[CH];
+    if (vert) {
+        /* input rows 2i-2 .. 2i+2, clamped into the plane (the flags drop
+         * what lies outside), 8 columns at a time */
+        const uint8_t* r[5];
 #pragma unroll
-        for (int j = 0; j < 5; j++) {
-            const int row = min(max((int)(2 * i) + j - 2, 0), (int)n - 1);
-            r[j] = base + (size_t)row * pitch;
+        for (int j = 0; j < 5; j++)
+            r[j] = base + (size_t)min(max((int)(2 * ir) + j - 2, 0), (int)n - 1) * pitch;
+#pragma unroll
+        for (int h0 = 0; h0 < CH; h0 += 8) {
+            int32_t x[5][8];
+#pragma unroll
+            for (int j = 0; j < 5; j++)
+                load_chunk<KIND, 8>(r[j], x0 + h0, w, step, vec, sc, x[j]);
+#pragma unroll
+            for (int k = 0; k < 8; k++)
+                vert53(f, x[0][k], x[1][k], x[2][k], x[3][k], x[4][k], lo[h0 + k], hi[h0 + k]);
         }
-        vert_fill<KIND, VEC>(r, f, w, step, sc, lo, hi);
     } else {
-        row_fill<KIND, VEC>(base + (size_t)i * pitch, w, step, sc, lo);
+#pragma unroll
+        for (int h0 = 0; h0 < CH; h0 += 8) {
+            int32_t x[8];
+            load_chunk<KIND, 8>(base + (size_t)ir * pitch, x0 + h0, w, step, vec, sc, x);
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+                lo[h0 + k] = x[k];
+                hi[h0 + k] = 0;
+            }
+        }
     }
+    dw_level<CH, 0>(J, L, R, lo, hi, has_hi, w, w);
+    /* short band rows, one group per thread, from LDS */
     __syncthreads();
-    horz_emit(lo, w, J.lev_l, J.hf, J.lf_is_band, J.lf, J.lf_is_band ? nullptr : J.lf_i32 + (size_t)i * J.lf_stride,
-              i, hcnt);
-    if (J.vert && i < n / 2) {
-        __syncthreads();
-        horz_emit(hi, w, 1, &J.hh, 1, J.lh, nullptr, i, hcnt);
+#pragma unroll
+    for (int k = 0; k < DW_STAGED; k++) {
+        if (R.st_count[k] == 0)
+            continue;
+        const uint32_t b = R.st_band[k];
+        const JxsBandOut ob = band_out(b == DW_LF ? J.lf : J.hf[b]);
+        emit_staged(ob, ir, &L.stage[k][R.slot * R.tpr * 2], R.t, R.st_count[k], ob.hist ? L.hc[R.slot][b] : nullptr,
+                    R.act && R.t < ob.gw, R.lane, R.lm);
+    }
+    /* histogram records of the band rows written */
+    __syncthreads();
+    for (uint32_t e = R.tid; e < DW_SLOTS * DW_NBAND * JXS_HREC; e += DW_THREADS) {
+        const uint32_t s = e / (DW_NBAND * JXS_HREC), b = (e / JXS_HREC) % DW_NBAND, k = e % JXS_HREC;
+        const uint32_t is = bi * J.rpb + s;
+        if (s >= J.rpb || is >= J.rows)
+            continue;
+        bool used;
+        const CAS JxsBandOut* od;
+        if (b < DW_LF) {
+            used = b < J.lev_l;
+            od = &J.hf[b];
+        } else if (b == DW_LF) {
+            used = J.lf_is_band != 0;
+            od = &J.lf;
+        } else {
+            used = vert && is < n / 2;
+            od = b == DW_LH ? &J.lh : &J.hh;
+        }
+        uint16_t* hist = od->hist;
+        if (!used || !hist)
+            continue;
+        const uint32_t c = L.hc[s][b][k];
+        uint16_t* rec = hist + ((size_t)(is / od->hl) * od->n_bl + od->bl + is % od->hl) * JXS_HREC;
+        gst(rec + k, (uint16_t)(k == HREC_TAIL ? (c ? c - 1 : 0xFFFFu) : c));
     }
 }
 
-/* One block per output row (pair).  With a vertical level, block i computes
- * low row i and high row i from input rows 2i-2 .. 2i+2 straight into LDS,
- * lifts the low row through the horizontal levels of the low rows and
- * writes its bands (or the int32 low-pass plane of the next vertical level),
- * then the high row through one horizontal level into the LH and HH bands.
- * The intermediate rows never leave LDS.  Without a vertical level, block i
- * lifts input row i.  Dynamic LDS: two rows of the widest job. */
-__global__ void __launch_bounds__(256) k_dwt(const JxsDwtJob* __restrict__ jobs, const uint32_t* __restrict__ bases,
-                                             int njobs, uint32_t nblocks, uint32_t wmax) {
-    extern __shared__ __align__(16) int32_t rows_lds[];
-    __shared__ uint32_t hcnt[JXS_HREC];
-    int32_t* lo = rows_lds;
-    int32_t* hi = rows_lds + ((wmax + 3) & ~3u); /* 16-byte aligned */
+/* One block per J.rpb output rows (row pairs with a vertical level) of one
+ * job: with a vertical level, row pair i = low row i and high row i from
+ * input rows 2i-2 .. 2i+2; the low row goes through the job's horizontal
+ * levels (bands of every level, then the LL band or the int32 low-pass plane
+ * of the next vertical level), the high row through one level into the LH
+ * and HH bands.  Without a vertical level, input row i through the
+ * horizontal levels.  The first launch of a plan also clears the frames'
+ * error flags (status, nstatus words). */
+template <int CH>
+__global__ void __launch_bounds__(DW_THREADS) k_dwt(const JxsDwtJob* __restrict__ jobs,
+                                                    const uint32_t* __restrict__ bases, int njobs, uint32_t nblocks,
+                                                    uint32_t* status, uint32_t nstatus) {
+    __shared__ DwLds L;
     const uint32_t blk = xcd_remap(blockIdx.x, nblocks);
     const int j = find_job_u32(bases, njobs, blk);
     const CAS JxsDwtJob& J = ((const CAS JxsDwtJob*)jobs)[j];
-    const uint32_t i = blk - J.base;
-    if (threadIdx.x < JXS_HREC)
-        hcnt[threadIdx.x] = 0;
+    const uint32_t bi = blk - J.base;
+    if (status && blockIdx.x == 0)
+        for (uint32_t k = threadIdx.x; k < nstatus; k += DW_THREADS)
+            gst(status + k, 0u);
+    for (uint32_t e = threadIdx.x; e < DW_SLOTS * DW_NBAND * JXS_HREC; e += DW_THREADS)
+        (&L.hc[0][0][0])[e] = 0;
+    __syncthreads();
     const uint32_t kind = J.kind;
-    const bool vec = J.vec != 0;
-    if (kind == JXS_SRC_U16) {
-        if (vec)
-            dwt_block<JXS_SRC_U16, true>(J, i, lo, hi, hcnt);
-        else
-            dwt_block<JXS_SRC_U16, false>(J, i, lo, hi, hcnt);
-    } else if (kind == JXS_SRC_U8) {
-        if (vec)
-            dwt_block<JXS_SRC_U8, true>(J, i, lo, hi, hcnt);
-        else
-            dwt_block<JXS_SRC_U8, false>(J, i, lo, hi, hcnt);
-    } else {
-        if (vec)
-            dwt_block<JXS_SRC_I32, true>(J, i, lo, hi, hcnt);
-        else
-            dwt_block<JXS_SRC_I32, false>(J, i, lo, hi, hcnt);
-    }
+    if (kind == JXS_SRC_U16)
+        dwt_slots<JXS_SRC_U16, CH>(J, bi, L);
+    else if (kind == JXS_SRC_U8)
+        dwt_slots<JXS_SRC_U8, CH>(J, bi, L);
+    else
+        dwt_slots<JXS_SRC_I32, CH>(J, bi, L);
 }
 
 /* Copies a frame's geometry into LDS (all GEOM_THREADS threads of the block
@@ -1424,14 +1562,17 @@ __global__ void __launch_bounds__(64 * PK_WAVES) k_pack(const JxsUnit* __restric
 /* ------------------------------------------------------------------ */
 extern "C" {
 
-hipError_t jxs_launch_dwt(const JxsDwtJob* jobs, const uint32_t* bases, int njobs, uint32_t blocks, uint32_t wmax,
-                          hipStream_t st) {
+/* jobs: all of one chunk width (16 or 32 columns per thread) */
+hipError_t jxs_launch_dwt(const JxsDwtJob* jobs, const uint32_t* bases, int njobs, uint32_t blocks, uint32_t chunk,
+                          uint32_t* status, uint32_t nstatus, hipStream_t st) {
     if (!njobs || !blocks)
         return hipSuccess;
-    if (wmax == 0 || wmax > HMAX)
+    if (chunk == 16)
+        hipLaunchKernelGGL(k_dwt<16>, dim3(blocks), dim3(DW_THREADS), 0, st, jobs, bases, njobs, blocks, status, nstatus);
+    else if (chunk == 32)
+        hipLaunchKernelGGL(k_dwt<32>, dim3(blocks), dim3(DW_THREADS), 0, st, jobs, bases, njobs, blocks, status, nstatus);
+    else
         return hipErrorInvalidValue;
-    const size_t lds = 2 * (size_t)((wmax + 3) & ~3u) * sizeof(int32_t);
-    hipLaunchKernelGGL(k_dwt, dim3(blocks), dim3(256), lds, st, jobs, bases, njobs, blocks, wmax);
     return hipGetLastError();
 }
 
