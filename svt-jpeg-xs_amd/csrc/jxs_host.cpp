@@ -730,11 +730,14 @@ static JxsBandOut band_out(const JxsGeom& g, uint32_t c, uint32_t b, uint16_t* c
 
 /* Launch shape of a transform job: columns per thread (16, or 32 for five
  * horizontal levels or rows wider than 256 x 16), threads per row (whole
- * waves), rows per 256-thread block; rows are read 16 bytes per load when
- * they are contiguous and every row starts 16-byte aligned. */
+ * waves), rows per 256-thread block; rows are read 8 samples per load when
+ * they are contiguous, every row starts 16-byte aligned and the width is a
+ * multiple of 8. */
 static JxsDwtJob finish_job(JxsDwtJob j) {
     const uint32_t elem = j.kind == JXS_SRC_U8 ? 1u : j.kind == JXS_SRC_U16 ? 2u : 4u;
-    j.vec = (j.step == 1 && (j.stride * elem) % 16 == 0 && ((uintptr_t)j.src & 15) == 0) ? 1u : 0u;
+    j.vec = (j.step == 1 && (j.stride * elem) % 16 == 0 && ((uintptr_t)j.src & 15) == 0 && j.w % 8 == 0 && j.w >= 8)
+                ? 1u
+                : 0u;
     j.chunk = (j.lev_l >= 5 || (j.w + 15) / 16 > 256) ? 32u : 16u;
     j.tpr = ((j.w + j.chunk - 1) / j.chunk + 63) / 64 * 64;
     j.rpb = 256 / j.tpr;

@@ -101,6 +101,17 @@ __device__ __forceinline__ T cld_words(const T* p) {
         d[i] = s[i];
     return v;
 }
+/* A value the same in every lane of the wave, declared so: what depends
+ * on it (addresses, loops, branches) then stays on the scalar unit. */
+__device__ __forceinline__ uint32_t wave_uniform(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_readfirstlane((int)v);
+}
+
+/* Block barrier for LDS traffic only: waits for this wave's LDS accesses,
+ * not for its global loads and stores as __syncthreads() does (the blocks
+ * that use it exchange nothing through global memory). */
+__device__ __forceinline__ void lds_barrier() { __asm__ volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
 /* Device-scope atomic OR into global memory (a global, not a flat, atomic:
  * it does not hold up the wave's later LDS accesses). */
 __device__ __forceinline__ void gor(uint32_t* p, uint32_t v) {
@@ -208,56 +219,72 @@ __device__ __forceinline__ int32_t ld1(const void* row, uint32_t x, uint32_t ste
     return gld((const int32_t*)row + (size_t)x * step);
 }
 
-/* CH samples of a row from column x0: 16-byte loads when the row allows
- * them (contiguous, aligned: checked by the host) and the chunk lies inside
- * the row, else one load per sample with the column clamped into the row. */
-template <int KIND, int CH>
-__device__ __forceinline__ void load_chunk(const void* row, uint32_t x0, uint32_t w, uint32_t step, bool vec,
-                                           const Scale& sc, int32_t (&out)[CH]) {
-    if (vec && x0 + CH <= w) {
-        if (KIND == JXS_SRC_U8 && CH == 8) {
-            const u32x2 v = gld((const u32x2*)((const uint8_t*)row + x0));
+/* Samples of a row for the transform, 16 columns [x0, x0 + 16) per call.
+ * With `vec` (contiguous rows, 16-byte aligned, width a multiple of 8 and at
+ * least 8: checked by the host) the row is read 8 samples per load, every
+ * load issued before any is used; a load past the row's end reads the row's
+ * last 8 samples instead, whose values then stand in for columns that no
+ * output uses.  Otherwise one load per sample, the column clamped into the
+ * row. */
+template <int KIND>
+struct RowRaw {
+    /* 16 samples: two 8-sample loads (four for int32 rows) */
+    u32x4 q[KIND == JXS_SRC_I32 ? 4 : 2];
+};
+template <int KIND>
+__device__ __forceinline__ RowRaw<KIND> load16_vec(const void* row, uint32_t x0, uint32_t w) {
+    RowRaw<KIND> r;
+    if constexpr (KIND == JXS_SRC_I32) {
 #pragma unroll
-            for (int k = 0; k < 8; k++)
-                out[k] = sc(((k < 4 ? v.x : v.y) >> (8 * (k & 3))) & 0xFFu);
-        } else if (KIND == JXS_SRC_U16) {
-            const u32x4* p = (const u32x4*)((const uint16_t*)row + x0);
+        for (int k = 0; k < 4; k++)
+            r.q[k] = gld((const u32x4*)((const int32_t*)row + min(x0 + 4 * k, w - 4)));
+    } else if constexpr (KIND == JXS_SRC_U16) {
 #pragma unroll
-            for (int q = 0; q < CH / 8; q++) {
-                const u32x4 v = gld(p + q);
-                const uint32_t wd[4] = {v.x, v.y, v.z, v.w};
+        for (int k = 0; k < 2; k++)
+            r.q[k] = gld((const u32x4*)((const uint16_t*)row + min(x0 + 8 * k, w - 8)));
+    } else {
 #pragma unroll
-                for (int k = 0; k < 4; k++) {
-                    out[8 * q + 2 * k] = sc(wd[k] & 0xFFFFu);
-                    out[8 * q + 2 * k + 1] = sc(wd[k] >> 16);
-                }
-            }
-        } else if (KIND == JXS_SRC_U8) {
-            const u32x4* p = (const u32x4*)((const uint8_t*)row + x0);
+        for (int k = 0; k < 2; k++) {
+            const u32x2 v = gld((const u32x2*)((const uint8_t*)row + min(x0 + 8 * k, w - 8)));
+            r.q[k] = {v.x, v.y, 0u, 0u};
+        }
+    }
+    return r;
+}
+template <int KIND>
+__device__ __forceinline__ void unpack16(const RowRaw<KIND>& r, const Scale& sc, int32_t (&out)[16]) {
+    if constexpr (KIND == JXS_SRC_I32) {
 #pragma unroll
-            for (int q = 0; q < CH / 16; q++) {
-                const u32x4 v = gld(p + q);
-                const uint32_t wd[4] = {v.x, v.y, v.z, v.w};
+        for (int k = 0; k < 4; k++) {
+            out[4 * k] = (int32_t)r.q[k].x;
+            out[4 * k + 1] = (int32_t)r.q[k].y;
+            out[4 * k + 2] = (int32_t)r.q[k].z;
+            out[4 * k + 3] = (int32_t)r.q[k].w;
+        }
+    } else if constexpr (KIND == JXS_SRC_U16) {
 #pragma unroll
-                for (int k = 0; k < 16; k++)
-                    out[16 * q + k] = sc((wd[k >> 2] >> (8 * (k & 3))) & 0xFFu);
-            }
-        } else {
-            const u32x4* p = (const u32x4*)((const int32_t*)row + x0);
+        for (int k = 0; k < 2; k++) {
+            const uint32_t wd[4] = {r.q[k].x, r.q[k].y, r.q[k].z, r.q[k].w};
 #pragma unroll
-            for (int q = 0; q < CH / 4; q++) {
-                const u32x4 v = gld(p + q);
-                out[4 * q] = (int32_t)v.x;
-                out[4 * q + 1] = (int32_t)v.y;
-                out[4 * q + 2] = (int32_t)v.z;
-                out[4 * q + 3] = (int32_t)v.w;
+            for (int j = 0; j < 4; j++) {
+                out[8 * k + 2 * j] = sc(wd[j] & 0xFFFFu);
+                out[8 * k + 2 * j + 1] = sc(wd[j] >> 16);
             }
         }
     } else {
 #pragma unroll
-        for (int k = 0; k < CH; k++)
-            out[k] = ld1<KIND>(row, min(x0 + k, w - 1), step, sc);
+        for (int k = 0; k < 2; k++)
+#pragma unroll
+            for (int j = 0; j < 8; j++)
+                out[8 * k + j] = sc(((j < 4 ? r.q[k].x : r.q[k].y) >> (8 * (j & 3))) & 0xFFu);
     }
+}
+template <int KIND>
+__device__ __forceinline__ void load16_scalar(const void* row, uint32_t x0, uint32_t w, uint32_t step, const Scale& sc,
+                                              int32_t (&out)[16]) {
+#pragma unroll
+    for (int k = 0; k < 16; k++)
+        out[k] = ld1<KIND>(row, min(x0 + k, w - 1), step, sc);
 }
 
 /* The vertical 5/3 step of one column from its input samples x(2i-2) ..
@@ -480,7 +507,7 @@ __device__ __forceinline__ void dw_level(const CAS JxsDwtJob& J, DwLds& L, DwRow
     L.exr[0][tid] = lo[0];
     if constexpr (LVL == 0)
         L.exr[1][tid] = hi[0];
-    __syncthreads();
+    lds_barrier();
     lift_predict<CH, S>(lo, R.t, nlev, R.right ? L.exr[0][tid + 1] : 0);
     if constexpr (LVL == 0)
         if (has_hi)
@@ -488,7 +515,7 @@ __device__ __forceinline__ void dw_level(const CAS JxsDwtJob& J, DwLds& L, DwRow
     L.exl[0][tid] = lo[(M - 1) * S];
     if constexpr (LVL == 0)
         L.exl[1][tid] = hi[CH - 1];
-    __syncthreads();
+    lds_barrier();
     lift_update<CH, S>(lo, R.t, nlev, R.left ? L.exl[0][tid - 1] : 0);
     if constexpr (LVL == 0)
         if (has_hi)
@@ -546,11 +573,11 @@ __device__ __forceinline__ void dwt_slots(const CAS JxsDwtJob& J, uint32_t bi, D
     R.tid = threadIdx.x;
     R.lane = R.tid & (WAVE - 1);
     R.tpr = J.tpr;
-    R.slot = R.tid / R.tpr;
+    R.slot = wave_uniform(R.tid / R.tpr); /* a slot is whole waves */
     R.t = R.tid - R.slot * R.tpr;
     const uint32_t i = bi * J.rpb + R.slot; /* output row (pair) */
     R.act = R.slot < J.rpb && i < J.rows;
-    R.ir = R.act ? i : 0; /* inactive slots work on row 0 and write nothing */
+    R.ir = wave_uniform(R.act ? i : 0); /* inactive slots work on row 0 and write nothing */
     R.right = R.t + 1 < R.tpr;
     R.left = R.t > 0;
     R.lm.m0 = (R.lane & 1u) ? ~0ull : 0ull;
@@ -571,28 +598,41 @@ __device__ __forceinline__ void dwt_slots(const CAS JxsDwtJob& J, uint32_t bi, D
     int32_t lo[CH], hi[CH];
     if (vert) {
         /* input rows 2i-2 .. 2i+2, clamped into the plane (the flags drop
-         * what lies outside), 8 columns at a time */
+         * what lies outside), 16 columns at a time */
         const uint8_t* r[5];
 #pragma unroll
         for (int j = 0; j < 5; j++)
             r[j] = base + (size_t)min(max((int)(2 * ir) + j - 2, 0), (int)n - 1) * pitch;
 #pragma unroll
-        for (int h0 = 0; h0 < CH; h0 += 8) {
-            int32_t x[5][8];
+        for (int h0 = 0; h0 < CH; h0 += 16) {
+            int32_t x[5][16];
+            if (vec) {
+                RowRaw<KIND> raw[5];
 #pragma unroll
-            for (int j = 0; j < 5; j++)
-                load_chunk<KIND, 8>(r[j], x0 + h0, w, step, vec, sc, x[j]);
+                for (int j = 0; j < 5; j++)
+                    raw[j] = load16_vec<KIND>(r[j], x0 + h0, w);
 #pragma unroll
-            for (int k = 0; k < 8; k++)
+                for (int j = 0; j < 5; j++)
+                    unpack16<KIND>(raw[j], sc, x[j]);
+            } else {
+#pragma unroll
+                for (int j = 0; j < 5; j++)
+                    load16_scalar<KIND>(r[j], x0 + h0, w, step, sc, x[j]);
+            }
+#pragma unroll
+            for (int k = 0; k < 16; k++)
                 vert53(f, x[0][k], x[1][k], x[2][k], x[3][k], x[4][k], lo[h0 + k], hi[h0 + k]);
         }
     } else {
 #pragma unroll
-        for (int h0 = 0; h0 < CH; h0 += 8) {
-            int32_t x[8];
-            load_chunk<KIND, 8>(base + (size_t)ir * pitch, x0 + h0, w, step, vec, sc, x);
+        for (int h0 = 0; h0 < CH; h0 += 16) {
+            int32_t x[16];
+            if (vec)
+                unpack16<KIND>(load16_vec<KIND>(base + (size_t)ir * pitch, x0 + h0, w), sc, x);
+            else
+                load16_scalar<KIND>(base + (size_t)ir * pitch, x0 + h0, w, step, sc, x);
 #pragma unroll
-            for (int k = 0; k < 8; k++) {
+            for (int k = 0; k < 16; k++) {
                 lo[h0 + k] = x[k];
                 hi[h0 + k] = 0;
             }
@@ -600,7 +640,7 @@ __device__ __forceinline__ void dwt_slots(const CAS JxsDwtJob& J, uint32_t bi, D
     }
     dw_level<CH, 0>(J, L, R, lo, hi, has_hi, w, w);
     /* short band rows, one group per thread, from LDS */
-    __syncthreads();
+    lds_barrier();
 #pragma unroll
     for (int k = 0; k < DW_STAGED; k++) {
         if (R.st_count[k] == 0)
@@ -611,7 +651,7 @@ __device__ __forceinline__ void dwt_slots(const CAS JxsDwtJob& J, uint32_t bi, D
                     R.act && R.t < ob.gw, R.lane, R.lm);
     }
     /* histogram records of the band rows written */
-    __syncthreads();
+    lds_barrier();
     for (uint32_t e = R.tid; e < DW_SLOTS * DW_NBAND * JXS_HREC; e += DW_THREADS) {
         const uint32_t s = e / (DW_NBAND * JXS_HREC), b = (e / JXS_HREC) % DW_NBAND, k = e % JXS_HREC;
         const uint32_t is = bi * J.rpb + s;
@@ -660,7 +700,7 @@ __global__ void __launch_bounds__(DW_THREADS) k_dwt(const JxsDwtJob* __restrict_
             gst(status + k, 0u);
     for (uint32_t e = threadIdx.x; e < DW_SLOTS * DW_NBAND * JXS_HREC; e += DW_THREADS)
         (&L.hc[0][0][0])[e] = 0;
-    __syncthreads();
+    lds_barrier();
     const uint32_t kind = J.kind;
     if (kind == JXS_SRC_U16)
         dwt_slots<JXS_SRC_U16, CH>(J, bi, L);
