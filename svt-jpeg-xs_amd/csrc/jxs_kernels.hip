@@ -1371,7 +1371,7 @@ struct WaveOut {
             if (b0 >= lo && b0 + 32 <= hi)
                 gst(ob + w, __builtin_bswap32(v));
             else if (v)
-                atomicOr(ob + w, __builtin_bswap32(v));
+                gor(ob + w, __builtin_bswap32(v));
         }
     }
     __device__ __forceinline__ void maybe_flush() {
@@ -1425,128 +1425,186 @@ __device__ __forceinline__ uint64_t interleave4(uint32_t q0, uint32_t q1, uint32
 
 /* Significance sub-packet segment of one band line: one bit per group of
  * 8 GCLIs, 1 when its maximum is <= gt (PackPrecinct.c, significance
- * flags).  Lane L codes flags 4L .. 4L+3 of a step of 256. */
-__device__ __forceinline__ void emit_sig(WaveOut& W, const uint8_t* sl, uint32_t sw, uint32_t gt) {
-    for (uint32_t base = 0; base < sw; base += IPL * WAVE) {
-        const uint32_t i0 = base + IPL * W.lane;
-        uint32_t a[IPL];
+ * flags).  Lane L codes flags 4L .. 4L+3 from the maxima sf[] it loaded;
+ * a line has at most 256 significance groups (one step). */
+__device__ __forceinline__ void sig_step(WaveOut& W, const uint32_t (&sf)[IPL], uint32_t sw, uint32_t gt) {
+    const uint32_t i0 = IPL * W.lane;
+    uint32_t nb = 0;
+    uint64_t v = 0;
 #pragma unroll
-        for (uint32_t k = 0; k < IPL; k++)
-            a[k] = gld(sl + min(i0 + k, sw - 1));
-        uint32_t nb = 0;
-        uint64_t v = 0;
-#pragma unroll
-        for (uint32_t k = 0; k < IPL; k++)
-            if (i0 + k < sw) {
-                v = (v << 1) | (a[k] <= gt ? 1u : 0u);
-                nb++;
-            }
-        const uint32_t incl = wave_incl_scan(nb, W.lane);
-        if (nb && v)
-            W.put(W.pos + incl - nb, v, nb);
-        W.pos += wave_last(incl);
-        W.maybe_flush();
-    }
+    for (uint32_t k = 0; k < IPL; k++)
+        if (i0 + k < sw) {
+            v = (v << 1) | (sf[k] <= gt ? 1u : 0u);
+            nb++;
+        }
+    const uint32_t incl = wave_incl_scan(nb, W.lane);
+    if (nb && v)
+        W.put(W.pos + incl - nb, v, nb);
+    W.pos += wave_last(incl);
 }
 
-/* Bit-plane-count and data segments of one band line together, 64 * N
- * groups per step: lane L codes groups N*L .. N*L+N-1 of the step (N = 4
- * for wide lines, 1 for lines of at most 64 groups).  Bit-plane counts are
- * unary (gcli - gt ones and a zero, none for a group inside an
- * insignificant significance group) or raw 4-bit GCLIs; a lane's codes are
- * concatenated (at most 64 bits).  Data: per coded group a sign nibble,
- * then bit-planes gcli-1 .. gt of the quantised magnitudes, 4 bits each;
- * groups no lane codes are skipped.  The inputs of a step are loaded before
- * any of them is coded. */
+/* The inputs of one step of a band line's bit-plane-count and data
+ * segments for one lane: groups i0 .. i0+N-1 (clamped into the line, so
+ * every load is in bounds): GCLIs, significance maxima, coefficients. */
+template <uint32_t N>
+struct GdIn {
+    uint32_t a[N], s8[N];
+    uint2 cv[N];
+};
+template <uint32_t N>
+__device__ __forceinline__ GdIn<N> gd_load(const uint8_t* gl, const uint8_t* sl, const uint16_t* cl, uint32_t gw,
+                                           uint32_t ms, uint32_t i0) {
+    GdIn<N> in;
+#pragma unroll
+    for (uint32_t k = 0; k < N; k++) {
+        const uint32_t idx = min(i0 + k, gw - 1);
+        in.a[k] = gld(gl + idx);
+        in.s8[k] = ms ? gld(sl + idx / 8) : 0u;
+        in.cv[k] = gld2(cl + 4 * idx);
+    }
+    return in;
+}
+
+/* One step of the bit-plane-count and data segments, 64 * N groups: lane L
+ * codes groups i0 .. i0+N-1 (i0 = base + N * L).  Bit-plane counts are unary
+ * (gcli - gt ones and a zero, none for a group inside an insignificant
+ * significance group) or raw 4-bit GCLIs; a lane's codes are concatenated
+ * (at most 64 bits).  Data: per coded group a sign nibble, then bit-planes
+ * gcli-1 .. gt of the quantised magnitudes, 4 bits each; groups no lane
+ * codes are skipped. */
 template <uint32_t N, bool UNIFORM>
-__device__ __forceinline__ void emit_gd(WaveOut& B, WaveOut& C, const uint8_t* gl, const uint8_t* sl,
-                                        const uint16_t* cl, uint32_t w, uint32_t gw, uint32_t gt, uint32_t ms,
-                                        uint32_t raw) {
+__device__ __forceinline__ void gd_step(WaveOut& B, WaveOut& C, const GdIn<N>& in, uint32_t i0, uint32_t w,
+                                        uint32_t gw, uint32_t gt, uint32_t ms, uint32_t raw) {
     const uint32_t lane = B.lane;
-    for (uint32_t base = 0; base < gw; base += N * WAVE) {
-        const uint32_t i0 = base + N * lane;
-        uint32_t a[N], s8[N];
-        uint2 cv[N];
+    /* bit-plane counts */
+    uint32_t nb = 0;
+    uint64_t v = 0;
 #pragma unroll
-        for (uint32_t k = 0; k < N; k++) {
-            const uint32_t idx = min(i0 + k, gw - 1); /* clamped: every load in bounds */
-            a[k] = gld(gl + idx);
-            s8[k] = ms ? gld(sl + idx / 8) : 0u;
-            cv[k] = gld2(cl + 4 * idx);
+    for (uint32_t k = 0; k < N; k++) {
+        if (i0 + k >= gw)
+            break;
+        uint32_t len, code;
+        if (raw) {
+            len = 4;
+            code = in.a[k];
+        } else if (ms && in.s8[k] <= gt) {
+            len = 0;
+            code = 0;
+        } else {
+            const uint32_t m = in.a[k] > gt ? in.a[k] - gt : 0u;
+            len = m + 1;
+            code = ((1u << m) - 1) << 1;
         }
-        /* bit-plane counts */
-        uint32_t nb = 0;
-        uint64_t v = 0;
-#pragma unroll
-        for (uint32_t k = 0; k < N; k++) {
-            if (i0 + k >= gw)
-                break;
-            uint32_t len, code;
-            if (raw) {
-                len = 4;
-                code = a[k];
-            } else if (ms && s8[k] <= gt) {
-                len = 0;
-                code = 0;
-            } else {
-                const uint32_t m = a[k] > gt ? a[k] - gt : 0u;
-                len = m + 1;
-                code = ((1u << m) - 1) << 1;
-            }
-            v = (v << len) | code;
-            nb += len;
-        }
-        const uint32_t incl = wave_incl_scan(nb, lane);
-        if (nb && v)
-            B.put(B.pos + incl - nb, v, nb);
-        B.pos += wave_last(incl);
-        /* data */
-        uint64_t dv[N];
-        uint32_t dl[N], dn = 0;
-#pragma unroll
-        for (uint32_t k = 0; k < N; k++) {
-            const uint32_t i = i0 + k, gc = a[k];
-            const bool coded = i < gw && gc > gt;
-            dl[k] = 0;
-            dv[k] = 0;
-            if (__ballot(coded) == 0)
-                continue;
-            const uint32_t np = coded ? gc - gt : 0u;
-            const uint32_t c[4] = {cv[k].x & 0xFFFFu, cv[k].x >> 16, cv[k].y & 0xFFFFu, cv[k].y >> 16};
-            uint32_t q[4], signs = 0;
-#pragma unroll
-            for (uint32_t e = 0; e < 4; e++) {
-                q[e] = (coded && 4 * i + e < w) ? quant_mag<UNIFORM>(c[e], gc, gt) : 0u;
-                signs |= (q[e] && (c[e] & 0x8000u)) ? (8u >> e) : 0u;
-            }
-            const bool any_high = __ballot(np > 8) != 0;
-            const uint64_t planes = interleave4(q[0], q[1], q[2], q[3], any_high);
-            dl[k] = coded ? 4 * np + 4 : 0u;
-            dv[k] = coded ? ((uint64_t)signs << (4 * np)) | planes : 0u;
-            dn += dl[k];
-        }
-        const uint32_t dincl = wave_incl_scan(dn, lane);
-        uint32_t at = C.pos + dincl - dn;
-#pragma unroll
-        for (uint32_t k = 0; k < N; k++) {
-            if (dl[k] && dv[k])
-                C.put(at, dv[k], dl[k]);
-            at += dl[k];
-        }
-        C.pos += wave_last(dincl);
-        B.maybe_flush();
-        C.maybe_flush();
+        v = (v << len) | code;
+        nb += len;
     }
+    const uint32_t incl = wave_incl_scan(nb, lane);
+    if (nb && v)
+        B.put(B.pos + incl - nb, v, nb);
+    B.pos += wave_last(incl);
+    /* data */
+    uint64_t dv[N];
+    uint32_t dl[N], dn = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < N; k++) {
+        const uint32_t i = i0 + k, gc = in.a[k];
+        const bool coded = i < gw && gc > gt;
+        dl[k] = 0;
+        dv[k] = 0;
+        if (__ballot(coded) == 0)
+            continue;
+        const uint32_t np = coded ? gc - gt : 0u;
+        const uint32_t c[4] = {in.cv[k].x & 0xFFFFu, in.cv[k].x >> 16, in.cv[k].y & 0xFFFFu, in.cv[k].y >> 16};
+        uint32_t q[4], signs = 0;
+#pragma unroll
+        for (uint32_t e = 0; e < 4; e++) {
+            q[e] = (coded && 4 * i + e < w) ? quant_mag<UNIFORM>(c[e], gc, gt) : 0u;
+            signs |= (q[e] && (c[e] & 0x8000u)) ? (8u >> e) : 0u;
+        }
+        const bool any_high = __ballot(np > 8) != 0;
+        const uint64_t planes = interleave4(q[0], q[1], q[2], q[3], any_high);
+        dl[k] = coded ? 4 * np + 4 : 0u;
+        dv[k] = coded ? ((uint64_t)signs << (4 * np)) | planes : 0u;
+        dn += dl[k];
+    }
+    const uint32_t dincl = wave_incl_scan(dn, lane);
+    uint32_t at = C.pos + dincl - dn;
+#pragma unroll
+    for (uint32_t k = 0; k < N; k++) {
+        if (dl[k] && dv[k])
+            C.put(at, dv[k], dl[k]);
+        at += dl[k];
+    }
+    C.pos += wave_last(dincl);
 }
 
-/* One wave per band line of a precinct (PackPrecinct.c:235-572): its
- * significance flags, bit-plane counts (unary or raw) and data bit-planes
- * with the quantisation (Quant.c:9-45) fused, each segment into its own
- * sub-packet; the first band line of a packet also writes the packet
- * header (PackHeaders.c:98-127, sign size 0).  Every unit is described by
- * one self-contained record of the packet plan, so a wave makes one
- * descriptor load; the GCLIs and coefficients of the line are read once.
- * Waves never wait on each other. */
+/* One packing unit (one band line of a precinct, PackPrecinct.c:235-572)
+ * by one wave: its significance flags, bit-plane counts (unary or raw) and
+ * data bit-planes with the quantisation (Quant.c:9-45) fused, each segment
+ * into its own sub-packet; the first band line of a packet also writes the
+ * packet header (PackHeaders.c:98-127, sign size 0).  The significance
+ * maxima and the first step's inputs are loaded together as soon as the
+ * unit record is in, and every step's successor is loaded before the step
+ * is coded.  Returns whether a segment's length differs from the plan's. */
+template <uint32_t N, bool UNIFORM>
+__device__ __forceinline__ bool pack_unit(const CAS JxsUnit& U, uint32_t lane, uint32_t* ringb, uint32_t* ringc) {
+    const uint8_t* gl = U.gl;
+    const uint8_t* sl = U.sl;
+    const uint16_t* cl = U.cl;
+    const uint32_t w = U.w, gw = U.gw, sw = U.sw, gt = U.gt, ms = U.ms, raw = U.raw;
+    const bool has_sig = U.len[0] != 0, has_gd = (U.len[1] | U.len[2]) != 0;
+    uint32_t sf[IPL] = {0, 0, 0, 0};
+    if (has_sig) {
+#pragma unroll
+        for (uint32_t k = 0; k < IPL; k++)
+            sf[k] = gld(sl + min(IPL * lane + k, sw - 1));
+    }
+    GdIn<N> cur;
+    if (has_gd)
+        cur = gd_load<N>(gl, sl, cl, gw, ms, N * lane);
+    if (U.hlen) {
+        uint32_t w0, w1, w2;
+        split_bits(U.hdr, U.hlen, U.hlo, w0, w1, w2);
+        const uint32_t piece = lane == 0 ? w0 : lane == 1 ? w1 : w2;
+        if (lane < 3 && piece)
+            gor(U.hob + lane, __builtin_bswap32(piece));
+    }
+    bool bad = false;
+    WaveOut B, C;
+    if (has_sig) {
+        B.start(ringb, RINGB - 1, U.ob[0], U.lo[0], U.len[0], lane);
+        wave_sync();
+        sig_step(B, sf, sw, gt);
+        B.finish();
+        bad |= B.pos != B.hi;
+    }
+    if (has_gd) {
+        wave_sync();
+        B.start(ringb, RINGB - 1, U.ob[1], U.lo[1], U.len[1], lane);
+        C.start(ringc, RINGC - 1, U.ob[2], U.lo[2], U.len[2], lane);
+        wave_sync();
+        for (uint32_t base = 0; base < gw; base += N * WAVE) {
+            const bool more = base + N * WAVE < gw;
+            GdIn<N> nxt;
+            if (more)
+                nxt = gd_load<N>(gl, sl, cl, gw, ms, base + N * WAVE + N * lane);
+            gd_step<N, UNIFORM>(B, C, cur, base + N * lane, w, gw, gt, ms, raw);
+            B.maybe_flush();
+            C.maybe_flush();
+            if (more)
+                cur = nxt;
+        }
+        B.finish();
+        C.finish();
+        bad |= B.pos != B.hi || C.pos != C.hi;
+    }
+    return bad;
+}
+
+/* One wave per packing unit (band line of a precinct).  Every unit is
+ * described by one self-contained record of the packet plan, so a wave
+ * makes one descriptor load; the GCLIs and coefficients of the line are
+ * read once.  Waves never wait on each other. */
 __global__ void __launch_bounds__(64 * PK_WAVES) k_pack(const JxsUnit* __restrict__ units, uint32_t total_units) {
     __shared__ uint32_t ringb[PK_WAVES][RINGB];
     __shared__ uint32_t ringc[PK_WAVES][RINGC];
@@ -1555,46 +1613,17 @@ __global__ void __launch_bounds__(64 * PK_WAVES) k_pack(const JxsUnit* __restric
     if (gu >= total_units)
         return;
     const CAS JxsUnit& U = ((const CAS JxsUnit*)units)[gu];
-    bool bad = false;
-    if (U.hlen) {
-        uint32_t w0, w1, w2;
-        split_bits(U.hdr, U.hlen, U.hlo, w0, w1, w2);
-        const uint32_t piece = lane == 0 ? w0 : lane == 1 ? w1 : w2;
-        if (lane < 3 && piece)
-            atomicOr(U.hob + lane, __builtin_bswap32(piece));
-    }
-    WaveOut B, C;
-    if (U.len[0]) {
-        B.start(ringb[wid], RINGB - 1, U.ob[0], U.lo[0], U.len[0], lane);
-        wave_sync();
-        emit_sig(B, U.sl, U.sw, U.gt);
-        B.finish();
-        bad |= B.pos != B.hi;
-    }
-    if (U.len[1] | U.len[2]) {
-        wave_sync();
-        B.start(ringb[wid], RINGB - 1, U.ob[1], U.lo[1], U.len[1], lane);
-        C.start(ringc[wid], RINGC - 1, U.ob[2], U.lo[2], U.len[2], lane);
-        wave_sync();
-        const bool narrow = U.gw <= WAVE;
-        if (U.qpih) {
-            if (narrow)
-                emit_gd<1, true>(B, C, U.gl, U.sl, U.cl, U.w, U.gw, U.gt, U.ms, U.raw);
-            else
-                emit_gd<IPL, true>(B, C, U.gl, U.sl, U.cl, U.w, U.gw, U.gt, U.ms, U.raw);
-        } else {
-            if (narrow)
-                emit_gd<1, false>(B, C, U.gl, U.sl, U.cl, U.w, U.gw, U.gt, U.ms, U.raw);
-            else
-                emit_gd<IPL, false>(B, C, U.gl, U.sl, U.cl, U.w, U.gw, U.gt, U.ms, U.raw);
-        }
-        B.finish();
-        C.finish();
-        bad |= B.pos != B.hi || C.pos != C.hi;
-    }
+    const bool narrow = U.gw <= WAVE;
+    bool bad;
+    if (U.qpih)
+        bad = narrow ? pack_unit<1, true>(U, lane, ringb[wid], ringc[wid])
+                     : pack_unit<IPL, true>(U, lane, ringb[wid], ringc[wid]);
+    else
+        bad = narrow ? pack_unit<1, false>(U, lane, ringb[wid], ringc[wid])
+                     : pack_unit<IPL, false>(U, lane, ringb[wid], ringc[wid]);
     /* the plan's lengths are the segments' exact bit counts */
     if (bad && lane == 0)
-        atomicOr(U.status, 2u);
+        gor(U.status, 2u);
 }
 
 /* ------------------------------------------------------------------ */
