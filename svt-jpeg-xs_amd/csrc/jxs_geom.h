@@ -72,6 +72,7 @@ struct JxsGeom {
      * packet, u_gi global band, u_info the unit word below) */
     uint32_t n_units, pad_u[3];
     uint8_t u_p[JXS_NU], u_gi[JXS_NU];
+    uint8_t u_rank[JXS_NU]; /* place of the unit in the frame's unit array: widest lines first */
     uint32_t u_info[JXS_NU];
     uint32_t ln_info[JXS_NBL]; /* per band line of a precinct, the line word below */
     /* the rate control's walk over the band lines present in a normal / the
@@ -179,7 +180,7 @@ struct JxsFrameDesc {
     uint8_t* smax;
     uint16_t* hist;  /* rows x n_bl x JXS_HREC */
     JxsPrec* prec;   /* rows */
-    JxsUnit* units;  /* rows x n_units packing units */
+    JxsUnit* units;  /* packing units: unit u of precinct row r at u_rank[u] * rows + r */
     uint8_t* out;    /* codestream */
     uint32_t* status;
     uint32_t rows, npk;

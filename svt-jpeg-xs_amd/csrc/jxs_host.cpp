@@ -217,6 +217,26 @@ static int unit_words(JxsGeom& g) {
         g.u_info[u] = line | (bd.gain << 8) | (bd.prio << 16) | ((l < bd.lines[0] ? 1u : 0u) << UI_NORMAL) |
                       ((l < bd.lines[1] ? 1u : 0u) << (UI_NORMAL + 1)) | (last << UI_LAST) | (p << UI_PKT);
     }
+    /* packing order: the units of the widest band lines of every precinct
+     * first, so the longest packing waves start first */
+    {
+        uint32_t order[JXS_NU];
+        for (uint32_t u = 0; u < g.n_units; u++)
+            order[u] = u;
+        auto width = [&](uint32_t u) {
+            const uint32_t gi = g.u_gi[u];
+            return g.c[g.gc[gi]].b[g.gb[gi]].gw;
+        };
+        for (uint32_t a = 1; a < g.n_units; a++) /* stable insertion sort, widest first */
+            for (uint32_t b = a; b > 0 && width(order[b]) > width(order[b - 1]); b--) {
+                const uint32_t t = order[b];
+                order[b] = order[b - 1];
+                order[b - 1] = t;
+            }
+        memset(g.u_rank, 0, sizeof(g.u_rank));
+        for (uint32_t r = 0; r < g.n_units; r++)
+            g.u_rank[order[r]] = (uint8_t)r;
+    }
     memset(g.rq, 0, sizeof(g.rq));
     memset(g.rq_raw, 0, sizeof(g.rq_raw));
     for (uint32_t t = 0; t < 2; t++) {

@@ -991,11 +991,12 @@ __device__ __forceinline__ void plan_precinct(const JxsFrameDesc& F, const JxsGe
                                               uint32_t lane) {
     const uint32_t nu = g.n_units;
     const bool unit = lane < nu;
-    JxsUnit* UR = F.units + (size_t)row * nu;
+    /* this lane's unit record: unit-major order, widest lines first */
+    JxsUnit* UR = F.units + (size_t)g.u_rank[lane] * g.rows + row;
     if (P.err) {
         /* nothing of a failed precinct is packed */
         if (unit)
-            unit_empty(UR + lane);
+            unit_empty(UR);
         return;
     }
     const uint32_t p = unit ? g.u_p[lane] : 0u, gi = unit ? g.u_gi[lane] : 0u, l = g.pk_line[p];
@@ -1031,7 +1032,7 @@ __device__ __forceinline__ void plan_precinct(const JxsFrameDesc& F, const JxsGe
         (uint64_t)P.offset + P.total + eoc <= g.lcod;
     if (!valid) {
         if (unit)
-            unit_empty(UR + lane);
+            unit_empty(UR);
         if (lane == 0)
             gor(F.status, 2u);
         return;
@@ -1087,7 +1088,7 @@ __device__ __forceinline__ void plan_precinct(const JxsFrameDesc& F, const JxsGe
                                 : ((uint64_t)raw << 55) | ((uint64_t)db << 35) | ((uint64_t)gbb << 15);
         }
         U.status = F.status;
-        gst_words(UR + lane, U); /* units are 144-byte records of a 256-byte aligned array */
+        gst_words(UR, U); /* units are 144-byte records of a 256-byte aligned array */
     }
     uint8_t* o8 = F.out;
     /* packet bodies and padding start as zeros: the packing units OR their
