@@ -814,7 +814,7 @@ void fwd_jobs(const JxsGeom& g, const void* const planes[4], const uint32_t stri
             continue;
         }
         int32_t* B = scr;
-        scr += (size_t)w1 * h1;
+        scr += ((size_t)w1 * h1 + 3) & ~(size_t)3; /* every plane 16-byte aligned */
         j.lev_l = 1;
         j.lf_is_band = 0;
         j.lf_i32 = B;

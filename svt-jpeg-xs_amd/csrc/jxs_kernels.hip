@@ -487,9 +487,22 @@ __device__ __forceinline__ void dw_lowpass(const CAS JxsDwtJob& J, DwLds& L, DwR
         }
     } else if (R.act) {
         int32_t* dst = J.lf_i32 + (size_t)R.ir * J.lf_stride;
+        const uint32_t x0 = R.t * M;
+        if constexpr (M % 4 == 0) {
+            if ((J.lf_stride & 3) == 0 && x0 + M <= nlev) {
+                /* 16-byte stores: plane 16-byte aligned, rows a multiple of 4 samples */
+#pragma unroll
+                for (int k = 0; k < M; k += 4) {
+                    const u32x4 v = {(uint32_t)lo[k * S], (uint32_t)lo[(k + 1) * S], (uint32_t)lo[(k + 2) * S],
+                                     (uint32_t)lo[(k + 3) * S]};
+                    gst((u32x4*)(dst + x0 + k), v);
+                }
+                return;
+            }
+        }
 #pragma unroll
         for (int k = 0; k < M; k++) {
-            const uint32_t x = R.t * M + k;
+            const uint32_t x = x0 + k;
             if (x < nlev)
                 gst(dst + x, lo[k * S]);
         }
@@ -650,31 +663,32 @@ __device__ __forceinline__ void dwt_slots(const CAS JxsDwtJob& J, uint32_t bi, D
         emit_staged(ob, ir, &L.stage[k][R.slot * R.tpr * 2], R.t, R.st_count[k], ob.hist ? L.hc[R.slot][b] : nullptr,
                     R.act && R.t < ob.gw, R.lane, R.lm);
     }
-    /* histogram records of the band rows written */
+    /* histogram records of the band rows written, by the first wave of
+     * every slot, band after band (uniform descriptors) */
     lds_barrier();
-    for (uint32_t e = R.tid; e < DW_SLOTS * DW_NBAND * JXS_HREC; e += DW_THREADS) {
-        const uint32_t s = e / (DW_NBAND * JXS_HREC), b = (e / JXS_HREC) % DW_NBAND, k = e % JXS_HREC;
-        const uint32_t is = bi * J.rpb + s;
-        if (s >= J.rpb || is >= J.rows)
-            continue;
-        bool used;
-        const CAS JxsBandOut* od;
-        if (b < DW_LF) {
-            used = b < J.lev_l;
-            od = &J.hf[b];
-        } else if (b == DW_LF) {
-            used = J.lf_is_band != 0;
-            od = &J.lf;
-        } else {
-            used = vert && is < n / 2;
-            od = b == DW_LH ? &J.lh : &J.hh;
+    if (R.act && R.t < WAVE) {
+        for (uint32_t b = 0; b < DW_NBAND; b++) {
+            bool used;
+            const CAS JxsBandOut* od;
+            if (b < DW_LF) {
+                used = b < J.lev_l;
+                od = &J.hf[b];
+            } else if (b == DW_LF) {
+                used = J.lf_is_band != 0;
+                od = &J.lf;
+            } else {
+                used = has_hi;
+                od = b == DW_LH ? &J.lh : &J.hh;
+            }
+            uint16_t* hist = od->hist;
+            if (!used || !hist)
+                continue;
+            if (R.lane < JXS_HREC) {
+                const uint32_t k = R.lane, c = L.hc[R.slot][b][k];
+                uint16_t* rec = hist + ((size_t)(ir / od->hl) * od->n_bl + od->bl + ir % od->hl) * JXS_HREC;
+                gst(rec + k, (uint16_t)(k == HREC_TAIL ? (c ? c - 1 : 0xFFFFu) : c));
+            }
         }
-        uint16_t* hist = od->hist;
-        if (!used || !hist)
-            continue;
-        const uint32_t c = L.hc[s][b][k];
-        uint16_t* rec = hist + ((size_t)(is / od->hl) * od->n_bl + od->bl + is % od->hl) * JXS_HREC;
-        gst(rec + k, (uint16_t)(k == HREC_TAIL ? (c ? c - 1 : 0xFFFFu) : c));
     }
 }
 
