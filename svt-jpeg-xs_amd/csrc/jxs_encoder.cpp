@@ -118,6 +118,13 @@ struct EncoderPriv {
     };
     std::vector<Slot> slots;
     std::deque<int> pending; /* in send order */
+    /* slice packetization (FinalStageProcess.c:126-213): the frame whose
+     * packets are being handed out, the next packet (0: header, 1 + s: slice
+     * s) and its byte offset in the codestream */
+    bool slice_mode = false;
+    int rel_slot = -1;
+    uint32_t rel_pkt = 0, rel_off = 0;
+    bool rel_failed = false;
     std::mutex mu;
     std::condition_variable cv_free, cv_pending;
     hipStream_t stream = nullptr;
@@ -354,6 +361,7 @@ static SvtJxsErrorType_t encoder_create(svt_jpeg_xs_encoder_api_t* api) {
     if (!e)
         return SvtJxsErrorInsufficientResources;
     e->verbose = api->verbose;
+    e->slice_mode = api->slice_packetization_mode == 1;
     int32_t err = build_encoder_geometry(p, e->geom.host, api->verbose >= VERBOSE_ERRORS);
     if (err) {
         delete e;
@@ -580,43 +588,102 @@ PREFIX_API SvtJxsErrorType_t svt_jpeg_xs_encoder_send_picture(svt_jpeg_xs_encode
     return SvtJxsErrorNone;
 }
 
+/* Takes the oldest frame off the pending queue once its GPU work is done
+ * (waiting for it when blocking); -1 when none is ready. */
+static int next_finished(EncoderPriv* e, bool blocking, bool& failed) {
+    int slot;
+    {
+        std::unique_lock<std::mutex> lk(e->mu);
+        if (e->pending.empty()) {
+            if (!blocking)
+                return -1;
+            e->cv_pending.wait(lk, [&] { return !e->pending.empty(); });
+        }
+        slot = e->pending.front();
+        if (!blocking && hipEventQuery(e->slots[slot].done) == hipErrorNotReady)
+            return -1;
+        e->pending.pop_front();
+    }
+    EncoderPriv::Slot& s = e->slots[slot];
+    const hipError_t he = hipEventSynchronize(s.done);
+    failed = he != hipSuccess || *s.h_status != 0;
+    return slot;
+}
+
+/* Returns a slot to the free pool. */
+static void release_slot(EncoderPriv* e, svt_jpeg_xs_encoder_api_t* enc_api, int slot) {
+    {
+        std::lock_guard<std::mutex> lk(e->mu);
+        e->slots[slot].busy = false;
+    }
+    e->cv_free.notify_one();
+    if (enc_api->callback_send_data_available)
+        enc_api->callback_send_data_available(enc_api, enc_api->callback_send_data_available_context);
+}
+
+/* Whole frames (EncHandle.c:1030-1060, FinalStageProcess.c:216-239), or with
+ * slice packetization the frame header, then one packet per slice into
+ * the same buffer, the last one marked last and releasable
+ * (FinalStageProcess.c:126-213).  Callers hand packets out in order from one
+ * thread (as the reference's single output queue does). */
 PREFIX_API SvtJxsErrorType_t svt_jpeg_xs_encoder_get_packet(svt_jpeg_xs_encoder_api_t* enc_api,
                                                             svt_jpeg_xs_frame_t* out, uint8_t blocking_flag) {
     EncoderPriv* e = priv_of(enc_api);
     if (!e || !out)
         return SvtJxsErrorBadParameter;
-    int slot;
-    {
-        std::unique_lock<std::mutex> lk(e->mu);
-        if (e->pending.empty()) {
-            if (!blocking_flag) {
-                out->user_prv_ctx_ptr = nullptr;
-                return SvtJxsErrorNoErrorEmptyQueue;
-            }
-            e->cv_pending.wait(lk, [&] { return !e->pending.empty(); });
-        }
-        slot = e->pending.front();
-        if (!blocking_flag && hipEventQuery(e->slots[slot].done) == hipErrorNotReady) {
+    const JxsGeom& g = e->geom.host.g;
+    if (!e->slice_mode) {
+        bool failed = false;
+        const int slot = next_finished(e, blocking_flag != 0, failed);
+        if (slot < 0) {
             out->user_prv_ctx_ptr = nullptr;
             return SvtJxsErrorNoErrorEmptyQueue;
         }
-        e->pending.pop_front();
+        *out = e->slots[slot].frame;
+        out->bitstream.used_size = g.lcod;
+        out->bitstream.ready_to_release = 1;
+        out->bitstream.last_packet_in_frame = 1;
+        out->image.ready_to_release = 1;
+        release_slot(e, enc_api, slot);
+        return failed ? SvtJxsErrorEncodeFrameError : SvtJxsErrorNone;
     }
-    EncoderPriv::Slot& s = e->slots[slot];
-    hipError_t he = hipEventSynchronize(s.done);
-    *out = s.frame;
-    out->bitstream.used_size = e->geom.host.g.lcod;
-    out->bitstream.ready_to_release = 1;
-    out->bitstream.last_packet_in_frame = 1;
-    out->image.ready_to_release = 1;
-    const bool failed = he != hipSuccess || *s.h_status != 0;
-    {
-        std::lock_guard<std::mutex> lk(e->mu);
-        s.busy = false;
+    if (e->rel_slot < 0) {
+        bool failed = false;
+        const int slot = next_finished(e, blocking_flag != 0, failed);
+        if (slot < 0) {
+            out->user_prv_ctx_ptr = nullptr;
+            return SvtJxsErrorNoErrorEmptyQueue;
+        }
+        e->rel_slot = slot;
+        e->rel_pkt = 0;
+        e->rel_off = 0;
+        e->rel_failed = failed;
     }
-    e->cv_free.notify_one();
-    if (enc_api->callback_send_data_available)
-        enc_api->callback_send_data_available(enc_api, enc_api->callback_send_data_available_context);
+    const int slot = e->rel_slot;
+    *out = e->slots[slot].frame;
+    out->bitstream.last_packet_in_frame = 0;
+    out->bitstream.ready_to_release = 0;
+    out->image.ready_to_release = 0;
+    bool last = false;
+    if (e->rel_pkt == 0) {
+        out->bitstream.used_size = g.hdr_len;
+        e->rel_off = g.hdr_len;
+    } else {
+        const uint32_t k = e->rel_pkt - 1;
+        out->bitstream.buffer += e->rel_off;
+        out->bitstream.used_size = e->geom.host.slice_size[k];
+        e->rel_off += out->bitstream.used_size;
+        last = k + 1 == g.slices;
+    }
+    e->rel_pkt++;
+    const bool failed = e->rel_failed;
+    if (last) {
+        out->bitstream.last_packet_in_frame = 1;
+        out->bitstream.ready_to_release = 1;
+        out->image.ready_to_release = 1;
+        e->rel_slot = -1;
+        release_slot(e, enc_api, slot);
+    }
     return failed ? SvtJxsErrorEncodeFrameError : SvtJxsErrorNone;
 }
 

@@ -97,3 +97,46 @@ def test_gpu_full_size_workloads_equal_reference(name):
     rc, cs = _gpu_encode(_G["full"][name]["params"])
     assert rc == 0
     assert hashlib.sha256(cs).hexdigest() == _G["full"][name]["codestream_sha256"]
+
+
+@pytest.mark.gpu
+def test_slice_packetization_packets():
+    """slice_packetization_mode 1 (FinalStageProcess.c:126-213): get_packet
+    hands out the frame header, then one packet per slice, back to back in
+    the caller's buffer; only the last is marked last / releasable.  The
+    packets put together are the oracle's codestream, every slice packet
+    opens with its SLH (FF20, Lslh = 4, slice index) and the last one ends
+    with EOC (FF11).  Two frames in a row."""
+    W, H, D, F, bpp, V, Hh = 256, 128, 10, 3, 3, 2, 5
+    S, O, P = load_synth(), load_oracle(), load_package()
+    enc = P.Encoder(W, H, D, F, bpp, 1, V, Hh, 0, 16, 1, slice_packetization=1)
+    slices = H // 16
+    for frame in range(2):
+        print("case slice packetization frame", frame, flush=True)
+        planes = S.make_frame(W, H, D, F, frame=frame)
+        e_ref, ref = O.encode(O.make_params(W, H, D, F, bpp, 1, V, Hh, 0, 16, 1), planes)
+        assert e_ref == 0
+        out = np.zeros(enc.bytes_per_frame, dtype=np.uint8)
+        assert enc.send(planes, out) == 0
+        pkts = []
+        for _ in range(slices + 1):
+            rc, f = enc.get()
+            assert rc == 0
+            b = f.bitstream
+            pkts.append((b.buffer - out.ctypes.data, b.used_size, b.last_packet_in_frame, b.ready_to_release,
+                         f.image.ready_to_release))
+            if b.last_packet_in_frame:
+                break
+        assert len(pkts) == slices + 1
+        pos = 0
+        for k, (off, size, last, rdy, img) in enumerate(pkts):
+            assert off == pos
+            pos += size
+            final = k == slices
+            assert (last, rdy, img) == ((1, 1, 1) if final else (0, 0, 0))
+            if k > 0:
+                assert bytes(out[off:off + 6]) == bytes([0xFF, 0x20, 0x00, 0x04, (k - 1) >> 8, (k - 1) & 0xFF])
+        assert pos == len(ref)
+        assert bytes(out[pos - 2:pos]) == b"\xff\x11"
+        assert out[:pos].tobytes() == ref
+    enc.close()
