@@ -90,11 +90,11 @@ struct svt_jpeg_xs_amd_plan {
     JxsFrameDesc* d_frames = nullptr;
     JxsUnit* d_units = nullptr; /* packing units of every frame, frame after frame */
     /* transform job lists per stage (first / second vertical level) and
-     * chunk class (16 / 32 columns per thread), one launch each */
-    JxsDwtJob* d_dwt[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
-    uint32_t* d_dbase[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
-    int ndwt[2][2] = {{0, 0}, {0, 0}};
-    uint32_t dblocks[2][2] = {{0, 0}, {0, 0}};
+     * chunk class (8 / 16 / 32 columns per thread), one launch each */
+    JxsDwtJob* d_dwt[2][3] = {};
+    uint32_t* d_dbase[2][3] = {};
+    int ndwt[2][3] = {};
+    uint32_t dblocks[2][3] = {};
     uint32_t total_prec = 0, total_slices = 0, total_units = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
@@ -148,7 +148,7 @@ static void plan_free(svt_jpeg_xs_amd_plan_t* p) {
     (void)hipFree(p->d_frames);
     (void)hipFree(p->d_units);
     for (int s = 0; s < 2; s++) {
-        for (int c = 0; c < 2; c++) {
+        for (int c = 0; c < 3; c++) {
             (void)hipFree(p->d_dwt[s][c]);
             (void)hipFree(p->d_dbase[s][c]);
         }
@@ -245,15 +245,15 @@ static svt_jpeg_xs_amd_plan_t* plan_build(const svt_jpeg_xs_amd_frame_spec_t* sp
         frames[i].units = p->d_units + frames[i].unit_base;
     bool ok = upload_vec(frames, &p->d_frames) == hipSuccess;
     for (int s = 0; s < 2 && ok; s++)
-        for (int c = 0; c < 2 && ok; c++) {
-            const uint32_t chunk = c == 0 ? 16u : 32u;
+        for (int c = 0; c < 3 && ok; c++) {
+            const uint32_t chunk = 8u << c;
             std::vector<JxsDwtJob> list;
             std::vector<uint32_t> bases;
             uint32_t blk = 0;
             for (auto j : dj[s]) {
                 if (j.chunk != chunk)
                     continue;
-                if (j.lev_l < 1 || (1u << j.lev_l) > j.chunk || j.tpr > 256 || j.rpb == 0)
+                if (j.lev_l < 1 || (1u << j.lev_l) > j.chunk || j.tpr > (j.chunk == 8 ? 512u : 256u) || j.rpb == 0)
                     ok = false; /* outside what the transform kernel's layout covers */
                 j.base = blk;
                 bases.push_back(blk);
@@ -285,17 +285,22 @@ static hipError_t plan_launch(svt_jpeg_xs_amd_plan_t* p, hipStream_t st, hipEven
         if (e != hipSuccess)                \
             return e;                       \
     } while (0)
-    /* the first transform launch also clears the frames' error flags */
-    const int c0 = p->ndwt[0][0] ? 0 : 1;
-    STEP(0, jxs_launch_dwt(p->d_dwt[0][c0], p->d_dbase[0][c0], p->ndwt[0][c0], p->dblocks[0][c0], 16u << c0,
-                           p->d_status, p->nfr, st));
-    if ((e = jxs_launch_dwt(p->d_dwt[0][1 - c0], p->d_dbase[0][1 - c0], p->ndwt[0][1 - c0], p->dblocks[0][1 - c0],
-                            16u << (1 - c0), nullptr, 0, st)) != hipSuccess)
-        return e;
-    STEP(1, jxs_launch_dwt(p->d_dwt[1][0], p->d_dbase[1][0], p->ndwt[1][0], p->dblocks[1][0], 16u, nullptr, 0, st));
-    if ((e = jxs_launch_dwt(p->d_dwt[1][1], p->d_dbase[1][1], p->ndwt[1][1], p->dblocks[1][1], 32u, nullptr, 0, st)) !=
-        hipSuccess)
-        return e;
+    /* every non-empty chunk class of a stage is one launch; the first launch
+     * of the plan also clears the frames' error flags */
+    bool cleared = false;
+    for (int s = 0; s < 2; s++) {
+        if (ev)
+            (void)hipEventRecord(ev[s], st);
+        for (int c = 0; c < 3; c++) {
+            if (!p->ndwt[s][c])
+                continue;
+            e = jxs_launch_dwt(p->d_dwt[s][c], p->d_dbase[s][c], p->ndwt[s][c], p->dblocks[s][c], 8u << c,
+                               cleared ? nullptr : p->d_status, cleared ? 0 : p->nfr, st);
+            if (e != hipSuccess)
+                return e;
+            cleared = true;
+        }
+    }
     STEP(2, jxs_launch_rate(p->d_frames, nf, p->total_slices, st));
     STEP(3, jxs_launch_pack(p->d_units, p->total_units, st));
     if (ev)

@@ -748,9 +748,10 @@ static JxsBandOut band_out(const JxsGeom& g, uint32_t c, uint32_t b, uint16_t* c
     return o;
 }
 
-/* Launch shape of a transform job: columns per thread (16, or 32 for five
- * horizontal levels or rows wider than 256 x 16), threads per row (whole
- * waves), rows per 256-thread block; rows are read 8 samples per load when
+/* Launch shape of a transform job: columns per thread (8 for one horizontal
+ * level, 16, or 32 for five horizontal levels or rows wider than 256 x 16),
+ * threads per row (whole waves), rows per block (512 threads for 8 columns
+ * per thread, else 256); rows are read 8 samples per load when
  * they are contiguous, every row starts 16-byte aligned and the width is a
  * multiple of 8. */
 static JxsDwtJob finish_job(JxsDwtJob j) {
@@ -758,9 +759,15 @@ static JxsDwtJob finish_job(JxsDwtJob j) {
     j.vec = (j.step == 1 && (j.stride * elem) % 16 == 0 && ((uintptr_t)j.src & 15) == 0 && j.w % 8 == 0 && j.w >= 8)
                 ? 1u
                 : 0u;
-    j.chunk = (j.lev_l >= 5 || (j.w + 15) / 16 > 256) ? 32u : 16u;
+    /* one horizontal level: 8 columns per thread in 512-thread blocks (half
+     * the registers, twice the waves in flight) */
+    if (j.lev_l == 1 && (j.w + 7) / 8 <= 512)
+        j.chunk = 8;
+    else
+        j.chunk = (j.lev_l >= 5 || (j.w + 15) / 16 > 256) ? 32u : 16u;
+    const uint32_t block = j.chunk == 8 ? 512u : 256u;
     j.tpr = ((j.w + j.chunk - 1) / j.chunk + 63) / 64 * 64;
-    j.rpb = 256 / j.tpr;
+    j.rpb = block / j.tpr;
     j.nblk = (j.rows + j.rpb - 1) / j.rpb;
     return j;
 }

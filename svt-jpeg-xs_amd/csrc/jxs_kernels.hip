@@ -190,8 +190,8 @@ __device__ __forceinline__ uint32_t xcd_remap(uint32_t b, uint32_t n) {
  * of its row(s) in registers, through the vertical step, every horizontal
  * level and the band output.  Neighbouring threads exchange one sample per
  * lifting step through LDS. */
-#define DW_THREADS 256
-#define DW_SLOTS 4  /* row slots per block (at most one per wave) */
+#define DW_MAXT 512              /* threads of the largest transform block */
+#define DW_SLOTS (DW_MAXT / WAVE) /* row slots per block (at most one per wave) */
 #define DW_NBAND 8  /* band rows of a slot: horizontal levels 0..4, low-pass, LH, HH */
 #define DW_LF 5
 #define DW_LH 6
@@ -219,43 +219,43 @@ __device__ __forceinline__ int32_t ld1(const void* row, uint32_t x, uint32_t ste
     return gld((const int32_t*)row + (size_t)x * step);
 }
 
-/* Samples of a row for the transform, 16 columns [x0, x0 + 16) per call.
+/* Samples of a row for the transform, NC columns [x0, x0 + NC) per call.
  * With `vec` (contiguous rows, 16-byte aligned, width a multiple of 8 and at
  * least 8: checked by the host) the row is read 8 samples per load, every
  * load issued before any is used; a load past the row's end reads the row's
  * last 8 samples instead, whose values then stand in for columns that no
  * output uses.  Otherwise one load per sample, the column clamped into the
  * row. */
-template <int KIND>
+template <int KIND, int NC>
 struct RowRaw {
-    /* 16 samples: two 8-sample loads (four for int32 rows) */
-    u32x4 q[KIND == JXS_SRC_I32 ? 4 : 2];
+    /* NC (8 or 16) samples: 8-sample loads (4-sample loads for int32 rows) */
+    u32x4 q[KIND == JXS_SRC_I32 ? NC / 4 : NC / 8];
 };
-template <int KIND>
-__device__ __forceinline__ RowRaw<KIND> load16_vec(const void* row, uint32_t x0, uint32_t w) {
-    RowRaw<KIND> r;
+template <int KIND, int NC>
+__device__ __forceinline__ RowRaw<KIND, NC> load_vec(const void* row, uint32_t x0, uint32_t w) {
+    RowRaw<KIND, NC> r;
     if constexpr (KIND == JXS_SRC_I32) {
 #pragma unroll
-        for (int k = 0; k < 4; k++)
+        for (int k = 0; k < NC / 4; k++)
             r.q[k] = gld((const u32x4*)((const int32_t*)row + min(x0 + 4 * k, w - 4)));
     } else if constexpr (KIND == JXS_SRC_U16) {
 #pragma unroll
-        for (int k = 0; k < 2; k++)
+        for (int k = 0; k < NC / 8; k++)
             r.q[k] = gld((const u32x4*)((const uint16_t*)row + min(x0 + 8 * k, w - 8)));
     } else {
 #pragma unroll
-        for (int k = 0; k < 2; k++) {
+        for (int k = 0; k < NC / 8; k++) {
             const u32x2 v = gld((const u32x2*)((const uint8_t*)row + min(x0 + 8 * k, w - 8)));
             r.q[k] = {v.x, v.y, 0u, 0u};
         }
     }
     return r;
 }
-template <int KIND>
-__device__ __forceinline__ void unpack16(const RowRaw<KIND>& r, const Scale& sc, int32_t (&out)[16]) {
+template <int KIND, int NC>
+__device__ __forceinline__ void unpack(const RowRaw<KIND, NC>& r, const Scale& sc, int32_t (&out)[NC]) {
     if constexpr (KIND == JXS_SRC_I32) {
 #pragma unroll
-        for (int k = 0; k < 4; k++) {
+        for (int k = 0; k < NC / 4; k++) {
             out[4 * k] = (int32_t)r.q[k].x;
             out[4 * k + 1] = (int32_t)r.q[k].y;
             out[4 * k + 2] = (int32_t)r.q[k].z;
@@ -263,7 +263,7 @@ __device__ __forceinline__ void unpack16(const RowRaw<KIND>& r, const Scale& sc,
         }
     } else if constexpr (KIND == JXS_SRC_U16) {
 #pragma unroll
-        for (int k = 0; k < 2; k++) {
+        for (int k = 0; k < NC / 8; k++) {
             const uint32_t wd[4] = {r.q[k].x, r.q[k].y, r.q[k].z, r.q[k].w};
 #pragma unroll
             for (int j = 0; j < 4; j++) {
@@ -273,17 +273,17 @@ __device__ __forceinline__ void unpack16(const RowRaw<KIND>& r, const Scale& sc,
         }
     } else {
 #pragma unroll
-        for (int k = 0; k < 2; k++)
+        for (int k = 0; k < NC / 8; k++)
 #pragma unroll
             for (int j = 0; j < 8; j++)
                 out[8 * k + j] = sc(((j < 4 ? r.q[k].x : r.q[k].y) >> (8 * (j & 3))) & 0xFFu);
     }
 }
-template <int KIND>
-__device__ __forceinline__ void load16_scalar(const void* row, uint32_t x0, uint32_t w, uint32_t step, const Scale& sc,
-                                              int32_t (&out)[16]) {
+template <int KIND, int NC>
+__device__ __forceinline__ void load_scalar(const void* row, uint32_t x0, uint32_t w, uint32_t step, const Scale& sc,
+                                            int32_t (&out)[NC]) {
 #pragma unroll
-    for (int k = 0; k < 16; k++)
+    for (int k = 0; k < NC; k++)
         out[k] = ld1<KIND>(row, min(x0 + k, w - 1), step, sc);
 }
 
@@ -362,11 +362,6 @@ __device__ __forceinline__ JxsBandOut band_out(const CAS JxsBandOut& b) {
     return o;
 }
 
-/* Lane masks of the GCLI counting: ~(ballot of GCLI bit k XOR mask k) has a
- * one for every lane whose GCLI bit k equals bit k of this lane's index. */
-struct LaneMasks {
-    uint64_t m0, m1, m2, m3;
-};
 
 /* Band row output of G groups of 4 coefficients per thread (groups g0 ..
  * g0 + G - 1 of band row y, count coefficients in the row): 16-bit
@@ -379,8 +374,7 @@ struct LaneMasks {
  * + 1 at HREC_TAIL.  Called by whole waves (act: this lane's row exists). */
 template <int G>
 __device__ __forceinline__ void emit_groups(const JxsBandOut& o, uint32_t y, const int32_t (&val)[4 * G], uint32_t g0,
-                                            uint32_t count, uint32_t* hc, bool act, uint32_t lane,
-                                            const LaneMasks& lm) {
+                                            uint32_t count, uint32_t* hc, bool act, uint32_t lane) {
     uint16_t* crow = o.coeff + (size_t)y * o.cs;
     uint8_t* grow = o.gcli + (size_t)y * o.gw;
     uint8_t* srow = o.smax + (size_t)y * o.sw;
@@ -407,7 +401,11 @@ __device__ __forceinline__ void emit_groups(const JxsBandOut& o, uint32_t y, con
         if (hc) {
             const uint64_t vg = __ballot(gv);
             const uint64_t b0 = __ballot(gc & 1u), b1 = __ballot(gc & 2u), b2 = __ballot(gc & 4u), b3 = __ballot(gc & 8u);
-            mycnt += (uint32_t)__popcll(vg & ~(b0 ^ lm.m0) & ~(b1 ^ lm.m1) & ~(b2 ^ lm.m2) & ~(b3 ^ lm.m3));
+            /* lanes whose GCLI equals this lane's index: bit k of the GCLI
+             * matches bit k of the lane */
+            const uint64_t e0 = (lane & 1u) ? b0 : ~b0, e1 = (lane & 2u) ? b1 : ~b1;
+            const uint64_t e2 = (lane & 4u) ? b2 : ~b2, e3 = (lane & 8u) ? b3 : ~b3;
+            mycnt += (uint32_t)__popcll(vg & e0 & e1 & e2 & e3);
         }
     }
     /* maximum over the 8 groups of a significance group: 8 / G lanes */
@@ -435,21 +433,20 @@ __device__ __forceinline__ void emit_groups(const JxsBandOut& o, uint32_t y, con
 /* A band row staged in LDS (values st[0 .. count)), one group per thread of
  * the slot. */
 __device__ __forceinline__ void emit_staged(const JxsBandOut& o, uint32_t y, const int32_t* st, uint32_t t,
-                                            uint32_t count, uint32_t* hc, bool act, uint32_t lane,
-                                            const LaneMasks& lm) {
+                                            uint32_t count, uint32_t* hc, bool act, uint32_t lane) {
     int32_t val[4];
     const uint32_t last = count > 0 ? count - 1 : 0;
 #pragma unroll
     for (int k = 0; k < 4; k++)
         val[k] = st[min(4 * t + k, last)];
-    emit_groups<1>(o, y, val, t, count, hc, act, lane, lm);
+    emit_groups<1>(o, y, val, t, count, hc, act, lane);
 }
 
 /* Per-block state of the transform (LDS). */
 struct DwLds {
-    int32_t exr[2][DW_THREADS]; /* first sample of every thread (low row, high row) */
-    int32_t exl[2][DW_THREADS]; /* last sample of every thread after the predict */
-    int32_t stage[DW_STAGED][2 * DW_THREADS]; /* short band rows: at most 2 samples per thread */
+    int32_t exr[2][DW_MAXT]; /* first sample of every thread (low row, high row) */
+    int32_t exl[2][DW_MAXT]; /* last sample of every thread after the predict */
+    int32_t stage[DW_STAGED][2 * DW_MAXT]; /* short band rows: at most 2 samples per thread */
     uint32_t hc[DW_SLOTS][DW_NBAND][JXS_HREC];
 };
 
@@ -457,7 +454,6 @@ struct DwLds {
 struct DwRow {
     uint32_t tid, lane, slot, t, tpr, ir;
     bool act, right, left;
-    LaneMasks lm;
     uint32_t st_count[DW_STAGED]; /* staged band rows: coefficient count (0: none) */
     uint32_t st_band[DW_STAGED];
 };
@@ -477,7 +473,7 @@ __device__ __forceinline__ void dw_lowpass(const CAS JxsDwtJob& J, DwLds& L, DwR
             for (int k = 0; k < M; k++)
                 v[k] = lo[k * S];
             emit_groups<M / 4>(ob, R.ir, v, R.t * (M / 4), nlev, ob.hist ? L.hc[R.slot][DW_LF] : nullptr, R.act,
-                               R.lane, R.lm);
+                               R.lane);
         } else {
 #pragma unroll
             for (int k = 0; k < M; k++)
@@ -540,8 +536,7 @@ __device__ __forceinline__ void dw_level(const CAS JxsDwtJob& J, DwLds& L, DwRow
 #pragma unroll
         for (int k = 0; k < M / 2; k++)
             v[k] = lo[(2 * k + 1) * S];
-        emit_groups<M / 8>(ob, R.ir, v, R.t * (M / 8), nh, ob.hist ? L.hc[R.slot][LVL] : nullptr, R.act, R.lane,
-                           R.lm);
+        emit_groups<M / 8>(ob, R.ir, v, R.t * (M / 8), nh, ob.hist ? L.hc[R.slot][LVL] : nullptr, R.act, R.lane);
     } else {
         /* M = 4 or 2: the first or second short level */
         constexpr int SI = M == 4 ? 0 : 1;
@@ -562,9 +557,9 @@ __device__ __forceinline__ void dw_level(const CAS JxsDwtJob& J, DwLds& L, DwRow
             }
             const JxsBandOut olh = band_out(J.lh), ohh = band_out(J.hh);
             emit_groups<CH / 8>(olh, R.ir, vl, R.t * (CH / 8), w - w / 2, olh.hist ? L.hc[R.slot][DW_LH] : nullptr,
-                                R.act, R.lane, R.lm);
+                                R.act, R.lane);
             emit_groups<CH / 8>(ohh, R.ir, vh, R.t * (CH / 8), w / 2, ohh.hist ? L.hc[R.slot][DW_HH] : nullptr,
-                                R.act, R.lane, R.lm);
+                                R.act, R.lane);
         }
     }
     const uint32_t nl = nlev - nh;
@@ -593,10 +588,6 @@ __device__ __forceinline__ void dwt_slots(const CAS JxsDwtJob& J, uint32_t bi, D
     R.ir = wave_uniform(R.act ? i : 0); /* inactive slots work on row 0 and write nothing */
     R.right = R.t + 1 < R.tpr;
     R.left = R.t > 0;
-    R.lm.m0 = (R.lane & 1u) ? ~0ull : 0ull;
-    R.lm.m1 = (R.lane & 2u) ? ~0ull : 0ull;
-    R.lm.m2 = (R.lane & 4u) ? ~0ull : 0ull;
-    R.lm.m3 = (R.lane & 8u) ? ~0ull : 0ull;
 #pragma unroll
     for (int k = 0; k < DW_STAGED; k++)
         R.st_count[k] = R.st_band[k] = 0;
@@ -616,36 +607,38 @@ __device__ __forceinline__ void dwt_slots(const CAS JxsDwtJob& J, uint32_t bi, D
 #pragma unroll
         for (int j = 0; j < 5; j++)
             r[j] = base + (size_t)min(max((int)(2 * ir) + j - 2, 0), (int)n - 1) * pitch;
+        constexpr int NC = CH < 16 ? CH : 16;
 #pragma unroll
-        for (int h0 = 0; h0 < CH; h0 += 16) {
-            int32_t x[5][16];
+        for (int h0 = 0; h0 < CH; h0 += NC) {
+            int32_t x[5][NC];
             if (vec) {
-                RowRaw<KIND> raw[5];
+                RowRaw<KIND, NC> raw[5];
 #pragma unroll
                 for (int j = 0; j < 5; j++)
-                    raw[j] = load16_vec<KIND>(r[j], x0 + h0, w);
+                    raw[j] = load_vec<KIND, NC>(r[j], x0 + h0, w);
 #pragma unroll
                 for (int j = 0; j < 5; j++)
-                    unpack16<KIND>(raw[j], sc, x[j]);
+                    unpack<KIND, NC>(raw[j], sc, x[j]);
             } else {
 #pragma unroll
                 for (int j = 0; j < 5; j++)
-                    load16_scalar<KIND>(r[j], x0 + h0, w, step, sc, x[j]);
+                    load_scalar<KIND, NC>(r[j], x0 + h0, w, step, sc, x[j]);
             }
 #pragma unroll
-            for (int k = 0; k < 16; k++)
+            for (int k = 0; k < NC; k++)
                 vert53(f, x[0][k], x[1][k], x[2][k], x[3][k], x[4][k], lo[h0 + k], hi[h0 + k]);
         }
     } else {
+        constexpr int NC = CH < 16 ? CH : 16;
 #pragma unroll
-        for (int h0 = 0; h0 < CH; h0 += 16) {
-            int32_t x[16];
+        for (int h0 = 0; h0 < CH; h0 += NC) {
+            int32_t x[NC];
             if (vec)
-                unpack16<KIND>(load16_vec<KIND>(base + (size_t)ir * pitch, x0 + h0, w), sc, x);
+                unpack<KIND, NC>(load_vec<KIND, NC>(base + (size_t)ir * pitch, x0 + h0, w), sc, x);
             else
-                load16_scalar<KIND>(base + (size_t)ir * pitch, x0 + h0, w, step, sc, x);
+                load_scalar<KIND, NC>(base + (size_t)ir * pitch, x0 + h0, w, step, sc, x);
 #pragma unroll
-            for (int k = 0; k < 16; k++) {
+            for (int k = 0; k < NC; k++) {
                 lo[h0 + k] = x[k];
                 hi[h0 + k] = 0;
             }
@@ -661,7 +654,7 @@ __device__ __forceinline__ void dwt_slots(const CAS JxsDwtJob& J, uint32_t bi, D
         const uint32_t b = R.st_band[k];
         const JxsBandOut ob = band_out(b == DW_LF ? J.lf : J.hf[b]);
         emit_staged(ob, ir, &L.stage[k][R.slot * R.tpr * 2], R.t, R.st_count[k], ob.hist ? L.hc[R.slot][b] : nullptr,
-                    R.act && R.t < ob.gw, R.lane, R.lm);
+                    R.act && R.t < ob.gw, R.lane);
     }
     /* histogram records of the band rows written, by the first wave of
      * every slot, band after band (uniform descriptors) */
@@ -700,8 +693,8 @@ __device__ __forceinline__ void dwt_slots(const CAS JxsDwtJob& J, uint32_t bi, D
  * and HH bands.  Without a vertical level, input row i through the
  * horizontal levels.  The first launch of a plan also clears the frames'
  * error flags (status, nstatus words). */
-template <int CH>
-__global__ void __launch_bounds__(DW_THREADS) k_dwt(const JxsDwtJob* __restrict__ jobs,
+template <int CH, int NT>
+__global__ void __launch_bounds__(NT) k_dwt(const JxsDwtJob* __restrict__ jobs,
                                                     const uint32_t* __restrict__ bases, int njobs, uint32_t nblocks,
                                                     uint32_t* status, uint32_t nstatus) {
     __shared__ DwLds L;
@@ -710,9 +703,9 @@ __global__ void __launch_bounds__(DW_THREADS) k_dwt(const JxsDwtJob* __restrict_
     const CAS JxsDwtJob& J = ((const CAS JxsDwtJob*)jobs)[j];
     const uint32_t bi = blk - J.base;
     if (status && blockIdx.x == 0)
-        for (uint32_t k = threadIdx.x; k < nstatus; k += DW_THREADS)
+        for (uint32_t k = threadIdx.x; k < nstatus; k += NT)
             gst(status + k, 0u);
-    for (uint32_t e = threadIdx.x; e < DW_SLOTS * DW_NBAND * JXS_HREC; e += DW_THREADS)
+    for (uint32_t e = threadIdx.x; e < DW_SLOTS * DW_NBAND * JXS_HREC; e += NT)
         (&L.hc[0][0][0])[e] = 0;
     lds_barrier();
     const uint32_t kind = J.kind;
@@ -1553,30 +1546,32 @@ __device__ __forceinline__ void gd_step(WaveOut& B, WaveOut& C, const GdIn<N>& i
     C.pos += wave_last(dincl);
 }
 
-/* One packing unit (one band line of a precinct, PackPrecinct.c:235-572)
- * by one wave: its significance flags, bit-plane counts (unary or raw) and
- * data bit-planes with the quantisation (Quant.c:9-45) fused, each segment
- * into its own sub-packet; the first band line of a packet also writes the
- * packet header (PackHeaders.c:98-127, sign size 0).  The significance
- * maxima and the first step's inputs are loaded together as soon as the
- * unit record is in, and every step's successor is loaded before the step
- * is coded.  Returns whether a segment's length differs from the plan's. */
-template <uint32_t N, bool UNIFORM>
-__device__ __forceinline__ bool pack_unit(const CAS JxsUnit& U, uint32_t lane, uint32_t* ringb, uint32_t* ringc) {
-    const uint8_t* gl = U.gl;
-    const uint8_t* sl = U.sl;
-    const uint16_t* cl = U.cl;
-    const uint32_t w = U.w, gw = U.gw, sw = U.sw, gt = U.gt, ms = U.ms, raw = U.raw;
-    const bool has_sig = U.len[0] != 0, has_gd = (U.len[1] | U.len[2]) != 0;
-    uint32_t sf[IPL] = {0, 0, 0, 0};
-    if (has_sig) {
+/* What a packing wave loads first for a unit: the significance maxima it
+ * codes (IPL per lane) and the first step's GCLIs, maxima and coefficients
+ * (absent segments load nothing). */
+struct PackIn {
+    uint32_t sf[IPL];
+    GdIn<IPL> gd;
+};
+__device__ __forceinline__ PackIn pack_load(const JxsUnit& U, uint32_t lane) {
+    PackIn in;
+#pragma unroll
+    for (uint32_t k = 0; k < IPL; k++)
+        in.sf[k] = 0;
+    if (U.len[0]) {
 #pragma unroll
         for (uint32_t k = 0; k < IPL; k++)
-            sf[k] = gld(sl + min(IPL * lane + k, sw - 1));
+            in.sf[k] = gld(U.sl + min(IPL * lane + k, U.sw - 1));
     }
-    GdIn<N> cur;
-    if (has_gd)
-        cur = gd_load<N>(gl, sl, cl, gw, ms, N * lane);
+    if (U.len[1] | U.len[2])
+        in.gd = gd_load<IPL>(U.gl, U.sl, U.cl, U.gw, U.ms, IPL * lane);
+    return in;
+}
+
+/* The packet header (PackHeaders.c:98-127, sign size 0) of the first band
+ * line of a packet and the line's significance segment.  Returns whether
+ * the segment's length differs from the plan's. */
+__device__ __forceinline__ bool pack_sig(const JxsUnit& U, const PackIn& in, uint32_t lane, uint32_t* ringb) {
     if (U.hlen) {
         uint32_t w0, w1, w2;
         split_bits(U.hdr, U.hlen, U.hlo, w0, w1, w2);
@@ -1584,61 +1579,91 @@ __device__ __forceinline__ bool pack_unit(const CAS JxsUnit& U, uint32_t lane, u
         if (lane < 3 && piece)
             gor(U.hob + lane, __builtin_bswap32(piece));
     }
-    bool bad = false;
-    WaveOut B, C;
-    if (has_sig) {
-        B.start(ringb, RINGB - 1, U.ob[0], U.lo[0], U.len[0], lane);
-        wave_sync();
-        sig_step(B, sf, sw, gt);
-        B.finish();
-        bad |= B.pos != B.hi;
-    }
-    if (has_gd) {
-        wave_sync();
-        B.start(ringb, RINGB - 1, U.ob[1], U.lo[1], U.len[1], lane);
-        C.start(ringc, RINGC - 1, U.ob[2], U.lo[2], U.len[2], lane);
-        wave_sync();
-        for (uint32_t base = 0; base < gw; base += N * WAVE) {
-            const bool more = base + N * WAVE < gw;
-            GdIn<N> nxt;
-            if (more)
-                nxt = gd_load<N>(gl, sl, cl, gw, ms, base + N * WAVE + N * lane);
-            gd_step<N, UNIFORM>(B, C, cur, base + N * lane, w, gw, gt, ms, raw);
-            B.maybe_flush();
-            C.maybe_flush();
-            if (more)
-                cur = nxt;
-        }
-        B.finish();
-        C.finish();
-        bad |= B.pos != B.hi || C.pos != C.hi;
-    }
-    return bad;
+    if (!U.len[0])
+        return false;
+    WaveOut B;
+    B.start(ringb, RINGB - 1, U.ob[0], U.lo[0], U.len[0], lane);
+    wave_sync();
+    sig_step(B, in.sf, U.sw, U.gt);
+    B.finish();
+    wave_sync();
+    return B.pos != B.hi;
 }
 
-/* One wave per packing unit (band line of a precinct).  Every unit is
- * described by one self-contained record of the packet plan, so a wave
- * makes one descriptor load; the GCLIs and coefficients of the line are
- * read once.  Waves never wait on each other. */
+/* The bit-plane-count and data segments of a band line, IPL groups per lane
+ * and step, each step's successor loaded before the step is coded.
+ * Returns whether a segment's length differs from the plan's. */
+template <bool UNIFORM>
+__device__ __forceinline__ bool pack_gd(const JxsUnit& U, const GdIn<IPL>& first, uint32_t lane, uint32_t* ringb,
+                                        uint32_t* ringc) {
+    if (!(U.len[1] | U.len[2]))
+        return false;
+    const uint8_t* gl = U.gl;
+    const uint8_t* sl = U.sl;
+    const uint16_t* cl = U.cl;
+    const uint32_t w = U.w, gw = U.gw, gt = U.gt, ms = U.ms, raw = U.raw;
+    WaveOut B, C;
+    B.start(ringb, RINGB - 1, U.ob[1], U.lo[1], U.len[1], lane);
+    C.start(ringc, RINGC - 1, U.ob[2], U.lo[2], U.len[2], lane);
+    wave_sync();
+    GdIn<IPL> cur = first;
+    for (uint32_t base = 0; base < gw; base += IPL * WAVE) {
+        const bool more = base + IPL * WAVE < gw;
+        GdIn<IPL> nxt;
+        if (more)
+            nxt = gd_load<IPL>(gl, sl, cl, gw, ms, base + IPL * WAVE + IPL * lane);
+        gd_step<IPL, UNIFORM>(B, C, cur, base + IPL * lane, w, gw, gt, ms, raw);
+        B.maybe_flush();
+        C.maybe_flush();
+        if (more)
+            cur = nxt;
+    }
+    B.finish();
+    C.finish();
+    wave_sync();
+    return B.pos != B.hi || C.pos != C.hi;
+}
+
+/* Persistent packing waves (PackPrecinct.c:235-572): each wave codes the
+ * packing units (band lines of precincts) gu, gu + W, gu + 2W, ... (W =
+ * waves of the launch; units ordered widest first), every one's
+ * significance flags, bit-plane counts (unary or raw) and data bit-planes
+ * with the quantisation (Quant.c:9-45) fused, each segment into its own
+ * sub-packet, plus the packet header of a packet's first band line.  While
+ * a unit is coded, the next unit's record and first inputs are already on
+ * their way.  Waves never wait on each other. */
+#define PK_BLOCKS 1024 /* resident blocks: 256 CUs x 4 (the kernel's occupancy) */
 __global__ void __launch_bounds__(64 * PK_WAVES) k_pack(const JxsUnit* __restrict__ units, uint32_t total_units) {
     __shared__ uint32_t ringb[PK_WAVES][RINGB];
     __shared__ uint32_t ringc[PK_WAVES][RINGC];
-    const uint32_t wid = threadIdx.x / WAVE, lane = threadIdx.x & (WAVE - 1);
-    const uint32_t gu = blockIdx.x * PK_WAVES + wid;
+    const uint32_t wid = wave_uniform(threadIdx.x / WAVE), lane = threadIdx.x & (WAVE - 1);
+    const uint32_t stride = gridDim.x * PK_WAVES;
+    uint32_t gu = blockIdx.x * PK_WAVES + wid;
     if (gu >= total_units)
         return;
-    const CAS JxsUnit& U = ((const CAS JxsUnit*)units)[gu];
-    const bool narrow = U.gw <= WAVE;
-    bool bad;
-    if (U.qpih)
-        bad = narrow ? pack_unit<1, true>(U, lane, ringb[wid], ringc[wid])
-                     : pack_unit<IPL, true>(U, lane, ringb[wid], ringc[wid]);
-    else
-        bad = narrow ? pack_unit<1, false>(U, lane, ringb[wid], ringc[wid])
-                     : pack_unit<IPL, false>(U, lane, ringb[wid], ringc[wid]);
-    /* the plan's lengths are the segments' exact bit counts */
-    if (bad && lane == 0)
-        gor(U.status, 2u);
+    JxsUnit U = cld_words(units + gu);
+    PackIn in = pack_load(U, lane);
+    for (;;) {
+        const uint32_t gn = gu + stride;
+        const bool more = gn < total_units;
+        JxsUnit Un;
+        if (more)
+            Un = cld_words(units + gn);
+        bool bad = pack_sig(U, in, lane, ringb[wid]);
+        PackIn inn;
+        if (more)
+            inn = pack_load(Un, lane);
+        bad |= U.qpih ? pack_gd<true>(U, in.gd, lane, ringb[wid], ringc[wid])
+                      : pack_gd<false>(U, in.gd, lane, ringb[wid], ringc[wid]);
+        /* the plan's lengths are the segments' exact bit counts */
+        if (bad && lane == 0)
+            gor(U.status, 2u);
+        if (!more)
+            break;
+        gu = gn;
+        U = Un;
+        in = inn;
+    }
 }
 
 /* ------------------------------------------------------------------ */
@@ -1646,15 +1671,18 @@ __global__ void __launch_bounds__(64 * PK_WAVES) k_pack(const JxsUnit* __restric
 /* ------------------------------------------------------------------ */
 extern "C" {
 
-/* jobs: all of one chunk width (16 or 32 columns per thread) */
+/* jobs: all of one chunk width: 8 columns per thread in 512-thread blocks,
+ * 16 or 32 in 256-thread blocks */
 hipError_t jxs_launch_dwt(const JxsDwtJob* jobs, const uint32_t* bases, int njobs, uint32_t blocks, uint32_t chunk,
                           uint32_t* status, uint32_t nstatus, hipStream_t st) {
     if (!njobs || !blocks)
         return hipSuccess;
-    if (chunk == 16)
-        hipLaunchKernelGGL(k_dwt<16>, dim3(blocks), dim3(DW_THREADS), 0, st, jobs, bases, njobs, blocks, status, nstatus);
+    if (chunk == 8)
+        hipLaunchKernelGGL((k_dwt<8, 512>), dim3(blocks), dim3(512), 0, st, jobs, bases, njobs, blocks, status, nstatus);
+    else if (chunk == 16)
+        hipLaunchKernelGGL((k_dwt<16, 256>), dim3(blocks), dim3(256), 0, st, jobs, bases, njobs, blocks, status, nstatus);
     else if (chunk == 32)
-        hipLaunchKernelGGL(k_dwt<32>, dim3(blocks), dim3(DW_THREADS), 0, st, jobs, bases, njobs, blocks, status, nstatus);
+        hipLaunchKernelGGL((k_dwt<32, 256>), dim3(blocks), dim3(256), 0, st, jobs, bases, njobs, blocks, status, nstatus);
     else
         return hipErrorInvalidValue;
     return hipGetLastError();
@@ -1670,7 +1698,8 @@ hipError_t jxs_launch_rate(const JxsFrameDesc* frames, int nfr, uint32_t slices,
 hipError_t jxs_launch_pack(const JxsUnit* units, uint32_t n, hipStream_t st) {
     if (!n)
         return hipSuccess;
-    hipLaunchKernelGGL(k_pack, dim3((n + PK_WAVES - 1) / PK_WAVES), dim3(64 * PK_WAVES), 0, st, units, n);
+    const uint32_t blocks = min((n + PK_WAVES - 1) / PK_WAVES, (uint32_t)PK_BLOCKS);
+    hipLaunchKernelGGL(k_pack, dim3(blocks), dim3(64 * PK_WAVES), 0, st, units, n);
     return hipGetLastError();
 }
 
