@@ -139,6 +139,265 @@ def api_host_path(P, torch, frames, n):
                     "PCIe copies included; up to 3 frames in flight"}
 
 
+def _dev_planes(torch, dev, planes):
+    import numpy as np
+    return [torch.from_numpy(p.view(np.int16) if p.dtype == np.uint16 else p).to(dev) for p in planes]
+
+
+def _spec(enc, d_in, out_ptr):
+    return (enc, [t.data_ptr() for t in d_in], [t.shape[1] for t in d_in], out_ptr)
+
+
+def _time_plans(torch, stream, plans, batch, steps):
+    """Milliseconds of `steps` passes of the one-frame plans and of the
+    batch plan (graph replay), HIP events on the launch stream."""
+    sh = stream.cuda_stream
+    for p in plans:
+        p.run(sh)
+    batch.run(sh, use_graph=True)
+    torch.cuda.synchronize()
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    ev0.record(stream)
+    for _ in range(steps):
+        for p in plans:
+            p.run(sh)
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    ms_plain = ev0.elapsed_time(ev1)
+    ev0.record(stream)
+    for _ in range(steps):
+        batch.run(sh, use_graph=True)
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    return ms_plain, ev0.elapsed_time(ev1)
+
+
+def plain_vs_batched_1080p(P, S, torch, dev, steps, check, nframes=64):
+    """1920x1080 4:2:2 8-bit, bpp 5, decomp_v 2, decomp_h 5: one frame per
+    launch sequence against all frames in one launch sequence."""
+    import numpy as np
+    W1, H1, D1, F1, B1 = 1920, 1080, 8, 3, 5
+    frames = S.make_frames(W1, H1, D1, F1, nframes)
+    enc = P.Encoder(W1, H1, D1, F1, B1, 1, 2, 5, verbose=0)
+    ostride = (enc.bytes_per_frame + 3) & ~3
+    d_in = [_dev_planes(torch, dev, fr) for fr in frames]
+    out_p = torch.zeros((nframes, ostride), dtype=torch.uint8, device=dev)
+    out_b = torch.zeros((nframes, ostride), dtype=torch.uint8, device=dev)
+    plans = [P.Plan([_spec(enc, d_in[i], out_p[i].data_ptr())]) for i in range(nframes)]
+    batch = P.Plan([_spec(enc, d_in[i], out_b[i].data_ptr()) for i in range(nframes)])
+    stream = torch.cuda.current_stream(dev)
+    ms_plain, ms_batch = _time_plans(torch, stream, plans, batch, steps)
+    hp = out_p[:, :enc.bytes_per_frame].cpu().numpy()
+    hb = out_b[:, :enc.bytes_per_frame].cpu().numpy()
+    exact = bool(np.array_equal(hp, hb))
+    if check:
+        from jxs_util import load_oracle
+        O = load_oracle()
+        prm = O.make_params(W1, H1, D1, F1, B1, 1, 2, 5)
+        for k in (0, nframes - 1):
+            e, ref = O.encode(prm, frames[k])
+            exact = exact and e == 0 and ref == hp[k].tobytes()
+    fp = nframes * steps / (ms_plain / 1e3)
+    fb = nframes * steps / (ms_batch / 1e3)
+    for p in plans:
+        p.destroy()
+    batch.destroy()
+    enc.close()
+    return {"workload": "1920x1080 YUV422 8-bit planar, bpp 5, decomp_v 2, decomp_h 5, %d frames" % nframes,
+            "plain_fps": round(fp, 1), "batched_fps": round(fb, 1), "batched_over_plain": round(fb / fp, 3),
+            "bit_exact": bool(exact)}
+
+
+def mixed_batch(P, S, torch, dev, steps, check, nframes=48):
+    """The uneven batch: frames of four kinds (720p 4:2:0 8-bit bpp 1, 1080p
+    4:4:4 12-bit bpp 4, 2160p planar RGB 10-bit bpp 3, 1920x540 4:2:2 10-bit
+    bpp 5), decomp_v 1 or 2, drawn by seed, in one launch sequence."""
+    import numpy as np
+    draw = S.mixed_frames(nframes)
+    jobs = []
+    for i, (k, v) in enumerate(draw):
+        W1, H1, D1, F1, _ = S.MIXED_KINDS[k]
+        jobs.append((W1, H1, D1, F1, i, S.MIXED_BASE))
+    frames = S.make_frames_of(jobs)
+    encs = {}
+    for k, v in sorted(set(draw)):
+        W1, H1, D1, F1, B1 = S.MIXED_KINDS[k]
+        encs[(k, v)] = P.Encoder(W1, H1, D1, F1, B1, 1, v, 5, verbose=0)
+    d_in = [_dev_planes(torch, dev, fr) for fr in frames]
+    lcods = [encs[kv].bytes_per_frame for kv in draw]
+    ostride = (max(lcods) + 3) & ~3
+    out_p = torch.zeros((nframes, ostride), dtype=torch.uint8, device=dev)
+    out_b = torch.zeros((nframes, ostride), dtype=torch.uint8, device=dev)
+    plans = [P.Plan([_spec(encs[draw[i]], d_in[i], out_p[i].data_ptr())]) for i in range(nframes)]
+    batch = P.Plan([_spec(encs[draw[i]], d_in[i], out_b[i].data_ptr()) for i in range(nframes)])
+    stream = torch.cuda.current_stream(dev)
+    ms_plain, ms_batch = _time_plans(torch, stream, plans, batch, steps)
+    hp = out_p.cpu().numpy()
+    hb = out_b.cpu().numpy()
+    exact = all(np.array_equal(hp[i, :lcods[i]], hb[i, :lcods[i]]) for i in range(nframes))
+    exact = exact and batch.status()[0] == 0
+    if check:
+        from jxs_util import load_oracle
+        O = load_oracle()
+        seen = set()
+        for i, (k, v) in enumerate(draw):
+            if k in seen:
+                continue
+            seen.add(k)
+            W1, H1, D1, F1, B1 = S.MIXED_KINDS[k]
+            e, ref = O.encode(O.make_params(W1, H1, D1, F1, B1, 1, v, 5), frames[i])
+            exact = exact and e == 0 and ref == hb[i, :lcods[i]].tobytes()
+    pixels = sum(S.MIXED_KINDS[k][0] * S.MIXED_KINDS[k][1] for k, _ in draw)
+    in_bytes = sum(sum(p.nbytes for p in fr) for fr in frames) + sum(lcods)
+    fp = nframes * steps / (ms_plain / 1e3)
+    fb = nframes * steps / (ms_batch / 1e3)
+    for p in plans:
+        p.destroy()
+    batch.destroy()
+    for e in encs.values():
+        e.close()
+    counts = {}
+    for k, v in draw:
+        key = "%dx%d_%s_v%d" % (S.MIXED_KINDS[k][0], S.MIXED_KINDS[k][1], ["", "", "420", "422", "444"][
+            S.MIXED_KINDS[k][3]], v)
+        counts[key] = counts.get(key, 0) + 1
+    return {"workload": "%d frames drawn by seed from 4 kinds (720p 420 8b bpp1, 1080p 444 12b bpp4, 2160p "
+                        "planar RGB 10b bpp3, 1920x540 422 10b bpp5), decomp_v 1-2, decomp_h 5" % nframes,
+            "mix": counts, "batched_fps": round(fb, 1), "plain_fps": round(fp, 1),
+            "batched_gpixel_per_s": round(pixels * steps / (ms_batch / 1e3) / 1e9, 3),
+            "batched_hbm_frac": round(in_bytes * steps / (ms_batch / 1e3) / 1e9 / HBM_PEAK_GBS, 5),
+            "ms_per_batch": round(ms_batch / steps, 3), "bit_exact": bool(exact)}
+
+
+def serving(P, S, torch, dev, check, nstreams=32, seconds=3.0, budget_gib=16, max_batch=32, inflight=3):
+    """The serving loop: nstreams live streams (kind and levels drawn from
+    the mixed set, Poisson arrivals at 30-60 frames/s each) submitted in real
+    time to one server; latency is submission to observed completion.  Then
+    a burst of 8 frames per stream at once measures the served capacity."""
+    import numpy as np
+    POOL, RING = 2, 8
+    streams = S.serving_streams(nstreams)
+    kinds = sorted({k for k, _, _ in streams})
+    jobs = [(S.MIXED_KINDS[k][0], S.MIXED_KINDS[k][1], S.MIXED_KINDS[k][2], S.MIXED_KINDS[k][3], 1000 * k + f,
+             S.SERVING_BASE) for k in kinds for f in range(POOL)]
+    frames = S.make_frames_of(jobs)
+    pool = {}
+    for (W1, H1, D1, F1, fr, _), planes in zip(jobs, frames):
+        k = fr // 1000
+        pool.setdefault(k, []).append(_dev_planes(torch, dev, planes))
+    encs, outs = [], []
+    for k, v, _ in streams:
+        W1, H1, D1, F1, B1 = S.MIXED_KINDS[k]
+        e = P.Encoder(W1, H1, D1, F1, B1, 1, v, 5, verbose=0)
+        encs.append(e)
+        outs.append(torch.zeros((RING, (e.bytes_per_frame + 3) & ~3), dtype=torch.uint8, device=dev))
+    # each (stream kind, levels, pool frame) encoded alone, in a one-frame plan
+    expected = {}
+    stream = torch.cuda.current_stream(dev)
+    for s, (k, v, _) in enumerate(streams):
+        if (k, v) in expected:
+            continue
+        got = []
+        for f in range(POOL):
+            scratch = torch.zeros_like(outs[s][0])
+            plan = P.Plan([_spec(encs[s], pool[k][f], scratch.data_ptr())])
+            plan.run(stream.cuda_stream)
+            torch.cuda.synchronize()
+            got.append(scratch[:encs[s].bytes_per_frame].cpu().numpy().tobytes())
+            plan.destroy()
+        expected[(k, v)] = got
+    srv = P.Server(budget_gib << 30, max_batch, inflight)
+
+    def submit(s, n, tag):
+        k = streams[s][0]
+        rc = srv.submit(encs[s], [t.data_ptr() for t in pool[k][n % POOL]], [t.shape[1] for t in pool[k][n % POOL]],
+                        outs[s][n % RING].data_ptr(), tag)
+        if rc:
+            raise RuntimeError("server_submit rc=%d" % rc)
+
+    # warm-up: one frame per stream
+    for s in range(nstreams):
+        submit(s, 0, s)
+    srv.drain(nstreams)
+    # real-time Poisson arrivals
+    events = sorted((t, s) for s, (_, _, fps) in enumerate(streams)
+                    for t in S.poisson_arrivals(fps, seconds, S.splitmix64(S.SERVING_BASE ^ (s + 1))))
+    count = [0] * nstreams
+    comps = []
+    t0 = time.monotonic_ns() + 20_000_000
+    for i, (t, s) in enumerate(events):
+        target = t0 + int(t * 1e9)
+        while True:
+            now = time.monotonic_ns()
+            if now >= target:
+                break
+            if target - now > 2_000_000:
+                time.sleep((target - now) / 1e9 - 0.001)
+        submit(s, count[s], i)
+        count[s] += 1
+        if (i & 15) == 15:
+            comps += srv.poll(1024)
+    comps += srv.drain(len(events) - len(comps))
+    st_rt = srv.stats()
+    lat = np.array([(c["done_ns"] - c["submit_ns"]) / 1e6 for c in comps])
+    wait = np.array([(c["launch_ns"] - c["submit_ns"]) / 1e6 for c in comps])
+    bsz = np.array([c["batch_frames"] for c in comps])
+    errs = sum(1 for c in comps if c["status"] != 0)
+    span = (max(c["done_ns"] for c in comps) - min(c["submit_ns"] for c in comps)) / 1e9
+    exact = errs == 0 and len(comps) == len(events)
+
+    def ring_ok():
+        ok = True
+        for s, (k, v, _) in enumerate(streams):
+            host = outs[s].cpu().numpy()
+            lc = encs[s].bytes_per_frame
+            for j in range(min(RING, count[s])):
+                ok = ok and host[j, :lc].tobytes() == expected[(k, v)][j % POOL]
+        return ok
+
+    exact = exact and ring_ok()
+    # burst: RING frames per stream at once (every frame its own buffer)
+    for o in outs:
+        o.zero_()
+    torch.cuda.synchronize()
+    b0 = srv.now_ns()
+    for n in range(RING):
+        for s in range(nstreams):
+            submit(s, n, (1 << 32) + n * nstreams + s)
+    bc = srv.drain(RING * nstreams)
+    b1 = max(c["done_ns"] for c in bc)
+    st = srv.stats()
+    count = [RING] * nstreams
+    burst_exact = len(bc) == RING * nstreams and all(c["status"] == 0 for c in bc) and ring_ok()
+    burst_lat = np.array([(c["done_ns"] - c["submit_ns"]) / 1e6 for c in bc])
+    srv.destroy()
+    for e in encs:
+        e.close()
+    mix = {}
+    for k, v, _ in streams:
+        key = "%dx%d_%s_v%d" % (S.MIXED_KINDS[k][0], S.MIXED_KINDS[k][1], ["", "", "420", "422", "444"][
+            S.MIXED_KINDS[k][3]], v)
+        mix[key] = mix.get(key, 0) + 1
+    return {"workload": "%d live streams (kind and decomp_v drawn from the mixed set), Poisson arrivals at "
+                        "30-60 frames/s each, %.1f s real time, frames resident in HBM" % (nstreams, seconds),
+            "streams": mix, "offered_fps": round(sum(f for _, _, f in streams), 1), "frames": len(comps),
+            "aggregate_fps": round(len(comps) / span, 1),
+            "p50_latency_ms": round(float(np.percentile(lat, 50)), 4),
+            "p99_latency_ms": round(float(np.percentile(lat, 99)), 4),
+            "p999_latency_ms": round(float(np.percentile(lat, 99.9)), 4),
+            "max_latency_ms": round(float(lat.max()), 4),
+            "p99_queue_wait_ms": round(float(np.percentile(wait, 99)), 4),
+            "mean_batch_frames": round(float(bsz.mean()), 2), "batches": int(st_rt["batches"]) - nstreams,
+            "frame_errors": errs, "bit_exact_vs_alone": bool(exact),
+            "hbm_budget_bytes": int(st["hbm_budget"]), "hbm_peak_bytes": int(st["hbm_peak"]),
+            "max_batch": max_batch, "max_inflight": inflight,
+            "burst": {"frames": len(bc), "served_fps": round(len(bc) / ((b1 - b0) / 1e9), 1),
+                      "p99_latency_ms": round(float(np.percentile(burst_lat, 99)), 4),
+                      "largest_batch": int(st["largest_batch"]), "bit_exact_vs_alone": bool(burst_exact)},
+            "target_p99_ms": round(1000.0 / 60.0, 3)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -149,6 +408,10 @@ def main():
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--api-frames", type=int, default=32,
                     help="frames pushed through the C API with host buffers (PCIe-inclusive rate; 0: skip)")
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving-loop run")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed batch and the 1080p comparison")
+    ap.add_argument("--serve-seconds", type=float, default=3.0)
+    ap.add_argument("--streams", type=int, default=32)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="run one device-to-device copy of MIB MiB first: the reference point for the "
                          "FETCH_SIZE / WRITE_SIZE counters of a profiled run")
@@ -277,6 +540,16 @@ def main():
     if args.api_frames:
         api = api_host_path(P, torch, frames, args.api_frames)
 
+    mixed = hd = srv = None
+    if not args.no_mixed:
+        log("1080p plain vs batched")
+        hd = plain_vs_batched_1080p(P, S, torch, dev, args.steps, not args.no_check)
+        log("mixed batch")
+        mixed = mixed_batch(P, S, torch, dev, args.steps, not args.no_check)
+    if not args.no_serving:
+        log("serving loop")
+        srv = serving(P, S, torch, dev, not args.no_check, nstreams=args.streams, seconds=args.serve_seconds)
+
     cpu = None
     if not args.no_cpu_baseline:
         log("cpu baseline")
@@ -315,6 +588,9 @@ def main():
                      "pipeline_frac": round(bytes_per_frame * fps_plain / 1e9 / HBM_PEAK_GBS, 5)},
         "cpu_baseline": cpu,
         "api_host_buffers": api,
+        "plain_vs_batched_1080p": hd,
+        "mixed_batch": mixed,
+        "serving": srv,
         "wall_s_plain": round(wall_plain, 3),
     }
     print(json.dumps(out), flush=True)

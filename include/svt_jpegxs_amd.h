@@ -96,6 +96,71 @@ PREFIX_API SvtJxsErrorType_t svt_jpeg_xs_amd_decode_plan_run(svt_jpeg_xs_amd_dec
 PREFIX_API SvtJxsErrorType_t svt_jpeg_xs_amd_decode_plan_status(svt_jpeg_xs_amd_decode_plan_t* plan, uint32_t* errors);
 PREFIX_API void svt_jpeg_xs_amd_decode_plan_destroy(svt_jpeg_xs_amd_decode_plan_t* plan);
 
+/* ---------------- serving loop ----------------
+ * A request loop for many live streams whose frames arrive in HBM (from a
+ * capture DMA, a decoder, or a host copy the caller made).  Frames are
+ * admitted while the server's HBM budget has room for their work buffers and
+ * descriptors, and grouped into batches by continuous batching: whenever a
+ * launch slot is free, every queued frame that fits (up to max_batch, in
+ * arrival order) goes into one launch; frames that arrive meanwhile wait for
+ * the next.  Up to max_inflight batches run at once, each on its own HIP
+ * stream, so their kernels overlap on the device.  The per-stream state is
+ * the encoder's (band layout, weight tables, slice budgets), resident in HBM
+ * from svt_jpeg_xs_encoder_init; a frame's codestream is the same whatever
+ * batch carries it.  The reference has no counterpart: its encoder takes one
+ * stream per handle (EncHandle.c). */
+
+typedef struct svt_jpeg_xs_amd_server_config {
+    uint64_t hbm_budget;   /* device bytes for admitted frames' work buffers and batch descriptors */
+    uint32_t max_batch;    /* most frames in one launch, 1 .. 4096 */
+    uint32_t max_inflight; /* batches in flight at once, one HIP stream each, 1 .. 4 */
+} svt_jpeg_xs_amd_server_config_t;
+
+typedef struct svt_jpeg_xs_amd_completion {
+    uint64_t tag;          /* as submitted */
+    int32_t status;        /* SvtJxsErrorNone, SvtJxsErrorEncodeFrameError, or SvtJxsErrorUndefined (launch failed) */
+    uint32_t batch_frames; /* frames in the launch that carried this frame */
+    uint64_t submit_ns;    /* times on the svt_jpeg_xs_amd_server_now_ns clock: submitted, */
+    uint64_t launch_ns;    /* its batch launched, */
+    uint64_t done_ns;      /* completion observed (the codestream is in d_out) */
+} svt_jpeg_xs_amd_completion_t;
+
+typedef struct svt_jpeg_xs_amd_server_stats {
+    uint64_t submitted, completed, batches;
+    uint64_t largest_batch;
+    uint64_t hbm_budget, hbm_in_use, hbm_peak; /* bytes of the budget held by batches in flight */
+    uint32_t queued, inflight;
+} svt_jpeg_xs_amd_server_stats_t;
+
+typedef struct svt_jpeg_xs_amd_server svt_jpeg_xs_amd_server_t;
+
+/* Allocates the HBM budget on the current device and starts the loop.
+ * Returns NULL on a bad configuration or when the budget cannot be allocated. */
+PREFIX_API svt_jpeg_xs_amd_server_t* svt_jpeg_xs_amd_server_create(const svt_jpeg_xs_amd_server_config_t* cfg);
+
+/* Queues one frame; never blocks.  The encoder, planes and codestream buffer
+ * must stay valid until the frame's completion has been polled.
+ * SvtJxsErrorBadParameter: invalid spec (as for a plan).
+ * SvtJxsErrorInsufficientResources: the frame alone exceeds the budget. */
+PREFIX_API SvtJxsErrorType_t svt_jpeg_xs_amd_server_submit(svt_jpeg_xs_amd_server_t* server,
+                                                           const svt_jpeg_xs_amd_frame_spec_t* frame, uint64_t tag);
+
+/* Moves up to max completions, in the order they were observed, into out.
+ * With blocking set, waits until at least one is available or nothing is
+ * queued or in flight.  Returns how many were written. */
+PREFIX_API uint32_t svt_jpeg_xs_amd_server_poll(svt_jpeg_xs_amd_server_t* server, svt_jpeg_xs_amd_completion_t* out,
+                                                uint32_t max, uint32_t blocking);
+
+PREFIX_API SvtJxsErrorType_t svt_jpeg_xs_amd_server_stats(svt_jpeg_xs_amd_server_t* server,
+                                                          svt_jpeg_xs_amd_server_stats_t* stats);
+
+/* Monotonic clock of the completion records, in nanoseconds. */
+PREFIX_API uint64_t svt_jpeg_xs_amd_server_now_ns(void);
+
+/* Finishes every queued and in-flight frame, stops the loop and releases the
+ * budget.  Completions not yet polled are dropped. */
+PREFIX_API void svt_jpeg_xs_amd_server_destroy(svt_jpeg_xs_amd_server_t* server);
+
 /* Version / build string of the library. */
 PREFIX_API const char* svt_jpeg_xs_amd_version(void);
 

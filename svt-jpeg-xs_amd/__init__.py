@@ -4,7 +4,8 @@ Mirrors the reference C API (Source/API/SvtJpegxsEnc.h, SvtJpegxsDec.h):
 the ctypes structures below have the reference's layout, and the Encoder /
 Decoder classes call svt_jpeg_xs_encoder_* / svt_jpeg_xs_decoder_* exactly as
 the reference's own tools do (load defaults, init, send, get, close).  The
-Plan class drives the device-resident batch entry points of svt_jpegxs_amd.h.
+Plan class drives the device-resident batch entry points of svt_jpegxs_amd.h,
+and the Server class its serving loop.
 
 The package has no CPU fallback: importing it on a machine without the built
 library raises, and the GPU entry points fail loudly without a GPU.
@@ -99,6 +100,24 @@ class FrameSpec(ctypes.Structure):
                 ("strides", ctypes.c_uint32 * 4), ("d_out", ctypes.c_void_p)]
 
 
+class ServerConfig(ctypes.Structure):
+    """svt_jpeg_xs_amd_server_config_t (include/svt_jpegxs_amd.h)."""
+    _fields_ = [("hbm_budget", ctypes.c_uint64), ("max_batch", ctypes.c_uint32), ("max_inflight", ctypes.c_uint32)]
+
+
+class Completion(ctypes.Structure):
+    """svt_jpeg_xs_amd_completion_t (include/svt_jpegxs_amd.h)."""
+    _fields_ = [("tag", ctypes.c_uint64), ("status", ctypes.c_int32), ("batch_frames", ctypes.c_uint32),
+                ("submit_ns", ctypes.c_uint64), ("launch_ns", ctypes.c_uint64), ("done_ns", ctypes.c_uint64)]
+
+
+class ServerStats(ctypes.Structure):
+    """svt_jpeg_xs_amd_server_stats_t (include/svt_jpegxs_amd.h)."""
+    _fields_ = [("submitted", ctypes.c_uint64), ("completed", ctypes.c_uint64), ("batches", ctypes.c_uint64),
+                ("largest_batch", ctypes.c_uint64), ("hbm_budget", ctypes.c_uint64), ("hbm_in_use", ctypes.c_uint64),
+                ("hbm_peak", ctypes.c_uint64), ("queued", ctypes.c_uint32), ("inflight", ctypes.c_uint32)]
+
+
 class DecodeSpec(ctypes.Structure):
     """svt_jpeg_xs_amd_decode_spec_t (include/svt_jpegxs_amd.h)."""
     _fields_ = [("decoder", ctypes.POINTER(DecoderApi)), ("d_codestream", ctypes.c_void_p),
@@ -123,6 +142,8 @@ EXPORTED_SYMBOLS = [
     "svt_jpeg_xs_amd_version",
     "svt_jpeg_xs_amd_decode_plan_create", "svt_jpeg_xs_amd_decode_plan_run", "svt_jpeg_xs_amd_decode_plan_status",
     "svt_jpeg_xs_amd_decode_plan_destroy", "svt_jpeg_xs_amd_plan_precinct_offsets", "svt_jpeg_xs_amd_plan_debug_copy",
+    "svt_jpeg_xs_amd_server_create", "svt_jpeg_xs_amd_server_submit", "svt_jpeg_xs_amd_server_poll",
+    "svt_jpeg_xs_amd_server_stats", "svt_jpeg_xs_amd_server_now_ns", "svt_jpeg_xs_amd_server_destroy",
 ]
 
 _lib = None
@@ -180,6 +201,17 @@ def lib():
         L.svt_jpeg_xs_amd_plan_footprint.argtypes = [ctypes.POINTER(FrameSpec), ctypes.c_uint32]
         L.svt_jpeg_xs_amd_plan_footprint.restype = ctypes.c_uint64
         L.svt_jpeg_xs_amd_version.restype = ctypes.c_char_p
+        L.svt_jpeg_xs_amd_server_create.argtypes = [ctypes.POINTER(ServerConfig)]
+        L.svt_jpeg_xs_amd_server_create.restype = ctypes.c_void_p
+        L.svt_jpeg_xs_amd_server_submit.argtypes = [ctypes.c_void_p, ctypes.POINTER(FrameSpec), ctypes.c_uint64]
+        L.svt_jpeg_xs_amd_server_poll.argtypes = [ctypes.c_void_p, ctypes.POINTER(Completion), ctypes.c_uint32,
+                                                  ctypes.c_uint32]
+        L.svt_jpeg_xs_amd_server_poll.restype = ctypes.c_uint32
+        L.svt_jpeg_xs_amd_server_stats.argtypes = [ctypes.c_void_p, ctypes.POINTER(ServerStats)]
+        L.svt_jpeg_xs_amd_server_now_ns.argtypes = []
+        L.svt_jpeg_xs_amd_server_now_ns.restype = ctypes.c_uint64
+        L.svt_jpeg_xs_amd_server_destroy.argtypes = [ctypes.c_void_p]
+        L.svt_jpeg_xs_amd_server_destroy.restype = None
         for name in ("svt_jpeg_xs_encoder_load_default_parameters", "svt_jpeg_xs_encoder_get_image_config",
                      "svt_jpeg_xs_encoder_init", "svt_jpeg_xs_encoder_send_picture", "svt_jpeg_xs_encoder_get_packet",
                      "svt_jpeg_xs_decoder_init", "svt_jpeg_xs_decoder_get_single_frame_size_with_proxy",
@@ -187,7 +219,8 @@ def lib():
                      "svt_jpeg_xs_decoder_send_packet", "svt_jpeg_xs_decoder_get_frame", "svt_jpeg_xs_decoder_send_eoc",
                      "svt_jpeg_xs_amd_plan_run", "svt_jpeg_xs_amd_plan_run_timed", "svt_jpeg_xs_amd_plan_status",
                      "svt_jpeg_xs_amd_decode_plan_run", "svt_jpeg_xs_amd_decode_plan_status",
-                     "svt_jpeg_xs_amd_plan_precinct_offsets", "svt_jpeg_xs_amd_plan_debug_copy"):
+                     "svt_jpeg_xs_amd_plan_precinct_offsets", "svt_jpeg_xs_amd_plan_debug_copy",
+                     "svt_jpeg_xs_amd_server_submit", "svt_jpeg_xs_amd_server_stats"):
             getattr(L, name).restype = ctypes.c_int32
         _lib = L
     return _lib
@@ -328,24 +361,38 @@ class Decoder:
         return rc, planes
 
 
+def _spec(spec, enc, planes, strides, out):
+    spec.encoder = ctypes.pointer(enc.api)
+    for c, (p, s) in enumerate(zip(planes, strides)):
+        spec.d_planes[c] = p
+        spec.strides[c] = s
+    spec.d_out = out
+
+
 class Plan:
     """A device-resident batch (svt_jpeg_xs_amd_plan_*).  `frames` is a list
     of (encoder, [device plane pointers], [strides], device output pointer)."""
 
     def __init__(self, frames):
         self.frames = frames
-        specs = (FrameSpec * len(frames))()
-        for i, (enc, planes, strides, out) in enumerate(frames):
-            specs[i].encoder = ctypes.pointer(enc.api)
-            for c, (p, s) in enumerate(zip(planes, strides)):
-                specs[i].d_planes[c] = p
-                specs[i].strides[c] = s
-            specs[i].d_out = out
+        specs = self._specs_of(frames)
         self._specs = specs
         self.footprint = lib().svt_jpeg_xs_amd_plan_footprint(specs, len(frames))
         self.handle = lib().svt_jpeg_xs_amd_plan_create(specs, len(frames))
         if not self.handle:
             raise RuntimeError("svt_jpeg_xs_amd_plan_create failed")
+
+    @staticmethod
+    def _specs_of(frames):
+        specs = (FrameSpec * len(frames))()
+        for i, (enc, planes, strides, out) in enumerate(frames):
+            _spec(specs[i], enc, planes, strides, out)
+        return specs
+
+    @staticmethod
+    def footprint_of(frames):
+        """Device bytes a plan (or a serving batch) of these frames takes."""
+        return lib().svt_jpeg_xs_amd_plan_footprint(Plan._specs_of(frames), len(frames))
 
     def run(self, stream=0, use_graph=False):
         rc = lib().svt_jpeg_xs_amd_plan_run(self.handle, stream, 1 if use_graph else 0)
@@ -375,6 +422,65 @@ class Plan:
     def destroy(self):
         if self.handle:
             lib().svt_jpeg_xs_amd_plan_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+class Server:
+    """The serving loop (svt_jpeg_xs_amd_server_*): frames of many streams,
+    resident in HBM, admitted by HBM footprint and encoded by continuous
+    batching on up to `max_inflight` HIP streams."""
+
+    def __init__(self, hbm_budget, max_batch=64, max_inflight=2):
+        cfg = ServerConfig(hbm_budget, max_batch, max_inflight)
+        self.handle = lib().svt_jpeg_xs_amd_server_create(ctypes.byref(cfg))
+        if not self.handle:
+            raise RuntimeError("svt_jpeg_xs_amd_server_create failed")
+        self._keep = {}
+
+    @staticmethod
+    def now_ns():
+        return lib().svt_jpeg_xs_amd_server_now_ns()
+
+    def submit(self, enc, planes, strides, out, tag):
+        """Queues one frame: (encoder, [device plane pointers], [strides],
+        device output pointer); returns the library's status code."""
+        spec = FrameSpec()
+        _spec(spec, enc, planes, strides, out)
+        return lib().svt_jpeg_xs_amd_server_submit(self.handle, ctypes.byref(spec), tag)
+
+    def poll(self, max_n=256, blocking=False):
+        """Completed frames as dicts (tag, status, batch_frames, submit_ns,
+        launch_ns, done_ns)."""
+        buf = (Completion * max_n)()
+        n = lib().svt_jpeg_xs_amd_server_poll(self.handle, buf, max_n, 1 if blocking else 0)
+        return [{f: getattr(buf[i], f) for f, _ in Completion._fields_} for i in range(n)]
+
+    def drain(self, count):
+        """Blocks until `count` completions have been collected."""
+        out = []
+        while len(out) < count:
+            got = self.poll(max(1, min(1024, count - len(out))), blocking=True)
+            if not got:
+                break
+            out += got
+        return out
+
+    def stats(self):
+        st = ServerStats()
+        rc = lib().svt_jpeg_xs_amd_server_stats(self.handle, ctypes.byref(st))
+        if rc:
+            raise JxsError(rc, "server_stats")
+        return {f: getattr(st, f) for f, _ in ServerStats._fields_}
+
+    def destroy(self):
+        if self.handle:
+            lib().svt_jpeg_xs_amd_server_destroy(self.handle)
             self.handle = None
 
     def __del__(self):

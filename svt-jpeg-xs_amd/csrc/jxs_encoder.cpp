@@ -15,8 +15,8 @@
 #include <vector>
 
 #include "SvtJpegxsEnc.h"
+#include "jxs_enc.h"
 #include "svt_jpegxs_amd.h"
-#include "jxs_host.h"
 
 extern "C" {
 hipError_t jxs_launch_dwt(const JxsDwtJob*, const uint32_t*, int, uint32_t, uint32_t, uint32_t*, uint32_t, hipStream_t);
@@ -35,36 +35,6 @@ namespace jxs {
         }                                                                                   \
     } while (0)
 
-static inline uint64_t align256(uint64_t x) { return (x + 255) & ~(uint64_t)255; }
-
-/* Geometry of one encoder, host copy plus resident device copy. */
-struct DevGeom {
-    HostGeom host;
-    JxsGeom* d_geom = nullptr;
-    uint32_t* d_slice_off = nullptr;
-    uint32_t* d_slice_bud = nullptr;
-    void release() {
-        if (d_geom)
-            (void)hipFree(d_geom);
-        if (d_slice_off)
-            (void)hipFree(d_slice_off);
-        if (d_slice_bud)
-            (void)hipFree(d_slice_bud);
-        d_geom = nullptr;
-        d_slice_off = d_slice_bud = nullptr;
-    }
-    SvtJxsErrorType_t upload() {
-        CHECK_HIP(hipMalloc(&d_geom, sizeof(JxsGeom)));
-        CHECK_HIP(hipMemcpy(d_geom, &host.g, sizeof(JxsGeom), hipMemcpyHostToDevice));
-        const size_t n = host.g.slices * sizeof(uint32_t);
-        CHECK_HIP(hipMalloc(&d_slice_off, n));
-        CHECK_HIP(hipMalloc(&d_slice_bud, n));
-        CHECK_HIP(hipMemcpy(d_slice_off, host.slice_off.data(), n, hipMemcpyHostToDevice));
-        CHECK_HIP(hipMemcpy(d_slice_bud, host.slice_bud.data(), n, hipMemcpyHostToDevice));
-        return SvtJxsErrorNone;
-    }
-};
-
 /* Work bytes of one frame of a geometry (inputs and outputs excluded). */
 static uint64_t frame_work_bytes(const JxsGeom& g) {
     uint64_t b = 0;
@@ -77,25 +47,66 @@ static uint64_t frame_work_bytes(const JxsGeom& g) {
     return b;
 }
 
+void DevGeom::release() {
+    if (d_geom)
+        (void)hipFree(d_geom);
+    if (d_slice_off)
+        (void)hipFree(d_slice_off);
+    if (d_slice_bud)
+        (void)hipFree(d_slice_bud);
+    d_geom = nullptr;
+    d_slice_off = d_slice_bud = nullptr;
+}
+
+SvtJxsErrorType_t DevGeom::upload() {
+    CHECK_HIP(hipMalloc(&d_geom, sizeof(JxsGeom)));
+    CHECK_HIP(hipMemcpy(d_geom, &host.g, sizeof(JxsGeom), hipMemcpyHostToDevice));
+    const size_t n = host.g.slices * sizeof(uint32_t);
+    CHECK_HIP(hipMalloc(&d_slice_off, n));
+    CHECK_HIP(hipMalloc(&d_slice_bud, n));
+    CHECK_HIP(hipMemcpy(d_slice_off, host.slice_off.data(), n, hipMemcpyHostToDevice));
+    CHECK_HIP(hipMemcpy(d_slice_bud, host.slice_bud.data(), n, hipMemcpyHostToDevice));
+    work = frame_work_bytes(host.g);
+    /* how many transform jobs one frame has per stage and chunk class does
+     * not depend on where its buffers lie: count them on placeholder
+     * addresses (nothing is dereferenced) */
+    static const uint8_t placeholder[256] = {};
+    const void* planes[4] = {placeholder, placeholder, placeholder, placeholder};
+    const uint32_t strides[4] = {host.g.w, host.g.w, host.g.w, host.g.w};
+    FwdJobs jobs;
+    fwd_jobs(host.g, planes, strides, nullptr, nullptr, nullptr, nullptr, nullptr, jobs);
+    for (int s = 0; s < 2; s++)
+        for (const auto& j : jobs.dwt[s]) {
+            const int c = j.chunk == 8 ? 0 : j.chunk == 16 ? 1 : 2;
+            njobs[s][c]++;
+        }
+    return SvtJxsErrorNone;
+}
+
+uint64_t BatchSizer::upload_bytes() const {
+    uint64_t b = align256((uint64_t)n * sizeof(JxsFrameDesc));
+    for (int s = 0; s < 2; s++)
+        for (int c = 0; c < 3; c++)
+            b += align256((uint64_t)njobs[s][c] * sizeof(JxsDwtJob)) + align256((uint64_t)njobs[s][c] * 4);
+    return b;
+}
+
+uint64_t BatchSizer::bytes() const {
+    return upload_bytes() + align256((uint64_t)n * 4) + align256((uint64_t)units * sizeof(JxsUnit)) + work;
+}
+
 struct EncoderPriv;
 
 }  // namespace jxs
 
-/* A launch plan: everything one encode launch sequence needs, in HBM. */
+/* A launch plan: one device block holding a batch (descriptors, job lists,
+ * packing units and work buffers of every frame), optionally captured in a
+ * HIP graph. */
 struct svt_jpeg_xs_amd_plan {
     uint32_t nfr = 0;
     std::vector<const jxs::DevGeom*> geoms;
-    void* d_work = nullptr;
-    uint32_t* d_status = nullptr;
-    JxsFrameDesc* d_frames = nullptr;
-    JxsUnit* d_units = nullptr; /* packing units of every frame, frame after frame */
-    /* transform job lists per stage (first / second vertical level) and
-     * chunk class (8 / 16 / 32 columns per thread), one launch each */
-    JxsDwtJob* d_dwt[2][3] = {};
-    uint32_t* d_dbase[2][3] = {};
-    int ndwt[2][3] = {};
-    uint32_t dblocks[2][3] = {};
-    uint32_t total_prec = 0, total_slices = 0, total_units = 0;
+    void* d_block = nullptr;
+    jxs::BatchLaunch L;
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
     hipStream_t graph_stream = nullptr;
@@ -136,68 +147,66 @@ struct EncoderPriv {
     uint32_t nplanes = 0;
 };
 
-static void plan_free(svt_jpeg_xs_amd_plan_t* p) {
-    if (!p)
-        return;
-    if (p->graph_exec)
-        (void)hipGraphExecDestroy(p->graph_exec);
-    if (p->graph)
-        (void)hipGraphDestroy(p->graph);
-    (void)hipFree(p->d_work);
-    (void)hipFree(p->d_status);
-    (void)hipFree(p->d_frames);
-    (void)hipFree(p->d_units);
-    for (int s = 0; s < 2; s++) {
-        for (int c = 0; c < 3; c++) {
-            (void)hipFree(p->d_dwt[s][c]);
-            (void)hipFree(p->d_dbase[s][c]);
-        }
-    }
-    delete p;
+const DevGeom* encoder_geom(const svt_jpeg_xs_encoder_api_t* api) {
+    return api && api->private_ptr ? &((const EncoderPriv*)api->private_ptr)->geom : nullptr;
 }
 
-template <typename T>
-static hipError_t upload_vec(const std::vector<T>& v, T** dst) {
-    *dst = nullptr;
-    if (v.empty())
-        return hipSuccess;
-    hipError_t e = hipMalloc(dst, v.size() * sizeof(T));
-    if (e != hipSuccess)
-        return e;
-    return hipMemcpy(*dst, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice);
-}
-
-static svt_jpeg_xs_amd_plan_t* plan_build(const svt_jpeg_xs_amd_frame_spec_t* specs, uint32_t count) {
-    if (!specs || count == 0)
+const DevGeom* spec_geom(const svt_jpeg_xs_amd_frame_spec_t& s) {
+    /* the packing kernel writes the codestream as aligned dwords */
+    if (!s.d_out || ((uintptr_t)s.d_out & 3) != 0)
         return nullptr;
-    auto* p = new (std::nothrow) svt_jpeg_xs_amd_plan;
-    if (!p)
+    const DevGeom* dg = encoder_geom(s.encoder);
+    if (!dg)
         return nullptr;
-    p->nfr = count;
-    uint64_t work = 0;
-    for (uint32_t i = 0; i < count; i++) {
-        /* the packing kernel writes the codestream as aligned dwords */
-        if (!specs[i].encoder || !specs[i].encoder->private_ptr || !specs[i].d_out ||
-            ((uintptr_t)specs[i].d_out & 3) != 0) {
-            plan_free(p);
+    const JxsGeom& g = dg->host.g;
+    const uint32_t np = g.packed ? 1 : g.nc;
+    for (uint32_t c = 0; c < np; c++)
+        if (!s.d_planes[c] || s.strides[c] < (g.packed ? 3 * g.w : g.c[c].w))
             return nullptr;
-        }
-        const DevGeom* dg = &((EncoderPriv*)specs[i].encoder->private_ptr)->geom;
-        p->geoms.push_back(dg);
-        work += frame_work_bytes(dg->host.g);
-    }
-    if (hipMalloc(&p->d_work, work) != hipSuccess || hipMalloc(&p->d_status, count * sizeof(uint32_t)) != hipSuccess) {
-        plan_free(p);
-        return nullptr;
-    }
-    std::vector<JxsFrameDesc> frames(count);
-    std::vector<JxsDwtJob> dj[2];
-    uint8_t* cur = (uint8_t*)p->d_work;
-    uint32_t prec_base = 0, slice_base = 0, unit_base = 0;
+    return dg;
+}
+
+bool batch_place(const svt_jpeg_xs_amd_frame_spec_t* specs, uint32_t count, uint8_t* base, BatchLaunch& L,
+                 std::vector<uint8_t>& img) {
+    BatchSizer sz;
+    std::vector<const DevGeom*> geoms(count);
     for (uint32_t i = 0; i < count; i++) {
-        const DevGeom* dg = p->geoms[i];
+        geoms[i] = spec_geom(specs[i]);
+        if (!geoms[i])
+            return false;
+        sz.add(*geoms[i]);
+    }
+    img.assign(sz.upload_bytes(), 0);
+    L = BatchLaunch();
+    L.nfr = count;
+    /* block: frame descriptors | job list and block bases per stage and
+     * chunk class | (uploaded part ends) error words | packing units | the
+     * frames' work buffers */
+    uint64_t off = 0;
+    L.frames = (JxsFrameDesc*)(base + off);
+    const uint64_t frames_off = off;
+    off += align256((uint64_t)count * sizeof(JxsFrameDesc));
+    uint64_t job_off[2][3], base_off[2][3];
+    for (int s = 0; s < 2; s++)
+        for (int c = 0; c < 3; c++) {
+            job_off[s][c] = off;
+            L.dwt[s][c] = (JxsDwtJob*)(base + off);
+            off += align256((uint64_t)sz.njobs[s][c] * sizeof(JxsDwtJob));
+            base_off[s][c] = off;
+            L.dbase[s][c] = (uint32_t*)(base + off);
+            off += align256((uint64_t)sz.njobs[s][c] * 4);
+        }
+    L.status = (uint32_t*)(base + off);
+    off += align256((uint64_t)count * 4);
+    L.units = (JxsUnit*)(base + off);
+    off += align256((uint64_t)sz.units * sizeof(JxsUnit));
+    uint8_t* cur = base + off;
+    uint32_t prec_base = 0, slice_base = 0, unit_base = 0;
+    uint32_t fill[2][3] = {}, blk[2][3] = {};
+    for (uint32_t i = 0; i < count; i++) {
+        const DevGeom* dg = geoms[i];
         const JxsGeom& g = dg->host.g;
-        JxsFrameDesc& F = frames[i];
+        JxsFrameDesc F;
         memset(&F, 0, sizeof(F));
         F.g = dg->d_geom;
         F.slice_off = dg->d_slice_off;
@@ -214,8 +223,9 @@ static svt_jpeg_xs_amd_plan_t* plan_build(const svt_jpeg_xs_amd_frame_spec_t* sp
         cur += align256((uint64_t)g.rows * g.n_bl * JXS_HREC * 2);
         F.prec = (JxsPrec*)cur;
         cur += align256((uint64_t)g.rows * sizeof(JxsPrec));
+        F.units = L.units + unit_base;
         F.out = specs[i].d_out;
-        F.status = p->d_status + i;
+        F.status = L.status + i;
         F.rows = g.rows;
         F.npk = g.npk;
         F.prec_base = prec_base;
@@ -225,88 +235,104 @@ static svt_jpeg_xs_amd_plan_t* plan_build(const svt_jpeg_xs_amd_frame_spec_t* sp
         F.pps = g.pps;
         F.slices = g.slices;
         F.n_bl = g.n_bl;
+        memcpy(img.data() + frames_off + (uint64_t)i * sizeof(JxsFrameDesc), &F, sizeof(F));
         prec_base += g.rows;
         slice_base += g.slices;
         unit_base += g.rows * g.n_units;
         FwdJobs jobs;
         fwd_jobs(g, specs[i].d_planes, specs[i].strides, F.coeff, F.gcli, F.smax, F.hist, scratch, jobs);
-        for (int s = 0; s < 2; s++) {
-            dj[s].insert(dj[s].end(), jobs.dwt[s].begin(), jobs.dwt[s].end());
-        }
-    }
-    p->total_prec = prec_base;
-    p->total_slices = slice_base;
-    p->total_units = unit_base;
-    if (hipMalloc(&p->d_units, (size_t)(unit_base ? unit_base : 1) * sizeof(JxsUnit)) != hipSuccess) {
-        plan_free(p);
-        return nullptr;
-    }
-    for (uint32_t i = 0; i < count; i++)
-        frames[i].units = p->d_units + frames[i].unit_base;
-    bool ok = upload_vec(frames, &p->d_frames) == hipSuccess;
-    for (int s = 0; s < 2 && ok; s++)
-        for (int c = 0; c < 3 && ok; c++) {
-            const uint32_t chunk = 8u << c;
-            std::vector<JxsDwtJob> list;
-            std::vector<uint32_t> bases;
-            uint32_t blk = 0;
-            for (auto j : dj[s]) {
-                if (j.chunk != chunk)
-                    continue;
-                if (j.lev_l < 1 || (1u << j.lev_l) > j.chunk || j.tpr > (j.chunk == 8 ? 512u : 256u) || j.rpb == 0)
-                    ok = false; /* outside what the transform kernel's layout covers */
-                j.base = blk;
-                bases.push_back(blk);
-                blk += j.nblk;
-                list.push_back(j);
+        for (int s = 0; s < 2; s++)
+            for (auto j : jobs.dwt[s]) {
+                const int c = j.chunk == 8 ? 0 : j.chunk == 16 ? 1 : j.chunk == 32 ? 2 : -1;
+                /* outside what the transform kernel's layout covers */
+                if (c < 0 || j.lev_l < 1 || (1u << j.lev_l) > j.chunk || j.tpr > (j.chunk == 8 ? 512u : 256u) ||
+                    j.rpb == 0 || fill[s][c] >= sz.njobs[s][c])
+                    return false;
+                j.base = blk[s][c];
+                memcpy(img.data() + job_off[s][c] + (uint64_t)fill[s][c] * sizeof(JxsDwtJob), &j, sizeof(j));
+                memcpy(img.data() + base_off[s][c] + (uint64_t)fill[s][c] * 4, &blk[s][c], 4);
+                fill[s][c]++;
+                blk[s][c] += j.nblk;
             }
-            p->dblocks[s][c] = blk;
-            p->ndwt[s][c] = (int)list.size();
-            ok = ok && upload_vec(list, &p->d_dwt[s][c]) == hipSuccess &&
-                 upload_vec(bases, &p->d_dbase[s][c]) == hipSuccess;
-        }
-    if (!ok) {
-        plan_free(p);
-        return nullptr;
     }
-    return p;
+    for (int s = 0; s < 2; s++)
+        for (int c = 0; c < 3; c++) {
+            if (fill[s][c] != sz.njobs[s][c])
+                return false;
+            L.ndwt[s][c] = (int)fill[s][c];
+            L.dblocks[s][c] = blk[s][c];
+        }
+    L.total_slices = slice_base;
+    L.total_units = unit_base;
+    return (uint64_t)(cur - base) == sz.bytes();
 }
 
-/* The launch sequence of a plan.  ev (optional) gets an event recorded
- * before every stage and after the last. */
-static hipError_t plan_launch(svt_jpeg_xs_amd_plan_t* p, hipStream_t st, hipEvent_t* ev) {
+hipError_t batch_launch(const BatchLaunch& L, hipStream_t st, hipEvent_t* ev) {
     hipError_t e;
-    const int nf = (int)p->nfr;
-#define STEP(k, call)                       \
-    do {                                    \
-        if (ev)                             \
-            (void)hipEventRecord(ev[k], st); \
-        e = (call);                         \
-        if (e != hipSuccess)                \
-            return e;                       \
-    } while (0)
     /* every non-empty chunk class of a stage is one launch; the first launch
-     * of the plan also clears the frames' error flags */
+     * of the batch also clears the frames' error words */
     bool cleared = false;
     for (int s = 0; s < 2; s++) {
         if (ev)
             (void)hipEventRecord(ev[s], st);
         for (int c = 0; c < 3; c++) {
-            if (!p->ndwt[s][c])
+            if (!L.ndwt[s][c])
                 continue;
-            e = jxs_launch_dwt(p->d_dwt[s][c], p->d_dbase[s][c], p->ndwt[s][c], p->dblocks[s][c], 8u << c,
-                               cleared ? nullptr : p->d_status, cleared ? 0 : p->nfr, st);
+            e = jxs_launch_dwt(L.dwt[s][c], L.dbase[s][c], L.ndwt[s][c], L.dblocks[s][c], 8u << c,
+                               cleared ? nullptr : L.status, cleared ? 0 : L.nfr, st);
             if (e != hipSuccess)
                 return e;
             cleared = true;
         }
     }
-    STEP(2, jxs_launch_rate(p->d_frames, nf, p->total_slices, st));
-    STEP(3, jxs_launch_pack(p->d_units, p->total_units, st));
+    if (ev)
+        (void)hipEventRecord(ev[2], st);
+    if ((e = jxs_launch_rate(L.frames, (int)L.nfr, L.total_slices, st)) != hipSuccess)
+        return e;
+    if (ev)
+        (void)hipEventRecord(ev[3], st);
+    if ((e = jxs_launch_pack(L.units, L.total_units, st)) != hipSuccess)
+        return e;
     if (ev)
         (void)hipEventRecord(ev[SVT_JPEG_XS_AMD_STAGES], st);
-#undef STEP
     return hipSuccess;
+}
+
+static void plan_free(svt_jpeg_xs_amd_plan_t* p) {
+    if (!p)
+        return;
+    if (p->graph_exec)
+        (void)hipGraphExecDestroy(p->graph_exec);
+    if (p->graph)
+        (void)hipGraphDestroy(p->graph);
+    (void)hipFree(p->d_block);
+    delete p;
+}
+
+static svt_jpeg_xs_amd_plan_t* plan_build(const svt_jpeg_xs_amd_frame_spec_t* specs, uint32_t count) {
+    if (!specs || count == 0)
+        return nullptr;
+    BatchSizer sz;
+    std::vector<const DevGeom*> geoms(count);
+    for (uint32_t i = 0; i < count; i++) {
+        geoms[i] = spec_geom(specs[i]);
+        if (!geoms[i])
+            return nullptr;
+        sz.add(*geoms[i]);
+    }
+    auto* p = new (std::nothrow) svt_jpeg_xs_amd_plan;
+    if (!p)
+        return nullptr;
+    p->nfr = count;
+    p->geoms = geoms;
+    std::vector<uint8_t> img;
+    if (hipMalloc(&p->d_block, sz.bytes()) != hipSuccess ||
+        !batch_place(specs, count, (uint8_t*)p->d_block, p->L, img) ||
+        hipMemcpy(p->d_block, img.data(), img.size(), hipMemcpyHostToDevice) != hipSuccess) {
+        plan_free(p);
+        return nullptr;
+    }
+    return p;
 }
 
 static EncoderPriv* priv_of(svt_jpeg_xs_encoder_api_t* api) {
@@ -575,11 +601,11 @@ PREFIX_API SvtJxsErrorType_t svt_jpeg_xs_encoder_send_picture(svt_jpeg_xs_encode
                               e->plane_row_bytes[c], e->plane_rows[c], hipMemcpyHostToDevice, e->stream);
     }
     if (he == hipSuccess)
-        he = plan_launch(s.plan, e->stream, nullptr);
+        he = batch_launch(s.plan->L, e->stream, nullptr);
     if (he == hipSuccess)
         he = hipMemcpyAsync(in->bitstream.buffer, s.d_out, g.lcod, hipMemcpyDeviceToHost, e->stream);
     if (he == hipSuccess)
-        he = hipMemcpyAsync(s.h_status, s.plan->d_status, sizeof(uint32_t), hipMemcpyDeviceToHost, e->stream);
+        he = hipMemcpyAsync(s.h_status, s.plan->L.status, sizeof(uint32_t), hipMemcpyDeviceToHost, e->stream);
     if (he == hipSuccess)
         he = hipEventRecord(s.done, e->stream);
     std::lock_guard<std::mutex> lk(e->mu);
@@ -699,14 +725,15 @@ PREFIX_API svt_jpeg_xs_amd_plan_t* svt_jpeg_xs_amd_plan_create(const svt_jpeg_xs
 }
 
 PREFIX_API uint64_t svt_jpeg_xs_amd_plan_footprint(const svt_jpeg_xs_amd_frame_spec_t* frames, uint32_t count) {
-    uint64_t b = 0;
+    if (!frames)
+        return 0;
+    BatchSizer sz;
     for (uint32_t i = 0; i < count; i++) {
-        EncoderPriv* e = frames[i].encoder ? priv_of(frames[i].encoder) : nullptr;
-        if (e)
-            b += frame_work_bytes(e->geom.host.g) +
-                (uint64_t)e->geom.host.g.rows * e->geom.host.g.n_units * sizeof(JxsUnit);
+        const DevGeom* dg = encoder_geom(frames[i].encoder);
+        if (dg)
+            sz.add(*dg);
     }
-    return b;
+    return sz.n ? sz.bytes() : 0;
 }
 
 PREFIX_API SvtJxsErrorType_t svt_jpeg_xs_amd_plan_run(svt_jpeg_xs_amd_plan_t* plan, void* stream, uint32_t use_graph) {
@@ -723,7 +750,7 @@ PREFIX_API SvtJxsErrorType_t svt_jpeg_xs_amd_plan_run(svt_jpeg_xs_amd_plan_t* pl
         if (!plan->graph_exec) {
             if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess)
                 return SvtJxsErrorUndefined;
-            hipError_t le = plan_launch(plan, st, nullptr);
+            hipError_t le = batch_launch(plan->L, st, nullptr);
             hipError_t ee = hipStreamEndCapture(st, &plan->graph);
             if (le != hipSuccess || ee != hipSuccess)
                 return SvtJxsErrorUndefined;
@@ -733,7 +760,7 @@ PREFIX_API SvtJxsErrorType_t svt_jpeg_xs_amd_plan_run(svt_jpeg_xs_amd_plan_t* pl
         }
         return hipGraphLaunch(plan->graph_exec, st) == hipSuccess ? SvtJxsErrorNone : SvtJxsErrorUndefined;
     }
-    return plan_launch(plan, st, nullptr) == hipSuccess ? SvtJxsErrorNone : SvtJxsErrorUndefined;
+    return batch_launch(plan->L, st, nullptr) == hipSuccess ? SvtJxsErrorNone : SvtJxsErrorUndefined;
 }
 
 PREFIX_API SvtJxsErrorType_t svt_jpeg_xs_amd_plan_run_timed(svt_jpeg_xs_amd_plan_t* plan, void* stream, float* stage_ms) {
@@ -744,7 +771,7 @@ PREFIX_API SvtJxsErrorType_t svt_jpeg_xs_amd_plan_run_timed(svt_jpeg_xs_amd_plan
     for (int i = 0; i <= SVT_JPEG_XS_AMD_STAGES; i++)
         if (hipEventCreate(&ev[i]) != hipSuccess)
             return SvtJxsErrorInsufficientResources;
-    hipError_t e = plan_launch(plan, st, ev);
+    hipError_t e = batch_launch(plan->L, st, ev);
     if (e == hipSuccess)
         e = hipEventSynchronize(ev[SVT_JPEG_XS_AMD_STAGES]);
     for (int i = 0; i < SVT_JPEG_XS_AMD_STAGES && e == hipSuccess; i++)
@@ -758,7 +785,7 @@ PREFIX_API SvtJxsErrorType_t svt_jpeg_xs_amd_plan_status(svt_jpeg_xs_amd_plan_t*
     if (!plan)
         return SvtJxsErrorBadParameter;
     std::vector<uint32_t> st(plan->nfr);
-    if (hipMemcpy(st.data(), plan->d_status, plan->nfr * sizeof(uint32_t), hipMemcpyDeviceToHost) != hipSuccess)
+    if (hipMemcpy(st.data(), plan->L.status, plan->nfr * sizeof(uint32_t), hipMemcpyDeviceToHost) != hipSuccess)
         return SvtJxsErrorUndefined;
     bool any = false;
     for (uint32_t i = 0; i < plan->nfr; i++) {
@@ -776,7 +803,7 @@ PREFIX_API SvtJxsErrorType_t svt_jpeg_xs_amd_plan_debug_copy(svt_jpeg_xs_amd_pla
     if (!plan || frame >= plan->nfr)
         return SvtJxsErrorBadParameter;
     JxsFrameDesc F;
-    if (hipMemcpy(&F, plan->d_frames + frame, sizeof(F), hipMemcpyDeviceToHost) != hipSuccess)
+    if (hipMemcpy(&F, plan->L.frames + frame, sizeof(F), hipMemcpyDeviceToHost) != hipSuccess)
         return SvtJxsErrorUndefined;
     const JxsGeom& g = plan->geoms[frame]->host.g;
     if (coeff && hipMemcpy(coeff, F.coeff, g.n_coeff * 2, hipMemcpyDeviceToHost) != hipSuccess)
@@ -793,7 +820,7 @@ PREFIX_API SvtJxsErrorType_t svt_jpeg_xs_amd_plan_precinct_offsets(svt_jpeg_xs_a
     if (!plan || frame >= plan->nfr || !d_offsets || !stride)
         return SvtJxsErrorBadParameter;
     JxsFrameDesc F;
-    if (hipMemcpy(&F, plan->d_frames + frame, sizeof(F), hipMemcpyDeviceToHost) != hipSuccess)
+    if (hipMemcpy(&F, plan->L.frames + frame, sizeof(F), hipMemcpyDeviceToHost) != hipSuccess)
         return SvtJxsErrorUndefined;
     *d_offsets = &F.prec->offset;
     *stride = sizeof(JxsPrec) / sizeof(uint32_t);

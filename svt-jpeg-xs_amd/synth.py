@@ -119,3 +119,53 @@ def frame_bytes(planes):
 
 def frame_sha256(planes):
     return hashlib.sha256(frame_bytes(planes)).hexdigest()
+
+
+# The mixed workload: four stream kinds (width, height, depth, format, bpp),
+# each coded with decomp_h 5 and decomp_v 1 or 2.
+MIXED_KINDS = [
+    (1280, 720, 8, FORMAT_YUV420, 1),    # 720p 4:2:0 8-bit
+    (1920, 1080, 12, FORMAT_YUV444, 4),  # 1080p 4:4:4 12-bit
+    (3840, 2160, 10, FORMAT_YUV444, 3),  # 2160p planar RGB 10-bit
+    (1920, 540, 10, FORMAT_YUV422, 5),   # one interlaced 1080i field, 4:2:2 10-bit
+]
+MIXED_BASE = BASE_SEED + 3
+SERVING_BASE = BASE_SEED + 4
+
+
+def mixed_frames(count, base=MIXED_BASE):
+    """(kind, decomp_v) of each frame of a mixed batch, drawn by seed."""
+    rng = np.random.Generator(np.random.PCG64(splitmix64(base)))
+    return [(int(rng.integers(0, len(MIXED_KINDS))), int(rng.integers(1, 3))) for _ in range(count)]
+
+
+def serving_streams(count, base=SERVING_BASE):
+    """(kind, decomp_v, fps) of each live stream: kind and levels from the
+    mixed set, frame rate uniform in [30, 60]."""
+    rng = np.random.Generator(np.random.PCG64(splitmix64(base)))
+    return [(int(rng.integers(0, len(MIXED_KINDS))), int(rng.integers(1, 3)), float(rng.uniform(30.0, 60.0)))
+            for _ in range(count)]
+
+
+def poisson_arrivals(fps, duration_s, seed):
+    """Arrival times (s) in [0, duration_s) of a Poisson process of rate fps."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    out = []
+    t = float(rng.exponential(1.0 / fps))
+    while t < duration_s:
+        out.append(t)
+        t += float(rng.exponential(1.0 / fps))
+    return out
+
+
+def make_frames_of(jobs, workers=None):
+    """Frames for a list of (width, height, depth, fmt, frame, base) by a
+    process pool."""
+    import multiprocessing as mp
+    import os
+    jobs = [tuple(j) + (4.0,) for j in jobs]
+    n = workers or min(16, os.cpu_count() or 1, len(jobs))
+    if n <= 1:
+        return [_frame_job(j) for j in jobs]
+    with mp.get_context("fork").Pool(n) as pool:
+        return pool.map(_frame_job, jobs)

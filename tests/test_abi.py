@@ -164,3 +164,42 @@ def test_get_single_frame_size(name):
     size = ctypes.c_uint32()
     rc = L.svt_jpeg_xs_decoder_get_single_frame_size(buf.ctypes.data, len(buf) // 2, None, ctypes.byref(size), 0)
     assert rc == P.ERR_DEC_TOO_SHORT
+
+
+def test_server_structs_match_header_and_bad_configs_are_refused():
+    """The ctypes mirrors of the serving-loop structs have the C layout, and
+    invalid configurations or handles are refused before any device call."""
+    prog = r'''
+#include <stdio.h>
+#include <stddef.h>
+#include "svt_jpegxs_amd.h"
+int main(void) {
+  printf("%zu %zu %zu %zu %zu %zu\n", sizeof(svt_jpeg_xs_amd_server_config_t), sizeof(svt_jpeg_xs_amd_completion_t),
+         sizeof(svt_jpeg_xs_amd_server_stats_t), offsetof(svt_jpeg_xs_amd_completion_t, done_ns),
+         offsetof(svt_jpeg_xs_amd_server_stats_t, queued), sizeof(svt_jpeg_xs_amd_frame_spec_t));
+  return 0;
+}
+'''
+    with tempfile.TemporaryDirectory() as td:
+        c = os.path.join(td, "s.c")
+        open(c, "w").write(prog)
+        exe = os.path.join(td, "s")
+        subprocess.check_call(["gcc", "-I" + os.path.join(ROOT, "include"), c, "-o", exe])
+        sizes = [int(x) for x in subprocess.check_output([exe]).decode().split()]
+    P = load_package()
+    assert sizes == [ctypes.sizeof(P.ServerConfig), ctypes.sizeof(P.Completion), ctypes.sizeof(P.ServerStats),
+                     P.Completion.done_ns.offset, P.ServerStats.queued.offset, ctypes.sizeof(P.FrameSpec)]
+    L = P.lib()
+    assert not L.svt_jpeg_xs_amd_server_create(None)
+    for bad in [(0, 4, 1), (1 << 20, 0, 1), (1 << 20, 4097, 1), (1 << 20, 4, 0), (1 << 20, 4, 5)]:
+        assert not L.svt_jpeg_xs_amd_server_create(ctypes.byref(P.ServerConfig(*bad))), bad
+    spec = P.FrameSpec()
+    assert L.svt_jpeg_xs_amd_server_submit(None, ctypes.byref(spec), 0) == P.ERR_BAD_PARAMETER
+    assert L.svt_jpeg_xs_amd_server_poll(None, (P.Completion * 1)(), 1, 1) == 0
+    assert L.svt_jpeg_xs_amd_server_stats(None, ctypes.byref(P.ServerStats())) == P.ERR_BAD_PARAMETER
+    L.svt_jpeg_xs_amd_server_destroy(None)
+    t0 = L.svt_jpeg_xs_amd_server_now_ns()
+    t1 = L.svt_jpeg_xs_amd_server_now_ns()
+    assert 0 < t0 <= t1
+    # an uninitialised encoder makes a footprint of nothing
+    assert P.lib().svt_jpeg_xs_amd_plan_footprint(ctypes.byref(spec), 1) == 0
