@@ -1316,7 +1316,8 @@ __global__ void __launch_bounds__(WAVE * RT_WAVES) k_rate(const JxsFrameDesc* __
 /* packing                                                             */
 /* ------------------------------------------------------------------ */
 #define PK_WAVES 4
-#define RINGB 512      /* words of a wave's significance / bit-plane-count ring (2 KiB): a step adds <= 128 */
+#define RINGA 16       /* words of a wave's significance ring: a band line has at most 256 flags */
+#define RINGB 512      /* words of a wave's bit-plane-count ring (2 KiB): a step adds <= 128 */
 #define RINGC 1024     /* words of a wave's data ring (4 KiB): a step adds <= 512 */
 #define RING_FLUSH 256 /* flush once this many complete words are pending */
 
@@ -1353,6 +1354,10 @@ struct WaveOut {
         for (uint32_t k = lane; k < words; k += WAVE)
             ring[k] = 0;
     }
+    /* no segment: finish() writes nothing */
+    __device__ __forceinline__ void clear() {
+        lo = hi = pos = flushed = 0;
+    }
     /* ORs the n (1..64) low bits of v into the ring at bit p (words that
      * receive only zeros are skipped) */
     __device__ __forceinline__ void put(uint32_t p, uint64_t v, uint32_t n) {
@@ -1369,7 +1374,11 @@ struct WaveOut {
     /* Writes ring words [from, to) out and clears them.  A word wholly inside
      * the segment is stored; the segment's first and last words, which it
      * may share with its neighbours, are ORed into the zeroed output
-     * (device-scope atomics: the neighbours may run on other XCDs). */
+     * (device-scope atomics: the neighbours may run on other XCDs).  The
+     * stores are device-scope too, written through the XCD's L2: a
+     * neighbour's atomic on a line this wave left dirty in its L2 would
+     * first have to wait for that line's write-back, which made the
+     * write-back stores cost more than the coding itself. */
     __device__ __forceinline__ void flush(uint32_t from, uint32_t to) {
         for (uint32_t w = from + lane; w < to; w += WAVE) {
             uint32_t* slot = &ring[w & mask];
@@ -1377,7 +1386,8 @@ struct WaveOut {
             *slot = 0;
             const uint32_t b0 = 32 * w;
             if (b0 >= lo && b0 + 32 <= hi)
-                gst(ob + w, __builtin_bswap32(v));
+                __hip_atomic_store((GAS uint32_t*)(ob + w), __builtin_bswap32(v), __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
             else if (v)
                 gor(ob + w, __builtin_bswap32(v));
         }
@@ -1409,24 +1419,31 @@ __device__ __forceinline__ uint32_t quant_mag(uint32_t c, uint32_t gc, uint32_t 
     return gt == 0 ? m : u;
 }
 
-/* bit i of x (x < 2^8) -> bit 4i */
-__device__ __forceinline__ uint32_t spread8(uint32_t x) {
-    x = (x | (x << 12)) & 0x000F000Fu;
-    x = (x | (x << 6)) & 0x03030303u;
-    x = (x | (x << 3)) & 0x11111111u;
-    return x;
+typedef uint16_t u16x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ u16x2 as_u16x2(uint32_t x) { return __builtin_bit_cast(u16x2, x); }
+__device__ __forceinline__ uint32_t as_u32(u16x2 x) { return __builtin_bit_cast(uint32_t, x); }
+
+/* Swaps the bits at positions p and p + d of x for every p in m. */
+__device__ __forceinline__ uint32_t delta_swap(uint32_t x, uint32_t m, uint32_t d) {
+    const uint32_t t = ((x >> d) ^ x) & m;
+    return x ^ (t | (t << d));
 }
 
-/* Bit-plane nibbles of four magnitudes (< 2^15): nibble j holds bit j of
- * q0, q1, q2, q3 from its MSB down.  The high word (planes 8..14) is built
- * only when some lane of the wave needs it. */
-__device__ __forceinline__ uint64_t interleave4(uint32_t q0, uint32_t q1, uint32_t q2, uint32_t q3, bool any_high) {
-    const uint32_t lo = (spread8(q0 & 0xFFu) << 3) | (spread8(q1 & 0xFFu) << 2) | (spread8(q2 & 0xFFu) << 1) |
-        spread8(q3 & 0xFFu);
-    uint32_t hi = 0;
-    if (any_high)
-        hi = (spread8(q0 >> 8) << 3) | (spread8(q1 >> 8) << 2) | (spread8(q2 >> 8) << 1) | spread8(q3 >> 8);
-    return ((uint64_t)hi << 32) | lo;
+/* Bit-plane nibbles of one group's four 16-bit values r0 .. r3, held as
+ * rx = r0 | r1 << 16 and ry = r2 | r3 << 16: nibble j of the result holds
+ * bit j of r0, r1, r2, r3 from its MSB down, for planes 0..7 (sel
+ * PLANES_LO) or 8..15 (PLANES_HI).  One byte permute gathers the byte of
+ * each value into byte 3, 1, 2, 0 (the value index bit-reversed), which makes
+ * the wanted layout a rotation of the bit address [row:2 | plane:3] to
+ * [plane:3 | row:2]: three delta swaps exchange address bits 4 and 2, 3 and
+ * 1, 2 and 0. */
+#define PLANES_LO 0x04000602u
+#define PLANES_HI 0x05010703u
+__device__ __forceinline__ uint32_t planes8(uint32_t rx, uint32_t ry, uint32_t sel) {
+    uint32_t w = __builtin_amdgcn_perm(rx, ry, sel);
+    w = delta_swap(w, 0x0000F0F0u, 12);
+    w = delta_swap(w, 0x00CC00CCu, 6);
+    return delta_swap(w, 0x0A0A0A0Au, 3);
 }
 
 #define IPL 4 /* consecutive items coded by one lane per step */
@@ -1452,24 +1469,31 @@ __device__ __forceinline__ void sig_step(WaveOut& W, const uint32_t (&sf)[IPL], 
 }
 
 /* The inputs of one step of a band line's bit-plane-count and data
- * segments for one lane: groups i0 .. i0+N-1 (clamped into the line, so
- * every load is in bounds): GCLIs, significance maxima, coefficients. */
+ * segments for one lane: groups i0 .. i0+N-1 (N divides 8, i0 a multiple of
+ * N): GCLIs, coefficients, and the maximum of their significance group (the
+ * same for all N).  Lanes past the line's end load the last group's
+ * neighbourhood; the groups of a lane past the end (at most N - 1 beyond the
+ * line's last group) are read but not coded: the GCLI and coefficient
+ * buffers continue past every band line (later lines, then the next buffers
+ * of the frame's work block). */
 template <uint32_t N>
 struct GdIn {
-    uint32_t a[N], s8[N];
+    uint32_t a[N], s8;
     uint2 cv[N];
 };
 template <uint32_t N>
 __device__ __forceinline__ GdIn<N> gd_load(const uint8_t* gl, const uint8_t* sl, const uint16_t* cl, uint32_t gw,
                                            uint32_t ms, uint32_t i0) {
     GdIn<N> in;
+    const uint32_t b = min(i0, gw - 1);
+    const uint8_t* g = gl + b;
+    const uint16_t* c = cl + 4 * b;
 #pragma unroll
     for (uint32_t k = 0; k < N; k++) {
-        const uint32_t idx = min(i0 + k, gw - 1);
-        in.a[k] = gld(gl + idx);
-        in.s8[k] = ms ? gld(sl + idx / 8) : 0u;
-        in.cv[k] = gld2(cl + 4 * idx);
+        in.a[k] = gld(g + k);
+        in.cv[k] = gld2(c + 4 * k);
     }
+    in.s8 = ms ? gld(sl + b / 8) : 0u;
     return in;
 }
 
@@ -1479,70 +1503,97 @@ __device__ __forceinline__ GdIn<N> gd_load(const uint8_t* gl, const uint8_t* sl,
  * significance group) or raw 4-bit GCLIs; a lane's codes are concatenated
  * (at most 64 bits).  Data: per coded group a sign nibble, then bit-planes
  * gcli-1 .. gt of the quantised magnitudes, 4 bits each; groups no lane
- * codes are skipped. */
+ * codes are skipped.  The coefficients of a group arrive as two words of
+ * two 16-bit sign-magnitude values; quantisation and signs work on both
+ * halves at once.  With np = gcli - gt, r = q | sign << np (the sign of a
+ * nonzero quantised value) makes planes np .. 0 of r the sign nibble and the
+ * bit-planes in order, so one transpose gives the group's whole code.  A
+ * lane's data codes are concatenated up to 64 bits per write.  Coefficients
+ * past a band line's end are zero in the buffer, so they quantise to zero. */
 template <uint32_t N, bool UNIFORM>
-__device__ __forceinline__ void gd_step(WaveOut& B, WaveOut& C, const GdIn<N>& in, uint32_t i0, uint32_t w,
-                                        uint32_t gw, uint32_t gt, uint32_t ms, uint32_t raw) {
+__device__ __forceinline__ void gd_step(WaveOut& B, WaveOut& C, const GdIn<N>& in, uint32_t i0, uint32_t gw,
+                                        uint32_t gt, uint32_t ms, uint32_t raw) {
     const uint32_t lane = B.lane;
     /* bit-plane counts */
+    const uint32_t rem = gw - min(i0, gw); /* groups of this lane inside the line, capped below */
     uint32_t nb = 0;
     uint64_t v = 0;
+    if (raw) {
 #pragma unroll
-    for (uint32_t k = 0; k < N; k++) {
-        if (i0 + k >= gw)
-            break;
-        uint32_t len, code;
-        if (raw) {
-            len = 4;
-            code = in.a[k];
-        } else if (ms && in.s8[k] <= gt) {
-            len = 0;
-            code = 0;
-        } else {
-            const uint32_t m = in.a[k] > gt ? in.a[k] - gt : 0u;
-            len = m + 1;
-            code = ((1u << m) - 1) << 1;
+        for (uint32_t k = 0; k < N; k++) {
+            const uint32_t len = k < rem ? 4u : 0u;
+            v = (v << len) | (k < rem ? in.a[k] : 0u);
+            nb += len;
         }
-        v = (v << len) | code;
-        nb += len;
+    } else {
+        /* a group of an insignificant significance group has no code */
+        const bool sig_skip = ms && in.s8 <= gt;
+#pragma unroll
+        for (uint32_t k = 0; k < N; k++) {
+            const bool skip = sig_skip || k >= rem;
+            const uint32_t m = max(in.a[k], gt) - gt;
+            const uint32_t len = skip ? 0u : m + 1;
+            v = (v << len) | (skip ? 0u : (2u << m) - 2);
+            nb += len;
+        }
     }
     const uint32_t incl = wave_incl_scan(nb, lane);
     if (nb && v)
         B.put(B.pos + incl - nb, v, nb);
     B.pos += wave_last(incl);
-    /* data */
-    uint64_t dv[N];
+    /* data lengths first: the lane's position comes from their scan */
     uint32_t dl[N], dn = 0;
 #pragma unroll
     for (uint32_t k = 0; k < N; k++) {
-        const uint32_t i = i0 + k, gc = in.a[k];
-        const bool coded = i < gw && gc > gt;
-        dl[k] = 0;
-        dv[k] = 0;
-        if (__ballot(coded) == 0)
-            continue;
-        const uint32_t np = coded ? gc - gt : 0u;
-        const uint32_t c[4] = {in.cv[k].x & 0xFFFFu, in.cv[k].x >> 16, in.cv[k].y & 0xFFFFu, in.cv[k].y >> 16};
-        uint32_t q[4], signs = 0;
-#pragma unroll
-        for (uint32_t e = 0; e < 4; e++) {
-            q[e] = (coded && 4 * i + e < w) ? quant_mag<UNIFORM>(c[e], gc, gt) : 0u;
-            signs |= (q[e] && (c[e] & 0x8000u)) ? (8u >> e) : 0u;
-        }
-        const bool any_high = __ballot(np > 8) != 0;
-        const uint64_t planes = interleave4(q[0], q[1], q[2], q[3], any_high);
-        dl[k] = coded ? 4 * np + 4 : 0u;
-        dv[k] = coded ? ((uint64_t)signs << (4 * np)) | planes : 0u;
+        const bool coded = k < rem && in.a[k] > gt;
+        dl[k] = coded ? 4 * (in.a[k] - gt) + 4 : 0u;
         dn += dl[k];
     }
     const uint32_t dincl = wave_incl_scan(dn, lane);
     uint32_t at = C.pos + dincl - dn;
+    const uint32_t gtt = gt * 0x10001u;
+    uint64_t acc = 0;
+    uint32_t an = 0;
 #pragma unroll
     for (uint32_t k = 0; k < N; k++) {
-        if (dl[k] && dv[k])
-            C.put(at, dv[k], dl[k]);
-        at += dl[k];
+        if (__ballot(dl[k] != 0) == 0)
+            continue;
+        const uint32_t gc = in.a[k];
+        /* magnitudes; zero for a group this lane does not code */
+        const uint32_t mx = dl[k] ? in.cv[k].x & 0x7FFF7FFFu : 0u;
+        const uint32_t my = dl[k] ? in.cv[k].y & 0x7FFF7FFFu : 0u;
+        uint32_t qx, qy;
+        if (UNIFORM) {
+            qx = quant_mag<true>(mx & 0xFFFFu, gc, gt) | (quant_mag<true>(mx >> 16, gc, gt) << 16);
+            qy = quant_mag<true>(my & 0xFFFFu, gc, gt) | (quant_mag<true>(my >> 16, gc, gt) << 16);
+        } else {
+            qx = as_u32(as_u16x2(mx) >> as_u16x2(gtt));
+            qy = as_u32(as_u16x2(my) >> as_u16x2(gtt));
+        }
+        /* sign bit 15 moved down to bit np, kept where q != 0: q + 0x7FFF
+         * reaches bit 15 exactly when q > 0 (q < 2^15, so no carry crosses
+         * into the other half) */
+        const u16x2 rs = as_u16x2((15 + gt - gc) * 0x10001u);
+        const uint32_t nzx = (qx + 0x7FFF7FFFu) & 0x80008000u, nzy = (qy + 0x7FFF7FFFu) & 0x80008000u;
+        const uint32_t rx = qx | as_u32(as_u16x2(in.cv[k].x & nzx) >> rs);
+        const uint32_t ry = qy | as_u32(as_u16x2(in.cv[k].y & nzy) >> rs);
+        uint64_t dv = planes8(rx, ry, PLANES_LO);
+        if (__ballot(dl[k] > 32) != 0)
+            dv |= (uint64_t)planes8(rx, ry, PLANES_HI) << 32;
+        if (dl[k]) {
+            if (an + dl[k] > 64) {
+                if (acc)
+                    C.put(at, acc, an);
+                at += an;
+                acc = 0;
+                an = 0;
+            }
+            acc = an ? (acc << dl[k]) | dv : dv;
+            an += dl[k];
+        }
     }
+    if (an && acc)
+        C.put(at, acc, an);
     C.pos += wave_last(dincl);
 }
 
@@ -1568,10 +1619,17 @@ __device__ __forceinline__ PackIn pack_load(const JxsUnit& U, uint32_t lane) {
     return in;
 }
 
+/* s_waitcnt vmcnt(0) alone (expcnt and lgkmcnt fields at their no-wait
+ * maxima): every vector memory access of the wave has completed.  On this
+ * architecture stores and loads share the counter and complete in order, so
+ * waiting for a load also waits for every store issued before it; the packer
+ * places this wait where all outstanding accesses were issued one unit (or
+ * one step) of coding earlier. */
+__device__ __forceinline__ void vm_wait_all() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
 /* The packet header (PackHeaders.c:98-127, sign size 0) of the first band
- * line of a packet and the line's significance segment.  Returns whether
- * the segment's length differs from the plan's. */
-__device__ __forceinline__ bool pack_sig(const JxsUnit& U, const PackIn& in, uint32_t lane, uint32_t* ringb) {
+ * line of a packet, ORed into the zeroed output. */
+__device__ __forceinline__ void put_header(const JxsUnit& U, uint32_t lane) {
     if (U.hlen) {
         uint32_t w0, w1, w2;
         split_bits(U.hdr, U.hlen, U.hlo, w0, w1, w2);
@@ -1579,48 +1637,59 @@ __device__ __forceinline__ bool pack_sig(const JxsUnit& U, const PackIn& in, uin
         if (lane < 3 && piece)
             gor(U.hob + lane, __builtin_bswap32(piece));
     }
-    if (!U.len[0])
-        return false;
-    WaveOut B;
-    B.start(ringb, RINGB - 1, U.ob[0], U.lo[0], U.len[0], lane);
-    wave_sync();
-    sig_step(B, in.sf, U.sw, U.gt);
-    B.finish();
-    wave_sync();
-    return B.pos != B.hi;
 }
 
-/* The bit-plane-count and data segments of a band line, IPL groups per lane
- * and step, each step's successor loaded before the step is coded.
- * Returns whether a segment's length differs from the plan's. */
-template <bool UNIFORM>
-__device__ __forceinline__ bool pack_gd(const JxsUnit& U, const GdIn<IPL>& first, uint32_t lane, uint32_t* ringb,
-                                        uint32_t* ringc) {
-    if (!(U.len[1] | U.len[2]))
+/* The significance segment of a band line into ring A (written out later).
+ * Returns whether its length differs from the plan's. */
+__device__ __forceinline__ bool code_sig(const JxsUnit& U, const PackIn& in, uint32_t lane, WaveOut& A,
+                                         uint32_t* ringa) {
+    if (!U.len[0]) {
+        A.clear();
         return false;
+    }
+    A.start(ringa, RINGA - 1, U.ob[0], U.lo[0], U.len[0], lane);
+    wave_sync();
+    sig_step(A, in.sf, U.sw, U.gt);
+    return A.pos != A.hi;
+}
+
+/* The bit-plane-count and data segments of a band line into rings B and C,
+ * IPL groups per lane and step, each step's successor loaded before the step
+ * is coded.  From the second step on, a step starts by waiting for its
+ * inputs and writing out the words completed so far (when many are pending),
+ * then issues the next step's loads.  The segments' remaining words are
+ * written out later.  Returns whether a segment's length differs from the
+ * plan's. */
+template <bool UNIFORM>
+__device__ __forceinline__ bool code_gd(const JxsUnit& U, const GdIn<IPL>& first, uint32_t lane, WaveOut& B,
+                                        WaveOut& C, uint32_t* ringb, uint32_t* ringc) {
+    if (!(U.len[1] | U.len[2])) {
+        B.clear();
+        C.clear();
+        return false;
+    }
     const uint8_t* gl = U.gl;
     const uint8_t* sl = U.sl;
     const uint16_t* cl = U.cl;
-    const uint32_t w = U.w, gw = U.gw, gt = U.gt, ms = U.ms, raw = U.raw;
-    WaveOut B, C;
+    const uint32_t gw = U.gw, gt = U.gt, ms = U.ms, raw = U.raw;
     B.start(ringb, RINGB - 1, U.ob[1], U.lo[1], U.len[1], lane);
     C.start(ringc, RINGC - 1, U.ob[2], U.lo[2], U.len[2], lane);
     wave_sync();
     GdIn<IPL> cur = first;
     for (uint32_t base = 0; base < gw; base += IPL * WAVE) {
         const bool more = base + IPL * WAVE < gw;
+        if (base) {
+            vm_wait_all();
+            B.maybe_flush();
+            C.maybe_flush();
+        }
         GdIn<IPL> nxt;
         if (more)
             nxt = gd_load<IPL>(gl, sl, cl, gw, ms, base + IPL * WAVE + IPL * lane);
-        gd_step<IPL, UNIFORM>(B, C, cur, base + IPL * lane, w, gw, gt, ms, raw);
-        B.maybe_flush();
-        C.maybe_flush();
+        gd_step<IPL, UNIFORM>(B, C, cur, base + IPL * lane, gw, gt, ms, raw);
         if (more)
             cur = nxt;
     }
-    B.finish();
-    C.finish();
-    wave_sync();
     return B.pos != B.hi || C.pos != C.hi;
 }
 
@@ -1629,11 +1698,17 @@ __device__ __forceinline__ bool pack_gd(const JxsUnit& U, const GdIn<IPL>& first
  * waves of the launch; units ordered widest first), every one's
  * significance flags, bit-plane counts (unary or raw) and data bit-planes
  * with the quantisation (Quant.c:9-45) fused, each segment into its own
- * sub-packet, plus the packet header of a packet's first band line.  While
- * a unit is coded, the next unit's record and first inputs are already on
- * their way.  Waves never wait on each other. */
-#define PK_BLOCKS 1024 /* resident blocks: 256 CUs x 4 (the kernel's occupancy) */
-__global__ void __launch_bounds__(64 * PK_WAVES) k_pack(const JxsUnit* __restrict__ units, uint32_t total_units) {
+ * sub-packet, plus the packet header of a packet's first band line.  A unit
+ * is coded in the wave's LDS rings and written out at the start of the next
+ * unit's turn: wait for the next unit's inputs (loaded while this one was
+ * coded), write out the previous unit's segments and this unit's header,
+ * issue the loads for the unit after, then code from registers.  So every
+ * wait covers loads and stores issued a whole unit earlier.  Waves never
+ * wait on each other. */
+#define PK_BLOCKS 1280 /* resident blocks: 256 CUs x 5 (5 waves per SIMD: <= 96 VGPRs, 24.3 KiB LDS per block) */
+__global__ void __launch_bounds__(64 * PK_WAVES) __attribute__((amdgpu_waves_per_eu(5)))
+k_pack(const JxsUnit* __restrict__ units, uint32_t total_units) {
+    __shared__ uint32_t ringa[PK_WAVES][RINGA];
     __shared__ uint32_t ringb[PK_WAVES][RINGB];
     __shared__ uint32_t ringc[PK_WAVES][RINGC];
     const uint32_t wid = wave_uniform(threadIdx.x / WAVE), lane = threadIdx.x & (WAVE - 1);
@@ -1641,6 +1716,11 @@ __global__ void __launch_bounds__(64 * PK_WAVES) k_pack(const JxsUnit* __restric
     uint32_t gu = blockIdx.x * PK_WAVES + wid;
     if (gu >= total_units)
         return;
+    WaveOut A, B, C; /* the previous unit's segments, not yet written out */
+    A.clear();
+    B.clear();
+    C.clear();
+    A.lane = B.lane = C.lane = lane;
     JxsUnit U = cld_words(units + gu);
     PackIn in = pack_load(U, lane);
     for (;;) {
@@ -1649,12 +1729,18 @@ __global__ void __launch_bounds__(64 * PK_WAVES) k_pack(const JxsUnit* __restric
         JxsUnit Un;
         if (more)
             Un = cld_words(units + gn);
-        bool bad = pack_sig(U, in, lane, ringb[wid]);
+        vm_wait_all();
+        A.finish();
+        B.finish();
+        C.finish();
+        wave_sync();
+        put_header(U, lane);
         PackIn inn;
         if (more)
             inn = pack_load(Un, lane);
-        bad |= U.qpih ? pack_gd<true>(U, in.gd, lane, ringb[wid], ringc[wid])
-                      : pack_gd<false>(U, in.gd, lane, ringb[wid], ringc[wid]);
+        bool bad = code_sig(U, in, lane, A, ringa[wid]);
+        bad |= U.qpih ? code_gd<true>(U, in.gd, lane, B, C, ringb[wid], ringc[wid])
+                      : code_gd<false>(U, in.gd, lane, B, C, ringb[wid], ringc[wid]);
         /* the plan's lengths are the segments' exact bit counts */
         if (bad && lane == 0)
             gor(U.status, 2u);
@@ -1664,6 +1750,9 @@ __global__ void __launch_bounds__(64 * PK_WAVES) k_pack(const JxsUnit* __restric
         U = Un;
         in = inn;
     }
+    A.finish();
+    B.finish();
+    C.finish();
 }
 
 /* ------------------------------------------------------------------ */
