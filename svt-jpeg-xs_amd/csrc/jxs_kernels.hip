@@ -678,7 +678,9 @@ __device__ __forceinline__ void dwt_slots(const CAS JxsDwtJob& J, uint32_t bi, D
                 continue;
             if (R.lane < JXS_HREC) {
                 const uint32_t k = R.lane, c = L.hc[R.slot][b][k];
-                uint16_t* rec = hist + ((size_t)(ir / od->hl) * od->n_bl + od->bl + ir % od->hl) * JXS_HREC;
+                /* a band has 1 or 2 lines per precinct (checked by the host) */
+                const uint32_t hs = od->hl >> 1;
+                uint16_t* rec = hist + ((size_t)(ir >> hs) * od->n_bl + od->bl + (ir & hs)) * JXS_HREC;
                 gst(rec + k, (uint16_t)(k == HREC_TAIL ? (c ? c - 1 : 0xFFFFu) : c));
             }
         }
