@@ -21,7 +21,8 @@
 extern "C" {
 hipError_t jxs_launch_dwt(const JxsDwtJob*, const uint32_t*, int, uint32_t, uint32_t, uint32_t*, uint32_t, hipStream_t);
 hipError_t jxs_launch_rate(const JxsFrameDesc*, int, uint32_t, hipStream_t);
-hipError_t jxs_launch_pack(const JxsUnit*, uint32_t, hipStream_t);
+hipError_t jxs_launch_pack(const JxsUnit*, uint32_t, uint32_t, hipStream_t);
+hipError_t jxs_launch_join(const JxsUnit*, uint32_t, hipStream_t);
 }
 
 namespace jxs {
@@ -44,6 +45,7 @@ static uint64_t frame_work_bytes(const JxsGeom& g) {
     b += align256(g.n_i32 * 4);
     b += align256((uint64_t)g.rows * g.n_bl * JXS_HREC * 2);
     b += align256((uint64_t)g.rows * sizeof(JxsPrec));
+    b += align256(((uint64_t)g.lcod + 3) / 4 * 4 + 16); /* shadow words */
     return b;
 }
 
@@ -223,6 +225,8 @@ bool batch_place(const svt_jpeg_xs_amd_frame_spec_t* specs, uint32_t count, uint
         cur += align256((uint64_t)g.rows * g.n_bl * JXS_HREC * 2);
         F.prec = (JxsPrec*)cur;
         cur += align256((uint64_t)g.rows * sizeof(JxsPrec));
+        F.shadow = (uint32_t*)cur;
+        cur += align256(((uint64_t)g.lcod + 3) / 4 * 4 + 16);
         F.units = L.units + unit_base;
         F.out = specs[i].d_out;
         F.status = L.status + i;
@@ -291,7 +295,12 @@ hipError_t batch_launch(const BatchLaunch& L, hipStream_t st, hipEvent_t* ev) {
         return e;
     if (ev)
         (void)hipEventRecord(ev[3], st);
-    if ((e = jxs_launch_pack(L.units, L.total_units, st)) != hipSuccess)
+    /* launches of more than about one 4K frame's packing units take the
+     * shadow-word path of the packer and its join kernel */
+    const bool shadow = L.total_units > JXS_SHADOW_UNITS;
+    if ((e = jxs_launch_pack(L.units, L.total_units, shadow ? 1u : 0u, st)) != hipSuccess)
+        return e;
+    if (shadow && (e = jxs_launch_join(L.units, L.total_units, st)) != hipSuccess)
         return e;
     if (ev)
         (void)hipEventRecord(ev[SVT_JPEG_XS_AMD_STAGES], st);

@@ -24,6 +24,7 @@
 #define JXS_HDR_MAX 256
 #define JXS_NU 64  /* packing units (band lines) per precinct: one per lane of a wave */
 #define JXS_MAX_ROW 7680 /* widest component row the row kernels stage in LDS */
+#define JXS_SHADOW_UNITS 32768 /* packing units of a launch above which boundary words go through shadow words */
 
 /* Band-line histogram record (u16 units), written by the transform for
  * every band line of every precinct:
@@ -133,8 +134,8 @@ struct JxsUnit {
     uint32_t lo[3], len[3]; /* first bit inside *ob (0..31) and bits of each segment */
     uint32_t hlo, hlen;     /* the same for the packet header (hlen 0: none) */
     uint32_t gt, ms, raw, qpih;
-    uint32_t w, gw, sw;
-    uint32_t pad_[3];
+    uint32_t w, gw, sw, pad_;
+    int64_t sdelta; /* the shadow word of an output dword lies sdelta bytes after it */
 };
 
 struct JxsBandOut {
@@ -183,6 +184,7 @@ struct JxsFrameDesc {
     JxsUnit* units;  /* packing units: unit u of precinct row r at u_rank[u] * rows + r */
     uint8_t* out;    /* codestream */
     uint32_t* status;
+    uint32_t* shadow; /* one word per codestream dword: bits of words several packing units share */
     uint32_t rows, npk;
     uint32_t prec_base; /* first precinct of this frame in the launch */
     uint32_t slice_base;

@@ -117,6 +117,11 @@ __device__ __forceinline__ void lds_barrier() { __asm__ volatile("s_waitcnt lgkm
 __device__ __forceinline__ void gor(uint32_t* p, uint32_t v) {
     (void)__hip_atomic_fetch_or((GAS uint32_t*)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+/* The shadow word of codestream dword p (a frame's shadow array lies at a
+ * fixed byte distance from its codestream). */
+__device__ __forceinline__ uint32_t* shadow_of(uint32_t* p, int64_t sdelta) {
+    return (uint32_t*)((uint8_t*)p + sdelta);
+}
 
 __device__ __forceinline__ int32_t load_src(const void* src, uint32_t kind, size_t idx, uint32_t shift, uint32_t mask,
                                             int32_t offset) {
@@ -1097,7 +1102,19 @@ __device__ __forceinline__ void plan_precinct(const JxsFrameDesc& F, const JxsGe
                                 : ((uint64_t)raw << 55) | ((uint64_t)db << 35) | ((uint64_t)gbb << 15);
         }
         U.status = F.status;
+        U.sdelta = (int64_t)((uint8_t*)F.shadow - F.out);
         gst_words(UR, U); /* units are 144-byte records of a 256-byte aligned array */
+        /* the shadow words the packer ORs into: first and last dword of each
+         * segment, the packet header's dwords */
+#pragma unroll
+        for (uint32_t k = 0; k < 3; k++)
+            if (U.len[k]) {
+                gst(shadow_of(U.ob[k], U.sdelta), 0u);
+                gst(shadow_of(U.ob[k] + ((U.lo[k] + U.len[k] - 1) >> 5), U.sdelta), 0u);
+            }
+        if (U.hlen)
+            for (uint32_t k = 0; k <= ((U.hlo + U.hlen - 1) >> 5); k++)
+                gst(shadow_of(U.hob + k, U.sdelta), 0u);
     }
     uint8_t* o8 = F.out;
     /* packet bodies and padding start as zeros: the packing units OR their
@@ -1341,12 +1358,16 @@ struct WaveOut {
     uint32_t* ring;
     uint32_t mask;
     uint32_t* ob;
+    int64_t sd;  /* shadow words at ob + sd bytes (see flush) */
+    uint32_t sh; /* boundary words into the shadow (else into the codestream) */
     uint32_t lo, hi, pos, flushed, lane;
-    __device__ __forceinline__ void start(uint32_t* r, uint32_t m, uint32_t* o, uint32_t first, uint32_t len,
-                                          uint32_t ln) {
+    __device__ __forceinline__ void start(uint32_t* r, uint32_t m, uint32_t* o, int64_t sdelta, uint32_t shadow,
+                                          uint32_t first, uint32_t len, uint32_t ln) {
         ring = r;
         mask = m;
         ob = o;
+        sd = sdelta;
+        sh = shadow;
         lo = first;
         hi = first + len;
         pos = first;
@@ -1375,23 +1396,29 @@ struct WaveOut {
     }
     /* Writes ring words [from, to) out and clears them.  A word wholly inside
      * the segment is stored; the segment's first and last words, which it
-     * may share with its neighbours, are ORed into the zeroed output
-     * (device-scope atomics: the neighbours may run on other XCDs).  The
-     * stores are device-scope too, written through the XCD's L2: a
-     * neighbour's atomic on a line this wave left dirty in its L2 would
-     * first have to wait for that line's write-back, which made the
-     * write-back stores cost more than the coding itself. */
+     * may share with its neighbours, are ORed in with device-scope atomics
+     * (the neighbours may run on other XCDs).  A device-scope atomic on a
+     * line that another XCD's L2 holds dirty first waits for that line's
+     * write-back, so plain stores and atomics must not share lines.  Large
+     * launches OR the boundary words into the frame's shadow words, which a
+     * later kernel ORs into the codestream, and keep the stores in L2; a
+     * single frame's launch, where that kernel's latency would cost more,
+     * writes the stores through to memory and ORs into the codestream. */
     __device__ __forceinline__ void flush(uint32_t from, uint32_t to) {
         for (uint32_t w = from + lane; w < to; w += WAVE) {
             uint32_t* slot = &ring[w & mask];
             const uint32_t v = *slot;
             *slot = 0;
             const uint32_t b0 = 32 * w;
-            if (b0 >= lo && b0 + 32 <= hi)
-                __hip_atomic_store((GAS uint32_t*)(ob + w), __builtin_bswap32(v), __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_AGENT);
-            else if (v)
-                gor(ob + w, __builtin_bswap32(v));
+            if (b0 >= lo && b0 + 32 <= hi) {
+                if (sh)
+                    gst(ob + w, __builtin_bswap32(v));
+                else
+                    __hip_atomic_store((GAS uint32_t*)(ob + w), __builtin_bswap32(v), __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+            } else if (v) {
+                gor(sh ? shadow_of(ob + w, sd) : ob + w, __builtin_bswap32(v));
+            }
         }
     }
     __device__ __forceinline__ void maybe_flush() {
@@ -1631,25 +1658,25 @@ __device__ __forceinline__ void vm_wait_all() { __builtin_amdgcn_s_waitcnt(0x0F7
 
 /* The packet header (PackHeaders.c:98-127, sign size 0) of the first band
  * line of a packet, ORed into the zeroed output. */
-__device__ __forceinline__ void put_header(const JxsUnit& U, uint32_t lane) {
+__device__ __forceinline__ void put_header(const JxsUnit& U, uint32_t lane, uint32_t shadow) {
     if (U.hlen) {
         uint32_t w0, w1, w2;
         split_bits(U.hdr, U.hlen, U.hlo, w0, w1, w2);
         const uint32_t piece = lane == 0 ? w0 : lane == 1 ? w1 : w2;
         if (lane < 3 && piece)
-            gor(U.hob + lane, __builtin_bswap32(piece));
+            gor(shadow ? shadow_of(U.hob + lane, U.sdelta) : U.hob + lane, __builtin_bswap32(piece));
     }
 }
 
 /* The significance segment of a band line into ring A (written out later).
  * Returns whether its length differs from the plan's. */
 __device__ __forceinline__ bool code_sig(const JxsUnit& U, const PackIn& in, uint32_t lane, WaveOut& A,
-                                         uint32_t* ringa) {
+                                         uint32_t* ringa, uint32_t shadow) {
     if (!U.len[0]) {
         A.clear();
         return false;
     }
-    A.start(ringa, RINGA - 1, U.ob[0], U.lo[0], U.len[0], lane);
+    A.start(ringa, RINGA - 1, U.ob[0], U.sdelta, shadow, U.lo[0], U.len[0], lane);
     wave_sync();
     sig_step(A, in.sf, U.sw, U.gt);
     return A.pos != A.hi;
@@ -1664,7 +1691,7 @@ __device__ __forceinline__ bool code_sig(const JxsUnit& U, const PackIn& in, uin
  * plan's. */
 template <bool UNIFORM>
 __device__ __forceinline__ bool code_gd(const JxsUnit& U, const GdIn<IPL>& first, uint32_t lane, WaveOut& B,
-                                        WaveOut& C, uint32_t* ringb, uint32_t* ringc) {
+                                        WaveOut& C, uint32_t* ringb, uint32_t* ringc, uint32_t shadow) {
     if (!(U.len[1] | U.len[2])) {
         B.clear();
         C.clear();
@@ -1674,8 +1701,8 @@ __device__ __forceinline__ bool code_gd(const JxsUnit& U, const GdIn<IPL>& first
     const uint8_t* sl = U.sl;
     const uint16_t* cl = U.cl;
     const uint32_t gw = U.gw, gt = U.gt, ms = U.ms, raw = U.raw;
-    B.start(ringb, RINGB - 1, U.ob[1], U.lo[1], U.len[1], lane);
-    C.start(ringc, RINGC - 1, U.ob[2], U.lo[2], U.len[2], lane);
+    B.start(ringb, RINGB - 1, U.ob[1], U.sdelta, shadow, U.lo[1], U.len[1], lane);
+    C.start(ringc, RINGC - 1, U.ob[2], U.sdelta, shadow, U.lo[2], U.len[2], lane);
     wave_sync();
     GdIn<IPL> cur = first;
     for (uint32_t base = 0; base < gw; base += IPL * WAVE) {
@@ -1709,7 +1736,7 @@ __device__ __forceinline__ bool code_gd(const JxsUnit& U, const GdIn<IPL>& first
  * wait on each other. */
 #define PK_BLOCKS 1280 /* resident blocks: 256 CUs x 5 (5 waves per SIMD: <= 96 VGPRs, 24.3 KiB LDS per block) */
 __global__ void __launch_bounds__(64 * PK_WAVES) __attribute__((amdgpu_waves_per_eu(5)))
-k_pack(const JxsUnit* __restrict__ units, uint32_t total_units) {
+k_pack(const JxsUnit* __restrict__ units, uint32_t total_units, uint32_t shadow) {
     __shared__ uint32_t ringa[PK_WAVES][RINGA];
     __shared__ uint32_t ringb[PK_WAVES][RINGB];
     __shared__ uint32_t ringc[PK_WAVES][RINGC];
@@ -1736,13 +1763,13 @@ k_pack(const JxsUnit* __restrict__ units, uint32_t total_units) {
         B.finish();
         C.finish();
         wave_sync();
-        put_header(U, lane);
+        put_header(U, lane, shadow);
         PackIn inn;
         if (more)
             inn = pack_load(Un, lane);
-        bool bad = code_sig(U, in, lane, A, ringa[wid]);
-        bad |= U.qpih ? code_gd<true>(U, in.gd, lane, B, C, ringb[wid], ringc[wid])
-                      : code_gd<false>(U, in.gd, lane, B, C, ringb[wid], ringc[wid]);
+        bool bad = code_sig(U, in, lane, A, ringa[wid], shadow);
+        bad |= U.qpih ? code_gd<true>(U, in.gd, lane, B, C, ringb[wid], ringc[wid], shadow)
+                      : code_gd<false>(U, in.gd, lane, B, C, ringb[wid], ringc[wid], shadow);
         /* the plan's lengths are the segments' exact bit counts */
         if (bad && lane == 0)
             gor(U.status, 2u);
@@ -1755,6 +1782,44 @@ k_pack(const JxsUnit* __restrict__ units, uint32_t total_units) {
     A.finish();
     B.finish();
     C.finish();
+}
+
+/* The codestream dwords packing units share (a segment's first and last
+ * dwords, a packet header's dwords): their bits are in the frame's shadow
+ * words and are ORed into the codestream here, which already holds the
+ * rate kernel's bytes (headers) and zeros elsewhere.  Units sharing a dword
+ * each OR the same complete shadow word in: the result is the same whichever
+ * goes first.  JOIN_SLOTS threads per unit, one per candidate dword (slots
+ * 0..2: first dword of segment 0..2, 3..5: their last dwords, 6..8: the
+ * packet header's dwords), so every thread reads its shadow and codestream
+ * words together. */
+#define JOIN_SLOTS 16
+__global__ void __launch_bounds__(256) k_join(const JxsUnit* __restrict__ units, uint32_t total_units) {
+    const uint32_t gid = blockIdx.x * 256 + threadIdx.x;
+    const uint32_t u = gid / JOIN_SLOTS, k = gid % JOIN_SLOTS;
+    if (u >= total_units || k >= 9)
+        return;
+    const GAS JxsUnit& U = ((const GAS JxsUnit*)units)[u];
+    uint32_t* p = nullptr;
+    if (k < 6) {
+        const uint32_t sgm = k < 3 ? k : k - 3, lo = U.lo[sgm], len = U.len[sgm];
+        if (len) {
+            const uint32_t last = (lo + len - 1) >> 5;
+            if (k < 3 && (lo != 0 || len < 32))
+                p = U.ob[sgm];
+            else if (k >= 3 && last > 0 && ((lo + len) & 31) != 0)
+                p = U.ob[sgm] + last;
+        }
+    } else {
+        const uint32_t hlen = U.hlen;
+        if (hlen && k - 6 <= ((U.hlo + hlen - 1) >> 5))
+            p = U.hob + (k - 6);
+    }
+    if (!p)
+        return;
+    const uint32_t v = gld(shadow_of(p, U.sdelta)), o = gld(p);
+    if (v)
+        gst(p, o | v);
 }
 
 /* ------------------------------------------------------------------ */
@@ -1786,11 +1851,18 @@ hipError_t jxs_launch_rate(const JxsFrameDesc* frames, int nfr, uint32_t slices,
     return hipGetLastError();
 }
 
-hipError_t jxs_launch_pack(const JxsUnit* units, uint32_t n, hipStream_t st) {
+hipError_t jxs_launch_join(const JxsUnit* units, uint32_t n, hipStream_t st) {
+    if (!n)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_join, dim3((n * JOIN_SLOTS + 255) / 256), dim3(256), 0, st, units, n);
+    return hipGetLastError();
+}
+
+hipError_t jxs_launch_pack(const JxsUnit* units, uint32_t n, uint32_t shadow, hipStream_t st) {
     if (!n)
         return hipSuccess;
     const uint32_t blocks = min((n + PK_WAVES - 1) / PK_WAVES, (uint32_t)PK_BLOCKS);
-    hipLaunchKernelGGL(k_pack, dim3(blocks), dim3(64 * PK_WAVES), 0, st, units, n);
+    hipLaunchKernelGGL(k_pack, dim3(blocks), dim3(64 * PK_WAVES), 0, st, units, n, shadow);
     return hipGetLastError();
 }
 

@@ -84,3 +84,50 @@ def test_batch_equals_single_frame_plans():
         assert p.status()[0] == 0
         p.destroy()
     _check(items, "single")
+
+
+@pytest.mark.gpu
+def test_large_batch_shadow_words_match_oracle_and_single_frames():
+    """A launch with more packing units than one 4K frame has (24 frames of
+    320x180 4:2:2 10-bit, about 42,000 units) takes the packer's shadow-word
+    path and its join kernel; every codestream must equal the oracle's and the
+    one-frame plan's (which takes the direct path)."""
+    import torch
+    dev = torch.device("cuda:0")
+    S, O, P = load_synth(), load_oracle(), load_package()
+    W, H, D, F, bpp = 320, 180, 10, 3, 3
+    n = 24
+    print("batch of", n, "frames", W, H, flush=True)
+    enc = P.Encoder(W, H, D, F, bpp, 1, 2, 5, verbose=0)
+    prm = O.make_params(W, H, D, F, bpp, 1, 2, 5)
+    lc = enc.bytes_per_frame
+    refs, d_in = [], []
+    for k in range(n):
+        planes = S.make_frame(W, H, D, F, frame=500 + k)
+        e, ref = O.encode(prm, planes)
+        assert e == 0
+        refs.append(ref)
+        d_in.append([torch.from_numpy(p.view(np.int16)).to(dev) for p in planes])
+    stride = (lc + 3) & ~3
+    out_b = torch.zeros((n, stride), dtype=torch.uint8, device=dev)
+    out_1 = torch.zeros((n, stride), dtype=torch.uint8, device=dev)
+    spec = lambda k, out: (enc, [t.data_ptr() for t in d_in[k]], [t.shape[1] for t in d_in[k]], out[k].data_ptr())
+    batch = P.Plan([spec(k, out_b) for k in range(n)])
+    stream = torch.cuda.current_stream(dev)
+    for rep in range(2):
+        out_b.zero_()
+        batch.run(stream.cuda_stream, use_graph=rep == 1)
+        torch.cuda.synchronize()
+        assert batch.status()[0] == 0
+        host = out_b.cpu().numpy()
+        for k in range(n):
+            assert host[k, :lc].tobytes() == refs[k], "batch frame %d (replay %d) differs" % (k, rep)
+    for k in (0, n - 1):
+        plan = P.Plan([spec(k, out_1)])
+        plan.run(stream.cuda_stream)
+        torch.cuda.synchronize()
+        assert plan.status()[0] == 0
+        assert out_1[k, :lc].cpu().numpy().tobytes() == refs[k]
+        plan.destroy()
+    batch.destroy()
+    enc.close()
