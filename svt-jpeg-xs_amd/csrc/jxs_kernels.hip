@@ -214,6 +214,9 @@ struct Scale {
 
 typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef uint16_t u16x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ u16x2 as_u16x2(uint32_t x) { return __builtin_bit_cast(u16x2, x); }
+__device__ __forceinline__ uint32_t as_u32(u16x2 x) { return __builtin_bit_cast(uint32_t, x); }
 
 template <int KIND>
 __device__ __forceinline__ int32_t ld1(const void* row, uint32_t x, uint32_t step, const Scale& sc) {
@@ -310,6 +313,54 @@ __device__ __forceinline__ void vert53(const VertFlags& f, int32_t xm2, int32_t 
     else
         lf = xc + ((hprev + 1) >> 1);
     hf = hcur;
+}
+
+/* The vertical 5/3 step on unscaled samples.  With x' = x * 2^s - offset
+ * (s = 20 - depth >= 6) and D = 2 x(2i+1) - x(2i) - x(2i+2) (|D| < 2^15
+ * for depths up to 14), the high sample is D * 2^(s-1) exactly, and the
+ * low sample's rounded shifts drop no bits:
+ *   lf = (8 x(2i) + P + C) * 2^(s-3) - offset,
+ * with P = D(i-1), C = D(i) inside the plane; P = C = D(i) at the top row
+ * pair; P = C = D(i-1) when there is no high row; x(2i+2) := x(2i) when it
+ * lies past the end (the same values vert53 gives on scaled samples).  So
+ * the differences are formed on 16-bit sample pairs, two columns per
+ * instruction, and only the results are widened. */
+__device__ __forceinline__ int32_t lo16(uint32_t w) { return (int32_t)(w << 16) >> 16; }
+__device__ __forceinline__ int32_t hi16(uint32_t w) { return (int32_t)w >> 16; }
+template <int KIND, int NC>
+__device__ __forceinline__ uint32_t pair16(const RowRaw<KIND, NC>& r, int k) {
+    /* samples 2k and 2k + 1 of the row as two 16-bit halves */
+    if constexpr (KIND == JXS_SRC_U16) {
+        const u32x4 q = r.q[k / 4];
+        const int e = k % 4;
+        return e == 0 ? q.x : e == 1 ? q.y : e == 2 ? q.z : q.w;
+    } else {
+        const u32x4 q = r.q[k / 4];
+        const uint32_t v = (k % 4) < 2 ? q.x : q.y;
+        const uint32_t b = 2 * (k % 2);
+        return __builtin_amdgcn_perm(0u, v, 0x0C000C00u | ((b + 1) << 16) | b);
+    }
+}
+template <int KIND, int NC>
+__device__ __forceinline__ void vert_raw(const VertFlags& f, const RowRaw<KIND, NC> (&raw)[5], const Scale& sc,
+                                         int32_t* lo, int32_t* hi) {
+    const uint32_t m2 = sc.mask * 0x10001u, s = sc.shift;
+#pragma unroll
+    for (int k = 0; k < NC / 2; k++) {
+        const u16x2 a = as_u16x2(pair16<KIND, NC>(raw[0], k) & m2), b = as_u16x2(pair16<KIND, NC>(raw[1], k) & m2);
+        const u16x2 c = as_u16x2(pair16<KIND, NC>(raw[2], k) & m2), d = as_u16x2(pair16<KIND, NC>(raw[3], k) & m2);
+        const u16x2 e = as_u16x2(pair16<KIND, NC>(raw[4], k) & m2);
+        const u16x2 one = {1, 1};
+        const uint32_t dc = as_u32((d << one) - c - e), dp = as_u32((b << one) - a - c);
+        const uint32_t C = f.h ? dc : 0u;
+        const uint32_t P = f.prev ? dp : C;
+        const uint32_t C2 = f.h ? C : P;
+        const uint32_t cw = as_u32(c);
+        lo[2 * k] = (int32_t)((8 * (cw & 0xFFFFu) + lo16(P) + lo16(C2)) << (s - 3)) - sc.offset;
+        lo[2 * k + 1] = (int32_t)((8 * (cw >> 16) + hi16(P) + hi16(C2)) << (s - 3)) - sc.offset;
+        hi[2 * k] = lo16(C) * (1 << (s - 1));
+        hi[2 * k + 1] = hi16(C) * (1 << (s - 1));
+    }
 }
 
 /* Horizontal 5/3 lifting of one level on the CH / S samples v[k * S] a
@@ -605,7 +656,26 @@ __device__ __forceinline__ void dwt_slots(const CAS JxsDwtJob& J, uint32_t bi, D
     const VertFlags f = {ir > 0, ir < n / 2, 2 * ir + 2 < n};
     const bool has_hi = vert && f.h;
     int32_t lo[CH], hi[CH];
-    if (vert) {
+    if (KIND != JXS_SRC_I32 && vert && vec) {
+        /* input rows 2i-2 .. 2i+2 (clamped into the plane, the flags drop
+         * what lies outside; row 2i+2 := row 2i past the end), unscaled */
+        const int i2 = (int)(2 * ir), last = (int)n - 1;
+        const uint8_t* r[5];
+        r[0] = base + (size_t)max(i2 - 2, 0) * pitch;
+        r[1] = base + (size_t)max(i2 - 1, 0) * pitch;
+        r[2] = base + (size_t)i2 * pitch;
+        r[3] = base + (size_t)min(i2 + 1, last) * pitch;
+        r[4] = base + (size_t)(f.next ? i2 + 2 : i2) * pitch;
+        constexpr int NC = CH < 16 ? CH : 16;
+#pragma unroll
+        for (int h0 = 0; h0 < CH; h0 += NC) {
+            RowRaw<KIND, NC> raw[5];
+#pragma unroll
+            for (int j = 0; j < 5; j++)
+                raw[j] = load_vec<KIND, NC>(r[j], x0 + h0, w);
+            vert_raw<KIND, NC>(f, raw, sc, lo + h0, hi + h0);
+        }
+    } else if (vert) {
         /* input rows 2i-2 .. 2i+2, clamped into the plane (the flags drop
          * what lies outside), 16 columns at a time */
         const uint8_t* r[5];
@@ -1448,9 +1518,6 @@ __device__ __forceinline__ uint32_t quant_mag(uint32_t c, uint32_t gc, uint32_t 
     return gt == 0 ? m : u;
 }
 
-typedef uint16_t u16x2 __attribute__((ext_vector_type(2)));
-__device__ __forceinline__ u16x2 as_u16x2(uint32_t x) { return __builtin_bit_cast(u16x2, x); }
-__device__ __forceinline__ uint32_t as_u32(u16x2 x) { return __builtin_bit_cast(uint32_t, x); }
 
 /* Swaps the bits at positions p and p + d of x for every p in m. */
 __device__ __forceinline__ uint32_t delta_swap(uint32_t x, uint32_t m, uint32_t d) {
