@@ -148,6 +148,11 @@ def _spec(enc, d_in, out_ptr):
     return (enc, [t.data_ptr() for t in d_in], [t.shape[1] for t in d_in], out_ptr)
 
 
+def _tiles(tiles):
+    """Autotuner choices: 'width/levels(v) -> chunk' per transform job shape."""
+    return ["%d/%d%s->%d" % (t["width"], t["levels"], "v" if t["vertical"] else "", t["chunk"]) for t in tiles]
+
+
 def _time_plans(torch, stream, plans, batch, steps):
     """Milliseconds of `steps` passes of the one-frame plans and of the
     batch plan (graph replay), HIP events on the launch stream."""
@@ -180,6 +185,7 @@ def plain_vs_batched_1080p(P, S, torch, dev, steps, check, nframes=64):
     W1, H1, D1, F1, B1 = 1920, 1080, 8, 3, 5
     frames = S.make_frames(W1, H1, D1, F1, nframes)
     enc = P.Encoder(W1, H1, D1, F1, B1, 1, 2, 5, verbose=0)
+    tiles = enc.autotune(torch.cuda.current_stream(dev).cuda_stream)
     ostride = (enc.bytes_per_frame + 3) & ~3
     d_in = [_dev_planes(torch, dev, fr) for fr in frames]
     out_p = torch.zeros((nframes, ostride), dtype=torch.uint8, device=dev)
@@ -206,7 +212,7 @@ def plain_vs_batched_1080p(P, S, torch, dev, steps, check, nframes=64):
     enc.close()
     return {"workload": "1920x1080 YUV422 8-bit planar, bpp 5, decomp_v 2, decomp_h 5, %d frames" % nframes,
             "plain_fps": round(fp, 1), "batched_fps": round(fb, 1), "batched_over_plain": round(fb / fp, 3),
-            "bit_exact": bool(exact)}
+            "tiles": _tiles(tiles), "bit_exact": bool(exact)}
 
 
 def mixed_batch(P, S, torch, dev, steps, check, nframes=48):
@@ -220,10 +226,11 @@ def mixed_batch(P, S, torch, dev, steps, check, nframes=48):
         W1, H1, D1, F1, _ = S.MIXED_KINDS[k]
         jobs.append((W1, H1, D1, F1, i, S.MIXED_BASE))
     frames = S.make_frames_of(jobs)
-    encs = {}
+    encs, tiles = {}, []
     for k, v in sorted(set(draw)):
         W1, H1, D1, F1, B1 = S.MIXED_KINDS[k]
         encs[(k, v)] = P.Encoder(W1, H1, D1, F1, B1, 1, v, 5, verbose=0)
+        tiles += encs[(k, v)].autotune(torch.cuda.current_stream(dev).cuda_stream)
     d_in = [_dev_planes(torch, dev, fr) for fr in frames]
     lcods = [encs[kv].bytes_per_frame for kv in draw]
     ostride = (max(lcods) + 3) & ~3
@@ -264,7 +271,7 @@ def mixed_batch(P, S, torch, dev, steps, check, nframes=48):
         counts[key] = counts.get(key, 0) + 1
     return {"workload": "%d frames drawn by seed from 4 kinds (720p 420 8b bpp1, 1080p 444 12b bpp4, 2160p "
                         "planar RGB 10b bpp3, 1920x540 422 10b bpp5), decomp_v 1-2, decomp_h 5" % nframes,
-            "mix": counts, "batched_fps": round(fb, 1), "plain_fps": round(fp, 1),
+            "mix": counts, "tiles": _tiles(tiles), "batched_fps": round(fb, 1), "plain_fps": round(fp, 1),
             "batched_gpixel_per_s": round(pixels * steps / (ms_batch / 1e3) / 1e9, 3),
             "batched_hbm_frac": round(in_bytes * steps / (ms_batch / 1e3) / 1e9 / HBM_PEAK_GBS, 5),
             "ms_per_batch": round(ms_batch / steps, 3), "bit_exact": bool(exact)}
@@ -286,10 +293,13 @@ def serving(P, S, torch, dev, check, nstreams=32, seconds=3.0, budget_gib=16, ma
     for (W1, H1, D1, F1, fr, _), planes in zip(jobs, frames):
         k = fr // 1000
         pool.setdefault(k, []).append(_dev_planes(torch, dev, planes))
-    encs, outs = [], []
+    encs, outs, tuned = [], [], set()
     for k, v, _ in streams:
         W1, H1, D1, F1, B1 = S.MIXED_KINDS[k]
         e = P.Encoder(W1, H1, D1, F1, B1, 1, v, 5, verbose=0)
+        if (k, v) not in tuned:
+            e.autotune(torch.cuda.current_stream(dev).cuda_stream)
+            tuned.add((k, v))
         encs.append(e)
         outs.append(torch.zeros((RING, (e.bytes_per_frame + 3) & ~3), dtype=torch.uint8, device=dev))
     # each (stream kind, levels, pool frame) encoded alone, in a one-frame plan
@@ -432,6 +442,8 @@ def main():
     log("generating %d frames" % nf)
     frames = S.make_frames(W, H, DEPTH, FMT, nf)
     enc = P.Encoder(W, H, DEPTH, FMT, BPP, 1, DV, DH)
+    log("autotuning transform tiles")
+    tiles4k = enc.autotune(torch.cuda.current_stream(dev).cuda_stream)
     lcod = enc.bytes_per_frame
     in_bytes = sum(p.nbytes for p in frames[0])
     bytes_per_frame = in_bytes + lcod
@@ -570,7 +582,8 @@ def main():
         "data": "synthetic (seeded sinusoids + rectangles + Gaussian noise, per-frame seeds)",
         "config": {"workload": "3840x2160 YUV422 10-bit planar, bpp 3, decomp_v 2, decomp_h 5, %d frames, "
                                "one frame per launch, inputs resident in HBM" % nf,
-                   "frames_per_step": nf, "width": W, "height": H, "bit_depth": DEPTH, "bpp": BPP},
+                   "frames_per_step": nf, "width": W, "height": H, "bit_depth": DEPTH, "bpp": BPP,
+                   "transform_tiles": _tiles(tiles4k)},
         "gpixel_per_s": round(fps_plain * W * H / 1e9, 3),
         "p50_frame_latency_ms": round(pct(lat, 50), 4),
         "p99_frame_latency_ms": round(pct(lat, 99), 4),

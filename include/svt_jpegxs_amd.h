@@ -96,6 +96,27 @@ PREFIX_API SvtJxsErrorType_t svt_jpeg_xs_amd_decode_plan_run(svt_jpeg_xs_amd_dec
 PREFIX_API SvtJxsErrorType_t svt_jpeg_xs_amd_decode_plan_status(svt_jpeg_xs_amd_decode_plan_t* plan, uint32_t* errors);
 PREFIX_API void svt_jpeg_xs_amd_decode_plan_destroy(svt_jpeg_xs_amd_decode_plan_t* plan);
 
+/* ---------------- transform tile autotuner ----------------
+ * The transform kernels give each thread a chunk of 8, 16 or 32 consecutive
+ * columns of a row (8: 512-thread blocks; 16 and 32: 256-thread blocks).
+ * svt_jpeg_xs_amd_autotune times one frame of the encoder's configuration
+ * with every width the kernels support for each of its transform job shapes
+ * (input kind, vertical level, horizontal levels, width), on `stream`, and
+ * records the fastest in a per-process table that every plan, batch, server
+ * launch and encoder built afterwards uses; the encoder's own plain-path
+ * plans are rebuilt.  Output is identical for every width.  Call it with no
+ * frame of this encoder in flight.  Fills up to max_out entries of out (may be
+ * NULL) and returns the number of job shapes tuned, or -1 on error.  The
+ * reference has no counterpart (its tiles are fixed by its SIMD kernels). */
+typedef struct svt_jpeg_xs_amd_tile {
+    uint32_t width, levels, vertical; /* job shape: plane width, horizontal levels, vertical level or not */
+    uint32_t chunk;                   /* columns per thread chosen */
+    float frame_us;                   /* transform time of one frame with that choice */
+} svt_jpeg_xs_amd_tile_t;
+
+PREFIX_API int32_t svt_jpeg_xs_amd_autotune(svt_jpeg_xs_encoder_api_t* encoder, void* stream,
+                                            svt_jpeg_xs_amd_tile_t* out, uint32_t max_out);
+
 /* ---------------- serving loop ----------------
  * A request loop for many live streams whose frames arrive in HBM (from a
  * capture DMA, a decoder, or a host copy the caller made).  Frames are

@@ -100,6 +100,12 @@ class FrameSpec(ctypes.Structure):
                 ("strides", ctypes.c_uint32 * 4), ("d_out", ctypes.c_void_p)]
 
 
+class Tile(ctypes.Structure):
+    """svt_jpeg_xs_amd_tile_t (include/svt_jpegxs_amd.h)."""
+    _fields_ = [("width", ctypes.c_uint32), ("levels", ctypes.c_uint32), ("vertical", ctypes.c_uint32),
+                ("chunk", ctypes.c_uint32), ("frame_us", ctypes.c_float)]
+
+
 class ServerConfig(ctypes.Structure):
     """svt_jpeg_xs_amd_server_config_t (include/svt_jpegxs_amd.h)."""
     _fields_ = [("hbm_budget", ctypes.c_uint64), ("max_batch", ctypes.c_uint32), ("max_inflight", ctypes.c_uint32)]
@@ -144,6 +150,7 @@ EXPORTED_SYMBOLS = [
     "svt_jpeg_xs_amd_decode_plan_destroy", "svt_jpeg_xs_amd_plan_precinct_offsets", "svt_jpeg_xs_amd_plan_debug_copy",
     "svt_jpeg_xs_amd_server_create", "svt_jpeg_xs_amd_server_submit", "svt_jpeg_xs_amd_server_poll",
     "svt_jpeg_xs_amd_server_stats", "svt_jpeg_xs_amd_server_now_ns", "svt_jpeg_xs_amd_server_destroy",
+    "svt_jpeg_xs_amd_autotune",
 ]
 
 _lib = None
@@ -201,6 +208,8 @@ def lib():
         L.svt_jpeg_xs_amd_plan_footprint.argtypes = [ctypes.POINTER(FrameSpec), ctypes.c_uint32]
         L.svt_jpeg_xs_amd_plan_footprint.restype = ctypes.c_uint64
         L.svt_jpeg_xs_amd_version.restype = ctypes.c_char_p
+        L.svt_jpeg_xs_amd_autotune.argtypes = [E, ctypes.c_void_p, ctypes.POINTER(Tile), ctypes.c_uint32]
+        L.svt_jpeg_xs_amd_autotune.restype = ctypes.c_int32
         L.svt_jpeg_xs_amd_server_create.argtypes = [ctypes.POINTER(ServerConfig)]
         L.svt_jpeg_xs_amd_server_create.restype = ctypes.c_void_p
         L.svt_jpeg_xs_amd_server_submit.argtypes = [ctypes.c_void_p, ctypes.POINTER(FrameSpec), ctypes.c_uint64]
@@ -269,6 +278,16 @@ class Encoder:
     def close(self):
         if self.api.private_ptr:
             lib().svt_jpeg_xs_encoder_close(ctypes.byref(self.api))
+
+    def autotune(self, stream=0):
+        """Times the transform of one frame of this configuration with every
+        supported tile width and keeps the fastest per job shape
+        (svt_jpeg_xs_amd_autotune).  Returns the choices as dicts."""
+        out = (Tile * 16)()
+        n = lib().svt_jpeg_xs_amd_autotune(ctypes.byref(self.api), stream, out, 16)
+        if n < 0:
+            raise RuntimeError("svt_jpeg_xs_amd_autotune failed")
+        return [{f: getattr(out[i], f) for f, _ in Tile._fields_} for i in range(min(n, 16))]
 
     def __del__(self):
         try:

@@ -28,7 +28,7 @@ struct DevGeom {
     JxsGeom* d_geom = nullptr;
     uint32_t* d_slice_off = nullptr;
     uint32_t* d_slice_bud = nullptr;
-    uint32_t njobs[2][3] = {}; /* transform jobs of one frame per stage and chunk class (8 / 16 / 32 columns) */
+    uint32_t njobs[2] = {}; /* transform jobs of one frame per stage (their tile widths may change: see autotune) */
     uint64_t work = 0;         /* work bytes of one frame (inputs and codestream excluded) */
     void release();
     SvtJxsErrorType_t upload();
@@ -41,14 +41,13 @@ const DevGeom* encoder_geom(const svt_jpeg_xs_encoder_api_t* api);
  * batch_place lays out has this size). */
 struct BatchSizer {
     uint32_t n = 0, units = 0;
-    uint32_t njobs[2][3] = {};
+    uint32_t njobs[2] = {}; /* every chunk class's job list has room for all of a stage's jobs */
     uint64_t work = 0;
     void add(const DevGeom& g) {
         n++;
         units += g.host.g.rows * g.host.g.n_units;
         for (int s = 0; s < 2; s++)
-            for (int c = 0; c < 3; c++)
-                njobs[s][c] += g.njobs[s][c];
+            njobs[s] += g.njobs[s];
         work += g.work;
     }
     /* bytes of the descriptor part that is uploaded from the host */

@@ -69,27 +69,23 @@ SvtJxsErrorType_t DevGeom::upload() {
     CHECK_HIP(hipMemcpy(d_slice_off, host.slice_off.data(), n, hipMemcpyHostToDevice));
     CHECK_HIP(hipMemcpy(d_slice_bud, host.slice_bud.data(), n, hipMemcpyHostToDevice));
     work = frame_work_bytes(host.g);
-    /* how many transform jobs one frame has per stage and chunk class does
-     * not depend on where its buffers lie: count them on placeholder
-     * addresses (nothing is dereferenced) */
+    /* how many transform jobs one frame has per stage does not depend on
+     * where its buffers lie: count them on placeholder addresses (nothing is
+     * dereferenced) */
     static const uint8_t placeholder[256] = {};
     const void* planes[4] = {placeholder, placeholder, placeholder, placeholder};
     const uint32_t strides[4] = {host.g.w, host.g.w, host.g.w, host.g.w};
     FwdJobs jobs;
     fwd_jobs(host.g, planes, strides, nullptr, nullptr, nullptr, nullptr, nullptr, jobs);
     for (int s = 0; s < 2; s++)
-        for (const auto& j : jobs.dwt[s]) {
-            const int c = j.chunk == 8 ? 0 : j.chunk == 16 ? 1 : 2;
-            njobs[s][c]++;
-        }
+        njobs[s] = (uint32_t)jobs.dwt[s].size();
     return SvtJxsErrorNone;
 }
 
 uint64_t BatchSizer::upload_bytes() const {
     uint64_t b = align256((uint64_t)n * sizeof(JxsFrameDesc));
     for (int s = 0; s < 2; s++)
-        for (int c = 0; c < 3; c++)
-            b += align256((uint64_t)njobs[s][c] * sizeof(JxsDwtJob)) + align256((uint64_t)njobs[s][c] * 4);
+        b += 3 * (align256((uint64_t)njobs[s] * sizeof(JxsDwtJob)) + align256((uint64_t)njobs[s] * 4));
     return b;
 }
 
@@ -193,10 +189,10 @@ bool batch_place(const svt_jpeg_xs_amd_frame_spec_t* specs, uint32_t count, uint
         for (int c = 0; c < 3; c++) {
             job_off[s][c] = off;
             L.dwt[s][c] = (JxsDwtJob*)(base + off);
-            off += align256((uint64_t)sz.njobs[s][c] * sizeof(JxsDwtJob));
+            off += align256((uint64_t)sz.njobs[s] * sizeof(JxsDwtJob));
             base_off[s][c] = off;
             L.dbase[s][c] = (uint32_t*)(base + off);
-            off += align256((uint64_t)sz.njobs[s][c] * 4);
+            off += align256((uint64_t)sz.njobs[s] * 4);
         }
     L.status = (uint32_t*)(base + off);
     off += align256((uint64_t)count * 4);
@@ -250,7 +246,7 @@ bool batch_place(const svt_jpeg_xs_amd_frame_spec_t* specs, uint32_t count, uint
                 const int c = j.chunk == 8 ? 0 : j.chunk == 16 ? 1 : j.chunk == 32 ? 2 : -1;
                 /* outside what the transform kernel's layout covers */
                 if (c < 0 || j.lev_l < 1 || (1u << j.lev_l) > j.chunk || j.tpr > (j.chunk == 8 ? 512u : 256u) ||
-                    j.rpb == 0 || fill[s][c] >= sz.njobs[s][c])
+                    j.rpb == 0 || fill[s][c] >= sz.njobs[s])
                     return false;
                 j.base = blk[s][c];
                 memcpy(img.data() + job_off[s][c] + (uint64_t)fill[s][c] * sizeof(JxsDwtJob), &j, sizeof(j));
@@ -261,8 +257,6 @@ bool batch_place(const svt_jpeg_xs_amd_frame_spec_t* specs, uint32_t count, uint
     }
     for (int s = 0; s < 2; s++)
         for (int c = 0; c < 3; c++) {
-            if (fill[s][c] != sz.njobs[s][c])
-                return false;
             L.ndwt[s][c] = (int)fill[s][c];
             L.dblocks[s][c] = blk[s][c];
         }
@@ -374,6 +368,21 @@ static void encoder_destroy(EncoderPriv* e) {
 
 static const int kSlots = 3;
 
+/* The batch spec of one plain-path slot: its device input planes and
+ * codestream buffer. */
+static svt_jpeg_xs_amd_frame_spec_t slot_spec(const EncoderPriv* e, svt_jpeg_xs_encoder_api_t* api,
+                                              const EncoderPriv::Slot& s) {
+    svt_jpeg_xs_amd_frame_spec_t spec;
+    memset(&spec, 0, sizeof(spec));
+    spec.encoder = api;
+    for (uint32_t c = 0; c < e->nplanes; c++) {
+        spec.d_planes[c] = (const uint8_t*)s.d_in + e->plane_off[c];
+        spec.strides[c] = e->plane_stride[c];
+    }
+    spec.d_out = s.d_out;
+    return spec;
+}
+
 static SvtJxsErrorType_t encoder_create(svt_jpeg_xs_encoder_api_t* api) {
     EncParams p;
     p.width = api->source_width;
@@ -442,14 +451,7 @@ static SvtJxsErrorType_t encoder_create(svt_jpeg_xs_encoder_api_t* api) {
             encoder_destroy(e);
             return SvtJxsErrorInsufficientResources;
         }
-        svt_jpeg_xs_amd_frame_spec_t spec;
-        memset(&spec, 0, sizeof(spec));
-        spec.encoder = api;
-        for (uint32_t c = 0; c < e->nplanes; c++) {
-            spec.d_planes[c] = (const uint8_t*)s.d_in + e->plane_off[c];
-            spec.strides[c] = e->plane_stride[c];
-        }
-        spec.d_out = s.d_out;
+        const svt_jpeg_xs_amd_frame_spec_t spec = slot_spec(e, api, s);
         s.plan = plan_build(&spec, 1);
         if (!s.plan) {
             api->private_ptr = nullptr;
@@ -834,6 +836,148 @@ PREFIX_API SvtJxsErrorType_t svt_jpeg_xs_amd_plan_precinct_offsets(svt_jpeg_xs_a
     *d_offsets = &F.prec->offset;
     *stride = sizeof(JxsPrec) / sizeof(uint32_t);
     return SvtJxsErrorNone;
+}
+
+PREFIX_API int32_t svt_jpeg_xs_amd_autotune(svt_jpeg_xs_encoder_api_t* enc_api, void* stream,
+                                            svt_jpeg_xs_amd_tile_t* out, uint32_t max_out) {
+    EncoderPriv* e = priv_of(enc_api);
+    if (!e)
+        return -1;
+    {
+        std::lock_guard<std::mutex> lk(e->mu);
+        for (const auto& s : e->slots)
+            if (s.busy)
+                return -1; /* frames in flight */
+    }
+    hipStream_t st = (hipStream_t)stream;
+    /* the job shapes of one frame (placeholder addresses: only shapes matter) */
+    static const uint8_t placeholder[256] = {};
+    const void* planes[4] = {placeholder, placeholder, placeholder, placeholder};
+    const uint32_t strides[4] = {e->geom.host.g.w, e->geom.host.g.w, e->geom.host.g.w, e->geom.host.g.w};
+    FwdJobs jobs;
+    fwd_jobs(e->geom.host.g, planes, strides, nullptr, nullptr, nullptr, nullptr, nullptr, jobs);
+    std::vector<JxsDwtJob> shapes;
+    for (int sg = 0; sg < 2; sg++)
+        for (const auto& j : jobs.dwt[sg]) {
+            bool seen = false;
+            for (const auto& k : shapes)
+                seen |= k.kind == j.kind && k.step == j.step && k.vert == j.vert && k.lev_l == j.lev_l && k.w == j.w;
+            if (!seen)
+                shapes.push_back(j);
+        }
+    EncoderPriv::Slot& s0 = e->slots[0];
+    {
+        /* a pseudo-random frame within the bit depth (xorshift32): every
+         * candidate must give the built-in widths' codestream for it */
+        const JxsGeom& g = e->geom.host.g;
+        const uint32_t mask = (1u << g.depth) - 1, elem = g.depth <= 8 ? 1 : 2;
+        std::vector<uint8_t> host(e->in_bytes);
+        uint32_t x = 0x9E3779B9u;
+        for (uint64_t i = 0; i + elem <= host.size(); i += elem) {
+            x ^= x << 13;
+            x ^= x >> 17;
+            x ^= x << 5;
+            const uint32_t v = (x >> 7) & mask;
+            host[i] = (uint8_t)v;
+            if (elem == 2)
+                host[i + 1] = (uint8_t)(v >> 8);
+        }
+        if (hipMemcpy(s0.d_in, host.data(), host.size(), hipMemcpyHostToDevice) != hipSuccess)
+            return -1;
+    }
+    const uint32_t lcod = e->geom.host.g.lcod;
+    std::vector<uint8_t> ref_cs, cs(lcod);
+    const svt_jpeg_xs_amd_frame_spec_t spec = slot_spec(e, enc_api, s0);
+    hipEvent_t ev[SVT_JPEG_XS_AMD_STAGES + 1];
+    for (int i = 0; i <= SVT_JPEG_XS_AMD_STAGES; i++)
+        if (hipEventCreate(&ev[i]) != hipSuccess)
+            return -1;
+    /* the transform's time (both vertical levels) of one frame, best of a few
+     * runs, with the tile widths now in the table */
+    auto time_transform = [&]() -> float {
+        svt_jpeg_xs_amd_plan_t* p = plan_build(&spec, 1);
+        if (!p)
+            return -1.0f;
+        float best = -1.0f;
+        for (int rep = 0; rep < 6; rep++) {
+            float ms = 0.0f;
+            if (batch_launch(p->L, st, ev) != hipSuccess || hipEventSynchronize(ev[SVT_JPEG_XS_AMD_STAGES]) != hipSuccess ||
+                hipEventElapsedTime(&ms, ev[0], ev[2]) != hipSuccess) {
+                best = -1.0f;
+                break;
+            }
+            if (rep > 0 && (best < 0.0f || ms < best))
+                best = ms;
+        }
+        /* the codestream must not depend on the tile width */
+        if (best >= 0.0f && hipMemcpy(cs.data(), s0.d_out, lcod, hipMemcpyDeviceToHost) != hipSuccess)
+            best = -1.0f;
+        if (best >= 0.0f) {
+            if (ref_cs.empty())
+                ref_cs = cs;
+            else if (cs != ref_cs)
+                best = -1.0f;
+        }
+        plan_free(p);
+        return best;
+    };
+    /* the built-in rule's codestream first */
+    std::vector<std::pair<TileKey, uint32_t>> saved;
+    for (const auto& j : shapes) {
+        saved.push_back({tile_key(j), tile_get(tile_key(j))});
+        tile_set(tile_key(j), 0);
+    }
+    bool ok = time_transform() >= 0.0f;
+    for (const auto& kv : saved)
+        tile_set(kv.first, kv.second);
+    int32_t tuned = 0;
+    for (const auto& j : shapes) {
+        if (!ok)
+            break;
+        const TileKey key = tile_key(j);
+        const uint32_t before = tile_get(key);
+        uint32_t best_c = 0;
+        float best_t = 0.0f;
+        for (uint32_t c : {8u, 16u, 32u}) {
+            if (!tile_legal(j.lev_l, j.w, c))
+                continue;
+            tile_set(key, c);
+            const float t = time_transform();
+            if (t < 0.0f) {
+                ok = false;
+                break;
+            }
+            if (!best_c || t < best_t) {
+                best_c = c;
+                best_t = t;
+            }
+        }
+        tile_set(key, ok ? best_c : before);
+        if (!ok)
+            break;
+        if (out && (uint32_t)tuned < max_out) {
+            out[tuned].width = j.w;
+            out[tuned].levels = j.lev_l;
+            out[tuned].vertical = j.vert;
+            out[tuned].chunk = best_c;
+            out[tuned].frame_us = best_t * 1000.0f;
+        }
+        tuned++;
+    }
+    for (int i = 0; i <= SVT_JPEG_XS_AMD_STAGES; i++)
+        (void)hipEventDestroy(ev[i]);
+    /* the encoder's own plans take the chosen widths */
+    for (auto& s : e->slots) {
+        const svt_jpeg_xs_amd_frame_spec_t sp = slot_spec(e, enc_api, s);
+        svt_jpeg_xs_amd_plan_t* p = plan_build(&sp, 1);
+        if (!p) {
+            ok = false;
+            continue;
+        }
+        plan_free(s.plan);
+        s.plan = p;
+    }
+    return ok ? tuned : -1;
 }
 
 PREFIX_API const char* svt_jpeg_xs_amd_version(void) { return "svt-jpeg-xs-amd 0.1 (gfx950, API 0.10)"; }

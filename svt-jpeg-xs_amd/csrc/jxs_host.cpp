@@ -13,6 +13,10 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <map>
+#include <mutex>
+#include <tuple>
+
 namespace jxs {
 
 enum : int32_t {
@@ -754,6 +758,33 @@ static JxsBandOut band_out(const JxsGeom& g, uint32_t c, uint32_t b, uint16_t* c
  * per thread, else 256); rows are read 8 samples per load when
  * they are contiguous, every row starts 16-byte aligned and the width is a
  * multiple of 8. */
+static std::mutex g_tile_mu;
+static std::map<std::tuple<uint32_t, uint32_t, uint32_t, uint32_t, uint32_t>, uint32_t> g_tile;
+
+TileKey tile_key(const JxsDwtJob& j) { return TileKey{j.kind, j.step, j.vert, j.lev_l, j.w}; }
+
+bool tile_legal(uint32_t lev_l, uint32_t w, uint32_t chunk) {
+    if (lev_l < 1 || (chunk != 8 && chunk != 16 && chunk != 32) || (1u << lev_l) > chunk)
+        return false;
+    const uint32_t block = chunk == 8 ? 512u : 256u;
+    return ((w + chunk - 1) / chunk + 63) / 64 * 64 <= block;
+}
+
+void tile_set(const TileKey& k, uint32_t chunk) {
+    std::lock_guard<std::mutex> lk(g_tile_mu);
+    const auto key = std::make_tuple(k.kind, k.step, k.vert, k.lev_l, k.w);
+    if (chunk)
+        g_tile[key] = chunk;
+    else
+        g_tile.erase(key);
+}
+
+uint32_t tile_get(const TileKey& k) {
+    std::lock_guard<std::mutex> lk(g_tile_mu);
+    const auto it = g_tile.find(std::make_tuple(k.kind, k.step, k.vert, k.lev_l, k.w));
+    return it == g_tile.end() ? 0u : it->second;
+}
+
 static JxsDwtJob finish_job(JxsDwtJob j) {
     const uint32_t elem = j.kind == JXS_SRC_U8 ? 1u : j.kind == JXS_SRC_U16 ? 2u : 4u;
     j.vec = (j.step == 1 && (j.stride * elem) % 16 == 0 && ((uintptr_t)j.src & 15) == 0 && j.w % 8 == 0 && j.w >= 8)
@@ -765,6 +796,10 @@ static JxsDwtJob finish_job(JxsDwtJob j) {
         j.chunk = 8;
     else
         j.chunk = (j.lev_l >= 5 || (j.w + 15) / 16 > 256) ? 32u : 16u;
+    /* the autotuner's choice for this job shape, when it made one */
+    const uint32_t tuned = tile_get(tile_key(j));
+    if (tuned && tile_legal(j.lev_l, j.w, tuned))
+        j.chunk = tuned;
     const uint32_t block = j.chunk == 8 ? 512u : 256u;
     j.tpr = ((j.w + j.chunk - 1) / j.chunk + 63) / 64 * 64;
     j.rpb = block / j.tpr;

@@ -54,6 +54,18 @@ struct FwdJobs {
 void fwd_jobs(const JxsGeom& g, const void* const planes[4], const uint32_t strides[4], uint16_t* coeff, uint8_t* gcli,
               uint8_t* smax, uint16_t* hist, int32_t* scratch, FwdJobs& jobs);
 
+/* Transform tile widths (columns per thread: 8, 16 or 32) chosen by the
+ * autotuner, per job shape; a job with no choice recorded keeps the
+ * built-in rule.  The table is per process and thread-safe. */
+struct TileKey {
+    uint32_t kind, step, vert, lev_l, w;
+};
+TileKey tile_key(const JxsDwtJob& j);
+/* whether the transform kernel's layout covers this width for the job */
+bool tile_legal(uint32_t lev_l, uint32_t w, uint32_t chunk);
+void tile_set(const TileKey& k, uint32_t chunk); /* chunk 0: back to the rule */
+uint32_t tile_get(const TileKey& k);              /* 0: none recorded */
+
 /* Inverse-transform job lists (decoder). */
 struct InvVertJob;
 struct InvHorzJob;

@@ -131,3 +131,39 @@ def test_large_batch_shadow_words_match_oracle_and_single_frames():
         plan.destroy()
     batch.destroy()
     enc.close()
+
+
+@pytest.mark.gpu
+def test_autotuned_tiles_keep_codestreams():
+    """The transform tile autotuner picks a width per job shape (checking
+    that every candidate gives the same codestream on its own test frame);
+    encoding afterwards, plain and batched, still equals the oracle."""
+    import torch
+    dev = torch.device("cuda:0")
+    S, O, P = load_synth(), load_oracle(), load_package()
+    # shapes with one, two and five horizontal levels, 8- and 16-bit input
+    cfgs = [(256, 64, 10, 3, 3, 2, 5), (200, 200, 8, 4, 3, 1, 5), (128, 64, 11, 4, 3, 0, 3)]
+    items = []
+    for k, (W, H, D, F, bpp, V, Hh) in enumerate(cfgs):
+        enc = P.Encoder(W, H, D, F, bpp, 1, V, Hh, verbose=0)
+        tiles = enc.autotune()
+        print("autotune", (W, H, D, F, V, Hh), tiles, flush=True)
+        assert tiles and all(t["chunk"] in (8, 16, 32) and t["frame_us"] > 0 for t in tiles)
+        planes = S.make_frame(W, H, D, F, frame=700 + k)
+        e, ref = O.encode(O.make_params(W, H, D, F, bpp, 1, V, Hh), planes)
+        assert e == 0
+        rc, cs = enc.encode(planes)
+        assert rc == 0 and cs == ref, "plain path after autotune, config %d" % k
+        d_in = [torch.from_numpy(p.view(np.int16) if p.dtype == np.uint16 else p).to(dev) for p in planes]
+        d_out = torch.zeros((enc.bytes_per_frame + 3) & ~3, dtype=torch.uint8, device=dev)
+        items.append((enc, d_in, d_out, ref))
+    plan = P.Plan([(enc, [t.data_ptr() for t in d_in], [t.shape[1] for t in d_in], d_out.data_ptr())
+                   for enc, d_in, d_out, _ in items])
+    plan.run(torch.cuda.current_stream(dev).cuda_stream)
+    torch.cuda.synchronize()
+    assert plan.status()[0] == 0
+    for k, (enc, _, d_out, ref) in enumerate(items):
+        assert d_out.cpu().numpy().tobytes()[:len(ref)] == ref, "batch after autotune, config %d" % k
+    plan.destroy()
+    for enc, _, _, _ in items:
+        enc.close()
