@@ -277,6 +277,74 @@ def mixed_batch(P, S, torch, dev, steps, check, nframes=48):
             "ms_per_batch": round(ms_batch / steps, 3), "bit_exact": bool(exact)}
 
 
+def roundtrip_8k(P, S, torch, dev, steps, nframes=16, distinct=4):
+    """7680x4320 packed RGB 14-bit, bpp 2, decomp_v 1, decomp_h 5 (noise sigma
+    16): the frames encoded in one batch, then decoded in one batch (the
+    decoder takes the encoder's precinct tables).  Frame 0's codestream and
+    decoded planes are hashed against the reference encoder's and decoder's
+    (tests/golden); the other frames repeat `distinct` generated inputs."""
+    import hashlib
+    import numpy as np
+    W5, H5, D5, F5, B5 = 7680, 4320, 14, 21, 2
+    frames = S.make_frames(W5, H5, D5, F5, distinct, sigma=16.0)
+    enc = P.Encoder(W5, H5, D5, F5, B5, 1, 1, 5, verbose=0)
+    stream = torch.cuda.current_stream(dev)
+    sh = stream.cuda_stream
+    tiles = enc.autotune(sh)
+    lc = enc.bytes_per_frame
+    d_in = [torch.from_numpy(fr[0].view(np.int16)).to(dev) for fr in frames]
+    del frames
+    d_cs = torch.zeros((nframes, (lc + 3) & ~3), dtype=torch.uint8, device=dev)
+    plan = P.Plan([(enc, [d_in[i % distinct].data_ptr()], [3 * W5], d_cs[i].data_ptr()) for i in range(nframes)])
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    plan.run(sh, use_graph=True)
+    torch.cuda.synchronize()
+    ev0.record(stream)
+    for _ in range(steps):
+        plan.run(sh, use_graph=True)
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    ms_enc = ev0.elapsed_time(ev1)
+    enc_ok = plan.status()[0] == 0
+    cs0 = d_cs[0, :lc].cpu().numpy().tobytes()
+    dec = P.Decoder(cs0, verbose=0)
+    d_out = [[torch.zeros((H5, W5), dtype=torch.int16, device=dev) for _ in range(3)] for _ in range(nframes)]
+    specs = []
+    for i in range(nframes):
+        prec, pstride = plan.precinct_offsets(i)
+        specs.append((dec, d_cs[i].data_ptr(), lc, [t.data_ptr() for t in d_out[i]], [W5] * 3, prec, pstride))
+    dplan = P.DecodePlan(specs)
+    dplan.run(sh)
+    torch.cuda.synchronize()
+    ev0.record(stream)
+    for _ in range(steps):
+        dplan.run(sh)
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    ms_dec = ev0.elapsed_time(ev1)
+    dec_ok = dplan.status()[0] == 0
+    g = json.load(open(os.path.join(ROOT, "tests", "golden", "golden.json")))["full"]["c5_7680x4320_rgbpacked_14b_bpp2_v1"]
+    cs_ref = hashlib.sha256(cs0).hexdigest() == g["codestream_sha256"]
+    dec0 = b"".join(t.cpu().numpy().tobytes() for t in d_out[0])
+    dec_ref = hashlib.sha256(dec0).hexdigest() == g["decoded_sha256"]
+    per_frame = 3 * W5 * H5 * 2 + lc
+    fe = nframes * steps / (ms_enc / 1e3)
+    fd = nframes * steps / (ms_dec / 1e3)
+    dplan.destroy()
+    plan.destroy()
+    dec.close()
+    enc.close()
+    return {"workload": "7680x4320 packed RGB 14-bit, bpp 2, decomp_v 1, decomp_h 5, %d frames per batch, encode "
+                        "then decode, inputs resident in HBM" % nframes,
+            "tiles": _tiles(tiles), "encode_fps": round(fe, 1), "decode_fps": round(fd, 1),
+            "encode_gpixel_per_s": round(fe * W5 * H5 / 1e9, 2), "decode_gpixel_per_s": round(fd * W5 * H5 / 1e9, 2),
+            "encode_hbm_frac": round(fe * per_frame / 1e9 / HBM_PEAK_GBS, 4),
+            "decode_hbm_frac": round(fd * per_frame / 1e9 / HBM_PEAK_GBS, 4),
+            "frame_errors": int(not enc_ok) + int(not dec_ok),
+            "codestream_equals_reference": bool(cs_ref), "decoded_equals_reference": bool(dec_ref)}
+
+
 def serving(P, S, torch, dev, check, nstreams=32, seconds=3.0, budget_gib=16, max_batch=32, inflight=3):
     """The serving loop: nstreams live streams (kind and levels drawn from
     the mixed set, Poisson arrivals at 30-60 frames/s each) submitted in real
@@ -420,6 +488,7 @@ def main():
                     help="frames pushed through the C API with host buffers (PCIe-inclusive rate; 0: skip)")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving-loop run")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed batch and the 1080p comparison")
+    ap.add_argument("--no-roundtrip", action="store_true", help="skip the 8K encode + decode round trip")
     ap.add_argument("--serve-seconds", type=float, default=3.0)
     ap.add_argument("--streams", type=int, default=32)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -558,6 +627,10 @@ def main():
         hd = plain_vs_batched_1080p(P, S, torch, dev, args.steps, not args.no_check)
         log("mixed batch")
         mixed = mixed_batch(P, S, torch, dev, args.steps, not args.no_check)
+    rt = None
+    if not args.no_roundtrip:
+        log("8K round trip")
+        rt = roundtrip_8k(P, S, torch, dev, args.steps)
     if not args.no_serving:
         log("serving loop")
         srv = serving(P, S, torch, dev, not args.no_check, nstreams=args.streams, seconds=args.serve_seconds)
@@ -603,6 +676,7 @@ def main():
         "api_host_buffers": api,
         "plain_vs_batched_1080p": hd,
         "mixed_batch": mixed,
+        "roundtrip_8k": rt,
         "serving": srv,
         "wall_s_plain": round(wall_plain, 3),
     }

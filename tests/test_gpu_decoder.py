@@ -107,3 +107,38 @@ def test_device_roundtrip_packed_rgb14_v1():
             assert np.array_equal(got, ref[c]), "frame %d component %d differs" % (i, c)
     dplan.destroy()
     plan.destroy()
+
+
+# ---- decoded planes at full size against the reference decoder's (tests/golden) ----
+import hashlib  # noqa: E402
+import json  # noqa: E402
+import os  # noqa: E402
+
+from jxs_util import ROOT  # noqa: E402
+
+with open(os.path.join(ROOT, "tests", "golden", "golden.json")) as _f:
+    _FULL = json.load(_f)["full"]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", sorted(n for n in _FULL if _FULL[n].get("decoded_sha256")))
+def test_gpu_decode_full_size_equals_reference_decoder(name):
+    """First frame of every benchmark workload at full size (1080p .. 8K):
+    GPU encode (whose codestream hashes to the reference's), then the GPU
+    decoder through the C API; the decoded planes, concatenated as the
+    reference decoder writes them, hash to the reference decoder's output."""
+    p = _FULL[name]["params"]
+    print("full decode", name, flush=True)
+    P, S = load_package(), load_synth()
+    planes = S.make_frame(p["width"], p["height"], p["depth"], p["format"], frame=p["frame"], sigma=p["sigma"])
+    enc = P.Encoder(p["width"], p["height"], p["depth"], p["format"], p["bpp_num"], p["bpp_den"], p["decomp_v"],
+                    p["decomp_h"], p["quantization"], p["slice_height"], p["significance"], verbose=0)
+    rc, cs = enc.encode(planes)
+    enc.close()
+    assert rc == 0 and hashlib.sha256(cs).hexdigest() == _FULL[name]["codestream_sha256"]
+    dec = P.Decoder(cs, verbose=0)
+    rc, out = dec.decode(cs)
+    dec.close()
+    assert rc == 0
+    got = hashlib.sha256(b"".join(np.ascontiguousarray(q).tobytes() for q in out)).hexdigest()
+    assert got == _FULL[name]["decoded_sha256"]
