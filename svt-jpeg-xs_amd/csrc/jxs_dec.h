@@ -21,8 +21,17 @@ struct JxsDecFrame {
     uint32_t prec_base; /* first precinct of this frame in the launch */
     const uint32_t* prec_in; /* known precinct offsets (stride prec_in_stride), or NULL: walk the codestream */
     uint32_t prec_in_stride;
-    uint32_t pad_;
+    uint32_t pkt_stride; /* u32 words per precinct in pkt */
+    uint32_t* pkt;       /* per precinct: JXS_DPKT_HDR words of band modes and flags, then 5 words per packet */
 };
+
+/* Packet table of one precinct (written by k_dec_packets, read by
+ * k_dec_unpack): words 0..19 hold one u16 per (component, band):
+ * D-mode | GTLI << 8; word 20 is non-zero when the precinct is invalid; then
+ * per packet {significance bit, GCLI bit, data bit, data end bit, flags}
+ * with flags bit 0 = the packet exists in this precinct, bit 1 = raw mode. */
+#define JXS_DPKT_HDR 24
+#define JXS_DPKT_WORDS 5
 
 /* Inverse horizontal job: one block per row.  Inputs are the final low band
  * (int16 band or int32 plane) and the high bands of `levels` levels (finest

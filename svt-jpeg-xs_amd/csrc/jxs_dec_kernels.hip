@@ -4,8 +4,10 @@
  *   k_dec_scan    one lane per codestream: walks slice headers and the
  *                 precinct length chain (Lprc) to find every precinct; only
  *                 used when the caller has no precinct table.
- *   k_dec_unpack  one block per (packet, precinct): packet headers, the
- *                 significance flags, the unary bit-plane counts decoded in
+ *   k_dec_packets one lane per precinct: band D-modes and GTLIs, and the
+ *                 packet chain walked once into a table of sub-packet
+ *                 offsets.
+ *   k_dec_unpack  one block per (packet, precinct): the significance flags, the unary bit-plane counts decoded in
  *                 parallel (code j ends at the j-th zero bit: zero counts per
  *                 32-bit word + block prefix sum), then the data nibbles at
  *                 prefix-summed bit offsets, dequantised and turned into
@@ -47,6 +49,46 @@ __device__ __forceinline__ uint32_t cs_bits(const JxsDecFrame& F, uint64_t bp, u
     for (int k = 0; k < 5; k++)
         acc = (acc << 8) | cs_byte(F, b + k);
     return (uint32_t)((acc >> (40 - sh - n)) & (n == 32 ? 0xFFFFFFFFull : ((1ull << n) - 1)));
+}
+
+/* The codestream as aligned big-endian 32-bit words.  Bit positions given to
+ * cs_word/cs_bits64 count from the aligned word below F.cs (sh = 8 x the
+ * misalignment).  A word is loaded only if it holds at least one codestream
+ * byte, so the load never leaves the pages of the codestream; bytes of such a
+ * word outside the codestream are never among the bits a caller keeps (every
+ * caller bounds its bits by a sub-packet end inside the codestream). */
+struct CsView {
+    const uint32_t* w;
+    uint32_t nw, sh;
+};
+
+__device__ __forceinline__ CsView cs_view(const JxsDecFrame& F) {
+    const uintptr_t a = (uintptr_t)F.cs;
+    CsView v;
+    v.w = (const uint32_t*)(a & ~(uintptr_t)3);
+    v.sh = (uint32_t)(a & 3) * 8;
+    v.nw = (uint32_t)(((uint64_t)(a & 3) + F.cs_len + 3) >> 2);
+    return v;
+}
+
+__device__ __forceinline__ uint32_t cs_word(const CsView& v, uint32_t wi) { return wi < v.nw ? __builtin_bswap32(v.w[wi]) : 0u; }
+
+/* the 64 bits from aligned bit position abp on, MSB first */
+__device__ __forceinline__ uint64_t cs_bits64(const CsView& v, uint64_t abp) {
+    const uint32_t wi = (uint32_t)(abp >> 5), sh = (uint32_t)(abp & 31);
+    const uint64_t a = ((uint64_t)cs_word(v, wi) << 32) | cs_word(v, wi + 1);
+    const uint32_t c = cs_word(v, wi + 2);
+    return sh ? (a << sh) | (uint64_t)(c >> (32 - sh)) : a;
+}
+
+/* Every fourth bit of x, from bit 63 down (bits 63, 59, ..., 3), packed into a
+ * 16-bit value with bit 63 as its MSB. */
+__device__ __forceinline__ uint32_t gather4(uint64_t x) {
+    x = (x >> 3) & 0x1111111111111111ull;
+    x = (x | (x >> 3)) & 0x0303030303030303ull;
+    x = (x | (x >> 6)) & 0x000F000F000F000Full;
+    x = (x | (x >> 12)) & 0x000000FF000000FFull;
+    return (uint32_t)((x | (x >> 24)) & 0xFFFF);
 }
 
 __device__ __forceinline__ uint32_t dblock_scan(uint32_t v, uint32_t* tmp, uint32_t* total) {
@@ -135,6 +177,89 @@ struct UnpackCtx {
     uint32_t scan[8];
 };
 
+/* One lane per precinct of the launch: parses the precinct header's band
+ * D-modes, derives every band's GTLI from the precinct's quantisation and
+ * refinement (Precinct.c:9-29 restated), and walks the packet chain once
+ * (Packing.c:513-868: packet header, then significance, GCLI and data
+ * sub-packets), recording where each packet's sub-packets start and end. */
+__global__ void __launch_bounds__(64) k_dec_packets(const JxsDecFrame* __restrict__ frames, int nfr, uint32_t precincts) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= precincts)
+        return;
+    const int f = find_dec_frame(frames, nfr, t);
+    const JxsDecFrame& F = frames[f];
+    const JxsGeom& g = *F.g;
+    const uint32_t row = t - F.prec_base;
+    if (row >= g.rows)
+        return;
+    uint32_t* T = F.pkt + (size_t)row * F.pkt_stride;
+    const uint32_t type = row + 1 >= g.rows ? 1 : 0;
+    const uint64_t off = F.prec_off[row];
+    const uint32_t lprc = cs_bits(F, off * 8, 24);
+    const uint32_t q = cs_byte(F, off + 3), r = cs_byte(F, off + 4);
+    uint16_t mode[JXS_MAXC * JXS_NBC];
+    for (uint32_t k = 0; k < JXS_MAXC * JXS_NBC; k++)
+        mode[k] = 0;
+    uint64_t bit = off * 8 + 40;
+    bool bad = false;
+    for (uint32_t gi = 0; gi < g.nb_all; gi++) {
+        if (g.gb[gi] == 0xFF)
+            continue;
+        const uint32_t c = g.gc[gi], b = g.gb[gi];
+        const uint32_t d = cs_bits(F, bit, 2);
+        bit += 2;
+        mode[c * JXS_NBC + b] = (uint16_t)(d | dtrunc(g.c[c].b[b].gain, g.c[c].b[b].prio, q, r) << 8);
+        if (d & 1)
+            bad = true; /* vertical prediction: not produced by this library's encoder */
+    }
+    for (uint32_t k = 0; k < JXS_MAXC * JXS_NBC / 2; k++)
+        T[k] = mode[2 * k] | (uint32_t)mode[2 * k + 1] << 16;
+    const uint64_t pend = off + g.prec_hdr_bytes + (uint64_t)lprc;
+    uint64_t pos = off + g.prec_hdr_bytes;
+    for (uint32_t pp = 0; pp < g.npk; pp++) {
+        uint32_t* E = T + JXS_DPKT_HDR + JXS_DPKT_WORDS * pp;
+        bool any = false;
+        uint32_t sig_bits = 0;
+        for (uint32_t gi = g.pk_bs[pp]; gi < g.pk_be[pp]; gi++) {
+            const JxsBand& bd = g.c[g.gc[gi]].b[g.gb[gi]];
+            if (g.pk_line[pp] < bd.lines[type]) {
+                any = true;
+                if (mode[g.gc[gi] * JXS_NBC + g.gb[gi]] & 2)
+                    sig_bits += bd.sw;
+            }
+        }
+        if (!any || bad) {
+            E[4] = 0;
+            continue;
+        }
+        uint32_t raw, ld, lg;
+        if (g.short_hdr) {
+            raw = cs_bits(F, pos * 8, 1);
+            ld = cs_bits(F, pos * 8 + 1, 15);
+            lg = cs_bits(F, pos * 8 + 16, 13);
+        } else {
+            raw = cs_bits(F, pos * 8, 1);
+            ld = cs_bits(F, pos * 8 + 1, 20);
+            lg = cs_bits(F, pos * 8 + 21, 20);
+        }
+        const uint32_t sb = raw ? 0u : (sig_bits + 7) / 8;
+        const uint64_t s_off = pos + g.pkt_hdr_bytes, g_off = s_off + sb, d_off = g_off + lg, nxt = d_off + ld;
+        E[0] = (uint32_t)(s_off * 8);
+        E[1] = (uint32_t)(g_off * 8);
+        E[2] = (uint32_t)(d_off * 8);
+        E[3] = (uint32_t)(nxt * 8);
+        E[4] = 1u | raw << 1;
+        pos = nxt;
+        if (pos > pend)
+            bad = true; /* later packets are not decoded */
+    }
+    if (pend > F.cs_len || pos > pend || (uint64_t)pend * 8 >= 0xFFFFFFFFull)
+        bad = true;
+    if (bad)
+        atomicOr(F.status, 1u);
+    T[JXS_MAXC * JXS_NBC / 2] = bad ? 1u : 0u;
+}
+
 __global__ void __launch_bounds__(256) k_dec_unpack(const JxsDecFrame* __restrict__ frames, int nfr) {
     __shared__ UnpackCtx C;
     const int f = find_dec_frame(frames, nfr, blockIdx.y);
@@ -146,68 +271,24 @@ __global__ void __launch_bounds__(256) k_dec_unpack(const JxsDecFrame* __restric
         return;
     const uint32_t type = row + 1 >= g.rows ? 1 : 0;
     const uint32_t l = g.pk_line[p];
-    if (threadIdx.x == 0) {
-        C.ok = 0;
-        const uint64_t off = F.prec_off[row];
-        const uint32_t lprc = cs_bits(F, off * 8, 24);
-        const uint32_t q = cs_byte(F, off + 3), r = cs_byte(F, off + 4);
-        uint64_t bit = off * 8 + 40;
-        bool bad = false;
-        for (uint32_t gi = 0; gi < g.nb_all; gi++) {
-            if (g.gb[gi] == 0xFF)
-                continue;
-            const uint32_t c = g.gc[gi], b = g.gb[gi];
-            const uint32_t d = cs_bits(F, bit, 2);
-            bit += 2;
-            C.dmode[c][b] = (uint8_t)d;
-            C.gtli[c][b] = (uint8_t)dtrunc(g.c[c].b[b].gain, g.c[c].b[b].prio, q, r);
-            if (d & 1)
-                bad = true; /* vertical prediction: not produced by this library's encoder */
+    const CsView V = cs_view(F);
+    {
+        const uint32_t* T = F.pkt + (size_t)row * F.pkt_stride;
+        const uint32_t* E = T + JXS_DPKT_HDR + JXS_DPKT_WORDS * p;
+        if (threadIdx.x < JXS_MAXC * JXS_NBC) {
+            const uint32_t w = T[threadIdx.x >> 1] >> (16 * (threadIdx.x & 1));
+            C.dmode[threadIdx.x / JXS_NBC][threadIdx.x % JXS_NBC] = (uint8_t)(w & 0xFF);
+            C.gtli[threadIdx.x / JXS_NBC][threadIdx.x % JXS_NBC] = (uint8_t)((w >> 8) & 0xFF);
+        } else if (threadIdx.x == 64) {
+            const uint32_t fl = E[4];
+            C.ok = (fl & 1u) && T[JXS_MAXC * JXS_NBC / 2] == 0 ? 1u : 0u;
+            C.raw = (fl >> 1) & 1u;
+            C.s_bit = E[0];
+            C.g_bit = E[1];
+            C.g_end = E[2];
+            C.d_bit = E[2];
+            C.d_end = E[3];
         }
-        const uint64_t pend = off + g.prec_hdr_bytes + (uint64_t)lprc;
-        uint64_t pos = off + g.prec_hdr_bytes;
-        bool exists = false;
-        for (uint32_t pp = 0; pp <= p && !bad; pp++) {
-            bool any = false;
-            uint32_t sig_bits = 0;
-            for (uint32_t gi = g.pk_bs[pp]; gi < g.pk_be[pp]; gi++) {
-                const JxsBand& bd = g.c[g.gc[gi]].b[g.gb[gi]];
-                if (g.pk_line[pp] < bd.lines[type]) {
-                    any = true;
-                    if (C.dmode[g.gc[gi]][g.gb[gi]] & 2)
-                        sig_bits += bd.sw;
-                }
-            }
-            if (!any)
-                continue;
-            uint32_t raw, ld, lg;
-            if (g.short_hdr) {
-                raw = cs_bits(F, pos * 8, 1);
-                ld = cs_bits(F, pos * 8 + 1, 15);
-                lg = cs_bits(F, pos * 8 + 16, 13);
-            } else {
-                raw = cs_bits(F, pos * 8, 1);
-                ld = cs_bits(F, pos * 8 + 1, 20);
-                lg = cs_bits(F, pos * 8 + 21, 20);
-            }
-            const uint32_t sb = raw ? 0u : (sig_bits + 7) / 8;
-            const uint64_t s_off = pos + g.pkt_hdr_bytes, g_off = s_off + sb, d_off = g_off + lg, nxt = d_off + ld;
-            if (pp == p) {
-                exists = true;
-                C.raw = raw;
-                C.s_bit = (uint32_t)(s_off * 8);
-                C.g_bit = (uint32_t)(g_off * 8);
-                C.g_end = (uint32_t)(d_off * 8);
-                C.d_bit = (uint32_t)(d_off * 8);
-                C.d_end = (uint32_t)(nxt * 8);
-            }
-            pos = nxt;
-        }
-        if (pend > F.cs_len || pos > pend || (uint64_t)pend * 8 >= 0xFFFFFFFFull)
-            bad = true;
-        if (bad)
-            atomicOr(F.status, 1u);
-        C.ok = (!bad && exists) ? 1u : 0u;
     }
     __syncthreads();
     if (!C.ok)
@@ -238,7 +319,8 @@ __global__ void __launch_bounds__(256) k_dec_unpack(const JxsDecFrame* __restric
                     const uint32_t s = base + threadIdx.x;
                     uint32_t cnt = 0;
                     if (s < sw) {
-                        const uint32_t fl = cs_bits(F, (uint64_t)s0 + s, 1);
+                        const uint32_t ab = s0 + s + V.sh;
+                        const uint32_t fl = (cs_word(V, ab >> 5) >> (31 - (ab & 31))) & 1u;
                         C.sflag[s] = (uint8_t)fl;
                         const uint32_t size = min(8u, gw - 8 * s);
                         cnt = fl ? 0u : size;
@@ -255,7 +337,7 @@ __global__ void __launch_bounds__(256) k_dec_unpack(const JxsDecFrame* __restric
                     C.s_bit = s0 + sw;
             }
             /* unary codes: the j-th zero bit from the start ends code j */
-            const uint32_t g0 = C.g_bit, gend = C.g_end;
+            const uint32_t g0 = C.g_bit + V.sh, gend = C.g_end + V.sh; /* aligned bit positions */
             uint32_t found = 0;
             for (uint32_t w = g0 >> 5; found < ncodes; w += blockDim.x) {
                 if ((uint64_t)w * 32 >= gend) {
@@ -265,8 +347,7 @@ __global__ void __launch_bounds__(256) k_dec_unpack(const JxsDecFrame* __restric
                 const uint32_t wi = w + threadIdx.x;
                 uint32_t word = 0xFFFFFFFFu;
                 if ((uint64_t)wi * 32 < gend) {
-                    word = (cs_byte(F, 4ull * wi) << 24) | (cs_byte(F, 4ull * wi + 1) << 16) | (cs_byte(F, 4ull * wi + 2) << 8) |
-                        cs_byte(F, 4ull * wi + 3);
+                    word = cs_word(V, wi);
                     const uint32_t lo = wi * 32;
                     if (lo < g0)
                         word |= ~(0xFFFFFFFFu >> (g0 - lo)); /* bits before the line start */
@@ -309,7 +390,7 @@ __global__ void __launch_bounds__(256) k_dec_unpack(const JxsDecFrame* __restric
                 }
                 __syncthreads();
                 if (threadIdx.x == 0)
-                    C.g_bit = ncodes ? C.zpos[ncodes - 1] + 1 : g0;
+                    C.g_bit = (ncodes ? C.zpos[ncodes - 1] + 1 : g0) - V.sh;
             }
         }
         if (err) {
@@ -338,21 +419,18 @@ __global__ void __launch_bounds__(256) k_dec_unpack(const JxsDecFrame* __restric
                     if (bp + nb > C.d_end) {
                         atomicOr(F.status, 1u);
                     } else {
-                        sg = cs_bits(F, bp, 4);
-                        uint32_t k = 0;
-                        for (int bit = (int)gc - 1; bit >= (int)gt; bit--, k++) {
-                            const uint32_t nib = cs_bits(F, bp + 4 + 4 * k, 4);
+                        /* nb <= 64 bits: the sign nibble, then plane gc-1 .. gt, one bit per coefficient each */
+                        const uint64_t v = cs_bits64(V, bp + V.sh);
+                        sg = (uint32_t)(v >> 60);
+                        const uint32_t np = gc - gt;
 #pragma unroll
-                            for (int t4 = 0; t4 < 4; t4++)
-                                mag[t4] |= ((nib >> (3 - t4)) & 1u) << bit;
-                        }
+                        for (int t4 = 0; t4 < 4; t4++)
+                            mag[t4] = (gather4(v << (4 + t4)) >> (16 - np)) << gt;
                     }
                 }
+                uint32_t val[4];
 #pragma unroll
                 for (uint32_t t4 = 0; t4 < 4; t4++) {
-                    const uint32_t ci = 4 * i + t4;
-                    if (ci >= bd.w)
-                        continue;
                     uint32_t m = mag[t4];
                     if (gt > 0 && m) {
                         if (g.qpih == 0) {
@@ -366,7 +444,14 @@ __global__ void __launch_bounds__(256) k_dec_unpack(const JxsDecFrame* __restric
                         }
                     }
                     const bool neg = (sg >> (3 - t4)) & 1u;
-                    crow[ci] = (int16_t)(neg ? -(int32_t)(m & 0x7FFF) : (int32_t)(m & 0x7FFF));
+                    val[t4] = (uint16_t)(neg ? -(int32_t)(m & 0x7FFF) : (int32_t)(m & 0x7FFF));
+                }
+                int16_t* cp = crow + 4 * i;
+                if (4 * i + 4 <= bd.w && ((uintptr_t)cp & 7) == 0) {
+                    *(uint2*)cp = make_uint2(val[0] | val[1] << 16, val[2] | val[3] << 16);
+                } else {
+                    for (uint32_t t4 = 0; t4 < 4 && 4 * i + t4 < bd.w; t4++)
+                        cp[t4] = (int16_t)val[t4];
                 }
             }
             pos += t;
@@ -509,6 +594,13 @@ extern "C" {
 
 hipError_t jxs_launch_dec_scan(const JxsDecFrame* frames, int nfr, hipStream_t st) {
     hipLaunchKernelGGL(k_dec_scan, dim3(nfr), dim3(256), 0, st, frames, nfr);
+    return hipGetLastError();
+}
+
+hipError_t jxs_launch_dec_packets(const JxsDecFrame* frames, int nfr, uint32_t precincts, hipStream_t st) {
+    if (!precincts)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_dec_packets, dim3((precincts + 63) / 64), dim3(64), 0, st, frames, nfr, precincts);
     return hipGetLastError();
 }
 

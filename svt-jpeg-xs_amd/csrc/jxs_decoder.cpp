@@ -21,6 +21,7 @@
 
 extern "C" {
 hipError_t jxs_launch_dec_scan(const JxsDecFrame*, int, hipStream_t);
+hipError_t jxs_launch_dec_packets(const JxsDecFrame*, int, uint32_t, hipStream_t);
 hipError_t jxs_launch_dec_unpack(const JxsDecFrame*, int, uint32_t, uint32_t, hipStream_t);
 hipError_t jxs_launch_inv_rows(const JxsInvRowJob*, const uint32_t*, int, uint32_t, hipStream_t);
 hipError_t jxs_launch_inv_vert(const JxsInvVertJob*, const uint32_t*, int, uint32_t, hipStream_t);
@@ -194,8 +195,11 @@ struct DecGeom {
     JxsGeom* d_geom = nullptr;
 };
 
+static uint32_t dec_pkt_stride(const JxsGeom& g) { return JXS_DPKT_HDR + JXS_DPKT_WORDS * g.npk; }
+
 static uint64_t dec_work_bytes(const JxsGeom& g) {
-    return al256(g.n_coeff * 2) + al256(g.n_i32 * 4) + al256((uint64_t)g.rows * 4);
+    return al256(g.n_coeff * 2) + al256(g.n_i32 * 4) + al256((uint64_t)g.rows * 4) +
+        al256((uint64_t)g.rows * dec_pkt_stride(g) * 4);
 }
 
 }  // namespace jxs
@@ -310,6 +314,9 @@ static svt_jpeg_xs_amd_decode_plan_t* dplan_build(const std::vector<DecSpec>& sp
         cur += al256(g.n_i32 * 4);
         F.prec_off = (uint32_t*)cur;
         cur += al256((uint64_t)g.rows * 4);
+        F.pkt_stride = dec_pkt_stride(g);
+        F.pkt = (uint32_t*)cur;
+        cur += al256((uint64_t)g.rows * F.pkt_stride * 4);
         F.status = p->d_status + i;
         F.prec_base = base;
         F.prec_in = s.prec_in;
@@ -361,6 +368,8 @@ static hipError_t dplan_launch(svt_jpeg_xs_amd_decode_plan_t* p, hipStream_t st)
     if ((e = jxs_launch_zero(p->d_status, p->nfr, st)) != hipSuccess)
         return e;
     if ((e = jxs_launch_dec_scan(p->d_frames, (int)p->nfr, st)) != hipSuccess)
+        return e;
+    if ((e = jxs_launch_dec_packets(p->d_frames, (int)p->nfr, p->total_prec, st)) != hipSuccess)
         return e;
     if ((e = jxs_launch_dec_unpack(p->d_frames, (int)p->nfr, p->max_npk, p->total_prec, st)) != hipSuccess)
         return e;
