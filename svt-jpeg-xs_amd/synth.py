@@ -91,6 +91,18 @@ def _frame_job(args):
     return make_frame(*args)
 
 
+def _pool_map(mp, n, jobs):
+    """pool.map with workers that end by themselves (close + join): a pool
+    left by its context manager terminates its workers with SIGTERM, which a
+    profiler's signal handler in the forked workers may not survive."""
+    pool = mp.get_context("fork").Pool(n)
+    try:
+        return pool.map(_frame_job, jobs)
+    finally:
+        pool.close()
+        pool.join()
+
+
 def make_frames(width, height, depth, fmt, count, base=BASE_SEED, sigma=4.0, workers=None):
     """`count` frames with per-frame seeds, generated by a process pool."""
     import multiprocessing as mp
@@ -99,8 +111,7 @@ def make_frames(width, height, depth, fmt, count, base=BASE_SEED, sigma=4.0, wor
     n = workers or min(16, os.cpu_count() or 1, count)
     if n <= 1:
         return [_frame_job(j) for j in jobs]
-    with mp.get_context("fork").Pool(n) as pool:
-        return pool.map(_frame_job, jobs)
+    return _pool_map(mp, n, jobs)
 
 
 def make_frame(width, height, depth, fmt, frame=0, base=BASE_SEED, sigma=4.0):
@@ -167,5 +178,4 @@ def make_frames_of(jobs, workers=None):
     n = workers or min(16, os.cpu_count() or 1, len(jobs))
     if n <= 1:
         return [_frame_job(j) for j in jobs]
-    with mp.get_context("fork").Pool(n) as pool:
-        return pool.map(_frame_job, jobs)
+    return _pool_map(mp, n, jobs)
