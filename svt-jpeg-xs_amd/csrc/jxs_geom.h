@@ -165,7 +165,7 @@ struct JxsDwtJob {
     uint32_t nblk;       /* blocks of the job */
     uint32_t lf_is_band; /* low rows' low-pass output: band, or int32 plane */
     uint32_t lf_stride;
-    uint32_t vec; /* rows readable 16 bytes per load (step 1, plane and row pitch 16-byte aligned) */
+    uint32_t vec; /* 1: rows readable 16 bytes per load (step 1, plane and row pitch 16-byte aligned); 2 + c: packed 16-bit rows so readable, component c */
     int32_t* lf_i32;
     JxsBandOut lf, hf[5]; /* low rows: final low-pass and high-pass of each level */
     JxsBandOut lh, hh;    /* high rows: low-pass and high-pass of their level */

@@ -785,11 +785,18 @@ uint32_t tile_get(const TileKey& k) {
     return it == g_tile.end() ? 0u : it->second;
 }
 
-static JxsDwtJob finish_job(JxsDwtJob j) {
+/* comp: the component of a packed (step 3) job, whose src is the row start + comp samples */
+static JxsDwtJob finish_job(JxsDwtJob j, uint32_t comp = 0) {
     const uint32_t elem = j.kind == JXS_SRC_U8 ? 1u : j.kind == JXS_SRC_U16 ? 2u : 4u;
-    j.vec = (j.step == 1 && (j.stride * elem) % 16 == 0 && ((uintptr_t)j.src & 15) == 0 && j.w % 8 == 0 && j.w >= 8)
-                ? 1u
-                : 0u;
+    /* vec: 1 = contiguous rows read 16 bytes per load; 2 + c = packed 16-bit
+     * rows read 16 bytes per load, component c picked in registers */
+    const bool rows16 = (j.stride * elem) % 16 == 0 && j.w % 8 == 0 && j.w >= 8;
+    if (j.step == 1)
+        j.vec = rows16 && ((uintptr_t)j.src & 15) == 0 ? 1u : 0u;
+    else if (j.step == 3 && j.kind == JXS_SRC_U16 && comp < 3)
+        j.vec = rows16 && (((uintptr_t)j.src - comp * elem) & 15) == 0 ? 2u + comp : 0u;
+    else
+        j.vec = 0;
     /* one horizontal level: 8 columns per thread in 512-thread blocks (half
      * the registers, twice the waves in flight) */
     if (j.lev_l == 1 && (j.w + 7) / 8 <= 512)
@@ -834,7 +841,7 @@ void fwd_jobs(const JxsGeom& g, const void* const planes[4], const uint32_t stri
                 j.hf[k] = band(nb - 1 - k);
             j.lf_is_band = 1;
             j.lf = band(0);
-            jobs.dwt[0].push_back(finish_job(j));
+            jobs.dwt[0].push_back(finish_job(j, g.packed ? c : 0));
             continue;
         }
         const uint32_t w1 = (cp.w + 1) / 2, h1 = (cp.h + 1) / 2;
@@ -852,7 +859,7 @@ void fwd_jobs(const JxsGeom& g, const void* const planes[4], const uint32_t stri
                 j.hf[k] = band(cp.dh - k);
             j.lf_is_band = 1;
             j.lf = band(0);
-            jobs.dwt[0].push_back(finish_job(j));
+            jobs.dwt[0].push_back(finish_job(j, g.packed ? c : 0));
             continue;
         }
         int32_t* B = scr;
@@ -861,7 +868,7 @@ void fwd_jobs(const JxsGeom& g, const void* const planes[4], const uint32_t stri
         j.lf_is_band = 0;
         j.lf_i32 = B;
         j.lf_stride = w1;
-        jobs.dwt[0].push_back(finish_job(j));
+        jobs.dwt[0].push_back(finish_job(j, g.packed ? c : 0));
         /* second vertical level on the w1 x h1 low-pass plane */
         JxsDwtJob k2;
         memset(&k2, 0, sizeof(k2));

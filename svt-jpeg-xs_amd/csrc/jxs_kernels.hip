@@ -259,6 +259,46 @@ __device__ __forceinline__ RowRaw<KIND, NC> load_vec(const void* row, uint32_t x
     }
     return r;
 }
+/* NC samples of component `comp` (0..2) of a packed 16-bit row (three
+ * interleaved components; row and plane 16-byte aligned, width a multiple of
+ * 8: checked by the host), as the RowRaw of a planar row: eight samples take
+ * the 24 interleaved samples [3x, 3x + 24) in three 16-byte loads, and every
+ * sample pair of the component is picked from two of their words.  `row`
+ * points at the component's first sample (the row start + comp). */
+template <int NC>
+__device__ __forceinline__ RowRaw<JXS_SRC_U16, NC> load_vec3(const void* row, uint32_t x0, uint32_t w, uint32_t comp) {
+    RowRaw<JXS_SRC_U16, NC> r;
+    const uint16_t* base = (const uint16_t*)row - comp;
+    /* pair p of eight samples = samples 2p, 2p + 1 = interleaved 6p + comp, 6p + 3 + comp:
+     * low half from word 3p (comp 0, 1) or 3p + 1 (comp 2), high half from word 3p + 1 (comp 0) or 3p + 2 */
+    const uint32_t sel = comp == 1 ? 0x05040302u : 0x07060100u;
+#pragma unroll
+    for (int k = 0; k < NC / 8; k++) {
+        const uint16_t* p = base + 3 * (size_t)min(x0 + 8 * k, w - 8);
+        const u32x4 a = gld((const u32x4*)p), b = gld((const u32x4*)(p + 8)), c = gld((const u32x4*)(p + 16));
+        const uint32_t d[12] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w, c.x, c.y, c.z, c.w};
+        uint32_t pr[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const uint32_t lo = comp == 2 ? d[3 * q + 1] : d[3 * q];
+            const uint32_t hi = comp == 0 ? d[3 * q + 1] : d[3 * q + 2];
+            pr[q] = __builtin_amdgcn_perm(hi, lo, sel);
+        }
+        r.q[k] = {pr[0], pr[1], pr[2], pr[3]};
+    }
+    return r;
+}
+
+/* load_vec3 for the kernels of every input kind (the host sets packed vector
+ * rows for 16-bit samples only). */
+template <int KIND, int NC>
+__device__ __forceinline__ RowRaw<KIND, NC> load_packed(const void* row, uint32_t x0, uint32_t w, uint32_t comp) {
+    if constexpr (KIND == JXS_SRC_U16)
+        return load_vec3<NC>(row, x0, w, comp);
+    else
+        return load_vec<KIND, NC>(row, x0, w);
+}
+
 template <int KIND, int NC>
 __device__ __forceinline__ void unpack(const RowRaw<KIND, NC>& r, const Scale& sc, int32_t (&out)[NC]) {
     if constexpr (KIND == JXS_SRC_I32) {
@@ -656,7 +696,26 @@ __device__ __forceinline__ void dwt_slots(const CAS JxsDwtJob& J, uint32_t bi, D
     const VertFlags f = {ir > 0, ir < n / 2, 2 * ir + 2 < n};
     const bool has_hi = vert && f.h;
     int32_t lo[CH], hi[CH];
-    if (KIND != JXS_SRC_I32 && vert && vec) {
+    if (KIND == JXS_SRC_U16 && vert && J.vec >= 2) {
+        /* packed 16-bit rows: the same unscaled vertical step on component
+         * J.vec - 2 of input rows 2i-2 .. 2i+2, eight columns at a time */
+        const int i2 = (int)(2 * ir), last = (int)n - 1;
+        const uint8_t* r[5];
+        r[0] = base + (size_t)max(i2 - 2, 0) * pitch;
+        r[1] = base + (size_t)max(i2 - 1, 0) * pitch;
+        r[2] = base + (size_t)i2 * pitch;
+        r[3] = base + (size_t)min(i2 + 1, last) * pitch;
+        r[4] = base + (size_t)(f.next ? i2 + 2 : i2) * pitch;
+        const uint32_t comp = J.vec - 2;
+#pragma unroll
+        for (int h0 = 0; h0 < CH; h0 += 8) {
+            RowRaw<KIND, 8> raw[5];
+#pragma unroll
+            for (int j = 0; j < 5; j++)
+                raw[j] = load_packed<KIND, 8>(r[j], x0 + h0, w, comp);
+            vert_raw<KIND, 8>(f, raw, sc, lo + h0, hi + h0);
+        }
+    } else if (KIND != JXS_SRC_I32 && vert && vec) {
         /* input rows 2i-2 .. 2i+2 (clamped into the plane, the flags drop
          * what lies outside; row 2i+2 := row 2i past the end), unscaled */
         const int i2 = (int)(2 * ir), last = (int)n - 1;
@@ -708,7 +767,9 @@ __device__ __forceinline__ void dwt_slots(const CAS JxsDwtJob& J, uint32_t bi, D
 #pragma unroll
         for (int h0 = 0; h0 < CH; h0 += NC) {
             int32_t x[NC];
-            if (vec)
+            if (KIND == JXS_SRC_U16 && J.vec >= 2)
+                unpack<KIND, NC>(load_packed<KIND, NC>(base + (size_t)ir * pitch, x0 + h0, w, J.vec - 2), sc, x);
+            else if (vec)
                 unpack<KIND, NC>(load_vec<KIND, NC>(base + (size_t)ir * pitch, x0 + h0, w), sc, x);
             else
                 load_scalar<KIND, NC>(base + (size_t)ir * pitch, x0 + h0, w, step, sc, x);

@@ -19,6 +19,11 @@ SMALL = [
     (96, 64, 8, 21, 3, 2, 5, 0, 1),
     (128, 64, 11, 4, 3, 0, 3, 0, 1),
     (130, 66, 9, 3, 4, 1, 2, 0, 1),
+    # packed 16-bit rows read as 16-byte vectors (width a multiple of 8) at every
+    # vertical depth, and a width that is not (per-sample loads), odd height
+    (104, 40, 12, 21, 3, 2, 5, 0, 1),
+    (64, 30, 10, 21, 3, 0, 3, 0, 1),
+    (100, 33, 14, 21, 2, 1, 5, 0, 1),
 ]
 
 
