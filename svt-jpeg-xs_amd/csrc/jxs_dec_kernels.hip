@@ -13,7 +13,8 @@
  *                 prefix-summed bit offsets, dequantised and turned into
  *                 two's complement (Packing.c:513-868, Dequant.c:9-57,
  *                 Precinct.c:9-29).
- *   k_inv_rows    one block per row: all inverse horizontal levels of a row in
+ *   k_inv_rows    one block per row: every band sample of the row loaded at
+ *                 once (one memory round trip), all inverse horizontal levels in
  *                 LDS (Idwt.c:11-53 semantics), into an int32 row or, for
  *                 components without vertical levels, straight to the output
  *                 plane with the linear output scaling (NltDec.c:242-265).
@@ -26,9 +27,18 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "jxs_dec.h"
 
 #define WAVE 64
+#define GAS __attribute__((address_space(1)))
+
+/* a load through the global address space (not a flat load) */
+template <typename T>
+__device__ __forceinline__ T gld(const T* p) {
+    return *(const GAS T*)p;
+}
 
 __device__ __forceinline__ uint32_t dtrunc(uint32_t gain, uint32_t prio, uint32_t q, uint32_t r) {
     const uint32_t pump = prio < r ? 1u : 0u;
@@ -504,14 +514,71 @@ __global__ void __launch_bounds__(256) k_inv_rows(const JxsInvRowJob* __restrict
     for (uint32_t k = 0; k < J.levels; k++)
         n[k + 1] = (n[k] + 1) / 2;
     const uint32_t L = J.levels;
-    for (uint32_t i = threadIdx.x; i < n[L]; i += blockDim.x)
-        buf[i << L] = J.lf32 ? J.lf32[(size_t)y * J.lf_stride + i] : (int32_t)J.lf16[(size_t)y * J.lf_stride + i] * 256;
-    for (uint32_t k = 0; k < L; k++) {
-        const uint32_t nh = n[k] / 2;
-        const int16_t* h = J.hf[k] + (size_t)y * J.hf_stride[k];
-        for (uint32_t i = threadIdx.x; i < nh; i += blockDim.x)
-            buf[(2 * i + 1) << k] = (int32_t)h[i] * 256;
+    /* Every thread issues the loads of all its samples of the row (low band,
+     * then each level's high band; at most JXS_MAX_ROW / 256 of a band)
+     * before it writes any into LDS, so the row costs one memory round trip. */
+    const uint32_t tid = threadIdx.x;
+    const size_t lfo = (size_t)y * J.lf_stride;
+    const int32_t* lf32 = J.lf32 ? J.lf32 + lfo : nullptr;
+    const int16_t* lf16 = J.lf16 + (J.lf32 ? 0 : lfo);
+    const uint32_t nL = n[L];
+    constexpr int NL = (JXS_MAX_ROW + 255) / 256;
+    int32_t vl[NL];
+#pragma unroll
+    for (int j = 0; j < NL; j++)
+        vl[j] = 0;
+    if (lf32) {
+#pragma unroll
+        for (int j = 0; j < NL; j++)
+            if (tid + 256u * j < nL)
+                vl[j] = gld(lf32 + tid + 256u * j);
+    } else {
+#pragma unroll
+        for (int j = 0; j < NL; j++)
+            if (tid + 256u * j < nL)
+                vl[j] = gld(lf16 + tid + 256u * j); /* scaled by 256 when stored */
     }
+    /* high band of level k: floor(n[k] / 2) <= JXS_MAX_ROW >> (k + 1) samples */
+    constexpr int NH0 = ((JXS_MAX_ROW >> 1) + 255) / 256, NH1 = ((JXS_MAX_ROW >> 2) + 255) / 256,
+                  NH2 = ((JXS_MAX_ROW >> 3) + 255) / 256, NH3 = ((JXS_MAX_ROW >> 4) + 255) / 256,
+                  NH4 = ((JXS_MAX_ROW >> 5) + 255) / 256;
+    int32_t h0[NH0], h1[NH1], h2[NH2], h3[NH3], h4[NH4]; /* full registers: no 16-bit packing waits */
+    auto ldh = [&](auto& arr, auto cnt, uint32_t k) {
+        constexpr int N = decltype(cnt)::value;
+        const uint32_t nh = k < L ? n[k] / 2 : 0u;
+        const int16_t* hp = k < L ? J.hf[k] + (size_t)y * J.hf_stride[k] : nullptr;
+#pragma unroll
+        for (int j = 0; j < N; j++) {
+            const uint32_t i = tid + 256u * j;
+            arr[j] = 0;
+            if (i < nh)
+                arr[j] = (int32_t)gld(hp + i);
+        }
+    };
+    ldh(h0, std::integral_constant<int, NH0>{}, 0);
+    ldh(h1, std::integral_constant<int, NH1>{}, 1);
+    ldh(h2, std::integral_constant<int, NH2>{}, 2);
+    ldh(h3, std::integral_constant<int, NH3>{}, 3);
+    ldh(h4, std::integral_constant<int, NH4>{}, 4);
+#pragma unroll
+    for (int j = 0; j < NL; j++)
+        if (tid + 256u * j < nL)
+            buf[(tid + 256u * j) << L] = lf32 ? vl[j] : vl[j] * 256;
+    auto sth = [&](const auto& arr, auto cnt, uint32_t k) {
+        constexpr int N = decltype(cnt)::value;
+        const uint32_t nh = k < L ? n[k] / 2 : 0u;
+#pragma unroll
+        for (int j = 0; j < N; j++) {
+            const uint32_t i = tid + 256u * j;
+            if (i < nh)
+                buf[(2 * i + 1) << k] = arr[j] * 256;
+        }
+    };
+    sth(h0, std::integral_constant<int, NH0>{}, 0);
+    sth(h1, std::integral_constant<int, NH1>{}, 1);
+    sth(h2, std::integral_constant<int, NH2>{}, 2);
+    sth(h3, std::integral_constant<int, NH3>{}, 3);
+    sth(h4, std::integral_constant<int, NH4>{}, 4);
     __syncthreads();
     for (int k = (int)L - 1; k >= 0; k--) {
         const uint32_t s = 1u << k, nk = n[k], nh = nk / 2, nl = nk - nh;
