@@ -174,16 +174,25 @@ __global__ void k_dec_scan(const JxsDecFrame* __restrict__ frames, int nfr) {
     }
 }
 
-#define DMAXG 1024 /* groups of one band line (7680 / 2 / 4 = 960) */
+#define PMAXG 2048            /* groups of a packet decoded per pass */
+#define PGPT (PMAXG / 256)    /* consecutive groups of a thread in a pass */
+#define PNB (JXS_MAXC * JXS_NBC) /* band lines of a packet, at most */
+
+/* one band line of the packet being decoded, in codestream order */
+struct PkBand {
+    uint32_t g0;  /* first group of the line in the packet's group sequence */
+    uint32_t gt;  /* GTLI */
+    uint32_t sig; /* significance flags coded */
+    uint32_t s0;  /* first flag of the line in the significance sub-packet */
+    uint32_t w;   /* coefficients of the line */
+    uint32_t pad_;
+    int16_t* row; /* coefficient row */
+};
 
 struct UnpackCtx {
-    uint8_t dmode[JXS_MAXC][JXS_NBC];
-    uint8_t gtli[JXS_MAXC][JXS_NBC];
-    uint32_t s_bit, g_bit, d_bit, g_end, d_end, raw, ok;
-    uint32_t zpos[DMAXG];
-    uint8_t gl[DMAXG];
-    uint8_t sflag[DMAXG / 8];
-    uint32_t sbase[DMAXG / 8];
+    PkBand band[PNB + 1]; /* + a sentinel whose g0 is the packet's group count */
+    uint32_t nband, s_bit, g_bit, d_bit, g_end, d_end, raw, ok;
+    uint32_t zpos[PMAXG]; /* aligned bit position of the zero ending code j of the pass */
     uint32_t scan[8];
 };
 
@@ -270,6 +279,16 @@ __global__ void __launch_bounds__(64) k_dec_packets(const JxsDecFrame* __restric
     T[JXS_MAXC * JXS_NBC / 2] = bad ? 1u : 0u;
 }
 
+/* One block per (packet, precinct).  Inside a packet the significance,
+ * GCLI and data sub-packets each concatenate the packet's band lines at bit
+ * granularity (Packing.c:513-868), so the packet is decoded as one sequence
+ * of groups, PMAXG at a time: every thread takes PGPT consecutive groups,
+ * reads their significance flags, a block scan numbers the coded groups,
+ * the unary GCLI codes are located all at once (code j ends at the j-th zero
+ * bit: zero counts per 32-bit word + block prefix sum), a second scan places
+ * every group's data bits, and each group's data (the sign nibble, then
+ * gcli - gtli bit planes of 4 bits, at most 64 bits) is read as one 64-bit
+ * window, dequantised (Dequant.c:9-57) and stored in two's complement. */
 __global__ void __launch_bounds__(256) k_dec_unpack(const JxsDecFrame* __restrict__ frames, int nfr) {
     __shared__ UnpackCtx C;
     const int f = find_dec_frame(frames, nfr, blockIdx.y);
@@ -282,85 +301,125 @@ __global__ void __launch_bounds__(256) k_dec_unpack(const JxsDecFrame* __restric
     const uint32_t type = row + 1 >= g.rows ? 1 : 0;
     const uint32_t l = g.pk_line[p];
     const CsView V = cs_view(F);
-    {
-        const uint32_t* T = F.pkt + (size_t)row * F.pkt_stride;
+    const uint32_t tid = threadIdx.x;
+    const uint32_t* T = F.pkt + (size_t)row * F.pkt_stride;
+    if (tid == 64) {
         const uint32_t* E = T + JXS_DPKT_HDR + JXS_DPKT_WORDS * p;
-        if (threadIdx.x < JXS_MAXC * JXS_NBC) {
-            const uint32_t w = T[threadIdx.x >> 1] >> (16 * (threadIdx.x & 1));
-            C.dmode[threadIdx.x / JXS_NBC][threadIdx.x % JXS_NBC] = (uint8_t)(w & 0xFF);
-            C.gtli[threadIdx.x / JXS_NBC][threadIdx.x % JXS_NBC] = (uint8_t)((w >> 8) & 0xFF);
-        } else if (threadIdx.x == 64) {
-            const uint32_t fl = E[4];
-            C.ok = (fl & 1u) && T[JXS_MAXC * JXS_NBC / 2] == 0 ? 1u : 0u;
-            C.raw = (fl >> 1) & 1u;
-            C.s_bit = E[0];
-            C.g_bit = E[1];
-            C.g_end = E[2];
-            C.d_bit = E[2];
-            C.d_end = E[3];
+        const uint32_t fl = E[4];
+        C.ok = (fl & 1u) && T[JXS_MAXC * JXS_NBC / 2] == 0 ? 1u : 0u;
+        C.raw = (fl >> 1) & 1u;
+        C.s_bit = E[0];
+        C.g_bit = E[1];
+        C.g_end = E[2];
+        C.d_bit = E[2];
+        C.d_end = E[3];
+    } else if (tid < WAVE) {
+        /* the packet's band lines, one lane per candidate band (at most PNB < 64),
+         * compacted in band order with running group and flag counts */
+        const uint32_t gb0 = g.pk_bs[p], gb1 = g.pk_be[p];
+        const uint32_t gi = gb0 + tid;
+        bool ex = false;
+        uint32_t gw = 0, sws = 0, mode = 0, w = 0;
+        int16_t* rp = nullptr;
+        if (gi < gb1) {
+            const uint32_t c = g.gc[gi], b = g.gb[gi];
+            const JxsBand& bd = g.c[c].b[b];
+            if (l < bd.lines[type]) {
+                ex = true;
+                const uint32_t k = c * JXS_NBC + b;
+                mode = (T[k >> 1] >> (16 * (k & 1))) & 0xFFFFu;
+                gw = bd.gw;
+                sws = (mode & 2) ? bd.sw : 0u;
+                w = bd.w;
+                rp = F.coef + bd.coeff_off + (size_t)(row * bd.hl + l) * bd.cs;
+            }
+        }
+        const uint64_t m = __ballot(ex);
+        const uint32_t idx = (uint32_t)__popcll(m & ((1ull << tid) - 1));
+        uint32_t G = gw, S = sws; /* inclusive scans over the lanes */
+#pragma unroll
+        for (uint32_t o = 1; o < WAVE; o <<= 1) {
+            const uint32_t yg = __shfl_up(G, o, WAVE), ys = __shfl_up(S, o, WAVE);
+            if (tid >= o) {
+                G += yg;
+                S += ys;
+            }
+        }
+        if (ex) {
+            PkBand& B = C.band[idx];
+            B.g0 = G - gw;
+            B.gt = mode >> 8;
+            B.sig = (mode & 2) ? 1u : 0u;
+            B.s0 = S - sws;
+            B.w = w;
+            B.row = rp;
+        }
+        if (tid == WAVE - 1) {
+            const uint32_t n = (uint32_t)__popcll(m);
+            C.band[n].g0 = G;
+            C.nband = n;
         }
     }
     __syncthreads();
     if (!C.ok)
         return;
-    for (uint32_t gi = g.pk_bs[p]; gi < g.pk_be[p]; gi++) {
-        const uint32_t c = g.gc[gi], b = g.gb[gi];
-        const JxsBand& bd = g.c[c].b[b];
-        if (l >= bd.lines[type])
-            continue;
-        const uint32_t gt = C.gtli[c][b], gw = bd.gw, sw = bd.sw;
-        const bool use_sig = (C.dmode[c][b] & 2) != 0;
-        const uint32_t y = row * bd.hl + l;
-        bool err = false;
-        if (C.raw) {
-            const uint32_t g0 = C.g_bit;
-            for (uint32_t i = threadIdx.x; i < gw; i += blockDim.x)
-                C.gl[i] = (uint8_t)cs_bits(F, (uint64_t)g0 + 4 * i, 4);
-            __syncthreads();
-            if (threadIdx.x == 0)
-                C.g_bit = g0 + 4 * gw;
-        } else {
-            /* significance flags: 1 = insignificant group of 8 GCLIs */
-            uint32_t ncodes = gw;
-            if (use_sig) {
-                const uint32_t s0 = C.s_bit;
-                uint32_t tot = 0;
-                for (uint32_t base = 0; base < sw; base += blockDim.x) {
-                    const uint32_t s = base + threadIdx.x;
-                    uint32_t cnt = 0;
-                    if (s < sw) {
-                        const uint32_t ab = s0 + s + V.sh;
-                        const uint32_t fl = (cs_word(V, ab >> 5) >> (31 - (ab & 31))) & 1u;
-                        C.sflag[s] = (uint8_t)fl;
-                        const uint32_t size = min(8u, gw - 8 * s);
-                        cnt = fl ? 0u : size;
-                    }
-                    uint32_t t;
-                    const uint32_t ex = dblock_scan(cnt, C.scan, &t);
-                    if (s < sw)
-                        C.sbase[s] = tot + ex;
-                    tot += t;
-                }
-                ncodes = tot;
-                __syncthreads();
-                if (threadIdx.x == 0)
-                    C.s_bit = s0 + sw;
+    const uint32_t nband = C.nband, total = C.band[nband].g0, raw = C.raw;
+    const uint32_t s_bit = C.s_bit + V.sh, gend = C.g_end + V.sh, d_end = C.d_end;
+    uint32_t gp = C.g_bit + V.sh; /* aligned bit position of the pass's first GCLI code */
+    uint32_t dp = C.d_bit;        /* bit position of the pass's first data bit */
+    bool err = false;
+    for (uint32_t c0 = 0; c0 < total; c0 += PMAXG) {
+        const uint32_t cn = min((uint32_t)PMAXG, total - c0);
+        const uint32_t i0 = c0 + tid * PGPT; /* this thread's first group */
+        /* band line of the first group (binary search over the lines' first groups) */
+        uint32_t bl = 0;
+        {
+            uint32_t lo = 0, hi = nband - 1;
+            while (lo < hi) {
+                const uint32_t mid = (lo + hi + 1) >> 1;
+                if (C.band[mid].g0 <= i0)
+                    lo = mid;
+                else
+                    hi = mid - 1;
             }
-            /* unary codes: the j-th zero bit from the start ends code j */
-            const uint32_t g0 = C.g_bit + V.sh, gend = C.g_end + V.sh; /* aligned bit positions */
+            bl = lo;
+        }
+        /* significance: a coded group is one whose group of 8 is significant */
+        uint32_t bidx[PGPT], coded = 0, ncoded = 0;
+#pragma unroll
+        for (uint32_t k = 0; k < PGPT; k++) {
+            const uint32_t i = i0 + k;
+            bidx[k] = 0;
+            if (tid * PGPT + k >= cn)
+                continue;
+            while (C.band[bl + 1].g0 <= i)
+                bl++;
+            bidx[k] = bl;
+            bool cd = true;
+            if (!raw && C.band[bl].sig) {
+                const uint32_t ab = s_bit + C.band[bl].s0 + (i - C.band[bl].g0) / 8;
+                cd = ((cs_word(V, ab >> 5) >> (31 - (ab & 31))) & 1u) == 0; /* flag 1: insignificant */
+            }
+            coded |= (cd ? 1u : 0u) << k;
+            ncoded += cd ? 1u : 0u;
+        }
+        uint32_t ncodes;
+        const uint32_t cbase = dblock_scan(ncoded, C.scan, &ncodes);
+        /* GCLI codes */
+        if (!raw) {
             uint32_t found = 0;
-            for (uint32_t w = g0 >> 5; found < ncodes; w += blockDim.x) {
+            for (uint32_t w = gp >> 5; found < ncodes; w += blockDim.x) {
                 if ((uint64_t)w * 32 >= gend) {
                     err = true;
                     break;
                 }
-                const uint32_t wi = w + threadIdx.x;
+                const uint32_t wi = w + tid;
                 uint32_t word = 0xFFFFFFFFu;
                 if ((uint64_t)wi * 32 < gend) {
                     word = cs_word(V, wi);
                     const uint32_t lo = wi * 32;
-                    if (lo < g0)
-                        word |= ~(0xFFFFFFFFu >> (g0 - lo)); /* bits before the line start */
+                    if (lo < gp)
+                        word |= ~(0xFFFFFFFFu >> (gp - lo)); /* bits before the pass's first code */
                     if (lo + 32 > gend)
                         word |= (0xFFFFFFFFu >> (gend - lo)); /* bits after the sub-packet end */
                 }
@@ -370,107 +429,116 @@ __global__ void __launch_bounds__(256) k_dec_unpack(const JxsDecFrame* __restric
                 uint32_t k = found + ex;
                 while (z) {
                     const uint32_t lead = __clz(z);
-                    if (k < ncodes && k < DMAXG)
+                    if (k < ncodes)
                         C.zpos[k] = wi * 32 + lead;
                     k++;
                     z &= ~(0x80000000u >> lead);
                 }
                 found += t;
-                __syncthreads();
             }
-            if (!err) {
-                for (uint32_t i = threadIdx.x; i < gw; i += blockDim.x) {
-                    uint32_t j = i;
-                    bool coded = true;
-                    if (use_sig) {
-                        const uint32_t s = i / 8;
-                        coded = !C.sflag[s];
-                        j = C.sbase[s] + (i - 8 * s);
-                    }
-                    uint32_t gc = gt;
-                    if (coded) {
-                        const uint32_t start = j == 0 ? g0 : C.zpos[j - 1] + 1;
-                        gc = gt + (C.zpos[j] - start);
-                    }
-                    if (gc > 15) {
-                        gc = 15;
-                        atomicOr(F.status, 1u);
-                    }
-                    C.gl[i] = (uint8_t)gc;
-                }
-                __syncthreads();
-                if (threadIdx.x == 0)
-                    C.g_bit = (ncodes ? C.zpos[ncodes - 1] + 1 : g0) - V.sh;
-            }
+            __syncthreads();
         }
-        if (err) {
-            if (threadIdx.x == 0)
+        /* GCLI of every group, data bit counts */
+        uint32_t gcs[PGPT], nbits = 0;
+        uint32_t j = cbase;
+#pragma unroll
+        for (uint32_t k = 0; k < PGPT; k++) {
+            gcs[k] = 0;
+            if (tid * PGPT + k >= cn || err)
+                continue;
+            const uint32_t gt = C.band[bidx[k]].gt;
+            uint32_t gc = gt;
+            if (raw) {
+                const uint32_t ab = gp + 4 * (tid * PGPT + k);
+                gc = (uint32_t)(cs_bits64(V, ab) >> 60);
+            } else if ((coded >> k) & 1u) {
+                const uint32_t start = j == 0 ? gp : C.zpos[j - 1] + 1;
+                gc = gt + (C.zpos[j] - start);
+                j++;
+            }
+            if (gc > 15) {
+                gc = 15;
                 atomicOr(F.status, 1u);
-            return;
-        }
-        __syncthreads();
-        /* data: 4 sign bits then (gcli - gtli) bit planes of 4 bits per group */
-        const uint32_t d0 = C.d_bit;
-        int16_t* crow = F.coef + bd.coeff_off + (size_t)y * bd.cs;
-        uint32_t pos = 0;
-        for (uint32_t base = 0; base < gw; base += blockDim.x) {
-            const uint32_t i = base + threadIdx.x;
-            uint32_t gc = 0, nb = 0;
-            if (i < gw) {
-                gc = C.gl[i];
-                nb = gc > gt ? 4 * (gc - gt + 1) : 0;
             }
-            uint32_t t;
-            const uint32_t ex = dblock_scan(nb, C.scan, &t);
-            if (i < gw) {
-                uint32_t mag[4] = {0, 0, 0, 0}, sg = 0;
-                if (nb) {
-                    const uint64_t bp = (uint64_t)d0 + pos + ex;
-                    if (bp + nb > C.d_end) {
-                        atomicOr(F.status, 1u);
-                    } else {
-                        /* nb <= 64 bits: the sign nibble, then plane gc-1 .. gt, one bit per coefficient each */
-                        const uint64_t v = cs_bits64(V, bp + V.sh);
-                        sg = (uint32_t)(v >> 60);
-                        const uint32_t np = gc - gt;
+            gcs[k] = gc;
+            nbits += gc > gt ? 4 * (gc - gt + 1) : 0u;
+        }
+        uint32_t dtotal;
+        const uint32_t dbase = dblock_scan(nbits, C.scan, &dtotal);
+        /* data windows, all loaded before any is decoded */
+        uint64_t win[PGPT];
+        uint32_t bp = dp + dbase;
 #pragma unroll
-                        for (int t4 = 0; t4 < 4; t4++)
-                            mag[t4] = (gather4(v << (4 + t4)) >> (16 - np)) << gt;
-                    }
-                }
-                uint32_t val[4];
-#pragma unroll
-                for (uint32_t t4 = 0; t4 < 4; t4++) {
-                    uint32_t m = mag[t4];
-                    if (gt > 0 && m) {
-                        if (g.qpih == 0) {
-                            m |= 1u << (gt - 1);
-                        } else {
-                            const uint32_t sc = gc - gt + 1;
-                            uint32_t acc = 0;
-                            for (uint32_t x = m; x > 0; x >>= sc)
-                                acc += x;
-                            m = acc & 0xFFFFu;
-                        }
-                    }
-                    const bool neg = (sg >> (3 - t4)) & 1u;
-                    val[t4] = (uint16_t)(neg ? -(int32_t)(m & 0x7FFF) : (int32_t)(m & 0x7FFF));
-                }
-                int16_t* cp = crow + 4 * i;
-                if (4 * i + 4 <= bd.w && ((uintptr_t)cp & 7) == 0) {
-                    *(uint2*)cp = make_uint2(val[0] | val[1] << 16, val[2] | val[3] << 16);
+        for (uint32_t k = 0; k < PGPT; k++) {
+            win[k] = 0;
+            if (tid * PGPT + k >= cn || err)
+                continue;
+            const uint32_t gt = C.band[bidx[k]].gt, gc = gcs[k];
+            const uint32_t nb = gc > gt ? 4 * (gc - gt + 1) : 0u;
+            if (nb) {
+                if ((uint64_t)bp + nb > d_end) {
+                    atomicOr(F.status, 1u);
+                    gcs[k] = gt; /* decoded as zeros */
                 } else {
-                    for (uint32_t t4 = 0; t4 < 4 && 4 * i + t4 < bd.w; t4++)
-                        cp[t4] = (int16_t)val[t4];
+                    win[k] = cs_bits64(V, (uint64_t)bp + V.sh);
                 }
             }
-            pos += t;
+            bp += nb;
         }
-        __syncthreads();
-        if (threadIdx.x == 0)
-            C.d_bit = d0 + pos;
-        __syncthreads();
+#pragma unroll
+        for (uint32_t k = 0; k < PGPT; k++) {
+            if (tid * PGPT + k >= cn || err)
+                continue;
+            const PkBand& B = C.band[bidx[k]];
+            const uint32_t gt = B.gt, gc = gcs[k];
+            const uint32_t i = i0 + k - B.g0; /* group within its line */
+            uint32_t mag[4] = {0, 0, 0, 0}, sg = 0;
+            if (gc > gt) {
+                const uint64_t v = win[k];
+                sg = (uint32_t)(v >> 60);
+                const uint32_t np = gc - gt;
+#pragma unroll
+                for (int t4 = 0; t4 < 4; t4++)
+                    mag[t4] = (gather4(v << (4 + t4)) >> (16 - np)) << gt;
+            }
+            uint32_t val[4];
+#pragma unroll
+            for (uint32_t t4 = 0; t4 < 4; t4++) {
+                uint32_t m = mag[t4];
+                if (gt > 0 && m) {
+                    if (g.qpih == 0) {
+                        m |= 1u << (gt - 1);
+                    } else {
+                        const uint32_t sc = gc - gt + 1;
+                        uint32_t acc = 0;
+                        for (uint32_t x = m; x > 0; x >>= sc)
+                            acc += x;
+                        m = acc & 0xFFFFu;
+                    }
+                }
+                const bool neg = (sg >> (3 - t4)) & 1u;
+                val[t4] = (uint16_t)(neg ? -(int32_t)(m & 0x7FFF) : (int32_t)(m & 0x7FFF));
+            }
+            int16_t* cp = B.row + 4 * i;
+            if (4 * i + 4 <= B.w && ((uintptr_t)cp & 7) == 0) {
+                *(uint2*)cp = make_uint2(val[0] | val[1] << 16, val[2] | val[3] << 16);
+            } else {
+                for (uint32_t t4 = 0; t4 < 4 && 4 * i + t4 < B.w; t4++)
+                    cp[t4] = (int16_t)val[t4];
+            }
+        }
+        if (err)
+            break;
+        /* the next pass starts after this pass's codes and data */
+        if (raw)
+            gp += 4 * cn;
+        else if (ncodes)
+            gp = C.zpos[ncodes - 1] + 1;
+        dp += dtotal;
+        __syncthreads(); /* zpos and the scan words are reused */
     }
+    if (err && tid == 0)
+        atomicOr(F.status, 1u);
 }
 
 /* ------------------------------------------------------------------ */
