@@ -136,6 +136,22 @@ __device__ __forceinline__ int find_dec_frame(const JxsDecFrame* fr, int nfr, ui
     return lo;
 }
 
+/* The frame of block blk, one memory round trip for up to 64 frames: each
+ * lane of the wave reads one frame's first block (increasing with the frame)
+ * and the wave counts those at or below blk. */
+__device__ __forceinline__ int find_dec_frame_wave(const JxsDecFrame* fr, int nfr, uint32_t blk) {
+    const uint32_t lane = threadIdx.x & (WAVE - 1);
+    int f = 0;
+    for (int c0 = 0; c0 < nfr; c0 += WAVE) {
+        const int i = c0 + (int)lane;
+        const uint64_t m = __ballot(i < nfr && fr[i].prec_base <= blk);
+        f += __popcll(m);
+        if (m != ~0ull)
+            break;
+    }
+    return f - 1;
+}
+
 /* One block per codestream: copies a known precinct table, or walks the
  * slice headers and the precinct length chain (lane 0). */
 __global__ void k_dec_scan(const JxsDecFrame* __restrict__ frames, int nfr) {
@@ -175,7 +191,8 @@ __global__ void k_dec_scan(const JxsDecFrame* __restrict__ frames, int nfr) {
 }
 
 #define PMAXG 2048            /* groups of a packet decoded per pass */
-#define PGPT (PMAXG / 256)    /* consecutive groups of a thread in a pass */
+#define PK_THREADS 512        /* threads of an unpack block */
+#define PGPT (PMAXG / PK_THREADS) /* consecutive groups of a thread in a pass */
 #define PNB (JXS_MAXC * JXS_NBC) /* band lines of a packet, at most */
 
 /* one band line of the packet being decoded, in codestream order */
@@ -193,7 +210,7 @@ struct UnpackCtx {
     PkBand band[PNB + 1]; /* + a sentinel whose g0 is the packet's group count */
     uint32_t nband, s_bit, g_bit, d_bit, g_end, d_end, raw, ok;
     uint32_t zpos[PMAXG]; /* aligned bit position of the zero ending code j of the pass */
-    uint32_t scan[8];
+    uint32_t scan[PK_THREADS / WAVE];
 };
 
 /* One lane per precinct of the launch: parses the precinct header's band
@@ -282,16 +299,17 @@ __global__ void __launch_bounds__(64) k_dec_packets(const JxsDecFrame* __restric
 /* One block per (packet, precinct).  Inside a packet the significance,
  * GCLI and data sub-packets each concatenate the packet's band lines at bit
  * granularity (Packing.c:513-868), so the packet is decoded as one sequence
- * of groups, PMAXG at a time: every thread takes PGPT consecutive groups,
+ * of groups, PMAXG at a time: each of the block's PK_THREADS threads takes
+ * PGPT consecutive groups,
  * reads their significance flags, a block scan numbers the coded groups,
  * the unary GCLI codes are located all at once (code j ends at the j-th zero
  * bit: zero counts per 32-bit word + block prefix sum), a second scan places
  * every group's data bits, and each group's data (the sign nibble, then
  * gcli - gtli bit planes of 4 bits, at most 64 bits) is read as one 64-bit
  * window, dequantised (Dequant.c:9-57) and stored in two's complement. */
-__global__ void __launch_bounds__(256) k_dec_unpack(const JxsDecFrame* __restrict__ frames, int nfr) {
+__global__ void __launch_bounds__(PK_THREADS) __attribute__((amdgpu_waves_per_eu(8))) k_dec_unpack(const JxsDecFrame* __restrict__ frames, int nfr) {
     __shared__ UnpackCtx C;
-    const int f = find_dec_frame(frames, nfr, blockIdx.y);
+    const int f = find_dec_frame_wave(frames, nfr, blockIdx.y);
     const JxsDecFrame& F = frames[f];
     const JxsGeom& g = *F.g;
     const uint32_t row = blockIdx.y - F.prec_base;
@@ -742,7 +760,7 @@ hipError_t jxs_launch_dec_packets(const JxsDecFrame* frames, int nfr, uint32_t p
 hipError_t jxs_launch_dec_unpack(const JxsDecFrame* frames, int nfr, uint32_t max_npk, uint32_t precincts, hipStream_t st) {
     if (!precincts)
         return hipSuccess;
-    hipLaunchKernelGGL(k_dec_unpack, dim3(max_npk, precincts), dim3(256), 0, st, frames, nfr);
+    hipLaunchKernelGGL(k_dec_unpack, dim3(max_npk, precincts), dim3(PK_THREADS), 0, st, frames, nfr);
     return hipGetLastError();
 }
 
