@@ -335,12 +335,12 @@ __global__ void __launch_bounds__(PK_THREADS) __attribute__((amdgpu_waves_per_eu
         /* the packet's band lines, one lane per candidate band (at most PNB < 64),
          * compacted in band order with running group and flag counts */
         const uint32_t gb0 = g.pk_bs[p], gb1 = g.pk_be[p];
-        const uint32_t gi = gb0 + tid;
+        const uint32_t gi = tid; /* every band's indices are read with the packet's range */
+        const uint32_t c = gi < g.nb_all ? g.gc[gi] : 0u, b = gi < g.nb_all ? g.gb[gi] : 0u;
         bool ex = false;
         uint32_t gw = 0, sws = 0, mode = 0, w = 0;
         int16_t* rp = nullptr;
-        if (gi < gb1) {
-            const uint32_t c = g.gc[gi], b = g.gb[gi];
+        if (gi >= gb0 && gi < gb1) {
             const JxsBand& bd = g.c[c].b[b];
             if (l < bd.lines[type]) {
                 ex = true;
@@ -389,6 +389,11 @@ __global__ void __launch_bounds__(PK_THREADS) __attribute__((amdgpu_waves_per_eu
     for (uint32_t c0 = 0; c0 < total; c0 += PMAXG) {
         const uint32_t cn = min((uint32_t)PMAXG, total - c0);
         const uint32_t i0 = c0 + tid * PGPT; /* this thread's first group */
+        /* the first GCLI words of the pass, loaded with the significance flags */
+        const uint32_t w0 = gp >> 5;
+        uint32_t pre = 0xFFFFFFFFu;
+        if (!raw && (uint64_t)(w0 + tid) * 32 < gend)
+            pre = cs_word(V, w0 + tid);
         /* band line of the first group (binary search over the lines' first groups) */
         uint32_t bl = 0;
         {
@@ -434,7 +439,7 @@ __global__ void __launch_bounds__(PK_THREADS) __attribute__((amdgpu_waves_per_eu
                 const uint32_t wi = w + tid;
                 uint32_t word = 0xFFFFFFFFu;
                 if ((uint64_t)wi * 32 < gend) {
-                    word = cs_word(V, wi);
+                    word = w == w0 ? pre : cs_word(V, wi);
                     const uint32_t lo = wi * 32;
                     if (lo < gp)
                         word |= ~(0xFFFFFFFFu >> (gp - lo)); /* bits before the pass's first code */
