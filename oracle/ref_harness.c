@@ -9,7 +9,9 @@
  *   in.raw holds `frames` frames back to back; each frame is the planes of the
  *   format (planar) or one interleaved plane (packed RGB), u8 for depth 8 and
  *   u16 little-endian above.  Writes the concatenated codestreams and prints one
- *   JSON line with fps and send->get latency percentiles (ms).
+ *   JSON line with fps and send->get latency percentiles (ms).  With
+ *   JXS_REF_REPEAT=R in the environment the frames are sent R times over
+ *   (timing of a longer sample; only the first pass's codestreams are written).
  *
  * Decoder (HARNESS_DECODER):
  *   ref_dec in.jxs out.raw frames threads
@@ -40,15 +42,21 @@ typedef struct {
     svt_jpeg_xs_encoder_api_t* enc;
     svt_jpeg_xs_frame_t* frames;
     double* t_send;
-    uint32_t n;
+    uint32_t n;      /* sends */
+    uint32_t nfr;    /* distinct frames, sent in turn */
+    uint8_t* bs;     /* nslots codestream buffers */
+    uint32_t nslots, bpf;
     int err;
 } sender_t;
 
 static void* sender(void* p) {
     sender_t* s = (sender_t*)p;
     for (uint32_t i = 0; i < s->n; i++) {
+        svt_jpeg_xs_frame_t fr = s->frames[i % s->nfr];
+        fr.bitstream.buffer = s->bs + (size_t)(i % s->nslots) * s->bpf;
+        fr.user_prv_ctx_ptr = (void*)(uintptr_t)(i + 1);
         s->t_send[i] = now_ms();
-        SvtJxsErrorType_t e = svt_jpeg_xs_encoder_send_picture(s->enc, &s->frames[i], 1);
+        SvtJxsErrorType_t e = svt_jpeg_xs_encoder_send_picture(s->enc, &fr, 1);
         if (e) {
             s->err = (int)e;
             return NULL;
@@ -113,8 +121,13 @@ int main(int argc, char** argv) {
         return 7;
     }
     fclose(fi);
+    const char* rep_env = getenv("JXS_REF_REPEAT");
+    const uint32_t repeat = rep_env && atoi(rep_env) > 1 ? (uint32_t)atoi(rep_env) : 1u;
+    const uint32_t total = nframes * repeat;
+    /* one codestream buffer per send of the first pass, reused cyclically after it */
+    const uint32_t nslots = nframes < 64 ? (total < 64 ? total : 64) : nframes;
     svt_jpeg_xs_frame_t* frames = (svt_jpeg_xs_frame_t*)calloc(nframes, sizeof(*frames));
-    uint8_t* bs = (uint8_t*)malloc((size_t)bytes_per_frame * nframes);
+    uint8_t* bs = (uint8_t*)malloc((size_t)bytes_per_frame * nslots);
     for (uint32_t f = 0; f < nframes; f++) {
         size_t off = f * frame_bytes;
         for (int c = 0; c < cfg.components_num; c++) {
@@ -137,15 +150,15 @@ int main(int argc, char** argv) {
         frames[f].bitstream.used_size = 0;
         frames[f].user_prv_ctx_ptr = (void*)(uintptr_t)(f + 1);
     }
-    double* t_send = (double*)calloc(nframes, sizeof(double));
-    double* lat = (double*)calloc(nframes, sizeof(double));
-    sender_t s = {&enc, frames, t_send, nframes, 0};
+    double* t_send = (double*)calloc(total, sizeof(double));
+    double* lat = (double*)calloc(total, sizeof(double));
+    sender_t s = {&enc, frames, t_send, total, nframes, bs, nslots, bytes_per_frame, 0};
     double t0 = now_ms();
     pthread_t th;
     pthread_create(&th, NULL, sender, &s);
     FILE* fo = fopen(out_name, "wb");
     int rc = 0;
-    for (uint32_t f = 0; f < nframes; f++) {
+    for (uint32_t f = 0; f < total; f++) {
         svt_jpeg_xs_frame_t out;
         e = svt_jpeg_xs_encoder_get_packet(&enc, &out, 1);
         double t = now_ms();
@@ -156,7 +169,8 @@ int main(int argc, char** argv) {
         }
         uint32_t idx = (uint32_t)(uintptr_t)out.user_prv_ctx_ptr - 1;
         lat[f] = t - t_send[idx];
-        fwrite(out.bitstream.buffer, 1, out.bitstream.used_size, fo);
+        if (idx < nframes)
+            fwrite(out.bitstream.buffer, 1, out.bitstream.used_size, fo);
     }
     double t1 = now_ms();
     pthread_join(th, NULL);
@@ -164,10 +178,10 @@ int main(int argc, char** argv) {
     svt_jpeg_xs_encoder_close(&enc);
     if (rc || s.err)
         return rc ? rc : 9;
-    qsort(lat, nframes, sizeof(double), cmp_d);
-    uint32_t i50 = (nframes - 1) * 50 / 100, i99 = (nframes - 1) * 99 / 100;
+    qsort(lat, total, sizeof(double), cmp_d);
+    uint32_t i50 = (total - 1) * 50 / 100, i99 = (total - 1) * 99 / 100;
     printf("{\"frames\": %u, \"bytes_per_frame\": %u, \"fps\": %.3f, \"p50_ms\": %.3f, \"p99_ms\": %.3f, \"threads\": %u}\n",
-           nframes, bytes_per_frame, nframes * 1e3 / (t1 - t0), lat[i50], lat[i99], enc.threads_num);
+           total, bytes_per_frame, total * 1e3 / (t1 - t0), lat[i50], lat[i99], enc.threads_num);
     free(raw);
     free(bs);
     free(frames);
