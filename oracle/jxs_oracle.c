@@ -60,6 +60,7 @@ typedef struct {
     uint8_t maxq, maxr;
     /* encoder side */
     uint32_t depth, format, lcod, qpih, sig, hdr_len;
+    uint32_t rc_mode; /* 0: budget per precinct; 1: unused padding moves to the next precinct of the slice */
     uint8_t hdr[512];
     uint32_t* slice_sizes;
 } ob_pi;
@@ -527,12 +528,15 @@ void jxso_default_params(jxso_params_t* p) {
     p->quantization = 0;
     p->slice_height = 16;
     p->significance = 1;
+    p->rc_mode = 1; /* RC_CBR_PER_PRECINCT_MOVE_PADDING (EncHandle.c:519) */
 }
 
 static int32_t enc_configure(const jxso_params_t* prm, ob_pi* pi) {
     memset(pi, 0, sizeof(*pi));
     uint32_t nc, sx[4], sy[4];
     if (prm->depth < 8 || prm->depth > 14)
+        return E_BAD_PARAM;
+    if (prm->rc_mode > 1) /* the per-slice modes 2 and 3 are not restated */
         return E_BAD_PARAM;
     if (sampling(prm->format, &nc, sx, sy))
         return E_BAD_PARAM;
@@ -596,6 +600,7 @@ static int32_t enc_configure(const jxso_params_t* prm, ob_pi* pi) {
     pi->lcod = (uint32_t)bpf;
     pi->qpih = prm->quantization;
     pi->sig = !!prm->significance;
+    pi->rc_mode = prm->rc_mode;
     if (weights_assign(pi, prm->format))
         return E_BAD_PARAM;
     max_qr(pi);
@@ -1211,7 +1216,10 @@ static void pack_precinct(const ob_pi* pi, const enc_frame* f, prec_state* ps, u
 }
 
 /* One whole frame: PreRcStageProcess.c:31-116 (slice byte ranges),
- * PackStageProcess.c:567-749 (per-slice precinct budgets, move padding),
+ * PackStageProcess.c:567-749 (per-slice precinct budgets; with rate control
+ * mode 1 the first precinct gets 10 % more and each precinct's unused bytes
+ * move to the next one of its slice, PackStageProcess.c:621-640, 293-307;
+ * mode 0 splits the slice evenly and pads every precinct to its budget),
  * PackStageProcess.c:274-338 (per precinct). */
 static int32_t enc_code(const ob_pi* pi, const enc_frame* f, uint8_t* out, uint8_t* qr_dump) {
     memset(out, 0, pi->lcod);
@@ -1234,15 +1242,17 @@ static int32_t enc_code(const ob_pi* pi, const enc_frame* f, uint8_t* out, uint8
         bw_put(&w, 4, 16);
         bw_put(&w, s, 16);
         uint32_t minb = sbudget / nprec;
-        uint32_t first = minb * 110 / 100;
-        if (first > sbudget)
-            first = sbudget;
-        uint32_t left;
-        if (nprec > 1) {
-            minb = (sbudget - first) / (nprec - 1);
-            left = sbudget - minb * (nprec - 1) - first;
-        } else {
-            left = 0;
+        uint32_t first = minb, left = sbudget - minb * nprec;
+        if (pi->rc_mode == 1) {
+            first = minb * 110 / 100;
+            if (first > sbudget)
+                first = sbudget;
+            if (nprec > 1) {
+                minb = (sbudget - first) / (nprec - 1);
+                left = sbudget - minb * (nprec - 1) - first;
+            } else {
+                left = 0;
+            }
         }
         uint32_t carry = 0;
         for (uint32_t i = 0; i < nprec; i++) {
@@ -1261,7 +1271,7 @@ static int32_t enc_code(const ob_pi* pi, const enc_frame* f, uint8_t* out, uint8
             }
             uint32_t pad = budget - hdr - data;
             uint32_t total = budget;
-            if (i + 1 < nprec) {
+            if (pi->rc_mode == 1 && i + 1 < nprec) {
                 total -= pad;
                 carry = pad;
                 pad = 0;

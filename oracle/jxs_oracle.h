@@ -27,6 +27,7 @@ typedef struct jxso_params {
     uint32_t quantization; /* 0 deadzone, 1 uniform */
     uint32_t slice_height;
     uint32_t significance; /* coding_significance flag */
+    uint32_t rc_mode;      /* rate_control_mode: 0 budget per precinct, 1 the same moving padding on (default) */
 } jxso_params_t;
 
 /* Fills defaults exactly as svt_jpeg_xs_encoder_load_default_parameters

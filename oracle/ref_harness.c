@@ -5,7 +5,7 @@
  * Source/API/SvtJpegxsDec.h:59-148), always with use_cpu_flags = CPU_FLAGS_C.
  *
  * Encoder (HARNESS_ENCODER):
- *   ref_enc in.raw out.jxs W H depth format bpp_num bpp_den decomp_v decomp_h frames threads [slice_height [quant [significance]]]
+ *   ref_enc in.raw out.jxs W H depth format bpp_num bpp_den decomp_v decomp_h frames threads [slice_height [quant [significance [rc_mode]]]]
  *   in.raw holds `frames` frames back to back; each frame is the planes of the
  *   format (planar) or one interleaved plane (packed RGB), u8 for depth 8 and
  *   u16 little-endian above.  Writes the concatenated codestreams and prints one
@@ -91,6 +91,8 @@ int main(int argc, char** argv) {
         enc.quantization = (uint32_t)atoi(argv[14]);
     if (argc > 15)
         enc.coding_significance = (uint32_t)atoi(argv[15]);
+    if (argc > 16)
+        enc.rate_control_mode = (uint32_t)atoi(argv[16]);
     /* JXS_REF_SIMD=1 enables the reference's AVX2 / AVX-512 intrinsic kernels
      * (the SSE2 bit stays clear: the only SSE2 entries are the unassembled NASM
      * routines, whose C twins are then selected).  Output is identical either

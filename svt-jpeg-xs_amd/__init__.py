@@ -251,7 +251,7 @@ class Encoder:
     the given fields, then svt_jpeg_xs_encoder_init."""
 
     def __init__(self, width, height, depth, fmt, bpp_num, bpp_den=1, decomp_v=2, decomp_h=5, quantization=0,
-                 slice_height=16, significance=1, verbose=1, slice_packetization=0):
+                 slice_height=16, significance=1, verbose=1, slice_packetization=0, rate_control_mode=1):
         L = lib()
         self.api = EncoderApi()
         rc = L.svt_jpeg_xs_encoder_load_default_parameters(API_MAJOR, API_MINOR, ctypes.byref(self.api))
@@ -262,6 +262,7 @@ class Encoder:
         a.bpp_numerator, a.bpp_denominator, a.ndecomp_v, a.ndecomp_h = bpp_num, bpp_den, decomp_v, decomp_h
         a.quantization, a.slice_height, a.coding_significance, a.verbose = quantization, slice_height, significance, verbose
         a.slice_packetization_mode = slice_packetization
+        a.rate_control_mode = rate_control_mode
         self.config = ImageConfig()
         nb = ctypes.c_uint32(0)
         rc = L.svt_jpeg_xs_encoder_get_image_config(API_MAJOR, API_MINOR, ctypes.byref(a), ctypes.byref(self.config),

@@ -80,7 +80,9 @@ struct JxsGeom {
      * last precinct row, in packet order: walk words (below) and the RAW
      * bytes of each line's packet; zero past the end */
     uint32_t rq[2][JXS_NU], rq_raw[2][JXS_NU];
-    uint32_t rq_n[2], pad_rq[2];
+    uint32_t rq_n[2];
+    uint32_t rc_mode; /* rate control: 0 budget per precinct, 1 unused padding moved to the next precinct */
+    uint32_t pad_rq;
     uint64_t n_coeff, n_gcli, n_smax; /* buffer sizes per frame */
     uint64_t n_i32;                   /* int32 transform scratch per frame */
     uint8_t hdr[JXS_HDR_MAX];

@@ -552,6 +552,7 @@ int32_t build_encoder_geometry(const EncParams& p, HostGeom& out, int verbose) {
     g.lcod = (uint32_t)bpf;
     g.qpih = p.quantization;
     g.sig = p.significance ? 1 : 0;
+    g.rc_mode = p.rc_mode;
     WTab t;
     bool ok = nc == 3 && wtab_default(p.format, p.decomp_v, p.decomp_h, t) == 0;
     if (!ok && (nc != 3 || wtab_derived(p.format, p.decomp_v, p.decomp_h, t)))

@@ -17,6 +17,7 @@ struct EncParams {
     uint32_t width, height, depth, format;
     uint32_t bpp_num, bpp_den, decomp_v, decomp_h;
     uint32_t quantization, slice_height, significance;
+    uint32_t rc_mode; /* rate_control_mode 0 or 1 */
 };
 
 /* Host geometry = device geometry + slice tables. */

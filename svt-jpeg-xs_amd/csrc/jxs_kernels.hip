@@ -1357,14 +1357,20 @@ __global__ void __launch_bounds__(WAVE * RT_WAVES) k_rate(const JxsFrameDesc* __
     const uint32_t em = g.empty_mask[lane & (JXS_NQ - 1)], maxq = uni(g.maxq), maxr = uni(g.maxr);
     const uint32_t eq = (uint32_t)__ballot(em & 1u); /* bit q: (q, 0) truncates every band away */
     const uint32_t sbud = gld(F.slice_bud + s); /* global, not flat: a uniform value */
+    /* mode 1: the first precinct 10 % more, the rest split evenly
+     * (PackStageProcess.c:621-640); mode 0: an even split (600-601) */
+    const bool move_pad = uni(g.rc_mode) == 1;
     uint32_t minb = sbud / nprec;
-    uint32_t first = minb * 110 / 100;
-    if (first > sbud)
-        first = sbud;
-    uint32_t left = 0;
-    if (nprec > 1) {
-        minb = (sbud - first) / (nprec - 1);
-        left = sbud - minb * (nprec - 1) - first;
+    uint32_t first = minb, left = sbud - minb * nprec;
+    if (move_pad) {
+        first = minb * 110 / 100;
+        if (first > sbud)
+            first = sbud;
+        left = 0;
+        if (nprec > 1) {
+            minb = (sbud - first) / (nprec - 1);
+            left = sbud - minb * (nprec - 1) - first;
+        }
     }
     uint32_t carry = 0, off = gld(F.slice_off + s) + 6;
     bool failed = false;
@@ -1426,7 +1432,7 @@ __global__ void __launch_bounds__(WAVE * RT_WAVES) k_rate(const JxsFrameDesc* __
                             if (ok) {
                                 const uint32_t data = lane_of(cr, base + r);
                                 uint32_t pad = avail - data, total = budget;
-                                if (i + 1 < nprec) {
+                                if (move_pad && i + 1 < nprec) {
                                     total -= pad;
                                     carry = pad;
                                     pad = 0;

@@ -27,7 +27,8 @@ REF = "/root/reference"
 
 from jxs_util import load_oracle, load_synth  # noqa: E402
 
-# name: (W, H, depth, format, bpp_num, bpp_den, decomp_v, decomp_h, quant, slice_h, significance, sigma, frame)
+# name: (W, H, depth, format, bpp_num, bpp_den, decomp_v, decomp_h, quant, slice_h, significance, sigma, frame
+#        [, rate_control_mode (default 1)])
 SMALL = {
     "s_64x32_422_8b_bpp5_v2": (64, 32, 8, 3, 5, 1, 2, 5, 0, 16, 1, 4.0, 0),
     "s_256x64_422_10b_bpp3_v2": (256, 64, 10, 3, 3, 1, 2, 5, 0, 16, 1, 4.0, 1),
@@ -43,6 +44,10 @@ SMALL = {
     "s_320x180_422_10b_bpp7_2_slice32": (320, 180, 10, 3, 7, 2, 2, 5, 0, 32, 1, 4.0, 11),
     "s_256x192_422_10b_bpp20": (256, 192, 10, 3, 20, 1, 2, 5, 0, 16, 1, 4.0, 12),
     "s_30x60_444_8b_bpp4_error": (30, 60, 8, 4, 4, 1, 2, 5, 0, 16, 1, 4.0, 13),
+    # rate control mode 0: budget per precinct, padding kept in each precinct
+    "s_256x64_422_10b_bpp3_v2_rc0": (256, 64, 10, 3, 3, 1, 2, 5, 0, 16, 1, 4.0, 1, 0),
+    "s_200x200_444_8b_bpp3_v1_rc0": (200, 200, 8, 4, 3, 1, 1, 5, 0, 16, 1, 4.0, 2, 0),
+    "s_320x180_422_10b_bpp7_2_slice32_rc0": (320, 180, 10, 3, 7, 2, 2, 5, 0, 32, 1, 4.0, 11, 0),
 }
 
 # The configurations of the benchmark workloads, first frame of each.
@@ -65,7 +70,8 @@ def sha(b):
 
 
 def ref_encode(cfg, planes, td):
-    W, H, D, F, bn, bd, V, Hh, q, sh, sig, _, _ = cfg
+    W, H, D, F, bn, bd, V, Hh, q, sh, sig = cfg[:11]
+    rc = cfg[13] if len(cfg) > 13 else 1
     O = load_oracle()
     raw = os.path.join(td, "in.raw")
     out = os.path.join(td, "out.jxs")
@@ -73,7 +79,7 @@ def ref_encode(cfg, planes, td):
     with open(raw, "wb") as f:
         f.write(S.frame_bytes(planes))
     cmd = [O.REF_ENC, raw, out, str(W), str(H), str(D), str(F), str(bn), str(bd), str(V), str(Hh), "1", "1", str(sh),
-           str(q), str(sig)]
+           str(q), str(sig), str(rc)]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         return None, r.returncode
@@ -93,12 +99,13 @@ def ref_decode(cs, td):
 
 def case_entry(name, cfg, td, keep):
     S = load_synth()
-    W, H, D, F, bn, bd, V, Hh, q, sh, sig, sigma, frame = cfg
+    W, H, D, F, bn, bd, V, Hh, q, sh, sig, sigma, frame = cfg[:13]
     planes = S.make_frame(W, H, D, F, frame=frame, sigma=sigma)
     cs, rc = ref_encode(cfg, planes, td)
     e = {"params": dict(zip(["width", "height", "depth", "format", "bpp_num", "bpp_den", "decomp_v", "decomp_h",
-                             "quantization", "slice_height", "significance", "sigma", "frame"], cfg)),
+                             "quantization", "slice_height", "significance", "sigma", "frame"], cfg[:13])),
          "input_sha256": S.frame_sha256(planes)}
+    e["params"]["rate_control_mode"] = cfg[13] if len(cfg) > 13 else 1
     if cs is None:
         e["reference_error"] = True
         return e

@@ -7,7 +7,7 @@ import pytest
 
 from jxs_util import load_oracle, load_package, load_synth
 
-# (width, height, depth, format, bpp, decomp_v, decomp_h, quantization, significance)
+# (width, height, depth, format, bpp, decomp_v, decomp_h, quantization, significance[, rate control mode])
 MIX = [
     (256, 64, 10, 3, 3, 2, 5, 0, 1),
     (200, 200, 8, 4, 3, 1, 5, 0, 1),
@@ -17,17 +17,21 @@ MIX = [
     (130, 66, 9, 3, 4, 1, 2, 0, 1),
     (128, 64, 11, 4, 3, 0, 3, 0, 1),
     (160, 90, 10, 3, 5, 2, 5, 0, 0),
+    (256, 64, 10, 3, 3, 2, 5, 0, 1, 0),
+    (200, 200, 8, 4, 3, 1, 5, 0, 1, 0),
 ]
 
 
 def _setup(torch, dev):
     S, O, P = load_synth(), load_oracle(), load_package()
     items = []
-    for k, (W, H, D, F, bpp, V, Hh, q, sig) in enumerate(MIX):
+    for k, cfg in enumerate(MIX):
+        W, H, D, F, bpp, V, Hh, q, sig = cfg[:9]
+        rcm = cfg[9] if len(cfg) > 9 else 1
         planes = S.make_frame(W, H, D, F, frame=20 + k)
-        e, ref = O.encode(O.make_params(W, H, D, F, bpp, 1, V, Hh, q, 16, sig), planes)
+        e, ref = O.encode(O.make_params(W, H, D, F, bpp, 1, V, Hh, q, 16, sig, rcm), planes)
         assert e == 0
-        enc = P.Encoder(W, H, D, F, bpp, 1, V, Hh, q, 16, sig)
+        enc = P.Encoder(W, H, D, F, bpp, 1, V, Hh, q, 16, sig, rate_control_mode=rcm)
         d_in = [torch.from_numpy(p.view(np.int16) if p.dtype == np.uint16 else p).to(dev) for p in planes]
         d_out = torch.zeros((enc.bytes_per_frame + 3) & ~3, dtype=torch.uint8, device=dev)
         items.append((enc, d_in, d_out, ref))

@@ -78,7 +78,8 @@ def _gpu_encode(p):
     P, S = load_package(), load_synth()
     planes = S.make_frame(p["width"], p["height"], p["depth"], p["format"], frame=p["frame"], sigma=p["sigma"])
     enc = P.Encoder(p["width"], p["height"], p["depth"], p["format"], p["bpp_num"], p["bpp_den"], p["decomp_v"],
-                    p["decomp_h"], p["quantization"], p["slice_height"], p["significance"])
+                    p["decomp_h"], p["quantization"], p["slice_height"], p["significance"],
+                    rate_control_mode=p.get("rate_control_mode", 1))
     rc, cs = enc.encode(planes)
     enc.close()
     return rc, cs
@@ -145,3 +146,41 @@ def test_slice_packetization_packets():
         assert bytes(out[pos - 2:pos]) == b"\xff\x11"
         assert out[:pos].tobytes() == ref
     enc.close()
+
+
+# (width, height, depth, format, bpp, decomp_v, decomp_h, slice_height)
+RC0 = [
+    (1920, 1080, 8, 3, 5, 2, 5, 16),
+    (640, 360, 10, 4, 3, 1, 5, 32),
+    (352, 288, 12, 2, 2, 2, 4, 16),
+    (96, 64, 14, 21, 2, 1, 5, 16),
+    (250, 130, 9, 3, 1, 0, 2, 8),
+]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("cfg", RC0, ids=lambda c: "x".join(str(v) for v in c))
+def test_rate_control_mode0_bit_exact(cfg):
+    """Rate control mode 0 (budget per precinct, padding kept in each
+    precinct) against the oracle's restatement, which the reference's mode-0
+    codestreams pin (tests/golden *_rc0)."""
+    print("rc0", cfg, flush=True)
+    W, H, D, F, bpp, V, Hh, sh = cfg
+    S, O, P = load_synth(), load_oracle(), load_package()
+    planes = S.make_frame(W, H, D, F, frame=3, sigma=6.0)
+    e_ref, ref = O.encode(O.make_params(W, H, D, F, bpp, 1, V, Hh, 0, sh, 1, 0), planes)
+    enc = P.Encoder(W, H, D, F, bpp, 1, V, Hh, 0, sh, 1, rate_control_mode=0)
+    rc, got = enc.encode(planes)
+    enc.close()
+    assert (rc == 0) == (e_ref == 0)
+    if e_ref == 0:
+        assert got == ref
+
+
+@pytest.mark.gpu
+def test_rate_control_modes_reject_per_slice_modes():
+    """Modes 2 and 3 (budget per slice) have no implementation: init refuses them."""
+    P = load_package()
+    for mode in (2, 3, 4):
+        with pytest.raises(P.JxsError):
+            P.Encoder(64, 32, 8, 3, 3, rate_control_mode=mode, verbose=0)

@@ -24,7 +24,8 @@ def sha(b):
 def _params(p):
     O = load_oracle()
     return O.make_params(p["width"], p["height"], p["depth"], p["format"], p["bpp_num"], p["bpp_den"], p["decomp_v"],
-                         p["decomp_h"], p["quantization"], p["slice_height"], p["significance"])
+                         p["decomp_h"], p["quantization"], p["slice_height"], p["significance"],
+                         p.get("rate_control_mode", 1))
 
 
 def _planes(p):
