@@ -34,6 +34,9 @@
 #define WAVE 64
 #define GAS __attribute__((address_space(1)))
 
+typedef int32_t i32x4 __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+
 /* a load through the global address space (not a flat load) */
 template <typename T>
 __device__ __forceinline__ T gld(const T* p) {
@@ -597,7 +600,7 @@ __device__ __forceinline__ int find_base(const uint32_t* bases, int n, uint32_t 
 
 __global__ void __launch_bounds__(256) k_inv_rows(const JxsInvRowJob* __restrict__ jobs, const uint32_t* __restrict__ bases,
                                                   int njobs) {
-    __shared__ int32_t buf[JXS_MAX_ROW];
+    __shared__ __align__(16) int32_t buf[JXS_MAX_ROW];
     const JxsInvRowJob& J = jobs[find_base(bases, njobs, blockIdx.x)];
     const uint32_t y = blockIdx.x - J.base;
     uint32_t n[6];
@@ -690,13 +693,36 @@ __global__ void __launch_bounds__(256) k_inv_rows(const JxsInvRowJob* __restrict
         }
         __syncthreads();
     }
+    /* four consecutive samples per thread and store where the row allows it */
+    const uint32_t w = J.w, n4 = w / 4;
     if (J.out32) {
         int32_t* o = J.out32 + (size_t)y * J.out_stride;
-        for (uint32_t i = threadIdx.x; i < J.w; i += blockDim.x)
+        uint32_t done = 0;
+        if (((uintptr_t)o & 15) == 0) {
+            for (uint32_t q = threadIdx.x; q < n4; q += blockDim.x)
+                *(GAS i32x4*)(o + 4 * q) = *(const i32x4*)&buf[4 * q];
+            done = 4 * n4;
+        }
+        for (uint32_t i = done + threadIdx.x; i < w; i += blockDim.x)
             o[i] = buf[i];
     } else {
-        for (uint32_t i = threadIdx.x; i < J.w; i += blockDim.x)
-            store_out(J.out, (size_t)y * J.out_stride + i, buf[i], J.depth);
+        const size_t r0 = (size_t)y * J.out_stride;
+        const bool b8 = J.depth <= 8;
+        uint32_t done = 0;
+        if (b8 ? (((uintptr_t)((uint8_t*)J.out + r0) & 3) == 0) : (((uintptr_t)((uint16_t*)J.out + r0) & 7) == 0)) {
+            for (uint32_t q = threadIdx.x; q < n4; q += blockDim.x) {
+                const i32x4 v = *(const i32x4*)&buf[4 * q];
+                const uint32_t a = nlt_out(v.x, J.depth), b = nlt_out(v.y, J.depth), c = nlt_out(v.z, J.depth),
+                               d = nlt_out(v.w, J.depth);
+                if (b8)
+                    *(GAS uint32_t*)((uint8_t*)J.out + r0 + 4 * q) = a | b << 8 | c << 16 | d << 24;
+                else
+                    *(GAS u32x2*)((uint16_t*)J.out + r0 + 4 * q) = u32x2{a | b << 16, c | d << 16};
+            }
+            done = 4 * n4;
+        }
+        for (uint32_t i = done + threadIdx.x; i < w; i += blockDim.x)
+            store_out(J.out, r0 + i, buf[i], J.depth);
     }
 }
 

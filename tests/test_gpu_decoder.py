@@ -17,6 +17,11 @@ CASES = [
     (96, 64, 14, 21, 2, 1, 5, 0, 1),
     (128, 64, 11, 4, 3, 0, 3, 0, 1),
     (1024, 768, 10, 3, 20, 2, 5, 0, 1),
+    # no vertical level, rows written straight to the output plane: 8-bit
+    # (4-sample stores), odd widths (per-sample tails), 16-bit
+    (96, 40, 8, 3, 3, 0, 2, 0, 1),
+    (131, 37, 8, 4, 4, 0, 1, 0, 1),
+    (250, 30, 12, 3, 3, 0, 4, 0, 1),
 ]
 
 
