@@ -7,12 +7,14 @@
  *   k_dec_packets one lane per precinct: band D-modes and GTLIs, and the
  *                 packet chain walked once into a table of sub-packet
  *                 offsets.
- *   k_dec_unpack  one block per (packet, precinct): the significance flags, the unary bit-plane counts decoded in
- *                 parallel (code j ends at the j-th zero bit: zero counts per
- *                 32-bit word + block prefix sum), then the data nibbles at
- *                 prefix-summed bit offsets, dequantised and turned into
- *                 two's complement (Packing.c:513-868, Dequant.c:9-57,
- *                 Precinct.c:9-29).
+ *   k_dec_unpack  one block per (packet, precinct): the packet's band lines
+ *                 decoded as one group sequence, 2048 groups per pass: the
+ *                 significance flags, the unary bit-plane counts located all
+ *                 at once (code j ends at the j-th zero bit: zero counts per
+ *                 32-bit word + block prefix sum), then each group's data bits
+ *                 at prefix-summed offsets, read as one 64-bit window,
+ *                 dequantised and turned into two's complement
+ *                 (Packing.c:513-868, Dequant.c:9-57, Precinct.c:9-29).
  *   k_inv_rows    one block per row: every band sample of the row loaded at
  *                 once (one memory round trip), all inverse horizontal levels in
  *                 LDS (Idwt.c:11-53 semantics), into an int32 row or, for
@@ -22,7 +24,8 @@
  *                 output scaling fused when it produces the final plane.
  *
  * Every codestream read is bounds-checked: a corrupt or truncated
- * codestream sets the frame's status word and never reads outside it.
+ * codestream sets the frame's status word, and no read leaves the aligned
+ * 32-bit words that hold the codestream's bytes.
  */
 #include <hip/hip_runtime.h>
 #include <stdint.h>
