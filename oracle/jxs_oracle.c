@@ -60,7 +60,8 @@ typedef struct {
     uint8_t maxq, maxr;
     /* encoder side */
     uint32_t depth, format, lcod, qpih, sig, hdr_len;
-    uint32_t rc_mode; /* 0: budget per precinct; 1: unused padding moves to the next precinct of the slice */
+    uint32_t rc_mode; /* 0: budget per precinct; 1: unused padding moves to the next precinct of the slice;
+                         2, 3: one Q and R for the slice sizes the precinct budgets (3: with a minimum size) */
     uint8_t hdr[512];
     uint32_t* slice_sizes;
 } ob_pi;
@@ -536,7 +537,7 @@ static int32_t enc_configure(const jxso_params_t* prm, ob_pi* pi) {
     uint32_t nc, sx[4], sy[4];
     if (prm->depth < 8 || prm->depth > 14)
         return E_BAD_PARAM;
-    if (prm->rc_mode > 1) /* the per-slice modes 2 and 3 are not restated */
+    if (prm->rc_mode > 3)
         return E_BAD_PARAM;
     if (sampling(prm->format, &nc, sx, sy))
         return E_BAD_PARAM;
@@ -1215,6 +1216,136 @@ static void pack_precinct(const ob_pi* pi, const enc_frame* f, prec_state* ps, u
     w->bit += 8ull * pad_bytes; /* zero padding */
 }
 
+/* Precinct budgets of one slice for rate control modes 2 and 3
+ * (PackStageProcess.c:340-481): the data budget (the slice's bytes less
+ * every precinct's headers, RateControl.c:258-271) decides one Q and then one
+ * R for all the slice's precincts with the per-precinct searches' binary
+ * search on the summed bytes (RateControl.c:1086-1210, 1336-1425); each
+ * precinct gets its headers and data at (Q, R), the last one the leftover.
+ * Mode 3 raises every precinct to a quarter of the largest and takes the
+ * bytes back from the leftover, else one byte at a time from the first
+ * precincts on.  The leftover then goes 10 % of the first precinct's bytes
+ * to the first one and the rest evenly, the first precincts one byte more. */
+static int32_t slice_budgets(const ob_pi* pi, const enc_frame* f, uint32_t first_row, uint32_t nprec, uint32_t sbudget,
+                             uint32_t* bud) {
+    if ((sbudget + nprec - 1) / nprec > 1048575u)
+        return E_ENCODE_FRAME;
+    prec_state* st = (prec_state*)calloc(nprec, sizeof(prec_state));
+    uint32_t* pad = (uint32_t*)calloc(nprec, sizeof(uint32_t));
+    int32_t err = E_NONE;
+    uint32_t hsum = 0;
+    for (uint32_t i = 0; i < nprec; i++) {
+        st[i].row = first_row + i;
+        st[i].type = (st[i].row + 1 >= pi->rows) ? 1 : 0;
+        build_luts(pi, f, &st[i]);
+        bud[i] = bytes_of_bits(40 + pi->nb_exist * 2) +
+            bytes_of_bits((pi->short_hdr ? 40u : 56u) * (uint32_t)pi->pk_exist[st[i].type]);
+        hsum += bud[i];
+    }
+    uint32_t q = 0, r = 0;
+    if (sbudget <= hsum) {
+        err = E_ENCODE_FRAME;
+    } else {
+        const uint64_t avail = sbudget - hsum;
+        jxso_bsearch_t s;
+        uint32_t step = 6;
+        if (step > pi->maxq)
+            step = 0;
+        jxso_bsearch_init(&s, 0, pi->maxq, 0, step);
+        int32_t res, next = 0;
+        while ((res = jxso_bsearch_next(&s, next, &q)) == 0) {
+            if (all_truncated(pi, (uint8_t)q, 0)) {
+                next = 3;
+                continue;
+            }
+            uint64_t sum = 0;
+            for (uint32_t i = 0; i < nprec; i++)
+                sum += precinct_bytes(pi, &st[i], (uint8_t)q, 0);
+            next = sum > avail ? 1 : 2;
+        }
+        if (res != 1)
+            err = E_ENCODE_FRAME;
+        if (!err) {
+            step = 6;
+            if (step >= pi->maxr)
+                step = 0;
+            jxso_bsearch_init(&s, 0, pi->maxr, 1, step);
+            next = 0;
+            while ((res = jxso_bsearch_next(&s, next, &r)) == 0) {
+                if (all_truncated(pi, (uint8_t)q, (uint8_t)r)) {
+                    next = 3;
+                    continue;
+                }
+                uint64_t sum = 0;
+                for (uint32_t i = 0; i < nprec; i++)
+                    sum += precinct_bytes(pi, &st[i], (uint8_t)q, (uint8_t)r);
+                next = sum <= avail ? 1 : 2;
+            }
+            if (res != 1)
+                err = E_ENCODE_FRAME;
+        }
+        if (!err) {
+            uint32_t left = (uint32_t)avail;
+            for (uint32_t i = 0; i < nprec; i++) {
+                const uint32_t d = precinct_bytes(pi, &st[i], (uint8_t)q, (uint8_t)r);
+                left -= d;
+                bud[i] += d;
+            }
+            const uint32_t last = nprec - 1;
+            pad[last] = left;
+            bud[last] += left;
+            if (pi->rc_mode == 3) {
+                uint32_t mx = 0, get = 0;
+                for (uint32_t i = 0; i < nprec; i++)
+                    if (bud[i] - pad[i] > mx)
+                        mx = bud[i] - pad[i];
+                const uint32_t mn = mx / 4;
+                for (uint32_t i = 0; i < nprec; i++) {
+                    const uint32_t ds = bud[i] - pad[i];
+                    if (ds < mn) {
+                        bud[i] += mn - ds;
+                        get += mn - ds;
+                    }
+                }
+                if (get) {
+                    if (pad[last] >= get) {
+                        pad[last] -= get;
+                        bud[last] -= get;
+                    } else {
+                        get -= pad[last];
+                        bud[last] -= pad[last];
+                        pad[last] = 0;
+                        while (get)
+                            for (uint32_t i = 0; i < nprec && get; i++) {
+                                get--;
+                                bud[i]--;
+                            }
+                    }
+                }
+            }
+            if (pad[last] > 0) {
+                uint32_t add_first = bud[0] * 10 / 100;
+                if (add_first > pad[last])
+                    add_first = pad[last];
+                pad[last] -= add_first;
+                bud[last] -= add_first;
+                bud[0] += add_first;
+                if (pad[last]) {
+                    const uint32_t lb = pad[last];
+                    bud[last] -= lb;
+                    pad[last] = 0;
+                    const uint32_t mb = lb / nprec, rem = lb - mb * nprec;
+                    for (uint32_t i = 0; i < nprec; i++)
+                        bud[i] += mb + (i < rem ? 1u : 0u);
+                }
+            }
+        }
+    }
+    free(pad);
+    free(st);
+    return err;
+}
+
 /* One whole frame: PreRcStageProcess.c:31-116 (slice byte ranges),
  * PackStageProcess.c:567-749 (per-slice precinct budgets; with rate control
  * mode 1 the first precinct gets 10 % more and each precinct's unused bytes
@@ -1254,9 +1385,18 @@ static int32_t enc_code(const ob_pi* pi, const enc_frame* f, uint8_t* out, uint8
                 left = 0;
             }
         }
+        uint32_t* sbud = NULL;
+        if (pi->rc_mode >= 2) {
+            sbud = (uint32_t*)calloc(nprec, sizeof(uint32_t));
+            if (slice_budgets(pi, f, first_row, nprec, sbudget, sbud)) {
+                free(sbud);
+                err = E_ENCODE_FRAME;
+                break;
+            }
+        }
         uint32_t carry = 0;
         for (uint32_t i = 0; i < nprec; i++) {
-            uint32_t budget = (i == 0 ? first : minb) + (i < left ? 1 : 0) + carry;
+            uint32_t budget = (sbud ? sbud[i] : (i == 0 ? first : minb) + (i < left ? 1 : 0)) + carry;
             ps->row = first_row + i;
             ps->type = (ps->row + 1 >= pi->rows) ? 1 : 0;
             build_luts(pi, f, ps);
@@ -1271,7 +1411,7 @@ static int32_t enc_code(const ob_pi* pi, const enc_frame* f, uint8_t* out, uint8
             }
             uint32_t pad = budget - hdr - data;
             uint32_t total = budget;
-            if (pi->rc_mode == 1 && i + 1 < nprec) {
+            if (pi->rc_mode != 0 && i + 1 < nprec) {
                 total -= pad;
                 carry = pad;
                 pad = 0;
@@ -1282,6 +1422,7 @@ static int32_t enc_code(const ob_pi* pi, const enc_frame* f, uint8_t* out, uint8
             }
             pack_precinct(pi, f, ps, q, r, total, pad, &w);
         }
+        free(sbud);
         pos = w.bit >> 3;
         if (s == pi->slices - 1 && !err) {
             out[pos] = 0xFF;

@@ -124,6 +124,20 @@ def make_frame(width, height, depth, fmt, frame=0, base=BASE_SEED, sigma=4.0):
     return planes
 
 
+def flatten_rows(planes, height, period=16, flat=8):
+    """Uneven content for rate-control tests: in every `period` rows of the
+    picture the first `flat` are set to the plane's middle value (a plane of
+    fewer rows is subsampled: its row y is picture row y * height / rows)."""
+    out = []
+    for p in planes:
+        p = p.copy()
+        sy = height // p.shape[0]
+        rows = (np.arange(p.shape[0]) * sy) % period < flat
+        p[rows] = (int(p.max()) + int(p.min())) // 2
+        out.append(p)
+    return out
+
+
 def frame_bytes(planes):
     return b"".join(np.ascontiguousarray(p).tobytes() for p in planes)
 

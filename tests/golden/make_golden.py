@@ -48,6 +48,20 @@ SMALL = {
     "s_256x64_422_10b_bpp3_v2_rc0": (256, 64, 10, 3, 3, 1, 2, 5, 0, 16, 1, 4.0, 1, 0),
     "s_200x200_444_8b_bpp3_v1_rc0": (200, 200, 8, 4, 3, 1, 1, 5, 0, 16, 1, 4.0, 2, 0),
     "s_320x180_422_10b_bpp7_2_slice32_rc0": (320, 180, 10, 3, 7, 2, 2, 5, 0, 32, 1, 4.0, 11, 0),
+    # rate control modes 2 and 3: one Q and R per slice sizes the precinct budgets
+    "s_256x64_422_10b_bpp3_v2_rc2": (256, 64, 10, 3, 3, 1, 2, 5, 0, 16, 1, 4.0, 1, 2),
+    "s_200x200_444_8b_bpp3_v1_rc2": (200, 200, 8, 4, 3, 1, 1, 5, 0, 16, 1, 4.0, 2, 2),
+    "s_320x180_422_10b_bpp7_2_slice32_rc2": (320, 180, 10, 3, 7, 2, 2, 5, 0, 32, 1, 4.0, 11, 2),
+    "s_256x64_422_10b_bpp3_v2_rc3": (256, 64, 10, 3, 3, 1, 2, 5, 0, 16, 1, 4.0, 1, 3),
+    "s_200x200_444_8b_bpp3_v1_rc3": (200, 200, 8, 4, 3, 1, 1, 5, 0, 16, 1, 4.0, 2, 3),
+    "s_320x180_422_10b_bpp7_2_slice32_rc3": (320, 180, 10, 3, 7, 2, 2, 5, 0, 32, 1, 4.0, 11, 3),
+    "s_96x64_rgbpacked_14b_bpp2_v1_rc3": (96, 64, 14, 21, 2, 1, 1, 5, 0, 16, 1, 16.0, 8, 3),
+    # uneven content (the first 8 rows of every 16 flat, synth.flatten_rows), so
+    # that mode 3 raises the small precincts to a quarter of the largest
+    "u_256x64_422_10b_bpp3_v2_rc2": (256, 64, 10, 3, 3, 1, 2, 5, 0, 16, 1, 8.0, 1, 2),
+    "u_256x64_422_10b_bpp3_v2_rc3": (256, 64, 10, 3, 3, 1, 2, 5, 0, 16, 1, 8.0, 1, 3),
+    "u_200x96_444_8b_bpp2_v1_rc3": (200, 96, 8, 4, 2, 1, 1, 5, 0, 16, 1, 8.0, 2, 3),
+    "u_320x96_420_8b_bpp2_v2_slice32_rc3": (320, 96, 8, 2, 2, 1, 2, 5, 0, 32, 1, 8.0, 3, 3),
 }
 
 # The configurations of the benchmark workloads, first frame of each.
@@ -101,11 +115,15 @@ def case_entry(name, cfg, td, keep):
     S = load_synth()
     W, H, D, F, bn, bd, V, Hh, q, sh, sig, sigma, frame = cfg[:13]
     planes = S.make_frame(W, H, D, F, frame=frame, sigma=sigma)
+    if name.startswith("u_"):
+        planes = S.flatten_rows(planes, H)
     cs, rc = ref_encode(cfg, planes, td)
     e = {"params": dict(zip(["width", "height", "depth", "format", "bpp_num", "bpp_den", "decomp_v", "decomp_h",
                              "quantization", "slice_height", "significance", "sigma", "frame"], cfg[:13])),
          "input_sha256": S.frame_sha256(planes)}
     e["params"]["rate_control_mode"] = cfg[13] if len(cfg) > 13 else 1
+    if name.startswith("u_"):
+        e["params"]["pattern"] = "flatten_rows"
     if cs is None:
         e["reference_error"] = True
         return e

@@ -77,6 +77,8 @@ with open(os.path.join(_GOLDEN, "golden.json")) as _f:
 def _gpu_encode(p):
     P, S = load_package(), load_synth()
     planes = S.make_frame(p["width"], p["height"], p["depth"], p["format"], frame=p["frame"], sigma=p["sigma"])
+    if p.get("pattern") == "flatten_rows":
+        planes = S.flatten_rows(planes, p["height"])
     enc = P.Encoder(p["width"], p["height"], p["depth"], p["format"], p["bpp_num"], p["bpp_den"], p["decomp_v"],
                     p["decomp_h"], p["quantization"], p["slice_height"], p["significance"],
                     rate_control_mode=p.get("rate_control_mode", 1))

@@ -30,7 +30,10 @@ def _params(p):
 
 def _planes(p):
     S = load_synth()
-    return S.make_frame(p["width"], p["height"], p["depth"], p["format"], frame=p["frame"], sigma=p["sigma"])
+    planes = S.make_frame(p["width"], p["height"], p["depth"], p["format"], frame=p["frame"], sigma=p["sigma"])
+    if p.get("pattern") == "flatten_rows":
+        planes = S.flatten_rows(planes, p["height"])
+    return planes
 
 
 # ---------------------------------------------------------------- generator
