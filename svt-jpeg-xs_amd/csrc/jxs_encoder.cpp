@@ -398,12 +398,13 @@ static SvtJxsErrorType_t encoder_create(svt_jpeg_xs_encoder_api_t* api) {
     p.significance = api->coding_significance;
     p.rc_mode = api->rate_control_mode;
     /* Coding tools outside this set have no GPU implementation yet: rate
-     * control modes 0 and 1 (budget per precinct, without or with padding
-     * movement), no vertical prediction, no sign handling. */
-    if (api->rate_control_mode > 1 || api->coding_vertical_prediction_mode != 0 || api->coding_signs_handling != 0) {
+     * control modes 0..3 (budget per precinct without or with padding
+     * movement, budget per slice without or with a minimum precinct size),
+     * no vertical prediction, no sign handling. */
+    if (api->rate_control_mode > 3 || api->coding_vertical_prediction_mode != 0 || api->coding_signs_handling != 0) {
         if (api->verbose >= VERBOSE_ERRORS)
             fprintf(stderr,
-                    "svt-jpeg-xs-amd: only rate_control_mode 0 or 1 without vertical prediction or sign handling is "
+                    "svt-jpeg-xs-amd: rate_control_mode 0..3 without vertical prediction or sign handling is "
                     "implemented\n");
         return SvtJxsErrorBadParameter;
     }

@@ -81,7 +81,8 @@ struct JxsGeom {
      * bytes of each line's packet; zero past the end */
     uint32_t rq[2][JXS_NU], rq_raw[2][JXS_NU];
     uint32_t rq_n[2];
-    uint32_t rc_mode; /* rate control: 0 budget per precinct, 1 unused padding moved to the next precinct */
+    uint32_t rc_mode; /* rate control: 0 budget per precinct, 1 unused padding moved to the next precinct,
+                         2 / 3 budgets from one (Q, R) per slice (3: with a minimum precinct size) */
     uint32_t pad_rq;
     uint64_t n_coeff, n_gcli, n_smax; /* buffer sizes per frame */
     uint64_t n_i32;                   /* int32 transform scratch per frame */

@@ -1307,6 +1307,59 @@ __device__ __forceinline__ void plan_precinct(const JxsFrameDesc& F, const JxsGe
             gst(o8 + k, g.hdr[k]);
 }
 
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1)
+        v += (uint32_t)__shfl_xor((int)v, o, WAVE);
+    return v;
+}
+
+/* Precinct budgets of a slice under rate control modes 2 and 3
+ * (PackStageProcess.c:340-481), by wave 0 with one lane per precinct, from
+ * ds_i = headers + data bytes at the slice's (Q, R), stored in the precincts'
+ * `total` words: the leftover of the slice's data budget goes to the last
+ * precinct; mode 3 raises every precinct below a quarter of the largest ds
+ * to that quarter, taking the bytes from the leftover, else one byte at a
+ * time from the first precincts on (the closed form: D / n each, one more
+ * for the first D mod n); what is left then goes 10 % of the first
+ * precinct's bytes to the first precinct and the rest evenly, one byte more
+ * for the first ones.  The budgets replace the `total` words. */
+__device__ void slice_budgets(const JxsFrameDesc& F, uint32_t first_row, uint32_t nprec, uint32_t mode, uint32_t left,
+                              uint32_t dmax, uint32_t lane) {
+    const uint32_t mn = mode == 3 ? dmax / 4 : 0u;
+    uint32_t get = 0;
+    for (uint32_t i = lane; i < nprec; i += WAVE) {
+        const uint32_t ds = gld(&F.prec[first_row + i].total);
+        get += ds < mn ? mn - ds : 0u;
+    }
+    get = uni(wave_sum(get));
+    uint32_t pad1 = left, dec = 0;
+    if (get) {
+        if (left >= get) {
+            pad1 = left - get;
+        } else {
+            pad1 = 0;
+            dec = get - left;
+        }
+    }
+    const uint32_t dq = dec / nprec, dr = dec - dq * nprec;
+    /* the first precinct after the mode-3 step, for its 10 % */
+    const uint32_t ds0 = uni(gld(&F.prec[first_row].total));
+    const uint32_t b0 = ds0 + (ds0 < mn ? mn - ds0 : 0u) - (dq + (0 < dr ? 1u : 0u)) + (nprec == 1 ? pad1 : 0u);
+    uint32_t af = 0, lb = 0;
+    if (pad1 > 0) {
+        af = min(b0 * 10 / 100, pad1);
+        lb = pad1 - af;
+    }
+    const uint32_t lq = lb / nprec, lr = lb - lq * nprec;
+    for (uint32_t i = lane; i < nprec; i += WAVE) {
+        const uint32_t ds = gld(&F.prec[first_row + i].total);
+        const uint32_t b = ds + (ds < mn ? mn - ds : 0u) - (dq + (i < dr ? 1u : 0u)) + (i == 0 ? af : 0u) + lq +
+            (i < lr ? 1u : 0u);
+        gst(&F.prec[first_row + i].total, b);
+    }
+}
+
 /* One block per slice (RT_WAVES waves), the rate control and packet plan of
  * its precincts, RT_WAVES precincts at a time:
  *   1. each wave builds one precinct's rate table from the histogram records,
@@ -1360,11 +1413,13 @@ __global__ void __launch_bounds__(WAVE * RT_WAVES) k_rate(const JxsFrameDesc* __
     const uint32_t eq = (uint32_t)__ballot(em & 1u); /* bit q: (q, 0) truncates every band away */
     const uint32_t sbud = gld(F.slice_bud + s); /* global, not flat: a uniform value */
     /* mode 1: the first precinct 10 % more, the rest split evenly
-     * (PackStageProcess.c:621-640); mode 0: an even split (600-601) */
-    const bool move_pad = uni(g.rc_mode) == 1;
+     * (PackStageProcess.c:621-640); mode 0: an even split (600-601); modes 2
+     * and 3: budgets from one (Q, R) for the slice, below */
+    const uint32_t rc_mode = uni(g.rc_mode);
+    const bool move_pad = rc_mode != 0;
     uint32_t minb = sbud / nprec;
     uint32_t first = minb, left = sbud - minb * nprec;
-    if (move_pad) {
+    if (rc_mode == 1) {
         first = minb * 110 / 100;
         if (first > sbud)
             first = sbud;
@@ -1376,6 +1431,73 @@ __global__ void __launch_bounds__(WAVE * RT_WAVES) k_rate(const JxsFrameDesc* __
     }
     uint32_t carry = 0, off = gld(F.slice_off + s) + 6;
     bool failed = false;
+    if (rc_mode >= 2) {
+        /* one Q, then one R, for the whole slice by the per-precinct binary
+         * searches on the slice's summed bytes (RateControl.c:1336-1425):
+         * pass 0 sums the (q, 0) columns, pass 1 the (Q, r) rows, pass 2
+         * stores every precinct's headers + data bytes at (Q, R) */
+        const uint32_t has_last = first_row + nprec >= rows ? 1u : 0u;
+        const uint32_t hsum = (nprec - has_last) * g.rc_hdr_bytes[0] + has_last * g.rc_hdr_bytes[1];
+        bool ok = sbud > hsum && (sbud + nprec - 1) / nprec <= 1048575u;
+        const uint32_t avail = ok ? sbud - hsum : 0u;
+        uint32_t Q = 0, R = 0;
+        for (uint32_t pass = 0; pass < 3 && ok; pass++) {
+            uint32_t acc = 0, dmax = 0;
+            for (uint32_t i0 = 0; i0 < nprec; i0 += RT_WAVES) {
+                const uint32_t n = min((uint32_t)RT_WAVES, nprec - i0);
+                if (wid < n) {
+                    const uint32_t i = i0 + wid, row = first_row + i, type = row + 1 >= rows ? 1 : 0;
+                    const Rec rp = i0 == 0 ? rc : load_rec(F.hist + (size_t)row * n_bl * JXS_HREC, rec_line);
+                    RateTab& T = tabs[wid];
+                    rate_lines(T, rp, n_bl, li, type, g.sig, lane);
+                    wave_sync();
+                    const uint32_t rq = type ? rq1 : rq0, rw = type ? rw1 : rw0, nl = g.rq_n[type];
+                    uint32_t v;
+                    if (pass == 0) {
+                        v = rate_query(rq, rw, nl, T, lane & (JXS_NQ - 1), 0);
+                    } else if (pass == 1) {
+                        v = rate_query(rq, rw, nl, T, Q, lane & (JXS_NR - 1));
+                    } else {
+                        v = uni(rate_query(rq, rw, nl, T, Q, R)) + uni(g.rc_hdr_bytes[type]);
+                        if (lane == 0)
+                            gst(&F.prec[row].total, v);
+                    }
+                    col[wid][lane] = v;
+                }
+                __syncthreads();
+                if (wid == 0)
+                    for (uint32_t k = 0; k < n; k++) {
+                        const uint32_t v = col[k][lane];
+                        acc += v;
+                        dmax = max(dmax, v);
+                    }
+                __syncthreads();
+            }
+            if (wid == 0) {
+                if (pass == 0) {
+                    ok = search_q(maxq, eq, acc, avail, Q);
+                } else if (pass == 1) {
+                    ok = search_r(maxr, lane_of(em, Q), acc, 0, avail, R);
+                } else {
+                    /* acc: every lane the slice's headers + data bytes */
+                    slice_budgets(F, first_row, nprec, rc_mode, avail + hsum - uni(acc), uni(dmax), lane);
+                }
+                if (lane == 0)
+                    spec[0] = (ok ? 1u : 0u) | Q << 8 | R << 16;
+            }
+            __syncthreads();
+            const uint32_t sp = uni(spec[0]);
+            ok = (sp & 1u) != 0;
+            Q = (sp >> 8) & 0xFFu;
+            R = (sp >> 16) & 0xFFu;
+            __syncthreads();
+        }
+        if (!ok) {
+            failed = true;
+            if (wid == 0 && lane == 0)
+                gor(F.status, 1u);
+        }
+    }
     for (uint32_t i0 = 0; i0 < nprec; i0 += RT_WAVES) {
         const uint32_t n = min((uint32_t)RT_WAVES, nprec - i0);
         if (wid < n) {
@@ -1393,7 +1515,7 @@ __global__ void __launch_bounds__(WAVE * RT_WAVES) k_rate(const JxsFrameDesc* __
              * a carry only ever adds bytes, and rarely more than one step of
              * Q's worth */
             const uint32_t hdr = g.rc_hdr_bytes[type];
-            const uint32_t b0 = (i == 0 ? first : minb) + (i < left ? 1 : 0);
+            const uint32_t b0 = rc_mode >= 2 ? uni(gld(&F.prec[row].total)) : (i == 0 ? first : minb) + (i < left ? 1 : 0);
             uint32_t q0 = 0;
             if (!(b0 <= 1048575u && b0 > hdr && search_q(maxq, eq, cq, b0 - hdr, q0)))
                 q0 = 0;
@@ -1414,7 +1536,8 @@ __global__ void __launch_bounds__(WAVE * RT_WAVES) k_rate(const JxsFrameDesc* __
                 P.offset = off;
                 P.err = 1;
                 if (!failed) {
-                    const uint32_t budget = uni((i == 0 ? first : minb) + (i < left ? 1 : 0) + carry);
+                    const uint32_t base = rc_mode >= 2 ? uni(gld(&F.prec[row].total)) : (i == 0 ? first : minb) + (i < left ? 1 : 0);
+                    const uint32_t budget = uni(base + carry);
                     const uint32_t hdr = uni(g.rc_hdr_bytes[type]);
                     bool ok = budget <= 1048575u && budget > hdr;
                     uint32_t q = 0, r = 0;

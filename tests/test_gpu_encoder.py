@@ -180,9 +180,31 @@ def test_rate_control_mode0_bit_exact(cfg):
 
 
 @pytest.mark.gpu
-def test_rate_control_modes_reject_per_slice_modes():
-    """Modes 2 and 3 (budget per slice) have no implementation: init refuses them."""
+@pytest.mark.parametrize("mode", [2, 3])
+@pytest.mark.parametrize("cfg", RC0, ids=lambda c: "x".join(str(v) for v in c))
+def test_rate_control_slice_modes_bit_exact(cfg, mode):
+    """Rate control modes 2 and 3 (one Q and R per slice sizing the precinct
+    budgets, 3 with a minimum precinct size) against the oracle, on even and
+    on uneven content (the first 8 rows of every 16 flat, where mode 3 raises
+    the small precincts), which the reference's codestreams pin
+    (tests/golden *_rc2, *_rc3)."""
+    print("rc%d" % mode, cfg, flush=True)
+    W, H, D, F, bpp, V, Hh, sh = cfg
+    S, O, P = load_synth(), load_oracle(), load_package()
+    for planes in (S.make_frame(W, H, D, F, frame=4, sigma=6.0),
+                   S.flatten_rows(S.make_frame(W, H, D, F, frame=5, sigma=8.0), H)):
+        e_ref, ref = O.encode(O.make_params(W, H, D, F, bpp, 1, V, Hh, 0, sh, 1, mode), planes)
+        enc = P.Encoder(W, H, D, F, bpp, 1, V, Hh, 0, sh, 1, rate_control_mode=mode)
+        rc, got = enc.encode(planes)
+        enc.close()
+        assert (rc == 0) == (e_ref == 0)
+        if e_ref == 0:
+            assert got == ref
+
+
+@pytest.mark.gpu
+def test_rate_control_mode_out_of_range_is_rejected():
     P = load_package()
-    for mode in (2, 3, 4):
+    for mode in (4, 7):
         with pytest.raises(P.JxsError):
             P.Encoder(64, 32, 8, 3, 3, rate_control_mode=mode, verbose=0)
