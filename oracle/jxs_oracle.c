@@ -1601,38 +1601,85 @@ static int32_t parse_header(const uint8_t* cs, uint32_t len, dec_hdr* d) {
     return E_NONE;
 }
 
-int32_t jxso_probe(const uint8_t* cs, uint32_t len, jxso_image_info_t* info) {
-    dec_hdr d;
-    int32_t e = parse_header(cs, len, &d);
-    if (e)
-        return e;
-    info->width = d.pi.w;
-    info->height = d.pi.h;
-    info->depth = d.depth;
-    info->comps = d.pi.nc;
-    for (uint32_t c = 0; c < d.pi.nc; c++) {
-        info->comp_w[c] = d.pi.c[c].w;
-        info->comp_h[c] = d.pi.c[c].h;
+/* Proxy decoding, pi_update_proxy_mode (Pi.c:567-626): half (1) or quarter
+ * (2) resolution drops that many vertical and horizontal levels from the
+ * picture and from every component; only the first packets of each
+ * precinct, which carry the remaining bands, are decoded (one packet, or
+ * four at half resolution with two vertical levels).  The remaining bands
+ * are the first 2V' + H' + 1 of each component, unchanged. */
+static int32_t proxy_geometry(const ob_pi* pi, uint32_t proxy, ob_comp out[4], uint32_t* npk_dec, uint32_t* w,
+                              uint32_t* h) {
+    *npk_dec = pi->npk;
+    *w = pi->w;
+    *h = pi->h;
+    for (uint32_t c = 0; c < pi->nc; c++)
+        out[c] = pi->c[c];
+    if (proxy == 0)
+        return E_NONE;
+    if (proxy > 2)
+        return E_BAD_PARAM;
+    *npk_dec = proxy == 1 && pi->dv != 1 ? 4u : 1u;
+    if (proxy > pi->dv || proxy > pi->dh)
+        return E_BAD_PARAM;
+    *w = (pi->w + (1u << proxy) - 1) >> proxy;
+    *h = (pi->h + (1u << proxy) - 1) >> proxy;
+    for (uint32_t c = 0; c < pi->nc; c++) {
+        if (proxy > out[c].dv || proxy > out[c].dh)
+            return E_BAD_PARAM;
+        out[c].w = (out[c].w + (1u << proxy) - 1) >> proxy;
+        out[c].h = (out[c].h + (1u << proxy) - 1) >> proxy;
+        out[c].dv -= proxy;
+        out[c].dh -= proxy;
+        out[c].nb = 2 * out[c].dv + out[c].dh + 1;
     }
     return E_NONE;
 }
 
+static void fill_info(const dec_hdr* d, const ob_comp comps[4], uint32_t w, uint32_t h, jxso_image_info_t* info) {
+    info->width = w;
+    info->height = h;
+    info->depth = d->depth;
+    info->comps = d->pi.nc;
+    for (uint32_t c = 0; c < d->pi.nc; c++) {
+        info->comp_w[c] = comps[c].w;
+        info->comp_h[c] = comps[c].h;
+    }
+}
+
+int32_t jxso_probe_proxy(const uint8_t* cs, uint32_t len, uint32_t proxy, jxso_image_info_t* info) {
+    dec_hdr d;
+    int32_t e = parse_header(cs, len, &d);
+    if (e)
+        return e;
+    ob_comp comps[4];
+    uint32_t npk_dec, w, h;
+    e = proxy_geometry(&d.pi, proxy, comps, &npk_dec, &w, &h);
+    if (e)
+        return e;
+    fill_info(&d, comps, w, h, info);
+    return E_NONE;
+}
+
+int32_t jxso_probe(const uint8_t* cs, uint32_t len, jxso_image_info_t* info) { return jxso_probe_proxy(cs, len, 0, info); }
+
 int32_t jxso_decode(const uint8_t* cs, uint32_t len, void* const planes[4], jxso_image_info_t* info) {
+    return jxso_decode_proxy(cs, len, 0, planes, info);
+}
+
+int32_t jxso_decode_proxy(const uint8_t* cs, uint32_t len, uint32_t proxy, void* const planes[4],
+                          jxso_image_info_t* info) {
     dec_hdr d;
     int32_t e = parse_header(cs, len, &d);
     if (e)
         return e;
     ob_pi* pi = &d.pi;
-    if (info) {
-        info->width = pi->w;
-        info->height = pi->h;
-        info->depth = d.depth;
-        info->comps = pi->nc;
-        for (uint32_t c = 0; c < pi->nc; c++) {
-            info->comp_w[c] = pi->c[c].w;
-            info->comp_h[c] = pi->c[c].h;
-        }
-    }
+    ob_comp ocomp[4];
+    uint32_t npk_dec, ow, oh;
+    e = proxy_geometry(pi, proxy, ocomp, &npk_dec, &ow, &oh);
+    if (e)
+        return e;
+    if (info)
+        fill_info(&d, ocomp, ow, oh, info);
     uint16_t* coef[4][NBC];
     uint8_t* gcl[4][NBC];
     for (uint32_t c = 0; c < pi->nc; c++)
@@ -1676,7 +1723,7 @@ int32_t jxso_decode(const uint8_t* cs, uint32_t len, void* const planes[4], jxso
                     if (dmode[c][b] & 1)
                         e = E_INVALID_BS; /* vertical prediction not produced by this encoder */
                 }
-            for (uint32_t p = 0; p < pi->npk && !e; p++) {
+            for (uint32_t p = 0; p < npk_dec && !e; p++) {
                 uint32_t l = pi->pk_line[p];
                 int any = 0;
                 for (uint32_t gi = pi->pk_bs[p]; gi < pi->pk_be[p]; gi++)
@@ -1798,7 +1845,7 @@ int32_t jxso_decode(const uint8_t* cs, uint32_t len, void* const planes[4], jxso
         const uint32_t dz = d.bw - d.depth;
         const int32_t maxv = (1 << d.depth) - 1;
         for (uint32_t c = 0; c < pi->nc; c++) {
-            const ob_comp* cp = &pi->c[c];
+            const ob_comp* cp = &ocomp[c]; /* the component as output (proxy: fewer levels, the same first bands) */
             comp_bands32 b32;
             for (uint32_t b = 0; b < cp->nb; b++) {
                 size_t n = (size_t)cp->b[b].w * cp->b[b].h;

@@ -52,6 +52,11 @@ typedef struct jxso_image_info {
 } jxso_image_info_t;
 int32_t jxso_probe(const uint8_t* cs, uint32_t len, jxso_image_info_t* info);
 int32_t jxso_decode(const uint8_t* cs, uint32_t len, void* const planes[4], jxso_image_info_t* info);
+/* The same at half (proxy 1) or quarter (proxy 2) resolution: the reference
+ * decoder's proxy_mode (Pi.c:567-626). */
+int32_t jxso_probe_proxy(const uint8_t* cs, uint32_t len, uint32_t proxy, jxso_image_info_t* info);
+int32_t jxso_decode_proxy(const uint8_t* cs, uint32_t len, uint32_t proxy, void* const planes[4],
+                          jxso_image_info_t* info);
 
 /* Stage dumps used by the tests to localise a mismatch (same encoder run):
  *  coeff: all bands of all components, band-major, u16 sign-magnitude,

@@ -166,11 +166,12 @@ def encode_dump(p, planes, total_coeffs, total_gcli, rows):
     return e, coeff[:sizes[0]], gcli[:sizes[1]], qr[:sizes[2]]
 
 
-def decode(cs):
-    """Returns (error_code, [planes])."""
+def decode(cs, proxy=0):
+    """Returns (error_code, [planes]); proxy 1 / 2 decodes at half / quarter resolution."""
     buf = np.frombuffer(cs, dtype=np.uint8).copy()
     info = ImageInfo()
-    e = lib().jxso_probe(buf.ctypes.data, len(buf), ctypes.byref(info))
+    e = lib().jxso_probe_proxy(ctypes.c_void_p(buf.ctypes.data), ctypes.c_uint32(len(buf)), ctypes.c_uint32(proxy),
+                               ctypes.byref(info))
     if e:
         return e, []
     dt = np.uint8 if info.depth <= 8 else np.uint16
@@ -178,7 +179,8 @@ def decode(cs):
     ptrs = (ctypes.c_void_p * 4)()
     for c, pl in enumerate(planes):
         ptrs[c] = pl.ctypes.data
-    e = lib().jxso_decode(buf.ctypes.data, len(buf), ptrs, ctypes.byref(info))
+    e = lib().jxso_decode_proxy(ctypes.c_void_p(buf.ctypes.data), ctypes.c_uint32(len(buf)), ctypes.c_uint32(proxy),
+                                ptrs, ctypes.byref(info))
     return e, planes
 
 

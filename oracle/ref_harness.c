@@ -14,8 +14,9 @@
  *   (timing of a longer sample; only the first pass's codestreams are written).
  *
  * Decoder (HARNESS_DECODER):
- *   ref_dec in.jxs out.raw frames threads
- *   Decodes `frames` concatenated codestreams and writes the decoded planes.
+ *   ref_dec in.jxs out.raw frames threads [proxy_mode]
+ *   Decodes `frames` concatenated codestreams and writes the decoded planes
+ *   (at half or quarter resolution with proxy_mode 1 or 2).
  */
 #include <stdint.h>
 #include <stdio.h>
@@ -215,7 +216,7 @@ int main(int argc, char** argv) {
     dec.verbose = VERBOSE_ERRORS;
     dec.threads_num = (uint32_t)atoi(argv[4]);
     dec.use_cpu_flags = CPU_FLAGS_C;
-    dec.proxy_mode = proxy_mode_full;
+    dec.proxy_mode = argc > 5 ? (proxy_mode_t)atoi(argv[5]) : proxy_mode_full;
     uint32_t frame_size = 0;
     SvtJxsErrorType_t e = svt_jpeg_xs_decoder_get_single_frame_size_with_proxy(data, total, NULL, &frame_size, 1, proxy_mode_full);
     if (e || frame_size == 0)

@@ -100,12 +100,12 @@ def ref_encode(cfg, planes, td):
     return open(out, "rb").read(), 0
 
 
-def ref_decode(cs, td):
+def ref_decode(cs, td, proxy=0):
     O = load_oracle()
     p = os.path.join(td, "d.jxs")
     o = os.path.join(td, "d.raw")
     open(p, "wb").write(cs)
-    r = subprocess.run([O.REF_DEC, p, o, "1", "1"], capture_output=True, text=True)
+    r = subprocess.run([O.REF_DEC, p, o, "1", "1", str(proxy)], capture_output=True, text=True)
     if r.returncode != 0:
         return None
     return open(o, "rb").read()
@@ -132,6 +132,10 @@ def case_entry(name, cfg, td, keep):
     dec = ref_decode(cs, td)
     e["decoded_sha256"] = sha(dec) if dec is not None else None
     if keep:
+        # proxy decoding at half and quarter resolution (None: the reference refuses it)
+        for proxy, key in ((1, "decoded_half_sha256"), (2, "decoded_quarter_sha256")):
+            pd = ref_decode(cs, td, proxy)
+            e[key] = sha(pd) if pd is not None else None
         open(os.path.join(HERE, name + ".jxs"), "wb").write(cs)
     return e
 

@@ -192,3 +192,26 @@ def test_reference_sample_codestream_header():
     assert (info.comp_w[1], info.comp_h[1]) == (8, 16)
     e, _ = O.decode(cs)
     assert e == -0x7FFFCFFF  # SvtJxsErrorDecoderInvalidBitstream
+
+
+@pytest.mark.parametrize("proxy", [1, 2])
+@pytest.mark.parametrize("name", sorted(n for n in G["small"] if "decoded_half_sha256" in G["small"][n]))
+def test_oracle_proxy_decode_matches_reference(name, proxy):
+    """Decoding at half / quarter resolution (the reference decoder's
+    proxy_mode, Pi.c:567-626) against the reference decoder's planes; where
+    the reference refuses the mode (fewer levels than it drops) the oracle
+    refuses it too."""
+    O = load_oracle()
+    entry = G["small"][name]
+    if entry.get("reference_error"):
+        return
+    ref = open(os.path.join(GOLDEN, name + ".jxs"), "rb").read()
+    want = entry["decoded_half_sha256" if proxy == 1 else "decoded_quarter_sha256"]
+    e, planes = O.decode(ref, proxy)
+    if want is None:
+        assert e != 0
+        return
+    assert e == 0
+    p = entry["params"]
+    assert planes[0].shape[1] == (p["width"] + (1 << proxy) - 1) >> proxy
+    assert sha(b"".join(pl.tobytes() for pl in planes)) == want
