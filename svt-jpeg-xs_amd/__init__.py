@@ -338,10 +338,11 @@ class Encoder:
 class Decoder:
     """svt_jpeg_xs_decoder_init on the first codestream, then send/get."""
 
-    def __init__(self, codestream, verbose=1):
+    def __init__(self, codestream, verbose=1, proxy_mode=0):
         L = lib()
         self.api = DecoderApi()
         self.api.verbose = verbose
+        self.api.proxy_mode = proxy_mode
         self.config = ImageConfig()
         buf = np.frombuffer(codestream, dtype=np.uint8)
         rc = L.svt_jpeg_xs_decoder_init(API_MAJOR, API_MINOR, ctypes.byref(self.api), buf.ctypes.data, len(buf),

@@ -22,6 +22,8 @@ struct JxsDecFrame {
     const uint32_t* prec_in; /* known precinct offsets (stride prec_in_stride), or NULL: walk the codestream */
     uint32_t prec_in_stride;
     uint32_t pkt_stride; /* u32 words per precinct in pkt */
+    uint32_t npk_dec;    /* packets of each precinct decoded (fewer for a proxy decode) */
+    uint32_t pad_;
     uint32_t* pkt;       /* per precinct: JXS_DPKT_HDR words of band modes and flags, then 5 words per packet */
 };
 

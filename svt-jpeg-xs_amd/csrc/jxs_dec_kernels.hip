@@ -320,7 +320,7 @@ __global__ void __launch_bounds__(PK_THREADS) __attribute__((amdgpu_waves_per_eu
     const JxsGeom& g = *F.g;
     const uint32_t row = blockIdx.y - F.prec_base;
     const uint32_t p = blockIdx.x;
-    if (p >= g.npk || row >= g.rows)
+    if (p >= g.npk || p >= F.npk_dec || row >= g.rows)
         return;
     const uint32_t type = row + 1 >= g.rows ? 1 : 0;
     const uint32_t l = g.pk_line[p];

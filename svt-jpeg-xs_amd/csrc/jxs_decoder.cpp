@@ -190,7 +190,9 @@ static void inv_jobs(const JxsGeom& g, int16_t* coef, int32_t* scratch, void* co
 }
 
 struct DecGeom {
-    HostGeom host;
+    HostGeom host;   /* the codestream's geometry (packets are walked with it) */
+    JxsGeom out;     /* the geometry of the output: host.g, or with levels dropped for a proxy decode */
+    uint32_t npk_dec = 0; /* packets of each precinct decoded */
     uint32_t hdr_end = 0;
     JxsGeom* d_geom = nullptr;
 };
@@ -319,12 +321,13 @@ static svt_jpeg_xs_amd_decode_plan_t* dplan_build(const std::vector<DecSpec>& sp
         cur += al256((uint64_t)g.rows * F.pkt_stride * 4);
         F.status = p->d_status + i;
         F.prec_base = base;
+        F.npk_dec = s.geom->npk_dec;
         F.prec_in = s.prec_in;
         F.prec_in_stride = s.prec_stride ? s.prec_stride : 1;
         base += g.rows;
-        p->max_npk = g.npk > p->max_npk ? g.npk : p->max_npk;
+        p->max_npk = s.geom->npk_dec > p->max_npk ? s.geom->npk_dec : p->max_npk;
         InvJobs one;
-        inv_jobs(g, F.coef, scratch, s.planes, s.strides, one);
+        inv_jobs(s.geom->out, F.coef, scratch, s.planes, s.strides, one);
         for (int k = 0; k < 2; k++) {
             J.rows[k].insert(J.rows[k].end(), one.rows[k].begin(), one.rows[k].end());
             J.vert[k].insert(J.vert[k].end(), one.vert[k].begin(), one.vert[k].end());
@@ -382,7 +385,38 @@ static hipError_t dplan_launch(svt_jpeg_xs_amd_decode_plan_t* p, hipStream_t st)
     return jxs_launch_inv_vert(p->d_vert[1], p->d_vbase[1], p->nvert[1], p->vblocks[1], st);
 }
 
-static int32_t geom_from_codestream(const uint8_t* cs, size_t len, DecGeom& dg) {
+/* Proxy decoding (pi_update_proxy_mode, Pi.c:567-626): half or quarter
+ * resolution drops one or two vertical and horizontal levels from the
+ * picture and every component, whose first 2V' + H' + 1 bands stay as they
+ * are; only the packets carrying them are decoded (one per precinct, or four
+ * at half resolution with two vertical levels).  A codestream with fewer
+ * levels than the mode drops is refused. */
+static int32_t apply_proxy(DecGeom& dg, uint32_t proxy) {
+    const JxsGeom& g = dg.host.g;
+    dg.out = g;
+    dg.npk_dec = g.npk;
+    if (proxy == proxy_mode_full)
+        return SvtJxsErrorNone;
+    if (proxy >= proxy_mode_max || proxy > g.dv || proxy > g.dh)
+        return SvtJxsErrorBadParameter;
+    dg.npk_dec = proxy == proxy_mode_half && g.dv != 1 ? 4u : 1u;
+    JxsGeom& o = dg.out;
+    o.w = (g.w + (1u << proxy) - 1) >> proxy;
+    o.h = (g.h + (1u << proxy) - 1) >> proxy;
+    for (uint32_t c = 0; c < o.nc; c++) {
+        JxsComp& cp = o.c[c];
+        if (proxy > cp.dv || proxy > cp.dh)
+            return SvtJxsErrorBadParameter;
+        cp.w = (cp.w + (1u << proxy) - 1) >> proxy;
+        cp.h = (cp.h + (1u << proxy) - 1) >> proxy;
+        cp.dv -= proxy;
+        cp.dh -= proxy;
+        cp.nb = 2 * cp.dv + cp.dh + 1;
+    }
+    return SvtJxsErrorNone;
+}
+
+static int32_t geom_from_codestream(const uint8_t* cs, size_t len, DecGeom& dg, uint32_t proxy) {
     int32_t e = parse_codestream_header(cs, len, dg.host, &dg.hdr_end);
     if (e)
         return e;
@@ -391,7 +425,7 @@ static int32_t geom_from_codestream(const uint8_t* cs, size_t len, DecGeom& dg) 
             return SvtJxsErrorDecoderInvalidBitstream;
     if (dg.host.g.npk > JXS_NPK || dg.host.g.n_bl > JXS_NBL)
         return SvtJxsErrorDecoderInvalidBitstream;
-    return SvtJxsErrorNone;
+    return apply_proxy(dg, proxy);
 }
 
 static void fill_image_config(const JxsGeom& g, svt_jpeg_xs_image_config_t* cfg) {
@@ -439,6 +473,7 @@ static SvtJxsErrorType_t decode_one(DecoderPriv* d, const uint8_t* cs, size_t le
     if (e)
         return (SvtJxsErrorType_t)e;
     const JxsGeom& g = d->geom.host.g;
+    const JxsGeom& og = d->geom.out; /* output planes */
     /* a different configuration needs a new decoder (DecHandle.c) */
     if (hg.g.w != g.w || hg.g.h != g.h || hg.g.nc != g.nc || hg.g.depth != g.depth || hg.g.dv != g.dv ||
         hg.g.dh != g.dh || hg.g.pps != g.pps || memcmp(hg.g.c, g.c, sizeof(g.c)) != 0)
@@ -447,9 +482,9 @@ static SvtJxsErrorType_t decode_one(DecoderPriv* d, const uint8_t* cs, size_t le
         return SvtJxsErrorDecoderInvalidBitstream;
     const uint32_t pix = g.depth <= 8 ? 1 : 2;
     for (uint32_t c = 0; c < g.nc; c++) {
-        if (!frame->image.data_yuv[c] || frame->image.stride[c] < g.c[c].w)
+        if (!frame->image.data_yuv[c] || frame->image.stride[c] < og.c[c].w)
             return SvtJxsErrorDecoderInvalidPointer;
-        const uint64_t need = (uint64_t)frame->image.stride[c] * pix * (g.c[c].h - 1) + (uint64_t)g.c[c].w * pix;
+        const uint64_t need = (uint64_t)frame->image.stride[c] * pix * (og.c[c].h - 1) + (uint64_t)og.c[c].w * pix;
         if (frame->image.alloc_size[c] < need)
             return SvtJxsErrorBadParameter;
     }
@@ -460,8 +495,8 @@ static SvtJxsErrorType_t decode_one(DecoderPriv* d, const uint8_t* cs, size_t le
         he = dplan_launch(d->plan, d->stream);
     for (uint32_t c = 0; c < g.nc && he == hipSuccess; c++)
         he = hipMemcpy2DAsync(frame->image.data_yuv[c], (size_t)frame->image.stride[c] * pix,
-                              (uint8_t*)d->d_planes + d->plane_off[c], (size_t)g.c[c].w * pix, (size_t)g.c[c].w * pix,
-                              g.c[c].h, hipMemcpyDeviceToHost, d->stream);
+                              (uint8_t*)d->d_planes + d->plane_off[c], (size_t)og.c[c].w * pix, (size_t)og.c[c].w * pix,
+                              og.c[c].h, hipMemcpyDeviceToHost, d->stream);
     if (he == hipSuccess)
         he = hipMemcpyAsync(d->h_status, d->plan->d_status, 4, hipMemcpyDeviceToHost, d->stream);
     if (he == hipSuccess)
@@ -482,11 +517,13 @@ PREFIX_API SvtJxsErrorType_t svt_jpeg_xs_decoder_get_single_frame_size_with_prox
     uint32_t* frame_size_out, uint32_t fast_search, proxy_mode_t proxy_mode) {
     if (!bitstream_buf || !frame_size_out)
         return SvtJxsErrorDecoderInvalidPointer;
-    if (proxy_mode != proxy_mode_full)
-        return SvtJxsErrorBadParameter;
-    HostGeom hg;
+    DecGeom dg;
+    HostGeom& hg = dg.host;
     uint32_t hdr_end = 0;
     int32_t e = parse_codestream_header(bitstream_buf, bitstream_buf_size, hg, &hdr_end);
+    if (e)
+        return (SvtJxsErrorType_t)e;
+    e = apply_proxy(dg, (uint32_t)proxy_mode);
     if (e)
         return (SvtJxsErrorType_t)e;
     if (fast_search && hg.g.lcod > 0) {
@@ -498,7 +535,7 @@ PREFIX_API SvtJxsErrorType_t svt_jpeg_xs_decoder_get_single_frame_size_with_prox
             return (SvtJxsErrorType_t)e;
     }
     if (out_image_config)
-        fill_image_config(hg.g, out_image_config);
+        fill_image_config(dg.out, out_image_config);
     return SvtJxsErrorNone;
 }
 
@@ -518,23 +555,24 @@ PREFIX_API SvtJxsErrorType_t svt_jpeg_xs_decoder_init(uint64_t version_api_major
     if (!dec_api || !bitstream_buf)
         return SvtJxsErrorDecoderInvalidPointer;
     dec_api->private_ptr = nullptr;
-    if (dec_api->proxy_mode != proxy_mode_full)
+    if ((uint32_t)dec_api->proxy_mode >= proxy_mode_max)
         return SvtJxsErrorBadParameter;
     auto* d = new (std::nothrow) DecoderPriv;
     if (!d)
         return SvtJxsErrorInsufficientResources;
     d->verbose = dec_api->verbose;
-    int32_t e = geom_from_codestream(bitstream_buf, codestream_size, d->geom);
+    int32_t e = geom_from_codestream(bitstream_buf, codestream_size, d->geom, (uint32_t)dec_api->proxy_mode);
     if (e) {
         delete d;
         return (SvtJxsErrorType_t)e;
     }
     const JxsGeom& g = d->geom.host.g;
+    const JxsGeom& og = d->geom.out;
     const uint32_t pix = g.depth <= 8 ? 1 : 2;
     uint64_t planes = 0;
     for (uint32_t c = 0; c < g.nc; c++) {
         d->plane_off[c] = planes;
-        planes = al256(planes + (uint64_t)g.c[c].w * g.c[c].h * pix);
+        planes = al256(planes + (uint64_t)og.c[c].w * og.c[c].h * pix);
     }
     /* the largest codestream of this geometry: every precinct at the 20-bit
      * Lprc limit is not realistic; a constant-rate stream carries Lcod */
@@ -555,7 +593,7 @@ PREFIX_API SvtJxsErrorType_t svt_jpeg_xs_decoder_init(uint64_t version_api_major
     s.cs_len = d->cs_cap;
     for (uint32_t c = 0; c < g.nc; c++) {
         s.planes[c] = (uint8_t*)d->d_planes + d->plane_off[c];
-        s.strides[c] = g.c[c].w;
+        s.strides[c] = og.c[c].w;
     }
     s.prec_in = d->d_prec;
     s.prec_stride = 1;
@@ -565,7 +603,7 @@ PREFIX_API SvtJxsErrorType_t svt_jpeg_xs_decoder_init(uint64_t version_api_major
         return SvtJxsErrorInsufficientResources;
     }
     if (out_image_config)
-        fill_image_config(g, out_image_config);
+        fill_image_config(og, out_image_config);
     dec_api->private_ptr = d;
     if (dec_api->callback_send_data_available)
         dec_api->callback_send_data_available(dec_api, dec_api->callback_send_data_available_context);

@@ -121,8 +121,10 @@ import os  # noqa: E402
 
 from jxs_util import ROOT  # noqa: E402
 
-with open(os.path.join(ROOT, "tests", "golden", "golden.json")) as _f:
-    _FULL = json.load(_f)["full"]
+_GOLDEN = os.path.join(ROOT, "tests", "golden")
+with open(os.path.join(_GOLDEN, "golden.json")) as _f:
+    _G = json.load(_f)
+_FULL = _G["full"]
 
 
 @pytest.mark.gpu
@@ -147,3 +149,32 @@ def test_gpu_decode_full_size_equals_reference_decoder(name):
     assert rc == 0
     got = hashlib.sha256(b"".join(np.ascontiguousarray(q).tobytes() for q in out)).hexdigest()
     assert got == _FULL[name]["decoded_sha256"]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("proxy", [1, 2])
+@pytest.mark.parametrize("name", sorted(n for n in _G["small"] if "decoded_half_sha256" in _G["small"][n]
+                                        and not _G["small"][n].get("reference_error")))
+def test_gpu_proxy_decode_equals_reference(name, proxy):
+    """Decoding at half / quarter resolution (proxy_mode) through the C-ABI:
+    the planes hash to the reference decoder's, and a mode the codestream's
+    levels do not allow is refused at init as by the reference."""
+    print("proxy", proxy, name, flush=True)
+    P, O = load_package(), load_oracle()
+    entry = _G["small"][name]
+    cs = open(os.path.join(_GOLDEN, name + ".jxs"), "rb").read()
+    want = entry["decoded_half_sha256" if proxy == 1 else "decoded_quarter_sha256"]
+    if want is None:
+        with pytest.raises(P.JxsError):
+            P.Decoder(cs, verbose=0, proxy_mode=proxy)
+        return
+    dec = P.Decoder(cs, verbose=0, proxy_mode=proxy)
+    rc, got = dec.decode(cs)
+    dec.close()
+    assert rc == 0
+    e, ref = O.decode(cs, proxy)
+    assert e == 0
+    for c, (a, b) in enumerate(zip(got, ref)):
+        assert a.shape == b.shape
+        assert np.array_equal(a, b), "component %d differs" % c
+    assert hashlib.sha256(b"".join(p.tobytes() for p in got)).hexdigest() == want
