@@ -178,3 +178,38 @@ def test_gpu_proxy_decode_equals_reference(name, proxy):
         assert a.shape == b.shape
         assert np.array_equal(a, b), "component %d differs" % c
     assert hashlib.sha256(b"".join(p.tobytes() for p in got)).hexdigest() == want
+
+
+@pytest.mark.gpu
+def test_decode_plan_mixes_full_and_proxy_decoders():
+    """One device-resident decode batch of three codestreams decoded at full,
+    half and quarter resolution (a decoder per mode): each writes its own
+    plane sizes and equals the oracle's decode at that resolution."""
+    import torch
+    dev = torch.device("cuda:0")
+    print("decode plan: full + half + quarter", flush=True)
+    P, O = load_package(), load_oracle()
+    cfg = (256, 64, 10, 3, 3, 2, 5, 0, 1)
+    _, cs = _codestream(cfg, frame=2)
+    d_cs = torch.from_numpy(np.frombuffer(cs, dtype=np.uint8).copy()).to(dev)
+    specs, outs, refs, decs = [], [], [], []
+    for proxy in (0, 1, 2):
+        dec = P.Decoder(cs, verbose=0, proxy_mode=proxy)
+        e, ref = O.decode(cs, proxy)
+        assert e == 0
+        planes = [torch.zeros(r.shape, dtype=torch.int16, device=dev) for r in ref]
+        specs.append((dec, d_cs.data_ptr(), len(cs), [t.data_ptr() for t in planes], [r.shape[1] for r in ref], None, 1))
+        outs.append(planes)
+        refs.append(ref)
+        decs.append(dec)
+    plan = P.DecodePlan(specs)
+    plan.run(torch.cuda.current_stream(dev).cuda_stream)
+    torch.cuda.synchronize()
+    assert plan.status()[0] == 0
+    for proxy, (planes, ref) in enumerate(zip(outs, refs)):
+        for c, (t, r) in enumerate(zip(planes, ref)):
+            got = t.cpu().numpy().view(np.uint16)
+            assert np.array_equal(got, r), "proxy %d component %d" % (proxy, c)
+    plan.destroy()
+    for dec in decs:
+        dec.close()
