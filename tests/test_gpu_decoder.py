@@ -213,3 +213,56 @@ def test_decode_plan_mixes_full_and_proxy_decoders():
     plan.destroy()
     for dec in decs:
         dec.close()
+
+
+@pytest.mark.gpu
+def test_send_packet_decodes_chunked_codestreams():
+    """svt_jpeg_xs_decoder_send_packet (SvtJpegxsDec.h:108-118): two
+    codestreams fed back to back in 700-byte chunks, one chunk straddling
+    them.  Every chunk is consumed; a call returns BitstreamTooShort until a
+    frame is complete, then None, and get_frame returns that frame, equal to
+    the oracle's decode."""
+    import ctypes
+    print("send_packet chunks", flush=True)
+    P, O = load_package(), load_oracle()
+    cfg = (256, 64, 10, 3, 3, 2, 5, 0, 1)
+    css = [_codestream(cfg, frame=k)[1] for k in range(2)]
+    dec = P.Decoder(css[0], verbose=0)
+    L = P.lib()
+    c = dec.config
+
+    def new_image():
+        planes = [np.zeros((c.components[k].height, c.components[k].width), dtype=np.uint16)
+                  for k in range(c.components_num)]
+        f = P.Frame()
+        for k, p in enumerate(planes):
+            f.image.data_yuv[k] = p.ctypes.data
+            f.image.stride[k] = p.shape[1]
+            f.image.alloc_size[k] = p.nbytes
+        return planes, f
+
+    stream = np.frombuffer(b"".join(css), dtype=np.uint8).copy()
+    planes, f = new_image()
+    done = []
+    for pos in range(0, len(stream), 700):
+        chunk = stream[pos:pos + 700]
+        f.bitstream.buffer = chunk.ctypes.data
+        f.bitstream.allocation_size = len(chunk)
+        f.bitstream.used_size = len(chunk)
+        used = ctypes.c_uint32(0)
+        rc = L.svt_jpeg_xs_decoder_send_packet(ctypes.byref(dec.api), ctypes.byref(f), ctypes.byref(used))
+        assert used.value == len(chunk)
+        if rc == P.ERR_DEC_TOO_SHORT:
+            continue
+        assert rc == 0
+        out = P.Frame()
+        assert L.svt_jpeg_xs_decoder_get_frame(ctypes.byref(dec.api), ctypes.byref(out), 1) == 0
+        done.append(planes)
+        planes, f = new_image()
+    dec.close()
+    assert len(done) == 2
+    for k, got in enumerate(done):
+        e, ref = O.decode(css[k])
+        assert e == 0
+        for a, b in zip(got, ref):
+            assert np.array_equal(a, b), "frame %d" % k
