@@ -74,8 +74,9 @@ PREFIX_API SvtJxsErrorType_t svt_jpeg_xs_amd_plan_precinct_offsets(svt_jpeg_xs_a
 
 /* One codestream of a device-resident decode batch.  `decoder` is a decoder
  * initialised on a codestream of the same configuration; planes receive the
- * decoded components (stride in samples).  d_precinct_offsets may be NULL,
- * then the precinct chain is walked on the device. */
+ * decoded components (stride in samples) at the decoder's resolution (its
+ * proxy_mode: full, half or quarter).  d_precinct_offsets may be NULL, then
+ * the precinct chain is walked on the device. */
 typedef struct svt_jpeg_xs_amd_decode_spec {
     svt_jpeg_xs_decoder_api_t* decoder;
     const uint8_t* d_codestream;
