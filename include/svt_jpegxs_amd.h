@@ -58,13 +58,6 @@ PREFIX_API void svt_jpeg_xs_amd_plan_destroy(svt_jpeg_xs_amd_plan_t* plan);
 /* Device bytes a plan of these frames allocates (inputs and outputs excluded). */
 PREFIX_API uint64_t svt_jpeg_xs_amd_plan_footprint(const svt_jpeg_xs_amd_frame_spec_t* frames, uint32_t count);
 
-/* Diagnostics: copies the work buffers of frame `frame` of a completed plan
- * to host memory (any pointer may be NULL): coefficients (u16, band-major as
- * laid out by the plan), GCLI (u8), and per precinct {q, r, total, pad,
- * offset, err} (6 x u32). */
-PREFIX_API SvtJxsErrorType_t svt_jpeg_xs_amd_plan_debug_copy(svt_jpeg_xs_amd_plan_t* plan, uint32_t frame, uint16_t* coeff,
-                                                             uint8_t* gcli, uint32_t* prec);
-
 /* Device address of the precinct-offset table the encode plan wrote for
  * frame `frame` (byte offset of every precinct header in its codestream,
  * one u32 every *stride u32).  Valid after the plan has run; a decode plan of

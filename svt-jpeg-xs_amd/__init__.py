@@ -33,6 +33,9 @@ ERR_ENCODE_FRAME = -0x7FFFDFCB           # 0x80002035
 ERR_DEC_INVALID_POINTER = -0x7FFFD000    # 0x80003000
 ERR_DEC_INVALID_BITSTREAM = -0x7FFFCFFF  # 0x80003001
 ERR_DEC_TOO_SHORT = -0x7FFFCFFD          # 0x80003003
+ERR_DEC_CONFIG_CHANGE = -0x7FFFCFFC      # 0x80003004
+ERR_DEC_EOC = -0x7FFFCFFB                # 0x80003005
+ERR_UNDEFINED = -0x7FFFEFFF              # 0x80001001
 
 FORMAT_YUV420, FORMAT_YUV422, FORMAT_YUV444, FORMAT_PACKED444 = 2, 3, 4, 21
 
@@ -147,7 +150,7 @@ EXPORTED_SYMBOLS = [
     "svt_jpeg_xs_amd_plan_status", "svt_jpeg_xs_amd_plan_destroy", "svt_jpeg_xs_amd_plan_footprint",
     "svt_jpeg_xs_amd_version",
     "svt_jpeg_xs_amd_decode_plan_create", "svt_jpeg_xs_amd_decode_plan_run", "svt_jpeg_xs_amd_decode_plan_status",
-    "svt_jpeg_xs_amd_decode_plan_destroy", "svt_jpeg_xs_amd_plan_precinct_offsets", "svt_jpeg_xs_amd_plan_debug_copy",
+    "svt_jpeg_xs_amd_decode_plan_destroy", "svt_jpeg_xs_amd_plan_precinct_offsets",
     "svt_jpeg_xs_amd_server_create", "svt_jpeg_xs_amd_server_submit", "svt_jpeg_xs_amd_server_poll",
     "svt_jpeg_xs_amd_server_stats", "svt_jpeg_xs_amd_server_now_ns", "svt_jpeg_xs_amd_server_destroy",
     "svt_jpeg_xs_amd_autotune",
@@ -196,8 +199,6 @@ def lib():
         L.svt_jpeg_xs_amd_plan_precinct_offsets.argtypes = [ctypes.c_void_p, ctypes.c_uint32,
                                                             ctypes.POINTER(ctypes.c_void_p),
                                                             ctypes.POINTER(ctypes.c_uint32)]
-        L.svt_jpeg_xs_amd_plan_debug_copy.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p,
-                                                      ctypes.c_void_p, ctypes.c_void_p]
         L.svt_jpeg_xs_amd_plan_create.argtypes = [ctypes.POINTER(FrameSpec), ctypes.c_uint32]
         L.svt_jpeg_xs_amd_plan_create.restype = ctypes.c_void_p
         L.svt_jpeg_xs_amd_plan_run.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32]
@@ -221,6 +222,22 @@ def lib():
         L.svt_jpeg_xs_amd_server_now_ns.restype = ctypes.c_uint64
         L.svt_jpeg_xs_amd_server_destroy.argtypes = [ctypes.c_void_p]
         L.svt_jpeg_xs_amd_server_destroy.restype = None
+        L.svt_jpeg_xs_image_buffer_alloc.argtypes = [ctypes.POINTER(ImageConfig)]
+        L.svt_jpeg_xs_image_buffer_alloc.restype = ctypes.POINTER(ImageBuffer)
+        L.svt_jpeg_xs_image_buffer_free.argtypes = [ctypes.POINTER(ImageBuffer)]
+        L.svt_jpeg_xs_image_buffer_free.restype = None
+        L.svt_jpeg_xs_bitstream_alloc.argtypes = [ctypes.c_uint32]
+        L.svt_jpeg_xs_bitstream_alloc.restype = ctypes.POINTER(BitstreamBuffer)
+        L.svt_jpeg_xs_bitstream_free.argtypes = [ctypes.POINTER(BitstreamBuffer)]
+        L.svt_jpeg_xs_bitstream_free.restype = None
+        L.svt_jpeg_xs_frame_pool_alloc.argtypes = [ctypes.POINTER(ImageConfig), ctypes.c_uint32, ctypes.c_uint32]
+        L.svt_jpeg_xs_frame_pool_alloc.restype = ctypes.c_void_p
+        L.svt_jpeg_xs_frame_pool_free.argtypes = [ctypes.c_void_p]
+        L.svt_jpeg_xs_frame_pool_free.restype = None
+        L.svt_jpeg_xs_frame_pool_get.argtypes = [ctypes.c_void_p, ctypes.POINTER(Frame), ctypes.c_uint8]
+        L.svt_jpeg_xs_frame_pool_get.restype = ctypes.c_int32
+        L.svt_jpeg_xs_frame_pool_release.argtypes = [ctypes.c_void_p, ctypes.POINTER(Frame)]
+        L.svt_jpeg_xs_frame_pool_release.restype = None
         for name in ("svt_jpeg_xs_encoder_load_default_parameters", "svt_jpeg_xs_encoder_get_image_config",
                      "svt_jpeg_xs_encoder_init", "svt_jpeg_xs_encoder_send_picture", "svt_jpeg_xs_encoder_get_packet",
                      "svt_jpeg_xs_decoder_init", "svt_jpeg_xs_decoder_get_single_frame_size_with_proxy",
@@ -228,7 +245,7 @@ def lib():
                      "svt_jpeg_xs_decoder_send_packet", "svt_jpeg_xs_decoder_get_frame", "svt_jpeg_xs_decoder_send_eoc",
                      "svt_jpeg_xs_amd_plan_run", "svt_jpeg_xs_amd_plan_run_timed", "svt_jpeg_xs_amd_plan_status",
                      "svt_jpeg_xs_amd_decode_plan_run", "svt_jpeg_xs_amd_decode_plan_status",
-                     "svt_jpeg_xs_amd_plan_precinct_offsets", "svt_jpeg_xs_amd_plan_debug_copy",
+                     "svt_jpeg_xs_amd_plan_precinct_offsets",
                      "svt_jpeg_xs_amd_server_submit", "svt_jpeg_xs_amd_server_stats"):
             getattr(L, name).restype = ctypes.c_int32
         _lib = L
@@ -251,13 +268,23 @@ class Encoder:
     the given fields, then svt_jpeg_xs_encoder_init."""
 
     def __init__(self, width, height, depth, fmt, bpp_num, bpp_den=1, decomp_v=2, decomp_h=5, quantization=0,
-                 slice_height=16, significance=1, verbose=1, slice_packetization=0, rate_control_mode=1):
+                 slice_height=16, significance=1, verbose=1, slice_packetization=0, rate_control_mode=1,
+                 callback_send=None, callback_get=None, vertical_prediction=0, signs_handling=0):
         L = lib()
         self.api = EncoderApi()
         rc = L.svt_jpeg_xs_encoder_load_default_parameters(API_MAJOR, API_MINOR, ctypes.byref(self.api))
         if rc:
             raise JxsError(rc, "load_default_parameters")
         a = self.api
+        # callbacks are called from the library's threads (SvtJpegxsEnc.h:57-73); the
+        # ctypes thunks must outlive the encoder
+        self._cbs = [_CB(lambda api, ctx, f=f: f()) if f else None for f in (callback_send, callback_get)]
+        if self._cbs[0]:
+            a.callback_send_data_available = self._cbs[0]
+        if self._cbs[1]:
+            a.callback_get_data_available = self._cbs[1]
+        a.coding_vertical_prediction_mode = vertical_prediction
+        a.coding_signs_handling = signs_handling
         a.source_width, a.source_height, a.input_bit_depth, a.colour_format = width, height, depth, fmt
         a.bpp_numerator, a.bpp_denominator, a.ndecomp_v, a.ndecomp_h = bpp_num, bpp_den, decomp_v, decomp_h
         a.quantization, a.slice_height, a.coding_significance, a.verbose = quantization, slice_height, significance, verbose
@@ -338,11 +365,18 @@ class Encoder:
 class Decoder:
     """svt_jpeg_xs_decoder_init on the first codestream, then send/get."""
 
-    def __init__(self, codestream, verbose=1, proxy_mode=0):
+    def __init__(self, codestream, verbose=1, proxy_mode=0, packetization_mode=0, callback_send=None,
+                 callback_get=None):
         L = lib()
         self.api = DecoderApi()
+        self._cbs = [_CB(lambda api, ctx, f=f: f()) if f else None for f in (callback_send, callback_get)]
+        if self._cbs[0]:
+            self.api.callback_send_data_available = self._cbs[0]
+        if self._cbs[1]:
+            self.api.callback_get_data_available = self._cbs[1]
         self.api.verbose = verbose
         self.api.proxy_mode = proxy_mode
+        self.api.packetization_mode = packetization_mode
         self.config = ImageConfig()
         buf = np.frombuffer(codestream, dtype=np.uint8)
         rc = L.svt_jpeg_xs_decoder_init(API_MAJOR, API_MINOR, ctypes.byref(self.api), buf.ctypes.data, len(buf),
@@ -359,6 +393,36 @@ class Decoder:
             self.close()
         except Exception:
             pass
+
+    def image_frame(self, planes):
+        """A Frame whose image buffer is the given output planes."""
+        f = Frame()
+        for c, p in enumerate(planes):
+            f.image.data_yuv[c] = p.ctypes.data
+            f.image.stride[c] = p.shape[1]
+            f.image.alloc_size[c] = p.nbytes
+        return f
+
+    def new_planes(self):
+        cfg = self.config
+        dt = np.uint8 if cfg.bit_depth <= 8 else np.uint16
+        return [np.zeros((cfg.components[c].height, cfg.components[c].width), dtype=dt)
+                for c in range(cfg.components_num)]
+
+    def send(self, buf, planes, ctx=None, blocking=1):
+        """svt_jpeg_xs_decoder_send_frame of the codestream in numpy array buf
+        into planes (both must stay alive until get returns the frame)."""
+        f = self.image_frame(planes)
+        f.bitstream.buffer = buf.ctypes.data
+        f.bitstream.allocation_size = len(buf)
+        f.bitstream.used_size = len(buf)
+        f.user_prv_ctx_ptr = ctx
+        return lib().svt_jpeg_xs_decoder_send_frame(ctypes.byref(self.api), ctypes.byref(f), blocking)
+
+    def get(self, blocking=1):
+        f = Frame()
+        rc = lib().svt_jpeg_xs_decoder_get_frame(ctypes.byref(self.api), ctypes.byref(f), blocking)
+        return rc, f
 
     def decode(self, codestream):
         cfg = self.config

@@ -26,13 +26,16 @@ static void host_free(void* p) {
         (void)hipHostFree(p);
 }
 
+/* A frame pool is two independent pools, one of image buffers and one of
+ * bitstream buffers (ImageBuffer.c:168-246): a frame takes one of each, and
+ * each part goes back on its own when the frame says it is releasable. */
 struct svt_jpeg_xs_frame_pool {
     bool use_image = false, use_bitstream = false;
     svt_jpeg_xs_image_config_t config;
     uint32_t bitstream_size = 0;
     std::vector<svt_jpeg_xs_image_buffer_t*> images;
     std::vector<svt_jpeg_xs_bitstream_buffer_t*> bitstreams;
-    std::deque<uint32_t> free_ids;
+    std::deque<uint32_t> free_images, free_bitstreams;
     std::mutex mu;
     std::condition_variable cv;
 };
@@ -109,6 +112,9 @@ PREFIX_API void svt_jpeg_xs_bitstream_free(svt_jpeg_xs_bitstream_buffer_t* b) {
 
 PREFIX_API svt_jpeg_xs_frame_pool_t* svt_jpeg_xs_frame_pool_alloc(const svt_jpeg_xs_image_config_t* cfg,
                                                                   uint32_t bitstream_size, uint32_t count) {
+    /* ImageBuffer.c:172-174 */
+    if (count == 0 || (!cfg && bitstream_size == 0) || (cfg && cfg->components_num > MAX_COMPONENTS_NUM))
+        return nullptr;
     auto* p = new (std::nothrow) svt_jpeg_xs_frame_pool;
     if (!p)
         return nullptr;
@@ -129,14 +135,17 @@ PREFIX_API svt_jpeg_xs_frame_pool_t* svt_jpeg_xs_frame_pool_alloc(const svt_jpeg
             svt_jpeg_xs_frame_pool_free(p);
             return nullptr;
         }
-        /* release_ctx_ptr carries the pool slot (index + 1) */
-        if (img)
+        /* release_ctx_ptr carries the buffer's slot (index + 1) in its pool */
+        if (img) {
             img->release_ctx_ptr = (void*)(uintptr_t)(i + 1);
-        if (bs)
+            p->free_images.push_back(i);
+        }
+        if (bs) {
             bs->release_ctx_ptr = (void*)(uintptr_t)(i + 1);
+            p->free_bitstreams.push_back(i);
+        }
         p->images.push_back(img);
         p->bitstreams.push_back(bs);
-        p->free_ids.push_back(i);
     }
     return p;
 }
@@ -151,38 +160,61 @@ PREFIX_API void svt_jpeg_xs_frame_pool_free(svt_jpeg_xs_frame_pool_t* p) {
     delete p;
 }
 
+/* One image and one bitstream buffer (ImageBuffer.c:248-296); blocking
+ * waits for both, non-blocking takes neither unless both are free. */
 PREFIX_API SvtJxsErrorType_t svt_jpeg_xs_frame_pool_get(svt_jpeg_xs_frame_pool_t* p, svt_jpeg_xs_frame_t* frame,
                                                         uint8_t blocking_flag) {
     if (!p || !frame)
         return SvtJxsErrorBadParameter;
     std::unique_lock<std::mutex> lk(p->mu);
-    if (p->free_ids.empty()) {
+    auto ready = [&] {
+        return (!p->use_image || !p->free_images.empty()) && (!p->use_bitstream || !p->free_bitstreams.empty());
+    };
+    if (!ready()) {
         if (!blocking_flag)
             return SvtJxsErrorNoErrorEmptyQueue;
-        p->cv.wait(lk, [&] { return !p->free_ids.empty(); });
+        p->cv.wait(lk, ready);
     }
-    const uint32_t id = p->free_ids.front();
-    p->free_ids.pop_front();
-    if (p->use_image)
-        frame->image = *p->images[id];
-    if (p->use_bitstream)
-        frame->bitstream = *p->bitstreams[id];
+    if (p->use_image) {
+        frame->image = *p->images[p->free_images.front()];
+        p->free_images.pop_front();
+    }
+    if (p->use_bitstream) {
+        frame->bitstream = *p->bitstreams[p->free_bitstreams.front()];
+        frame->bitstream.used_size = 0;
+        p->free_bitstreams.pop_front();
+    }
     return SvtJxsErrorNone;
 }
 
+/* Each part goes back only when it carries its pool slot and is marked
+ * releasable; the slot is then cleared from the frame, so a second release
+ * of the same frame returns nothing (ImageBuffer.c:306-318). */
 PREFIX_API void svt_jpeg_xs_frame_pool_release(svt_jpeg_xs_frame_pool_t* p, svt_jpeg_xs_frame_t* frame) {
     if (!p || !frame)
         return;
-    uintptr_t tag = 0;
-    if (p->use_image)
-        tag = (uintptr_t)frame->image.release_ctx_ptr;
-    else if (p->use_bitstream)
-        tag = (uintptr_t)frame->bitstream.release_ctx_ptr;
-    if (tag == 0 || tag > p->images.size())
-        return;
-    std::lock_guard<std::mutex> lk(p->mu);
-    p->free_ids.push_back((uint32_t)(tag - 1));
-    p->cv.notify_one();
+    bool freed = false;
+    {
+        std::lock_guard<std::mutex> lk(p->mu);
+        const uintptr_t it = (uintptr_t)frame->image.release_ctx_ptr;
+        if (p->use_image && it && frame->image.ready_to_release) {
+            if (it <= p->images.size()) {
+                p->free_images.push_back((uint32_t)(it - 1));
+                freed = true;
+            }
+            frame->image.release_ctx_ptr = nullptr;
+        }
+        const uintptr_t bt = (uintptr_t)frame->bitstream.release_ctx_ptr;
+        if (p->use_bitstream && bt && frame->bitstream.ready_to_release) {
+            if (bt <= p->bitstreams.size()) {
+                p->free_bitstreams.push_back((uint32_t)(bt - 1));
+                freed = true;
+            }
+            frame->bitstream.release_ctx_ptr = nullptr;
+        }
+    }
+    if (freed)
+        p->cv.notify_all();
 }
 
 }  // extern "C"

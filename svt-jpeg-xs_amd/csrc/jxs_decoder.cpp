@@ -11,8 +11,11 @@
 #include <condition_variable>
 #include <deque>
 #include <mutex>
+#include <algorithm>
 #include <new>
+#include <thread>
 #include <vector>
+#include <cstddef>
 
 #include "SvtJpegxsDec.h"
 #include "svt_jpegxs_amd.h"
@@ -222,31 +225,6 @@ struct svt_jpeg_xs_amd_decode_plan {
 
 namespace jxs {
 
-struct DecoderPriv {
-    DecGeom geom;
-    uint32_t verbose = 0;
-    uint8_t* d_cs = nullptr;
-    uint32_t cs_cap = 0;
-    uint32_t* d_prec = nullptr;
-    void* d_planes = nullptr;
-    size_t plane_off[4] = {0, 0, 0, 0};
-    svt_jpeg_xs_amd_decode_plan_t* plan = nullptr;
-    hipStream_t stream = nullptr;
-    uint32_t* h_status = nullptr;
-    std::vector<uint32_t> prec_off;
-    /* queue of decoded frames (decode completes inside send) */
-    struct Done {
-        svt_jpeg_xs_frame_t frame;
-        SvtJxsErrorType_t err;
-    };
-    std::deque<Done> done;
-    std::mutex mu;
-    std::condition_variable cv;
-    bool eoc = false;
-    /* packet mode accumulation */
-    std::vector<uint8_t> pending_bytes;
-};
-
 template <typename T>
 static hipError_t up(const std::vector<T>& v, T** dst) {
     *dst = nullptr;
@@ -448,62 +426,323 @@ static void fill_image_config(const JxsGeom& g, svt_jpeg_xs_image_config_t* cfg)
     }
 }
 
+/* ---------------- the C-API decoder ----------------
+ *
+ * Frames go through a small ring of device slots (codestream, precinct
+ * offsets, band planes, output planes, a decode plan bound to them).  A send
+ * stages the codestream in the slot's page-locked buffer, queues the copy and
+ * the decode on the decoder's stream and returns; a finisher thread takes the
+ * frames in send order, waits for each, copies the planes into the caller's
+ * image buffer, frees the slot and publishes the frame (callbacks as
+ * DecThreadInit.c:215-223, 338-341 and DecThreadFinal.c).  A frame whose
+ * header or precinct chain does not parse takes no slot; it is published in
+ * order with its error, as the reference's init thread forwards header errors
+ * to the final thread (DecThreadInit.c:291-330). */
+
+static const int kDecSlots = 3;
+
+struct DecSlot {
+    DecGeom geom;             /* the decoder's geometry; its device copy carries this slot's frame's Qpih */
+    uint32_t cap = 0;         /* codestream bytes the slot holds */
+    uint8_t* d_cs = nullptr;
+    uint8_t* h_cs = nullptr;  /* page-locked staging of the codestream */
+    uint32_t* d_prec = nullptr;
+    uint32_t* h_prec = nullptr; /* page-locked precinct offsets */
+    void* d_planes = nullptr;
+    JxsDecFrame* h_frame = nullptr; /* page-locked copy of the plan's frame descriptor */
+    uint32_t* h_qpih = nullptr;     /* page-locked Qpih word */
+    uint32_t qpih = 0;              /* Qpih in the slot's device geometry */
+    svt_jpeg_xs_amd_decode_plan_t* plan = nullptr;
+    hipEvent_t done = nullptr;
+    uint32_t* h_status = nullptr;
+    bool busy = false;
+};
+
+struct DecJob {
+    int slot; /* -1: the frame failed before reaching the device */
+    svt_jpeg_xs_frame_t frame;
+    SvtJxsErrorType_t err;
+};
+
+struct DecoderPriv {
+    svt_jpeg_xs_decoder_api_t* api = nullptr;
+    DecGeom geom;
+    uint32_t verbose = 0;
+    bool packet_mode = false;
+    size_t plane_off[4] = {0, 0, 0, 0};
+    uint64_t plane_bytes = 0;
+    std::vector<DecSlot> slots;
+    hipStream_t stream = nullptr;  /* copies in and decode launches */
+    hipStream_t fstream = nullptr; /* the finisher's copies out */
+    std::mutex mu;
+    std::condition_variable cv_free, cv_job, cv_done;
+    std::deque<DecJob> jobs;  /* sent, in send order, not yet published */
+    std::deque<DecJob> done;  /* published, for get_frame */
+    uint32_t unpublished = 0; /* sent but not yet in done */
+    bool eoc = false, stop = false;
+    std::thread finisher;
+    /* packet input (DecThreadInit.c:347-460): the frame being filled */
+    int pk_slot = -1;
+    uint32_t pk_filled = 0;
+    svt_jpeg_xs_frame_t pk_frame;
+};
+
+static void slot_release_cs(DecSlot& s) {
+    (void)hipFree(s.d_cs);
+    if (s.h_cs)
+        (void)hipHostFree(s.h_cs);
+    s.d_cs = nullptr;
+    s.h_cs = nullptr;
+    s.cap = 0;
+}
+
+static void slot_release(DecSlot& s) {
+    dplan_free(s.plan);
+    s.plan = nullptr;
+    slot_release_cs(s);
+    (void)hipFree(s.d_prec);
+    (void)hipFree(s.d_planes);
+    (void)hipFree(s.geom.d_geom);
+    for (void* p : {(void*)s.h_prec, (void*)s.h_frame, (void*)s.h_qpih, (void*)s.h_status})
+        if (p)
+            (void)hipHostFree(p);
+    if (s.done)
+        (void)hipEventDestroy(s.done);
+    s = DecSlot();
+}
+
+/* (Re)binds the slot's decode plan to its buffers, with room for cap
+ * codestream bytes. */
+static bool slot_bind(DecoderPriv* d, DecSlot& s, uint32_t cap) {
+    if (cap > s.cap) {
+        slot_release_cs(s);
+        if (hipMalloc(&s.d_cs, cap) != hipSuccess || hipHostMalloc((void**)&s.h_cs, cap, 0) != hipSuccess)
+            return false;
+        s.cap = cap;
+    }
+    dplan_free(s.plan);
+    DecSpec sp;
+    memset(&sp, 0, sizeof(sp));
+    sp.geom = &s.geom;
+    sp.d_cs = s.d_cs;
+    sp.cs_len = s.cap;
+    for (uint32_t c = 0; c < s.geom.out.nc; c++) {
+        sp.planes[c] = (uint8_t*)s.d_planes + d->plane_off[c];
+        sp.strides[c] = s.geom.out.c[c].w;
+    }
+    sp.prec_in = s.d_prec;
+    sp.prec_stride = 1;
+    s.plan = dplan_build({sp});
+    if (!s.plan)
+        return false;
+    return hipMemcpy(s.h_frame, s.plan->d_frames, sizeof(JxsDecFrame), hipMemcpyDeviceToHost) == hipSuccess;
+}
+
+static bool slot_create(DecoderPriv* d, DecSlot& s, uint32_t cap) {
+    const JxsGeom& g = d->geom.host.g;
+    s.geom = d->geom;
+    s.geom.d_geom = nullptr;
+    s.qpih = g.qpih;
+    if (hipMalloc(&s.geom.d_geom, sizeof(JxsGeom)) != hipSuccess ||
+        hipMemcpy(s.geom.d_geom, &g, sizeof(JxsGeom), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMalloc(&s.d_prec, (size_t)g.rows * 4) != hipSuccess || hipMalloc(&s.d_planes, d->plane_bytes) != hipSuccess ||
+        hipHostMalloc((void**)&s.h_prec, (size_t)g.rows * 4, 0) != hipSuccess ||
+        hipHostMalloc((void**)&s.h_frame, sizeof(JxsDecFrame), 0) != hipSuccess ||
+        hipHostMalloc((void**)&s.h_qpih, 4, 0) != hipSuccess || hipHostMalloc((void**)&s.h_status, 4, 0) != hipSuccess ||
+        hipEventCreateWithFlags(&s.done, hipEventDisableTiming) != hipSuccess)
+        return false;
+    return slot_bind(d, s, cap);
+}
+
+/* Publishes a finished frame: in send order, releasable, with its error. */
+static void publish(DecoderPriv* d, DecJob& j) {
+    j.frame.image.ready_to_release = 1;
+    j.frame.bitstream.ready_to_release = 1;
+    {
+        std::lock_guard<std::mutex> lk(d->mu);
+        d->done.push_back(j);
+        d->unpublished--;
+    }
+    d->cv_done.notify_all();
+    if (d->api->callback_get_data_available)
+        d->api->callback_get_data_available(d->api, d->api->callback_get_data_available_context);
+}
+
+static void finisher_loop(DecoderPriv* d) {
+    const JxsGeom& og = d->geom.out;
+    const uint32_t pix = og.depth <= 8 ? 1 : 2;
+    for (;;) {
+        DecJob j;
+        {
+            std::unique_lock<std::mutex> lk(d->mu);
+            d->cv_job.wait(lk, [&] { return d->stop || !d->jobs.empty(); });
+            if (d->jobs.empty())
+                return;
+            j = d->jobs.front();
+            d->jobs.pop_front();
+        }
+        if (j.slot >= 0) {
+            DecSlot& s = d->slots[j.slot];
+            hipError_t he = hipEventSynchronize(s.done);
+            if (he != hipSuccess)
+                j.err = SvtJxsErrorDecoderInternal;
+            else if (*s.h_status)
+                j.err = SvtJxsErrorDecoderInvalidBitstream;
+            else {
+                for (uint32_t c = 0; c < og.nc && he == hipSuccess; c++)
+                    he = hipMemcpy2DAsync(j.frame.image.data_yuv[c], (size_t)j.frame.image.stride[c] * pix,
+                                          (uint8_t*)s.d_planes + d->plane_off[c], (size_t)og.c[c].w * pix,
+                                          (size_t)og.c[c].w * pix, og.c[c].h, hipMemcpyDeviceToHost, d->fstream);
+                if (he == hipSuccess)
+                    he = hipStreamSynchronize(d->fstream);
+                if (he != hipSuccess)
+                    j.err = SvtJxsErrorDecoderInternal;
+            }
+            {
+                std::lock_guard<std::mutex> lk(d->mu);
+                s.busy = false;
+            }
+            d->cv_free.notify_all();
+            if (d->api->callback_send_data_available)
+                d->api->callback_send_data_available(d->api, d->api->callback_send_data_available_context);
+        }
+        publish(d, j);
+    }
+}
+
 static void decoder_destroy(DecoderPriv* d) {
     if (!d)
         return;
+    if (d->finisher.joinable()) {
+        {
+            std::lock_guard<std::mutex> lk(d->mu);
+            d->stop = true;
+        }
+        d->cv_job.notify_all();
+        d->finisher.join();
+    }
     if (d->stream)
         (void)hipStreamSynchronize(d->stream);
-    dplan_free(d->plan);
-    (void)hipFree(d->d_cs);
-    (void)hipFree(d->d_prec);
-    (void)hipFree(d->d_planes);
+    for (auto& s : d->slots)
+        slot_release(s);
     (void)hipFree(d->geom.d_geom);
-    if (d->h_status)
-        (void)hipHostFree(d->h_status);
     if (d->stream)
         (void)hipStreamDestroy(d->stream);
+    if (d->fstream)
+        (void)hipStreamDestroy(d->fstream);
     delete d;
 }
 
-/* Decodes one whole codestream (host memory) into the frame's image buffer. */
-static SvtJxsErrorType_t decode_one(DecoderPriv* d, const uint8_t* cs, size_t len, svt_jpeg_xs_frame_t* frame) {
-    HostGeom hg;
-    uint32_t fsize = 0;
-    int32_t e = frame_size(cs, len, hg, &fsize, &d->prec_off);
-    if (e)
-        return (SvtJxsErrorType_t)e;
+/* A free slot (waiting for one when blocking), marked busy; -1 if none. */
+static int acquire_slot(DecoderPriv* d, bool blocking) {
+    std::unique_lock<std::mutex> lk(d->mu);
+    for (;;) {
+        for (int i = 0; i < (int)d->slots.size(); i++)
+            if (!d->slots[i].busy) {
+                d->slots[i].busy = true;
+                return i;
+            }
+        if (!blocking)
+            return -1;
+        d->cv_free.wait(lk);
+    }
+}
+
+static void free_slot(DecoderPriv* d, int si) {
+    {
+        std::lock_guard<std::mutex> lk(d->mu);
+        d->slots[si].busy = false;
+    }
+    d->cv_free.notify_all();
+}
+
+static void enqueue(DecoderPriv* d, const DecJob& j) {
+    {
+        std::lock_guard<std::mutex> lk(d->mu);
+        d->jobs.push_back(j);
+        d->unpublished++;
+    }
+    d->cv_job.notify_all();
+}
+
+/* The frame's header against the decoder's: a different static
+ * configuration is SvtJxsErrorDecoderConfigChange (Decoder.c:388-390); Qpih
+ * may change from frame to frame (Decoder.c:576, ParseHeader.c:238-241). */
+static int32_t check_frame_header(const DecoderPriv* d, const JxsGeom& fg) {
     const JxsGeom& g = d->geom.host.g;
-    const JxsGeom& og = d->geom.out; /* output planes */
-    /* a different configuration needs a new decoder (DecHandle.c) */
-    if (hg.g.w != g.w || hg.g.h != g.h || hg.g.nc != g.nc || hg.g.depth != g.depth || hg.g.dv != g.dv ||
-        hg.g.dh != g.dh || hg.g.pps != g.pps || memcmp(hg.g.c, g.c, sizeof(g.c)) != 0)
+    if (fg.w != g.w || fg.h != g.h || fg.nc != g.nc || fg.depth != g.depth || fg.dv != g.dv || fg.dh != g.dh ||
+        fg.pps != g.pps || fg.sig != g.sig || fg.short_hdr != g.short_hdr || memcmp(fg.c, g.c, sizeof(g.c)) != 0)
         return SvtJxsErrorDecoderConfigChange;
-    if (fsize > d->cs_cap)
-        return SvtJxsErrorDecoderInvalidBitstream;
-    const uint32_t pix = g.depth <= 8 ? 1 : 2;
-    for (uint32_t c = 0; c < g.nc; c++) {
-        if (!frame->image.data_yuv[c] || frame->image.stride[c] < og.c[c].w)
-            return SvtJxsErrorDecoderInvalidPointer;
-        const uint64_t need = (uint64_t)frame->image.stride[c] * pix * (og.c[c].h - 1) + (uint64_t)og.c[c].w * pix;
-        if (frame->image.alloc_size[c] < need)
+    return SvtJxsErrorNone;
+}
+
+/* Queues the decode of the codestream staged in slot si (fsize bytes,
+ * precinct offsets in h_prec, Qpih qpih) into frame's image buffer. */
+static SvtJxsErrorType_t launch_slot(DecoderPriv* d, int si, uint32_t fsize, uint32_t qpih,
+                                     const svt_jpeg_xs_frame_t& frame) {
+    DecSlot& s = d->slots[si];
+    const JxsGeom& g = d->geom.host.g;
+    hipError_t he = hipSuccess;
+    if (qpih != s.qpih) {
+        *s.h_qpih = qpih;
+        he = hipMemcpyAsync((uint8_t*)s.geom.d_geom + offsetof(JxsGeom, qpih), s.h_qpih, 4, hipMemcpyHostToDevice,
+                            d->stream);
+        s.qpih = qpih;
+    }
+    s.h_frame->cs_len = fsize;
+    if (he == hipSuccess)
+        he = hipMemcpyAsync(s.plan->d_frames, s.h_frame, sizeof(JxsDecFrame), hipMemcpyHostToDevice, d->stream);
+    if (he == hipSuccess)
+        he = hipMemcpyAsync(s.d_cs, s.h_cs, fsize, hipMemcpyHostToDevice, d->stream);
+    if (he == hipSuccess)
+        he = hipMemcpyAsync(s.d_prec, s.h_prec, (size_t)g.rows * 4, hipMemcpyHostToDevice, d->stream);
+    if (he == hipSuccess)
+        he = dplan_launch(s.plan, d->stream);
+    if (he == hipSuccess)
+        he = hipMemcpyAsync(s.h_status, s.plan->d_status, 4, hipMemcpyDeviceToHost, d->stream);
+    if (he == hipSuccess)
+        he = hipEventRecord(s.done, d->stream);
+    if (he != hipSuccess) {
+        s.qpih = ~0u; /* unknown: upload again next time */
+        free_slot(d, si);
+        return SvtJxsErrorDecoderInternal;
+    }
+    enqueue(d, DecJob{si, frame, SvtJxsErrorNone});
+    return SvtJxsErrorNone;
+}
+
+/* Validates the caller's image buffer against the output geometry
+ * (DecHandle.c:379-401; the last row may be shorter than the stride). */
+static SvtJxsErrorType_t check_image(const DecoderPriv* d, const svt_jpeg_xs_image_buffer_t& im) {
+    const JxsGeom& og = d->geom.out;
+    const uint32_t pix = og.depth <= 8 ? 1 : 2;
+    for (uint32_t c = 0; c < og.nc; c++) {
+        if (!im.data_yuv[c] || im.stride[c] < og.c[c].w)
+            return SvtJxsErrorBadParameter;
+        const uint64_t need = (uint64_t)im.stride[c] * pix * (og.c[c].h - 1) + (uint64_t)og.c[c].w * pix;
+        if (im.alloc_size[c] < need)
             return SvtJxsErrorBadParameter;
     }
-    hipError_t he = hipMemcpyAsync(d->d_cs, cs, fsize, hipMemcpyHostToDevice, d->stream);
-    if (he == hipSuccess)
-        he = hipMemcpyAsync(d->d_prec, d->prec_off.data(), g.rows * 4, hipMemcpyHostToDevice, d->stream);
-    if (he == hipSuccess)
-        he = dplan_launch(d->plan, d->stream);
-    for (uint32_t c = 0; c < g.nc && he == hipSuccess; c++)
-        he = hipMemcpy2DAsync(frame->image.data_yuv[c], (size_t)frame->image.stride[c] * pix,
-                              (uint8_t*)d->d_planes + d->plane_off[c], (size_t)og.c[c].w * pix, (size_t)og.c[c].w * pix,
-                              og.c[c].h, hipMemcpyDeviceToHost, d->stream);
-    if (he == hipSuccess)
-        he = hipMemcpyAsync(d->h_status, d->plan->d_status, 4, hipMemcpyDeviceToHost, d->stream);
-    if (he == hipSuccess)
-        he = hipStreamSynchronize(d->stream);
-    if (he != hipSuccess)
-        return SvtJxsErrorDecoderInternal;
-    return *d->h_status ? SvtJxsErrorDecoderInvalidBitstream : SvtJxsErrorNone;
+    return SvtJxsErrorNone;
+}
+
+/* Parses the whole codestream staged in slot si and queues its decode; a
+ * codestream that does not parse is published with its error instead. */
+static SvtJxsErrorType_t submit_staged(DecoderPriv* d, int si, uint32_t len, const svt_jpeg_xs_frame_t& frame) {
+    DecSlot& s = d->slots[si];
+    HostGeom hg;
+    uint32_t fsize = 0;
+    std::vector<uint32_t> prec;
+    int32_t e = frame_size(s.h_cs, len, hg, &fsize, &prec);
+    if (!e)
+        e = check_frame_header(d, hg.g);
+    if (e) {
+        free_slot(d, si);
+        enqueue(d, DecJob{-1, frame, (SvtJxsErrorType_t)e});
+        return SvtJxsErrorNone;
+    }
+    memcpy(s.h_prec, prec.data(), prec.size() * 4);
+    return launch_slot(d, si, fsize, hg.g.qpih, frame);
 }
 
 }  // namespace jxs
@@ -555,18 +794,25 @@ PREFIX_API SvtJxsErrorType_t svt_jpeg_xs_decoder_init(uint64_t version_api_major
     if (!dec_api || !bitstream_buf)
         return SvtJxsErrorDecoderInvalidPointer;
     dec_api->private_ptr = nullptr;
-    if ((uint32_t)dec_api->proxy_mode >= proxy_mode_max)
+    if ((uint32_t)dec_api->proxy_mode >= proxy_mode_max || dec_api->packetization_mode > 1)
         return SvtJxsErrorBadParameter;
     auto* d = new (std::nothrow) DecoderPriv;
     if (!d)
         return SvtJxsErrorInsufficientResources;
+    d->api = dec_api;
     d->verbose = dec_api->verbose;
+    d->packet_mode = dec_api->packetization_mode == 1;
     int32_t e = geom_from_codestream(bitstream_buf, codestream_size, d->geom, (uint32_t)dec_api->proxy_mode);
     if (e) {
         delete d;
         return (SvtJxsErrorType_t)e;
     }
     const JxsGeom& g = d->geom.host.g;
+    /* packet input needs the frame size from the header (DecHandle.c:197-204) */
+    if (d->packet_mode && g.lcod == 0) {
+        delete d;
+        return SvtJxsErrorBadParameter;
+    }
     const JxsGeom& og = d->geom.out;
     const uint32_t pix = g.depth <= 8 ? 1 : 2;
     uint64_t planes = 0;
@@ -574,31 +820,28 @@ PREFIX_API SvtJxsErrorType_t svt_jpeg_xs_decoder_init(uint64_t version_api_major
         d->plane_off[c] = planes;
         planes = al256(planes + (uint64_t)og.c[c].w * og.c[c].h * pix);
     }
-    /* the largest codestream of this geometry: every precinct at the 20-bit
-     * Lprc limit is not realistic; a constant-rate stream carries Lcod */
-    d->cs_cap = g.lcod ? g.lcod : (uint32_t)codestream_size;
+    d->plane_bytes = planes;
+    /* a constant-rate stream carries its frame size (Lcod); otherwise start
+     * from the first codestream's size and grow a slot when a frame needs it */
+    uint32_t cap = g.lcod ? g.lcod : (uint32_t)codestream_size;
+    if (cap == 0)
+        cap = 1;
     if (hipMalloc(&d->geom.d_geom, sizeof(JxsGeom)) != hipSuccess ||
         hipMemcpy(d->geom.d_geom, &g, sizeof(JxsGeom), hipMemcpyHostToDevice) != hipSuccess ||
-        hipMalloc(&d->d_cs, d->cs_cap) != hipSuccess || hipMalloc(&d->d_prec, g.rows * 4) != hipSuccess ||
-        hipMalloc(&d->d_planes, planes) != hipSuccess ||
         hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipHostMalloc((void**)&d->h_status, 4, 0) != hipSuccess) {
+        hipStreamCreateWithFlags(&d->fstream, hipStreamNonBlocking) != hipSuccess) {
         decoder_destroy(d);
         return SvtJxsErrorInsufficientResources;
     }
-    DecSpec s;
-    memset(&s, 0, sizeof(s));
-    s.geom = &d->geom;
-    s.d_cs = d->d_cs;
-    s.cs_len = d->cs_cap;
-    for (uint32_t c = 0; c < g.nc; c++) {
-        s.planes[c] = (uint8_t*)d->d_planes + d->plane_off[c];
-        s.strides[c] = og.c[c].w;
-    }
-    s.prec_in = d->d_prec;
-    s.prec_stride = 1;
-    d->plan = dplan_build({s});
-    if (!d->plan) {
+    d->slots.resize(kDecSlots);
+    for (auto& s : d->slots)
+        if (!slot_create(d, s, cap)) {
+            decoder_destroy(d);
+            return SvtJxsErrorInsufficientResources;
+        }
+    try {
+        d->finisher = std::thread(finisher_loop, d);
+    } catch (...) {
         decoder_destroy(d);
         return SvtJxsErrorInsufficientResources;
     }
@@ -617,92 +860,141 @@ PREFIX_API void svt_jpeg_xs_decoder_close(svt_jpeg_xs_decoder_api_t* dec_api) {
     dec_api->private_ptr = nullptr;
 }
 
-static SvtJxsErrorType_t finish_frame(svt_jpeg_xs_decoder_api_t* api, DecoderPriv* d, svt_jpeg_xs_frame_t* in,
-                                      SvtJxsErrorType_t err) {
-    in->image.ready_to_release = 1;
-    in->bitstream.ready_to_release = 1;
-    {
-        std::lock_guard<std::mutex> lk(d->mu);
-        d->done.push_back({*in, err});
-    }
-    d->cv.notify_one();
-    if (api->callback_get_data_available)
-        api->callback_get_data_available(api, api->callback_get_data_available_context);
-    return SvtJxsErrorNone;
-}
-
+/* One whole codestream per call (DecHandle.c:363-425).  The codestream is
+ * staged before the call returns, so its buffer may be reused at once; the
+ * image buffer is written when get_frame hands the frame back.  Without a free
+ * slot a non-blocking call returns SvtJxsErrorNoErrorEmptyQueue. */
 PREFIX_API SvtJxsErrorType_t svt_jpeg_xs_decoder_send_frame(svt_jpeg_xs_decoder_api_t* dec_api,
                                                             svt_jpeg_xs_frame_t* dec_input, uint8_t blocking_flag) {
-    (void)blocking_flag;
-    if (!dec_api || !dec_api->private_ptr || !dec_input || !dec_input->bitstream.buffer)
+    if (!dec_api || !dec_api->private_ptr || !dec_input)
         return SvtJxsErrorDecoderInvalidPointer;
     DecoderPriv* d = (DecoderPriv*)dec_api->private_ptr;
+    if (d->packet_mode)
+        return SvtJxsErrorUndefined; /* DecHandle.c:371-377 */
+    SvtJxsErrorType_t e = check_image(d, dec_input->image);
+    if (e)
+        return e;
+    if (!dec_input->bitstream.buffer)
+        return SvtJxsErrorDecoderInvalidPointer;
     const size_t len = dec_input->bitstream.used_size ? dec_input->bitstream.used_size
                                                       : dec_input->bitstream.allocation_size;
-    SvtJxsErrorType_t e = decode_one(d, dec_input->bitstream.buffer, len, dec_input);
-    if (e == SvtJxsErrorDecoderConfigChange || e == SvtJxsErrorDecoderInvalidPointer || e == SvtJxsErrorBadParameter)
-        return e;
-    return finish_frame(dec_api, d, dec_input, e);
-}
-
-PREFIX_API SvtJxsErrorType_t svt_jpeg_xs_decoder_send_packet(svt_jpeg_xs_decoder_api_t* dec_api,
-                                                             svt_jpeg_xs_frame_t* dec_input, uint32_t* bytes_used) {
-    if (!dec_api || !dec_api->private_ptr || !dec_input || !dec_input->bitstream.buffer || !bytes_used)
-        return SvtJxsErrorDecoderInvalidPointer;
-    DecoderPriv* d = (DecoderPriv*)dec_api->private_ptr;
-    const uint8_t* b = dec_input->bitstream.buffer;
-    const size_t n = dec_input->bitstream.used_size;
-    d->pending_bytes.insert(d->pending_bytes.end(), b, b + n);
-    *bytes_used = (uint32_t)n;
+    const int si = acquire_slot(d, blocking_flag != 0);
+    if (si < 0)
+        return SvtJxsErrorNoErrorEmptyQueue;
+    DecSlot& s = d->slots[si];
+    /* the frame's size from its precinct chain; a longer buffer holds more
+     * than the frame, and only the frame is staged */
     HostGeom hg;
     uint32_t fsize = 0;
-    int32_t e = frame_size(d->pending_bytes.data(), d->pending_bytes.size(), hg, &fsize, nullptr);
-    if (e == SvtJxsErrorDecoderBitstreamTooShort)
-        return SvtJxsErrorDecoderBitstreamTooShort;
-    if (e) {
-        d->pending_bytes.clear();
-        return (SvtJxsErrorType_t)e;
+    int32_t pe = frame_size(dec_input->bitstream.buffer, len, hg, &fsize, nullptr);
+    if (!pe && fsize > s.cap && !slot_bind(d, s, fsize)) {
+        free_slot(d, si);
+        return SvtJxsErrorInsufficientResources;
     }
-    std::vector<uint8_t> frame_bytes(d->pending_bytes.begin(), d->pending_bytes.begin() + fsize);
-    d->pending_bytes.erase(d->pending_bytes.begin(), d->pending_bytes.begin() + fsize);
-    SvtJxsErrorType_t r = decode_one(d, frame_bytes.data(), frame_bytes.size(), dec_input);
-    if (r == SvtJxsErrorDecoderConfigChange || r == SvtJxsErrorDecoderInvalidPointer || r == SvtJxsErrorBadParameter)
-        return r;
-    return finish_frame(dec_api, d, dec_input, r);
+    if (pe) {
+        free_slot(d, si);
+        enqueue(d, DecJob{-1, *dec_input, (SvtJxsErrorType_t)pe});
+        return SvtJxsErrorNone;
+    }
+    memcpy(s.h_cs, dec_input->bitstream.buffer, fsize);
+    return submit_staged(d, si, fsize, *dec_input);
 }
 
+/* Packet input (DecThreadInit.c:347-460): a call takes at most the rest of
+ * the current frame (Lcod bytes from its first packet), so *bytes_used may be
+ * less than the chunk and the caller sends the remainder again.  The image
+ * buffer of a frame is the one passed with its first packet.  The call returns
+ * SvtJxsErrorDecoderBitstreamTooShort until the frame is complete, then
+ * queues it; a frame that does not parse is handed back by get_frame with its
+ * error. */
+PREFIX_API SvtJxsErrorType_t svt_jpeg_xs_decoder_send_packet(svt_jpeg_xs_decoder_api_t* dec_api,
+                                                             svt_jpeg_xs_frame_t* dec_input, uint32_t* bytes_used) {
+    if (!dec_api || !dec_api->private_ptr || !dec_input || !bytes_used)
+        return SvtJxsErrorDecoderInvalidPointer;
+    DecoderPriv* d = (DecoderPriv*)dec_api->private_ptr;
+    *bytes_used = 0;
+    if (!d->packet_mode)
+        return SvtJxsErrorUndefined;
+    if (!dec_input->bitstream.buffer)
+        return SvtJxsErrorDecoderInvalidPointer;
+    if (d->pk_slot < 0) {
+        SvtJxsErrorType_t e = check_image(d, dec_input->image);
+        if (e)
+            return e;
+        d->pk_slot = acquire_slot(d, true);
+        d->pk_filled = 0;
+        d->pk_frame = *dec_input;
+        memset(&d->pk_frame.bitstream, 0, sizeof(d->pk_frame.bitstream));
+    }
+    DecSlot& s = d->slots[d->pk_slot];
+    const uint32_t lcod = d->geom.host.g.lcod;
+    const uint32_t n = std::min<uint32_t>(lcod - d->pk_filled, dec_input->bitstream.used_size);
+    memcpy(s.h_cs + d->pk_filled, dec_input->bitstream.buffer, n);
+    d->pk_filled += n;
+    *bytes_used = n;
+    dec_input->bitstream.ready_to_release = 1;
+    dec_input->image.ready_to_release = 0;
+    HostGeom hg;
+    uint32_t fsize = 0;
+    const int32_t e = frame_size(s.h_cs, d->pk_filled, hg, &fsize, nullptr);
+    if (e == SvtJxsErrorDecoderBitstreamTooShort && d->pk_filled < lcod)
+        return SvtJxsErrorDecoderBitstreamTooShort;
+    const int si = d->pk_slot;
+    const uint32_t filled = d->pk_filled;
+    d->pk_slot = -1;
+    d->pk_filled = 0;
+    if (e) {
+        free_slot(d, si);
+        enqueue(d, DecJob{-1, d->pk_frame, (SvtJxsErrorType_t)e});
+        return SvtJxsErrorNone;
+    }
+    return submit_staged(d, si, filled, d->pk_frame);
+}
+
+/* Frames in send order (DecHandle.c:435-465); SvtJxsDecoderEndOfCodestream
+ * once send_eoc was called and every frame sent before it was handed out. */
 PREFIX_API SvtJxsErrorType_t svt_jpeg_xs_decoder_get_frame(svt_jpeg_xs_decoder_api_t* dec_api,
                                                            svt_jpeg_xs_frame_t* dec_output, uint8_t blocking_flag) {
     if (!dec_api || !dec_api->private_ptr || !dec_output)
         return SvtJxsErrorDecoderInvalidPointer;
     DecoderPriv* d = (DecoderPriv*)dec_api->private_ptr;
     std::unique_lock<std::mutex> lk(d->mu);
+    auto finished = [&] { return d->eoc && d->unpublished == 0 && d->pk_slot < 0; };
     if (d->done.empty()) {
-        if (d->eoc)
+        if (finished())
             return SvtJxsDecoderEndOfCodestream;
         if (!blocking_flag) {
             dec_output->user_prv_ctx_ptr = nullptr;
             return SvtJxsErrorNoErrorEmptyQueue;
         }
-        d->cv.wait(lk, [&] { return !d->done.empty() || d->eoc; });
+        d->cv_done.wait(lk, [&] { return !d->done.empty() || finished(); });
         if (d->done.empty())
             return SvtJxsDecoderEndOfCodestream;
     }
-    DecoderPriv::Done r = d->done.front();
+    DecJob r = d->done.front();
     d->done.pop_front();
     *dec_output = r.frame;
     return r.err;
 }
 
+/* End of input (DecHandle.c:466-540).  A frame whose packets had not all
+ * arrived is handed back with SvtJxsErrorDecoderBitstreamTooShort. */
 PREFIX_API SvtJxsErrorType_t svt_jpeg_xs_decoder_send_eoc(svt_jpeg_xs_decoder_api_t* dec_api) {
     if (!dec_api || !dec_api->private_ptr)
         return SvtJxsErrorDecoderInvalidPointer;
     DecoderPriv* d = (DecoderPriv*)dec_api->private_ptr;
+    if (d->pk_slot >= 0) {
+        const int si = d->pk_slot;
+        d->pk_slot = -1;
+        d->pk_filled = 0;
+        free_slot(d, si);
+        enqueue(d, DecJob{-1, d->pk_frame, SvtJxsErrorDecoderBitstreamTooShort});
+    }
     {
         std::lock_guard<std::mutex> lk(d->mu);
         d->eoc = true;
     }
-    d->cv.notify_all();
+    d->cv_done.notify_all();
     return SvtJxsErrorNone;
 }
 

@@ -12,6 +12,7 @@
 #include <deque>
 #include <mutex>
 #include <new>
+#include <thread>
 #include <vector>
 
 #include "SvtJpegxsEnc.h"
@@ -124,9 +125,15 @@ struct EncoderPriv {
         uint32_t* h_status = nullptr; /* pinned */
         svt_jpeg_xs_frame_t frame;
         bool busy = false;
+        bool finished = false; /* set by the finisher once the frame's work is done */
+        bool failed = false;
     };
     std::vector<Slot> slots;
-    std::deque<int> pending; /* in send order */
+    std::deque<int> pending; /* in send order, until get_packet takes them */
+    std::deque<int> watch;   /* in send order, until the finisher has seen them complete */
+    std::thread finisher;
+    bool stop = false;
+    svt_jpeg_xs_encoder_api_t* api = nullptr;
     /* slice packetization (FinalStageProcess.c:126-213): the frame whose
      * packets are being handed out, the next packet (0: header, 1 + s: slice
      * s) and its byte offset in the codestream */
@@ -135,7 +142,7 @@ struct EncoderPriv {
     uint32_t rel_pkt = 0, rel_off = 0;
     bool rel_failed = false;
     std::mutex mu;
-    std::condition_variable cv_free, cv_pending;
+    std::condition_variable cv_free, cv_pending, cv_watch;
     hipStream_t stream = nullptr;
     uint64_t in_bytes = 0;
     size_t plane_off[4] = {0, 0, 0, 0};
@@ -349,6 +356,14 @@ static bool version_ok(uint64_t maj, uint64_t min) {
 static void encoder_destroy(EncoderPriv* e) {
     if (!e)
         return;
+    if (e->finisher.joinable()) {
+        {
+            std::lock_guard<std::mutex> lk(e->mu);
+            e->stop = true;
+        }
+        e->cv_watch.notify_all();
+        e->finisher.join();
+    }
     if (e->stream)
         (void)hipStreamSynchronize(e->stream);
     for (auto& s : e->slots) {
@@ -381,6 +396,39 @@ static svt_jpeg_xs_amd_frame_spec_t slot_spec(const EncoderPriv* e, svt_jpeg_xs_
     }
     spec.d_out = s.d_out;
     return spec;
+}
+
+/* The finisher thread: takes the frames in send order, waits for each one's
+ * GPU work, marks it finished and announces it through
+ * callback_get_data_available, once per packet get_packet will hand out for
+ * it (the whole frame, or the header and one packet per slice), as the
+ * reference's final stage does when it posts output (FinalStageProcess.c:
+ * 173-175, 208-209, 236-237). */
+static void finisher_loop(EncoderPriv* e) {
+    const uint32_t packets = e->slice_mode ? 1 + e->geom.host.g.slices : 1;
+    for (;;) {
+        int slot;
+        {
+            std::unique_lock<std::mutex> lk(e->mu);
+            e->cv_watch.wait(lk, [&] { return e->stop || !e->watch.empty(); });
+            if (e->watch.empty())
+                return;
+            slot = e->watch.front();
+            e->watch.pop_front();
+        }
+        EncoderPriv::Slot& s = e->slots[slot];
+        const hipError_t he = hipEventSynchronize(s.done);
+        {
+            std::lock_guard<std::mutex> lk(e->mu);
+            s.failed = he != hipSuccess || *s.h_status != 0;
+            s.finished = true;
+        }
+        e->cv_pending.notify_all();
+        svt_jpeg_xs_encoder_api_t* api = e->api;
+        if (api->callback_get_data_available)
+            for (uint32_t k = 0; k < packets; k++)
+                api->callback_get_data_available(api, api->callback_get_data_available_context);
+    }
 }
 
 static SvtJxsErrorType_t encoder_create(svt_jpeg_xs_encoder_api_t* api) {
@@ -462,6 +510,14 @@ static SvtJxsErrorType_t encoder_create(svt_jpeg_xs_encoder_api_t* api) {
             encoder_destroy(e);
             return SvtJxsErrorInsufficientResources;
         }
+    }
+    e->api = api;
+    try {
+        e->finisher = std::thread(finisher_loop, e);
+    } catch (...) {
+        api->private_ptr = nullptr;
+        encoder_destroy(e);
+        return SvtJxsErrorInsufficientResources;
     }
     return SvtJxsErrorNone;
 }
@@ -629,30 +685,27 @@ PREFIX_API SvtJxsErrorType_t svt_jpeg_xs_encoder_send_picture(svt_jpeg_xs_encode
         e->cv_free.notify_one();
         return SvtJxsErrorUndefined;
     }
+    s.finished = false;
     e->pending.push_back(slot);
-    e->cv_pending.notify_one();
+    e->watch.push_back(slot);
+    e->cv_watch.notify_one();
     return SvtJxsErrorNone;
 }
 
-/* Takes the oldest frame off the pending queue once its GPU work is done
- * (waiting for it when blocking); -1 when none is ready. */
+/* Takes the oldest frame off the pending queue once the finisher has seen
+ * its GPU work complete (waiting for that when blocking); -1 when none is
+ * ready. */
 static int next_finished(EncoderPriv* e, bool blocking, bool& failed) {
-    int slot;
-    {
-        std::unique_lock<std::mutex> lk(e->mu);
-        if (e->pending.empty()) {
-            if (!blocking)
-                return -1;
-            e->cv_pending.wait(lk, [&] { return !e->pending.empty(); });
-        }
-        slot = e->pending.front();
-        if (!blocking && hipEventQuery(e->slots[slot].done) == hipErrorNotReady)
+    std::unique_lock<std::mutex> lk(e->mu);
+    auto ready = [&] { return !e->pending.empty() && e->slots[e->pending.front()].finished; };
+    if (!ready()) {
+        if (!blocking)
             return -1;
-        e->pending.pop_front();
+        e->cv_pending.wait(lk, ready);
     }
-    EncoderPriv::Slot& s = e->slots[slot];
-    const hipError_t he = hipEventSynchronize(s.done);
-    failed = he != hipSuccess || *s.h_status != 0;
+    const int slot = e->pending.front();
+    e->pending.pop_front();
+    failed = e->slots[slot].failed;
     return slot;
 }
 
@@ -812,23 +865,6 @@ PREFIX_API SvtJxsErrorType_t svt_jpeg_xs_amd_plan_status(svt_jpeg_xs_amd_plan_t*
 }
 
 PREFIX_API void svt_jpeg_xs_amd_plan_destroy(svt_jpeg_xs_amd_plan_t* plan) { plan_free(plan); }
-
-PREFIX_API SvtJxsErrorType_t svt_jpeg_xs_amd_plan_debug_copy(svt_jpeg_xs_amd_plan_t* plan, uint32_t frame, uint16_t* coeff,
-                                                             uint8_t* gcli, uint32_t* prec) {
-    if (!plan || frame >= plan->nfr)
-        return SvtJxsErrorBadParameter;
-    JxsFrameDesc F;
-    if (hipMemcpy(&F, plan->L.frames + frame, sizeof(F), hipMemcpyDeviceToHost) != hipSuccess)
-        return SvtJxsErrorUndefined;
-    const JxsGeom& g = plan->geoms[frame]->host.g;
-    if (coeff && hipMemcpy(coeff, F.coeff, g.n_coeff * 2, hipMemcpyDeviceToHost) != hipSuccess)
-        return SvtJxsErrorUndefined;
-    if (gcli && hipMemcpy(gcli, F.gcli, g.n_gcli, hipMemcpyDeviceToHost) != hipSuccess)
-        return SvtJxsErrorUndefined;
-    if (prec && hipMemcpy(prec, F.prec, g.rows * sizeof(JxsPrec), hipMemcpyDeviceToHost) != hipSuccess)
-        return SvtJxsErrorUndefined;
-    return SvtJxsErrorNone;
-}
 
 PREFIX_API SvtJxsErrorType_t svt_jpeg_xs_amd_plan_precinct_offsets(svt_jpeg_xs_amd_plan_t* plan, uint32_t frame,
                                                                    const uint32_t** d_offsets, uint32_t* stride) {

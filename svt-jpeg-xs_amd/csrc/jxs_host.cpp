@@ -681,6 +681,8 @@ int32_t parse_codestream_header(const uint8_t* cs, size_t len, HostGeom& out, ui
             lh = r.get(1);
             r.get(1);
             g.qpih = r.get(2);
+            if (g.qpih > 1) /* ParseHeader.c:238-241 */
+                return E_INVALID_BS;
             if (r.get(2) != 0) /* Fs: separate sign coding is not decoded */
                 return E_INVALID_BS;
             r.get(2); /* Rm: only used by vertically predicted bands, rejected per precinct */

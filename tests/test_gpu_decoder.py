@@ -217,52 +217,89 @@ def test_decode_plan_mixes_full_and_proxy_decoders():
 
 @pytest.mark.gpu
 def test_send_packet_decodes_chunked_codestreams():
-    """svt_jpeg_xs_decoder_send_packet (SvtJpegxsDec.h:108-118): two
-    codestreams fed back to back in 700-byte chunks, one chunk straddling
-    them.  Every chunk is consumed; a call returns BitstreamTooShort until a
-    frame is complete, then None, and get_frame returns that frame, equal to
-    the oracle's decode."""
+    """svt_jpeg_xs_decoder_send_packet in packetization mode
+    (DecThreadInit.c:347-460): three codestreams fed back to back, as
+    DecAppMain.c:218-239 does, in 700-byte chunks and then in chunks larger
+    than two frames.  A call takes at most the rest of the current frame
+    (bytes_used), returns BitstreamTooShort until the frame is complete, then
+    None; get_frame returns the frames in order, equal to the oracle's decode,
+    then EndOfCodestream after send_eoc."""
     import ctypes
     print("send_packet chunks", flush=True)
     P, O = load_package(), load_oracle()
     cfg = (256, 64, 10, 3, 3, 2, 5, 0, 1)
-    css = [_codestream(cfg, frame=k)[1] for k in range(2)]
-    dec = P.Decoder(css[0], verbose=0)
+    css = [_codestream(cfg, frame=k)[1] for k in range(3)]
     L = P.lib()
-    c = dec.config
+    for chunk_size in (700, 2 * len(css[0]) + 100):
+        dec = P.Decoder(css[0], verbose=0, packetization_mode=1)
+        c = dec.config
 
-    def new_image():
-        planes = [np.zeros((c.components[k].height, c.components[k].width), dtype=np.uint16)
-                  for k in range(c.components_num)]
-        f = P.Frame()
-        for k, p in enumerate(planes):
-            f.image.data_yuv[k] = p.ctypes.data
-            f.image.stride[k] = p.shape[1]
-            f.image.alloc_size[k] = p.nbytes
-        return planes, f
+        def new_image():
+            planes = [np.zeros((c.components[k].height, c.components[k].width), dtype=np.uint16)
+                      for k in range(c.components_num)]
+            f = P.Frame()
+            for k, p in enumerate(planes):
+                f.image.data_yuv[k] = p.ctypes.data
+                f.image.stride[k] = p.shape[1]
+                f.image.alloc_size[k] = p.nbytes
+            return planes, f
 
-    stream = np.frombuffer(b"".join(css), dtype=np.uint8).copy()
-    planes, f = new_image()
-    done = []
-    for pos in range(0, len(stream), 700):
-        chunk = stream[pos:pos + 700]
-        f.bitstream.buffer = chunk.ctypes.data
-        f.bitstream.allocation_size = len(chunk)
-        f.bitstream.used_size = len(chunk)
-        used = ctypes.c_uint32(0)
-        rc = L.svt_jpeg_xs_decoder_send_packet(ctypes.byref(dec.api), ctypes.byref(f), ctypes.byref(used))
-        assert used.value == len(chunk)
-        if rc == P.ERR_DEC_TOO_SHORT:
-            continue
-        assert rc == 0
+        stream = np.frombuffer(b"".join(css), dtype=np.uint8).copy()
+        pos, sent = 0, []
+        while pos < len(stream):
+            planes, f = new_image()
+            while True:
+                n = min(chunk_size, len(stream) - pos)
+                f.bitstream.buffer = stream[pos:].ctypes.data
+                f.bitstream.allocation_size = n
+                f.bitstream.used_size = n
+                used = ctypes.c_uint32(0)
+                rc = L.svt_jpeg_xs_decoder_send_packet(ctypes.byref(dec.api), ctypes.byref(f), ctypes.byref(used))
+                assert 0 < used.value <= n
+                pos += used.value
+                if rc != P.ERR_DEC_TOO_SHORT:
+                    break
+            assert rc == 0
+            sent.append(planes)
+        assert len(sent) == 3 and pos == len(stream)
+        assert L.svt_jpeg_xs_decoder_send_eoc(ctypes.byref(dec.api)) == 0
+        for k in range(3):
+            out = P.Frame()
+            assert L.svt_jpeg_xs_decoder_get_frame(ctypes.byref(dec.api), ctypes.byref(out), 1) == 0
         out = P.Frame()
-        assert L.svt_jpeg_xs_decoder_get_frame(ctypes.byref(dec.api), ctypes.byref(out), 1) == 0
-        done.append(planes)
-        planes, f = new_image()
-    dec.close()
-    assert len(done) == 2
-    for k, got in enumerate(done):
-        e, ref = O.decode(css[k])
+        assert L.svt_jpeg_xs_decoder_get_frame(ctypes.byref(dec.api), ctypes.byref(out), 1) == P.ERR_DEC_EOC
+        dec.close()
+        for k, got in enumerate(sent):
+            e, ref = O.decode(css[k])
+            assert e == 0
+            for a, b in zip(got, ref):
+                assert np.array_equal(a, b), "chunk %d frame %d" % (chunk_size, k)
+
+
+@pytest.mark.gpu
+def test_decoder_follows_qpih_per_frame():
+    """Qpih is a per-frame header field (ParseHeader.c:238-241, used per
+    precinct at Decoder.c:576): one decoder fed a deadzone codestream, then a
+    uniform one of the same geometry, and so on, decodes each to the oracle's
+    planes; a codestream with the reserved Qpih = 2 is refused."""
+    print("qpih per frame", flush=True)
+    P, O = load_package(), load_oracle()
+    dz = _codestream((160, 90, 12, 4, 4, 2, 5, 0, 1))[1]
+    un = _codestream((160, 90, 12, 4, 4, 2, 5, 1, 1))[1]
+    assert dz != un
+    dec = P.Decoder(dz, verbose=0)
+    for cs in (dz, un, dz, un):
+        rc, got = dec.decode(cs)
+        assert rc == 0
+        e, ref = O.decode(cs)
         assert e == 0
         for a, b in zip(got, ref):
-            assert np.array_equal(a, b), "frame %d" % k
+            assert np.array_equal(a, b)
+    # PIH byte 27 (from the marker) holds Lh, Rl, Qpih (2 bits), Fs, Rm
+    bad = bytearray(dz)
+    off = bytes(bad).index(b"\xff\x12") + 27
+    assert (bad[off] >> 4) & 3 == 0
+    bad[off] = (bad[off] & 0xCF) | 0x20
+    rc, _ = dec.decode(bytes(bad))
+    assert rc == P.ERR_DEC_INVALID_BITSTREAM
+    dec.close()
