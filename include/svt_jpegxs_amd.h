@@ -58,6 +58,23 @@ PREFIX_API void svt_jpeg_xs_amd_plan_destroy(svt_jpeg_xs_amd_plan_t* plan);
 /* Device bytes a plan of these frames allocates (inputs and outputs excluded). */
 PREFIX_API uint64_t svt_jpeg_xs_amd_plan_footprint(const svt_jpeg_xs_amd_frame_spec_t* frames, uint32_t count);
 
+/* Encodes one frame whose planes and codestream buffer are in HBM, on
+ * `stream`, without allocating: no hipMalloc, hipFree or other allocating
+ * routine is called.  No counterpart in the reference: this entry point is for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid allocation).  The frame's work
+ * block lives in a 64 MiB device array the library holds on every GPU it runs
+ * on until the process ends; a frame that needs more (2160p and larger) is
+ * refused with SvtJxsErrorBadParameter before anything is enqueued.
+ * `d_status` is the device address of a 4-byte-aligned u32 that receives the
+ * frame's error flags (0: the codestream is complete).  The result equals
+ * svt_jpeg_xs_amd_plan_create + _run + _status for the same frame.  Every call
+ * on one GPU shares that GPU's array: calls that overlap on one GPU (other
+ * streams, other threads) race and must be ordered by the caller; calls on one
+ * stream are ordered by it. */
+PREFIX_API SvtJxsErrorType_t svt_jpeg_xs_amd_encode_noalloc(const svt_jpeg_xs_amd_frame_spec_t* frame, void* stream,
+                                                            uint32_t* d_status);
+
 /* Device address of the precinct-offset table the encode plan wrote for
  * frame `frame` (byte offset of every precinct header in its codestream,
  * one u32 every *stride u32).  Valid after the plan has run; a decode plan of

@@ -153,7 +153,7 @@ EXPORTED_SYMBOLS = [
     "svt_jpeg_xs_amd_decode_plan_destroy", "svt_jpeg_xs_amd_plan_precinct_offsets",
     "svt_jpeg_xs_amd_server_create", "svt_jpeg_xs_amd_server_submit", "svt_jpeg_xs_amd_server_poll",
     "svt_jpeg_xs_amd_server_stats", "svt_jpeg_xs_amd_server_now_ns", "svt_jpeg_xs_amd_server_destroy",
-    "svt_jpeg_xs_amd_autotune",
+    "svt_jpeg_xs_amd_autotune", "svt_jpeg_xs_amd_encode_noalloc",
 ]
 
 _lib = None
@@ -211,6 +211,8 @@ def lib():
         L.svt_jpeg_xs_amd_version.restype = ctypes.c_char_p
         L.svt_jpeg_xs_amd_autotune.argtypes = [E, ctypes.c_void_p, ctypes.POINTER(Tile), ctypes.c_uint32]
         L.svt_jpeg_xs_amd_autotune.restype = ctypes.c_int32
+        L.svt_jpeg_xs_amd_encode_noalloc.argtypes = [ctypes.POINTER(FrameSpec), ctypes.c_void_p, ctypes.c_void_p]
+        L.svt_jpeg_xs_amd_encode_noalloc.restype = ctypes.c_int32
         L.svt_jpeg_xs_amd_server_create.argtypes = [ctypes.POINTER(ServerConfig)]
         L.svt_jpeg_xs_amd_server_create.restype = ctypes.c_void_p
         L.svt_jpeg_xs_amd_server_submit.argtypes = [ctypes.c_void_p, ctypes.POINTER(FrameSpec), ctypes.c_uint64]
@@ -452,6 +454,15 @@ def _spec(spec, enc, planes, strides, out):
         spec.d_planes[c] = p
         spec.strides[c] = s
     spec.d_out = out
+
+
+def encode_noalloc(enc, planes, strides, out, d_status, stream=0):
+    """svt_jpeg_xs_amd_encode_noalloc: one frame (device plane pointers,
+    strides, device output pointer) on `stream`, its error flags written to
+    the device u32 at d_status.  Returns the entry point's status."""
+    spec = FrameSpec()
+    _spec(spec, enc, planes, strides, out)
+    return lib().svt_jpeg_xs_amd_encode_noalloc(ctypes.byref(spec), stream, d_status)
 
 
 class Plan:

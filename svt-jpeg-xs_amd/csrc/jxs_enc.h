@@ -80,6 +80,10 @@ const DevGeom* spec_geom(const svt_jpeg_xs_amd_frame_spec_t& s);
  * what the transform kernel's layout covers. */
 bool batch_place(const svt_jpeg_xs_amd_frame_spec_t* specs, uint32_t count, uint8_t* base, BatchLaunch& L,
                  std::vector<uint8_t>& img);
+/* The same into a caller's buffer of exactly BatchSizer::upload_bytes()
+ * bytes, with no allocation. */
+bool batch_place_into(const svt_jpeg_xs_amd_frame_spec_t* specs, uint32_t count, uint8_t* base, BatchLaunch& L,
+                      uint8_t* img, uint64_t img_bytes);
 
 /* The launch sequence of a batch on one stream.  ev (optional) gets an event
  * recorded before every stage and after the last. */

@@ -79,7 +79,7 @@ SvtJxsErrorType_t DevGeom::upload() {
     FwdJobs jobs;
     fwd_jobs(host.g, planes, strides, nullptr, nullptr, nullptr, nullptr, nullptr, jobs);
     for (int s = 0; s < 2; s++)
-        njobs[s] = (uint32_t)jobs.dwt[s].size();
+        njobs[s] = jobs.n[s];
     return SvtJxsErrorNone;
 }
 
@@ -174,14 +174,28 @@ const DevGeom* spec_geom(const svt_jpeg_xs_amd_frame_spec_t& s) {
 bool batch_place(const svt_jpeg_xs_amd_frame_spec_t* specs, uint32_t count, uint8_t* base, BatchLaunch& L,
                  std::vector<uint8_t>& img) {
     BatchSizer sz;
-    std::vector<const DevGeom*> geoms(count);
     for (uint32_t i = 0; i < count; i++) {
-        geoms[i] = spec_geom(specs[i]);
-        if (!geoms[i])
+        const DevGeom* dg = spec_geom(specs[i]);
+        if (!dg)
             return false;
-        sz.add(*geoms[i]);
+        sz.add(*dg);
     }
     img.assign(sz.upload_bytes(), 0);
+    return batch_place_into(specs, count, base, L, img.data(), img.size());
+}
+
+bool batch_place_into(const svt_jpeg_xs_amd_frame_spec_t* specs, uint32_t count, uint8_t* base, BatchLaunch& L,
+                      uint8_t* img, uint64_t img_bytes) {
+    BatchSizer sz;
+    for (uint32_t i = 0; i < count; i++) {
+        const DevGeom* dg = spec_geom(specs[i]);
+        if (!dg)
+            return false;
+        sz.add(*dg);
+    }
+    if (img_bytes != sz.upload_bytes())
+        return false;
+    memset(img, 0, img_bytes);
     L = BatchLaunch();
     L.nfr = count;
     /* block: frame descriptors | job list and block bases per stage and
@@ -209,7 +223,7 @@ bool batch_place(const svt_jpeg_xs_amd_frame_spec_t* specs, uint32_t count, uint
     uint32_t prec_base = 0, slice_base = 0, unit_base = 0;
     uint32_t fill[2][3] = {}, blk[2][3] = {};
     for (uint32_t i = 0; i < count; i++) {
-        const DevGeom* dg = geoms[i];
+        const DevGeom* dg = spec_geom(specs[i]);
         const JxsGeom& g = dg->host.g;
         JxsFrameDesc F;
         memset(&F, 0, sizeof(F));
@@ -242,22 +256,23 @@ bool batch_place(const svt_jpeg_xs_amd_frame_spec_t* specs, uint32_t count, uint
         F.pps = g.pps;
         F.slices = g.slices;
         F.n_bl = g.n_bl;
-        memcpy(img.data() + frames_off + (uint64_t)i * sizeof(JxsFrameDesc), &F, sizeof(F));
+        memcpy(img + frames_off + (uint64_t)i * sizeof(JxsFrameDesc), &F, sizeof(F));
         prec_base += g.rows;
         slice_base += g.slices;
         unit_base += g.rows * g.n_units;
         FwdJobs jobs;
         fwd_jobs(g, specs[i].d_planes, specs[i].strides, F.coeff, F.gcli, F.smax, F.hist, scratch, jobs);
         for (int s = 0; s < 2; s++)
-            for (auto j : jobs.dwt[s]) {
+            for (uint32_t ji = 0; ji < jobs.n[s]; ji++) {
+                JxsDwtJob j = jobs.dwt[s][ji];
                 const int c = j.chunk == 8 ? 0 : j.chunk == 16 ? 1 : j.chunk == 32 ? 2 : -1;
                 /* outside what the transform kernel's layout covers */
                 if (c < 0 || j.lev_l < 1 || (1u << j.lev_l) > j.chunk || j.tpr > (j.chunk == 8 ? 512u : 256u) ||
                     j.rpb == 0 || fill[s][c] >= sz.njobs[s])
                     return false;
                 j.base = blk[s][c];
-                memcpy(img.data() + job_off[s][c] + (uint64_t)fill[s][c] * sizeof(JxsDwtJob), &j, sizeof(j));
-                memcpy(img.data() + base_off[s][c] + (uint64_t)fill[s][c] * 4, &blk[s][c], 4);
+                memcpy(img + job_off[s][c] + (uint64_t)fill[s][c] * sizeof(JxsDwtJob), &j, sizeof(j));
+                memcpy(img + base_off[s][c] + (uint64_t)fill[s][c] * 4, &blk[s][c], 4);
                 fill[s][c]++;
                 blk[s][c] += j.nblk;
             }
@@ -898,7 +913,8 @@ PREFIX_API int32_t svt_jpeg_xs_amd_autotune(svt_jpeg_xs_encoder_api_t* enc_api, 
     fwd_jobs(e->geom.host.g, planes, strides, nullptr, nullptr, nullptr, nullptr, nullptr, jobs);
     std::vector<JxsDwtJob> shapes;
     for (int sg = 0; sg < 2; sg++)
-        for (const auto& j : jobs.dwt[sg]) {
+        for (uint32_t ji = 0; ji < jobs.n[sg]; ji++) {
+            const JxsDwtJob& j = jobs.dwt[sg][ji];
             bool seen = false;
             for (const auto& k : shapes)
                 seen |= k.kind == j.kind && k.step == j.step && k.vert == j.vert && k.lev_l == j.lev_l && k.w == j.w;
@@ -1018,6 +1034,120 @@ PREFIX_API int32_t svt_jpeg_xs_amd_autotune(svt_jpeg_xs_encoder_api_t* enc_api, 
         s.plan = p;
     }
     return ok ? tuned : -1;
+}
+
+/* ---------------- encode without allocation ---------------- */
+
+}  // extern "C"
+
+namespace jxs {
+
+/* The work block of svt_jpeg_xs_amd_encode_noalloc: one frame's descriptors,
+ * job lists, packing units and work buffers (the block a one-frame plan
+ * allocates), in a file-scope device array that every GPU the library runs
+ * on holds from module load until the process ends.  64 MiB holds a 1080p
+ * 4:4:4 12-bit frame (46 MiB) or a 1080p 4:2:2 frame (32 MiB); 2160p frames
+ * need more and are refused.  Host code never uses the array's name as an
+ * address: it uses the per-device address from hipGetSymbolAddress. */
+#define JXS_NOALLOC_BYTES (64ull << 20)
+static __device__ __align__(256) uint8_t scratch[JXS_NOALLOC_BYTES];
+
+/* Host staging of one frame's descriptor image (on the stack of a call). */
+static const uint32_t kNoallocImg = 32768;
+static_assert(256 + 6 * (((JXS_MAXC * sizeof(JxsDwtJob) + 255) & ~(size_t)255) + 256) <= kNoallocImg,
+              "one frame's descriptor image must fit the staging buffer");
+
+/* Kernel-argument upload: the descriptor image travels in the arguments of
+ * small copy launches, which the runtime captures at launch, so the host
+ * staging may be reused as soon as the launch is enqueued. */
+#define JXS_UPLOAD_WORDS 128 /* 16-byte words per launch */
+struct JxsUploadChunk {
+    uint4 w[JXS_UPLOAD_WORDS];
+};
+
+__global__ void __launch_bounds__(JXS_UPLOAD_WORDS) k_upload(uint4* dst, uint32_t nw, JxsUploadChunk c) {
+    const uint32_t i = threadIdx.x;
+    if (i < nw)
+        dst[i] = c.w[i];
+}
+
+static hipError_t upload_by_args(uint8_t* dst, const uint8_t* src, uint32_t bytes, hipStream_t st) {
+    const uint32_t words = (bytes + 15) / 16;
+    for (uint32_t w0 = 0; w0 < words; w0 += JXS_UPLOAD_WORDS) {
+        JxsUploadChunk c;
+        const uint32_t n = words - w0 < JXS_UPLOAD_WORDS ? words - w0 : JXS_UPLOAD_WORDS;
+        memset(&c, 0, sizeof(c));
+        memcpy(&c, src + (size_t)w0 * 16, (size_t)n * 16);
+        hipLaunchKernelGGL(k_upload, dim3(1), dim3(JXS_UPLOAD_WORDS), 0, st, (uint4*)(dst + (size_t)w0 * 16), n, c);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess)
+            return e;
+    }
+    return hipSuccess;
+}
+
+#define JXS_MAX_DEVICES 64
+static std::mutex g_scratch_mu;
+static uint8_t* g_scratch_ptr[JXS_MAX_DEVICES];
+
+/* The scratch array's device address on the current device, looked up once
+ * per device (with that device current); null on failure. */
+static uint8_t* scratch_address(int dev) {
+    if (dev < 0 || dev >= JXS_MAX_DEVICES)
+        return nullptr;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        uint8_t* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress((void**)&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) {
+            fprintf(stderr, "svt-jpeg-xs-amd: hipGetSymbolAddress(scratch) failed: %s\n", hipGetErrorString(e));
+            return nullptr;
+        }
+        g_scratch_ptr[dev] = ptr;
+    }
+    return g_scratch_ptr[dev];
+}
+
+}  // namespace jxs
+
+extern "C" {
+
+PREFIX_API SvtJxsErrorType_t svt_jpeg_xs_amd_encode_noalloc(const svt_jpeg_xs_amd_frame_spec_t* frame, void* stream,
+                                                            uint32_t* d_status) {
+    if (!frame || !d_status || ((uintptr_t)d_status & 3) != 0)
+        return SvtJxsErrorBadParameter;
+    const DevGeom* dg = spec_geom(*frame);
+    if (!dg)
+        return SvtJxsErrorBadParameter;
+    BatchSizer sz;
+    sz.add(*dg);
+    if (sz.bytes() > JXS_NOALLOC_BYTES || sz.upload_bytes() > kNoallocImg) {
+        fprintf(stderr,
+                "svt-jpeg-xs-amd: encode_noalloc: the frame needs %llu work bytes, more than the %llu-byte scratch\n",
+                (unsigned long long)sz.bytes(), (unsigned long long)JXS_NOALLOC_BYTES);
+        return SvtJxsErrorBadParameter;
+    }
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess)
+        return SvtJxsErrorUndefined;
+    uint8_t* base = scratch_address(dev);
+    if (!base)
+        return SvtJxsErrorInsufficientResources;
+    alignas(16) uint8_t img[kNoallocImg];
+    BatchLaunch L;
+    if (!batch_place_into(frame, 1, base, L, img, sz.upload_bytes()))
+        return SvtJxsErrorBadParameter;
+    /* the frame's error word is the caller's (cleared by the first launch) */
+    JxsFrameDesc F;
+    memcpy(&F, img, sizeof(F));
+    F.status = d_status;
+    memcpy(img, &F, sizeof(F));
+    L.status = d_status;
+    hipStream_t st = (hipStream_t)stream;
+    if (upload_by_args(base, img, (uint32_t)sz.upload_bytes(), st) != hipSuccess ||
+        batch_launch(L, st, nullptr) != hipSuccess)
+        return SvtJxsErrorUndefined;
+    return SvtJxsErrorNone;
 }
 
 PREFIX_API const char* svt_jpeg_xs_amd_version(void) { return "svt-jpeg-xs-amd 0.1 (gfx950, API 0.10)"; }

@@ -844,7 +844,7 @@ void fwd_jobs(const JxsGeom& g, const void* const planes[4], const uint32_t stri
                 j.hf[k] = band(nb - 1 - k);
             j.lf_is_band = 1;
             j.lf = band(0);
-            jobs.dwt[0].push_back(finish_job(j, g.packed ? c : 0));
+            jobs.push(0, finish_job(j, g.packed ? c : 0));
             continue;
         }
         const uint32_t w1 = (cp.w + 1) / 2, h1 = (cp.h + 1) / 2;
@@ -862,7 +862,7 @@ void fwd_jobs(const JxsGeom& g, const void* const planes[4], const uint32_t stri
                 j.hf[k] = band(cp.dh - k);
             j.lf_is_band = 1;
             j.lf = band(0);
-            jobs.dwt[0].push_back(finish_job(j, g.packed ? c : 0));
+            jobs.push(0, finish_job(j, g.packed ? c : 0));
             continue;
         }
         int32_t* B = scr;
@@ -871,7 +871,7 @@ void fwd_jobs(const JxsGeom& g, const void* const planes[4], const uint32_t stri
         j.lf_is_band = 0;
         j.lf_i32 = B;
         j.lf_stride = w1;
-        jobs.dwt[0].push_back(finish_job(j, g.packed ? c : 0));
+        jobs.push(0, finish_job(j, g.packed ? c : 0));
         /* second vertical level on the w1 x h1 low-pass plane */
         JxsDwtJob k2;
         memset(&k2, 0, sizeof(k2));
@@ -891,7 +891,7 @@ void fwd_jobs(const JxsGeom& g, const void* const planes[4], const uint32_t stri
         k2.lf = band(0);
         k2.lh = band(nb - 5);
         k2.hh = band(nb - 4);
-        jobs.dwt[1].push_back(finish_job(k2));
+        jobs.push(1, finish_job(k2));
     }
 }
 

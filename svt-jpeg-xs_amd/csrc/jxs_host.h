@@ -50,7 +50,9 @@ void assign_buffers(JxsGeom& g);
  * levels); the input planes are device addresses.  dwt[0]: first vertical
  * level (or horizontal-only components), dwt[1]: second vertical level. */
 struct FwdJobs {
-    std::vector<JxsDwtJob> dwt[2];
+    JxsDwtJob dwt[2][JXS_MAXC]; /* at most one job per component and stage */
+    uint32_t n[2] = {0, 0};
+    void push(int s, const JxsDwtJob& j) { dwt[s][n[s]++] = j; }
 };
 void fwd_jobs(const JxsGeom& g, const void* const planes[4], const uint32_t strides[4], uint16_t* coeff, uint8_t* gcli,
               uint8_t* smax, uint16_t* hist, int32_t* scratch, FwdJobs& jobs);
