@@ -94,6 +94,10 @@ int main(int argc, char** argv) {
         enc.coding_significance = (uint32_t)atoi(argv[15]);
     if (argc > 16)
         enc.rate_control_mode = (uint32_t)atoi(argv[16]);
+    if (argc > 17)
+        enc.coding_vertical_prediction_mode = (uint32_t)atoi(argv[17]);
+    if (argc > 18)
+        enc.coding_signs_handling = (uint32_t)atoi(argv[18]);
     /* JXS_REF_SIMD=1 enables the reference's AVX2 / AVX-512 intrinsic kernels
      * (the SSE2 bit stays clear: the only SSE2 entries are the unassembled NASM
      * routines, whose C twins are then selected).  Output is identical either

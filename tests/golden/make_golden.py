@@ -73,6 +73,11 @@ FULL = {
     "c3c_3840x2160_rgb_10b_bpp3_v2": (3840, 2160, 10, 4, 3, 1, 2, 5, 0, 16, 1, 4.0, 0),
     "c3d_1920x540_422_10b_bpp5_v1": (1920, 540, 10, 3, 5, 1, 1, 5, 0, 16, 1, 4.0, 0),
     "c5_7680x4320_rgbpacked_14b_bpp2_v1": (7680, 4320, 14, 21, 2, 1, 1, 5, 0, 16, 1, 16.0, 0),
+    # the other decomp_v of each mixed-batch kind
+    "c3a_1280x720_420_8b_bpp1_v1": (1280, 720, 8, 2, 1, 1, 1, 5, 0, 16, 1, 4.0, 0),
+    "c3b_1920x1080_444_12b_bpp4_v1": (1920, 1080, 12, 4, 4, 1, 1, 5, 0, 16, 1, 4.0, 0),
+    "c3c_3840x2160_rgb_10b_bpp3_v1": (3840, 2160, 10, 4, 3, 1, 1, 5, 0, 16, 1, 4.0, 0),
+    "c3d_1920x540_422_10b_bpp5_v2": (1920, 540, 10, 3, 5, 1, 2, 5, 0, 16, 1, 4.0, 0),
 }
 
 # Quantisation known answers restated from tests/UnitTests/TestQuant.cc:27-136
