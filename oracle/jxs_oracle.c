@@ -62,6 +62,8 @@ typedef struct {
     uint32_t depth, format, lcod, qpih, sig, hdr_len;
     uint32_t rc_mode; /* 0: budget per precinct; 1: unused padding moves to the next precinct of the slice;
                          2, 3: one Q and R for the slice sizes the precinct budgets (3: with a minimum size) */
+    uint32_t vpred;   /* vertical prediction: 0 off, 1 zero residuals (Rm = 0), 2 zero coefficients (Rm = 1) */
+    uint32_t signs;   /* sign coding: 0 with the data (Fs = 0), 1 fast, 2 full (Fs = 1) */
     uint8_t hdr[512];
     uint32_t* slice_sizes;
 } ob_pi;
@@ -495,8 +497,8 @@ static uint32_t write_frame_header(ob_pi* pi, uint8_t* out) {
     bw_put(&w, !pi->short_hdr, 1);
     bw_put(&w, 1, 1); /* Rl */
     bw_put(&w, pi->qpih, 2);
-    bw_put(&w, 0, 2); /* Fs */
-    bw_put(&w, 0, 2); /* Rm */
+    bw_put(&w, pi->signs ? 1u : 0u, 2);     /* Fs (EncHandle.c:274-278) */
+    bw_put(&w, pi->vpred == 2 ? 1u : 0u, 2); /* Rm (EncHandle.c:250-253) */
     bw_put(&w, 0xFF13, 16); /* CDT */
     bw_put(&w, 2 * pi->nc + 2, 16);
     for (uint32_t c = 0; c < pi->nc; c++) {
@@ -530,6 +532,8 @@ void jxso_default_params(jxso_params_t* p) {
     p->slice_height = 16;
     p->significance = 1;
     p->rc_mode = 1; /* RC_CBR_PER_PRECINCT_MOVE_PADDING (EncHandle.c:519) */
+    p->vpred = 0;
+    p->signs = 0;
 }
 
 static int32_t enc_configure(const jxso_params_t* prm, ob_pi* pi) {
@@ -538,6 +542,16 @@ static int32_t enc_configure(const jxso_params_t* prm, ob_pi* pi) {
     if (prm->depth < 8 || prm->depth > 14)
         return E_BAD_PARAM;
     if (prm->rc_mode > 3)
+        return E_BAD_PARAM;
+    /* EncHandle.c:242-278 */
+    if (prm->vpred > 2 || prm->signs > 2)
+        return E_BAD_PARAM;
+    /* Not restated: full sign coding with the per-slice modes, where the
+     * reference's per-band cache of two truncations carries sizes from the
+     * slice-wide search (computed with fast sign coding,
+     * RateControl.c:1336-1342) into the first precinct's own search
+     * (RateControl.c:1285-1290 resets it only below another precinct). */
+    if (prm->signs == 2 && prm->rc_mode >= 2)
         return E_BAD_PARAM;
     if (sampling(prm->format, &nc, sx, sy))
         return E_BAD_PARAM;
@@ -602,6 +616,8 @@ static int32_t enc_configure(const jxso_params_t* prm, ob_pi* pi) {
     pi->qpih = prm->quantization;
     pi->sig = !!prm->significance;
     pi->rc_mode = prm->rc_mode;
+    pi->vpred = prm->vpred;
+    pi->signs = prm->signs;
     if (weights_assign(pi, prm->format))
         return E_BAD_PARAM;
     max_qr(pi);
@@ -884,12 +900,15 @@ typedef struct {
 typedef struct {
     uint32_t row, type;
     line_lut lut[4][NBC][2];
+    /* vertical prediction: the precinct above in the slice, its truncations */
+    int has_top;
+    uint8_t top_gtli[4][NBC];
     /* result of the last evaluation */
     uint8_t gtli[4][NBC];
-    uint8_t method_sig[4][NBC];
-    uint32_t bits_data[4][NBC][2], bits_gcli[4][NBC][2], bits_sig[4][NBC][2];
+    uint8_t method[4][NBC]; /* CodingMethodBand: 0 zero prediction, 1 + significance, 2 vertical, 3 vertical + significance */
+    uint32_t bits_data[4][NBC][2], bits_gcli[4][NBC][2], bits_sig[4][NBC][2], bits_sign[4][NBC][2];
     uint8_t raw[NPK];
-    uint32_t pk_data[NPK], pk_gcli[NPK], pk_sig[NPK];
+    uint32_t pk_data[NPK], pk_gcli[NPK], pk_sig[NPK], pk_sign[NPK];
 } prec_state;
 
 static void build_luts(const ob_pi* pi, const enc_frame* f, prec_state* ps) {
@@ -924,14 +943,116 @@ static void build_luts(const ob_pi* pi, const enc_frame* f, prec_state* ps) {
         }
 }
 
+/* ----------------------------------------------------------------------
+ * Quantisation (Quant.c:9-45) applied while packing: the value packed for a
+ * coefficient of a group with gcli > gtli.
+ * ---------------------------------------------------------------------- */
+static inline uint16_t quantize(uint16_t v, uint8_t gcli, uint8_t gtli, uint32_t qpih) {
+    if (gcli <= gtli)
+        return 0;
+    uint16_t sign = v & 0x8000u, m = v & 0x7FFFu, o;
+    if (gtli == 0)
+        return v; /* QuantStageProcess.c:20-22 skips bands with gtli 0 */
+    if (qpih == 0) {
+        o = (uint16_t)((m >> gtli) << gtli);
+    } else {
+        uint8_t sc = gcli - gtli + 1;
+        uint16_t d = (uint16_t)((((uint32_t)m << sc) - m + (1u << gcli)) >> (gcli + 1));
+        o = (uint16_t)(d << gtli);
+    }
+    return o ? (uint16_t)(o | sign) : 0;
+}
+
+/* Bits of a vertical-prediction code without its closing zero:
+ * vlc_encode_get_bits (PackPrecinct.h:84-104). */
+static inline uint32_t vlc_bits(int32_t x, int32_t r, int32_t t) {
+    int32_t mx = r - t;
+    if (mx < 0)
+        mx = 0;
+    if (x > mx)
+        return (uint32_t)(x + mx);
+    x *= 2;
+    return (uint32_t)(x < 0 ? -x - 1 : x);
+}
+
+/* The line above band line l of a precinct and its truncation: the previous
+ * line of the band, or the last line of the band in the precinct above
+ * (RateControl.c:603-613; Packing.c:265-276). */
+static const uint8_t* top_line(const ob_pi* pi, const enc_frame* f, const prec_state* ps, uint32_t c, uint32_t b, uint32_t l,
+                               uint8_t* gtli_top) {
+    const ob_band* bd = &pi->c[c].b[b];
+    const uint32_t gw = gcli_w(bd->w);
+    if (l > 0) {
+        *gtli_top = ps->gtli[c][b];
+        return f->gcli[c][b] + (size_t)(ps->row * bd->hl + l - 1) * gw;
+    }
+    *gtli_top = ps->top_gtli[c][b];
+    return f->gcli[c][b] + (size_t)((ps->row - 1) * bd->hl + pi->lines[0][c][b] - 1) * gw;
+}
+
+/* Vertical prediction of one band line: each group's code bits (without the
+ * closing zero) and, per significance group, whether the whole group is
+ * predicted exactly (Rm = 0) or below the truncation (Rm = 1):
+ * rate_control_calc_vpred_cost_nosigf_c / _sigf_c (RateControl.c:291-361). */
+static void vpred_line(const uint8_t* g, const uint8_t* top, uint32_t gw, uint8_t gtli, uint8_t gtli_top, uint32_t rm,
+                       uint8_t* bits, uint8_t* flag) {
+    const uint8_t gmax = gtli > gtli_top ? gtli : gtli_top;
+    for (uint32_t i = 0; i < gw; i++) {
+        int32_t mt = top[i] > gmax ? top[i] : gmax;
+        int32_t gc = g[i] > gtli ? g[i] : gtli;
+        bits[i] = (uint8_t)vlc_bits(gc - mt, mt, gtli);
+    }
+    for (uint32_t s = 0; s < (gw + 7) / 8; s++) {
+        uint8_t fl = 1;
+        for (uint32_t i = 8 * s; i < 8 * s + 8 && i < gw; i++) {
+            if (rm) {
+                if (g[i] > gtli)
+                    fl = 0;
+            } else {
+                int32_t mt = top[i] > gmax ? top[i] : gmax;
+                int32_t gc = g[i] > gtli ? g[i] : gtli;
+                if (mt != gc)
+                    fl = 0;
+            }
+        }
+        flag[s] = fl;
+    }
+}
+
+/* Signs a band line sends with full sign coding: its coefficients that stay
+ * non-zero after quantisation (rate_control_sign_coding_get_sum_nonzero_coeff_c,
+ * RateControl.c:363-455). */
+static uint32_t nonzero_coeffs(const ob_pi* pi, const enc_frame* f, const prec_state* ps, uint32_t c, uint32_t b, uint32_t l,
+                               uint8_t gtli) {
+    const ob_band* bd = &pi->c[c].b[b];
+    const uint32_t y = ps->row * bd->hl + l;
+    const uint8_t* g = f->gcli[c][b] + (size_t)y * gcli_w(bd->w);
+    const uint16_t* v = f->coeff[c][b] + (size_t)y * bd->w;
+    uint32_t n = 0;
+    for (uint32_t k = 0; k < bd->w; k++)
+        n += quantize(v[k], g[k / 4], gtli, pi->qpih) != 0;
+    return n;
+}
+
 /* Returns the bytes of all packets (without precinct/packet headers) for
- * (q, r) and records the per band / per packet choices in ps. */
-static uint32_t precinct_bytes(const ob_pi* pi, prec_state* ps, uint8_t q, uint8_t r) {
+ * (q, r) and records the per band / per packet choices in ps:
+ * precinct_get_budget_bytes (RateControl.c:704-781) over
+ * rate_control_calculate_band_best_method (RateControl.c:457-689).  vp: the
+ * vertical prediction mode in force (0 in the slice-wide searches), sg: the
+ * sign coding (fast in place of full there). */
+static uint32_t precinct_bytes(const ob_pi* pi, const enc_frame* f, prec_state* ps, uint8_t q, uint8_t r, uint32_t vp,
+                               uint32_t sg) {
+    uint8_t vbits[4096], vflag[512];
     for (uint32_t c = 0; c < pi->nc; c++)
         for (uint32_t b = 0; b < pi->c[c].nb; b++) {
             const ob_band* bd = &pi->c[c].b[b];
             uint8_t gtli = truncation(bd->gain, bd->prio, q, r);
             ps->gtli[c][b] = gtli;
+        }
+    for (uint32_t c = 0; c < pi->nc; c++)
+        for (uint32_t b = 0; b < pi->c[c].nb; b++) {
+            const ob_band* bd = &pi->c[c].b[b];
+            const uint8_t gtli = ps->gtli[c][b];
             uint32_t nl = pi->lines[ps->type][c][b];
             if (nl == 0)
                 continue;
@@ -939,13 +1060,15 @@ static uint32_t precinct_bytes(const ob_pi* pi, prec_state* ps, uint8_t q, uint8
             uint32_t nosig[2], best = 0;
             for (uint32_t l = 0; l < nl; l++) {
                 const line_lut* L = &ps->lut[c][b][l];
-                ps->bits_data[c][b][l] = L->data[gtli] * 4;
+                const uint32_t coded = gw - L->cnt_le[gtli]; /* groups above the truncation */
+                ps->bits_data[c][b][l] = (sg ? L->data[gtli] - coded : L->data[gtli]) * 4;
+                ps->bits_sign[c][b][l] = sg == 0 ? 0 : sg == 1 ? 4 * coded : nonzero_coeffs(pi, f, ps, c, b, l, gtli);
                 nosig[l] = L->data[gtli] + L->cnt_le[gtli];
                 ps->bits_gcli[c][b][l] = nosig[l];
                 ps->bits_sig[c][b][l] = 0;
                 best += nosig[l];
             }
-            ps->method_sig[c][b] = 0;
+            ps->method[c][b] = 0;
             if (pi->sig) {
                 uint32_t budget = sw * nl, withsig[2];
                 for (uint32_t l = 0; l < nl; l++) {
@@ -957,28 +1080,64 @@ static uint32_t precinct_bytes(const ob_pi* pi, prec_state* ps, uint8_t q, uint8
                     budget += withsig[l];
                 }
                 if (best > budget) {
-                    ps->method_sig[c][b] = 1;
+                    best = budget;
+                    ps->method[c][b] = 1;
                     for (uint32_t l = 0; l < nl; l++) {
                         ps->bits_sig[c][b][l] = sw;
                         ps->bits_gcli[c][b][l] = withsig[l];
                     }
                 }
             }
+            if (vp && ps->has_top) {
+                uint32_t vnosig[2], vsig[2], bud_nosig = 0, bud_sig = 0;
+                for (uint32_t l = 0; l < nl; l++) {
+                    uint8_t gtop;
+                    const uint8_t* top = top_line(pi, f, ps, c, b, l, &gtop);
+                    const uint8_t* g = f->gcli[c][b] + (size_t)(ps->row * bd->hl + l) * gw;
+                    vpred_line(g, top, gw, gtli, gtop, pi->vpred == 2, vbits, vflag);
+                    uint32_t sum = gw, red = 0;
+                    for (uint32_t i = 0; i < gw; i++)
+                        sum += vbits[i];
+                    for (uint32_t s = 0; s < sw; s++)
+                        if (vflag[s])
+                            for (uint32_t i = 8 * s; i < 8 * s + 8 && i < gw; i++)
+                                red += vbits[i] + 1u;
+                    vnosig[l] = sum;
+                    vsig[l] = sum - red;
+                    bud_nosig += sum;
+                    bud_sig += sum - red + sw;
+                }
+                if (best > bud_nosig) {
+                    best = bud_nosig;
+                    ps->method[c][b] = 2;
+                }
+                if (pi->sig && best > bud_sig) {
+                    best = bud_sig;
+                    ps->method[c][b] = 3;
+                }
+                if (ps->method[c][b] >= 2)
+                    for (uint32_t l = 0; l < nl; l++) {
+                        ps->bits_sig[c][b][l] = ps->method[c][b] == 3 ? sw : 0;
+                        ps->bits_gcli[c][b][l] = ps->method[c][b] == 3 ? vsig[l] : vnosig[l];
+                    }
+            }
         }
     uint32_t total = 0;
     for (uint32_t p = 0; p < pi->npk; p++) {
-        uint32_t d = 0, g = 0, s = 0;
+        uint32_t d = 0, g = 0, s = 0, sn = 0;
         for (uint32_t gi = pi->pk_bs[p]; gi < pi->pk_be[p]; gi++) {
             uint32_t c = pi->gc[gi], b = pi->gb[gi], l = pi->pk_line[p];
             if (l < pi->lines[ps->type][c][b]) {
                 d += ps->bits_data[c][b][l];
                 g += ps->bits_gcli[c][b][l];
                 s += ps->bits_sig[c][b][l];
+                sn += ps->bits_sign[c][b][l];
             }
         }
-        uint32_t db = bytes_of_bits(d), gb = bytes_of_bits(g), sb = bytes_of_bits(s);
+        uint32_t db = bytes_of_bits(d), gb = bytes_of_bits(g), sb = bytes_of_bits(s), nb = bytes_of_bits(sn);
         ps->pk_data[p] = db;
-        total += db;
+        ps->pk_sign[p] = nb;
+        total += db + nb;
         if (sb + gb > pi->raw_bytes[ps->type][p]) {
             ps->raw[p] = 1;
             ps->pk_sig[p] = 0;
@@ -1037,8 +1196,9 @@ int32_t jxso_bsearch_next(jxso_bsearch_t* b, int32_t result, uint32_t* next) {
 
 /* rate_control_precinct (RateControl.c:1259-1333) with the binary searches
  * of RateControl.c:889-1083.  On success fills q, r and the data bytes. */
-static int32_t rc_precinct(const ob_pi* pi, prec_state* ps, uint32_t budget, uint8_t* oq, uint8_t* or_, uint32_t* odata,
-                           uint32_t* ohdr) {
+static int32_t rc_precinct(const ob_pi* pi, const enc_frame* f, prec_state* ps, uint32_t budget, uint8_t* oq, uint8_t* or_,
+                           uint32_t* odata, uint32_t* ohdr) {
+    const uint32_t vp = pi->vpred, sg = pi->signs;
     if (budget > 1048575u)
         return E_ENCODE_FRAME;
     uint32_t hdr = bytes_of_bits(40 + pi->nb_exist * 2) + bytes_of_bits((pi->short_hdr ? 40u : 56u) * (uint32_t)pi->pk_exist[ps->type]);
@@ -1059,7 +1219,7 @@ static int32_t rc_precinct(const ob_pi* pi, prec_state* ps, uint32_t budget, uin
             next = 3;
             continue;
         }
-        next = precinct_bytes(pi, ps, (uint8_t)q, 0) > avail ? 1 : 2;
+        next = precinct_bytes(pi, f, ps, (uint8_t)q, 0, vp, sg) > avail ? 1 : 2;
     }
     if (res != 1)
         return E_ENCODE_FRAME;
@@ -1078,7 +1238,7 @@ static int32_t rc_precinct(const ob_pi* pi, prec_state* ps, uint32_t budget, uin
             next = 3;
             continue;
         }
-        uint32_t by = precinct_bytes(pi, ps, (uint8_t)q, (uint8_t)r);
+        uint32_t by = precinct_bytes(pi, f, ps, (uint8_t)q, (uint8_t)r, vp, sg);
         last_tested = (int32_t)r;
         last_bytes = by;
         next = by <= avail ? 1 : 2;
@@ -1086,37 +1246,35 @@ static int32_t rc_precinct(const ob_pi* pi, prec_state* ps, uint32_t budget, uin
     if (res != 1)
         return E_ENCODE_FRAME;
     if ((int32_t)r != last_tested)
-        last_bytes = precinct_bytes(pi, ps, (uint8_t)q, (uint8_t)r);
+        last_bytes = precinct_bytes(pi, f, ps, (uint8_t)q, (uint8_t)r, vp, sg);
     *oq = (uint8_t)q;
     *or_ = (uint8_t)r;
     *odata = last_bytes;
     return E_NONE;
 }
 
-/* ----------------------------------------------------------------------
- * Quantisation (Quant.c:9-45) applied while packing: the value packed for a
- * coefficient of a group with gcli > gtli.
- * ---------------------------------------------------------------------- */
-static inline uint16_t quantize(uint16_t v, uint8_t gcli, uint8_t gtli, uint32_t qpih) {
-    if (gcli <= gtli)
-        return 0;
-    uint16_t sign = v & 0x8000u, m = v & 0x7FFFu, o;
-    if (gtli == 0)
-        return v; /* QuantStageProcess.c:20-22 skips bands with gtli 0 */
-    if (qpih == 0) {
-        o = (uint16_t)((m >> gtli) << gtli);
-    } else {
-        uint8_t sc = gcli - gtli + 1;
-        uint16_t d = (uint16_t)((((uint32_t)m << sc) - m + (1u << gcli)) >> (gcli + 1));
-        o = (uint16_t)(d << gtli);
+/* Overwrites n bits at bit position pos (MSB first). */
+static void bw_set(bw_t* w, uint64_t pos, uint32_t v, uint32_t n) {
+    for (uint32_t i = 0; i < n; i++) {
+        const uint64_t p = pos + i;
+        const uint8_t m = (uint8_t)(0x80u >> (p & 7));
+        if ((v >> (n - 1 - i)) & 1)
+            w->m[p >> 3] |= m;
+        else
+            w->m[p >> 3] &= (uint8_t)~m;
     }
-    return o ? (uint16_t)(o | sign) : 0;
 }
 
-/* pack_precinct (PackPrecinct.c:235-572) with sign handling off. */
-static void pack_precinct(const ob_pi* pi, const enc_frame* f, prec_state* ps, uint8_t q, uint8_t r, uint32_t total_bytes,
-                          uint32_t pad_bytes, bw_t* w) {
+/* pack_precinct (PackPrecinct.c:235-572).  total_bytes: the precinct's bytes
+ * with headers; pad_bytes: zero bytes after it.  With fast sign coding the
+ * sign sub-packets take only the signs sent, each packet header gets their
+ * true length, and the bytes so saved either shorten the precinct (cut: Lprc
+ * is lowered) or pad it; they are returned (PackPrecinct.c:466-571). */
+static uint32_t pack_precinct(const ob_pi* pi, const enc_frame* f, prec_state* ps, uint8_t q, uint8_t r, uint32_t total_bytes,
+                              uint32_t pad_bytes, int cut, bw_t* w) {
+    uint8_t vbits[4096], vflag[512];
     uint32_t hlen = bytes_of_bits(40 + 2 * pi->nb_exist);
+    const uint64_t lprc_pos = w->bit;
     bw_put(w, total_bytes - hlen, 24);
     bw_put(w, q, 8);
     bw_put(w, r, 8);
@@ -1124,12 +1282,14 @@ static void pack_precinct(const ob_pi* pi, const enc_frame* f, prec_state* ps, u
         if (pi->gb[g] == NE)
             continue;
         uint32_t c = pi->gc[g], b = pi->gb[g];
+        static const uint8_t dpb[4] = {0, 2, 1, 3}; /* D[p,b]: bit 0 vertical prediction, bit 1 significance */
         uint32_t d = 0;
         if (pi->lines[ps->type][c][b] > 0)
-            d = ps->method_sig[c][b] ? 2 : 0;
+            d = dpb[ps->method[c][b]];
         bw_put(w, d, 2);
     }
     bw_align(w);
+    uint32_t retrieve = 0;
     for (uint32_t p = 0; p < pi->npk; p++) {
         uint32_t l = pi->pk_line[p];
         int any = 0;
@@ -1139,16 +1299,17 @@ static void pack_precinct(const ob_pi* pi, const enc_frame* f, prec_state* ps, u
         if (!any)
             continue;
         /* packet header: PackHeaders.c:98-127 */
+        const uint32_t sign_field = pi->short_hdr ? 11 : 15;
         bw_put(w, ps->raw[p], 1);
         if (pi->short_hdr) {
             bw_put(w, ps->pk_data[p], 15);
             bw_put(w, ps->pk_gcli[p], 13);
-            bw_put(w, 0, 11);
         } else {
             bw_put(w, ps->pk_data[p], 20);
             bw_put(w, ps->pk_gcli[p], 20);
-            bw_put(w, 0, 15);
         }
+        const uint64_t sign_pos = w->bit;
+        bw_put(w, ps->pk_sign[p], sign_field);
         if (ps->raw[p]) {
             for (uint32_t gi = pi->pk_bs[p]; gi < pi->pk_be[p]; gi++) {
                 uint32_t c = pi->gc[gi], b = pi->gb[gi];
@@ -1161,34 +1322,61 @@ static void pack_precinct(const ob_pi* pi, const enc_frame* f, prec_state* ps, u
             }
             bw_align(w);
         } else {
+            /* significance sub-packet */
             for (uint32_t gi = pi->pk_bs[p]; gi < pi->pk_be[p]; gi++) {
                 uint32_t c = pi->gc[gi], b = pi->gb[gi];
-                if (l >= pi->lines[ps->type][c][b] || !ps->method_sig[c][b])
+                const uint8_t m = ps->method[c][b];
+                if (l >= pi->lines[ps->type][c][b] || (m != 1 && m != 3))
                     continue;
                 const ob_band* bd = &pi->c[c].b[b];
-                const uint8_t* s = f->smax[c][b] + (size_t)(ps->row * bd->hl + l) * sig_w(bd->w);
-                for (uint32_t i = 0; i < sig_w(bd->w); i++)
-                    bw_put(w, s[i] <= ps->gtli[c][b], 1);
+                const uint32_t y = ps->row * bd->hl + l, gw = gcli_w(bd->w);
+                if (m == 1) {
+                    const uint8_t* s = f->smax[c][b] + (size_t)y * sig_w(bd->w);
+                    for (uint32_t i = 0; i < sig_w(bd->w); i++)
+                        bw_put(w, s[i] <= ps->gtli[c][b], 1);
+                } else {
+                    uint8_t gtop;
+                    const uint8_t* top = top_line(pi, f, ps, c, b, l, &gtop);
+                    vpred_line(f->gcli[c][b] + (size_t)y * gw, top, gw, ps->gtli[c][b], gtop, pi->vpred == 2, vbits,
+                               vflag);
+                    for (uint32_t i = 0; i < sig_w(bd->w); i++)
+                        bw_put(w, vflag[i], 1);
+                }
             }
             bw_align(w);
+            /* bit-plane count sub-packet */
             for (uint32_t gi = pi->pk_bs[p]; gi < pi->pk_be[p]; gi++) {
                 uint32_t c = pi->gc[gi], b = pi->gb[gi];
                 if (l >= pi->lines[ps->type][c][b])
                     continue;
                 const ob_band* bd = &pi->c[c].b[b];
                 uint32_t gw = gcli_w(bd->w);
-                const uint8_t* g = f->gcli[c][b] + (size_t)(ps->row * bd->hl + l) * gw;
-                const uint8_t* s = f->smax[c][b] + (size_t)(ps->row * bd->hl + l) * sig_w(bd->w);
-                uint8_t gt = ps->gtli[c][b];
+                const uint32_t y = ps->row * bd->hl + l;
+                const uint8_t* g = f->gcli[c][b] + (size_t)y * gw;
+                const uint8_t* s = f->smax[c][b] + (size_t)y * sig_w(bd->w);
+                const uint8_t gt = ps->gtli[c][b], m = ps->method[c][b];
+                if (m >= 2) {
+                    uint8_t gtop;
+                    const uint8_t* top = top_line(pi, f, ps, c, b, l, &gtop);
+                    vpred_line(g, top, gw, gt, gtop, pi->vpred == 2, vbits, vflag);
+                }
                 for (uint32_t i = 0; i < gw; i++) {
-                    if (ps->method_sig[c][b] && s[i / 8] <= gt)
+                    uint32_t n;
+                    if (m == 1 && s[i / 8] <= gt)
                         continue;
-                    uint32_t n = g[i] > gt ? g[i] - gt : 0;
+                    if (m == 3 && vflag[i / 8])
+                        continue;
+                    if (m >= 2)
+                        n = vbits[i];
+                    else
+                        n = g[i] > gt ? g[i] - gt : 0;
                     bw_put(w, ((1u << n) - 1) << 1, n + 1); /* n ones, then a zero */
                 }
             }
             bw_align(w);
         }
+        /* data sub-packet: per group the four signs (unless sent apart),
+         * then bit-planes gcli - 1 .. gtli (PackPrecinct.c:125-198) */
         for (uint32_t gi = pi->pk_bs[p]; gi < pi->pk_be[p]; gi++) {
             uint32_t c = pi->gc[gi], b = pi->gb[gi];
             if (l >= pi->lines[ps->type][c][b])
@@ -1204,16 +1392,49 @@ static void pack_precinct(const ob_pi* pi, const enc_frame* f, prec_state* ps, u
                 uint16_t qv[4];
                 for (int k = 0; k < 4; k++)
                     qv[k] = (4 * i + k < bd->w) ? quantize(v[4 * i + k], g[i], gt, pi->qpih) : 0;
-                for (int k = 0; k < 4; k++)
-                    bw_put(w, qv[k] >> 15, 1);
+                if (!pi->signs)
+                    for (int k = 0; k < 4; k++)
+                        bw_put(w, qv[k] >> 15, 1);
                 for (int bit = g[i] - 1; bit >= gt; bit--)
                     for (int k = 0; k < 4; k++)
                         bw_put(w, (qv[k] >> bit) & 1, 1);
             }
         }
         bw_align(w);
+        if (pi->signs) {
+            /* sign sub-packet: one bit per coefficient that stays non-zero
+             * (pack_sign_c, PackPrecinct.c:200-233) */
+            const uint64_t s0 = w->bit;
+            for (uint32_t gi = pi->pk_bs[p]; gi < pi->pk_be[p]; gi++) {
+                uint32_t c = pi->gc[gi], b = pi->gb[gi];
+                if (l >= pi->lines[ps->type][c][b])
+                    continue;
+                const ob_band* bd = &pi->c[c].b[b];
+                uint32_t y = ps->row * bd->hl + l;
+                const uint8_t* g = f->gcli[c][b] + (size_t)y * gcli_w(bd->w);
+                const uint16_t* v = f->coeff[c][b] + (size_t)y * bd->w;
+                for (uint32_t k = 0; k < bd->w; k++) {
+                    uint16_t qv = quantize(v[k], g[k / 4], ps->gtli[c][b], pi->qpih);
+                    if (qv)
+                        bw_put(w, qv >> 15, 1);
+                }
+            }
+            bw_align(w);
+            const uint32_t sent = (uint32_t)((w->bit - s0) >> 3);
+            if (pi->signs == 1) {
+                bw_set(w, sign_pos, sent, sign_field);
+                retrieve += ps->pk_sign[p] - sent;
+            }
+        }
+    }
+    if (pi->signs == 1) {
+        if (cut)
+            bw_set(w, lprc_pos, total_bytes - hlen - retrieve, 24);
+        else
+            w->bit += 8ull * retrieve;
     }
     w->bit += 8ull * pad_bytes; /* zero padding */
+    return pi->signs == 1 ? retrieve : 0;
 }
 
 /* Precinct budgets of one slice for rate control modes 2 and 3
@@ -1232,6 +1453,9 @@ static int32_t slice_budgets(const ob_pi* pi, const enc_frame* f, uint32_t first
         return E_ENCODE_FRAME;
     prec_state* st = (prec_state*)calloc(nprec, sizeof(prec_state));
     uint32_t* pad = (uint32_t*)calloc(nprec, sizeof(uint32_t));
+    /* the slice-wide searches run without vertical prediction and with fast
+     * sign coding in place of full (RateControl.c:1103-1106, 1120-1121) */
+    const uint32_t sg = pi->signs ? 1u : 0u;
     int32_t err = E_NONE;
     uint32_t hsum = 0;
     for (uint32_t i = 0; i < nprec; i++) {
@@ -1260,7 +1484,7 @@ static int32_t slice_budgets(const ob_pi* pi, const enc_frame* f, uint32_t first
             }
             uint64_t sum = 0;
             for (uint32_t i = 0; i < nprec; i++)
-                sum += precinct_bytes(pi, &st[i], (uint8_t)q, 0);
+                sum += precinct_bytes(pi, f, &st[i], (uint8_t)q, 0, 0, sg);
             next = sum > avail ? 1 : 2;
         }
         if (res != 1)
@@ -1278,7 +1502,7 @@ static int32_t slice_budgets(const ob_pi* pi, const enc_frame* f, uint32_t first
                 }
                 uint64_t sum = 0;
                 for (uint32_t i = 0; i < nprec; i++)
-                    sum += precinct_bytes(pi, &st[i], (uint8_t)q, (uint8_t)r);
+                    sum += precinct_bytes(pi, f, &st[i], (uint8_t)q, (uint8_t)r, 0, sg);
                 next = sum <= avail ? 1 : 2;
             }
             if (res != 1)
@@ -1287,7 +1511,7 @@ static int32_t slice_budgets(const ob_pi* pi, const enc_frame* f, uint32_t first
         if (!err) {
             uint32_t left = (uint32_t)avail;
             for (uint32_t i = 0; i < nprec; i++) {
-                const uint32_t d = precinct_bytes(pi, &st[i], (uint8_t)q, (uint8_t)r);
+                const uint32_t d = precinct_bytes(pi, f, &st[i], (uint8_t)q, (uint8_t)r, 0, sg);
                 left -= d;
                 bud[i] += d;
             }
@@ -1324,7 +1548,8 @@ static int32_t slice_budgets(const ob_pi* pi, const enc_frame* f, uint32_t first
                 }
             }
             if (pad[last] > 0) {
-                uint32_t add_first = bud[0] * 10 / 100;
+                /* 20 % with fast sign coding (PackStageProcess.c:440-450) */
+                uint32_t add_first = bud[0] * (pi->signs == 1 ? 20u : 10u) / 100;
                 if (add_first > pad[last])
                     add_first = pad[last];
                 pad[last] -= add_first;
@@ -1375,7 +1600,9 @@ static int32_t enc_code(const ob_pi* pi, const enc_frame* f, uint8_t* out, uint8
         uint32_t minb = sbudget / nprec;
         uint32_t first = minb, left = sbudget - minb * nprec;
         if (pi->rc_mode == 1) {
-            first = minb * 110 / 100;
+            /* 10 % more for the first precinct, 20 % with fast sign coding
+             * (PackStageProcess.c:621-640) */
+            first = minb * (pi->signs == 1 ? 120u : 110u) / 100;
             if (first > sbudget)
                 first = sbudget;
             if (nprec > 1) {
@@ -1397,12 +1624,17 @@ static int32_t enc_code(const ob_pi* pi, const enc_frame* f, uint8_t* out, uint8
         uint32_t carry = 0;
         for (uint32_t i = 0; i < nprec; i++) {
             uint32_t budget = (sbud ? sbud[i] : (i == 0 ? first : minb) + (i < left ? 1 : 0)) + carry;
+            /* the precinct above in the slice, for vertical prediction
+             * (PackStageProcess.c:655-660, 355-358) */
+            if (i > 0)
+                memcpy(ps->top_gtli, ps->gtli, sizeof(ps->gtli));
+            ps->has_top = pi->vpred && i > 0;
             ps->row = first_row + i;
             ps->type = (ps->row + 1 >= pi->rows) ? 1 : 0;
             build_luts(pi, f, ps);
             uint8_t q = 0, r = 0;
             uint32_t data = 0, hdr = 0;
-            err = rc_precinct(pi, ps, budget, &q, &r, &data, &hdr);
+            err = rc_precinct(pi, f, ps, budget, &q, &r, &data, &hdr);
             if (err) {
                 /* any slice error surfaces as a frame error (FinalStageProcess.c:120,
                  * EncHandle.c:1043-1046) */
@@ -1411,16 +1643,20 @@ static int32_t enc_code(const ob_pi* pi, const enc_frame* f, uint8_t* out, uint8
             }
             uint32_t pad = budget - hdr - data;
             uint32_t total = budget;
+            int cut = 0;
             if (pi->rc_mode != 0 && i + 1 < nprec) {
                 total -= pad;
                 carry = pad;
                 pad = 0;
+                cut = pi->signs == 1; /* PackStageProcess.c:305-307, 533-535 */
             }
             if (qr_dump) {
                 qr_dump[2 * ps->row] = q;
                 qr_dump[2 * ps->row + 1] = r;
             }
-            pack_precinct(pi, f, ps, q, r, total, pad, &w);
+            const uint32_t retrieved = pack_precinct(pi, f, ps, q, r, total, pad, cut, &w);
+            if (cut)
+                carry += retrieved; /* PackStageProcess.c:326-333, 556-560 */
         }
         free(sbud);
         pos = w.bit >> 3;
@@ -1491,8 +1727,8 @@ int32_t jxso_encode_dump(const jxso_params_t* p, const void* const planes[4], co
 
 /* ----------------------------------------------------------------------
  * Decoder: header parsing (ParseHeader.c:20-618, the markers this encoder
- * writes plus COM skipping), precinct unpacking (Packing.c:513-868, no
- * vertical prediction, Fs = 0), dequantisation and sign conversion
+ * writes plus COM skipping), precinct unpacking (Packing.c:513-868, with
+ * vertical prediction and sign sub-packets), dequantisation and sign conversion
  * (Dequant.c:9-57, Precinct.c:9-29), inverse transform, linear output
  * scaling (NltDec.c:242-265).
  * ---------------------------------------------------------------------- */
@@ -1500,6 +1736,7 @@ typedef struct {
     ob_pi pi;
     uint32_t hdr_end;
     uint32_t depth, bw, fq, qpih, lcod;
+    uint32_t fs, rm; /* separate sign sub-packets; run mode of vertically predicted significance groups */
 } dec_hdr;
 
 static int32_t parse_header(const uint8_t* cs, uint32_t len, dec_hdr* d) {
@@ -1547,9 +1784,11 @@ static int32_t parse_header(const uint8_t* cs, uint32_t len, dec_hdr* d) {
             lh = br_get(&r, 1);
             br_get(&r, 1); /* Rl */
             d->qpih = br_get(&r, 2);
-            if (br_get(&r, 2) != 0) /* Fs: separate sign coding is not decoded here */
+            d->fs = br_get(&r, 2);
+            d->rm = br_get(&r, 2);
+            /* ParseHeader.c:238-251 */
+            if (d->qpih > 1 || d->fs > 1 || d->rm > 1)
                 return E_INVALID_BS;
-            br_get(&r, 2); /* Rm: only matters for vertically predicted bands, rejected per precinct */
             have_pih = 1;
             break;
         case 0xFF13: /* CDT */
@@ -1699,6 +1938,7 @@ int32_t jxso_decode_proxy(const uint8_t* cs, uint32_t len, uint32_t proxy, void*
         }
         uint32_t first_row = s * pi->pps;
         uint32_t nprec = s == pi->slices - 1 ? pi->rows - first_row : pi->pps;
+        uint8_t prev_gtli[4][NBC] = {{0}};
         for (uint32_t i = 0; i < nprec && !e; i++) {
             uint32_t row = first_row + i, type = row + 1 >= pi->rows ? 1 : 0;
             uint64_t pstart = r.bit;
@@ -1720,8 +1960,10 @@ int32_t jxso_decode_proxy(const uint8_t* cs, uint32_t len, uint32_t proxy, void*
             for (uint32_t c = 0; c < pi->nc; c++)
                 for (uint32_t b = 0; b < pi->c[c].nb; b++) {
                     gtli[c][b] = truncation(pi->c[c].b[b].gain, pi->c[c].b[b].prio, q, rr);
-                    if (dmode[c][b] & 1)
-                        e = E_INVALID_BS; /* vertical prediction not produced by this encoder */
+                    /* the first precinct of a slice has nothing above it
+                     * (Packing.c:536-542) */
+                    if ((dmode[c][b] & 1) && i == 0)
+                        e = E_INVALID_BS;
                 }
             for (uint32_t p = 0; p < npk_dec && !e; p++) {
                 uint32_t l = pi->pk_line[p];
@@ -1731,15 +1973,15 @@ int32_t jxso_decode_proxy(const uint8_t* cs, uint32_t len, uint32_t proxy, void*
                         any = 1;
                 if (!any)
                     continue;
-                uint32_t raw = br_get(&r, 1), ld, lg;
+                uint32_t raw = br_get(&r, 1), ld, lg, ls;
                 if (pi->short_hdr) {
                     ld = br_get(&r, 15);
                     lg = br_get(&r, 13);
-                    br_get(&r, 11);
+                    ls = br_get(&r, 11);
                 } else {
                     ld = br_get(&r, 20);
                     lg = br_get(&r, 20);
-                    br_get(&r, 15);
+                    ls = br_get(&r, 15);
                 }
                 if (raw) {
                     uint64_t gend = r.bit + 8ull * lg;
@@ -1776,15 +2018,38 @@ int32_t jxso_decode_proxy(const uint8_t* cs, uint32_t len, uint32_t proxy, void*
                         const ob_band* bd = &pi->c[c].b[b];
                         uint32_t gw = gcli_w(bd->w);
                         uint8_t* g = gcl[c][b] + (size_t)(row * bd->hl + l) * gw;
+                        const uint8_t gt = gtli[c][b];
+                        /* vertical prediction (Packing.c:249-385): from the
+                         * line above, the band's previous line or the last
+                         * line of the precinct above */
+                        const int vp = dmode[c][b] & 1;
+                        const uint8_t* top = NULL;
+                        uint8_t t = gt;
+                        if (vp) {
+                            const uint8_t gtop = l > 0 ? gt : prev_gtli[c][b];
+                            t = gtop > gt ? gtop : gt;
+                            top = l > 0 ? g - gw : gcl[c][b] + (size_t)((row - 1) * bd->hl + pi->lines[0][c][b] - 1) * gw;
+                        }
                         for (uint32_t k = 0; k < gw; k++) {
+                            const int32_t mt = vp ? (top[k] > t ? top[k] : t) : 0;
                             if ((dmode[c][b] & 2) && sflag[soff[c][b] + k / 8]) {
-                                g[k] = gtli[c][b];
+                                g[k] = (uint8_t)(vp && !d.rm ? mt : gt);
                                 continue;
                             }
                             uint32_t n = 0;
-                            while (br_get(&r, 1) == 1 && !r.err)
+                            while (br_get(&r, 1) == 1 && !r.err && n < 64)
                                 n++;
-                            g[k] = (uint8_t)(gtli[c][b] + n);
+                            if (!vp) {
+                                g[k] = (uint8_t)(gt + n);
+                                continue;
+                            }
+                            const int32_t th = mt - gt > 0 ? mt - gt : 0;
+                            int32_t dm = 0;
+                            if ((int32_t)n > 2 * th)
+                                dm = (int32_t)n - th;
+                            else if (n > 0)
+                                dm = (n & 1) ? -(int32_t)((n + 1) / 2) : (int32_t)(n / 2);
+                            g[k] = (uint8_t)(mt + dm);
                         }
                     }
                     if (r.bit > gend)
@@ -1793,6 +2058,7 @@ int32_t jxso_decode_proxy(const uint8_t* cs, uint32_t len, uint32_t proxy, void*
                     (void)sbytes;
                 }
                 uint64_t dend = r.bit + 8ull * ld;
+                uint32_t pad_signs[4][NBC] = {{0}};
                 for (uint32_t gi = pi->pk_bs[p]; gi < pi->pk_be[p] && !e; gi++) {
                     uint32_t c = pi->gc[gi], b = pi->gb[gi];
                     if (l >= pi->lines[type][c][b])
@@ -1806,15 +2072,19 @@ int32_t jxso_decode_proxy(const uint8_t* cs, uint32_t len, uint32_t proxy, void*
                         if (g[k] <= gt || g[k] > 15)
                             continue;
                         uint16_t val[4] = {0};
-                        uint32_t sg = br_get(&r, 4);
+                        uint32_t sg = d.fs ? 0 : br_get(&r, 4);
                         for (int bit = g[k] - 1; bit >= gt; bit--) {
                             uint32_t nib = br_get(&r, 4);
                             for (int t = 0; t < 4; t++)
                                 val[t] |= (uint16_t)(((nib >> (3 - t)) & 1) << bit);
                         }
                         for (int t = 0; t < 4; t++) {
-                            if (4 * k + t >= bd->w)
+                            if (4 * k + t >= bd->w) {
+                                /* sign bits of non-zero padding coefficients are
+                                 * skipped (Packing.c:205-209, 240-246) */
+                                pad_signs[c][b] += val[t] != 0;
                                 continue;
+                            }
                             uint16_t m = val[t];
                             /* deadzone / uniform reconstruction (Dequant.c) */
                             if (gt > 0 && m) {
@@ -1835,7 +2105,28 @@ int32_t jxso_decode_proxy(const uint8_t* cs, uint32_t len, uint32_t proxy, void*
                 if (r.bit > dend || r.err)
                     e = E_INVALID_BS;
                 r.bit = dend;
+                if (d.fs && !e) {
+                    /* sign sub-packet: one bit per non-zero coefficient
+                     * (unpack_sign, Packing.c:221-247) */
+                    uint64_t send = r.bit + 8ull * ls;
+                    for (uint32_t gi = pi->pk_bs[p]; gi < pi->pk_be[p]; gi++) {
+                        uint32_t c = pi->gc[gi], b = pi->gb[gi];
+                        if (l >= pi->lines[type][c][b])
+                            continue;
+                        const ob_band* bd = &pi->c[c].b[b];
+                        uint16_t* v = coef[c][b] + (size_t)(row * bd->hl + l) * bd->w;
+                        for (uint32_t k = 0; k < bd->w; k++)
+                            if (v[k])
+                                v[k] |= (uint16_t)(br_get(&r, 1) << 15);
+                        r.bit += pad_signs[c][b];
+                    }
+                    br_align(&r);
+                    if (r.bit > send || r.err)
+                        e = E_INVALID_BS;
+                    r.bit = send;
+                }
             }
+            memcpy(prev_gtli, gtli, sizeof(gtli));
             r.bit = pend;
         }
         pos = r.bit >> 3;

@@ -5,11 +5,13 @@
  * smoke() entry point and bench.py's cpu_baseline leg may load it.  The
  * product (svt-jpeg-xs_amd) never links, loads or calls it.
  *
- * Scope: the reference's default coding tools -- rate-control mode 1 (CBR per
- * precinct, move padding), significance coding on/off, no vertical
- * prediction, no separate sign coding, deadzone or uniform quantisation,
- * Bw = 20, Fq = 8, one precinct column (Cw = 0).  Each function in the .c file
- * cites the reference file:line it restates.
+ * Scope: the reference encoder's coding tools -- rate-control modes 0-3,
+ * significance coding on/off, vertical prediction off / zero residuals / zero
+ * coefficients (Rm), sign coding off / fast / full (Fs; full not with the
+ * per-slice modes 2 and 3), deadzone or uniform quantisation, Bw = 20,
+ * Fq = 8, one precinct column (Cw = 0) -- and the decoder for what it
+ * writes.  Each function in the .c file cites the reference file:line it
+ * restates.
  */
 #ifndef JXS_ORACLE_H
 #define JXS_ORACLE_H
@@ -28,6 +30,8 @@ typedef struct jxso_params {
     uint32_t slice_height;
     uint32_t significance; /* coding_significance flag */
     uint32_t rc_mode;      /* rate_control_mode: 0 budget per precinct, 1 the same moving padding on (default) */
+    uint32_t vpred;        /* coding_vertical_prediction_mode: 0 off, 1 zero residuals, 2 zero coefficients */
+    uint32_t signs;        /* coding_signs_handling: 0 off, 1 fast, 2 full */
 } jxso_params_t;
 
 /* Fills defaults exactly as svt_jpeg_xs_encoder_load_default_parameters

@@ -17,7 +17,7 @@ REF_DEC = os.path.join(HERE, "_ref", "ref_dec")
 class Params(ctypes.Structure):
     _fields_ = [(n, ctypes.c_uint32) for n in (
         "width", "height", "depth", "format", "bpp_num", "bpp_den", "decomp_v", "decomp_h",
-        "quantization", "slice_height", "significance", "rc_mode")]
+        "quantization", "slice_height", "significance", "rc_mode", "vpred", "signs")]
 
 
 class ImageInfo(ctypes.Structure):
@@ -120,13 +120,14 @@ def lib():
 
 
 def make_params(width, height, depth, fmt, bpp_num, bpp_den=1, decomp_v=2, decomp_h=5, quantization=0,
-                slice_height=16, significance=1, rc_mode=1):
+                slice_height=16, significance=1, rc_mode=1, vpred=0, signs=0):
     p = Params()
     lib().jxso_default_params(ctypes.byref(p))
     p.width, p.height, p.depth, p.format = width, height, depth, fmt
     p.bpp_num, p.bpp_den, p.decomp_v, p.decomp_h = bpp_num, bpp_den, decomp_v, decomp_h
     p.quantization, p.slice_height, p.significance = quantization, slice_height, significance
     p.rc_mode = rc_mode
+    p.vpred, p.signs = vpred, signs
     return p
 
 

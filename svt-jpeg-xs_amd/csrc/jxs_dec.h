@@ -23,15 +23,21 @@ struct JxsDecFrame {
     uint32_t prec_in_stride;
     uint32_t pkt_stride; /* u32 words per precinct in pkt */
     uint32_t npk_dec;    /* packets of each precinct decoded (fewer for a proxy decode) */
-    uint32_t pad_;
+    uint32_t slice_base; /* first slice of this frame in the launch */
     uint32_t* pkt;       /* per precinct: JXS_DPKT_HDR words of band modes and flags, then 5 words per packet */
+    uint8_t* gcli;       /* decoded bit-plane counts (JxsBand::gcli_off layout), for slices decoded in order */
 };
 
 /* Packet table of one precinct (written by k_dec_packets, read by
  * k_dec_unpack): words 0..19 hold one u16 per (component, band):
- * D-mode | GTLI << 8; word 20 is non-zero when the precinct is invalid; then
- * per packet {significance bit, GCLI bit, data bit, data end bit, flags}
- * with flags bit 0 = the packet exists in this precinct, bit 1 = raw mode. */
+ * D-mode | GTLI << 8; word 20 is non-zero when the precinct is invalid;
+ * word 21 is non-zero when the precinct uses vertical prediction or the
+ * codestream sends signs apart (its slice is then decoded in order by
+ * k_dec_slice); then per packet {significance bit, GCLI bit, data bit, data
+ * end bit, flags} with flags bit 0 = the packet exists in this precinct,
+ * bit 1 = raw mode. */
+#define JXS_DPKT_BAD 20
+#define JXS_DPKT_ORDERED 21
 #define JXS_DPKT_HDR 24
 #define JXS_DPKT_WORDS 5
 

@@ -244,6 +244,7 @@ __global__ void __launch_bounds__(64) k_dec_packets(const JxsDecFrame* __restric
         mode[k] = 0;
     uint64_t bit = off * 8 + 40;
     bool bad = false;
+    bool ordered = g.fs != 0; /* separate sign sub-packets */
     for (uint32_t gi = 0; gi < g.nb_all; gi++) {
         if (g.gb[gi] == 0xFF)
             continue;
@@ -252,7 +253,7 @@ __global__ void __launch_bounds__(64) k_dec_packets(const JxsDecFrame* __restric
         bit += 2;
         mode[c * JXS_NBC + b] = (uint16_t)(d | dtrunc(g.c[c].b[b].gain, g.c[c].b[b].prio, q, r) << 8);
         if (d & 1)
-            bad = true; /* vertical prediction: not produced by this library's encoder */
+            ordered = true; /* vertical prediction: needs the line above */
     }
     for (uint32_t k = 0; k < JXS_MAXC * JXS_NBC / 2; k++)
         T[k] = mode[2 * k] | (uint32_t)mode[2 * k + 1] << 16;
@@ -274,18 +275,22 @@ __global__ void __launch_bounds__(64) k_dec_packets(const JxsDecFrame* __restric
             E[4] = 0;
             continue;
         }
-        uint32_t raw, ld, lg;
+        uint32_t raw, ld, lg, ls;
         if (g.short_hdr) {
             raw = cs_bits(F, pos * 8, 1);
             ld = cs_bits(F, pos * 8 + 1, 15);
             lg = cs_bits(F, pos * 8 + 16, 13);
+            ls = cs_bits(F, pos * 8 + 29, 11);
         } else {
             raw = cs_bits(F, pos * 8, 1);
             ld = cs_bits(F, pos * 8 + 1, 20);
             lg = cs_bits(F, pos * 8 + 21, 20);
+            ls = cs_bits(F, pos * 8 + 41, 15);
         }
+        if (!g.fs)
+            ls = 0; /* the field is only a sub-packet length with Fs = 1 */
         const uint32_t sb = raw ? 0u : (sig_bits + 7) / 8;
-        const uint64_t s_off = pos + g.pkt_hdr_bytes, g_off = s_off + sb, d_off = g_off + lg, nxt = d_off + ld;
+        const uint64_t s_off = pos + g.pkt_hdr_bytes, g_off = s_off + sb, d_off = g_off + lg, nxt = d_off + ld + ls;
         E[0] = (uint32_t)(s_off * 8);
         E[1] = (uint32_t)(g_off * 8);
         E[2] = (uint32_t)(d_off * 8);
@@ -299,7 +304,8 @@ __global__ void __launch_bounds__(64) k_dec_packets(const JxsDecFrame* __restric
         bad = true;
     if (bad)
         atomicOr(F.status, 1u);
-    T[JXS_MAXC * JXS_NBC / 2] = bad ? 1u : 0u;
+    T[JXS_DPKT_BAD] = bad ? 1u : 0u;
+    T[JXS_DPKT_ORDERED] = ordered ? 1u : 0u;
 }
 
 /* One block per (packet, precinct).  Inside a packet the significance,
@@ -330,7 +336,12 @@ __global__ void __launch_bounds__(PK_THREADS) __attribute__((amdgpu_waves_per_eu
     if (tid == 64) {
         const uint32_t* E = T + JXS_DPKT_HDR + JXS_DPKT_WORDS * p;
         const uint32_t fl = E[4];
-        C.ok = (fl & 1u) && T[JXS_MAXC * JXS_NBC / 2] == 0 ? 1u : 0u;
+        /* a slice with an ordered precinct is decoded by k_dec_slice */
+        uint32_t ordered = 0;
+        const uint32_t r0 = row - row % g.pps, r1 = min(r0 + g.pps, g.rows);
+        for (uint32_t rr = r0; rr < r1; rr++)
+            ordered |= F.pkt[(size_t)rr * F.pkt_stride + JXS_DPKT_ORDERED];
+        C.ok = (fl & 1u) && T[JXS_DPKT_BAD] == 0 && !ordered ? 1u : 0u;
         C.raw = (fl >> 1) & 1u;
         C.s_bit = E[0];
         C.g_bit = E[1];
@@ -571,6 +582,308 @@ __global__ void __launch_bounds__(PK_THREADS) __attribute__((amdgpu_waves_per_eu
 }
 
 /* ------------------------------------------------------------------ */
+/* in-order slice decoding: vertical prediction, separate sign sub-packets */
+/* ------------------------------------------------------------------ */
+#define SL_MAXG 8192 /* groups of one packet's band lines */
+
+__device__ __forceinline__ uint32_t dequant1(uint32_t m, uint32_t gc, uint32_t gt, uint32_t qpih) {
+    if (gt == 0 || m == 0)
+        return m;
+    if (qpih == 0)
+        return m | 1u << (gt - 1);
+    const uint32_t sc = gc - gt + 1;
+    uint32_t acc = 0;
+    for (uint32_t x = m; x > 0; x >>= sc)
+        acc += x;
+    return acc & 0xFFFFu;
+}
+
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x) {
+    const uint32_t lane = threadIdx.x & (WAVE - 1);
+#pragma unroll
+    for (uint32_t o = 1; o < WAVE; o <<= 1) {
+        const uint32_t y = __shfl_up(x, o, WAVE);
+        if (lane >= o)
+            x += y;
+    }
+    return x;
+}
+
+/* One wave per slice whose precincts use vertical prediction or whose
+ * codestream sends signs apart (Packing.c:513-868): precincts and packets in
+ * order, because a vertically predicted bit-plane count is coded against the
+ * line above (the band's previous line, or its last line in the precinct
+ * above: Packing.c:249-385) and the sign sub-packet follows the data of its
+ * packet (Packing.c:221-247, 829-866).  Lane 0 decodes the significance flags
+ * and bit-plane counts of a packet in order; the wave then decodes the data of
+ * each band line 64 groups at a time at prefix-summed bit offsets, and with
+ * Fs = 1 applies the signs, one bit per non-zero coefficient. */
+__global__ void __launch_bounds__(WAVE) k_dec_slice(const JxsDecFrame* __restrict__ frames, int nfr) {
+    __shared__ uint8_t gsh[SL_MAXG];
+    __shared__ uint32_t sh_err;
+    int lo = 0, hi = nfr - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (frames[mid].slice_base <= blockIdx.x)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    const JxsDecFrame& F = frames[lo];
+    const JxsGeom& g = *F.g;
+    const uint32_t s = blockIdx.x - F.slice_base;
+    if (s >= g.slices)
+        return;
+    const uint32_t r0 = s * g.pps, r1 = min(r0 + g.pps, g.rows);
+    uint32_t ordered = 0;
+    for (uint32_t rr = r0; rr < r1; rr++)
+        ordered |= F.pkt[(size_t)rr * F.pkt_stride + JXS_DPKT_ORDERED];
+    if (!ordered)
+        return;
+    const uint32_t lane = threadIdx.x;
+    const CsView V = cs_view(F);
+    if (lane == 0)
+        sh_err = 0;
+    __syncthreads();
+    for (uint32_t row = r0; row < r1; row++) {
+        const uint32_t* T = F.pkt + (size_t)row * F.pkt_stride;
+        if (T[JXS_DPKT_BAD])
+            break; /* already reported */
+        const uint32_t type = row + 1 >= g.rows ? 1 : 0;
+        const uint64_t off = F.prec_off[row];
+        const uint64_t pend = off + g.prec_hdr_bytes + (uint64_t)cs_bits(F, off * 8, 24);
+        uint64_t pos = off + g.prec_hdr_bytes;
+        bool bad = false;
+        if (row == r0) /* the first precinct of a slice has nothing above it (Packing.c:536-542) */
+            for (uint32_t k = 0; k < JXS_MAXC * JXS_NBC; k++)
+                bad |= ((T[k >> 1] >> (16 * (k & 1))) & 1u) != 0;
+        const uint32_t npk = min(g.npk, F.npk_dec);
+        for (uint32_t p = 0; p < npk && !bad; p++) {
+            const uint32_t l = g.pk_line[p];
+            uint32_t sig_bits = 0, any = 0;
+            for (uint32_t gi = g.pk_bs[p]; gi < g.pk_be[p]; gi++) {
+                const uint32_t c = g.gc[gi], b = g.gb[gi];
+                const JxsBand& bd = g.c[c].b[b];
+                if (l < bd.lines[type]) {
+                    any = 1;
+                    const uint32_t k = c * JXS_NBC + b;
+                    if ((T[k >> 1] >> (16 * (k & 1))) & 2u)
+                        sig_bits += bd.sw;
+                }
+            }
+            if (!any)
+                continue;
+            uint32_t raw, ld, lg, ls;
+            if (g.short_hdr) {
+                raw = cs_bits(F, pos * 8, 1);
+                ld = cs_bits(F, pos * 8 + 1, 15);
+                lg = cs_bits(F, pos * 8 + 16, 13);
+                ls = cs_bits(F, pos * 8 + 29, 11);
+            } else {
+                raw = cs_bits(F, pos * 8, 1);
+                ld = cs_bits(F, pos * 8 + 1, 20);
+                lg = cs_bits(F, pos * 8 + 21, 20);
+                ls = cs_bits(F, pos * 8 + 41, 15);
+            }
+            if (!g.fs)
+                ls = 0;
+            const uint64_t s_off = pos + g.pkt_hdr_bytes, g_off = s_off + (raw ? 0u : (sig_bits + 7) / 8);
+            const uint64_t d_off = g_off + lg, n_off = d_off + ld, nxt = n_off + ls;
+            if (nxt > pend || nxt > F.cs_len) {
+                bad = true;
+                break;
+            }
+            /* bit-plane counts of the packet's band lines, into LDS (and the
+             * frame's GCLI planes, for the lines below) */
+            if (raw) {
+                uint32_t base = 0;
+                for (uint32_t gi = g.pk_bs[p]; gi < g.pk_be[p]; gi++) {
+                    const uint32_t c = g.gc[gi], b = g.gb[gi];
+                    const JxsBand& bd = g.c[c].b[b];
+                    if (l >= bd.lines[type])
+                        continue;
+                    uint8_t* gl = F.gcli + bd.gcli_off + (size_t)(row * bd.hl + l) * bd.gw;
+                    for (uint32_t k = lane; k < bd.gw; k += WAVE) {
+                        const uint32_t v = (uint32_t)(cs_bits64(V, g_off * 8 + 4ull * (base + k) + V.sh) >> 60);
+                        gl[k] = (uint8_t)v;
+                        if (base + k < SL_MAXG)
+                            gsh[base + k] = (uint8_t)v;
+                    }
+                    base += bd.gw;
+                }
+                if (base > SL_MAXG)
+                    bad = true;
+            } else if (lane == 0) {
+                uint64_t cur = g_off * 8, sbit = s_off * 8;
+                const uint64_t gend = d_off * 8;
+                uint32_t base = 0;
+                for (uint32_t gi = g.pk_bs[p]; gi < g.pk_be[p] && !bad; gi++) {
+                    const uint32_t c = g.gc[gi], b = g.gb[gi];
+                    const JxsBand& bd = g.c[c].b[b];
+                    if (l >= bd.lines[type])
+                        continue;
+                    const uint32_t k = c * JXS_NBC + b;
+                    const uint32_t md = (T[k >> 1] >> (16 * (k & 1))) & 0xFFFFu, gt = md >> 8;
+                    const bool sg = md & 2u, vp = md & 1u;
+                    uint8_t* gl = F.gcli + bd.gcli_off + (size_t)(row * bd.hl + l) * bd.gw;
+                    const uint8_t* top = nullptr;
+                    uint32_t t = gt;
+                    if (vp) {
+                        uint32_t gtop = gt;
+                        if (l > 0) {
+                            top = gl - bd.gw;
+                        } else {
+                            top = gl - (size_t)(bd.hl - bd.lines[0] + 1) * bd.gw; /* last line of the precinct above */
+                            const uint32_t* Tt = T - F.pkt_stride;
+                            gtop = (Tt[k >> 1] >> (16 * (k & 1) + 8)) & 0xFFu;
+                        }
+                        t = gtop > gt ? gtop : gt;
+                    }
+                    if (base + bd.gw > SL_MAXG) {
+                        bad = true;
+                        break;
+                    }
+                    uint32_t flag = 0;
+                    for (uint32_t i = 0; i < bd.gw; i++) {
+                        if (sg && (i & 7) == 0)
+                            flag = cs_bits(F, sbit++, 1); /* 1: the group of 8 is insignificant */
+                        const uint32_t mt = vp ? max((uint32_t)top[i], t) : 0u;
+                        uint32_t v;
+                        if (sg && flag) {
+                            v = vp && !g.rm ? mt : gt;
+                        } else {
+                            uint32_t n = 0;
+                            for (;;) {
+                                const uint32_t x = cs_bits(F, cur, 32);
+                                const uint32_t lead = __clz(~x);
+                                if (lead < 32) {
+                                    n += lead;
+                                    cur += lead + 1;
+                                    break;
+                                }
+                                n += 32;
+                                cur += 32;
+                                if (n > 64)
+                                    break;
+                            }
+                            if (cur > gend || n > 64) {
+                                bad = true;
+                                break;
+                            }
+                            if (!vp) {
+                                v = gt + n;
+                            } else {
+                                const int32_t th = (int32_t)mt - (int32_t)gt > 0 ? (int32_t)mt - (int32_t)gt : 0;
+                                int32_t dm = 0;
+                                if ((int32_t)n > 2 * th)
+                                    dm = (int32_t)n - th;
+                                else if (n > 0)
+                                    dm = (n & 1) ? -(int32_t)((n + 1) / 2) : (int32_t)(n / 2);
+                                const int32_t vv = (int32_t)mt + dm;
+                                v = vv < 0 ? 0u : (uint32_t)vv;
+                            }
+                        }
+                        if (v > 15) {
+                            v = 15;
+                            bad = true;
+                        }
+                        gl[i] = (uint8_t)v;
+                        gsh[base + i] = (uint8_t)v;
+                    }
+                    base += bd.gw;
+                }
+            }
+            if (lane == 0 && bad)
+                sh_err = 1;
+            __syncthreads();
+            bad = sh_err != 0;
+            if (bad)
+                break;
+            /* data sub-packet, band line by band line */
+            uint64_t dbit = d_off * 8, nbit = n_off * 8;
+            uint32_t base = 0;
+            for (uint32_t gi = g.pk_bs[p]; gi < g.pk_be[p]; gi++) {
+                const uint32_t c = g.gc[gi], b = g.gb[gi];
+                const JxsBand& bd = g.c[c].b[b];
+                if (l >= bd.lines[type])
+                    continue;
+                const uint32_t k = c * JXS_NBC + b;
+                const uint32_t gt = ((T[k >> 1] >> (16 * (k & 1))) & 0xFFFFu) >> 8;
+                int16_t* rowp = F.coef + bd.coeff_off + (size_t)(row * bd.hl + l) * bd.cs;
+                uint32_t pad_nz = 0;
+                for (uint32_t i0 = 0; i0 < bd.gw; i0 += WAVE) {
+                    const uint32_t i = i0 + lane;
+                    const uint32_t gc = i < bd.gw ? gsh[base + i] : 0u;
+                    const uint32_t np = gc > gt ? gc - gt : 0u;
+                    const uint32_t nb = np ? 4 * (np + (g.fs ? 0u : 1u)) : 0u;
+                    const uint32_t incl = wave_incl_scan(nb);
+                    const uint32_t tot = __shfl(incl, WAVE - 1, WAVE);
+                    const uint64_t bp = dbit + incl - nb;
+                    uint32_t mag[4] = {0, 0, 0, 0}, sgn = 0;
+                    if (nb) {
+                        if (bp + nb > n_off * 8) {
+                            bad = true;
+                        } else {
+                            const uint64_t v = cs_bits64(V, bp + V.sh);
+                            const uint32_t skip = g.fs ? 0u : 4u;
+                            sgn = g.fs ? 0u : (uint32_t)(v >> 60);
+#pragma unroll
+                            for (int t4 = 0; t4 < 4; t4++)
+                                mag[t4] = (gather4(v << (skip + t4)) >> (16 - np)) << gt;
+                        }
+                    }
+                    if (i < bd.gw) {
+                        for (uint32_t t4 = 0; t4 < 4; t4++) {
+                            if (4 * i + t4 >= bd.w) {
+                                pad_nz += mag[t4] != 0; /* signs of padding coefficients are skipped */
+                                continue;
+                            }
+                            const uint32_t m = dequant1(mag[t4], gc, gt, g.qpih) & 0x7FFF;
+                            const bool neg = (sgn >> (3 - t4)) & 1u;
+                            rowp[4 * i + t4] = (int16_t)(neg ? -(int32_t)m : (int32_t)m);
+                        }
+                    }
+                    dbit += tot;
+                }
+                base += bd.gw;
+                if (g.fs) {
+                    /* the line's signs: one bit per non-zero coefficient */
+                    pad_nz = wave_incl_scan(pad_nz);
+                    pad_nz = __shfl(pad_nz, WAVE - 1, WAVE);
+                    __syncthreads(); /* the coefficients above were written by other lanes */
+                    for (uint32_t x0 = 0; x0 < bd.w; x0 += WAVE) {
+                        const uint32_t x = x0 + lane;
+                        const int16_t v = x < bd.w ? rowp[x] : (int16_t)0;
+                        const uint32_t nz = v != 0 ? 1u : 0u;
+                        const uint32_t incl = wave_incl_scan(nz);
+                        const uint32_t tot = __shfl(incl, WAVE - 1, WAVE);
+                        if (nz) {
+                            const uint64_t bp = nbit + incl - 1;
+                            if (bp >= nxt * 8)
+                                bad = true;
+                            else if (cs_bits(F, bp, 1))
+                                rowp[x] = (int16_t)-v;
+                        }
+                        nbit += tot;
+                    }
+                    nbit += pad_nz;
+                }
+            }
+            if (dbit > n_off * 8 || nbit > nxt * 8)
+                bad = true;
+            pos = nxt;
+            bad = __any(bad);
+            __syncthreads();
+        }
+        if (bad || (uint64_t)pos > pend) {
+            if (lane == 0)
+                atomicOr(F.status, 1u);
+            break;
+        }
+    }
+}
+
+/* ------------------------------------------------------------------ */
 /* inverse transform                                                   */
 /* ------------------------------------------------------------------ */
 __device__ __forceinline__ uint32_t nlt_out(int32_t v, uint32_t depth) {
@@ -795,6 +1108,13 @@ hipError_t jxs_launch_dec_unpack(const JxsDecFrame* frames, int nfr, uint32_t ma
     if (!precincts)
         return hipSuccess;
     hipLaunchKernelGGL(k_dec_unpack, dim3(max_npk, precincts), dim3(PK_THREADS), 0, st, frames, nfr);
+    return hipGetLastError();
+}
+
+hipError_t jxs_launch_dec_slice(const JxsDecFrame* frames, int nfr, uint32_t slices, hipStream_t st) {
+    if (!slices)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_dec_slice, dim3(slices), dim3(WAVE), 0, st, frames, nfr);
     return hipGetLastError();
 }
 

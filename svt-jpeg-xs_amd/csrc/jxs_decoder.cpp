@@ -26,6 +26,7 @@ extern "C" {
 hipError_t jxs_launch_dec_scan(const JxsDecFrame*, int, hipStream_t);
 hipError_t jxs_launch_dec_packets(const JxsDecFrame*, int, uint32_t, hipStream_t);
 hipError_t jxs_launch_dec_unpack(const JxsDecFrame*, int, uint32_t, uint32_t, hipStream_t);
+hipError_t jxs_launch_dec_slice(const JxsDecFrame*, int, uint32_t, hipStream_t);
 hipError_t jxs_launch_inv_rows(const JxsInvRowJob*, const uint32_t*, int, uint32_t, hipStream_t);
 hipError_t jxs_launch_inv_vert(const JxsInvVertJob*, const uint32_t*, int, uint32_t, hipStream_t);
 hipError_t jxs_launch_zero(uint32_t*, uint32_t, hipStream_t);
@@ -204,7 +205,7 @@ static uint32_t dec_pkt_stride(const JxsGeom& g) { return JXS_DPKT_HDR + JXS_DPK
 
 static uint64_t dec_work_bytes(const JxsGeom& g) {
     return al256(g.n_coeff * 2) + al256(g.n_i32 * 4) + al256((uint64_t)g.rows * 4) +
-        al256((uint64_t)g.rows * dec_pkt_stride(g) * 4);
+        al256((uint64_t)g.rows * dec_pkt_stride(g) * 4) + al256(g.n_gcli);
 }
 
 }  // namespace jxs
@@ -220,7 +221,7 @@ struct svt_jpeg_xs_amd_decode_plan {
     uint32_t* d_vbase[2] = {nullptr, nullptr};
     int nrows[2] = {0, 0}, nvert[2] = {0, 0};
     uint32_t rblocks[2] = {0, 0}, vblocks[2] = {0, 0};
-    uint32_t total_prec = 0, max_npk = 0;
+    uint32_t total_prec = 0, max_npk = 0, total_slices = 0;
 };
 
 namespace jxs {
@@ -297,6 +298,10 @@ static svt_jpeg_xs_amd_decode_plan_t* dplan_build(const std::vector<DecSpec>& sp
         F.pkt_stride = dec_pkt_stride(g);
         F.pkt = (uint32_t*)cur;
         cur += al256((uint64_t)g.rows * F.pkt_stride * 4);
+        F.gcli = cur;
+        cur += al256(g.n_gcli);
+        F.slice_base = p->total_slices;
+        p->total_slices += g.slices;
         F.status = p->d_status + i;
         F.prec_base = base;
         F.npk_dec = s.geom->npk_dec;
@@ -353,6 +358,8 @@ static hipError_t dplan_launch(svt_jpeg_xs_amd_decode_plan_t* p, hipStream_t st)
     if ((e = jxs_launch_dec_packets(p->d_frames, (int)p->nfr, p->total_prec, st)) != hipSuccess)
         return e;
     if ((e = jxs_launch_dec_unpack(p->d_frames, (int)p->nfr, p->max_npk, p->total_prec, st)) != hipSuccess)
+        return e;
+    if ((e = jxs_launch_dec_slice(p->d_frames, (int)p->nfr, p->total_slices, st)) != hipSuccess)
         return e;
     if ((e = jxs_launch_inv_rows(p->d_rows[0], p->d_rbase[0], p->nrows[0], p->rblocks[0], st)) != hipSuccess)
         return e;
@@ -442,7 +449,7 @@ static void fill_image_config(const JxsGeom& g, svt_jpeg_xs_image_config_t* cfg)
 static const int kDecSlots = 3;
 
 struct DecSlot {
-    DecGeom geom;             /* the decoder's geometry; its device copy carries this slot's frame's Qpih */
+    DecGeom geom;             /* the decoder's geometry; its device copy carries this slot's frame's Qpih, Fs, Rm */
     uint32_t cap = 0;         /* codestream bytes the slot holds */
     uint8_t* d_cs = nullptr;
     uint8_t* h_cs = nullptr;  /* page-locked staging of the codestream */
@@ -450,8 +457,8 @@ struct DecSlot {
     uint32_t* h_prec = nullptr; /* page-locked precinct offsets */
     void* d_planes = nullptr;
     JxsDecFrame* h_frame = nullptr; /* page-locked copy of the plan's frame descriptor */
-    uint32_t* h_qpih = nullptr;     /* page-locked Qpih word */
-    uint32_t qpih = 0;              /* Qpih in the slot's device geometry */
+    JxsGeom* h_geom = nullptr;      /* page-locked copy of the slot's geometry */
+    uint32_t dyn[3] = {0, 0, 0};    /* Qpih, Fs, Rm in the slot's device geometry */
     svt_jpeg_xs_amd_decode_plan_t* plan = nullptr;
     hipEvent_t done = nullptr;
     uint32_t* h_status = nullptr;
@@ -503,7 +510,7 @@ static void slot_release(DecSlot& s) {
     (void)hipFree(s.d_prec);
     (void)hipFree(s.d_planes);
     (void)hipFree(s.geom.d_geom);
-    for (void* p : {(void*)s.h_prec, (void*)s.h_frame, (void*)s.h_qpih, (void*)s.h_status})
+    for (void* p : {(void*)s.h_prec, (void*)s.h_frame, (void*)s.h_geom, (void*)s.h_status})
         if (p)
             (void)hipHostFree(p);
     if (s.done)
@@ -542,13 +549,16 @@ static bool slot_create(DecoderPriv* d, DecSlot& s, uint32_t cap) {
     const JxsGeom& g = d->geom.host.g;
     s.geom = d->geom;
     s.geom.d_geom = nullptr;
-    s.qpih = g.qpih;
+    s.dyn[0] = g.qpih;
+    s.dyn[1] = g.fs;
+    s.dyn[2] = g.rm;
     if (hipMalloc(&s.geom.d_geom, sizeof(JxsGeom)) != hipSuccess ||
         hipMemcpy(s.geom.d_geom, &g, sizeof(JxsGeom), hipMemcpyHostToDevice) != hipSuccess ||
         hipMalloc(&s.d_prec, (size_t)g.rows * 4) != hipSuccess || hipMalloc(&s.d_planes, d->plane_bytes) != hipSuccess ||
         hipHostMalloc((void**)&s.h_prec, (size_t)g.rows * 4, 0) != hipSuccess ||
         hipHostMalloc((void**)&s.h_frame, sizeof(JxsDecFrame), 0) != hipSuccess ||
-        hipHostMalloc((void**)&s.h_qpih, 4, 0) != hipSuccess || hipHostMalloc((void**)&s.h_status, 4, 0) != hipSuccess ||
+        hipHostMalloc((void**)&s.h_geom, sizeof(JxsGeom), 0) != hipSuccess ||
+        hipHostMalloc((void**)&s.h_status, 4, 0) != hipSuccess ||
         hipEventCreateWithFlags(&s.done, hipEventDisableTiming) != hipSuccess)
         return false;
     return slot_bind(d, s, cap);
@@ -666,8 +676,9 @@ static void enqueue(DecoderPriv* d, const DecJob& j) {
 }
 
 /* The frame's header against the decoder's: a different static
- * configuration is SvtJxsErrorDecoderConfigChange (Decoder.c:388-390); Qpih
- * may change from frame to frame (Decoder.c:576, ParseHeader.c:238-241). */
+ * configuration is SvtJxsErrorDecoderConfigChange (Decoder.c:388-390); Qpih,
+ * Fs and Rm may change from frame to frame (picture_header_dynamic_t,
+ * Pi.h:63-80; Decoder.c:576, Packing.c:616, 829). */
 static int32_t check_frame_header(const DecoderPriv* d, const JxsGeom& fg) {
     const JxsGeom& g = d->geom.host.g;
     if (fg.w != g.w || fg.h != g.h || fg.nc != g.nc || fg.depth != g.depth || fg.dv != g.dv || fg.dh != g.dh ||
@@ -678,16 +689,20 @@ static int32_t check_frame_header(const DecoderPriv* d, const JxsGeom& fg) {
 
 /* Queues the decode of the codestream staged in slot si (fsize bytes,
  * precinct offsets in h_prec, Qpih qpih) into frame's image buffer. */
-static SvtJxsErrorType_t launch_slot(DecoderPriv* d, int si, uint32_t fsize, uint32_t qpih,
+static SvtJxsErrorType_t launch_slot(DecoderPriv* d, int si, uint32_t fsize, const JxsGeom& fg,
                                      const svt_jpeg_xs_frame_t& frame) {
     DecSlot& s = d->slots[si];
     const JxsGeom& g = d->geom.host.g;
     hipError_t he = hipSuccess;
-    if (qpih != s.qpih) {
-        *s.h_qpih = qpih;
-        he = hipMemcpyAsync((uint8_t*)s.geom.d_geom + offsetof(JxsGeom, qpih), s.h_qpih, 4, hipMemcpyHostToDevice,
-                            d->stream);
-        s.qpih = qpih;
+    if (fg.qpih != s.dyn[0] || fg.fs != s.dyn[1] || fg.rm != s.dyn[2]) {
+        *s.h_geom = g;
+        s.h_geom->qpih = fg.qpih;
+        s.h_geom->fs = fg.fs;
+        s.h_geom->rm = fg.rm;
+        he = hipMemcpyAsync(s.geom.d_geom, s.h_geom, sizeof(JxsGeom), hipMemcpyHostToDevice, d->stream);
+        s.dyn[0] = fg.qpih;
+        s.dyn[1] = fg.fs;
+        s.dyn[2] = fg.rm;
     }
     s.h_frame->cs_len = fsize;
     if (he == hipSuccess)
@@ -703,7 +718,7 @@ static SvtJxsErrorType_t launch_slot(DecoderPriv* d, int si, uint32_t fsize, uin
     if (he == hipSuccess)
         he = hipEventRecord(s.done, d->stream);
     if (he != hipSuccess) {
-        s.qpih = ~0u; /* unknown: upload again next time */
+        s.dyn[0] = ~0u; /* unknown: upload again next time */
         free_slot(d, si);
         return SvtJxsErrorDecoderInternal;
     }
@@ -742,7 +757,7 @@ static SvtJxsErrorType_t submit_staged(DecoderPriv* d, int si, uint32_t len, con
         return SvtJxsErrorNone;
     }
     memcpy(s.h_prec, prec.data(), prec.size() * 4);
-    return launch_slot(d, si, fsize, hg.g.qpih, frame);
+    return launch_slot(d, si, fsize, hg.g, frame);
 }
 
 }  // namespace jxs

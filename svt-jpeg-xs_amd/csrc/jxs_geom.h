@@ -83,7 +83,11 @@ struct JxsGeom {
     uint32_t rq_n[2];
     uint32_t rc_mode; /* rate control: 0 budget per precinct, 1 unused padding moved to the next precinct,
                          2 / 3 budgets from one (Q, R) per slice (3: with a minimum precinct size) */
-    uint32_t pad_rq;
+    uint32_t fs;      /* Fs: signs in sub-packets of their own */
+    uint32_t rm;      /* Rm: vertically predicted significance groups are runs of zero coefficients */
+    uint32_t vpred;   /* encoder: coding_vertical_prediction_mode (0 off, 1 zero residuals, 2 zero coefficients) */
+    uint32_t signs;   /* encoder: coding_signs_handling (0 off, 1 fast, 2 full) */
+    uint32_t pad_t;
     uint64_t n_coeff, n_gcli, n_smax; /* buffer sizes per frame */
     uint64_t n_i32;                   /* int32 transform scratch per frame */
     uint8_t hdr[JXS_HDR_MAX];

@@ -681,11 +681,10 @@ int32_t parse_codestream_header(const uint8_t* cs, size_t len, HostGeom& out, ui
             lh = r.get(1);
             r.get(1);
             g.qpih = r.get(2);
-            if (g.qpih > 1) /* ParseHeader.c:238-241 */
+            g.fs = r.get(2);
+            g.rm = r.get(2);
+            if (g.qpih > 1 || g.fs > 1 || g.rm > 1) /* ParseHeader.c:238-251 */
                 return E_INVALID_BS;
-            if (r.get(2) != 0) /* Fs: separate sign coding is not decoded */
-                return E_INVALID_BS;
-            r.get(2); /* Rm: only used by vertically predicted bands, rejected per precinct */
             pih = true;
             break;
         case 0xFF13:

@@ -28,7 +28,7 @@ REF = "/root/reference"
 from jxs_util import load_oracle, load_synth  # noqa: E402
 
 # name: (W, H, depth, format, bpp_num, bpp_den, decomp_v, decomp_h, quant, slice_h, significance, sigma, frame
-#        [, rate_control_mode (default 1)])
+#        [, rate_control_mode (default 1) [, coding_vertical_prediction_mode, coding_signs_handling (default 0)]])
 SMALL = {
     "s_64x32_422_8b_bpp5_v2": (64, 32, 8, 3, 5, 1, 2, 5, 0, 16, 1, 4.0, 0),
     "s_256x64_422_10b_bpp3_v2": (256, 64, 10, 3, 3, 1, 2, 5, 0, 16, 1, 4.0, 1),
@@ -62,6 +62,22 @@ SMALL = {
     "u_256x64_422_10b_bpp3_v2_rc3": (256, 64, 10, 3, 3, 1, 2, 5, 0, 16, 1, 8.0, 1, 3),
     "u_200x96_444_8b_bpp2_v1_rc3": (200, 96, 8, 4, 2, 1, 1, 5, 0, 16, 1, 8.0, 2, 3),
     "u_320x96_420_8b_bpp2_v2_slice32_rc3": (320, 96, 8, 2, 2, 1, 2, 5, 0, 32, 1, 8.0, 3, 3),
+    # vertical prediction (1: zero residuals, 2: zero coefficients / Rm = 1)
+    # and sign coding (1: fast, 2: full)
+    "t_256x64_422_10b_bpp3_v2_vp1": (256, 64, 10, 3, 3, 1, 2, 5, 0, 16, 1, 4.0, 1, 1, 1, 0),
+    "t_256x64_422_10b_bpp3_v2_vp2": (256, 64, 10, 3, 3, 1, 2, 5, 0, 16, 1, 4.0, 1, 1, 2, 0),
+    "t_200x200_444_8b_bpp3_v1_vp1_rc0": (200, 200, 8, 4, 3, 1, 1, 5, 0, 16, 1, 4.0, 2, 0, 1, 0),
+    "t_160x90_422_10b_bpp5_nosig_vp1": (160, 90, 10, 3, 5, 1, 2, 5, 0, 16, 0, 4.0, 7, 1, 1, 0),
+    "t_256x128_420_8b_bpp2_v2_vp2_rc3": (256, 128, 8, 2, 2, 1, 2, 5, 0, 16, 1, 4.0, 4, 3, 2, 0),
+    "t_256x64_422_10b_bpp3_v2_sg1": (256, 64, 10, 3, 3, 1, 2, 5, 0, 16, 1, 4.0, 1, 1, 0, 1),
+    "t_256x64_422_10b_bpp3_v2_sg2": (256, 64, 10, 3, 3, 1, 2, 5, 0, 16, 1, 4.0, 1, 1, 0, 2),
+    "t_160x90_444_12b_bpp4_uniform_sg2": (160, 90, 12, 4, 4, 1, 2, 5, 1, 16, 1, 4.0, 6, 1, 0, 2),
+    "t_200x200_444_8b_bpp3_v1_sg1_rc0": (200, 200, 8, 4, 3, 1, 1, 5, 0, 16, 1, 4.0, 2, 0, 0, 1),
+    "t_320x180_422_10b_bpp7_2_slice32_sg1_rc2": (320, 180, 10, 3, 7, 2, 2, 5, 0, 32, 1, 4.0, 11, 2, 0, 1),
+    "t_96x64_rgbpacked_14b_bpp2_v1_vp1_sg2": (96, 64, 14, 21, 2, 1, 1, 5, 0, 16, 1, 16.0, 8, 1, 1, 2),
+    "t_256x64_422_10b_bpp3_v2_vp2_sg1_rc3": (256, 64, 10, 3, 3, 1, 2, 5, 0, 16, 1, 4.0, 1, 3, 2, 1),
+    "t_130x66_422_9b_bpp4_v1h2_vp1_sg2_rc0": (130, 66, 9, 3, 4, 1, 1, 2, 0, 16, 1, 4.0, 10, 0, 1, 2),
+    "t_160x90_444_12b_bpp4_uniform_vp1_sg1": (160, 90, 12, 4, 4, 1, 2, 5, 1, 16, 1, 4.0, 6, 1, 1, 1),
 }
 
 # The configurations of the benchmark workloads, first frame of each.
@@ -91,6 +107,7 @@ def sha(b):
 def ref_encode(cfg, planes, td):
     W, H, D, F, bn, bd, V, Hh, q, sh, sig = cfg[:11]
     rc = cfg[13] if len(cfg) > 13 else 1
+    vp, sg = (cfg[14], cfg[15]) if len(cfg) > 15 else (0, 0)
     O = load_oracle()
     raw = os.path.join(td, "in.raw")
     out = os.path.join(td, "out.jxs")
@@ -98,7 +115,7 @@ def ref_encode(cfg, planes, td):
     with open(raw, "wb") as f:
         f.write(S.frame_bytes(planes))
     cmd = [O.REF_ENC, raw, out, str(W), str(H), str(D), str(F), str(bn), str(bd), str(V), str(Hh), "1", "1", str(sh),
-           str(q), str(sig), str(rc)]
+           str(q), str(sig), str(rc), str(vp), str(sg)]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         return None, r.returncode
@@ -127,6 +144,9 @@ def case_entry(name, cfg, td, keep):
                              "quantization", "slice_height", "significance", "sigma", "frame"], cfg[:13])),
          "input_sha256": S.frame_sha256(planes)}
     e["params"]["rate_control_mode"] = cfg[13] if len(cfg) > 13 else 1
+    if len(cfg) > 15:
+        e["params"]["vertical_prediction"] = cfg[14]
+        e["params"]["signs_handling"] = cfg[15]
     if name.startswith("u_"):
         e["params"]["pattern"] = "flatten_rows"
     if cs is None:
@@ -223,6 +243,9 @@ def main():
         out["sample_16x16_422"] = {"codestream_sha256": sha(cs), "codestream_bytes": len(cs),
                                    "decoded_sha256": sha(dec), "decoded_bytes": len(dec),
                                    "uses_vertical_prediction": True}
+        # the reference decoder's planes of the sample, kept as a file so a
+        # mismatch can be located (16x16 + 2 x 8x16 bytes)
+        open(os.path.join(HERE, "sample_16x16_422.yuv"), "wb").write(dec)
     out["api_struct_sizes"] = api_struct_sizes()
     out["quant_kat"] = quant_kats()
     with open(os.path.join(HERE, "golden.json"), "w") as f:

@@ -3,7 +3,7 @@ the SvtJpegxsDec C-ABI and through device-resident decode plans."""
 import numpy as np
 import pytest
 
-from jxs_util import load_oracle, load_package, load_synth
+from jxs_util import ROOT, load_oracle, load_package, load_synth
 
 CASES = [
     (64, 32, 8, 3, 5, 2, 5, 0, 1),
@@ -303,3 +303,69 @@ def test_decoder_follows_qpih_per_frame():
     rc, _ = dec.decode(bytes(bad))
     assert rc == P.ERR_DEC_INVALID_BITSTREAM
     dec.close()
+
+
+_TOOLS = sorted(n for n in _G["small"] if n.startswith("t_")) + ["sample_16x16_422"]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", _TOOLS)
+def test_decode_vertical_prediction_and_sign_codestreams(name):
+    """Reference codestreams coded with vertical prediction (D[p,b] bit 0,
+    both run modes) and separate sign sub-packets (Fs = 1, fast and full),
+    and the reference's own unit-test sample (SampleFramesData.cc:8-33), which
+    uses vertical prediction: the GPU decoder's planes hash to the reference
+    decoder's."""
+    print("tools decode", name, flush=True)
+    P = load_package()
+    cs = open(os.path.join(_GOLDEN, name + ".jxs"), "rb").read()
+    want = (_G["small"][name] if name in _G["small"] else _G[name])["decoded_sha256"]
+    dec = P.Decoder(cs, verbose=0)
+    for _ in range(2):
+        rc, planes = dec.decode(cs)
+        assert rc == 0
+        assert hashlib.sha256(b"".join(p.tobytes() for p in planes)).hexdigest() == want
+    dec.close()
+
+
+@pytest.mark.gpu
+def test_decode_plan_of_mixed_tool_codestreams():
+    """One device-resident decode plan holding codestreams with and without
+    vertical prediction and sign sub-packets: slices that need in-order
+    decoding and slices decoded packet-parallel side by side, each frame equal
+    to the oracle's decode."""
+    import torch
+    print("tools decode plan", flush=True)
+    P, O = load_package(), load_oracle()
+    dev = torch.device("cuda:0")
+    names = ["t_256x64_422_10b_bpp3_v2_vp1", "s_256x64_422_10b_bpp3_v2", "t_256x64_422_10b_bpp3_v2_sg2",
+             "t_200x200_444_8b_bpp3_v1_vp1_rc0", "t_96x64_rgbpacked_14b_bpp2_v1_vp1_sg2"]
+    decs, specs, outs, refs, keep = [], [], [], [], []
+    for n in names:
+        cs = open(os.path.join(_GOLDEN, n + ".jxs"), "rb").read()
+        e, ref = O.decode(cs)
+        assert e == 0
+        dec = P.Decoder(cs, verbose=0)
+        decs.append(dec)
+        d_cs = torch.from_numpy(np.frombuffer(cs, dtype=np.uint8).copy()).to(dev)
+        c = dec.config
+        planes = [torch.zeros((c.components[k].height, c.components[k].width),
+                              dtype=torch.int16 if c.bit_depth > 8 else torch.uint8, device=dev)
+                  for k in range(c.components_num)]
+        keep.append(d_cs)
+        outs.append(planes)
+        refs.append(ref)
+        specs.append((dec, d_cs.data_ptr(), len(cs), [t.data_ptr() for t in planes], [t.shape[1] for t in planes],
+                      None, 0))
+    plan = P.DecodePlan(specs)
+    plan.run(torch.cuda.current_stream(dev).cuda_stream)
+    torch.cuda.synchronize()
+    assert plan.status()[0] == 0
+    for n, planes, ref in zip(names, outs, refs):
+        for t, r in zip(planes, ref):
+            got = t.cpu().numpy()
+            got = got.view(np.uint16) if got.dtype == np.int16 else got
+            assert np.array_equal(got, r), n
+    plan.destroy()
+    for dec in decs:
+        dec.close()

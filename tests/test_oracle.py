@@ -25,7 +25,7 @@ def _params(p):
     O = load_oracle()
     return O.make_params(p["width"], p["height"], p["depth"], p["format"], p["bpp_num"], p["bpp_den"], p["decomp_v"],
                          p["decomp_h"], p["quantization"], p["slice_height"], p["significance"],
-                         p.get("rate_control_mode", 1))
+                         p.get("rate_control_mode", 1), p.get("vertical_prediction", 0), p.get("signs_handling", 0))
 
 
 def _planes(p):
@@ -177,10 +177,10 @@ def test_oracle_full_size_workloads_match_reference(name):
     assert sha(b"".join(x.tobytes() for x in dec)) == entry["decoded_sha256"]
 
 
-def test_reference_sample_codestream_header():
+def test_reference_sample_codestream_decodes_to_reference_planes():
     """The 16x16 4:2:2 codestream of the reference's unit tests
-    (SampleFramesData.cc) uses vertical prediction, which this path does not
-    decode: the header parses, the decode reports an invalid bitstream."""
+    (SampleFramesData.cc:8-33, decoded by TestDecoder.cc) uses vertical
+    prediction: the oracle decodes it to the reference decoder's planes."""
     O = load_oracle()
     cs = open(os.path.join(GOLDEN, "sample_16x16_422.jxs"), "rb").read()
     assert sha(cs) == G["sample_16x16_422"]["codestream_sha256"]
@@ -190,8 +190,44 @@ def test_reference_sample_codestream_header():
     assert O.lib().jxso_probe(buf.ctypes.data, len(buf), ctypes.byref(info)) == 0
     assert (info.width, info.height, info.depth, info.comps) == (16, 16, 8, 3)
     assert (info.comp_w[1], info.comp_h[1]) == (8, 16)
-    e, _ = O.decode(cs)
-    assert e == -0x7FFFCFFF  # SvtJxsErrorDecoderInvalidBitstream
+    e, planes = O.decode(cs)
+    assert e == 0
+    got = b"".join(p.tobytes() for p in planes)
+    assert got == open(os.path.join(GOLDEN, "sample_16x16_422.yuv"), "rb").read()
+    assert sha(got) == G["sample_16x16_422"]["decoded_sha256"]
+
+
+def _clear_vpred_bits(cs):
+    """The codestream with bit 0 (vertical prediction) of every band's
+    D[p,b] cleared in every precinct header."""
+    cs = bytearray(cs)
+    wgt = cs.index(b"\xff\x14")
+    nb = ((cs[wgt + 2] << 8 | cs[wgt + 3]) - 2) // 2
+    pos = cs.index(b"\xff\x20")
+    hdr = (40 + 2 * nb + 7) // 8
+    while cs[pos:pos + 2] != b"\xff\x11":
+        if cs[pos:pos + 2] == b"\xff\x20":
+            pos += 6
+            continue
+        lprc = cs[pos] << 16 | cs[pos + 1] << 8 | cs[pos + 2]
+        for k in range(nb):
+            bit = 40 + 2 * k + 1  # low bit of the k-th 2-bit field
+            cs[pos + bit // 8] &= ~(0x80 >> (bit % 8)) & 0xFF
+        pos += hdr + lprc
+    return bytes(cs)
+
+
+def test_vertical_prediction_planted_mistake_is_caught():
+    """Planted mistake: read as zero-predicted (the D[p,b] vertical bits
+    cleared), the sample's bit-plane counts no longer give the reference
+    decoder's planes -- the vertical-prediction decode is what pins them."""
+    O = load_oracle()
+    cs = open(os.path.join(GOLDEN, "sample_16x16_422.jxs"), "rb").read()
+    bad = _clear_vpred_bits(cs)
+    assert bad != cs
+    e, planes = O.decode(bad)
+    ref = open(os.path.join(GOLDEN, "sample_16x16_422.yuv"), "rb").read()
+    assert e != 0 or b"".join(p.tobytes() for p in planes) != ref
 
 
 @pytest.mark.parametrize("proxy", [1, 2])
