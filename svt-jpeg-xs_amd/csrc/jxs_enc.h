@@ -45,7 +45,8 @@ struct BatchSizer {
     uint64_t work = 0;
     void add(const DevGeom& g) {
         n++;
-        units += g.host.g.rows * g.host.g.n_units;
+        /* frames with vertical prediction or sign coding are packed without units */
+        units += (g.host.g.vpred || g.host.g.signs) ? 0u : g.host.g.rows * g.host.g.n_units;
         for (int s = 0; s < 2; s++)
             njobs[s] += g.njobs[s];
         work += g.work;
@@ -67,6 +68,7 @@ struct BatchLaunch {
     int ndwt[2][3] = {};
     uint32_t dblocks[2][3] = {};
     uint32_t total_slices = 0, total_units = 0;
+    bool ordered = false; /* a frame is coded by k_enc_slice */
 };
 
 /* Checks one frame spec (initialised encoder, codestream buffer 4-byte

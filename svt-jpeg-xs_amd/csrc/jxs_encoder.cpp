@@ -24,6 +24,7 @@ hipError_t jxs_launch_dwt(const JxsDwtJob*, const uint32_t*, int, uint32_t, uint
 hipError_t jxs_launch_rate(const JxsFrameDesc*, int, uint32_t, hipStream_t);
 hipError_t jxs_launch_pack(const JxsUnit*, uint32_t, uint32_t, hipStream_t);
 hipError_t jxs_launch_join(const JxsUnit*, uint32_t, hipStream_t);
+hipError_t jxs_launch_enc_slice(const JxsFrameDesc*, int, uint32_t, hipStream_t);
 }
 
 namespace jxs {
@@ -37,6 +38,14 @@ namespace jxs {
         }                                                                                   \
     } while (0)
 
+/* Shadow words of a frame: one per codestream dword (the packer's boundary
+ * words), or, for a frame k_enc_slice codes, the staging words of its
+ * slices, each slice starting a word of its own. */
+static uint64_t shadow_bytes(const JxsGeom& g) { return ((uint64_t)g.lcod + 3) / 4 * 4 + 4ull * g.slices + 16; }
+
+/* Frames whose coding tools k_enc_slice handles. */
+static bool geom_ordered(const JxsGeom& g) { return g.vpred != 0 || g.signs != 0; }
+
 /* Work bytes of one frame of a geometry (inputs and outputs excluded). */
 static uint64_t frame_work_bytes(const JxsGeom& g) {
     uint64_t b = 0;
@@ -46,7 +55,7 @@ static uint64_t frame_work_bytes(const JxsGeom& g) {
     b += align256(g.n_i32 * 4);
     b += align256((uint64_t)g.rows * g.n_bl * JXS_HREC * 2);
     b += align256((uint64_t)g.rows * sizeof(JxsPrec));
-    b += align256(((uint64_t)g.lcod + 3) / 4 * 4 + 16); /* shadow words */
+    b += align256(shadow_bytes(g));
     return b;
 }
 
@@ -243,7 +252,7 @@ bool batch_place_into(const svt_jpeg_xs_amd_frame_spec_t* specs, uint32_t count,
         F.prec = (JxsPrec*)cur;
         cur += align256((uint64_t)g.rows * sizeof(JxsPrec));
         F.shadow = (uint32_t*)cur;
-        cur += align256(((uint64_t)g.lcod + 3) / 4 * 4 + 16);
+        cur += align256(shadow_bytes(g));
         F.units = L.units + unit_base;
         F.out = specs[i].d_out;
         F.status = L.status + i;
@@ -256,10 +265,12 @@ bool batch_place_into(const svt_jpeg_xs_amd_frame_spec_t* specs, uint32_t count,
         F.pps = g.pps;
         F.slices = g.slices;
         F.n_bl = g.n_bl;
+        F.ordered = geom_ordered(g) ? 1u : 0u;
+        L.ordered |= F.ordered != 0;
         memcpy(img + frames_off + (uint64_t)i * sizeof(JxsFrameDesc), &F, sizeof(F));
         prec_base += g.rows;
         slice_base += g.slices;
-        unit_base += g.rows * g.n_units;
+        unit_base += F.ordered ? 0u : g.rows * g.n_units; /* k_enc_slice packs without units */
         FwdJobs jobs;
         fwd_jobs(g, specs[i].d_planes, specs[i].strides, F.coeff, F.gcli, F.smax, F.hist, scratch, jobs);
         for (int s = 0; s < 2; s++)
@@ -317,6 +328,8 @@ hipError_t batch_launch(const BatchLaunch& L, hipStream_t st, hipEvent_t* ev) {
     if ((e = jxs_launch_pack(L.units, L.total_units, shadow ? 1u : 0u, st)) != hipSuccess)
         return e;
     if (shadow && (e = jxs_launch_join(L.units, L.total_units, st)) != hipSuccess)
+        return e;
+    if (L.ordered && (e = jxs_launch_enc_slice(L.frames, (int)L.nfr, L.total_slices, st)) != hipSuccess)
         return e;
     if (ev)
         (void)hipEventRecord(ev[SVT_JPEG_XS_AMD_STAGES], st);
@@ -460,15 +473,17 @@ static SvtJxsErrorType_t encoder_create(svt_jpeg_xs_encoder_api_t* api) {
     p.slice_height = api->slice_height;
     p.significance = api->coding_significance;
     p.rc_mode = api->rate_control_mode;
-    /* Coding tools outside this set have no GPU implementation yet: rate
-     * control modes 0..3 (budget per precinct without or with padding
-     * movement, budget per slice without or with a minimum precinct size),
-     * no vertical prediction, no sign handling. */
-    if (api->rate_control_mode > 3 || api->coding_vertical_prediction_mode != 0 || api->coding_signs_handling != 0) {
+    p.vpred = api->coding_vertical_prediction_mode;
+    p.signs = api->coding_signs_handling;
+    /* EncHandle.c:242-278.  Full sign coding with the per-slice rate control
+     * modes is refused: there the reference's per-band cache of two
+     * truncations carries sizes from the slice-wide search (made with fast
+     * sign coding) into the precincts' own searches, and that is not
+     * restated. */
+    if (api->rate_control_mode > 3 || api->coding_vertical_prediction_mode > 2 || api->coding_signs_handling > 2 ||
+        (api->coding_signs_handling == 2 && api->rate_control_mode >= 2)) {
         if (api->verbose >= VERBOSE_ERRORS)
-            fprintf(stderr,
-                    "svt-jpeg-xs-amd: rate_control_mode 0..3 without vertical prediction or sign handling is "
-                    "implemented\n");
+            fprintf(stderr, "svt-jpeg-xs-amd: unsupported rate control / vertical prediction / sign coding mode\n");
         return SvtJxsErrorBadParameter;
     }
     if (api->slice_packetization_mode > 1 || api->cpu_profile > 1)

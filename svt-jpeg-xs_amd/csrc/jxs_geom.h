@@ -197,7 +197,8 @@ struct JxsFrameDesc {
     uint32_t slice_base;
     uint32_t unit_base; /* first packing unit (rows x n_units numbering) of this frame in the launch */
     uint32_t n_units;
-    uint32_t pps, slices, n_bl, pad_; /* copies of the geometry's, readable before it is staged */
+    uint32_t pps, slices, n_bl; /* copies of the geometry's, readable before it is staged */
+    uint32_t ordered;           /* coded by k_enc_slice (vertical prediction or sign coding), not k_rate / k_pack */
 };
 
 #endif

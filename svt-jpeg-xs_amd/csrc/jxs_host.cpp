@@ -442,8 +442,8 @@ static uint32_t write_header(JxsGeom& g) {
     o.put(!g.short_hdr, 1);
     o.put(1, 1);
     o.put(g.qpih, 2);
-    o.put(0, 2);
-    o.put(0, 2);
+    o.put(g.fs, 2); /* Fs (EncHandle.c:274-278) */
+    o.put(g.rm, 2); /* Rm (EncHandle.c:250-253) */
     o.put(0xFF13, 16); /* CDT */
     o.put(2 * g.nc + 2, 16);
     for (uint32_t c = 0; c < g.nc; c++) {
@@ -553,6 +553,10 @@ int32_t build_encoder_geometry(const EncParams& p, HostGeom& out, int verbose) {
     g.qpih = p.quantization;
     g.sig = p.significance ? 1 : 0;
     g.rc_mode = p.rc_mode;
+    g.vpred = p.vpred;
+    g.signs = p.signs;
+    g.fs = p.signs ? 1u : 0u;
+    g.rm = p.vpred == 2 ? 1u : 0u;
     WTab t;
     bool ok = nc == 3 && wtab_default(p.format, p.decomp_v, p.decomp_h, t) == 0;
     if (!ok && (nc != 3 || wtab_derived(p.format, p.decomp_v, p.decomp_h, t)))

@@ -17,7 +17,9 @@ struct EncParams {
     uint32_t width, height, depth, format;
     uint32_t bpp_num, bpp_den, decomp_v, decomp_h;
     uint32_t quantization, slice_height, significance;
-    uint32_t rc_mode; /* rate_control_mode 0 or 1 */
+    uint32_t rc_mode; /* rate_control_mode 0..3 */
+    uint32_t vpred;   /* coding_vertical_prediction_mode 0..2 */
+    uint32_t signs;   /* coding_signs_handling 0..2 */
 };
 
 /* Host geometry = device geometry + slice tables. */

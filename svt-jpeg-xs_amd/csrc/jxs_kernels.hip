@@ -1392,6 +1392,8 @@ __global__ void __launch_bounds__(WAVE * RT_WAVES) k_rate(const JxsFrameDesc* __
     }
     /* the frame descriptor in scalar registers */
     const JxsFrameDesc F = cld_words(frames + fl);
+    if (F.ordered)
+        return; /* coded by k_enc_slice */
     /* the wave index is made visibly uniform: everything a wave derives from
      * it (its precinct, the searches, the plan) then stays in scalar
      * registers and scalar branches */

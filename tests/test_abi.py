@@ -125,8 +125,10 @@ BAD_CONFIGS = [
     dict(colour_format=2, ndecomp_v=0),
     dict(source_width=16, source_height=16, ndecomp_h=5, colour_format=3),
     dict(rate_control_mode=4),
-    dict(coding_vertical_prediction_mode=1),
-    dict(coding_signs_handling=2),
+    dict(coding_vertical_prediction_mode=3),
+    dict(coding_signs_handling=3),
+    dict(coding_signs_handling=2, rate_control_mode=2),
+    dict(coding_signs_handling=2, rate_control_mode=3, coding_vertical_prediction_mode=1),
 ]
 
 
