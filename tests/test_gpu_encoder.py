@@ -81,7 +81,8 @@ def _gpu_encode(p):
         planes = S.flatten_rows(planes, p["height"])
     enc = P.Encoder(p["width"], p["height"], p["depth"], p["format"], p["bpp_num"], p["bpp_den"], p["decomp_v"],
                     p["decomp_h"], p["quantization"], p["slice_height"], p["significance"],
-                    rate_control_mode=p.get("rate_control_mode", 1))
+                    rate_control_mode=p.get("rate_control_mode", 1), vertical_prediction=p.get("vertical_prediction", 0),
+                    signs_handling=p.get("signs_handling", 0))
     rc, cs = enc.encode(planes)
     enc.close()
     return rc, cs
