@@ -85,7 +85,9 @@ def cpu_baseline(frames, budget_s=20.0):
                 res = json.loads(r.stdout.strip().splitlines()[-1])
                 return {"value": round(res["fps"], 3), "unit": "frames/s", "cores": cores, "kind": "reference",
                         "sample": "%d distinct frames of the 3840x2160 4:2:2 10-bit bpp3 workload sent %d times "
-                                  "(%d frames), reference SvtJpegxs (AVX2/AVX-512 kernels, lp=%d), %.1f s wall" %
+                                  "(%d frames), reference SvtJpegxs built from source (AVX2/AVX-512 C kernels; its "
+                                  "three NASM routines run as their C twins, no assembler in this image), lp=%d "
+                                  "threads on this host, %.1f s wall" %
                                   (len(sample), rep, res["frames"], cores, time.time() - t0),
                         "p50_ms": res["p50_ms"], "p99_ms": res["p99_ms"]}
             log("reference baseline failed rc=%d: %s" % (r.returncode, r.stderr[-300:]))
@@ -676,6 +678,10 @@ def main():
         "roofline": {"bound": "hbm", "kernel": KERNELS[STAGES[dom]], "stage": STAGES[dom],
                      "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": measured_traffic(STAGES[dom]),
+                     "duration_us": round(dom_ms * 1e3, 3),
+                     "duration_source": "HIP events around the kernel on its launch stream, mean of %d frames; "
+                                        "frac from the kernel-trace mean instead is algorithmic_bytes_per_launch / "
+                                        "(trace mean) / peak" % nt,
                      "algorithmic_bytes_per_launch": bytes_per_frame,
                      "pipeline_frac": round(bytes_per_frame * fps_plain / 1e9 / HBM_PEAK_GBS, 5)},
         "cpu_baseline": cpu,

@@ -67,8 +67,9 @@ struct BatchLaunch {
     uint32_t* dbase[2][3] = {};
     int ndwt[2][3] = {};
     uint32_t dblocks[2][3] = {};
-    uint32_t total_slices = 0, total_units = 0;
+    uint32_t total_slices = 0, total_prec = 0, total_units = 0;
     bool ordered = false; /* a frame is coded by k_enc_slice */
+    uint32_t rate_kinds = 0; /* bit 0: a frame under rate control mode 0 / 1, bit 1: mode 2 / 3 (k_rate) */
 };
 
 /* Checks one frame spec (initialised encoder, codestream buffer 4-byte

@@ -21,7 +21,7 @@
 
 extern "C" {
 hipError_t jxs_launch_dwt(const JxsDwtJob*, const uint32_t*, int, uint32_t, uint32_t, uint32_t*, uint32_t, hipStream_t);
-hipError_t jxs_launch_rate(const JxsFrameDesc*, int, uint32_t, hipStream_t);
+hipError_t jxs_launch_rate(const JxsFrameDesc*, int, uint32_t, uint32_t, uint32_t, hipStream_t);
 hipError_t jxs_launch_pack(const JxsUnit*, uint32_t, uint32_t, hipStream_t);
 hipError_t jxs_launch_join(const JxsUnit*, uint32_t, hipStream_t);
 hipError_t jxs_launch_enc_slice(const JxsFrameDesc*, int, uint32_t, hipStream_t);
@@ -55,6 +55,7 @@ static uint64_t frame_work_bytes(const JxsGeom& g) {
     b += align256(g.n_i32 * 4);
     b += align256((uint64_t)g.rows * g.n_bl * JXS_HREC * 2);
     b += align256((uint64_t)g.rows * sizeof(JxsPrec));
+    b += align256((uint64_t)g.rows * sizeof(JxsRateRec));
     b += align256(shadow_bytes(g));
     return b;
 }
@@ -251,6 +252,8 @@ bool batch_place_into(const svt_jpeg_xs_amd_frame_spec_t* specs, uint32_t count,
         cur += align256((uint64_t)g.rows * g.n_bl * JXS_HREC * 2);
         F.prec = (JxsPrec*)cur;
         cur += align256((uint64_t)g.rows * sizeof(JxsPrec));
+        F.rrec = (JxsRateRec*)cur;
+        cur += align256((uint64_t)g.rows * sizeof(JxsRateRec));
         F.shadow = (uint32_t*)cur;
         cur += align256(shadow_bytes(g));
         F.units = L.units + unit_base;
@@ -266,7 +269,10 @@ bool batch_place_into(const svt_jpeg_xs_amd_frame_spec_t* specs, uint32_t count,
         F.slices = g.slices;
         F.n_bl = g.n_bl;
         F.ordered = geom_ordered(g) ? 1u : 0u;
+        F.slice_rc = g.rc_mode >= 2 ? 1u : 0u;
         L.ordered |= F.ordered != 0;
+        if (!F.ordered)
+            L.rate_kinds |= 1u << F.slice_rc;
         memcpy(img + frames_off + (uint64_t)i * sizeof(JxsFrameDesc), &F, sizeof(F));
         prec_base += g.rows;
         slice_base += g.slices;
@@ -294,6 +300,7 @@ bool batch_place_into(const svt_jpeg_xs_amd_frame_spec_t* specs, uint32_t count,
             L.dblocks[s][c] = blk[s][c];
         }
     L.total_slices = slice_base;
+    L.total_prec = prec_base;
     L.total_units = unit_base;
     return (uint64_t)(cur - base) == sz.bytes();
 }
@@ -318,7 +325,7 @@ hipError_t batch_launch(const BatchLaunch& L, hipStream_t st, hipEvent_t* ev) {
     }
     if (ev)
         (void)hipEventRecord(ev[2], st);
-    if ((e = jxs_launch_rate(L.frames, (int)L.nfr, L.total_slices, st)) != hipSuccess)
+    if ((e = jxs_launch_rate(L.frames, (int)L.nfr, L.total_slices, L.total_prec, L.rate_kinds, st)) != hipSuccess)
         return e;
     if (ev)
         (void)hipEventRecord(ev[3], st);

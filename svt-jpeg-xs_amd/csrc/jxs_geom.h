@@ -81,6 +81,10 @@ struct JxsGeom {
      * bytes of each line's packet; zero past the end */
     uint32_t rq[2][JXS_NU], rq_raw[2][JXS_NU];
     uint32_t rq_n[2];
+    /* the walk of a normal / the last precinct row cut into 2^lg runs of
+     * whole packets (lg 0..2): run k is walk entries [rq_seg[t][lg][k],
+     * rq_seg[t][lg][k + 1]) */
+    uint8_t rq_seg[2][3][8];
     uint32_t rc_mode; /* rate control: 0 budget per precinct, 1 unused padding moved to the next precinct,
                          2 / 3 budgets from one (Q, R) per slice (3: with a minimum precinct size) */
     uint32_t fs;      /* Fs: signs in sub-packets of their own */
@@ -119,6 +123,19 @@ static_assert(sizeof(JxsGeom) % 16 == 0, "JxsGeom size must be a multiple of 16 
 /* Per-precinct rate-control result. */
 struct JxsPrec {
     uint32_t q, r, total, pad, offset, err;
+};
+
+/* What the rate control of a precinct needs from its rate table, for the
+ * in-order pass over a slice (rate control modes 0 and 1): the precinct's
+ * bytes at (q, 0) for every q, at (q0, r) and (q0 - 1, r) for every r, where
+ * q0 is the Q its budget gets without a padding carry, the (Q, R) = (q0, r0)
+ * and bytes at that budget, and the smallest of the entries the two binary
+ * searches look at that exceeds that budget (any budget below it gets the
+ * same (Q, R); 0 when the searches failed there). */
+struct JxsRateRec {
+    uint32_t col[32];
+    uint32_t row[64];
+    uint32_t q0, r0, data0, thr;
 };
 
 /* Packet flags of the packet plan: the packet has a band line, and its
@@ -188,6 +205,7 @@ struct JxsFrameDesc {
     uint8_t* smax;
     uint16_t* hist;  /* rows x n_bl x JXS_HREC */
     JxsPrec* prec;   /* rows */
+    JxsRateRec* rrec; /* rows */
     JxsUnit* units;  /* packing units: unit u of precinct row r at u_rank[u] * rows + r */
     uint8_t* out;    /* codestream */
     uint32_t* status;
@@ -199,6 +217,7 @@ struct JxsFrameDesc {
     uint32_t n_units;
     uint32_t pps, slices, n_bl; /* copies of the geometry's, readable before it is staged */
     uint32_t ordered;           /* coded by k_enc_slice (vertical prediction or sign coding), not k_rate / k_pack */
+    uint32_t slice_rc;          /* rate control modes 2 / 3: one (Q, R) per slice */
 };
 
 #endif

@@ -258,6 +258,49 @@ static int unit_words(JxsGeom& g) {
             n++;
         }
         g.rq_n[t] = n;
+        /* the walk cut into 1, 2 and 4 runs of whole packets whose longest
+         * run is as short as it can be: several waves then walk one
+         * precinct's lines in parallel */
+        uint32_t runs[JXS_NU], nr = 0, len = 0;
+        for (uint32_t k = 0; k < n; k++) {
+            len++;
+            if ((g.rq[t][k] >> RQ_END) & 1u) {
+                runs[nr++] = len;
+                len = 0;
+            }
+        }
+        for (uint32_t lg = 0; lg < 3; lg++) {
+            const uint32_t parts = 1u << lg;
+            uint32_t cap = 0;
+            for (uint32_t k = 0; k < nr; k++)
+                cap = runs[k] > cap ? runs[k] : cap;
+            for (;; cap++) { /* the smallest cap that needs at most `parts` runs */
+                uint32_t used = 1, cur = 0;
+                for (uint32_t k = 0; k < nr; k++) {
+                    if (cur + runs[k] > cap) {
+                        used++;
+                        cur = 0;
+                    }
+                    cur += runs[k];
+                }
+                if (used <= parts || cap >= n)
+                    break;
+            }
+            uint32_t part = 0, cur = 0, pos = 0;
+            g.rq_seg[t][lg][0] = 0;
+            for (uint32_t k = 0; k < nr; k++) {
+                if (cur + runs[k] > cap) {
+                    g.rq_seg[t][lg][++part] = (uint8_t)pos;
+                    cur = 0;
+                }
+                cur += runs[k];
+                pos += runs[k];
+            }
+            while (part < parts)
+                g.rq_seg[t][lg][++part] = (uint8_t)n;
+            for (uint32_t k = parts + 1; k < 8; k++)
+                g.rq_seg[t][lg][k] = (uint8_t)n;
+        }
     }
     for (uint32_t c = 0; c < g.nc; c++)
         for (uint32_t b = 0; b < g.c[c].nb; b++) {

@@ -857,26 +857,27 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(CH == 3
         dwt_slots<JXS_SRC_I32, CH>(J, bi, L);
 }
 
-/* Copies a frame's geometry into LDS (all GEOM_THREADS threads of the block
- * take part; the caller synchronises).  The rate-control kernel indexes it
- * with data-dependent band and packet numbers. */
+/* Copies a frame's geometry into LDS (all NTH threads of the block take
+ * part; the caller synchronises).  The rate-control kernel indexes it with
+ * data-dependent band and packet numbers. */
 #define GEOM_THREADS 256 /* block size of the kernels that stage the geometry */
+template <uint32_t NTH = GEOM_THREADS>
 __device__ __forceinline__ void stage_geom(JxsGeom* dst, const JxsGeom* src) {
     const u32x4* s = (const u32x4*)src; /* read through global, not flat, loads */
     u32x4* d = (u32x4*)dst;
     constexpr uint32_t n = sizeof(JxsGeom) / 16;
-    constexpr uint32_t per = (n + GEOM_THREADS - 1) / GEOM_THREADS; /* copies per thread, loaded together */
+    constexpr uint32_t per = (n + NTH - 1) / NTH; /* copies per thread, loaded together */
     const uint32_t t = threadIdx.x;
     /* indexes clamped into the struct: every load is in bounds, only the
      * stores are predicated */
     u32x4 v[per];
 #pragma unroll
     for (uint32_t j = 0; j < per; j++)
-        v[j] = gld(s + min(t + j * GEOM_THREADS, n - 1));
+        v[j] = gld(s + min(t + j * NTH, n - 1));
 #pragma unroll
     for (uint32_t j = 0; j < per; j++)
-        if (t + j * GEOM_THREADS < n)
-            d[t + j * GEOM_THREADS] = v[j];
+        if (t + j * NTH < n)
+            d[t + j * NTH] = v[j];
 }
 
 /* ------------------------------------------------------------------ */
@@ -891,6 +892,28 @@ __device__ __forceinline__ void stage_geom(JxsGeom* dst, const JxsGeom* src) {
  * unit. */
 __device__ __forceinline__ bool bsearch_masks(uint32_t m1, uint32_t m3, int32_t end, bool below, int32_t step0,
                                               uint32_t& out) {
+    /* The common case in closed form: "out of range" only at the top of
+     * [0, end] (from position a on) and "too small" exactly at the first k
+     * positions below a.  The search then ends at the first "too big"
+     * position (k, if k < a) or at the last "too small" one (k - 1, if
+     * k > 0), whatever the first probe; the replay below agrees with this
+     * for every end <= 31, a and k (checked exhaustively). */
+    {
+        const uint32_t full = end >= 31 ? ~0u : ((1u << (end + 1)) - 1);
+        const uint32_t m3f = m3 & full;
+        const uint32_t a = m3f ? (uint32_t)__builtin_ctz(m3f) : (uint32_t)end + 1;
+        const uint32_t valid = a >= 32 ? ~0u : ((1u << a) - 1);
+        const uint32_t sm = m1 & valid;
+        if ((m3f | valid) == full && (sm & (sm + 1)) == 0) {
+            const uint32_t k = (uint32_t)__builtin_popcount(sm);
+            if (below) {
+                out = k - 1;
+                return k > 0;
+            }
+            out = k < a ? k : ~0u;
+            return k < a;
+        }
+    }
     int32_t beg = 0, best = -1;
     int32_t step = step0 ? step0 : end / 2;
     if (step > end - beg)
@@ -958,7 +981,8 @@ __device__ __forceinline__ uint32_t uni(uint32_t v) { return (uint32_t)__builtin
 /* ------------------------------------------------------------------ */
 #define RT_STRIDE JXS_RT_STRIDE
 #define RT_ZERO 16   /* the zero entry, read for band lines absent from the precinct */
-#define RT_WAVES 4   /* waves of a k_rate block = precincts whose rate tables are held at once */
+#define RT_NW 16     /* waves of a k_rate block */
+#define RT_NP 8      /* precinct rate tables a k_rate block holds at once */
 
 /* Rate table of one precinct, built by one wave from the transform's
  * band-line histogram records: for every band line and truncation t, the
@@ -986,19 +1010,20 @@ __device__ __forceinline__ Rec load_rec(const uint16_t* recs, uint32_t line) {
     return r;
 }
 
-/* Rate-table entries of band line `lane` (one lane per band line; every lane
- * of the wave takes part, lanes past the last line store nothing).  With h(v)
- * the groups whose GCLI is v, at truncation t the unary bit-plane counts
- * take sum_{v<=t} h(v) + sum_{v>t} (v - t) h(v) bits, the data
- * sum_{v>t} 4 (v - t + 1) h(v) bits (v - t magnitude bit-planes and a sign
- * per coefficient), and significance coding removes the groups of the
+/* Rate-table entries t in [T0, T0 + NT) of band line `lane` (one lane per
+ * band line; every lane of the wave takes part, lanes past the last line
+ * store nothing).  With h(v) the groups whose GCLI is v, at truncation t the
+ * unary bit-plane counts take sum_{v<=t} h(v) + sum_{v>t} (v - t) h(v) bits,
+ * the data sum_{v>t} 4 (v - t + 1) h(v) bits (v - t magnitude bit-planes and
+ * a sign per coefficient), and significance coding removes the groups of the
  * significance groups of 8 whose maximum GCLI is at most t (and of the
  * trailing partial group) at the cost of one flag per significance group;
- * all are running sums over t.  The coding method compares the two counts
- * over the band's lines in this precinct (the other line of a two-line band
- * is the neighbouring lane). */
-__device__ __forceinline__ void rate_lines(RateTab& T, const Rec& rc, uint32_t n_bl, uint32_t li, uint32_t type,
-                                           uint32_t sig_on, uint32_t lane) {
+ * all are running sums over t, started from their sums below T0.  The
+ * coding method compares the two counts over the band's lines in this
+ * precinct (the other line of a two-line band is the neighbouring lane). */
+template <uint32_t T0, uint32_t NT>
+__device__ __forceinline__ void rate_lines_part(RateTab& T, const Rec& rc, uint32_t n_bl, uint32_t li, uint32_t type,
+                                                uint32_t sig_on, uint32_t lane) {
     const uint32_t w[16] = {rc.a.x, rc.a.y, rc.a.z, rc.a.w, rc.b.x, rc.b.y, rc.b.z, rc.b.w,
                             rc.c.x, rc.c.y, rc.c.z, rc.c.w, rc.e.x, rc.e.y, rc.e.z, rc.e.w};
     uint32_t tot = 0, vtot = 0;
@@ -1013,62 +1038,91 @@ __device__ __forceinline__ void rate_lines(RateTab& T, const Rec& rc, uint32_t n
     const bool two = nl == 2;
     const int partner = (int)(two ? (((li >> LI_SECOND) & 1u) ? lane - 1 : lane + 1) : lane);
     const uint32_t tail = w[15] >> 16; /* 0xFFFF: no partial group */
-    uint32_t data[16], ns[16], zr[16];
     uint32_t cnt = 0, vcnt = 0, sig = 0;
 #pragma unroll
-    for (uint32_t t = 0; t < 16; t++) {
+    for (uint32_t v = 0; v < T0; v++) { /* T0 <= 12: every v < 15 */
+        const uint32_t h = (w[v >> 1] >> (16 * (v & 1))) & 0xFFFFu;
+        cnt += h;
+        vcnt += v * h;
+        sig += (w[8 + (v >> 1)] >> (16 * (v & 1))) & 0xFFFFu;
+    }
+    uint32_t data[NT], ns[NT], zr[NT];
+#pragma unroll
+    for (uint32_t j = 0; j < NT; j++) {
+        const uint32_t t = T0 + j;
         const uint32_t h = (w[t >> 1] >> (16 * (t & 1))) & 0xFFFFu;
         cnt += h;
         vcnt += t * h;
         const uint32_t above = tot - cnt, vabove = vtot - vcnt;
-        data[t] = vabove + above - t * above; /* sum_{v>t} (v - t + 1) h(v) */
+        data[j] = vabove + above - t * above; /* sum_{v>t} (v - t + 1) h(v) */
         if (t < 15)
             sig += (w[8 + (t >> 1)] >> (16 * (t & 1))) & 0xFFFFu;
         else
             sig = full;
-        ns[t] = data[t] + cnt;
-        zr[t] = 8 * sig + (tail <= t ? part : 0u);
+        ns[j] = data[j] + cnt;
+        zr[j] = 8 * sig + (tail <= t ? part : 0u);
     }
     /* the other line's counts, all exchanged before any is used */
-    uint32_t pn[16], pz[16];
+    uint32_t pn[NT], pz[NT];
 #pragma unroll
-    for (uint32_t t = 0; t < 16; t++) {
-        pn[t] = (uint32_t)__shfl((int)ns[t], partner);
-        pz[t] = (uint32_t)__shfl((int)zr[t], partner);
+    for (uint32_t j = 0; j < NT; j++) {
+        pn[j] = (uint32_t)__shfl((int)ns[j], partner);
+        pz[j] = (uint32_t)__shfl((int)zr[j], partner);
     }
     if (lane < n_bl) {
-        const uint32_t base = lane * RT_STRIDE;
+        const uint32_t base = lane * RT_STRIDE + T0;
 #pragma unroll
-        for (uint32_t t = 0; t < 16; t++) {
-            const uint32_t nosig = ns[t] + (two ? pn[t] : 0u);
-            const uint32_t withsig = sw * nl + (ns[t] - zr[t]) + (two ? pn[t] - pz[t] : 0u);
+        for (uint32_t j = 0; j < NT; j++) {
+            const uint32_t nosig = ns[j] + (two ? pn[j] : 0u);
+            const uint32_t withsig = sw * nl + (ns[j] - zr[j]) + (two ? pn[j] - pz[j] : 0u);
             const bool use = sig_on && nosig > withsig;
-            T.d[base + t] = 4 * data[t];
-            T.g[base + t] = use ? ns[t] - zr[t] : ns[t];
-            T.s[base + t] = use ? sw : 0u;
+            T.d[base + j] = 4 * data[j];
+            T.g[base + j] = use ? ns[j] - zr[j] : ns[j];
+            T.s[base + j] = use ? sw : 0u;
         }
-        T.d[base + RT_ZERO] = 0;
-        T.g[base + RT_ZERO] = 0;
-        T.s[base + RT_ZERO] = 0;
+        if (T0 == 0) {
+            T.d[lane * RT_STRIDE + RT_ZERO] = 0;
+            T.g[lane * RT_STRIDE + RT_ZERO] = 0;
+            T.s[lane * RT_STRIDE + RT_ZERO] = 0;
+        }
     }
 }
 
-/* Bytes of a precinct's packets at (q, r) (precinct_get_budget_bytes,
- * RateControl.c:700-781): per packet the data bytes plus the smaller of the
- * significance + bit-plane-count bytes and the packet's RAW size.  Each lane
- * answers its own (q, r).  The band lines present in the precinct are
- * walked in packet order from registers: lane k of rq holds the k-th one's
- * walk word, of rw its packet's RAW bytes.  The walk takes 8 lines per step,
- * their 24 table reads issued together; the words past the last line carry
- * no packet end, so what they add is never counted. */
-__device__ __forceinline__ uint32_t rate_query(uint32_t rq, uint32_t rw, uint32_t n, const RateTab& T, uint32_t q,
-                                               uint32_t r) {
+/* The share `sub` (of 2^lg, a wave each) of a precinct's rate table: one,
+ * two or all four quarters of the truncations, a quarter at a time (which
+ * keeps few values live). */
+__device__ __forceinline__ void rate_lines(RateTab& T, const Rec& rc, uint32_t n_bl, uint32_t li, uint32_t type,
+                                           uint32_t sig_on, uint32_t lane, uint32_t lg, uint32_t sub) {
+    const uint32_t nq = 4u >> lg;
+    for (uint32_t k = 0; k < nq; k++) {
+        const uint32_t qi = sub * nq + k;
+        if (qi == 0)
+            rate_lines_part<0, 4>(T, rc, n_bl, li, type, sig_on, lane);
+        else if (qi == 1)
+            rate_lines_part<4, 4>(T, rc, n_bl, li, type, sig_on, lane);
+        else if (qi == 2)
+            rate_lines_part<8, 4>(T, rc, n_bl, li, type, sig_on, lane);
+        else
+            rate_lines_part<12, 4>(T, rc, n_bl, li, type, sig_on, lane);
+    }
+}
+
+/* Bytes of the packets of walk entries [lo, hi) of a precinct at (q, r)
+ * (precinct_get_budget_bytes, RateControl.c:700-781; the entries are whole
+ * packets): per packet the data bytes plus the smaller of the significance
+ * + bit-plane-count bytes and the packet's RAW size.  Each lane answers its
+ * own (q, r).  The band lines are walked in packet order from registers:
+ * lane k of rq holds the k-th one's walk word, of rw its packet's RAW bytes.
+ * The walk takes 8 lines per step, their 24 table reads issued together;
+ * entries at or past hi are read (in bounds) but not counted. */
+__device__ __forceinline__ uint32_t rate_query_seg(uint32_t rq, uint32_t rw, uint32_t lo, uint32_t hi,
+                                                   const RateTab& T, uint32_t q, uint32_t r) {
     uint32_t total = 0, d = 0, gb = 0, s = 0;
-    for (uint32_t k0 = 0; k0 < n; k0 += 8) { /* k0 + 7 < 64: n <= JXS_NU */
+    for (uint32_t k0 = lo; k0 < hi; k0 += 8) {
         uint32_t vd[8], vg[8], vs[8];
 #pragma unroll
         for (uint32_t kk = 0; kk < 8; kk++) {
-            const uint32_t x = lane_of(rq, k0 + kk);
+            const uint32_t x = lane_of(rq, min(k0 + kk, (uint32_t)(WAVE - 1)));
             const int32_t t = (int32_t)q - (int32_t)((x >> RQ_GAIN) & 0xFFu) - (((x >> RQ_PRIO) & 0xFFu) < r ? 1 : 0);
             const uint32_t o = (x & 0x7FFu) + (uint32_t)min(max(t, 0), 15);
             vd[kk] = T.d[o];
@@ -1077,17 +1131,25 @@ __device__ __forceinline__ uint32_t rate_query(uint32_t rq, uint32_t rw, uint32_
         }
 #pragma unroll
         for (uint32_t kk = 0; kk < 8; kk++) {
-            const uint32_t x = lane_of(rq, k0 + kk);
-            d += vd[kk];
-            gb += vg[kk];
-            s += vs[kk];
-            if ((x >> RQ_END) & 1u) {
-                total += ((d + 7) >> 3) + min(((s + 7) >> 3) + ((gb + 7) >> 3), lane_of(rw, k0 + kk));
-                d = gb = s = 0;
+            if (k0 + kk < hi) {
+                const uint32_t x = lane_of(rq, k0 + kk);
+                d += vd[kk];
+                gb += vg[kk];
+                s += vs[kk];
+                if ((x >> RQ_END) & 1u) {
+                    total += ((d + 7) >> 3) + min(((s + 7) >> 3) + ((gb + 7) >> 3), lane_of(rw, k0 + kk));
+                    d = gb = s = 0;
+                }
             }
         }
     }
     return total;
+}
+
+/* The whole walk of n entries. */
+__device__ __forceinline__ uint32_t rate_query(uint32_t rq, uint32_t rw, uint32_t n, const RateTab& T, uint32_t q,
+                                               uint32_t r) {
+    return rate_query_seg(rq, rw, 0, n, T, q, r);
 }
 
 /* The binary search of Q (RateControl.c:889-1083): lane q of cq holds the
@@ -1124,7 +1186,9 @@ __device__ __forceinline__ uint32_t seg_before(uint32_t incl, uint32_t v, uint32
 }
 
 /* Packet plan of one precinct (one wave, one lane per packing unit), once
- * its (Q, R) is chosen: truncation and coding method of every band, the
+ * its (Q, R) is chosen and each unit's significance, bit-plane-count and
+ * data bits at its band's truncation are known (vs, vg, vd; zeros for a
+ * line absent from the precinct): truncation and coding method of every band, the
  * sub-packet sizes, RAW choice and codestream offset of every packet
  * (PackPrecinct.c:235-572, RateControl.c:457-781), the packing units, and the
  * bytes around the packets: slice header (PackHeaders.c:142-156), precinct
@@ -1134,8 +1198,8 @@ __device__ __forceinline__ uint32_t seg_before(uint32_t incl, uint32_t v, uint32
  * would not add up to the size the rate control chose is flagged and nothing
  * of it is written. */
 __device__ __forceinline__ void plan_precinct(const JxsFrameDesc& F, const JxsGeom& g, uint32_t row, uint32_t type,
-                                              const JxsPrec& P, const RateTab& T, uint32_t* ms_band, uint32_t ui,
-                                              uint32_t lane) {
+                                              const JxsPrec& P, uint32_t vs, uint32_t vg, uint32_t vd, uint32_t* ms_band,
+                                              uint32_t ui, uint32_t lane) {
     const uint32_t nu = g.n_units;
     const bool unit = lane < nu;
     /* this lane's unit record: unit-major order, widest lines first */
@@ -1151,8 +1215,6 @@ __device__ __forceinline__ void plan_precinct(const JxsFrameDesc& F, const JxsGe
     const uint32_t first = lane - j, last = unit ? first + (g.pk_be[p] - g.pk_bs[p]) - 1 : lane;
     const bool pres = unit && ((ui >> (UI_NORMAL + type)) & 1u);
     const uint32_t gt = truncation((ui >> 8) & 0xFFu, (ui >> 16) & 0xFFu, P.q, P.r);
-    const uint32_t o = (ui & 0xFFu) * RT_STRIDE + (pres ? gt : RT_ZERO);
-    const uint32_t vs = T.s[o], vg = T.g[o], vd = T.d[o];
     const uint32_t ms = vs != 0 ? 1u : 0u;
     if (unit && l == 0)
         ms_band[gi] = ms;
@@ -1313,6 +1375,18 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
         v += (uint32_t)__shfl_xor((int)v, o, WAVE);
     return v;
 }
+/* v, as a value the compiler cannot see through (nothing computed from it
+ * is hoisted out of a loop). */
+__device__ __forceinline__ uint32_t opaque_v(uint32_t v) {
+    asm volatile("" : "+v"(v));
+    return v;
+}
+__device__ __forceinline__ uint32_t wave_min(uint32_t v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1)
+        v = min(v, (uint32_t)__shfl_xor((int)v, o, WAVE));
+    return v;
+}
 
 /* Precinct budgets of a slice under rate control modes 2 and 3
  * (PackStageProcess.c:340-481), by wave 0 with one lane per precinct, from
@@ -1360,25 +1434,39 @@ __device__ void slice_budgets(const JxsFrameDesc& F, uint32_t first_row, uint32_
     }
 }
 
-/* One block per slice (RT_WAVES waves), the rate control and packet plan of
- * its precincts, RT_WAVES precincts at a time:
- *   1. each wave builds one precinct's rate table from the histogram records,
- *      the precinct's bytes at (q, 0) for every q, and the (q, r) rows at the
- *      two Q its budget most likely gets;
- *   2. wave 0 walks the precincts in order: budgets with the +10 % first
- *      precinct and the padding carry (PackStageProcess.c:600-684), the
- *      reference's binary search of Q on the (q, 0) column, then of R on the
- *      (Q, r) row, computed now if the carry moved Q elsewhere
- *      (RateControl.c:889-1083, BinarySearch.c); only these entries of the
- *      (Q, R) table are ever read;
- *   3. each wave plans its precinct. */
-__global__ void __launch_bounds__(WAVE * RT_WAVES) k_rate(const JxsFrameDesc* __restrict__ frames, int nfr,
-                                                         uint32_t total_slices) {
+/* Rate control modes 2 and 3 (one (Q, R) per slice sizing the precinct
+ * budgets): one block per slice (RT_NW waves), the rate control and packet
+ * plan of its precincts, 2^lg waves per precinct (4 while a slice has at
+ * most 4 precincts, 2 up to 8, else 1) and up to RT_NP precincts at a time:
+ *   1. the waves of a precinct build its rate table from the histogram
+ *      records, each a share of the truncations;
+ *   2. the slice's Q, then R, by the per-precinct binary searches on the
+ *      slice's summed bytes, each wave walking a run of whole packets of its
+ *      precinct, then every precinct's budget (PackStageProcess.c:340-481);
+ *   3. per precinct, the bytes at (q, 0) for every q and the (q, r) rows at
+ *      the Q its budget gets without a padding carry and at that Q - 1 (a
+ *      carry only ever adds bytes, rarely more than one step of Q's worth);
+ *   4. the first wave of each precinct runs the reference's binary searches
+ *      (RateControl.c:889-1083, BinarySearch.c) at that budget and notes the
+ *      smallest column or row entry above it: a search's outcome depends on
+ *      the budget only through which entries exceed it, so any budget up to
+ *      that entry gets the same (Q, R);
+ *   5. wave 0 walks the precincts in order with the padding carry
+ *      (PackStageProcess.c:600-684), taking the noted (Q, R) while the
+ *      carried budget stays below the noted entry, else searching again on
+ *      the column and rows (a row computed now if the carry moved Q
+ *      further);
+ *   6. the first wave of each precinct plans it. */
+__global__ void __launch_bounds__(WAVE * RT_NW) k_rate(const JxsFrameDesc* __restrict__ frames, int nfr,
+                                                      uint32_t total_slices) {
     __shared__ __align__(16) JxsGeom sg;
-    __shared__ RateTab tabs[RT_WAVES];
-    __shared__ uint32_t col[RT_WAVES][WAVE], rowq[RT_WAVES][WAVE], spec[RT_WAVES];
-    __shared__ JxsPrec ps[RT_WAVES];
-    __shared__ uint32_t ms_band[RT_WAVES][JXS_NGB];
+    __shared__ RateTab tabs[RT_NP];
+    __shared__ uint32_t part[RT_NW][WAVE];
+    __shared__ uint32_t col[RT_NP][WAVE], rowq[RT_NP][WAVE];
+    __shared__ uint32_t spec[RT_NP], sp_r[RT_NP], sp_data[RT_NP], sp_thr[RT_NP], slice_qr;
+    __shared__ JxsPrec ps[RT_NP];
+    __shared__ uint32_t ms_band[RT_NP][JXS_NGB];
+    __shared__ u32x4 recs[RT_NP][4][WAVE]; /* the first precincts' histogram records, quarter k of line l at [k][l] */
     const uint32_t gs = blockIdx.x;
     if (gs >= total_slices)
         return;
@@ -1392,48 +1480,69 @@ __global__ void __launch_bounds__(WAVE * RT_WAVES) k_rate(const JxsFrameDesc* __
     }
     /* the frame descriptor in scalar registers */
     const JxsFrameDesc F = cld_words(frames + fl);
-    if (F.ordered)
-        return; /* coded by k_enc_slice */
+    if (F.ordered || !F.slice_rc)
+        return; /* coded by k_enc_slice, or by k_rate_tab / k_rate_plan */
     /* the wave index is made visibly uniform: everything a wave derives from
      * it (its precinct, the searches, the plan) then stays in scalar
      * registers and scalar branches */
     const uint32_t tid = threadIdx.x, lane = tid & (WAVE - 1);
     const uint32_t wid = (uint32_t)__builtin_amdgcn_readfirstlane((int)(tid / WAVE));
-    /* the first precincts' records are loaded while the geometry is staged */
     const uint32_t pps = F.pps, rows = F.rows, n_bl = F.n_bl;
     const uint32_t s = gs - F.slice_base;
     const uint32_t first_row = s * pps;
     const uint32_t nprec = (s == F.slices - 1) ? rows - first_row : pps;
+    const uint32_t lg = pps <= 4 ? 2u : pps <= 8 ? 1u : 0u, wpp = 1u << lg;
+    const uint32_t np = min((uint32_t)RT_NW >> lg, (uint32_t)RT_NP); /* precincts at a time */
+    const uint32_t slot = wid >> lg, sub = wid & (wpp - 1);
+    const bool active = slot < np;
+    /* the first precincts' records are staged in LDS with the geometry:
+     * the waves of a precinct load a share of each line's record */
     const uint32_t rec_line = min(lane, n_bl - 1);
-    Rec rc = load_rec(F.hist + (size_t)min(first_row + wid, rows - 1) * n_bl * JXS_HREC, rec_line);
-    stage_geom(&sg, F.g);
+    {
+        const u32x4* src = (const u32x4*)(F.hist + ((size_t)min(first_row + slot, rows - 1) * n_bl + rec_line) * JXS_HREC);
+        const uint32_t nk = 4u >> lg;
+        u32x4 v[4];
+#pragma unroll
+        for (uint32_t k = 0; k < 4; k++)
+            if (active && k < nk)
+                v[k] = gld(src + sub * nk + k);
+        stage_geom<WAVE * RT_NW>(&sg, F.g);
+#pragma unroll
+        for (uint32_t k = 0; k < 4; k++)
+            if (active && k < nk)
+                recs[slot][sub * nk + k][lane] = v[k];
+    }
     __syncthreads();
     const JxsGeom& g = sg;
-    const uint32_t ui = g.u_info[lane], li = g.ln_info[lane];
-    const uint32_t rq0 = g.rq[0][lane], rq1 = g.rq[1][lane], rw0 = g.rq_raw[0][lane], rw1 = g.rq_raw[1][lane];
-    const uint32_t em = g.empty_mask[lane & (JXS_NQ - 1)], maxq = uni(g.maxq), maxr = uni(g.maxr);
-    const uint32_t eq = (uint32_t)__ballot(em & 1u); /* bit q: (q, 0) truncates every band away */
+    /* per-lane geometry words are read from LDS where they are used, which
+     * keeps them out of the registers live across the kernel */
+    const uint32_t maxq = uni(g.maxq), maxr = uni(g.maxr);
+    /* bit q: (q, 0) truncates every band away */
+    const uint32_t eq = (uint32_t)__ballot(g.empty_mask[lane & (JXS_NQ - 1)] & 1u);
     const uint32_t sbud = gld(F.slice_bud + s); /* global, not flat: a uniform value */
-    /* mode 1: the first precinct 10 % more, the rest split evenly
-     * (PackStageProcess.c:621-640); mode 0: an even split (600-601); modes 2
-     * and 3: budgets from one (Q, R) for the slice, below */
     const uint32_t rc_mode = uni(g.rc_mode);
-    const bool move_pad = rc_mode != 0;
-    uint32_t minb = sbud / nprec;
-    uint32_t first = minb, left = sbud - minb * nprec;
-    if (rc_mode == 1) {
-        first = minb * 110 / 100;
-        if (first > sbud)
-            first = sbud;
-        left = 0;
-        if (nprec > 1) {
-            minb = (sbud - first) / (nprec - 1);
-            left = sbud - minb * (nprec - 1) - first;
-        }
-    }
     uint32_t carry = 0, off = gld(F.slice_off + s) + 6;
     bool failed = false;
-    if (rc_mode >= 2) {
+    /* all of the slice's tables fit at once: built a single time */
+    const bool single = nprec <= np;
+    bool built = false;
+    /* the tables of precincts i0 .. i0 + np - 1 (the caller synchronises) */
+    auto build = [&](uint32_t i0) {
+        if (active && i0 + slot < nprec) {
+            const uint32_t row = first_row + i0 + slot, type = row + 1 >= rows ? 1 : 0;
+            Rec rp;
+            if (i0 == 0) {
+                rp.a = recs[slot][0][lane];
+                rp.b = recs[slot][1][lane];
+                rp.c = recs[slot][2][lane];
+                rp.e = recs[slot][3][lane];
+            } else {
+                rp = load_rec(F.hist + (size_t)row * n_bl * JXS_HREC, rec_line);
+            }
+            rate_lines(tabs[slot], rp, n_bl, g.ln_info[lane], type, g.sig, lane, lg, sub);
+        }
+    };
+    {
         /* one Q, then one R, for the whole slice by the per-precinct binary
          * searches on the slice's summed bytes (RateControl.c:1336-1425):
          * pass 0 sums the (q, 0) columns, pass 1 the (Q, r) rows, pass 2
@@ -1445,31 +1554,33 @@ __global__ void __launch_bounds__(WAVE * RT_WAVES) k_rate(const JxsFrameDesc* __
         uint32_t Q = 0, R = 0;
         for (uint32_t pass = 0; pass < 3 && ok; pass++) {
             uint32_t acc = 0, dmax = 0;
-            for (uint32_t i0 = 0; i0 < nprec; i0 += RT_WAVES) {
-                const uint32_t n = min((uint32_t)RT_WAVES, nprec - i0);
-                if (wid < n) {
-                    const uint32_t i = i0 + wid, row = first_row + i, type = row + 1 >= rows ? 1 : 0;
-                    const Rec rp = i0 == 0 ? rc : load_rec(F.hist + (size_t)row * n_bl * JXS_HREC, rec_line);
-                    RateTab& T = tabs[wid];
-                    rate_lines(T, rp, n_bl, li, type, g.sig, lane);
-                    wave_sync();
-                    const uint32_t rq = type ? rq1 : rq0, rw = type ? rw1 : rw0, nl = g.rq_n[type];
-                    uint32_t v;
-                    if (pass == 0) {
-                        v = rate_query(rq, rw, nl, T, lane & (JXS_NQ - 1), 0);
-                    } else if (pass == 1) {
-                        v = rate_query(rq, rw, nl, T, Q, lane & (JXS_NR - 1));
-                    } else {
-                        v = uni(rate_query(rq, rw, nl, T, Q, R)) + uni(g.rc_hdr_bytes[type]);
-                        if (lane == 0)
-                            gst(&F.prec[row].total, v);
-                    }
-                    col[wid][lane] = v;
+            for (uint32_t i0 = 0; i0 < nprec; i0 += np) {
+                const uint32_t n = min(np, nprec - i0);
+                if (!(single && built)) {
+                    build(i0);
+                    built = true;
+                    __syncthreads();
+                }
+                if (active && slot < n) {
+                    const uint32_t row = first_row + i0 + slot, type = row + 1 >= rows ? 1 : 0;
+                    const uint32_t rq = g.rq[type][lane], rw = g.rq_raw[type][lane];
+                    const uint32_t lo = g.rq_seg[type][lg][sub], hi = g.rq_seg[type][lg][sub + 1];
+                    const uint32_t q = pass == 0 ? (lane & (JXS_NQ - 1)) : Q;
+                    const uint32_t r = pass == 0 ? 0u : pass == 1 ? (lane & (JXS_NR - 1)) : R;
+                    part[wid][lane] = rate_query_seg(rq, rw, lo, hi, tabs[slot], q, r);
                 }
                 __syncthreads();
                 if (wid == 0)
                     for (uint32_t k = 0; k < n; k++) {
-                        const uint32_t v = col[k][lane];
+                        uint32_t v = 0;
+                        for (uint32_t u = 0; u < wpp; u++)
+                            v += part[(k << lg) + u][lane];
+                        if (pass == 2) {
+                            const uint32_t row = first_row + i0 + k, type = row + 1 >= rows ? 1 : 0;
+                            v = uni(v) + uni(g.rc_hdr_bytes[type]);
+                            if (lane == 0)
+                                gst(&F.prec[row].total, v);
+                        }
                         acc += v;
                         dmax = max(dmax, v);
                     }
@@ -1479,16 +1590,16 @@ __global__ void __launch_bounds__(WAVE * RT_WAVES) k_rate(const JxsFrameDesc* __
                 if (pass == 0) {
                     ok = search_q(maxq, eq, acc, avail, Q);
                 } else if (pass == 1) {
-                    ok = search_r(maxr, lane_of(em, Q), acc, 0, avail, R);
+                    ok = search_r(maxr, g.empty_mask[Q], acc, 0, avail, R);
                 } else {
                     /* acc: every lane the slice's headers + data bytes */
                     slice_budgets(F, first_row, nprec, rc_mode, avail + hsum - uni(acc), uni(dmax), lane);
                 }
                 if (lane == 0)
-                    spec[0] = (ok ? 1u : 0u) | Q << 8 | R << 16;
+                    slice_qr = (ok ? 1u : 0u) | Q << 8 | R << 16;
             }
             __syncthreads();
-            const uint32_t sp = uni(spec[0]);
+            const uint32_t sp = uni(slice_qr);
             ok = (sp & 1u) != 0;
             Q = (sp >> 8) & 0xFFu;
             R = (sp >> 16) & 0xFFu;
@@ -1500,77 +1611,124 @@ __global__ void __launch_bounds__(WAVE * RT_WAVES) k_rate(const JxsFrameDesc* __
                 gor(F.status, 1u);
         }
     }
-    for (uint32_t i0 = 0; i0 < nprec; i0 += RT_WAVES) {
-        const uint32_t n = min((uint32_t)RT_WAVES, nprec - i0);
-        if (wid < n) {
-            const uint32_t i = i0 + wid, row = first_row + i, type = row + 1 >= rows ? 1 : 0;
-            if (i0 > 0)
-                rc = load_rec(F.hist + (size_t)row * n_bl * JXS_HREC, rec_line);
-            RateTab& T = tabs[wid];
-            rate_lines(T, rc, n_bl, li, type, g.sig, lane);
-            wave_sync();
-            const uint32_t rq = type ? rq1 : rq0, rw = type ? rw1 : rw0, nl = g.rq_n[type];
-            /* lane q (and q + 32): bytes at (q, 0) */
-            const uint32_t cq = rate_query(rq, rw, nl, T, lane & (JXS_NQ - 1), 0);
-            /* the (q, r) rows at the Q this precinct gets without a padding
-             * carry (lanes 0..31: lane r) and at that Q - 1 (lanes 32..63):
-             * a carry only ever adds bytes, and rarely more than one step of
-             * Q's worth */
-            const uint32_t hdr = g.rc_hdr_bytes[type];
-            const uint32_t b0 = rc_mode >= 2 ? uni(gld(&F.prec[row].total)) : (i == 0 ? first : minb) + (i < left ? 1 : 0);
-            uint32_t q0 = 0;
-            if (!(b0 <= 1048575u && b0 > hdr && search_q(maxq, eq, cq, b0 - hdr, q0)))
+    const uint32_t lane_k = lane;
+    for (uint32_t i0 = 0; i0 < nprec; i0 += np) {
+        /* the lane index made opaque each time round: addresses derived from
+         * it are formed where used, not kept live across the loop */
+        const uint32_t lane = opaque_v(lane_k);
+        const uint32_t n = min(np, nprec - i0);
+        if (!(single && built)) {
+            build(i0);
+            built = true;
+            __syncthreads();
+        }
+        const bool mine = active && slot < n;
+        const uint32_t i = i0 + slot, row = first_row + i, type = row + 1 >= rows ? 1 : 0;
+        const uint32_t lo = mine ? uni(g.rq_seg[type][lg][sub]) : 0u, hi = mine ? uni(g.rq_seg[type][lg][sub + 1]) : 0u;
+        /* lane q (and q + 32): the run's bytes at (q, 0) */
+        if (mine)
+            part[wid][lane] = rate_query_seg(g.rq[type][lane], g.rq_raw[type][lane], lo, hi, tabs[slot],
+                                             lane & (JXS_NQ - 1), 0);
+        __syncthreads();
+        const uint32_t hdr = uni(g.rc_hdr_bytes[type]);
+        uint32_t cq = 0, q0 = 0, b0 = 0;
+        bool ok0 = false;
+        if (mine) {
+            for (uint32_t u = 0; u < wpp; u++)
+                cq += part[(slot << lg) + u][lane];
+            b0 = uni(gld(&F.prec[row].total));
+            ok0 = b0 <= 1048575u && b0 > hdr && search_q(maxq, eq, cq, b0 - hdr, q0);
+            if (!ok0)
                 q0 = 0;
+        }
+        __syncthreads();
+        /* the (q, r) rows at that Q (lanes 0..31: lane r) and at Q - 1
+         * (lanes 32..63) */
+        if (mine) {
             const uint32_t qc = lane < JXS_NR ? q0 : (q0 > 0 ? q0 - 1 : 0);
-            const uint32_t cr = rate_query(rq, rw, nl, T, qc, lane & (JXS_NR - 1));
-            col[wid][lane] = cq;
-            rowq[wid][lane] = cr;
-            if (lane == 0)
-                spec[wid] = q0;
+            part[wid][lane] = rate_query_seg(g.rq[type][lane], g.rq_raw[type][lane], lo, hi, tabs[slot], qc,
+                                             lane & (JXS_NR - 1));
+            if (sub == 0) {
+                col[slot][lane] = cq;
+                if (lane == 0)
+                    spec[slot] = q0;
+            }
+        }
+        __syncthreads();
+        if (mine && sub == 0) {
+            uint32_t cr = 0;
+            for (uint32_t u = 0; u < wpp; u++)
+                cr += part[(slot << lg) + u][lane];
+            rowq[slot][lane] = cr;
+            /* the (Q, R) at the budget without a carry, and the smallest
+             * entry above it of those the two searches look at */
+            uint32_t r0 = 0, data0 = 0, thr = 0;
+            if (ok0) {
+                const uint32_t avail0 = b0 - hdr, er = uni(g.empty_mask[q0]);
+                if (search_r(maxr, er, cr, 0, avail0, r0)) {
+                    data0 = lane_of(cr, r0);
+                    const bool lo32 = lane < 32;
+                    const bool cand_q = lo32 && lane <= maxq && !((eq >> (lane & 31)) & 1u) && cq > avail0;
+                    const bool cand_r = lo32 && lane <= maxr && !((er >> (lane & 31)) & 1u) && cr > avail0;
+                    thr = wave_min(min(cand_q ? cq : ~0u, cand_r ? cr : ~0u));
+                }
+            }
+            if (lane == 0) {
+                sp_r[slot] = r0;
+                sp_data[slot] = data0;
+                sp_thr[slot] = thr; /* 0: search at the carried budget */
+            }
         }
         __syncthreads();
         if (wid == 0) {
             for (uint32_t k = 0; k < n; k++) {
                 const uint32_t i = i0 + k, row = first_row + i, type = row + 1 >= rows ? 1 : 0;
-                const uint32_t cq = col[k][lane], crs = rowq[k][lane], q0 = spec[k];
                 JxsPrec P;
                 P.q = P.r = P.total = P.pad = 0;
                 P.offset = off;
                 P.err = 1;
                 if (!failed) {
-                    const uint32_t base = rc_mode >= 2 ? uni(gld(&F.prec[row].total)) : (i == 0 ? first : minb) + (i < left ? 1 : 0);
+                    const uint32_t base = uni(gld(&F.prec[row].total));
                     const uint32_t budget = uni(base + carry);
                     const uint32_t hdr = uni(g.rc_hdr_bytes[type]);
                     bool ok = budget <= 1048575u && budget > hdr;
-                    uint32_t q = 0, r = 0;
+                    uint32_t q = 0, r = 0, data = 0;
                     if (ok) {
                         const uint32_t avail = budget - hdr;
-                        ok = search_q(maxq, eq, cq, avail, q);
-                        if (ok) {
-                            /* the bytes at (q, r) are lane base + r of cr */
-                            uint32_t cr = crs, base = 0;
-                            if (q0 > 0 && q + 1 == q0) {
-                                base = JXS_NR;
-                            } else if (q != q0) {
-                                cr = rate_query(type ? rq1 : rq0, type ? rw1 : rw0, g.rq_n[type], tabs[k], q,
-                                                lane & (JXS_NR - 1));
-                            }
-                            ok = search_r(maxr, lane_of(em, q), cr, base, avail, r);
+                        if (avail < uni(sp_thr[k])) {
+                            q = uni(spec[k]);
+                            r = uni(sp_r[k]);
+                            data = uni(sp_data[k]);
+                        } else {
+                            const uint32_t cq = col[k][lane], crs = rowq[k][lane], q0 = uni(spec[k]);
+                            ok = search_q(maxq, eq, cq, avail, q);
                             if (ok) {
-                                const uint32_t data = lane_of(cr, base + r);
-                                uint32_t pad = avail - data, total = budget;
-                                if (move_pad && i + 1 < nprec) {
-                                    total -= pad;
-                                    carry = pad;
-                                    pad = 0;
+                                /* the bytes at (q, r) are lane base + r of cr */
+                                uint32_t cr = crs, base = 0;
+                                if (q0 > 0 && q + 1 == q0) {
+                                    base = JXS_NR;
+                                } else if (q != q0) {
+                                    cr = rate_query(g.rq[type][lane], g.rq_raw[type][lane], g.rq_n[type], tabs[k], q,
+                                                    lane & (JXS_NR - 1));
                                 }
-                                P.q = q;
-                                P.r = r;
-                                P.total = total;
-                                P.pad = pad;
-                                P.err = 0;
-                                off += total;
+                                ok = search_r(maxr, uni(g.empty_mask[q]), cr, base, avail, r);
+                                if (ok)
+                                    data = lane_of(cr, base + r);
                             }
+                        }
+                        if (ok) {
+                            uint32_t pad = avail - data, total = budget;
+                            if (i + 1 < nprec) { /* modes 2 and 3 move the padding on */
+                                total -= pad;
+                                carry = pad;
+                                pad = 0;
+                            }
+                            P.q = q;
+                            P.r = r;
+                            P.total = total;
+                            P.pad = pad;
+                            P.err = 0;
+                            off += total;
                         }
                     }
                     if (!ok) {
@@ -1586,10 +1744,14 @@ __global__ void __launch_bounds__(WAVE * RT_WAVES) k_rate(const JxsFrameDesc* __
             }
         }
         __syncthreads();
-        if (wid < n) {
-            const uint32_t row = first_row + i0 + wid, type = row + 1 >= rows ? 1 : 0;
-            const JxsPrec P = ps[wid];
-            plan_precinct(F, g, row, type, P, tabs[wid], ms_band[wid], ui, lane);
+        if (mine && sub == 0) {
+            const JxsPrec P = ps[slot];
+            const uint32_t ui = g.u_info[lane];
+            const bool pres = lane < g.n_units && ((ui >> (UI_NORMAL + type)) & 1u);
+            const uint32_t o = (ui & 0xFFu) * RT_STRIDE +
+                (pres ? truncation((ui >> 8) & 0xFFu, (ui >> 16) & 0xFFu, P.q, P.r) : RT_ZERO);
+            const RateTab& T = tabs[slot];
+            plan_precinct(F, g, row, type, P, T.s[o], T.g[o], T.d[o], ms_band[slot], ui, lane);
         }
         __syncthreads();
     }
@@ -2105,10 +2267,348 @@ hipError_t jxs_launch_dwt(const JxsDwtJob* jobs, const uint32_t* bases, int njob
     return hipGetLastError();
 }
 
-hipError_t jxs_launch_rate(const JxsFrameDesc* frames, int nfr, uint32_t slices, hipStream_t st) {
+/* ------------------------------------------------------------------ */
+/* rate control modes 0 and 1: one block per precinct for the rate      */
+/* table and its searches (k_rate_tab), then one per precinct for the    */
+/* in-order pass over its slice and its packet plan (k_rate_plan)       */
+/* ------------------------------------------------------------------ */
+
+/* The frame of precinct p of a launch (frames in precinct order). */
+__device__ __forceinline__ int frame_of_prec(const JxsFrameDesc* __restrict__ frames, int nfr, uint32_t p) {
+    int fl = 0, fh = nfr - 1;
+    while (fl < fh) {
+        const int mid = (fl + fh + 1) >> 1;
+        if (frames[mid].prec_base <= p)
+            fl = mid;
+        else
+            fh = mid - 1;
+    }
+    return fl;
+}
+
+/* The budget of precinct i of a slice of nprec under rate control mode 0
+ * (an even split, PackStageProcess.c:600-601) or 1 (the first precinct 10 %
+ * more, the rest split evenly, 621-640), before any padding carry. */
+__device__ __forceinline__ uint32_t prec_budget(uint32_t sbud, uint32_t nprec, uint32_t i, uint32_t rc_mode) {
+    uint32_t minb = sbud / nprec;
+    uint32_t first = minb, left = sbud - minb * nprec;
+    if (rc_mode == 1) {
+        first = minb * 110 / 100;
+        if (first > sbud)
+            first = sbud;
+        left = 0;
+        if (nprec > 1) {
+            minb = (sbud - first) / (nprec - 1);
+            left = sbud - minb * (nprec - 1) - first;
+        }
+    }
+    return (i == 0 ? first : minb) + (i < left ? 1u : 0u);
+}
+
+/* Byte k of a read-only descriptor, through a scalar load of its dword. */
+__device__ __forceinline__ uint32_t cld_u8(const uint8_t* p) {
+    const uintptr_t a = (uintptr_t)p;
+    const uint32_t w = *(const CAS uint32_t*)(a & ~(uintptr_t)3);
+    return (w >> (8 * (a & 3))) & 0xFFu;
+}
+__device__ __forceinline__ uint32_t cld_u32(const uint32_t* p) { return *(const CAS uint32_t*)p; }
+
+/* The rate-table entry of band line `lane` at truncation t alone (t >= 16:
+ * the line is absent from the precinct, zeros), from its histogram record:
+ * the sums of rate_lines_part at one t.  The other line of a two-line band
+ * (the neighbouring lane) must ask for the same t; every lane takes part. */
+__device__ __forceinline__ void rate_line_at(const Rec& rc, uint32_t n_bl, uint32_t li, uint32_t type,
+                                             uint32_t sig_on, uint32_t lane, uint32_t t, uint32_t& od, uint32_t& og,
+                                             uint32_t& os) {
+    const uint32_t w[16] = {rc.a.x, rc.a.y, rc.a.z, rc.a.w, rc.b.x, rc.b.y, rc.b.z, rc.b.w,
+                            rc.c.x, rc.c.y, rc.c.z, rc.c.w, rc.e.x, rc.e.y, rc.e.z, rc.e.w};
+    uint32_t tot = 0, vtot = 0, cnt = 0, vcnt = 0, sig = 0;
+#pragma unroll
+    for (uint32_t v = 0; v < 16; v++) {
+        const uint32_t h = (w[v >> 1] >> (16 * (v & 1))) & 0xFFFFu;
+        tot += h;
+        vtot += v * h;
+        if (v <= t) {
+            cnt += h;
+            vcnt += v * h;
+            if (v < 15)
+                sig += (w[8 + (v >> 1)] >> (16 * (v & 1))) & 0xFFFFu;
+        }
+    }
+    const uint32_t gw = li & 0xFFFu, full = gw >> 3, part = gw & 7, sw = (li >> LI_SW) & 0xFFu;
+    if (t >= 15)
+        sig = full;
+    const uint32_t nl = (li >> (LI_NL + 2 * type)) & 3u;
+    const bool two = nl == 2;
+    const int partner = (int)(two ? (((li >> LI_SECOND) & 1u) ? lane - 1 : lane + 1) : lane);
+    const uint32_t tail = w[15] >> 16; /* 0xFFFF: no partial group */
+    const uint32_t tt = min(t, 15u);
+    const uint32_t above = tot - cnt, vabove = vtot - vcnt;
+    const uint32_t data = vabove + above - tt * above; /* sum_{v>t} (v - t + 1) h(v) */
+    const uint32_t ns = data + cnt, zr = 8 * sig + (tail <= tt ? part : 0u);
+    const uint32_t pn = (uint32_t)__shfl((int)ns, partner), pz = (uint32_t)__shfl((int)zr, partner);
+    const uint32_t nosig = ns + (two ? pn : 0u);
+    const uint32_t withsig = sw * nl + (ns - zr) + (two ? pn - pz : 0u);
+    const bool use = sig_on && nosig > withsig;
+    const bool absent = t >= 16 || lane >= n_bl;
+    od = absent ? 0u : 4 * data;
+    og = absent ? 0u : (use ? ns - zr : ns);
+    os = absent ? 0u : (use ? sw : 0u);
+}
+
+/* One block per precinct (4 waves): its rate table (a quarter of the
+ * truncations a wave), its bytes at (q, 0) for every q and the (q, r) rows
+ * at the Q its budget gets without a padding carry and at that Q - 1 (a run
+ * of whole packets a wave, the runs summed), the binary searches
+ * (RateControl.c:889-1083, BinarySearch.c) at that budget, and the smallest
+ * entry above the budget of those the searches look at: all into the
+ * precinct's rate record. */
+#define RB_WAVES 4
+__global__ void __launch_bounds__(WAVE * RB_WAVES) k_rate_tab(const JxsFrameDesc* __restrict__ frames, int nfr,
+                                                              uint32_t total_prec) {
+    __shared__ RateTab T;
+    __shared__ uint32_t part[2][RB_WAVES][WAVE];
+    const uint32_t pg = blockIdx.x;
+    if (pg >= total_prec)
+        return;
+    const JxsFrameDesc F = cld_words(frames + frame_of_prec(frames, nfr, pg));
+    if (F.ordered || F.slice_rc)
+        return; /* coded by k_enc_slice, or by k_rate */
+    const uint32_t lane = threadIdx.x & (WAVE - 1);
+    const uint32_t w = (uint32_t)__builtin_amdgcn_readfirstlane((int)(threadIdx.x / WAVE));
+    const uint32_t row = pg - F.prec_base, rows = F.rows, pps = F.pps, n_bl = F.n_bl;
+    const uint32_t s = row / pps, i = row - s * pps, first_row = s * pps;
+    const uint32_t nprec = (s == F.slices - 1) ? rows - first_row : pps;
+    const uint32_t type = row + 1 >= rows ? 1u : 0u;
+    const JxsGeom* G = F.g;
+    /* every load the block needs, issued together */
+    const Rec rc = load_rec(F.hist + (size_t)row * n_bl * JXS_HREC, min(lane, n_bl - 1));
+    const uint32_t li = gld(&G->ln_info[lane]);
+    const uint32_t rq = gld(&G->rq[type][lane]), rw = gld(&G->rq_raw[type][lane]);
+    const uint32_t em = gld(&G->empty_mask[lane & (JXS_NQ - 1)]);
+    const uint32_t sig = cld_u32(&G->sig), maxq = cld_u32(&G->maxq), maxr = cld_u32(&G->maxr);
+    const uint32_t rc_mode = cld_u32(&G->rc_mode), hdr = cld_u32(&G->rc_hdr_bytes[type]);
+    const uint32_t lo = cld_u8(&G->rq_seg[type][2][w]), hi = cld_u8(&G->rq_seg[type][2][w + 1]);
+    const uint32_t b0 = prec_budget(cld_u32(F.slice_bud + s), nprec, i, rc_mode);
+    const uint32_t eq = (uint32_t)__ballot(em & 1u); /* bit q: (q, 0) truncates every band away */
+    rate_lines(T, rc, n_bl, li, type, sig, lane, 2, w);
+    lds_barrier();
+    /* lane q (and q + 32): the run's bytes at (q, 0) */
+    part[0][w][lane] = rate_query_seg(rq, rw, lo, hi, T, lane & (JXS_NQ - 1), 0);
+    lds_barrier();
+    const uint32_t cq = part[0][0][lane] + part[0][1][lane] + part[0][2][lane] + part[0][3][lane];
+    uint32_t q0 = 0;
+    const bool ok0 = b0 <= 1048575u && b0 > hdr && search_q(maxq, eq, cq, b0 - hdr, q0);
+    if (!ok0)
+        q0 = 0;
+    /* the (q, r) rows at that Q (lanes 0..31: lane r) and at Q - 1 (lanes 32..63) */
+    const uint32_t qc = lane < JXS_NR ? q0 : (q0 > 0 ? q0 - 1 : 0);
+    part[1][w][lane] = rate_query_seg(rq, rw, lo, hi, T, qc, lane & (JXS_NR - 1));
+    lds_barrier();
+    if (w != 0)
+        return;
+    const uint32_t cr = part[1][0][lane] + part[1][1][lane] + part[1][2][lane] + part[1][3][lane];
+    uint32_t r0 = 0, data0 = 0, thr = 0;
+    if (ok0) {
+        const uint32_t avail0 = b0 - hdr, er = lane_of(em, q0);
+        if (search_r(maxr, er, cr, 0, avail0, r0)) {
+            data0 = lane_of(cr, r0);
+            const bool lo32 = lane < 32;
+            const bool cand_q = lo32 && lane <= maxq && !((eq >> (lane & 31)) & 1u) && cq > avail0;
+            const bool cand_r = lo32 && lane <= maxr && !((er >> (lane & 31)) & 1u) && cr > avail0;
+            thr = wave_min(min(cand_q ? cq : ~0u, cand_r ? cr : ~0u));
+        }
+    }
+    JxsRateRec* R = F.rrec + row;
+    if (lane < 32)
+        gst(&R->col[lane], cq);
+    gst(&R->row[lane], cr);
+    if (lane == 0) {
+        const u32x4 sp = {q0, r0, data0, thr};
+        gst((u32x4*)&R->q0, sp);
+    }
+}
+
+/* One wave per precinct: the in-order pass over its slice up to it, which
+ * fixes its budget, (Q, R), padding and codestream offset (the budgets with
+ * the padding carry, PackStageProcess.c:600-684), then its packet plan.  A
+ * precinct's (Q, R) is the one its rate record notes while the carried
+ * budget stays below the noted entry; otherwise the searches run again on
+ * the record's column and rows, and on a row computed from a rate table
+ * rebuilt here when the carry moved Q by two steps or more. */
+__global__ void __launch_bounds__(WAVE) k_rate_plan(const JxsFrameDesc* __restrict__ frames, int nfr,
+                                                    uint32_t total_prec) {
+    __shared__ __align__(16) JxsGeom sg;
+    __shared__ RateTab T;
+    __shared__ uint32_t tl[WAVE], vd_l[WAVE], vg_l[WAVE], vs_l[WAVE];
+    __shared__ uint32_t ms_band[JXS_NGB];
+    const uint32_t pg = blockIdx.x;
+    if (pg >= total_prec)
+        return;
+    const JxsFrameDesc F = cld_words(frames + frame_of_prec(frames, nfr, pg));
+    if (F.ordered || F.slice_rc)
+        return;
+    const uint32_t lane = threadIdx.x;
+    const uint32_t row = pg - F.prec_base, rows = F.rows, pps = F.pps, n_bl = F.n_bl;
+    const uint32_t s = row / pps, i = row - s * pps, first_row = s * pps;
+    const uint32_t nprec = (s == F.slices - 1) ? rows - first_row : pps;
+    const uint32_t type = row + 1 >= rows ? 1u : 0u;
+    const uint32_t rec_line = min(lane, n_bl - 1);
+    const Rec rc = load_rec(F.hist + (size_t)row * n_bl * JXS_HREC, rec_line);
+    /* with the geometry: the noted (Q, R) of the slice's first precincts (up
+     * to 64) and the columns and rows of its first 4 (a carry past the noted
+     * entry then needs no further load there) */
+    const JxsRateRec* RR = F.rrec + first_row;
+    u32x4 sp = {0u, 0u, 0u, 0u};
+    if (lane <= i)
+        sp = gld((const u32x4*)&RR[lane].q0);
+    uint32_t pcol[4], prow[4];
+#pragma unroll
+    for (uint32_t j = 0; j < 4; j++) {
+        pcol[j] = j <= i ? gld(&RR[j].col[lane & (JXS_NQ - 1)]) : 0u;
+        prow[j] = j <= i ? gld(&RR[j].row[lane]) : 0u;
+    }
+    stage_geom<WAVE>(&sg, F.g);
+    const uint32_t sbud = cld_u32(F.slice_bud + s);
+    uint32_t off = cld_u32(F.slice_off + s) + 6;
+    __syncthreads();
+    const JxsGeom& g = sg;
+    const uint32_t maxq = uni(g.maxq), maxr = uni(g.maxr), rc_mode = uni(g.rc_mode);
+    const uint32_t eq = (uint32_t)__ballot(g.empty_mask[lane & (JXS_NQ - 1)] & 1u);
+    const bool move_pad = rc_mode != 0;
+    /* the slice budget's split (prec_budget's) */
+    uint32_t minb = sbud / nprec;
+    uint32_t first = minb, left = sbud - minb * nprec;
+    if (rc_mode == 1) {
+        first = minb * 110 / 100;
+        if (first > sbud)
+            first = sbud;
+        left = 0;
+        if (nprec > 1) {
+            minb = (sbud - first) / (nprec - 1);
+            left = sbud - minb * (nprec - 1) - first;
+        }
+    }
+    uint32_t carry = 0;
+    bool failed = false;
+    JxsPrec P;
+    P.q = P.r = P.total = P.pad = 0;
+    P.offset = off;
+    P.err = 1;
+    for (uint32_t k0 = 0; k0 <= i; k0 += WAVE) {
+        /* the noted (Q, R) of precincts k0 + lane */
+        if (k0 > 0)
+            sp = k0 + lane <= i ? gld((const u32x4*)&RR[k0 + lane].q0) : u32x4{0u, 0u, 0u, 0u};
+        const uint32_t kn = min((uint32_t)WAVE, i + 1 - k0);
+        for (uint32_t kk = 0; kk < kn; kk++) {
+            const uint32_t k = k0 + kk, rk = first_row + k, tk = rk + 1 >= rows ? 1u : 0u;
+            JxsPrec Q;
+            Q.q = Q.r = Q.total = Q.pad = 0;
+            Q.offset = off;
+            Q.err = 1;
+            if (!failed) {
+                const uint32_t budget = (k == 0 ? first : minb) + (k < left ? 1u : 0u) + carry;
+                const uint32_t hdr = uni(g.rc_hdr_bytes[tk]);
+                bool ok = budget <= 1048575u && budget > hdr;
+                uint32_t q = 0, r = 0, data = 0;
+                if (ok) {
+                    const uint32_t avail = budget - hdr;
+                    const uint32_t q0 = lane_of(sp.x, kk);
+                    if (avail < lane_of(sp.w, kk)) {
+                        q = q0;
+                        r = lane_of(sp.y, kk);
+                        data = lane_of(sp.z, kk);
+                    } else {
+                        uint32_t cq, cr, base = 0;
+                        if (k < 4) {
+                            cq = k == 0 ? pcol[0] : k == 1 ? pcol[1] : k == 2 ? pcol[2] : pcol[3];
+                            cr = k == 0 ? prow[0] : k == 1 ? prow[1] : k == 2 ? prow[2] : prow[3];
+                        } else {
+                            cq = gld(&RR[k].col[lane & (JXS_NQ - 1)]);
+                            cr = gld(&RR[k].row[lane]);
+                        }
+                        ok = search_q(maxq, eq, cq, avail, q);
+                        if (ok) {
+                            /* the bytes at (q, r) are lane base + r of cr */
+                            if (q0 > 0 && q + 1 == q0) {
+                                base = JXS_NR;
+                            } else if (q != q0) {
+                                const Rec rp = load_rec(F.hist + (size_t)rk * n_bl * JXS_HREC, rec_line);
+                                rate_lines(T, rp, n_bl, g.ln_info[lane], tk, g.sig, lane, 0, 0);
+                                wave_sync();
+                                cr = rate_query(g.rq[tk][lane], g.rq_raw[tk][lane], g.rq_n[tk], T, q,
+                                                lane & (JXS_NR - 1));
+                                wave_sync();
+                            }
+                            ok = search_r(maxr, uni(g.empty_mask[q]), cr, base, avail, r);
+                            if (ok)
+                                data = lane_of(cr, base + r);
+                        }
+                    }
+                    if (ok) {
+                        uint32_t pad = avail - data, total = budget;
+                        if (move_pad && k + 1 < nprec) {
+                            total -= pad;
+                            carry = pad;
+                            pad = 0;
+                        }
+                        Q.q = q;
+                        Q.r = r;
+                        Q.total = total;
+                        Q.pad = pad;
+                        Q.err = 0;
+                        off += total;
+                    }
+                }
+                if (!ok) {
+                    failed = true;
+                    if (lane == 0)
+                        gor(F.status, 1u);
+                }
+            }
+            if (k == i)
+                P = Q;
+        }
+    }
+    if (lane == 0)
+        gst_words(F.prec + row, P);
+    /* every unit's bits at its band's truncation, computed per band line */
+    const uint32_t ui = g.u_info[lane];
+    const bool pres = lane < g.n_units && ((ui >> (UI_NORMAL + type)) & 1u);
+    tl[lane] = 16u;
+    wave_sync();
+    if (lane < g.n_units)
+        tl[ui & 0xFFu] = pres ? truncation((ui >> 8) & 0xFFu, (ui >> 16) & 0xFFu, P.q, P.r) : 16u;
+    wave_sync();
+    uint32_t d, gb, sb;
+    rate_line_at(rc, n_bl, g.ln_info[lane], type, g.sig, lane, tl[lane], d, gb, sb);
+    vd_l[lane] = d;
+    vg_l[lane] = gb;
+    vs_l[lane] = sb;
+    wave_sync();
+    const uint32_t line = ui & 0xFFu & (WAVE - 1);
+    plan_precinct(F, g, row, type, P, vs_l[line], vg_l[line], vd_l[line], ms_band, ui, lane);
+}
+
+/* kinds: bit 0 launches the rate control of the frames under modes 0 / 1
+ * (k_rate_tab, then k_rate_plan), bit 1 that of the frames under modes 2 / 3
+ * (k_rate); each skips the other's frames. */
+hipError_t jxs_launch_rate(const JxsFrameDesc* frames, int nfr, uint32_t slices, uint32_t precincts, uint32_t kinds,
+                           hipStream_t st) {
     if (!slices)
         return hipSuccess;
-    hipLaunchKernelGGL(k_rate, dim3(slices), dim3(WAVE * RT_WAVES), 0, st, frames, nfr, slices);
+    if (kinds & 1u) {
+        hipLaunchKernelGGL(k_rate_tab, dim3(precincts), dim3(WAVE * RB_WAVES), 0, st, frames, nfr, precincts);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess)
+            return e;
+        hipLaunchKernelGGL(k_rate_plan, dim3(precincts), dim3(WAVE), 0, st, frames, nfr, precincts);
+        e = hipGetLastError();
+        if (e != hipSuccess)
+            return e;
+    }
+    if (kinds & 2u)
+        hipLaunchKernelGGL(k_rate, dim3(slices), dim3(WAVE * RT_NW), 0, st, frames, nfr, slices);
     return hipGetLastError();
 }
 
