@@ -1821,6 +1821,11 @@ static int32_t parse_header(const uint8_t* cs, uint32_t len, dec_hdr* d) {
     }
     if (!have_pih || !have_cdt || !have_wgt || cw != 0 || nc == 0 || nc > 4 || hsl == 0)
         return E_INVALID_BS;
+    /* sampling factors this restatement lays out (a damaged header can carry
+     * any 4-bit value; Pi.c:230-231 refuses 0) */
+    for (uint32_t c = 0; c < nc; c++)
+        if (sx[c] < 1 || sx[c] > 2 || sy[c] < 1 || sy[c] > 2)
+            return E_INVALID_BS;
     if (pi_build(&d->pi, nc, sx, sy, W, H, dh, dv, hsl << dv))
         return E_INVALID_BS;
     if (d->pi.short_hdr != !lh)

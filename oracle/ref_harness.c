@@ -17,6 +17,24 @@
  *   ref_dec in.jxs out.raw frames threads [proxy_mode]
  *   Decodes `frames` concatenated codestreams and writes the decoded planes
  *   (at half or quarter resolution with proxy_mode 1 or 2).
+ *
+ * Damaged-codestream sweep (HARNESS_SWEEP):
+ *   ref_sweep valid.jxs
+ *   Runs the damage patterns of the reference's decoder unit tests
+ *   (tests/UnitTests/TestDecoder.cc:336-618) on one valid codestream and
+ *   prints one line per damaged variant:
+ *     kind a b fs r1 r2 h1 h2
+ *   kind: T (first a bytes, used size a), Z / O (bytes from a on set to 0 /
+ *   1, full size), X (byte a inverted), B (bit 7 - b of byte a flipped),
+ *   M (the cumulative walk of TestDecoder.cc:574-604: bit 7 - b of byte a
+ *   flipped into a buffer that keeps every flip after which the buffer still
+ *   decodes, undone otherwise).  fs: svt_jpeg_xs_decoder_get_single_frame_size
+ *   of the variant; r1: a decoder initialised with the variant decodes it; r2:
+ *   a decoder initialised with the valid codestream decodes it, then the
+ *   variant; h1 / h2: the planes the successful decodes wrote, hashed as
+ *   crc32 << 32 | adler32 (zlib) of their bytes in component order (0
+ *   otherwise).  The walk keeps a flip when fs and r2 are both 0.  A final
+ *   line "F hash r2 h2" gives the walk's buffer (same hash) and its decode.
  */
 #include <stdint.h>
 #include <stdio.h>
@@ -276,3 +294,144 @@ int main(int argc, char** argv) {
     return 0;
 }
 #endif /*HARNESS_DECODER*/
+
+#ifdef HARNESS_SWEEP
+#include <zlib.h>
+
+/* running crc32 / adler32 of a byte sequence fed in parts */
+typedef struct {
+    uLong crc, adl;
+} hash_t;
+static void hash_init(hash_t* h) {
+    h->crc = crc32(0L, Z_NULL, 0);
+    h->adl = adler32(0L, Z_NULL, 0);
+}
+static void hash_add(hash_t* h, const uint8_t* p, size_t n) {
+    h->crc = crc32(h->crc, p, (uInt)n);
+    h->adl = adler32(h->adl, p, (uInt)n);
+}
+static uint64_t hash_get(const hash_t* h) { return ((uint64_t)(h->crc & 0xFFFFFFFFu) << 32) | (h->adl & 0xFFFFFFFFu); }
+
+/* One decoder over `first` (init + decode), then over `second` when given;
+ * returns the last call's status and the hash of what it wrote. */
+static int32_t decode_pair(const uint8_t* first, size_t first_size, const uint8_t* second, size_t second_size,
+                           uint64_t* hash) {
+    svt_jpeg_xs_decoder_api_t dec;
+    memset(&dec, 0, sizeof(dec));
+    dec.verbose = VERBOSE_NONE;
+    dec.threads_num = 1;
+    dec.use_cpu_flags = CPU_FLAGS_C;
+    svt_jpeg_xs_image_config_t cfg;
+    *hash = 0;
+    int32_t e = svt_jpeg_xs_decoder_init(SVT_JPEGXS_API_VER_MAJOR, SVT_JPEGXS_API_VER_MINOR, &dec, first, first_size, &cfg);
+    if (e)
+        return e;
+    svt_jpeg_xs_image_buffer_t img;
+    memset(&img, 0, sizeof(img));
+    uint32_t pix = cfg.bit_depth <= 8 ? 1 : 2;
+    for (int c = 0; c < cfg.components_num; c++) {
+        img.stride[c] = cfg.components[c].width;
+        img.alloc_size[c] = cfg.components[c].width * cfg.components[c].height * pix;
+        img.data_yuv[c] = calloc(1, img.alloc_size[c]);
+    }
+    const uint8_t* bufs[2] = {first, second};
+    size_t sizes[2] = {first_size, second_size};
+    for (int k = 0; k < (second ? 2 : 1) && !e; k++) {
+        for (int c = 0; c < cfg.components_num; c++)
+            memset(img.data_yuv[c], 0, img.alloc_size[c]);
+        svt_jpeg_xs_frame_t in, out;
+        memset(&in, 0, sizeof(in));
+        in.bitstream.buffer = (uint8_t*)bufs[k];
+        in.bitstream.allocation_size = (uint32_t)sizes[k];
+        in.bitstream.used_size = (uint32_t)sizes[k];
+        in.image = img;
+        e = svt_jpeg_xs_decoder_send_frame(&dec, &in, 1);
+        if (!e)
+            e = svt_jpeg_xs_decoder_get_frame(&dec, &out, 1);
+    }
+    if (!e) {
+        hash_t h;
+        hash_init(&h);
+        for (int c = 0; c < cfg.components_num; c++)
+            hash_add(&h, (const uint8_t*)img.data_yuv[c], img.alloc_size[c]);
+        *hash = hash_get(&h);
+    }
+    for (int c = 0; c < cfg.components_num; c++)
+        free(img.data_yuv[c]);
+    svt_jpeg_xs_decoder_close(&dec);
+    return e;
+}
+
+static int32_t variant(const uint8_t* valid, size_t n, const uint8_t* bad, size_t bad_size, char kind, uint32_t a,
+                       uint32_t b) {
+    /* the damaged bytes in an allocation of exactly their size */
+    uint8_t* v = (uint8_t*)malloc(bad_size ? bad_size : 1);
+    memcpy(v, bad, bad_size);
+    uint32_t fs_size = 0;
+    int32_t fs = svt_jpeg_xs_decoder_get_single_frame_size(v, bad_size, NULL, &fs_size, 0);
+    uint64_t h1, h2;
+    int32_t r1 = decode_pair(v, bad_size, NULL, 0, &h1);
+    int32_t r2 = decode_pair(valid, n, v, bad_size, &h2);
+    printf("%c %u %u %d %d %d %016llx %016llx\n", kind, a, b, fs, r1, r2, (unsigned long long)h1,
+           (unsigned long long)h2);
+    free(v);
+    return (fs == 0 && r2 == 0) ? 0 : -1;
+}
+
+int main(int argc, char** argv) {
+    if (argc < 2) {
+        fprintf(stderr, "usage: ref_sweep valid.jxs\n");
+        return 2;
+    }
+    FILE* fi = fopen(argv[1], "rb");
+    if (!fi)
+        return 3;
+    fseek(fi, 0, SEEK_END);
+    size_t n = (size_t)ftell(fi);
+    fseek(fi, 0, SEEK_SET);
+    uint8_t* valid = (uint8_t*)malloc(n);
+    if (fread(valid, 1, n, fi) != n)
+        return 4;
+    fclose(fi);
+    uint8_t* buf = (uint8_t*)malloc(n);
+    for (uint32_t s = 1; s < n; s++)
+        variant(valid, n, valid, s, 'T', s, 0);
+    for (uint32_t s = 1; s < n; s++) {
+        memcpy(buf, valid, n);
+        memset(buf + s, 0, n - s);
+        variant(valid, n, buf, n, 'Z', s, 0);
+    }
+    for (uint32_t s = 1; s < n; s++) {
+        memcpy(buf, valid, n);
+        memset(buf + s, 1, n - s);
+        variant(valid, n, buf, n, 'O', s, 0);
+    }
+    for (uint32_t i = 0; i < n; i++) {
+        memcpy(buf, valid, n);
+        buf[i] = (uint8_t)~buf[i];
+        variant(valid, n, buf, n, 'X', i, 0);
+    }
+    for (uint32_t i = 0; i < n; i++)
+        for (uint32_t j = 0; j < 8; j++) {
+            memcpy(buf, valid, n);
+            buf[i] ^= (uint8_t)(1u << (7 - j));
+            variant(valid, n, buf, n, 'B', i, j);
+        }
+    memcpy(buf, valid, n);
+    for (uint32_t i = 0; i < n; i++)
+        for (uint32_t j = 0; j < 8; j++) {
+            buf[i] ^= (uint8_t)(1u << (7 - j));
+            if (variant(valid, n, buf, n, 'M', i, j) != 0)
+                buf[i] ^= (uint8_t)(1u << (7 - j));
+        }
+    uint64_t h2;
+    int32_t r2 = decode_pair(buf, n, NULL, 0, &h2);
+    hash_t hb;
+    hash_init(&hb);
+    hash_add(&hb, buf, n);
+    printf("F %016llx %d %016llx\n", (unsigned long long)hash_get(&hb), r2, (unsigned long long)h2);
+    free(buf);
+    free(valid);
+    return 0;
+}
+#endif /*HARNESS_SWEEP*/

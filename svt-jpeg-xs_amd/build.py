@@ -12,7 +12,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libsvtjpegxs_amd.so")
 ARCH = os.environ.get("JXS_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["jxs_kernels.hip", "jxs_enc_ordered.hip", "jxs_host.cpp", "jxs_encoder.cpp", "jxs_decoder.cpp", "jxs_dec_kernels.hip",
+SOURCES = ["jxs_kernels.hip", "jxs_enc_ordered.hip", "jxs_host.cpp", "jxs_codestream.cpp", "jxs_encoder.cpp", "jxs_decoder.cpp", "jxs_dec_kernels.hip",
            "jxs_buffers.cpp", "jxs_server.cpp"]
 
 # Packed-FP32 instructions are disabled for all device code built here.

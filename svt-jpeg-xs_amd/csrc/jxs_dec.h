@@ -26,6 +26,7 @@ struct JxsDecFrame {
     uint32_t slice_base; /* first slice of this frame in the launch */
     uint32_t* pkt;       /* per precinct: JXS_DPKT_HDR words of band modes and flags, then 5 words per packet */
     uint8_t* gcli;       /* decoded bit-plane counts (JxsBand::gcli_off layout), for slices decoded in order */
+    uint32_t rows_valid; /* precinct rows decoded; the rows after them are skipped (their slices failed on the host) */
 };
 
 /* Packet table of one precinct (written by k_dec_packets, read by
@@ -54,6 +55,8 @@ struct JxsInvRowJob {
     int32_t* out32; /* int32 output rows (stride out_stride), or NULL */
     void* out;      /* output plane (u8 / u16), when out32 == NULL */
     uint32_t out_stride, depth;
+    uint32_t fq; /* fractional bits of the coefficients (Fq): band samples enter the lifting scaled by 2^fq */
+    uint32_t bw; /* nominal coefficient precision (Bw) of the output scaling */
 };
 
 /* Inverse vertical job: int32 low rows then high rows (stride w) -> int32
@@ -63,7 +66,7 @@ struct JxsInvVertJob {
     int32_t* out32;
     void* out;
     uint32_t w, h, out_stride, depth;
-    uint32_t tiles_x, tiles, base, pad_;
+    uint32_t tiles_x, tiles, base, bw;
 };
 
 #endif

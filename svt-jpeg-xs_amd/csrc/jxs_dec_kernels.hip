@@ -235,6 +235,13 @@ __global__ void __launch_bounds__(64) k_dec_packets(const JxsDecFrame* __restric
     if (row >= g.rows)
         return;
     uint32_t* T = F.pkt + (size_t)row * F.pkt_stride;
+    if (row >= F.rows_valid) {
+        /* a precinct of a slice the host found broken: skipped, the frame's
+         * error comes from the host */
+        T[JXS_DPKT_BAD] = 1u;
+        T[JXS_DPKT_ORDERED] = 0u;
+        return;
+    }
     const uint32_t type = row + 1 >= g.rows ? 1 : 0;
     const uint64_t off = F.prec_off[row];
     const uint32_t lprc = cs_bits(F, off * 8, 24);
@@ -262,11 +269,12 @@ __global__ void __launch_bounds__(64) k_dec_packets(const JxsDecFrame* __restric
     for (uint32_t pp = 0; pp < g.npk; pp++) {
         uint32_t* E = T + JXS_DPKT_HDR + JXS_DPKT_WORDS * pp;
         bool any = false;
-        uint32_t sig_bits = 0;
+        uint32_t sig_bits = 0, groups = 0;
         for (uint32_t gi = g.pk_bs[pp]; gi < g.pk_be[pp]; gi++) {
             const JxsBand& bd = g.c[g.gc[gi]].b[g.gb[gi]];
             if (g.pk_line[pp] < bd.lines[type]) {
                 any = true;
+                groups += bd.gw;
                 if (mode[g.gc[gi] * JXS_NBC + g.gb[gi]] & 2)
                     sig_bits += bd.sw;
             }
@@ -287,6 +295,15 @@ __global__ void __launch_bounds__(64) k_dec_packets(const JxsDecFrame* __restric
             lg = cs_bits(F, pos * 8 + 21, 20);
             ls = cs_bits(F, pos * 8 + 41, 15);
         }
+        /* the three lengths must fit what is left of the precinct after the
+         * header, the sign length counted even without a sign sub-packet
+         * (Packing.c:584-586) */
+        if (pos + g.pkt_hdr_bytes + (uint64_t)ld + lg + ls > pend)
+            bad = true;
+        /* raw bit-plane counts (4 bits a group) must fit their sub-packet
+         * (Packing.c:735-740) */
+        if (raw && (4ull * groups + 7) / 8 > lg)
+            bad = true;
         if (!g.fs)
             ls = 0; /* the field is only a sub-packet length with Fs = 1 */
         const uint32_t sb = raw ? 0u : (sig_bits + 7) / 8;
@@ -295,7 +312,7 @@ __global__ void __launch_bounds__(64) k_dec_packets(const JxsDecFrame* __restric
         E[1] = (uint32_t)(g_off * 8);
         E[2] = (uint32_t)(d_off * 8);
         E[3] = (uint32_t)(nxt * 8);
-        E[4] = 1u | raw << 1;
+        E[4] = bad ? 0u : (1u | raw << 1);
         pos = nxt;
         if (pos > pend)
             bad = true; /* later packets are not decoded */
@@ -306,6 +323,40 @@ __global__ void __launch_bounds__(64) k_dec_packets(const JxsDecFrame* __restric
         atomicOr(F.status, 1u);
     T[JXS_DPKT_BAD] = bad ? 1u : 0u;
     T[JXS_DPKT_ORDERED] = ordered ? 1u : 0u;
+}
+
+/* A coefficient as the reference decoder finishes it: its 16-bit word
+ * after unpacking (the magnitude bit-planes shifted left by the GTLI within
+ * 16 bits, the sign at bit 15: Packing.c:114-131, 221-247), dequantised when
+ * its group's bit-plane count, compared as a signed byte, exceeds the GTLI
+ * (Dequant.c:9-45), then turned into two's complement (Precinct.c:9-14).  A
+ * valid codestream never carries magnitude bits into bit 15; a damaged one
+ * decodes here to what the reference makes of it. */
+__device__ __forceinline__ int16_t coef_final(uint32_t u, uint32_t gc, uint32_t gt, uint32_t qpih) {
+    if (gt > 0 && (int32_t)(int8_t)gc > (int32_t)gt && (u & 0x7FFFu)) {
+        if (qpih == 0) {
+            u |= 1u << (gt - 1);
+        } else {
+            const uint32_t sign = u & 0x8000u, sc = (gc - gt + 1) & 0xFFu;
+            uint32_t val = u & 0x7FFFu, acc = val;
+            if (sc < 16)
+                for (val >>= sc; val > 0; val >>= sc)
+                    acc += val;
+            u = (acc & 0xFFFFu) | sign;
+        }
+    }
+    return (u & 0x8000u) ? (int16_t)(-(int32_t)(u & 0x7FFFu)) : (int16_t)u;
+}
+
+/* The 16-bit words (before the sign) of a group with np >= 1 magnitude
+ * bit-planes and truncation gt whose planes start at aligned bit position
+ * abp: only the last min(np, 16 - gt) planes reach the 16 bits kept. */
+__device__ __forceinline__ void group_words(const CsView& V, uint64_t abp, uint32_t np, uint32_t gt, uint32_t w[4]) {
+    const uint32_t q = min(np, 16u - gt);
+    const uint64_t v = cs_bits64(V, abp + 4ull * (np - q));
+#pragma unroll
+    for (int t4 = 0; t4 < 4; t4++)
+        w[t4] = ((gather4(v << t4) >> (16 - q)) << gt) & 0xFFFFu;
 }
 
 /* One block per (packet, precinct).  Inside a packet the significance,
@@ -480,6 +531,7 @@ __global__ void __launch_bounds__(PK_THREADS) __attribute__((amdgpu_waves_per_eu
         }
         /* GCLI of every group, data bit counts */
         uint32_t gcs[PGPT], nbits = 0;
+        bool run_err = false;
         uint32_t j = cbase;
 #pragma unroll
         for (uint32_t k = 0; k < PGPT; k++) {
@@ -493,16 +545,20 @@ __global__ void __launch_bounds__(PK_THREADS) __attribute__((amdgpu_waves_per_eu
                 gc = (uint32_t)(cs_bits64(V, ab) >> 60);
             } else if ((coded >> k) & 1u) {
                 const uint32_t start = j == 0 ? gp : C.zpos[j - 1] + 1;
-                gc = gt + (C.zpos[j] - start);
+                const uint32_t run = C.zpos[j] - start;
                 j++;
-            }
-            if (gc > 15) {
-                gc = 15;
-                atomicOr(F.status, 1u);
+                if (run >= 32) { /* no code of 32 bits or more (Packing.c:100-107) */
+                    run_err = true;
+                    gc = gt;
+                } else {
+                    gc = gt + run;
+                }
             }
             gcs[k] = gc;
             nbits += gc > gt ? 4 * (gc - gt + 1) : 0u;
         }
+        if (run_err)
+            atomicOr(F.status, 1u);
         uint32_t dtotal;
         const uint32_t dbase = dblock_scan(nbits, C.scan, &dtotal);
         /* data windows, all loaded before any is decoded */
@@ -525,10 +581,13 @@ __global__ void __launch_bounds__(PK_THREADS) __attribute__((amdgpu_waves_per_eu
             }
             bp += nb;
         }
+        uint32_t gbp = dp + dbase; /* bit position of the group's data */
 #pragma unroll
         for (uint32_t k = 0; k < PGPT; k++) {
             if (tid * PGPT + k >= cn || err)
                 continue;
+            const uint32_t gbp_k = gbp;
+            gbp += gcs[k] > C.band[bidx[k]].gt ? 4 * (gcs[k] - C.band[bidx[k]].gt + 1) : 0u;
             const PkBand& B = C.band[bidx[k]];
             const uint32_t gt = B.gt, gc = gcs[k];
             const uint32_t i = i0 + k - B.g0; /* group within its line */
@@ -537,28 +596,18 @@ __global__ void __launch_bounds__(PK_THREADS) __attribute__((amdgpu_waves_per_eu
                 const uint64_t v = win[k];
                 sg = (uint32_t)(v >> 60);
                 const uint32_t np = gc - gt;
+                if (np <= 15) {
 #pragma unroll
-                for (int t4 = 0; t4 < 4; t4++)
-                    mag[t4] = (gather4(v << (4 + t4)) >> (16 - np)) << gt;
+                    for (int t4 = 0; t4 < 4; t4++)
+                        mag[t4] = ((gather4(v << (4 + t4)) >> (16 - np)) << gt) & 0xFFFFu;
+                } else { /* only in a damaged codestream: planes past the first window */
+                    group_words(V, (uint64_t)gbp_k + 4 + V.sh, np, gt, mag);
+                }
             }
             uint32_t val[4];
 #pragma unroll
-            for (uint32_t t4 = 0; t4 < 4; t4++) {
-                uint32_t m = mag[t4];
-                if (gt > 0 && m) {
-                    if (g.qpih == 0) {
-                        m |= 1u << (gt - 1);
-                    } else {
-                        const uint32_t sc = gc - gt + 1;
-                        uint32_t acc = 0;
-                        for (uint32_t x = m; x > 0; x >>= sc)
-                            acc += x;
-                        m = acc & 0xFFFFu;
-                    }
-                }
-                const bool neg = (sg >> (3 - t4)) & 1u;
-                val[t4] = (uint16_t)(neg ? -(int32_t)(m & 0x7FFF) : (int32_t)(m & 0x7FFF));
-            }
+            for (uint32_t t4 = 0; t4 < 4; t4++)
+                val[t4] = (uint16_t)coef_final(mag[t4] | ((sg >> (3 - t4)) & 1u) << 15, gc, gt, g.qpih);
             int16_t* cp = B.row + 4 * i;
             if (4 * i + 4 <= B.w && ((uintptr_t)cp & 7) == 0) {
                 *(uint2*)cp = make_uint2(val[0] | val[1] << 16, val[2] | val[3] << 16);
@@ -586,17 +635,6 @@ __global__ void __launch_bounds__(PK_THREADS) __attribute__((amdgpu_waves_per_eu
 /* ------------------------------------------------------------------ */
 #define SL_MAXG 8192 /* groups of one packet's band lines */
 
-__device__ __forceinline__ uint32_t dequant1(uint32_t m, uint32_t gc, uint32_t gt, uint32_t qpih) {
-    if (gt == 0 || m == 0)
-        return m;
-    if (qpih == 0)
-        return m | 1u << (gt - 1);
-    const uint32_t sc = gc - gt + 1;
-    uint32_t acc = 0;
-    for (uint32_t x = m; x > 0; x >>= sc)
-        acc += x;
-    return acc & 0xFFFFu;
-}
 
 __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x) {
     const uint32_t lane = threadIdx.x & (WAVE - 1);
@@ -685,6 +723,10 @@ __global__ void __launch_bounds__(WAVE) k_dec_slice(const JxsDecFrame* __restric
                 lg = cs_bits(F, pos * 8 + 21, 20);
                 ls = cs_bits(F, pos * 8 + 41, 15);
             }
+            if (pos + g.pkt_hdr_bytes + (uint64_t)ld + lg + ls > pend) { /* Packing.c:584-586 */
+                bad = true;
+                break;
+            }
             if (!g.fs)
                 ls = 0;
             const uint64_t s_off = pos + g.pkt_hdr_bytes, g_off = s_off + (raw ? 0u : (sig_bits + 7) / 8);
@@ -711,7 +753,7 @@ __global__ void __launch_bounds__(WAVE) k_dec_slice(const JxsDecFrame* __restric
                     }
                     base += bd.gw;
                 }
-                if (base > SL_MAXG)
+                if (base > SL_MAXG || (4ull * base + 7) / 8 > lg) /* raw counts fit their sub-packet */
                     bad = true;
             } else if (lane == 0) {
                 uint64_t cur = g_off * 8, sbit = s_off * 8;
@@ -763,10 +805,12 @@ __global__ void __launch_bounds__(WAVE) k_dec_slice(const JxsDecFrame* __restric
                                 }
                                 n += 32;
                                 cur += 32;
-                                if (n > 64)
+                                if (n >= 32)
                                     break;
                             }
-                            if (cur > gend || n > 64) {
+                            /* no code of 32 bits or more (Packing.c:100-107), and
+                             * the codes end inside the sub-packet */
+                            if (cur > gend || n >= 32) {
                                 bad = true;
                                 break;
                             }
@@ -779,13 +823,8 @@ __global__ void __launch_bounds__(WAVE) k_dec_slice(const JxsDecFrame* __restric
                                     dm = (int32_t)n - th;
                                 else if (n > 0)
                                     dm = (n & 1) ? -(int32_t)((n + 1) / 2) : (int32_t)(n / 2);
-                                const int32_t vv = (int32_t)mt + dm;
-                                v = vv < 0 ? 0u : (uint32_t)vv;
+                                v = (uint32_t)((int32_t)mt + dm) & 0xFFu; /* kept as a byte (Packing.c:305) */
                             }
-                        }
-                        if (v > 15) {
-                            v = 15;
-                            bad = true;
                         }
                         gl[i] = (uint8_t)v;
                         gsh[base + i] = (uint8_t)v;
@@ -824,12 +863,16 @@ __global__ void __launch_bounds__(WAVE) k_dec_slice(const JxsDecFrame* __restric
                         if (bp + nb > n_off * 8) {
                             bad = true;
                         } else {
-                            const uint64_t v = cs_bits64(V, bp + V.sh);
                             const uint32_t skip = g.fs ? 0u : 4u;
+                            const uint64_t v = cs_bits64(V, bp + V.sh);
                             sgn = g.fs ? 0u : (uint32_t)(v >> 60);
+                            if (np <= 15) {
 #pragma unroll
-                            for (int t4 = 0; t4 < 4; t4++)
-                                mag[t4] = (gather4(v << (skip + t4)) >> (16 - np)) << gt;
+                                for (int t4 = 0; t4 < 4; t4++)
+                                    mag[t4] = ((gather4(v << (skip + t4)) >> (16 - np)) << gt) & 0xFFFFu;
+                            } else { /* only in a damaged codestream */
+                                group_words(V, bp + skip + V.sh, np, gt, mag);
+                            }
                         }
                     }
                     if (i < bd.gw) {
@@ -838,9 +881,9 @@ __global__ void __launch_bounds__(WAVE) k_dec_slice(const JxsDecFrame* __restric
                                 pad_nz += mag[t4] != 0; /* signs of padding coefficients are skipped */
                                 continue;
                             }
-                            const uint32_t m = dequant1(mag[t4], gc, gt, g.qpih) & 0x7FFF;
-                            const bool neg = (sgn >> (3 - t4)) & 1u;
-                            rowp[4 * i + t4] = (int16_t)(neg ? -(int32_t)m : (int32_t)m);
+                            /* with signs apart, the word waits for its sign (below) */
+                            rowp[4 * i + t4] = g.fs ? (int16_t)mag[t4]
+                                                    : coef_final(mag[t4] | ((sgn >> (3 - t4)) & 1u) << 15, gc, gt, g.qpih);
                         }
                     }
                     dbit += tot;
@@ -853,16 +896,18 @@ __global__ void __launch_bounds__(WAVE) k_dec_slice(const JxsDecFrame* __restric
                     __syncthreads(); /* the coefficients above were written by other lanes */
                     for (uint32_t x0 = 0; x0 < bd.w; x0 += WAVE) {
                         const uint32_t x = x0 + lane;
-                        const int16_t v = x < bd.w ? rowp[x] : (int16_t)0;
-                        const uint32_t nz = v != 0 ? 1u : 0u;
+                        const uint32_t u = x < bd.w ? (uint16_t)rowp[x] : 0u;
+                        const uint32_t nz = u != 0 ? 1u : 0u;
                         const uint32_t incl = wave_incl_scan(nz);
                         const uint32_t tot = __shfl(incl, WAVE - 1, WAVE);
                         if (nz) {
                             const uint64_t bp = nbit + incl - 1;
+                            uint32_t s1 = 0;
                             if (bp >= nxt * 8)
                                 bad = true;
-                            else if (cs_bits(F, bp, 1))
-                                rowp[x] = (int16_t)-v;
+                            else
+                                s1 = cs_bits(F, bp, 1);
+                            rowp[x] = coef_final(u | s1 << 15, gsh[base - bd.gw + x / 4], gt, g.qpih);
                         }
                         nbit += tot;
                     }
@@ -886,16 +931,18 @@ __global__ void __launch_bounds__(WAVE) k_dec_slice(const JxsDecFrame* __restric
 /* ------------------------------------------------------------------ */
 /* inverse transform                                                   */
 /* ------------------------------------------------------------------ */
-__device__ __forceinline__ uint32_t nlt_out(int32_t v, uint32_t depth) {
-    const int32_t dz = 20 - (int32_t)depth;
-    int64_t x = (int64_t)v + (1 << 19);
+/* Linear output scaling (NltDec.c:22-60): Bw-bit coefficients to depth-bit
+ * samples, rounded and clamped. */
+__device__ __forceinline__ uint32_t nlt_out(int32_t v, uint32_t depth, uint32_t bw) {
+    const int32_t dz = (int32_t)bw - (int32_t)depth;
+    int64_t x = (int64_t)v + ((1ll << bw) >> 1);
     x = (x + ((1ll << dz) >> 1)) >> dz;
     const int64_t m = (1ll << depth) - 1;
     return (uint32_t)(x < 0 ? 0 : (x > m ? m : x));
 }
 
-__device__ __forceinline__ void store_out(void* out, size_t idx, int32_t v, uint32_t depth) {
-    const uint32_t o = nlt_out(v, depth);
+__device__ __forceinline__ void store_out(void* out, size_t idx, int32_t v, uint32_t depth, uint32_t bw) {
+    const uint32_t o = nlt_out(v, depth, bw);
     if (depth <= 8)
         ((uint8_t*)out)[idx] = (uint8_t)o;
     else
@@ -946,7 +993,7 @@ __global__ void __launch_bounds__(256) k_inv_rows(const JxsInvRowJob* __restrict
 #pragma unroll
         for (int j = 0; j < NL; j++)
             if (tid + 256u * j < nL)
-                vl[j] = gld(lf16 + tid + 256u * j); /* scaled by 256 when stored */
+                vl[j] = gld(lf16 + tid + 256u * j); /* scaled by 2^Fq when stored */
     }
     /* high band of level k: floor(n[k] / 2) <= JXS_MAX_ROW >> (k + 1) samples */
     constexpr int NH0 = ((JXS_MAX_ROW >> 1) + 255) / 256, NH1 = ((JXS_MAX_ROW >> 2) + 255) / 256,
@@ -965,6 +1012,7 @@ __global__ void __launch_bounds__(256) k_inv_rows(const JxsInvRowJob* __restrict
                 arr[j] = (int32_t)gld(hp + i);
         }
     };
+    const int32_t fm = 1 << J.fq; /* LSHIFT32(coefficient, Fq), Decoder.c:789 */
     ldh(h0, std::integral_constant<int, NH0>{}, 0);
     ldh(h1, std::integral_constant<int, NH1>{}, 1);
     ldh(h2, std::integral_constant<int, NH2>{}, 2);
@@ -973,7 +1021,7 @@ __global__ void __launch_bounds__(256) k_inv_rows(const JxsInvRowJob* __restrict
 #pragma unroll
     for (int j = 0; j < NL; j++)
         if (tid + 256u * j < nL)
-            buf[(tid + 256u * j) << L] = lf32 ? vl[j] : vl[j] * 256;
+            buf[(tid + 256u * j) << L] = lf32 ? vl[j] : vl[j] * fm;
     auto sth = [&](const auto& arr, auto cnt, uint32_t k) {
         constexpr int N = decltype(cnt)::value;
         const uint32_t nh = k < L ? n[k] / 2 : 0u;
@@ -981,7 +1029,7 @@ __global__ void __launch_bounds__(256) k_inv_rows(const JxsInvRowJob* __restrict
         for (int j = 0; j < N; j++) {
             const uint32_t i = tid + 256u * j;
             if (i < nh)
-                buf[(2 * i + 1) << k] = arr[j] * 256;
+                buf[(2 * i + 1) << k] = arr[j] * fm;
         }
     };
     sth(h0, std::integral_constant<int, NH0>{}, 0);
@@ -1028,8 +1076,8 @@ __global__ void __launch_bounds__(256) k_inv_rows(const JxsInvRowJob* __restrict
         if (b8 ? (((uintptr_t)((uint8_t*)J.out + r0) & 3) == 0) : (((uintptr_t)((uint16_t*)J.out + r0) & 7) == 0)) {
             for (uint32_t q = threadIdx.x; q < n4; q += blockDim.x) {
                 const i32x4 v = *(const i32x4*)&buf[4 * q];
-                const uint32_t a = nlt_out(v.x, J.depth), b = nlt_out(v.y, J.depth), c = nlt_out(v.z, J.depth),
-                               d = nlt_out(v.w, J.depth);
+                const uint32_t a = nlt_out(v.x, J.depth, J.bw), b = nlt_out(v.y, J.depth, J.bw),
+                               c = nlt_out(v.z, J.depth, J.bw), d = nlt_out(v.w, J.depth, J.bw);
                 if (b8)
                     *(GAS uint32_t*)((uint8_t*)J.out + r0 + 4 * q) = a | b << 8 | c << 16 | d << 24;
                 else
@@ -1038,7 +1086,7 @@ __global__ void __launch_bounds__(256) k_inv_rows(const JxsInvRowJob* __restrict
             done = 4 * n4;
         }
         for (uint32_t i = done + threadIdx.x; i < w; i += blockDim.x)
-            store_out(J.out, r0 + i, buf[i], J.depth);
+            store_out(J.out, r0 + i, buf[i], J.depth, J.bw);
     }
 }
 
@@ -1082,9 +1130,9 @@ __global__ void __launch_bounds__(256) k_inv_vert(const JxsInvVertJob* __restric
             if (has_odd)
                 J.out32[(size_t)(2 * i + 1) * J.w + x] = odd;
         } else {
-            store_out(J.out, (size_t)(2 * i) * J.out_stride + x, e, J.depth);
+            store_out(J.out, (size_t)(2 * i) * J.out_stride + x, e, J.depth, J.bw);
             if (has_odd)
-                store_out(J.out, (size_t)(2 * i + 1) * J.out_stride + x, odd, J.depth);
+                store_out(J.out, (size_t)(2 * i + 1) * J.out_stride + x, odd, J.depth, J.bw);
         }
         e = e_next;
     }

@@ -40,42 +40,6 @@ static bool dec_version_ok(uint64_t maj, uint64_t min) {
     return !(maj > SVT_JPEGXS_API_VER_MAJOR || (maj == SVT_JPEGXS_API_VER_MAJOR && min > SVT_JPEGXS_API_VER_MINOR));
 }
 
-/* Codestream size of the frame starting at cs: header, every slice and the
- * EOC marker, following the precinct lengths (DecThreadInit.c:35-131). */
-static int32_t frame_size(const uint8_t* cs, size_t len, HostGeom& hg, uint32_t* out_size,
-                          std::vector<uint32_t>* prec_off) {
-    uint32_t hdr_end = 0;
-    int32_t e = parse_codestream_header(cs, len, hg, &hdr_end);
-    if (e)
-        return e;
-    const JxsGeom& g = hg.g;
-    uint64_t pos = hdr_end;
-    if (prec_off)
-        prec_off->assign(g.rows, 0);
-    for (uint32_t s = 0; s < g.slices; s++) {
-        if (pos + 6 > len)
-            return SvtJxsErrorDecoderBitstreamTooShort;
-        if (cs[pos] != 0xFF || cs[pos + 1] != 0x20 || ((cs[pos + 4] << 8) | cs[pos + 5]) != s)
-            return SvtJxsErrorDecoderInvalidBitstream;
-        pos += 6;
-        const uint32_t first = s * g.pps, n = (s == g.slices - 1) ? g.rows - first : g.pps;
-        for (uint32_t i = 0; i < n; i++) {
-            if (pos + g.prec_hdr_bytes > len)
-                return SvtJxsErrorDecoderBitstreamTooShort;
-            if (prec_off)
-                (*prec_off)[first + i] = (uint32_t)pos;
-            const uint32_t lprc = ((uint32_t)cs[pos] << 16) | ((uint32_t)cs[pos + 1] << 8) | cs[pos + 2];
-            pos += g.prec_hdr_bytes + (uint64_t)lprc;
-        }
-    }
-    if (pos + 2 > len)
-        return SvtJxsErrorDecoderBitstreamTooShort;
-    if (cs[pos] != 0xFF || cs[pos + 1] != 0x11)
-        return SvtJxsErrorDecoderInvalidBitstream;
-    *out_size = (uint32_t)(pos + 2);
-    return SvtJxsErrorNone;
-}
-
 /* Inverse-transform jobs of one frame (stage 0: rows of the second vertical
  * level, 1: its columns, 2: rows of the first level or of components without
  * vertical levels, 3: columns of the first level). */
@@ -84,8 +48,8 @@ struct InvJobs {
     std::vector<JxsInvVertJob> vert[2];
 };
 
-static void inv_jobs(const JxsGeom& g, int16_t* coef, int32_t* scratch, void* const planes[4], const uint32_t strides[4],
-                     InvJobs& J) {
+static void inv_jobs(const JxsGeom& g, uint32_t bw, uint32_t fq, int16_t* coef, int32_t* scratch, void* const planes[4],
+                     const uint32_t strides[4], InvJobs& J) {
     int32_t* scr = scratch;
     for (uint32_t c = 0; c < g.nc; c++) {
         const JxsComp& cp = g.c[c];
@@ -96,6 +60,8 @@ static void inv_jobs(const JxsGeom& g, int16_t* coef, int32_t* scratch, void* co
             j.w = w;
             j.rows = rows;
             j.depth = g.depth;
+            j.fq = fq;
+            j.bw = bw;
             return j;
         };
         const uint32_t nb = cp.nb;
@@ -162,6 +128,7 @@ static void inv_jobs(const JxsGeom& g, int16_t* coef, int32_t* scratch, void* co
             v2.h = h1;
             v2.out_stride = w1;
             v2.depth = g.depth;
+            v2.bw = bw;
             J.vert[0].push_back(v2);
             lo.lf32 = B;
             lo.lf_stride = w1;
@@ -189,6 +156,7 @@ static void inv_jobs(const JxsGeom& g, int16_t* coef, int32_t* scratch, void* co
         v1.h = h;
         v1.out_stride = strides[c];
         v1.depth = g.depth;
+        v1.bw = bw;
         J.vert[1].push_back(v1);
     }
 }
@@ -307,10 +275,11 @@ static svt_jpeg_xs_amd_decode_plan_t* dplan_build(const std::vector<DecSpec>& sp
         F.npk_dec = s.geom->npk_dec;
         F.prec_in = s.prec_in;
         F.prec_in_stride = s.prec_stride ? s.prec_stride : 1;
+        F.rows_valid = g.rows;
         base += g.rows;
         p->max_npk = s.geom->npk_dec > p->max_npk ? s.geom->npk_dec : p->max_npk;
         InvJobs one;
-        inv_jobs(s.geom->out, F.coef, scratch, s.planes, s.strides, one);
+        inv_jobs(s.geom->out, s.geom->host.bw, s.geom->host.fq, F.coef, scratch, s.planes, s.strides, one);
         for (int k = 0; k < 2; k++) {
             J.rows[k].insert(J.rows[k].end(), one.rows[k].begin(), one.rows[k].end());
             J.vert[k].insert(J.vert[k].end(), one.vert[k].begin(), one.vert[k].end());
@@ -401,8 +370,21 @@ static int32_t apply_proxy(DecGeom& dg, uint32_t proxy) {
     return SvtJxsErrorNone;
 }
 
-static int32_t geom_from_codestream(const uint8_t* cs, size_t len, DecGeom& dg, uint32_t proxy) {
-    int32_t e = parse_codestream_header(cs, len, dg.host, &dg.hdr_end);
+/* A decoder's geometry from its first codestream, with the reference's
+ * error order: header (ParseHeader.c), layout and proxy rules (Pi.c:189-405,
+ * 567-626; Decoder.c:133-139), then what this decoder does not decode. */
+static int32_t geom_from_codestream(const uint8_t* cs, size_t len, DecGeom& dg, uint32_t proxy, CsHeader& h,
+                                    bool packet_mode) {
+    uint32_t used = 0;
+    int32_t e = cs_get_header(cs, len, h, &used);
+    if (e)
+        return e;
+    /* packet input needs the frame size from the header (DecHandle.c:197-204) */
+    if (packet_mode && h.d.lcod == 0)
+        return SvtJxsErrorBadParameter;
+    e = cs_pi_check(h, proxy);
+    if (!e)
+        e = geom_from_header(h, used, dg.host, &dg.hdr_end);
     if (e)
         return e;
     for (uint32_t c = 0; c < dg.host.g.nc; c++)
@@ -458,22 +440,28 @@ struct DecSlot {
     void* d_planes = nullptr;
     JxsDecFrame* h_frame = nullptr; /* page-locked copy of the plan's frame descriptor */
     JxsGeom* h_geom = nullptr;      /* page-locked copy of the slot's geometry */
-    uint32_t dyn[3] = {0, 0, 0};    /* Qpih, Fs, Rm in the slot's device geometry */
+    uint32_t dyn[4] = {0, 0, 0, 0}; /* Qpih, Fs, Rm, short packet headers in the slot's device geometry */
+    std::vector<uint32_t> slice_off;
     svt_jpeg_xs_amd_decode_plan_t* plan = nullptr;
     hipEvent_t done = nullptr;
     uint32_t* h_status = nullptr;
     bool busy = false;
+    bool made = false; /* buffers allocated (slots past the first are made on first use) */
 };
 
 struct DecJob {
     int slot; /* -1: the frame failed before reaching the device */
     svt_jpeg_xs_frame_t frame;
     SvtJxsErrorType_t err;
+    /* the error of the first slice the device did not decode (0: it decoded
+     * them all); it stands unless a decoded slice failed first */
+    SvtJxsErrorType_t tail = SvtJxsErrorNone;
 };
 
 struct DecoderPriv {
     svt_jpeg_xs_decoder_api_t* api = nullptr;
     DecGeom geom;
+    CsHeader hdr; /* the first codestream's header: later frames must share its configuration */
     uint32_t verbose = 0;
     bool packet_mode = false;
     size_t plane_off[4] = {0, 0, 0, 0};
@@ -492,6 +480,7 @@ struct DecoderPriv {
     int pk_slot = -1;
     uint32_t pk_filled = 0;
     svt_jpeg_xs_frame_t pk_frame;
+    uint32_t slot_cap = 0; /* codestream bytes a new slot holds */
 };
 
 static void slot_release_cs(DecSlot& s) {
@@ -547,11 +536,14 @@ static bool slot_bind(DecoderPriv* d, DecSlot& s, uint32_t cap) {
 
 static bool slot_create(DecoderPriv* d, DecSlot& s, uint32_t cap) {
     const JxsGeom& g = d->geom.host.g;
+    s.made = true;
     s.geom = d->geom;
     s.geom.d_geom = nullptr;
     s.dyn[0] = g.qpih;
     s.dyn[1] = g.fs;
     s.dyn[2] = g.rm;
+    s.dyn[3] = g.short_hdr;
+    s.slice_off.assign(g.slices, 0);
     if (hipMalloc(&s.geom.d_geom, sizeof(JxsGeom)) != hipSuccess ||
         hipMemcpy(s.geom.d_geom, &g, sizeof(JxsGeom), hipMemcpyHostToDevice) != hipSuccess ||
         hipMalloc(&s.d_prec, (size_t)g.rows * 4) != hipSuccess || hipMalloc(&s.d_planes, d->plane_bytes) != hipSuccess ||
@@ -598,6 +590,8 @@ static void finisher_loop(DecoderPriv* d) {
                 j.err = SvtJxsErrorDecoderInternal;
             else if (*s.h_status)
                 j.err = SvtJxsErrorDecoderInvalidBitstream;
+            else if (j.tail)
+                j.err = j.tail;
             else {
                 for (uint32_t c = 0; c < og.nc && he == hipSuccess; c++)
                     he = hipMemcpy2DAsync(j.frame.image.data_yuv[c], (size_t)j.frame.image.stride[c] * pix,
@@ -643,14 +637,27 @@ static void decoder_destroy(DecoderPriv* d) {
     delete d;
 }
 
-/* A free slot (waiting for one when blocking), marked busy; -1 if none. */
+/* A free slot (waiting for one when blocking), marked busy; -1 if none.
+ * A slot not made yet is made when every made slot is busy, so a decoder
+ * that never has two frames in flight holds one slot. */
 static int acquire_slot(DecoderPriv* d, bool blocking) {
     std::unique_lock<std::mutex> lk(d->mu);
     for (;;) {
         for (int i = 0; i < (int)d->slots.size(); i++)
-            if (!d->slots[i].busy) {
+            if (d->slots[i].made && !d->slots[i].busy) {
                 d->slots[i].busy = true;
                 return i;
+            }
+        for (int i = 0; i < (int)d->slots.size(); i++)
+            if (!d->slots[i].made) {
+                d->slots[i].busy = true;
+                lk.unlock();
+                const bool ok = slot_create(d, d->slots[i], d->slot_cap);
+                lk.lock();
+                if (ok)
+                    return i;
+                slot_release(d->slots[i]); /* back to not made; another slot frees up in time */
+                break;
             }
         if (!blocking)
             return -1;
@@ -675,36 +682,45 @@ static void enqueue(DecoderPriv* d, const DecJob& j) {
     d->cv_job.notify_all();
 }
 
-/* The frame's header against the decoder's: a different static
- * configuration is SvtJxsErrorDecoderConfigChange (Decoder.c:388-390); Qpih,
- * Fs and Rm may change from frame to frame (picture_header_dynamic_t,
- * Pi.h:63-80; Decoder.c:576, Packing.c:616, 829). */
-static int32_t check_frame_header(const DecoderPriv* d, const JxsGeom& fg) {
-    const JxsGeom& g = d->geom.host.g;
-    if (fg.w != g.w || fg.h != g.h || fg.nc != g.nc || fg.depth != g.depth || fg.dv != g.dv || fg.dh != g.dh ||
-        fg.pps != g.pps || fg.sig != g.sig || fg.short_hdr != g.short_hdr || memcmp(fg.c, g.c, sizeof(g.c)) != 0)
-        return SvtJxsErrorDecoderConfigChange;
+/* A frame's per-frame coding flags (picture_header_dynamic_t, Pi.h:63-80:
+ * Qpih, Fs, Rm, Lh may change from frame to frame; Decoder.c:576, Packing.c:
+ * 532, 616, 829) over the decoder's geometry.  Precisions and non-linear
+ * transforms this decoder does not decode are SvtJxsErrorDecoderInvalidBitstream. */
+static int32_t frame_flags(const DecoderPriv* d, const CsHeader& h, JxsGeom& fg) {
+    if (h.d.tnlt != 0 || (h.d.bw != 20 && h.d.bw != 18))
+        return SvtJxsErrorDecoderInvalidBitstream;
+    fg.qpih = h.d.qpih;
+    fg.fs = h.d.fs;
+    fg.rm = h.d.rm;
+    const bool short_ok = ((uint64_t)d->hdr.c.w * d->hdr.c.nc) < 32752;
+    fg.short_hdr = (short_ok && !h.d.lh) ? 1u : 0u;
     return SvtJxsErrorNone;
 }
 
-/* Queues the decode of the codestream staged in slot si (fsize bytes,
- * precinct offsets in h_prec, Qpih qpih) into frame's image buffer. */
+/* Queues the decode of the first rows_valid precinct rows of the codestream
+ * staged in slot si (fsize bytes, precinct offsets in h_prec, per-frame flags
+ * in fg) into frame's image buffer; `tail` is the error of the rows after
+ * them. */
 static SvtJxsErrorType_t launch_slot(DecoderPriv* d, int si, uint32_t fsize, const JxsGeom& fg,
-                                     const svt_jpeg_xs_frame_t& frame) {
+                                     const svt_jpeg_xs_frame_t& frame, uint32_t rows_valid, SvtJxsErrorType_t tail) {
     DecSlot& s = d->slots[si];
     const JxsGeom& g = d->geom.host.g;
     hipError_t he = hipSuccess;
-    if (fg.qpih != s.dyn[0] || fg.fs != s.dyn[1] || fg.rm != s.dyn[2]) {
+    if (fg.qpih != s.dyn[0] || fg.fs != s.dyn[1] || fg.rm != s.dyn[2] || fg.short_hdr != s.dyn[3]) {
         *s.h_geom = g;
         s.h_geom->qpih = fg.qpih;
         s.h_geom->fs = fg.fs;
         s.h_geom->rm = fg.rm;
+        s.h_geom->short_hdr = fg.short_hdr;
+        s.h_geom->pkt_hdr_bytes = fg.short_hdr ? 5 : 7;
         he = hipMemcpyAsync(s.geom.d_geom, s.h_geom, sizeof(JxsGeom), hipMemcpyHostToDevice, d->stream);
         s.dyn[0] = fg.qpih;
         s.dyn[1] = fg.fs;
         s.dyn[2] = fg.rm;
+        s.dyn[3] = fg.short_hdr;
     }
     s.h_frame->cs_len = fsize;
+    s.h_frame->rows_valid = rows_valid;
     if (he == hipSuccess)
         he = hipMemcpyAsync(s.plan->d_frames, s.h_frame, sizeof(JxsDecFrame), hipMemcpyHostToDevice, d->stream);
     if (he == hipSuccess)
@@ -722,7 +738,9 @@ static SvtJxsErrorType_t launch_slot(DecoderPriv* d, int si, uint32_t fsize, con
         free_slot(d, si);
         return SvtJxsErrorDecoderInternal;
     }
-    enqueue(d, DecJob{si, frame, SvtJxsErrorNone});
+    DecJob j{si, frame, SvtJxsErrorNone};
+    j.tail = tail;
+    enqueue(d, j);
     return SvtJxsErrorNone;
 }
 
@@ -741,23 +759,56 @@ static SvtJxsErrorType_t check_image(const DecoderPriv* d, const svt_jpeg_xs_ima
     return SvtJxsErrorNone;
 }
 
-/* Parses the whole codestream staged in slot si and queues its decode; a
- * codestream that does not parse is published with its error instead. */
+/* The frame staged in slot si (len bytes), as the reference's init thread
+ * takes it (DecThreadInit.c:291-338, Decoder.c:377-396, 509-531): its header
+ * and configuration, then each slice's precinct walk; the slices before the
+ * first one that fails go to the device, and the frame's error is that of
+ * the first failing slice in slice order.  A frame that fails before any
+ * slice takes no device time and is handed back with its error. */
 static SvtJxsErrorType_t submit_staged(DecoderPriv* d, int si, uint32_t len, const svt_jpeg_xs_frame_t& frame) {
     DecSlot& s = d->slots[si];
-    HostGeom hg;
-    uint32_t fsize = 0;
-    std::vector<uint32_t> prec;
-    int32_t e = frame_size(s.h_cs, len, hg, &fsize, &prec);
+    const JxsGeom& g = d->geom.host.g;
+    CsHeader h;
+    uint32_t used = 0;
+    JxsGeom fg = g;
+    int32_t e = cs_get_header(s.h_cs, len, h, &used);
+    if (!e && !cs_same_config(h, d->hdr))
+        e = SvtJxsErrorDecoderConfigChange;
     if (!e)
-        e = check_frame_header(d, hg.g);
-    if (e) {
+        e = frame_flags(d, h, fg);
+    /* Bw and Fq enter the inverse transform jobs: a frame that changes them
+     * rebinds the slot's plan */
+    if (!e && (h.d.bw != s.geom.host.bw || h.d.fq != s.geom.host.fq)) {
+        s.geom.host.bw = h.d.bw;
+        s.geom.host.fq = h.d.fq;
+        if (!slot_bind(d, s, s.cap)) {
+            free_slot(d, si);
+            enqueue(d, DecJob{-1, frame, SvtJxsErrorInsufficientResources});
+            return SvtJxsErrorNone;
+        }
+    }
+    uint32_t bad = 0, end = 0;
+    if (!e) {
+        e = cs_slice_walk(s.h_cs, len, used, g.slices, g.pps, g.rows, g.nb_exist, s.h_prec, s.slice_off.data(), &end,
+                          &bad);
+        /* a slice header whose length field is not 4 fails that slice's decode */
+        for (uint32_t sl = 0; sl < bad; sl++)
+            if (cs_slice_header_len(s.h_cs, s.slice_off[sl]) != 4) {
+                bad = sl;
+                e = SvtJxsErrorDecoderInvalidBitstream;
+                break;
+            }
+        if (bad == 0 && !e)
+            e = SvtJxsErrorDecoderInvalidBitstream; /* no slice (not reached: a picture has one) */
+    }
+    if (e && bad == 0) {
         free_slot(d, si);
         enqueue(d, DecJob{-1, frame, (SvtJxsErrorType_t)e});
         return SvtJxsErrorNone;
     }
-    memcpy(s.h_prec, prec.data(), prec.size() * 4);
-    return launch_slot(d, si, fsize, hg.g, frame);
+    const uint32_t rows_valid = std::min(bad * g.pps, g.rows);
+    const uint32_t dev_len = bad < g.slices ? s.slice_off[bad] : end;
+    return launch_slot(d, si, dev_len, fg, frame, rows_valid, (SvtJxsErrorType_t)e);
 }
 
 }  // namespace jxs
@@ -766,31 +817,13 @@ using namespace jxs;
 
 extern "C" {
 
+/* ParseHeader.c:652-889: the frame's size from its markers and precinct
+ * lengths (Lcod with fast_search), without decoding. */
 PREFIX_API SvtJxsErrorType_t svt_jpeg_xs_decoder_get_single_frame_size_with_proxy(
     const uint8_t* bitstream_buf, size_t bitstream_buf_size, svt_jpeg_xs_image_config_t* out_image_config,
     uint32_t* frame_size_out, uint32_t fast_search, proxy_mode_t proxy_mode) {
-    if (!bitstream_buf || !frame_size_out)
-        return SvtJxsErrorDecoderInvalidPointer;
-    DecGeom dg;
-    HostGeom& hg = dg.host;
-    uint32_t hdr_end = 0;
-    int32_t e = parse_codestream_header(bitstream_buf, bitstream_buf_size, hg, &hdr_end);
-    if (e)
-        return (SvtJxsErrorType_t)e;
-    e = apply_proxy(dg, (uint32_t)proxy_mode);
-    if (e)
-        return (SvtJxsErrorType_t)e;
-    if (fast_search && hg.g.lcod > 0) {
-        /* constant-bit-rate codestreams carry their size in Lcod (PIH) */
-        *frame_size_out = hg.g.lcod;
-    } else {
-        e = frame_size(bitstream_buf, bitstream_buf_size, hg, frame_size_out, nullptr);
-        if (e)
-            return (SvtJxsErrorType_t)e;
-    }
-    if (out_image_config)
-        fill_image_config(dg.out, out_image_config);
-    return SvtJxsErrorNone;
+    return (SvtJxsErrorType_t)cs_frame_size(bitstream_buf, bitstream_buf_size, out_image_config, frame_size_out,
+                                            fast_search, (uint32_t)proxy_mode);
 }
 
 PREFIX_API SvtJxsErrorType_t svt_jpeg_xs_decoder_get_single_frame_size(const uint8_t* bitstream_buf,
@@ -806,10 +839,10 @@ PREFIX_API SvtJxsErrorType_t svt_jpeg_xs_decoder_init(uint64_t version_api_major
                                                       size_t codestream_size, svt_jpeg_xs_image_config_t* out_image_config) {
     if (!dec_version_ok(version_api_major, version_api_minor))
         return SvtJxsErrorInvalidApiVersion;
-    if (!dec_api || !bitstream_buf)
+    if (!dec_api || !bitstream_buf || codestream_size == 0)
         return SvtJxsErrorDecoderInvalidPointer;
     dec_api->private_ptr = nullptr;
-    if ((uint32_t)dec_api->proxy_mode >= proxy_mode_max || dec_api->packetization_mode > 1)
+    if (dec_api->packetization_mode > 1 || (uint32_t)dec_api->proxy_mode >= proxy_mode_max)
         return SvtJxsErrorBadParameter;
     auto* d = new (std::nothrow) DecoderPriv;
     if (!d)
@@ -817,17 +850,13 @@ PREFIX_API SvtJxsErrorType_t svt_jpeg_xs_decoder_init(uint64_t version_api_major
     d->api = dec_api;
     d->verbose = dec_api->verbose;
     d->packet_mode = dec_api->packetization_mode == 1;
-    int32_t e = geom_from_codestream(bitstream_buf, codestream_size, d->geom, (uint32_t)dec_api->proxy_mode);
+    int32_t e = geom_from_codestream(bitstream_buf, codestream_size, d->geom, (uint32_t)dec_api->proxy_mode, d->hdr,
+                                     d->packet_mode);
     if (e) {
         delete d;
         return (SvtJxsErrorType_t)e;
     }
     const JxsGeom& g = d->geom.host.g;
-    /* packet input needs the frame size from the header (DecHandle.c:197-204) */
-    if (d->packet_mode && g.lcod == 0) {
-        delete d;
-        return SvtJxsErrorBadParameter;
-    }
     const JxsGeom& og = d->geom.out;
     const uint32_t pix = g.depth <= 8 ? 1 : 2;
     uint64_t planes = 0;
@@ -837,10 +866,13 @@ PREFIX_API SvtJxsErrorType_t svt_jpeg_xs_decoder_init(uint64_t version_api_major
     }
     d->plane_bytes = planes;
     /* a constant-rate stream carries its frame size (Lcod); otherwise start
-     * from the first codestream's size and grow a slot when a frame needs it */
-    uint32_t cap = g.lcod ? g.lcod : (uint32_t)codestream_size;
-    if (cap == 0)
-        cap = 1;
+     * from the first codestream's size.  Lcod is a header field: a damaged one
+     * must not size the staging, which is bounded by twice the raw picture
+     * (a codestream never needs more); a slot grows when a frame needs it */
+    const uint64_t raw = (uint64_t)g.w * g.h * g.nc * (g.depth <= 8 ? 1u : 2u);
+    const uint64_t bound = 2 * raw + 65536;
+    uint64_t cap = g.lcod ? std::min<uint64_t>(g.lcod, bound) : (uint64_t)codestream_size;
+    cap = std::min<uint64_t>(std::max<uint64_t>(cap, 1), 0xFFFFFFFFull);
     if (hipMalloc(&d->geom.d_geom, sizeof(JxsGeom)) != hipSuccess ||
         hipMemcpy(d->geom.d_geom, &g, sizeof(JxsGeom), hipMemcpyHostToDevice) != hipSuccess ||
         hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking) != hipSuccess ||
@@ -849,11 +881,11 @@ PREFIX_API SvtJxsErrorType_t svt_jpeg_xs_decoder_init(uint64_t version_api_major
         return SvtJxsErrorInsufficientResources;
     }
     d->slots.resize(kDecSlots);
-    for (auto& s : d->slots)
-        if (!slot_create(d, s, cap)) {
-            decoder_destroy(d);
-            return SvtJxsErrorInsufficientResources;
-        }
+    d->slot_cap = (uint32_t)cap;
+    if (!slot_create(d, d->slots[0], (uint32_t)cap)) {
+        decoder_destroy(d);
+        return SvtJxsErrorInsufficientResources;
+    }
     try {
         d->finisher = std::thread(finisher_loop, d);
     } catch (...) {
@@ -897,22 +929,14 @@ PREFIX_API SvtJxsErrorType_t svt_jpeg_xs_decoder_send_frame(svt_jpeg_xs_decoder_
     if (si < 0)
         return SvtJxsErrorNoErrorEmptyQueue;
     DecSlot& s = d->slots[si];
-    /* the frame's size from its precinct chain; a longer buffer holds more
-     * than the frame, and only the frame is staged */
-    HostGeom hg;
-    uint32_t fsize = 0;
-    int32_t pe = frame_size(dec_input->bitstream.buffer, len, hg, &fsize, nullptr);
-    if (!pe && fsize > s.cap && !slot_bind(d, s, fsize)) {
+    /* the whole buffer is staged: the frame is found in it as the decode
+     * walks it, and only the frame's bytes go to the device */
+    if (len > s.cap && !slot_bind(d, s, (uint32_t)len)) {
         free_slot(d, si);
         return SvtJxsErrorInsufficientResources;
     }
-    if (pe) {
-        free_slot(d, si);
-        enqueue(d, DecJob{-1, *dec_input, (SvtJxsErrorType_t)pe});
-        return SvtJxsErrorNone;
-    }
-    memcpy(s.h_cs, dec_input->bitstream.buffer, fsize);
-    return submit_staged(d, si, fsize, *dec_input);
+    memcpy(s.h_cs, dec_input->bitstream.buffer, len);
+    return submit_staged(d, si, (uint32_t)len, *dec_input);
 }
 
 /* Packet input (DecThreadInit.c:347-460): a call takes at most the rest of
@@ -938,6 +962,13 @@ PREFIX_API SvtJxsErrorType_t svt_jpeg_xs_decoder_send_packet(svt_jpeg_xs_decoder
             return e;
         d->pk_slot = acquire_slot(d, true);
         d->pk_filled = 0;
+        /* a frame of Lcod bytes is staged whole */
+        DecSlot& ns = d->slots[d->pk_slot];
+        if (d->geom.host.g.lcod > ns.cap && !slot_bind(d, ns, d->geom.host.g.lcod)) {
+            free_slot(d, d->pk_slot);
+            d->pk_slot = -1;
+            return SvtJxsErrorInsufficientResources;
+        }
         d->pk_frame = *dec_input;
         memset(&d->pk_frame.bitstream, 0, sizeof(d->pk_frame.bitstream));
     }
@@ -949,20 +980,14 @@ PREFIX_API SvtJxsErrorType_t svt_jpeg_xs_decoder_send_packet(svt_jpeg_xs_decoder
     *bytes_used = n;
     dec_input->bitstream.ready_to_release = 1;
     dec_input->image.ready_to_release = 0;
-    HostGeom hg;
     uint32_t fsize = 0;
-    const int32_t e = frame_size(s.h_cs, d->pk_filled, hg, &fsize, nullptr);
+    const int32_t e = cs_frame_size(s.h_cs, d->pk_filled, nullptr, &fsize, 0, proxy_mode_full);
     if (e == SvtJxsErrorDecoderBitstreamTooShort && d->pk_filled < lcod)
         return SvtJxsErrorDecoderBitstreamTooShort;
     const int si = d->pk_slot;
     const uint32_t filled = d->pk_filled;
     d->pk_slot = -1;
     d->pk_filled = 0;
-    if (e) {
-        free_slot(d, si);
-        enqueue(d, DecJob{-1, d->pk_frame, (SvtJxsErrorType_t)e});
-        return SvtJxsErrorNone;
-    }
     return submit_staged(d, si, filled, d->pk_frame);
 }
 

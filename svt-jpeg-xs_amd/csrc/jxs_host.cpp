@@ -669,116 +669,76 @@ int32_t build_encoder_geometry(const EncParams& p, HostGeom& out, int verbose) {
 }
 
 /* ---------------- decoder header ---------------- */
-struct BitIn {
-    const uint8_t* m;
-    uint64_t bit, end;
-    bool err = false;
-    uint32_t get(uint32_t n) {
-        uint32_t v = 0;
-        for (uint32_t i = 0; i < n; i++, bit++) {
-            if (bit >= end) {
-                err = true;
-                return 0;
-            }
-            v = (v << 1) | ((m[bit >> 3] >> (7 - (bit & 7))) & 1);
-        }
-        return v;
-    }
-};
-
-int32_t parse_codestream_header(const uint8_t* cs, size_t len, HostGeom& out, uint32_t* header_end) {
+int32_t geom_from_header(const CsHeader& h, uint32_t header_end, HostGeom& out, uint32_t* header_end_out) {
     JxsGeom& g = out.g;
     memset(&g, 0, sizeof(g));
-    BitIn r{cs, 0, (uint64_t)len * 8};
-    if (len < 4)
-        return E_TOO_SHORT;
-    if (r.get(16) != 0xFF10)
+    const CsHeader::Const& c = h.c;
+    const CsHeader::Dyn& d = h.d;
+    /* what this decoder does not decode: components without decomposition
+     * (CWD), colour transforms (Cpih), non-linear transforms (NLT), and
+     * coefficient precisions other than Bw 18 or 20 */
+    if (c.sd != 0 || c.cpih != 0 || d.tnlt != 0 || (d.bw != 20 && d.bw != 18))
         return E_INVALID_BS;
-    bool pih = false, cdt = false, wgt = false;
-    uint32_t W = 0, H = 0, nc = 0, dh = 0, dv = 0, hsl = 0, cw = 0, lh = 0, bw = 0, fq = 0, nw = 0;
-    uint32_t sx[4] = {1, 1, 1, 1}, sy[4] = {1, 1, 1, 1};
-    uint8_t gains[JXS_NGB], prios[JXS_NGB];
-    for (;;) {
-        if (r.bit + 16 > r.end)
-            return E_TOO_SHORT;
-        const uint32_t marker = r.get(16);
-        if (marker == 0xFF20) {
-            r.bit -= 16;
-            break;
-        }
-        const uint32_t seglen = r.get(16);
-        const uint64_t start = r.bit - 16;
-        switch (marker) {
-        case 0xFF12:
-            g.lcod = r.get(32);
-            r.get(32);
-            W = r.get(16);
-            H = r.get(16);
-            cw = r.get(16);
-            hsl = r.get(16);
-            nc = r.get(8);
-            if (r.get(8) != 4 || r.get(8) != 8)
-                return E_INVALID_BS;
-            bw = r.get(8);
-            fq = r.get(4);
-            if (r.get(4) != 4 || r.get(8) != 0)
-                return E_INVALID_BS;
-            dh = r.get(4);
-            dv = r.get(4);
-            lh = r.get(1);
-            r.get(1);
-            g.qpih = r.get(2);
-            g.fs = r.get(2);
-            g.rm = r.get(2);
-            if (g.qpih > 1 || g.fs > 1 || g.rm > 1) /* ParseHeader.c:238-251 */
-                return E_INVALID_BS;
-            pih = true;
-            break;
-        case 0xFF13:
-            for (uint32_t c = 0; c < (seglen - 2) / 2 && c < 4; c++) {
-                g.depth = r.get(8);
-                sx[c] = r.get(4);
-                sy[c] = r.get(4);
-            }
-            cdt = true;
-            break;
-        case 0xFF14:
-            nw = (seglen - 2) / 2;
-            if (nw > JXS_NGB)
-                return E_INVALID_BS;
-            for (uint32_t i = 0; i < nw; i++) {
-                gains[i] = (uint8_t)r.get(8);
-                prios[i] = (uint8_t)r.get(8);
-            }
-            wgt = true;
-            break;
-        case 0xFF50:
-        case 0xFF15:
-            break;
-        default:
+    const uint32_t nc = c.nc;
+    if (nc == 0 || nc > JXS_MAXC)
+        return E_INVALID_BS;
+    /* a precinct width Cw != 0 that still covers the whole picture width is
+     * one precinct column, laid out as Cw = 0 (Pi.c:21-72) */
+    if (c.cw != 0) {
+        uint32_t sxm = 0;
+        for (uint32_t k = 0; k < nc; k++)
+            sxm = c.sx[k] > sxm ? c.sx[k] : sxm;
+        const uint64_t cs = 8ull * c.cw * sxm * (1ull << c.dh);
+        if ((uint64_t)(uint32_t)c.w > cs)
             return E_INVALID_BS;
-        }
-        r.bit = start + 8ull * seglen;
-        if (r.err || r.bit > r.end)
-            return E_TOO_SHORT;
     }
-    if (!pih || !cdt || !wgt || cw != 0 || nc == 0 || nc > 4 || hsl == 0 || bw != 20 || fq != 8)
+    uint32_t sx[4] = {1, 1, 1, 1}, sy[4] = {1, 1, 1, 1};
+    for (uint32_t k = 0; k < nc; k++) {
+        sx[k] = c.sx[k];
+        sy[k] = c.sy[k];
+        if (sx[k] < 1 || sx[k] > 2 || sy[k] < 1 || sy[k] > 2)
+            return E_INVALID_BS;
+    }
+    if (build_layout(g, nc, sx, sy, (uint32_t)c.w, (uint32_t)c.h, (uint32_t)c.dh, (uint32_t)c.dv, c.hsl << c.dv))
         return E_INVALID_BS;
-    if (build_layout(g, nc, sx, sy, W, H, dh, dv, hsl << dv))
-        return E_INVALID_BS;
-    if ((uint32_t)g.short_hdr != (lh ? 0u : 1u) || nw != g.nb_exist)
-        return E_INVALID_BS;
+    g.lcod = d.lcod;
+    g.depth = c.depth[0]; /* the output scaling uses the first component's depth (NltDec.c:196-200) */
+    out.bw = d.bw;
+    out.fq = d.fq;
+    g.qpih = d.qpih;
+    g.fs = d.fs;
+    g.rm = d.rm;
+    /* long packet headers when the frame asks for them (Lh) or the picture is
+     * too wide for short ones (Packing.c:532) */
+    if (d.lh) {
+        g.short_hdr = 0;
+        g.pkt_hdr_bytes = 7;
+    }
+    /* weights in band order; a table shorter than the band count leaves
+     * zeros (Decoder.c:34-49) */
     uint32_t k = 0;
     for (uint32_t gi = 0; gi < g.nb_all; gi++) {
         if (g.gc[gi] == NE)
             continue;
-        g.c[g.gc[gi]].b[g.gb[gi]].gain = gains[k];
-        g.c[g.gc[gi]].b[g.gb[gi]].prio = prios[k];
+        g.c[g.gc[gi]].b[g.gb[gi]].gain = k < CS_MAX_BANDS ? c.gain[k] : 0;
+        g.c[g.gc[gi]].b[g.gb[gi]].prio = k < CS_MAX_BANDS ? c.prio[k] : 0;
         k++;
     }
     assign_buffers(g);
-    *header_end = (uint32_t)(r.bit >> 3);
+    if (header_end_out)
+        *header_end_out = header_end;
     return E_NONE;
+}
+
+int32_t parse_codestream_header(const uint8_t* cs, size_t len, HostGeom& out, uint32_t* header_end) {
+    CsHeader h;
+    uint32_t used = 0;
+    int32_t e = cs_get_header(cs, len, h, &used);
+    if (!e)
+        e = cs_pi_check(h, 0);
+    if (!e)
+        e = geom_from_header(h, used, out, header_end);
+    return e;
 }
 
 /* ---------------- forward-transform jobs ---------------- */

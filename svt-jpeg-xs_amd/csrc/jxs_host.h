@@ -9,6 +9,7 @@
 #include <stdint.h>
 #include <vector>
 
+#include "jxs_codestream.h"
 #include "jxs_geom.h"
 
 namespace jxs {
@@ -28,6 +29,7 @@ struct HostGeom {
     std::vector<uint32_t> slice_size; /* bytes of each slice incl. its header (EOC in the last) */
     std::vector<uint32_t> slice_off;  /* codestream offset of each slice header */
     std::vector<uint32_t> slice_bud;  /* precinct bytes of each slice */
+    uint32_t bw = 20, fq = 8;         /* decoder: coefficient precision Bw and fractional bits Fq (PIH) */
 };
 
 /* Encoder configuration: EncHandle.c:128-454 (validation), Pi.c:189-406
@@ -35,7 +37,15 @@ struct HostGeom {
  * EncHandle.c:760-809.  Returns 0 or an SvtJxsErrorType_t value. */
 int32_t build_encoder_geometry(const EncParams& p, HostGeom& out, int verbose);
 
-/* Decoder configuration from a codestream header (ParseHeader.c). */
+/* Decoder configuration from a parsed codestream header: the band layout,
+ * weights and per-frame coding flags; a header this decoder does not decode
+ * (more than one precinct column, CWD, colour or non-linear transforms, a
+ * coefficient precision other than Bw 18 or 20) is
+ * SvtJxsErrorDecoderInvalidBitstream. */
+int32_t geom_from_header(const CsHeader& h, uint32_t header_end, HostGeom& out, uint32_t* header_end_out);
+
+/* cs_get_header + cs_pi_check + geom_from_header for a full-resolution
+ * decode. */
 int32_t parse_codestream_header(const uint8_t* cs, size_t len, HostGeom& out, uint32_t* header_end);
 
 /* Band layout only (shared by the encoder and decoder builders). */

@@ -11,7 +11,16 @@ Fixtures (data only):
   <case>.jxs          reference codestreams of the small cases.
   sample_16x16_422.jxs  the 16x16 4:2:2 codestream the reference's unit tests
                       decode (tests/UnitTests/SampleFramesData.cc:8-33).
+  sweep_<case>.txt.gz the reference decoder's verdict on every damaged variant
+                      of a codestream that its decoder unit tests build
+                      (TestDecoder.cc:336-618: every truncation, zero and one
+                      fill, every byte inverted, every bit flipped, and the
+                      cumulative bit walk), one line per variant as
+                      oracle/ref_harness.c (HARNESS_SWEEP) prints it.
+
+    python tests/golden/make_golden.py --sweeps   (only the sweep fixtures)
 """
+import gzip
 import hashlib
 import json
 import os
@@ -225,10 +234,29 @@ def sample_codestream():
     return bytes(int(x, 16) for x in re.findall(r"0x[0-9A-Fa-f]+", body))
 
 
+# codestreams swept: the reference's own unit-test sample (vertical
+# prediction, one slice) and a two-slice codestream without it
+SWEEPS = ["sample_16x16_422", "s_64x32_422_8b_bpp5_v2"]
+
+
+def sweeps():
+    exe = os.path.join(ROOT, "oracle", "_ref", "ref_sweep")
+    for name in SWEEPS:
+        r = subprocess.run([exe, os.path.join(HERE, name + ".jxs")], capture_output=True, text=True)
+        if r.returncode != 0:
+            sys.exit("ref_sweep failed on %s: %s" % (name, r.stderr[-300:]))
+        with gzip.open(os.path.join(HERE, "sweep_%s.txt.gz" % name), "wt", compresslevel=9) as f:
+            f.write(r.stdout)
+        print("sweep", name, len(r.stdout.splitlines()), "variants", flush=True)
+
+
 def main():
     O = load_oracle()
     if not O.have_reference():
         sys.exit("build the reference first: make -f oracle/Makefile.ref")
+    if "--sweeps" in sys.argv:
+        sweeps()
+        return
     out = {"small": {}, "full": {}}
     with tempfile.TemporaryDirectory() as td:
         for name, cfg in SMALL.items():
@@ -250,6 +278,7 @@ def main():
     out["quant_kat"] = quant_kats()
     with open(os.path.join(HERE, "golden.json"), "w") as f:
         json.dump(out, f, indent=1, sort_keys=True)
+    sweeps()
 
 
 if __name__ == "__main__":
