@@ -158,7 +158,8 @@ def _spec(enc, d_in, out_ptr):
 
 def _tiles(tiles):
     """Autotuner choices: 'width/levels(v) -> chunk' per transform job shape."""
-    return ["%d/%d%s->%d" % (t["width"], t["levels"], "v" if t["vertical"] else "", t["chunk"]) for t in tiles]
+    return ["rate/%d->%dw" % (t["width"], t["chunk"]) if t.get("stage") == 1 else
+            "%d/%d%s->%d" % (t["width"], t["levels"], "v" if t["vertical"] else "", t["chunk"]) for t in tiles]
 
 
 def _time_plans(torch, stream, plans, batch, steps):

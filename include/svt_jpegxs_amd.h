@@ -112,17 +112,21 @@ PREFIX_API void svt_jpeg_xs_amd_decode_plan_destroy(svt_jpeg_xs_amd_decode_plan_
  * columns of a row (8: 512-thread blocks; 16 and 32: 256-thread blocks).
  * svt_jpeg_xs_amd_autotune times one frame of the encoder's configuration
  * with every width the kernels support for each of its transform job shapes
- * (input kind, vertical level, horizontal levels, width), on `stream`, and
- * records the fastest in a per-process table that every plan, batch, server
- * launch and encoder built afterwards uses; the encoder's own plain-path
- * plans are rebuilt.  Output is identical for every width.  Call it with no
- * frame of this encoder in flight.  Fills up to max_out entries of out (may be
- * NULL) and returns the number of job shapes tuned, or -1 on error.  The
- * reference has no counterpart (its tiles are fixed by its SIMD kernels). */
+ * (input kind, vertical level, horizontal levels, width), and then with 4, 2
+ * and 1 waves per precinct in the rate table kernel (rate control modes 0 and
+ * 1), on `stream`, and records the fastest in a per-process table that every
+ * plan, batch, server launch and encoder built afterwards uses; the encoder's
+ * own plain-path plans are rebuilt.  Output is identical for every choice.
+ * Call it with no frame of this encoder in flight.  Fills up to max_out
+ * entries of out (may be NULL) and returns the number of choices made, or -1
+ * on error.  The reference has no counterpart (its tiles are fixed by its
+ * SIMD kernels). */
 typedef struct svt_jpeg_xs_amd_tile {
-    uint32_t width, levels, vertical; /* job shape: plane width, horizontal levels, vertical level or not */
-    uint32_t chunk;                   /* columns per thread chosen */
-    float frame_us;                   /* transform time of one frame with that choice */
+    uint32_t width, levels, vertical; /* stage 0: job shape (plane width, horizontal levels, vertical level or not);
+                                         stage 1: picture width */
+    uint32_t chunk;                   /* stage 0: columns per thread; stage 1: waves per precinct */
+    float frame_us;                   /* that stage's time of one frame with the choice */
+    uint32_t stage;                   /* 0: transform tile, 1: rate table waves per precinct */
 } svt_jpeg_xs_amd_tile_t;
 
 PREFIX_API int32_t svt_jpeg_xs_amd_autotune(svt_jpeg_xs_encoder_api_t* encoder, void* stream,

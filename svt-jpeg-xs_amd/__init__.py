@@ -106,7 +106,7 @@ class FrameSpec(ctypes.Structure):
 class Tile(ctypes.Structure):
     """svt_jpeg_xs_amd_tile_t (include/svt_jpegxs_amd.h)."""
     _fields_ = [("width", ctypes.c_uint32), ("levels", ctypes.c_uint32), ("vertical", ctypes.c_uint32),
-                ("chunk", ctypes.c_uint32), ("frame_us", ctypes.c_float)]
+                ("chunk", ctypes.c_uint32), ("frame_us", ctypes.c_float), ("stage", ctypes.c_uint32)]
 
 
 class ServerConfig(ctypes.Structure):

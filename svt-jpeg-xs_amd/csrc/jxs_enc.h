@@ -70,6 +70,7 @@ struct BatchLaunch {
     uint32_t total_slices = 0, total_prec = 0, total_units = 0;
     bool ordered = false; /* a frame is coded by k_enc_slice */
     uint32_t rate_kinds = 0; /* bit 0: a frame under rate control mode 0 / 1, bit 1: mode 2 / 3 (k_rate) */
+    uint32_t rate_waves = 4; /* waves per precinct of the rate table kernel: the choice for the batch's largest frame */
 };
 
 /* Checks one frame spec (initialised encoder, codestream buffer 4-byte

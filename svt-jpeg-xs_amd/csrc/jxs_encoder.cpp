@@ -21,7 +21,7 @@
 
 extern "C" {
 hipError_t jxs_launch_dwt(const JxsDwtJob*, const uint32_t*, int, uint32_t, uint32_t, uint32_t*, uint32_t, hipStream_t);
-hipError_t jxs_launch_rate(const JxsFrameDesc*, int, uint32_t, uint32_t, uint32_t, hipStream_t);
+hipError_t jxs_launch_rate(const JxsFrameDesc*, int, uint32_t, uint32_t, uint32_t, uint32_t, hipStream_t);
 hipError_t jxs_launch_pack(const JxsUnit*, uint32_t, uint32_t, hipStream_t);
 hipError_t jxs_launch_join(const JxsUnit*, uint32_t, hipStream_t);
 hipError_t jxs_launch_enc_slice(const JxsFrameDesc*, int, uint32_t, hipStream_t);
@@ -230,7 +230,7 @@ bool batch_place_into(const svt_jpeg_xs_amd_frame_spec_t* specs, uint32_t count,
     L.units = (JxsUnit*)(base + off);
     off += align256((uint64_t)sz.units * sizeof(JxsUnit));
     uint8_t* cur = base + off;
-    uint32_t prec_base = 0, slice_base = 0, unit_base = 0;
+    uint32_t prec_base = 0, slice_base = 0, unit_base = 0, most_rows = 0;
     uint32_t fill[2][3] = {}, blk[2][3] = {};
     for (uint32_t i = 0; i < count; i++) {
         const DevGeom* dg = spec_geom(specs[i]);
@@ -273,6 +273,10 @@ bool batch_place_into(const svt_jpeg_xs_amd_frame_spec_t* specs, uint32_t count,
         L.ordered |= F.ordered != 0;
         if (!F.ordered)
             L.rate_kinds |= 1u << F.slice_rc;
+        if (!F.ordered && !F.slice_rc && g.rows > most_rows) {
+            most_rows = g.rows;
+            L.rate_waves = rate_waves_get(g);
+        }
         memcpy(img + frames_off + (uint64_t)i * sizeof(JxsFrameDesc), &F, sizeof(F));
         prec_base += g.rows;
         slice_base += g.slices;
@@ -325,7 +329,8 @@ hipError_t batch_launch(const BatchLaunch& L, hipStream_t st, hipEvent_t* ev) {
     }
     if (ev)
         (void)hipEventRecord(ev[2], st);
-    if ((e = jxs_launch_rate(L.frames, (int)L.nfr, L.total_slices, L.total_prec, L.rate_kinds, st)) != hipSuccess)
+    if ((e = jxs_launch_rate(L.frames, (int)L.nfr, L.total_slices, L.total_prec, L.rate_kinds, L.rate_waves, st)) !=
+        hipSuccess)
         return e;
     if (ev)
         (void)hipEventRecord(ev[3], st);
@@ -1039,8 +1044,58 @@ PREFIX_API int32_t svt_jpeg_xs_amd_autotune(svt_jpeg_xs_encoder_api_t* enc_api, 
             out[tuned].vertical = j.vert;
             out[tuned].chunk = best_c;
             out[tuned].frame_us = best_t * 1000.0f;
+            out[tuned].stage = 0;
         }
         tuned++;
+    }
+    /* waves per precinct of the rate table kernel (rate control modes 0 and
+     * 1): the rate stage's time of one frame for 4, 2 and 1 waves */
+    const JxsGeom& rg = e->geom.host.g;
+    if (ok && !geom_ordered(rg) && rg.rc_mode < 2) {
+        const uint32_t before = rate_waves_get(rg);
+        uint32_t best_w = 0;
+        float best_t = 0.0f;
+        for (uint32_t nw : {4u, 2u, 1u}) {
+            rate_waves_set(rg, nw);
+            svt_jpeg_xs_amd_plan_t* p = plan_build(&spec, 1);
+            float best = -1.0f;
+            for (int rep = 0; p && rep < 6; rep++) {
+                float ms = 0.0f;
+                if (batch_launch(p->L, st, ev) != hipSuccess ||
+                    hipEventSynchronize(ev[SVT_JPEG_XS_AMD_STAGES]) != hipSuccess ||
+                    hipEventElapsedTime(&ms, ev[2], ev[3]) != hipSuccess) {
+                    best = -1.0f;
+                    break;
+                }
+                if (rep > 0 && (best < 0.0f || ms < best))
+                    best = ms;
+            }
+            if (p)
+                plan_free(p);
+            /* the codestream must not depend on the choice */
+            if (best >= 0.0f && (hipMemcpy(cs.data(), s0.d_out, lcod, hipMemcpyDeviceToHost) != hipSuccess ||
+                                 (!ref_cs.empty() && cs != ref_cs)))
+                best = -1.0f;
+            if (best < 0.0f) {
+                ok = false;
+                break;
+            }
+            if (!best_w || best < best_t) {
+                best_w = nw;
+                best_t = best;
+            }
+        }
+        rate_waves_set(rg, ok ? best_w : before);
+        if (ok && out && (uint32_t)tuned < max_out) {
+            out[tuned].width = rg.w;
+            out[tuned].levels = 0;
+            out[tuned].vertical = 0;
+            out[tuned].chunk = best_w;
+            out[tuned].frame_us = best_t * 1000.0f;
+            out[tuned].stage = 1;
+        }
+        if (ok)
+            tuned++;
     }
     for (int i = 0; i <= SVT_JPEG_XS_AMD_STAGES; i++)
         (void)hipEventDestroy(ev[i]);

@@ -794,6 +794,25 @@ uint32_t tile_get(const TileKey& k) {
     return it == g_tile.end() ? 0u : it->second;
 }
 
+/* waves per precinct of the rate table kernel, per precinct shape (band
+ * lines, precincts per slice, precinct rows) */
+static std::map<std::tuple<uint32_t, uint32_t, uint32_t>, uint32_t> g_rate_waves;
+
+uint32_t rate_waves_get(const JxsGeom& g) {
+    std::lock_guard<std::mutex> lk(g_tile_mu);
+    const auto it = g_rate_waves.find(std::make_tuple(g.n_bl, g.pps, g.rows));
+    return it == g_rate_waves.end() ? 4u : it->second;
+}
+
+void rate_waves_set(const JxsGeom& g, uint32_t waves) {
+    std::lock_guard<std::mutex> lk(g_tile_mu);
+    const auto key = std::make_tuple(g.n_bl, g.pps, g.rows);
+    if (waves == 1 || waves == 2 || waves == 4)
+        g_rate_waves[key] = waves;
+    else
+        g_rate_waves.erase(key);
+}
+
 /* comp: the component of a packed (step 3) job, whose src is the row start + comp samples */
 static JxsDwtJob finish_job(JxsDwtJob j, uint32_t comp = 0) {
     const uint32_t elem = j.kind == JXS_SRC_U8 ? 1u : j.kind == JXS_SRC_U16 ? 2u : 4u;

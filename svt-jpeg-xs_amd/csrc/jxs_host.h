@@ -80,6 +80,10 @@ TileKey tile_key(const JxsDwtJob& j);
 bool tile_legal(uint32_t lev_l, uint32_t w, uint32_t chunk);
 void tile_set(const TileKey& k, uint32_t chunk); /* chunk 0: back to the rule */
 uint32_t tile_get(const TileKey& k);              /* 0: none recorded */
+/* Waves per precinct (1, 2 or 4) of the rate table kernel for frames of this
+ * geometry: the autotuner's choice, 4 when none is recorded (0 resets). */
+uint32_t rate_waves_get(const JxsGeom& g);
+void rate_waves_set(const JxsGeom& g, uint32_t waves);
 
 /* Inverse-transform job lists (decoder). */
 struct InvVertJob;

@@ -2356,18 +2356,21 @@ __device__ __forceinline__ void rate_line_at(const Rec& rc, uint32_t n_bl, uint3
     os = absent ? 0u : (use ? sw : 0u);
 }
 
-/* One block per precinct (4 waves): its rate table (a quarter of the
- * truncations a wave), its bytes at (q, 0) for every q and the (q, r) rows
- * at the Q its budget gets without a padding carry and at that Q - 1 (a run
- * of whole packets a wave, the runs summed), the binary searches
- * (RateControl.c:889-1083, BinarySearch.c) at that budget, and the smallest
- * entry above the budget of those the searches look at: all into the
- * precinct's rate record. */
-#define RB_WAVES 4
-__global__ void __launch_bounds__(WAVE * RB_WAVES) k_rate_tab(const JxsFrameDesc* __restrict__ frames, int nfr,
-                                                              uint32_t total_prec) {
+/* One block per precinct of NW waves (1, 2 or 4: the autotuner's choice):
+ * its rate table (1/NW of the truncations a wave), its bytes at (q, 0) for
+ * every q and the (q, r) rows at the Q its budget gets without a padding
+ * carry and at that Q - 1 (a run of whole packets a wave, the runs summed),
+ * the binary searches (RateControl.c:889-1083, BinarySearch.c) at that
+ * budget, and the smallest entry above the budget of those the searches look
+ * at: all into the precinct's rate record. */
+}  // extern "C"
+
+template <uint32_t NW>
+__global__ void __launch_bounds__(WAVE * NW) k_rate_tab(const JxsFrameDesc* __restrict__ frames, int nfr,
+                                                        uint32_t total_prec) {
+    constexpr uint32_t LG = NW == 4 ? 2u : NW == 2 ? 1u : 0u;
     __shared__ RateTab T;
-    __shared__ uint32_t part[2][RB_WAVES][WAVE];
+    __shared__ uint32_t part[2][NW][WAVE];
     const uint32_t pg = blockIdx.x;
     if (pg >= total_prec)
         return;
@@ -2388,15 +2391,18 @@ __global__ void __launch_bounds__(WAVE * RB_WAVES) k_rate_tab(const JxsFrameDesc
     const uint32_t em = gld(&G->empty_mask[lane & (JXS_NQ - 1)]);
     const uint32_t sig = cld_u32(&G->sig), maxq = cld_u32(&G->maxq), maxr = cld_u32(&G->maxr);
     const uint32_t rc_mode = cld_u32(&G->rc_mode), hdr = cld_u32(&G->rc_hdr_bytes[type]);
-    const uint32_t lo = cld_u8(&G->rq_seg[type][2][w]), hi = cld_u8(&G->rq_seg[type][2][w + 1]);
+    const uint32_t lo = cld_u8(&G->rq_seg[type][LG][w]), hi = cld_u8(&G->rq_seg[type][LG][w + 1]);
     const uint32_t b0 = prec_budget(cld_u32(F.slice_bud + s), nprec, i, rc_mode);
     const uint32_t eq = (uint32_t)__ballot(em & 1u); /* bit q: (q, 0) truncates every band away */
-    rate_lines(T, rc, n_bl, li, type, sig, lane, 2, w);
+    rate_lines(T, rc, n_bl, li, type, sig, lane, LG, w);
     lds_barrier();
     /* lane q (and q + 32): the run's bytes at (q, 0) */
     part[0][w][lane] = rate_query_seg(rq, rw, lo, hi, T, lane & (JXS_NQ - 1), 0);
     lds_barrier();
-    const uint32_t cq = part[0][0][lane] + part[0][1][lane] + part[0][2][lane] + part[0][3][lane];
+    uint32_t cq = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < NW; k++)
+        cq += part[0][k][lane];
     uint32_t q0 = 0;
     const bool ok0 = b0 <= 1048575u && b0 > hdr && search_q(maxq, eq, cq, b0 - hdr, q0);
     if (!ok0)
@@ -2407,7 +2413,10 @@ __global__ void __launch_bounds__(WAVE * RB_WAVES) k_rate_tab(const JxsFrameDesc
     lds_barrier();
     if (w != 0)
         return;
-    const uint32_t cr = part[1][0][lane] + part[1][1][lane] + part[1][2][lane] + part[1][3][lane];
+    uint32_t cr = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < NW; k++)
+        cr += part[1][k][lane];
     uint32_t r0 = 0, data0 = 0, thr = 0;
     if (ok0) {
         const uint32_t avail0 = b0 - hdr, er = lane_of(em, q0);
@@ -2428,6 +2437,8 @@ __global__ void __launch_bounds__(WAVE * RB_WAVES) k_rate_tab(const JxsFrameDesc
         gst((u32x4*)&R->q0, sp);
     }
 }
+
+extern "C" {
 
 /* One wave per precinct: the in-order pass over its slice up to it, which
  * fixes its budget, (Q, R), padding and codestream offset (the budgets with
@@ -2591,14 +2602,20 @@ __global__ void __launch_bounds__(WAVE) k_rate_plan(const JxsFrameDesc* __restri
 }
 
 /* kinds: bit 0 launches the rate control of the frames under modes 0 / 1
- * (k_rate_tab, then k_rate_plan), bit 1 that of the frames under modes 2 / 3
- * (k_rate); each skips the other's frames. */
+ * (k_rate_tab with tab_waves waves per precinct, then k_rate_plan), bit 1
+ * that of the frames under modes 2 / 3 (k_rate); each skips the other's
+ * frames. */
 hipError_t jxs_launch_rate(const JxsFrameDesc* frames, int nfr, uint32_t slices, uint32_t precincts, uint32_t kinds,
-                           hipStream_t st) {
+                           uint32_t tab_waves, hipStream_t st) {
     if (!slices)
         return hipSuccess;
     if (kinds & 1u) {
-        hipLaunchKernelGGL(k_rate_tab, dim3(precincts), dim3(WAVE * RB_WAVES), 0, st, frames, nfr, precincts);
+        if (tab_waves == 1)
+            hipLaunchKernelGGL(k_rate_tab<1>, dim3(precincts), dim3(WAVE), 0, st, frames, nfr, precincts);
+        else if (tab_waves == 2)
+            hipLaunchKernelGGL(k_rate_tab<2>, dim3(precincts), dim3(WAVE * 2), 0, st, frames, nfr, precincts);
+        else
+            hipLaunchKernelGGL(k_rate_tab<4>, dim3(precincts), dim3(WAVE * 4), 0, st, frames, nfr, precincts);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess)
             return e;

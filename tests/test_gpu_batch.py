@@ -139,9 +139,10 @@ def test_large_batch_shadow_words_match_oracle_and_single_frames():
 
 @pytest.mark.gpu
 def test_autotuned_tiles_keep_codestreams():
-    """The transform tile autotuner picks a width per job shape (checking
-    that every candidate gives the same codestream on its own test frame);
-    encoding afterwards, plain and batched, still equals the oracle."""
+    """The autotuner picks a transform width per job shape and the rate table
+    kernel's waves per precinct (checking that every candidate gives the same
+    codestream on its own test frame); encoding afterwards, plain and
+    batched, still equals the oracle."""
     import torch
     dev = torch.device("cuda:0")
     S, O, P = load_synth(), load_oracle(), load_package()
@@ -152,7 +153,9 @@ def test_autotuned_tiles_keep_codestreams():
         enc = P.Encoder(W, H, D, F, bpp, 1, V, Hh, verbose=0)
         tiles = enc.autotune()
         print("autotune", (W, H, D, F, V, Hh), tiles, flush=True)
-        assert tiles and all(t["chunk"] in (8, 16, 32) and t["frame_us"] > 0 for t in tiles)
+        assert tiles and all(t["frame_us"] > 0 for t in tiles)
+        assert all(t["chunk"] in ((8, 16, 32) if t["stage"] == 0 else (1, 2, 4)) for t in tiles)
+        assert [t["stage"] for t in tiles].count(1) == 1  # every config here is rate control mode 1
         planes = S.make_frame(W, H, D, F, frame=700 + k)
         e, ref = O.encode(O.make_params(W, H, D, F, bpp, 1, V, Hh), planes)
         assert e == 0
