@@ -59,6 +59,24 @@ struct JxsInvRowJob {
     uint32_t bw; /* nominal coefficient precision (Bw) of the output scaling */
 };
 
+/* Final vertical level of a component with the finest horizontal level of
+ * its low and high rows.  Low row i at level-1 resolution comes from low32
+ * (int32, the coarser levels already inverted, or the next vertical level's
+ * plane) or low16 (a band, scaled by 2^fq); hl its level-1 high band; high row
+ * i from bands lh (low) and hh (high).  One thread per output column pair, a
+ * block of 256 pairs by `seg` row pairs. */
+struct JxsInvLastJob {
+    const int32_t* low32;
+    const int16_t* low16;
+    const int16_t* hl;
+    const int16_t* lh;
+    const int16_t* hh;
+    void* out;
+    uint32_t low_stride, hl_stride, lh_stride, hh_stride;
+    uint32_t out_stride, depth, bw, fq;
+    uint32_t w, h, seg, tiles_x, nblk, base;
+};
+
 /* Inverse vertical job: int32 low rows then high rows (stride w) -> int32
  * plane or output plane. */
 struct JxsInvVertJob {

@@ -1090,6 +1090,172 @@ __global__ void __launch_bounds__(256) k_inv_rows(const JxsInvRowJob* __restrict
     }
 }
 
+/* ------------------------------------------------------------------ */
+/* final vertical level with the finest horizontal level                */
+/* ------------------------------------------------------------------ */
+#define LV_NT 256 /* column pairs of a block */
+
+/* Level-1 inverse horizontal 5/3 at columns 2j and 2j + 1 of a row of width
+ * w (Idwt.c semantics, the same edges as the row kernel's finest level):
+ * lo(j), lo(j+1) are the low band's samples, hm/h0/hp its high band's at
+ * j - 1, j, j + 1 (an index outside the band's range is never used). */
+__device__ __forceinline__ void lv_h1(uint32_t j, uint32_t w, int32_t l0, int32_t l1, int32_t hm, int32_t h0, int32_t hp,
+                                      int32_t& ev, int32_t& od) {
+    const uint32_t nh = w / 2;
+    /* even sample 2j, and 2j + 2 for the odd one between them */
+    ev = j == 0 ? l0 - ((h0 + 1) >> 1) : (j < nh ? l0 - ((hm + h0 + 2) >> 2) : l0 - ((hm + 1) >> 1));
+    od = 0;
+    if (j < nh) {
+        if (2 * j + 2 < w) {
+            const int32_t e1 = j + 1 < nh ? l1 - ((h0 + hp + 2) >> 2) : l1 - ((h0 + 1) >> 1);
+            od = h0 + ((ev + e1) >> 1);
+        } else {
+            od = h0 + ev;
+        }
+    }
+}
+
+/* The inputs of one row pair's horizontal step for column pair j: the low
+ * and high bands of low row i and of high row i around j. */
+struct LvIn {
+    int32_t l0, l1, hm, h0, hp; /* low row */
+    int32_t m0, m1, gm, g0, gp; /* high row */
+};
+
+__device__ __forceinline__ LvIn lv_load(const JxsInvLastJob& J, uint32_t i, uint32_t j, bool low_row, bool high_row) {
+    LvIn v = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    const uint32_t w = J.w, n1 = (w + 1) / 2, nh = w / 2;
+    const uint32_t jm = j > 0 ? j - 1 : 0, j1 = min(j + 1, n1 - 1), jh = min(j, nh - 1), jp = min(j + 1, nh - 1);
+    const int32_t fm = 1 << J.fq;
+    if (low_row) {
+        if (J.low32) {
+            const int32_t* lr = J.low32 + (size_t)i * J.low_stride;
+            v.l0 = gld(lr + j);
+            v.l1 = gld(lr + j1);
+        } else {
+            const int16_t* lr = J.low16 + (size_t)i * J.low_stride;
+            v.l0 = (int32_t)gld(lr + j) * fm;
+            v.l1 = (int32_t)gld(lr + j1) * fm;
+        }
+        const int16_t* hr = J.hl + (size_t)i * J.hl_stride;
+        v.hm = (int32_t)gld(hr + jm) * fm;
+        v.h0 = (int32_t)gld(hr + jh) * fm;
+        v.hp = (int32_t)gld(hr + jp) * fm;
+    }
+    if (high_row) {
+        const int16_t* lr = J.lh + (size_t)i * J.lh_stride;
+        v.m0 = (int32_t)gld(lr + j) * fm;
+        v.m1 = (int32_t)gld(lr + j1) * fm;
+        const int16_t* hr = J.hh + (size_t)i * J.hh_stride;
+        v.gm = (int32_t)gld(hr + jm) * fm;
+        v.g0 = (int32_t)gld(hr + jh) * fm;
+        v.gp = (int32_t)gld(hr + jp) * fm;
+    }
+    return v;
+}
+
+/* Two output samples of row r at columns 2j, 2j + 1 (the second if inside). */
+__device__ __forceinline__ void lv_store2(const JxsInvLastJob& J, uint32_t r, uint32_t j, int32_t a, int32_t b) {
+    const uint32_t x = 2 * j;
+    const size_t o = (size_t)r * J.out_stride + x;
+    const uint32_t oa = nlt_out(a, J.depth, J.bw), ob = nlt_out(b, J.depth, J.bw);
+    if (x + 1 < J.w) {
+        if (J.depth <= 8) {
+            if (((uintptr_t)((uint8_t*)J.out + o) & 1) == 0) {
+                *(GAS uint16_t*)((uint8_t*)J.out + o) = (uint16_t)(oa | ob << 8);
+                return;
+            }
+        } else if (((uintptr_t)((uint16_t*)J.out + o) & 3) == 0) {
+            *(GAS uint32_t*)((uint16_t*)J.out + o) = oa | ob << 16;
+            return;
+        }
+        store_out(J.out, o, a, J.depth, J.bw);
+        store_out(J.out, o + 1, b, J.depth, J.bw);
+    } else {
+        store_out(J.out, o, a, J.depth, J.bw);
+    }
+}
+
+/* One thread per output column pair j and a strip of J.seg row pairs of a
+ * component's final vertical level: low row L(i) and high row H(i) at
+ * columns 2j, 2j + 1 come from their finest horizontal level (lv_h1), and
+ * the vertical 5/3 lifting (the steps and edges of k_inv_vert) runs down the
+ * strip in registers: the step for output rows 2i, 2i + 1 carries x(2i) and
+ * H(i) from the step before and makes L(i + 1), H(i + 1); the next step's
+ * loads are issued before this step's arithmetic.  No barrier: neighbours'
+ * samples are read again (from cache) instead of exchanged. */
+__global__ void __launch_bounds__(LV_NT) k_inv_last(const JxsInvLastJob* __restrict__ jobs,
+                                                   const uint32_t* __restrict__ bases, int njobs) {
+    const JxsInvLastJob& J = jobs[find_base(bases, njobs, blockIdx.x)];
+    const uint32_t t = blockIdx.x - J.base;
+    const uint32_t tx = t % J.tiles_x, ty = t / J.tiles_x;
+    const uint32_t j = tx * LV_NT + threadIdx.x;
+    const uint32_t w = J.w, n1 = (w + 1) / 2;
+    if (j >= n1)
+        return;
+    const uint32_t n = J.h, nh = n / 2, nl = n - nh;
+    const uint32_t i0 = ty * J.seg, i1 = min(i0 + J.seg, nl);
+    if (i0 >= i1)
+        return;
+    /* x(2 i0) and H(i0) */
+    int32_t ea, eb, ha, hb; /* even row and high row at columns 2j, 2j + 1 */
+    {
+        const bool has_h0 = i0 < nh, has_hm = i0 > 0;
+        const LvIn a = lv_load(J, i0, j, true, has_h0);
+        const LvIn m = has_hm ? lv_load(J, i0 - 1, j, false, true) : LvIn{};
+        int32_t la, lb, pa = 0, pb = 0;
+        lv_h1(j, w, a.l0, a.l1, a.hm, a.h0, a.hp, la, lb);
+        ha = hb = 0;
+        if (has_h0)
+            lv_h1(j, w, a.m0, a.m1, a.gm, a.g0, a.gp, ha, hb);
+        if (has_hm)
+            lv_h1(j, w, m.m0, m.m1, m.gm, m.g0, m.gp, pa, pb);
+        if (i0 == 0) {
+            ea = la - ((ha + 1) >> 1);
+            eb = lb - ((hb + 1) >> 1);
+        } else if (i0 < nh) {
+            ea = la - ((pa + ha + 2) >> 2);
+            eb = lb - ((pb + hb + 2) >> 2);
+        } else {
+            ea = la - ((pa + 1) >> 1);
+            eb = lb - ((pb + 1) >> 1);
+        }
+    }
+    LvIn nx = {};
+    if (i0 < nh && 2 * i0 + 2 < n)
+        nx = lv_load(J, i0 + 1, j, true, i0 + 1 < nh);
+    for (uint32_t i = i0; i < i1; i++) {
+        const bool has_odd = i < nh;
+        const bool need_next = has_odd && 2 * i + 2 < n;
+        const bool has_hn = i + 1 < nh;
+        const LvIn cur = nx;
+        /* the loads of the step after this one, issued before its arithmetic */
+        if (i + 1 < i1 && i + 1 < nh && 2 * i + 4 < n)
+            nx = lv_load(J, i + 2, j, true, i + 2 < nh);
+        int32_t oa = 0, ob = 0, na = 0, nb = 0;
+        if (need_next) {
+            int32_t la, lb, ga = 0, gb = 0;
+            lv_h1(j, w, cur.l0, cur.l1, cur.hm, cur.h0, cur.hp, la, lb);
+            if (has_hn)
+                lv_h1(j, w, cur.m0, cur.m1, cur.gm, cur.g0, cur.gp, ga, gb);
+            na = has_hn ? la - ((ha + ga + 2) >> 2) : la - ((ha + 1) >> 1);
+            nb = has_hn ? lb - ((hb + gb + 2) >> 2) : lb - ((hb + 1) >> 1);
+            oa = ha + ((ea + na) >> 1);
+            ob = hb + ((eb + nb) >> 1);
+            ha = ga;
+            hb = gb;
+        } else if (has_odd) {
+            oa = ha + ea;
+            ob = hb + eb;
+        }
+        lv_store2(J, 2 * i, j, ea, eb);
+        if (has_odd)
+            lv_store2(J, 2 * i + 1, j, oa, ob);
+        ea = na;
+        eb = nb;
+    }
+}
+
 #define IVSEG 16
 
 __global__ void __launch_bounds__(256) k_inv_vert(const JxsInvVertJob* __restrict__ jobs, const uint32_t* __restrict__ bases,
@@ -1170,6 +1336,14 @@ hipError_t jxs_launch_inv_rows(const JxsInvRowJob* jobs, const uint32_t* bases, 
     if (!njobs || !blocks)
         return hipSuccess;
     hipLaunchKernelGGL(k_inv_rows, dim3(blocks), dim3(256), 0, st, jobs, bases, njobs);
+    return hipGetLastError();
+}
+
+hipError_t jxs_launch_inv_last(const JxsInvLastJob* jobs, const uint32_t* bases, int njobs, uint32_t blocks,
+                              hipStream_t st) {
+    if (!njobs || !blocks)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_inv_last, dim3(blocks), dim3(LV_NT), 0, st, jobs, bases, njobs);
     return hipGetLastError();
 }
 

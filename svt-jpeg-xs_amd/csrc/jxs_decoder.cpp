@@ -29,6 +29,7 @@ hipError_t jxs_launch_dec_unpack(const JxsDecFrame*, int, uint32_t, uint32_t, hi
 hipError_t jxs_launch_dec_slice(const JxsDecFrame*, int, uint32_t, hipStream_t);
 hipError_t jxs_launch_inv_rows(const JxsInvRowJob*, const uint32_t*, int, uint32_t, hipStream_t);
 hipError_t jxs_launch_inv_vert(const JxsInvVertJob*, const uint32_t*, int, uint32_t, hipStream_t);
+hipError_t jxs_launch_inv_last(const JxsInvLastJob*, const uint32_t*, int, uint32_t, hipStream_t);
 hipError_t jxs_launch_zero(uint32_t*, uint32_t, hipStream_t);
 }
 
@@ -40,12 +41,17 @@ static bool dec_version_ok(uint64_t maj, uint64_t min) {
     return !(maj > SVT_JPEGXS_API_VER_MAJOR || (maj == SVT_JPEGXS_API_VER_MAJOR && min > SVT_JPEGXS_API_VER_MINOR));
 }
 
-/* Inverse-transform jobs of one frame (stage 0: rows of the second vertical
- * level, 1: its columns, 2: rows of the first level or of components without
- * vertical levels, 3: columns of the first level). */
+/* Inverse-transform jobs of one frame: rows[0] / vert[0] the second
+ * vertical level (rows, then columns, into an int32 plane); rows[1] the
+ * components without vertical levels (into the output plane) and the
+ * coarser horizontal levels of the first vertical level's low rows (into an
+ * int32 plane at half width); last the first vertical level with the finest
+ * horizontal level of its low and high rows. */
+#define INV_SEG 32 /* row pairs per block of the last stage */
 struct InvJobs {
     std::vector<JxsInvRowJob> rows[2];
     std::vector<JxsInvVertJob> vert[2];
+    std::vector<JxsInvLastJob> last;
 };
 
 static void inv_jobs(const JxsGeom& g, uint32_t bw, uint32_t fq, int16_t* coef, int32_t* scratch, void* const planes[4],
@@ -83,16 +89,27 @@ static void inv_jobs(const JxsGeom& g, uint32_t bw, uint32_t fq, int16_t* coef, 
         int32_t* A = scr;
         int32_t* B = scr + (size_t)w * h;
         scr += (size_t)w * h + (size_t)w1 * h1;
-        JxsInvRowJob lo = rowjob(w, h1);
-        lo.hf[0] = band(nb - 3);
-        lo.hf_stride[0] = cp.b[nb - 3].cs;
+        JxsInvLastJob f;
+        memset(&f, 0, sizeof(f));
         if (cp.dv == 1) {
-            lo.lf16 = band(0);
-            lo.lf_stride = cp.b[0].cs;
-            lo.levels = cp.dh;
-            for (uint32_t k = 1; k < cp.dh; k++) {
-                lo.hf[k] = band(cp.dh - k);
-                lo.hf_stride[k] = cp.b[cp.dh - k].cs;
+            if (cp.dh >= 2) {
+                /* the low rows' coarser horizontal levels, into A at half width */
+                JxsInvRowJob lo = rowjob(w1, h1);
+                lo.lf16 = band(0);
+                lo.lf_stride = cp.b[0].cs;
+                lo.levels = cp.dh - 1;
+                for (uint32_t k = 0; k + 1 < cp.dh; k++) {
+                    lo.hf[k] = band(cp.dh - 1 - k);
+                    lo.hf_stride[k] = cp.b[cp.dh - 1 - k].cs;
+                }
+                lo.out32 = A;
+                lo.out_stride = w1;
+                J.rows[1].push_back(lo);
+                f.low32 = A;
+                f.low_stride = w1;
+            } else {
+                f.low16 = band(0);
+                f.low_stride = cp.b[0].cs;
             }
         } else {
             const uint32_t g1 = (h1 + 1) / 2, g2 = h1 / 2;
@@ -130,34 +147,26 @@ static void inv_jobs(const JxsGeom& g, uint32_t bw, uint32_t fq, int16_t* coef, 
             v2.depth = g.depth;
             v2.bw = bw;
             J.vert[0].push_back(v2);
-            lo.lf32 = B;
-            lo.lf_stride = w1;
-            lo.levels = 1;
+            f.low32 = B;
+            f.low_stride = w1;
         }
-        lo.out32 = A;
-        lo.out_stride = w;
-        J.rows[1].push_back(lo);
-        if (h2 > 0) {
-            JxsInvRowJob hi = rowjob(w, h2);
-            hi.lf16 = band(nb - 2);
-            hi.lf_stride = cp.b[nb - 2].cs;
-            hi.levels = 1;
-            hi.hf[0] = band(nb - 1);
-            hi.hf_stride[0] = cp.b[nb - 1].cs;
-            hi.out32 = A + (size_t)h1 * w;
-            hi.out_stride = w;
-            J.rows[1].push_back(hi);
-        }
-        JxsInvVertJob v1;
-        memset(&v1, 0, sizeof(v1));
-        v1.src = A;
-        v1.out = planes[c];
-        v1.w = w;
-        v1.h = h;
-        v1.out_stride = strides[c];
-        v1.depth = g.depth;
-        v1.bw = bw;
-        J.vert[1].push_back(v1);
+        f.hl = band(nb - 3);
+        f.hl_stride = cp.b[nb - 3].cs;
+        f.lh = band(nb - 2);
+        f.lh_stride = cp.b[nb - 2].cs;
+        f.hh = band(nb - 1);
+        f.hh_stride = cp.b[nb - 1].cs;
+        f.out = planes[c];
+        f.out_stride = strides[c];
+        f.depth = g.depth;
+        f.bw = bw;
+        f.fq = fq;
+        f.w = w;
+        f.h = h;
+        f.seg = INV_SEG;
+        f.tiles_x = (w1 + 255) / 256;
+        f.nblk = f.tiles_x * ((h1 + INV_SEG - 1) / INV_SEG);
+        J.last.push_back(f);
     }
 }
 
@@ -189,6 +198,10 @@ struct svt_jpeg_xs_amd_decode_plan {
     uint32_t* d_vbase[2] = {nullptr, nullptr};
     int nrows[2] = {0, 0}, nvert[2] = {0, 0};
     uint32_t rblocks[2] = {0, 0}, vblocks[2] = {0, 0};
+    JxsInvLastJob* d_last = nullptr;
+    uint32_t* d_lbase = nullptr;
+    int nlast = 0;
+    uint32_t lblocks = 0;
     uint32_t total_prec = 0, max_npk = 0, total_slices = 0;
 };
 
@@ -217,6 +230,8 @@ static void dplan_free(svt_jpeg_xs_amd_decode_plan_t* p) {
         (void)hipFree(p->d_vert[s]);
         (void)hipFree(p->d_vbase[s]);
     }
+    (void)hipFree(p->d_last);
+    (void)hipFree(p->d_lbase);
     delete p;
 }
 
@@ -284,6 +299,7 @@ static svt_jpeg_xs_amd_decode_plan_t* dplan_build(const std::vector<DecSpec>& sp
             J.rows[k].insert(J.rows[k].end(), one.rows[k].begin(), one.rows[k].end());
             J.vert[k].insert(J.vert[k].end(), one.vert[k].begin(), one.vert[k].end());
         }
+        J.last.insert(J.last.end(), one.last.begin(), one.last.end());
     }
     p->total_prec = base;
     bool ok = up(frames, &p->d_frames) == hipSuccess;
@@ -311,6 +327,18 @@ static svt_jpeg_xs_amd_decode_plan_t* dplan_build(const std::vector<DecSpec>& sp
         ok = up(J.rows[k], &p->d_rows[k]) == hipSuccess && up(rb, &p->d_rbase[k]) == hipSuccess &&
             up(J.vert[k], &p->d_vert[k]) == hipSuccess && up(vb, &p->d_vbase[k]) == hipSuccess;
     }
+    if (ok) {
+        std::vector<uint32_t> lb;
+        uint32_t blk = 0;
+        for (auto& f : J.last) {
+            f.base = blk;
+            lb.push_back(blk);
+            blk += f.nblk;
+        }
+        p->lblocks = blk;
+        p->nlast = (int)J.last.size();
+        ok = up(J.last, &p->d_last) == hipSuccess && up(lb, &p->d_lbase) == hipSuccess;
+    }
     if (!ok) {
         dplan_free(p);
         return nullptr;
@@ -336,7 +364,9 @@ static hipError_t dplan_launch(svt_jpeg_xs_amd_decode_plan_t* p, hipStream_t st)
         return e;
     if ((e = jxs_launch_inv_rows(p->d_rows[1], p->d_rbase[1], p->nrows[1], p->rblocks[1], st)) != hipSuccess)
         return e;
-    return jxs_launch_inv_vert(p->d_vert[1], p->d_vbase[1], p->nvert[1], p->vblocks[1], st);
+    if ((e = jxs_launch_inv_vert(p->d_vert[1], p->d_vbase[1], p->nvert[1], p->vblocks[1], st)) != hipSuccess)
+        return e;
+    return jxs_launch_inv_last(p->d_last, p->d_lbase, p->nlast, p->lblocks, st);
 }
 
 /* Proxy decoding (pi_update_proxy_mode, Pi.c:567-626): half or quarter
