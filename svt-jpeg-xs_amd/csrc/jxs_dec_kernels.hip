@@ -196,8 +196,8 @@ __global__ void k_dec_scan(const JxsDecFrame* __restrict__ frames, int nfr) {
     }
 }
 
-#define PMAXG 2048            /* groups of a packet decoded per pass */
-#define PK_THREADS 512        /* threads of an unpack block */
+#define PMAXG 1024            /* groups of a packet decoded per pass */
+#define PK_THREADS 256        /* threads of an unpack block */
 #define PGPT (PMAXG / PK_THREADS) /* consecutive groups of a thread in a pass */
 #define PNB (JXS_MAXC * JXS_NBC) /* band lines of a packet, at most */
 
