@@ -73,15 +73,17 @@ __device__ __forceinline__ uint32_t cs_bits(const JxsDecFrame& F, uint64_t bp, u
  * byte, so the load never leaves the pages of the codestream; bytes of such a
  * word outside the codestream are never among the bits a caller keeps (every
  * caller bounds its bits by a sub-packet end inside the codestream). */
+typedef const __attribute__((address_space(1))) uint32_t gu32; /* a word known to be in device memory */
+
 struct CsView {
-    const uint32_t* w;
+    gu32* w;
     uint32_t nw, sh;
 };
 
 __device__ __forceinline__ CsView cs_view(const JxsDecFrame& F) {
     const uintptr_t a = (uintptr_t)F.cs;
     CsView v;
-    v.w = (const uint32_t*)(a & ~(uintptr_t)3);
+    v.w = (gu32*)(a & ~(uintptr_t)3);
     v.sh = (uint32_t)(a & 3) * 8;
     v.nw = (uint32_t)(((uint64_t)(a & 3) + F.cs_len + 3) >> 2);
     return v;
@@ -200,6 +202,7 @@ __global__ void k_dec_scan(const JxsDecFrame* __restrict__ frames, int nfr) {
 #define PK_THREADS 256        /* threads of an unpack block */
 #define PGPT (PMAXG / PK_THREADS) /* consecutive groups of a thread in a pass */
 #define PNB (JXS_MAXC * JXS_NBC) /* band lines of a packet, at most */
+#define PK_STAGE 3072         /* codestream words of a packet staged in LDS (12 KiB) */
 
 /* one band line of the packet being decoded, in codestream order */
 struct PkBand {
@@ -217,7 +220,29 @@ struct UnpackCtx {
     uint32_t nband, s_bit, g_bit, d_bit, g_end, d_end, raw, ok;
     uint32_t zpos[PMAXG]; /* aligned bit position of the zero ending code j of the pass */
     uint32_t scan[PK_THREADS / WAVE];
+    uint32_t stg[PK_STAGE + 1]; /* the packet's codestream words, byte-swapped, and a spare */
 };
+
+/* Codestream words read through the block's staged copy of its packet: words
+ * lo .. lo + n - 1 come from LDS, any other word from HBM (n = 0 when the
+ * packet did not fit). */
+struct PkSrc {
+    CsView V;
+    const uint32_t* s;
+    uint32_t lo, n;
+};
+
+__device__ __forceinline__ uint32_t pk_word(const PkSrc& S, uint32_t wi) {
+    const uint32_t r = wi - S.lo;
+    return r < S.n ? S.s[r] : cs_word(S.V, wi);
+}
+
+__device__ __forceinline__ uint64_t pk_bits64(const PkSrc& S, uint64_t abp) {
+    const uint32_t wi = (uint32_t)(abp >> 5), sh = (uint32_t)(abp & 31);
+    const uint64_t a = ((uint64_t)pk_word(S, wi) << 32) | pk_word(S, wi + 1);
+    const uint32_t c = pk_word(S, wi + 2);
+    return sh ? (a << sh) | (uint64_t)(c >> (32 - sh)) : a;
+}
 
 /* One lane per precinct of the launch: parses the precinct header's band
  * D-modes, derives every band's GTLI from the precinct's quantisation and
@@ -383,22 +408,50 @@ __global__ void __launch_bounds__(PK_THREADS) __attribute__((amdgpu_waves_per_eu
     const uint32_t l = g.pk_line[p];
     const CsView V = cs_view(F);
     const uint32_t tid = threadIdx.x;
-    const uint32_t* T = F.pkt + (size_t)row * F.pkt_stride;
-    if (tid == 64) {
-        const uint32_t* E = T + JXS_DPKT_HDR + JXS_DPKT_WORDS * p;
-        const uint32_t fl = E[4];
-        /* a slice with an ordered precinct is decoded by k_dec_slice */
-        uint32_t ordered = 0;
+    gu32* T = (gu32*)(uintptr_t)(F.pkt + (size_t)row * F.pkt_stride);
+    gu32* E = T + JXS_DPKT_HDR + JXS_DPKT_WORDS * p;
+    const uint32_t fl = E[4];
+    /* the packet's words, from its significance sub-packet to two words past
+     * its data, staged in LDS by the whole block with independent loads */
+    uint32_t st_lo = 0, st_n = 0;
+    if (fl & 1u) {
+        const uint32_t lo = (E[0] + V.sh) >> 5;
+        const uint32_t hi = min(V.nw, (uint32_t)(((uint64_t)E[3] + V.sh + 31) >> 5) + 2u);
+        if (hi > lo && hi - lo <= PK_STAGE) {
+            st_lo = lo;
+            st_n = hi - lo;
+        }
+    }
+    for (uint32_t k0 = 0; k0 < st_n; k0 += 4 * PK_THREADS) {
+        uint32_t sw[4];
+#pragma unroll
+        for (uint32_t j = 0; j < 4; j++)
+            sw[j] = __builtin_bswap32(V.w[st_lo + min(k0 + j * PK_THREADS + tid, st_n - 1)]);
+#pragma unroll
+        for (uint32_t j = 0; j < 4; j++) { /* past the packet: into the spare word */
+            const uint32_t k = k0 + j * PK_THREADS + tid;
+            C.stg[k < st_n ? k : PK_STAGE] = sw[j];
+        }
+    }
+    if (tid >= WAVE && tid < 2 * WAVE) {
+        /* a slice with an ordered precinct is decoded by k_dec_slice: one
+         * lane per precinct of the slice */
+        const uint32_t lane = tid - WAVE;
+        const uint32_t e0 = E[0], e1 = E[1], e2 = E[2], e3 = E[3], bad = T[JXS_DPKT_BAD];
         const uint32_t r0 = row - row % g.pps, r1 = min(r0 + g.pps, g.rows);
-        for (uint32_t rr = r0; rr < r1; rr++)
-            ordered |= F.pkt[(size_t)rr * F.pkt_stride + JXS_DPKT_ORDERED];
-        C.ok = (fl & 1u) && T[JXS_DPKT_BAD] == 0 && !ordered ? 1u : 0u;
-        C.raw = (fl >> 1) & 1u;
-        C.s_bit = E[0];
-        C.g_bit = E[1];
-        C.g_end = E[2];
-        C.d_bit = E[2];
-        C.d_end = E[3];
+        uint32_t ordered = 0;
+        for (uint32_t rr = r0 + lane; rr < r1; rr += WAVE)
+            ordered |= ((gu32*)(uintptr_t)F.pkt)[(size_t)rr * F.pkt_stride + JXS_DPKT_ORDERED];
+        const bool any_ordered = __ballot(ordered != 0) != 0ull;
+        if (lane == 0) {
+            C.ok = (fl & 1u) && bad == 0 && !any_ordered ? 1u : 0u;
+            C.raw = (fl >> 1) & 1u;
+            C.s_bit = e0;
+            C.g_bit = e1;
+            C.g_end = e2;
+            C.d_bit = e2;
+            C.d_end = e3;
+        }
     } else if (tid < WAVE) {
         /* the packet's band lines, one lane per candidate band (at most PNB < 64),
          * compacted in band order with running group and flag counts */
@@ -449,19 +502,20 @@ __global__ void __launch_bounds__(PK_THREADS) __attribute__((amdgpu_waves_per_eu
     __syncthreads();
     if (!C.ok)
         return;
-    const uint32_t nband = C.nband, total = C.band[nband].g0, raw = C.raw;
-    const uint32_t s_bit = C.s_bit + V.sh, gend = C.g_end + V.sh, d_end = C.d_end;
-    uint32_t gp = C.g_bit + V.sh; /* aligned bit position of the pass's first GCLI code */
-    uint32_t dp = C.d_bit;        /* bit position of the pass's first data bit */
+    /* block-uniform values, kept in scalar registers */
+    const uint32_t nband = __builtin_amdgcn_readfirstlane(C.nband);
+    const uint32_t total = __builtin_amdgcn_readfirstlane(C.band[nband].g0);
+    const uint32_t raw = __builtin_amdgcn_readfirstlane(C.raw);
+    const uint32_t s_bit = __builtin_amdgcn_readfirstlane(C.s_bit + V.sh);
+    const uint32_t gend = __builtin_amdgcn_readfirstlane(C.g_end + V.sh);
+    const uint32_t d_end = __builtin_amdgcn_readfirstlane(C.d_end);
+    uint32_t gp = __builtin_amdgcn_readfirstlane(C.g_bit + V.sh); /* aligned bit position of the pass's first GCLI code */
+    uint32_t dp = __builtin_amdgcn_readfirstlane(C.d_bit);        /* bit position of the pass's first data bit */
+    const PkSrc S = {V, C.stg, st_lo, st_n};
     bool err = false;
     for (uint32_t c0 = 0; c0 < total; c0 += PMAXG) {
         const uint32_t cn = min((uint32_t)PMAXG, total - c0);
         const uint32_t i0 = c0 + tid * PGPT; /* this thread's first group */
-        /* the first GCLI words of the pass, loaded with the significance flags */
-        const uint32_t w0 = gp >> 5;
-        uint32_t pre = 0xFFFFFFFFu;
-        if (!raw && (uint64_t)(w0 + tid) * 32 < gend)
-            pre = cs_word(V, w0 + tid);
         /* band line of the first group (binary search over the lines' first groups) */
         uint32_t bl = 0;
         {
@@ -489,7 +543,7 @@ __global__ void __launch_bounds__(PK_THREADS) __attribute__((amdgpu_waves_per_eu
             bool cd = true;
             if (!raw && C.band[bl].sig) {
                 const uint32_t ab = s_bit + C.band[bl].s0 + (i - C.band[bl].g0) / 8;
-                cd = ((cs_word(V, ab >> 5) >> (31 - (ab & 31))) & 1u) == 0; /* flag 1: insignificant */
+                cd = ((pk_word(S, ab >> 5) >> (31 - (ab & 31))) & 1u) == 0; /* flag 1: insignificant */
             }
             coded |= (cd ? 1u : 0u) << k;
             ncoded += cd ? 1u : 0u;
@@ -507,7 +561,7 @@ __global__ void __launch_bounds__(PK_THREADS) __attribute__((amdgpu_waves_per_eu
                 const uint32_t wi = w + tid;
                 uint32_t word = 0xFFFFFFFFu;
                 if ((uint64_t)wi * 32 < gend) {
-                    word = w == w0 ? pre : cs_word(V, wi);
+                    word = pk_word(S, wi);
                     const uint32_t lo = wi * 32;
                     if (lo < gp)
                         word |= ~(0xFFFFFFFFu >> (gp - lo)); /* bits before the pass's first code */
@@ -542,7 +596,7 @@ __global__ void __launch_bounds__(PK_THREADS) __attribute__((amdgpu_waves_per_eu
             uint32_t gc = gt;
             if (raw) {
                 const uint32_t ab = gp + 4 * (tid * PGPT + k);
-                gc = (uint32_t)(cs_bits64(V, ab) >> 60);
+                gc = (uint32_t)(pk_bits64(S, ab) >> 60);
             } else if ((coded >> k) & 1u) {
                 const uint32_t start = j == 0 ? gp : C.zpos[j - 1] + 1;
                 const uint32_t run = C.zpos[j] - start;
@@ -561,39 +615,30 @@ __global__ void __launch_bounds__(PK_THREADS) __attribute__((amdgpu_waves_per_eu
             atomicOr(F.status, 1u);
         uint32_t dtotal;
         const uint32_t dbase = dblock_scan(nbits, C.scan, &dtotal);
-        /* data windows, all loaded before any is decoded */
-        uint64_t win[PGPT];
-        uint32_t bp = dp + dbase;
-#pragma unroll
-        for (uint32_t k = 0; k < PGPT; k++) {
-            win[k] = 0;
-            if (tid * PGPT + k >= cn || err)
-                continue;
-            const uint32_t gt = C.band[bidx[k]].gt, gc = gcs[k];
-            const uint32_t nb = gc > gt ? 4 * (gc - gt + 1) : 0u;
-            if (nb) {
-                if ((uint64_t)bp + nb > d_end) {
-                    atomicOr(F.status, 1u);
-                    gcs[k] = gt; /* decoded as zeros */
-                } else {
-                    win[k] = cs_bits64(V, (uint64_t)bp + V.sh);
-                }
-            }
-            bp += nb;
-        }
+        /* each group's data (from LDS when the packet is staged) */
         uint32_t gbp = dp + dbase; /* bit position of the group's data */
 #pragma unroll
         for (uint32_t k = 0; k < PGPT; k++) {
             if (tid * PGPT + k >= cn || err)
                 continue;
-            const uint32_t gbp_k = gbp;
-            gbp += gcs[k] > C.band[bidx[k]].gt ? 4 * (gcs[k] - C.band[bidx[k]].gt + 1) : 0u;
             const PkBand& B = C.band[bidx[k]];
-            const uint32_t gt = B.gt, gc = gcs[k];
+            const uint32_t gt = B.gt;
+            uint32_t gc = gcs[k];
+            const uint32_t nb = gc > gt ? 4 * (gc - gt + 1) : 0u;
+            const uint32_t gbp_k = gbp;
+            gbp += nb;
+            uint64_t v = 0;
+            if (nb) {
+                if ((uint64_t)gbp_k + nb > d_end) {
+                    atomicOr(F.status, 1u);
+                    gc = gt; /* decoded as zeros */
+                } else {
+                    v = pk_bits64(S, (uint64_t)gbp_k + V.sh);
+                }
+            }
             const uint32_t i = i0 + k - B.g0; /* group within its line */
             uint32_t mag[4] = {0, 0, 0, 0}, sg = 0;
             if (gc > gt) {
-                const uint64_t v = win[k];
                 sg = (uint32_t)(v >> 60);
                 const uint32_t np = gc - gt;
                 if (np <= 15) {
@@ -622,8 +667,8 @@ __global__ void __launch_bounds__(PK_THREADS) __attribute__((amdgpu_waves_per_eu
         if (raw)
             gp += 4 * cn;
         else if (ncodes)
-            gp = C.zpos[ncodes - 1] + 1;
-        dp += dtotal;
+            gp = __builtin_amdgcn_readfirstlane(C.zpos[ncodes - 1] + 1);
+        dp += __builtin_amdgcn_readfirstlane(dtotal);
         __syncthreads(); /* zpos and the scan words are reused */
     }
     if (err && tid == 0)
