@@ -33,6 +33,7 @@
 #include <type_traits>
 
 #include "jxs_dec.h"
+#include "jxs_wave.h"
 
 #define WAVE 64
 #define GAS __attribute__((address_space(1)))
@@ -111,13 +112,7 @@ __device__ __forceinline__ uint32_t gather4(uint64_t x) {
 
 __device__ __forceinline__ uint32_t dblock_scan(uint32_t v, uint32_t* tmp, uint32_t* total) {
     const uint32_t lane = threadIdx.x & (WAVE - 1), wid = threadIdx.x / WAVE, nw = blockDim.x / WAVE;
-    uint32_t x = v;
-#pragma unroll
-    for (uint32_t o = 1; o < WAVE; o <<= 1) {
-        uint32_t y = __shfl_up(x, o, WAVE);
-        if (lane >= o)
-            x += y;
-    }
+    const uint32_t x = wave_scan_dpp(v);
     if (lane == WAVE - 1)
         tmp[wid] = x;
     __syncthreads();
@@ -384,6 +379,236 @@ __device__ __forceinline__ void group_words(const CsView& V, uint64_t abp, uint3
         w[t4] = ((gather4(v << t4) >> (16 - q)) << gt) & 0xFFFFu;
 }
 
+/* The packet-wide values the passes start from (all block-uniform). */
+struct PkPass {
+    uint32_t nband, total, raw;
+    uint32_t s_bit; /* aligned bit position of the significance sub-packet */
+    uint32_t gend;  /* aligned end of the GCLI sub-packet */
+    uint32_t d_end; /* end of the data sub-packet (unaligned bit position) */
+    uint32_t gp;    /* aligned bit position of the first GCLI code */
+    uint32_t dp;    /* bit position of the first data bit */
+    uint32_t qpih;
+};
+
+/* The four coefficients of a group with np <= 8 magnitude bit-planes,
+ * truncation gt with np + gt <= 15, deadzone reconstruction (Qpih = 0): the
+ * result of coef_final on each, as two words of two int16 each.  v holds the
+ * group's data from its sign nibble on.  Each plane nibble is spread to one
+ * bit per byte (n * 0x204081 puts nibble bit b at bit 8b, no carries), so the
+ * planes accumulate into four byte-wide magnitudes at once; the halves of a
+ * word then hold two coefficients through reconstruction and sign. */
+__device__ __forceinline__ uint2 group_fast(uint64_t v, uint32_t np, uint32_t gt) {
+    const uint32_t x = (uint32_t)(v >> 28); /* planes 0..7, one nibble each, first plane highest */
+    uint32_t A = 0;                         /* byte 3 - t: coefficient t's magnitude */
+    for (uint32_t q = 0; q < np; q++)
+        A = (A << 1) | ((((x >> (28 - 4 * q)) & 15u) * 0x204081u) & 0x01010101u);
+    const uint32_t sg = ((((uint32_t)(v >> 60)) * 0x204081u) & 0x01010101u); /* byte 3 - t: coefficient t's sign */
+    uint32_t r[2];
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+        /* coefficients 2h (low half) and 2h + 1 (high half) */
+        const uint32_t m = ((A >> (24 - 16 * h)) & 0xFFu) | (((A >> (16 - 16 * h)) & 0xFFu) << 16);
+        const uint32_t s = ((sg >> (24 - 16 * h)) & 1u) | (((sg >> (16 - 16 * h)) & 1u) << 16);
+        const uint32_t nz = ((m + 0x00FF00FFu) >> 8) & 0x00010001u;        /* 1 where m != 0 */
+        const uint32_t mag = ((((m << 1) | nz) << gt) >> 1) & 0x7FFF7FFFu; /* (2m + 1) << gt >> 1, 0 for m = 0 */
+        const uint32_t neg = s & nz;                                       /* -0 is 0 */
+        r[h] = (mag ^ (neg * 0xFFFFu)) + neg;                              /* two's complement per half */
+    }
+    return make_uint2(r[0], r[1]);
+}
+
+template <bool STAGED>
+__device__ __forceinline__ uint32_t pkx_word(const PkSrc& S, uint32_t wi) {
+    if (STAGED)
+        return S.s[min(wi - S.lo, (uint32_t)PK_STAGE)]; /* every read of a staged packet lies in its words */
+    return cs_word(S.V, wi);
+}
+
+template <bool STAGED>
+__device__ __forceinline__ uint64_t pkx_bits64(const PkSrc& S, uint32_t abp) {
+    const uint32_t wi = abp >> 5, sh = abp & 31;
+    const uint64_t a = ((uint64_t)pkx_word<STAGED>(S, wi) << 32) | pkx_word<STAGED>(S, wi + 1);
+    const uint32_t c = pkx_word<STAGED>(S, wi + 2);
+    return sh ? (a << sh) | (uint64_t)(c >> (32 - sh)) : a;
+}
+
+/* The passes of one packet (see k_dec_unpack); returns whether the GCLI
+ * sub-packet ran out before every code was found. */
+template <bool STAGED>
+__device__ __forceinline__ bool unpack_passes(UnpackCtx& C, const PkSrc& S, const PkPass& P0, uint32_t tid) {
+    const CsView& V = S.V;
+    const uint32_t nband = P0.nband, total = P0.total, raw = P0.raw, s_bit = P0.s_bit, gend = P0.gend,
+                   d_end = P0.d_end, qpih = P0.qpih;
+    uint32_t gp = P0.gp, dp = P0.dp;
+    uint32_t status = 0;
+    bool err = false;
+    for (uint32_t c0 = 0; c0 < total; c0 += PMAXG) {
+        const uint32_t cn = min((uint32_t)PMAXG, total - c0);
+        const uint32_t i0 = c0 + tid * PGPT; /* this thread's first group */
+        /* band line of the first group (binary search over the lines' first groups) */
+        uint32_t bl = 0;
+        {
+            uint32_t lo = 0, hi = nband - 1;
+            while (lo < hi) {
+                const uint32_t mid = (lo + hi + 1) >> 1;
+                if (C.band[mid].g0 <= i0)
+                    lo = mid;
+                else
+                    hi = mid - 1;
+            }
+            bl = lo;
+        }
+        /* significance: a coded group is one whose group of 8 is significant */
+        uint32_t bidx[PGPT], coded = 0, ncoded = 0;
+#pragma unroll
+        for (uint32_t k = 0; k < PGPT; k++) {
+            const uint32_t i = i0 + k;
+            bidx[k] = 0;
+            if (tid * PGPT + k >= cn)
+                continue;
+            while (C.band[bl + 1].g0 <= i)
+                bl++;
+            bidx[k] = bl;
+            bool cd = true;
+            if (!raw && C.band[bl].sig) {
+                const uint32_t ab = s_bit + C.band[bl].s0 + (i - C.band[bl].g0) / 8;
+                cd = ((pkx_word<STAGED>(S, ab >> 5) >> (31 - (ab & 31))) & 1u) == 0; /* flag 1: insignificant */
+            }
+            coded |= (cd ? 1u : 0u) << k;
+            ncoded += cd ? 1u : 0u;
+        }
+        uint32_t ncodes;
+        const uint32_t cbase = dblock_scan(ncoded, C.scan, &ncodes);
+        ncodes = __builtin_amdgcn_readfirstlane(ncodes);
+        /* GCLI codes */
+        if (!raw) {
+            uint32_t found = 0;
+            for (uint32_t w = gp >> 5; found < ncodes; w += blockDim.x) {
+                if ((uint64_t)w * 32 >= gend) {
+                    err = true;
+                    break;
+                }
+                const uint32_t wi = w + tid;
+                uint32_t word = 0xFFFFFFFFu;
+                if ((uint64_t)wi * 32 < gend) {
+                    word = pkx_word<STAGED>(S, wi);
+                    const uint32_t lo = wi * 32;
+                    if (lo < gp)
+                        word |= ~(0xFFFFFFFFu >> (gp - lo)); /* bits before the pass's first code */
+                    if (lo + 32 > gend)
+                        word |= (0xFFFFFFFFu >> (gend - lo)); /* bits after the sub-packet end */
+                }
+                uint32_t z = ~word;
+                uint32_t t;
+                const uint32_t ex = dblock_scan((uint32_t)__popc(z), C.scan, &t);
+                uint32_t k = found + ex;
+                while (z) {
+                    const uint32_t lead = __clz(z);
+                    if (k < ncodes)
+                        C.zpos[k] = wi * 32 + lead;
+                    k++;
+                    z &= ~(0x80000000u >> lead);
+                }
+                found += __builtin_amdgcn_readfirstlane(t);
+            }
+            __syncthreads();
+        }
+        /* GCLI of every group, data bit counts */
+        uint32_t gcs[PGPT], nbits = 0;
+        uint32_t j = cbase;
+#pragma unroll
+        for (uint32_t k = 0; k < PGPT; k++) {
+            gcs[k] = 0;
+            if (tid * PGPT + k >= cn || err)
+                continue;
+            const uint32_t gt = C.band[bidx[k]].gt;
+            uint32_t gc = gt;
+            if (raw) {
+                const uint32_t ab = gp + 4 * (tid * PGPT + k);
+                gc = (uint32_t)(pkx_bits64<STAGED>(S, ab) >> 60);
+            } else if ((coded >> k) & 1u) {
+                const uint32_t start = j == 0 ? gp : C.zpos[j - 1] + 1;
+                const uint32_t run = C.zpos[j] - start;
+                j++;
+                if (run >= 32) { /* no code of 32 bits or more (Packing.c:100-107) */
+                    status = 1;
+                    gc = gt;
+                } else {
+                    gc = gt + run;
+                }
+            }
+            gcs[k] = gc;
+            nbits += gc > gt ? 4 * (gc - gt + 1) : 0u;
+        }
+        uint32_t dtotal;
+        const uint32_t dbase = dblock_scan(nbits, C.scan, &dtotal);
+        /* each group's data */
+        uint32_t gbp = dp + dbase; /* bit position of the group's data */
+#pragma unroll
+        for (uint32_t k = 0; k < PGPT; k++) {
+            if (tid * PGPT + k >= cn || err)
+                continue;
+            const PkBand& B = C.band[bidx[k]];
+            const uint32_t gt = B.gt;
+            uint32_t gc = gcs[k];
+            uint32_t np = gc > gt ? gc - gt : 0u;
+            const uint32_t nb = np ? 4 * (np + 1) : 0u;
+            const uint32_t gbp_k = gbp;
+            gbp += nb;
+            uint64_t v = 0;
+            if (nb) {
+                if ((uint64_t)gbp_k + nb > d_end) {
+                    status = 1;
+                    gc = gt; /* decoded as zeros */
+                    np = 0;
+                } else {
+                    v = pkx_bits64<STAGED>(S, gbp_k + V.sh);
+                }
+            }
+            uint2 out;
+            if (qpih == 0 && np <= 8 && gt <= 15 && np + gt <= 15) {
+                out = group_fast(v, np, gt);
+            } else {
+                uint32_t mag[4] = {0, 0, 0, 0}, sg = 0;
+                if (np) {
+                    sg = (uint32_t)(v >> 60);
+                    if (np <= 15) {
+#pragma unroll
+                        for (int t4 = 0; t4 < 4; t4++)
+                            mag[t4] = ((gather4(v << (4 + t4)) >> (16 - np)) << gt) & 0xFFFFu;
+                    } else { /* only in a damaged codestream: planes past the first window */
+                        group_words(V, (uint64_t)gbp_k + 4 + V.sh, np, gt, mag);
+                    }
+                }
+                uint32_t val[4];
+#pragma unroll
+                for (uint32_t t4 = 0; t4 < 4; t4++)
+                    val[t4] = (uint16_t)coef_final(mag[t4] | ((sg >> (3 - t4)) & 1u) << 15, gc, gt, qpih);
+                out = make_uint2(val[0] | val[1] << 16, val[2] | val[3] << 16);
+            }
+            const uint32_t i = i0 + k - B.g0; /* group within its line */
+            int16_t* cp = B.row + 4 * i;
+            if (4 * i + 4 <= B.w && ((uintptr_t)cp & 7) == 0) {
+                *(uint2*)cp = out;
+            } else {
+                const uint32_t val[4] = {out.x & 0xFFFFu, out.x >> 16, out.y & 0xFFFFu, out.y >> 16};
+                for (uint32_t t4 = 0; t4 < 4 && 4 * i + t4 < B.w; t4++)
+                    cp[t4] = (int16_t)val[t4];
+            }
+        }
+        if (err)
+            break;
+        /* the next pass starts after this pass's codes and data */
+        if (raw)
+            gp += 4 * cn;
+        else if (ncodes)
+            gp = __builtin_amdgcn_readfirstlane(C.zpos[ncodes - 1] + 1);
+        dp += __builtin_amdgcn_readfirstlane(dtotal);
+        __syncthreads(); /* zpos and the scan words are reused */
+    }
+    return err || __syncthreads_or(status);
+}
+
 /* One block per (packet, precinct).  Inside a packet the significance,
  * GCLI and data sub-packets each concatenate the packet's band lines at bit
  * granularity (Packing.c:513-868), so the packet is decoded as one sequence
@@ -412,21 +637,29 @@ __global__ void __launch_bounds__(PK_THREADS) __attribute__((amdgpu_waves_per_eu
     gu32* E = T + JXS_DPKT_HDR + JXS_DPKT_WORDS * p;
     const uint32_t fl = E[4];
     /* the packet's words, from its significance sub-packet to two words past
-     * its data, staged in LDS by the whole block with independent loads */
+     * its data (every word a pass reads: the significance flags, the GCLI
+     * codes and each group's 64-bit data window), staged in LDS by the whole
+     * block with independent loads; words past the codestream stage as 0, as
+     * cs_word reads them */
     uint32_t st_lo = 0, st_n = 0;
-    if (fl & 1u) {
-        const uint32_t lo = (E[0] + V.sh) >> 5;
-        const uint32_t hi = min(V.nw, (uint32_t)(((uint64_t)E[3] + V.sh + 31) >> 5) + 2u);
+    if ((fl & 1u) && V.nw > 0) {
+        const uint64_t lo = (E[0] + (uint64_t)V.sh) >> 5;
+        const uint64_t hi = (((uint64_t)E[3] + V.sh + 31) >> 5) + 2u;
         if (hi > lo && hi - lo <= PK_STAGE) {
-            st_lo = lo;
-            st_n = hi - lo;
+            st_lo = (uint32_t)lo;
+            st_n = (uint32_t)(hi - lo);
         }
     }
+    st_lo = __builtin_amdgcn_readfirstlane(st_lo);
+    st_n = __builtin_amdgcn_readfirstlane(st_n);
     for (uint32_t k0 = 0; k0 < st_n; k0 += 4 * PK_THREADS) {
         uint32_t sw[4];
 #pragma unroll
-        for (uint32_t j = 0; j < 4; j++)
-            sw[j] = __builtin_bswap32(V.w[st_lo + min(k0 + j * PK_THREADS + tid, st_n - 1)]);
+        for (uint32_t j = 0; j < 4; j++) {
+            const uint32_t wi = st_lo + min(k0 + j * PK_THREADS + tid, st_n - 1);
+            const uint32_t wv = V.w[min(wi, V.nw - 1)];
+            sw[j] = wi < V.nw ? __builtin_bswap32(wv) : 0u;
+        }
 #pragma unroll
         for (uint32_t j = 0; j < 4; j++) { /* past the packet: into the spare word */
             const uint32_t k = k0 + j * PK_THREADS + tid;
@@ -503,174 +736,18 @@ __global__ void __launch_bounds__(PK_THREADS) __attribute__((amdgpu_waves_per_eu
     if (!C.ok)
         return;
     /* block-uniform values, kept in scalar registers */
-    const uint32_t nband = __builtin_amdgcn_readfirstlane(C.nband);
-    const uint32_t total = __builtin_amdgcn_readfirstlane(C.band[nband].g0);
-    const uint32_t raw = __builtin_amdgcn_readfirstlane(C.raw);
-    const uint32_t s_bit = __builtin_amdgcn_readfirstlane(C.s_bit + V.sh);
-    const uint32_t gend = __builtin_amdgcn_readfirstlane(C.g_end + V.sh);
-    const uint32_t d_end = __builtin_amdgcn_readfirstlane(C.d_end);
-    uint32_t gp = __builtin_amdgcn_readfirstlane(C.g_bit + V.sh); /* aligned bit position of the pass's first GCLI code */
-    uint32_t dp = __builtin_amdgcn_readfirstlane(C.d_bit);        /* bit position of the pass's first data bit */
+    PkPass P0;
+    P0.nband = __builtin_amdgcn_readfirstlane(C.nband);
+    P0.total = __builtin_amdgcn_readfirstlane(C.band[C.nband].g0);
+    P0.raw = __builtin_amdgcn_readfirstlane(C.raw);
+    P0.s_bit = __builtin_amdgcn_readfirstlane(C.s_bit + V.sh);
+    P0.gend = __builtin_amdgcn_readfirstlane(C.g_end + V.sh);
+    P0.d_end = __builtin_amdgcn_readfirstlane(C.d_end);
+    P0.gp = __builtin_amdgcn_readfirstlane(C.g_bit + V.sh);
+    P0.dp = __builtin_amdgcn_readfirstlane(C.d_bit);
+    P0.qpih = g.qpih;
     const PkSrc S = {V, C.stg, st_lo, st_n};
-    bool err = false;
-    for (uint32_t c0 = 0; c0 < total; c0 += PMAXG) {
-        const uint32_t cn = min((uint32_t)PMAXG, total - c0);
-        const uint32_t i0 = c0 + tid * PGPT; /* this thread's first group */
-        /* band line of the first group (binary search over the lines' first groups) */
-        uint32_t bl = 0;
-        {
-            uint32_t lo = 0, hi = nband - 1;
-            while (lo < hi) {
-                const uint32_t mid = (lo + hi + 1) >> 1;
-                if (C.band[mid].g0 <= i0)
-                    lo = mid;
-                else
-                    hi = mid - 1;
-            }
-            bl = lo;
-        }
-        /* significance: a coded group is one whose group of 8 is significant */
-        uint32_t bidx[PGPT], coded = 0, ncoded = 0;
-#pragma unroll
-        for (uint32_t k = 0; k < PGPT; k++) {
-            const uint32_t i = i0 + k;
-            bidx[k] = 0;
-            if (tid * PGPT + k >= cn)
-                continue;
-            while (C.band[bl + 1].g0 <= i)
-                bl++;
-            bidx[k] = bl;
-            bool cd = true;
-            if (!raw && C.band[bl].sig) {
-                const uint32_t ab = s_bit + C.band[bl].s0 + (i - C.band[bl].g0) / 8;
-                cd = ((pk_word(S, ab >> 5) >> (31 - (ab & 31))) & 1u) == 0; /* flag 1: insignificant */
-            }
-            coded |= (cd ? 1u : 0u) << k;
-            ncoded += cd ? 1u : 0u;
-        }
-        uint32_t ncodes;
-        const uint32_t cbase = dblock_scan(ncoded, C.scan, &ncodes);
-        /* GCLI codes */
-        if (!raw) {
-            uint32_t found = 0;
-            for (uint32_t w = gp >> 5; found < ncodes; w += blockDim.x) {
-                if ((uint64_t)w * 32 >= gend) {
-                    err = true;
-                    break;
-                }
-                const uint32_t wi = w + tid;
-                uint32_t word = 0xFFFFFFFFu;
-                if ((uint64_t)wi * 32 < gend) {
-                    word = pk_word(S, wi);
-                    const uint32_t lo = wi * 32;
-                    if (lo < gp)
-                        word |= ~(0xFFFFFFFFu >> (gp - lo)); /* bits before the pass's first code */
-                    if (lo + 32 > gend)
-                        word |= (0xFFFFFFFFu >> (gend - lo)); /* bits after the sub-packet end */
-                }
-                uint32_t z = ~word;
-                uint32_t t;
-                const uint32_t ex = dblock_scan((uint32_t)__popc(z), C.scan, &t);
-                uint32_t k = found + ex;
-                while (z) {
-                    const uint32_t lead = __clz(z);
-                    if (k < ncodes)
-                        C.zpos[k] = wi * 32 + lead;
-                    k++;
-                    z &= ~(0x80000000u >> lead);
-                }
-                found += t;
-            }
-            __syncthreads();
-        }
-        /* GCLI of every group, data bit counts */
-        uint32_t gcs[PGPT], nbits = 0;
-        bool run_err = false;
-        uint32_t j = cbase;
-#pragma unroll
-        for (uint32_t k = 0; k < PGPT; k++) {
-            gcs[k] = 0;
-            if (tid * PGPT + k >= cn || err)
-                continue;
-            const uint32_t gt = C.band[bidx[k]].gt;
-            uint32_t gc = gt;
-            if (raw) {
-                const uint32_t ab = gp + 4 * (tid * PGPT + k);
-                gc = (uint32_t)(pk_bits64(S, ab) >> 60);
-            } else if ((coded >> k) & 1u) {
-                const uint32_t start = j == 0 ? gp : C.zpos[j - 1] + 1;
-                const uint32_t run = C.zpos[j] - start;
-                j++;
-                if (run >= 32) { /* no code of 32 bits or more (Packing.c:100-107) */
-                    run_err = true;
-                    gc = gt;
-                } else {
-                    gc = gt + run;
-                }
-            }
-            gcs[k] = gc;
-            nbits += gc > gt ? 4 * (gc - gt + 1) : 0u;
-        }
-        if (run_err)
-            atomicOr(F.status, 1u);
-        uint32_t dtotal;
-        const uint32_t dbase = dblock_scan(nbits, C.scan, &dtotal);
-        /* each group's data (from LDS when the packet is staged) */
-        uint32_t gbp = dp + dbase; /* bit position of the group's data */
-#pragma unroll
-        for (uint32_t k = 0; k < PGPT; k++) {
-            if (tid * PGPT + k >= cn || err)
-                continue;
-            const PkBand& B = C.band[bidx[k]];
-            const uint32_t gt = B.gt;
-            uint32_t gc = gcs[k];
-            const uint32_t nb = gc > gt ? 4 * (gc - gt + 1) : 0u;
-            const uint32_t gbp_k = gbp;
-            gbp += nb;
-            uint64_t v = 0;
-            if (nb) {
-                if ((uint64_t)gbp_k + nb > d_end) {
-                    atomicOr(F.status, 1u);
-                    gc = gt; /* decoded as zeros */
-                } else {
-                    v = pk_bits64(S, (uint64_t)gbp_k + V.sh);
-                }
-            }
-            const uint32_t i = i0 + k - B.g0; /* group within its line */
-            uint32_t mag[4] = {0, 0, 0, 0}, sg = 0;
-            if (gc > gt) {
-                sg = (uint32_t)(v >> 60);
-                const uint32_t np = gc - gt;
-                if (np <= 15) {
-#pragma unroll
-                    for (int t4 = 0; t4 < 4; t4++)
-                        mag[t4] = ((gather4(v << (4 + t4)) >> (16 - np)) << gt) & 0xFFFFu;
-                } else { /* only in a damaged codestream: planes past the first window */
-                    group_words(V, (uint64_t)gbp_k + 4 + V.sh, np, gt, mag);
-                }
-            }
-            uint32_t val[4];
-#pragma unroll
-            for (uint32_t t4 = 0; t4 < 4; t4++)
-                val[t4] = (uint16_t)coef_final(mag[t4] | ((sg >> (3 - t4)) & 1u) << 15, gc, gt, g.qpih);
-            int16_t* cp = B.row + 4 * i;
-            if (4 * i + 4 <= B.w && ((uintptr_t)cp & 7) == 0) {
-                *(uint2*)cp = make_uint2(val[0] | val[1] << 16, val[2] | val[3] << 16);
-            } else {
-                for (uint32_t t4 = 0; t4 < 4 && 4 * i + t4 < B.w; t4++)
-                    cp[t4] = (int16_t)val[t4];
-            }
-        }
-        if (err)
-            break;
-        /* the next pass starts after this pass's codes and data */
-        if (raw)
-            gp += 4 * cn;
-        else if (ncodes)
-            gp = __builtin_amdgcn_readfirstlane(C.zpos[ncodes - 1] + 1);
-        dp += __builtin_amdgcn_readfirstlane(dtotal);
-        __syncthreads(); /* zpos and the scan words are reused */
-    }
+    const bool err = st_n ? unpack_passes<true>(C, S, P0, tid) : unpack_passes<false>(C, S, P0, tid);
     if (err && tid == 0)
         atomicOr(F.status, 1u);
 }

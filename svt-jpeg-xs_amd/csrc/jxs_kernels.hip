@@ -38,6 +38,7 @@
 #include <stdint.h>
 
 #include "jxs_geom.h"
+#include "jxs_wave.h"
 
 #define WAVE 64
 
@@ -939,23 +940,8 @@ __device__ __forceinline__ bool bsearch_masks(uint32_t m1, uint32_t m3, int32_t 
     return best != -1;
 }
 
-/* Inclusive sum over the 64 lanes of a wave with DPP moves: shifts by 1, 2,
- * 4, 8 inside each row of 16 lanes, then the row totals broadcast from lane
- * 15 into rows 1 and 3 and from lane 31 into rows 2 and 3.  Every lane of
- * the wave must be active. */
-#define DPP_ROW_SHR(n) (0x110 + (n))
-#define DPP_ROW_BCAST15 0x142
-#define DPP_ROW_BCAST31 0x143
-__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v, uint32_t /*lane*/) {
-    int x = (int)v;
-    x += __builtin_amdgcn_update_dpp(0, x, DPP_ROW_SHR(1), 0xF, 0xF, true);
-    x += __builtin_amdgcn_update_dpp(0, x, DPP_ROW_SHR(2), 0xF, 0xF, true);
-    x += __builtin_amdgcn_update_dpp(0, x, DPP_ROW_SHR(4), 0xF, 0xF, true);
-    x += __builtin_amdgcn_update_dpp(0, x, DPP_ROW_SHR(8), 0xF, 0xF, true);
-    x += __builtin_amdgcn_update_dpp(0, x, DPP_ROW_BCAST15, 0xA, 0xF, false);
-    x += __builtin_amdgcn_update_dpp(0, x, DPP_ROW_BCAST31, 0xC, 0xF, false);
-    return (uint32_t)x;
-}
+/* Inclusive sum over the 64 lanes of a wave (every lane active). */
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v, uint32_t /*lane*/) { return wave_scan_dpp(v); }
 
 __device__ __forceinline__ uint32_t wave_last(uint32_t v) {
     return (uint32_t)__builtin_amdgcn_readlane((int)v, WAVE - 1);
