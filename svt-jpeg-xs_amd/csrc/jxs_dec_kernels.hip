@@ -1057,6 +1057,12 @@ __global__ void __launch_bounds__(WAVE) k_dec_slice(const JxsDecFrame* __restric
  * samples, rounded and clamped. */
 __device__ __forceinline__ uint32_t nlt_out(int32_t v, uint32_t depth, uint32_t bw) {
     const int32_t dz = (int32_t)bw - (int32_t)depth;
+    if (__builtin_expect(v > -(1 << 28) && v < (1 << 28) && bw <= 24, 1)) {
+        /* the same in 32 bits: no sum below can leave the int32 range */
+        const int32_t y = (v + ((1 << bw) >> 1) + ((1 << dz) >> 1)) >> dz;
+        const int32_t m = (1 << depth) - 1;
+        return (uint32_t)(y < 0 ? 0 : (y > m ? m : y));
+    }
     int64_t x = (int64_t)v + ((1ll << bw) >> 1);
     x = (x + ((1ll << dz) >> 1)) >> dz;
     const int64_t m = (1ll << depth) - 1;
