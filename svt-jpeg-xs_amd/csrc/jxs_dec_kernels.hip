@@ -445,6 +445,7 @@ __device__ __forceinline__ bool unpack_passes(UnpackCtx& C, const PkSrc& S, cons
     for (uint32_t c0 = 0; c0 < total; c0 += PMAXG) {
         const uint32_t cn = min((uint32_t)PMAXG, total - c0);
         const uint32_t i0 = c0 + tid * PGPT; /* this thread's first group */
+        const uint32_t nv = tid * PGPT < cn ? min((uint32_t)PGPT, cn - tid * PGPT) : 0u; /* its groups in the pass */
         /* band line of the first group (binary search over the lines' first groups) */
         uint32_t bl = 0;
         {
@@ -458,24 +459,52 @@ __device__ __forceinline__ bool unpack_passes(UnpackCtx& C, const PkSrc& S, cons
             }
             bl = lo;
         }
+        /* uni: the thread's PGPT groups lie in one band line (all but the
+         * threads at a line's end and the pass's last ones), whose fields are
+         * then read once */
+        const uint32_t bl0 = bl, g00 = C.band[bl0].g0;
+        const bool uni = nv == PGPT && i0 + PGPT <= C.band[bl0 + 1].g0;
         /* significance: a coded group is one whose group of 8 is significant */
-        uint32_t bidx[PGPT], coded = 0, ncoded = 0;
+        uint32_t bidx[PGPT], gts[PGPT], coded = 0, ncoded = 0;
+        if (uni) {
+            const uint32_t gt0 = C.band[bl0].gt;
 #pragma unroll
-        for (uint32_t k = 0; k < PGPT; k++) {
-            const uint32_t i = i0 + k;
-            bidx[k] = 0;
-            if (tid * PGPT + k >= cn)
-                continue;
-            while (C.band[bl + 1].g0 <= i)
-                bl++;
-            bidx[k] = bl;
-            bool cd = true;
-            if (!raw && C.band[bl].sig) {
-                const uint32_t ab = s_bit + C.band[bl].s0 + (i - C.band[bl].g0) / 8;
-                cd = ((pkx_word<STAGED>(S, ab >> 5) >> (31 - (ab & 31))) & 1u) == 0; /* flag 1: insignificant */
+            for (uint32_t k = 0; k < PGPT; k++) {
+                bidx[k] = bl0;
+                gts[k] = gt0;
             }
-            coded |= (cd ? 1u : 0u) << k;
-            ncoded += cd ? 1u : 0u;
+            coded = (1u << PGPT) - 1;
+            if (!raw && C.band[bl0].sig) {
+                /* the flags of the groups of 8 holding groups i0 and i0 + 3 */
+                const uint32_t rel = i0 - g00, ab = s_bit + C.band[bl0].s0 + rel / 8;
+                const uint32_t sh = ab & 31, a0 = pkx_word<STAGED>(S, ab >> 5), a1 = pkx_word<STAGED>(S, (ab >> 5) + 1);
+                const uint32_t two = (sh ? (a0 << sh) | (a1 >> (32 - sh)) : a0) >> 30;
+                const uint32_t r = rel & 7;
+                const uint32_t m0 = r + PGPT <= 8 ? (1u << PGPT) - 1 : (1u << (8 - r)) - 1; /* groups under the first flag */
+                const uint32_t insig = ((two & 2u) ? m0 : 0u) | ((two & 1u) ? ((1u << PGPT) - 1) & ~m0 : 0u);
+                coded &= ~insig;
+            }
+            ncoded = (uint32_t)__popc(coded);
+        } else {
+#pragma unroll
+            for (uint32_t k = 0; k < PGPT; k++) {
+                const uint32_t i = i0 + k;
+                bidx[k] = 0;
+                gts[k] = 0;
+                if (k >= nv)
+                    continue;
+                while (C.band[bl + 1].g0 <= i)
+                    bl++;
+                bidx[k] = bl;
+                gts[k] = C.band[bl].gt;
+                bool cd = true;
+                if (!raw && C.band[bl].sig) {
+                    const uint32_t ab = s_bit + C.band[bl].s0 + (i - C.band[bl].g0) / 8;
+                    cd = ((pkx_word<STAGED>(S, ab >> 5) >> (31 - (ab & 31))) & 1u) == 0; /* flag 1: insignificant */
+                }
+                coded |= (cd ? 1u : 0u) << k;
+                ncoded += cd ? 1u : 0u;
+            }
         }
         uint32_t ncodes;
         const uint32_t cbase = dblock_scan(ncoded, C.scan, &ncodes);
@@ -519,9 +548,9 @@ __device__ __forceinline__ bool unpack_passes(UnpackCtx& C, const PkSrc& S, cons
 #pragma unroll
         for (uint32_t k = 0; k < PGPT; k++) {
             gcs[k] = 0;
-            if (tid * PGPT + k >= cn || err)
+            if (k >= nv || err)
                 continue;
-            const uint32_t gt = C.band[bidx[k]].gt;
+            const uint32_t gt = gts[k];
             uint32_t gc = gt;
             if (raw) {
                 const uint32_t ab = gp + 4 * (tid * PGPT + k);
@@ -544,12 +573,13 @@ __device__ __forceinline__ bool unpack_passes(UnpackCtx& C, const PkSrc& S, cons
         const uint32_t dbase = dblock_scan(nbits, C.scan, &dtotal);
         /* each group's data */
         uint32_t gbp = dp + dbase; /* bit position of the group's data */
+        uint2 outs[PGPT];
 #pragma unroll
         for (uint32_t k = 0; k < PGPT; k++) {
-            if (tid * PGPT + k >= cn || err)
+            outs[k] = make_uint2(0, 0);
+            if (k >= nv || err)
                 continue;
-            const PkBand& B = C.band[bidx[k]];
-            const uint32_t gt = B.gt;
+            const uint32_t gt = gts[k];
             uint32_t gc = gcs[k];
             uint32_t np = gc > gt ? gc - gt : 0u;
             const uint32_t nb = np ? 4 * (np + 1) : 0u;
@@ -565,35 +595,49 @@ __device__ __forceinline__ bool unpack_passes(UnpackCtx& C, const PkSrc& S, cons
                     v = pkx_bits64<STAGED>(S, gbp_k + V.sh);
                 }
             }
-            uint2 out;
-            if (qpih == 0 && np <= 8 && gt <= 15 && np + gt <= 15) {
-                out = group_fast(v, np, gt);
+            if (np == 0) {
+                /* no data: four zeros */
+            } else if (qpih == 0 && np <= 8 && gt <= 15 && np + gt <= 15) {
+                outs[k] = group_fast(v, np, gt);
             } else {
-                uint32_t mag[4] = {0, 0, 0, 0}, sg = 0;
-                if (np) {
-                    sg = (uint32_t)(v >> 60);
-                    if (np <= 15) {
+                uint32_t mag[4] = {0, 0, 0, 0};
+                const uint32_t sg = (uint32_t)(v >> 60);
+                if (np <= 15) {
 #pragma unroll
-                        for (int t4 = 0; t4 < 4; t4++)
-                            mag[t4] = ((gather4(v << (4 + t4)) >> (16 - np)) << gt) & 0xFFFFu;
-                    } else { /* only in a damaged codestream: planes past the first window */
-                        group_words(V, (uint64_t)gbp_k + 4 + V.sh, np, gt, mag);
-                    }
+                    for (int t4 = 0; t4 < 4; t4++)
+                        mag[t4] = ((gather4(v << (4 + t4)) >> (16 - np)) << gt) & 0xFFFFu;
+                } else { /* only in a damaged codestream: planes past the first window */
+                    group_words(V, (uint64_t)gbp_k + 4 + V.sh, np, gt, mag);
                 }
                 uint32_t val[4];
 #pragma unroll
                 for (uint32_t t4 = 0; t4 < 4; t4++)
                     val[t4] = (uint16_t)coef_final(mag[t4] | ((sg >> (3 - t4)) & 1u) << 15, gc, gt, qpih);
-                out = make_uint2(val[0] | val[1] << 16, val[2] | val[3] << 16);
+                outs[k] = make_uint2(val[0] | val[1] << 16, val[2] | val[3] << 16);
             }
-            const uint32_t i = i0 + k - B.g0; /* group within its line */
-            int16_t* cp = B.row + 4 * i;
-            if (4 * i + 4 <= B.w && ((uintptr_t)cp & 7) == 0) {
-                *(uint2*)cp = out;
-            } else {
-                const uint32_t val[4] = {out.x & 0xFFFFu, out.x >> 16, out.y & 0xFFFFu, out.y >> 16};
-                for (uint32_t t4 = 0; t4 < 4 && 4 * i + t4 < B.w; t4++)
-                    cp[t4] = (int16_t)val[t4];
+        }
+        /* stores: the thread's groups in one line and inside its width go out
+         * as 8-byte words from one base */
+        int16_t* cp0 = C.band[bl0].row + 4 * (i0 - g00);
+        if (uni && !err && 4 * (i0 - g00) + 4 * PGPT <= C.band[bl0].w && ((uintptr_t)cp0 & 7) == 0) {
+#pragma unroll
+            for (uint32_t k = 0; k < PGPT; k++)
+                *(uint2*)(cp0 + 4 * k) = outs[k];
+        } else {
+#pragma unroll
+            for (uint32_t k = 0; k < PGPT; k++) {
+                if (k >= nv || err)
+                    continue;
+                const PkBand& B = C.band[bidx[k]];
+                const uint32_t i = i0 + k - B.g0; /* group within its line */
+                int16_t* cp = B.row + 4 * i;
+                if (4 * i + 4 <= B.w && ((uintptr_t)cp & 7) == 0) {
+                    *(uint2*)cp = outs[k];
+                } else {
+                    const uint32_t val[4] = {outs[k].x & 0xFFFFu, outs[k].x >> 16, outs[k].y & 0xFFFFu, outs[k].y >> 16};
+                    for (uint32_t t4 = 0; t4 < 4 && 4 * i + t4 < B.w; t4++)
+                        cp[t4] = (int16_t)val[t4];
+                }
             }
         }
         if (err)
@@ -607,6 +651,12 @@ __device__ __forceinline__ bool unpack_passes(UnpackCtx& C, const PkSrc& S, cons
         __syncthreads(); /* zpos and the scan words are reused */
     }
     return err || __syncthreads_or(status);
+}
+
+/* The passes of a packet too long to stage (read from HBM): a call, so
+ * that its registers do not weigh on the staged path. */
+__device__ __noinline__ bool unpack_passes_hbm(UnpackCtx& C, const PkSrc& S, const PkPass& P0, uint32_t tid) {
+    return unpack_passes<false>(C, S, P0, tid);
 }
 
 /* One block per (packet, precinct).  Inside a packet the significance,
@@ -747,7 +797,7 @@ __global__ void __launch_bounds__(PK_THREADS) __attribute__((amdgpu_waves_per_eu
     P0.dp = __builtin_amdgcn_readfirstlane(C.d_bit);
     P0.qpih = g.qpih;
     const PkSrc S = {V, C.stg, st_lo, st_n};
-    const bool err = st_n ? unpack_passes<true>(C, S, P0, tid) : unpack_passes<false>(C, S, P0, tid);
+    const bool err = st_n ? unpack_passes<true>(C, S, P0, tid) : unpack_passes_hbm(C, S, P0, tid);
     if (err && tid == 0)
         atomicOr(F.status, 1u);
 }
