@@ -1139,9 +1139,12 @@ __device__ __forceinline__ int find_base(const uint32_t* bases, int n, uint32_t 
     return lo;
 }
 
+/* MAXW: the widest row of the launch (LDS and the per-thread sample
+ * registers are sized for it). */
+template <int MAXW>
 __global__ void __launch_bounds__(256) k_inv_rows(const JxsInvRowJob* __restrict__ jobs, const uint32_t* __restrict__ bases,
                                                   int njobs) {
-    __shared__ __align__(16) int32_t buf[JXS_MAX_ROW];
+    __shared__ __align__(16) int32_t buf[MAXW];
     const JxsInvRowJob& J = jobs[find_base(bases, njobs, blockIdx.x)];
     const uint32_t y = blockIdx.x - J.base;
     uint32_t n[6];
@@ -1150,14 +1153,14 @@ __global__ void __launch_bounds__(256) k_inv_rows(const JxsInvRowJob* __restrict
         n[k + 1] = (n[k] + 1) / 2;
     const uint32_t L = J.levels;
     /* Every thread issues the loads of all its samples of the row (low band,
-     * then each level's high band; at most JXS_MAX_ROW / 256 of a band)
+     * then each level's high band; at most MAXW / 256 of a band)
      * before it writes any into LDS, so the row costs one memory round trip. */
     const uint32_t tid = threadIdx.x;
     const size_t lfo = (size_t)y * J.lf_stride;
     const int32_t* lf32 = J.lf32 ? J.lf32 + lfo : nullptr;
     const int16_t* lf16 = J.lf16 + (J.lf32 ? 0 : lfo);
     const uint32_t nL = n[L];
-    constexpr int NL = (JXS_MAX_ROW + 255) / 256;
+    constexpr int NL = (MAXW + 255) / 256;
     int32_t vl[NL];
 #pragma unroll
     for (int j = 0; j < NL; j++)
@@ -1173,10 +1176,10 @@ __global__ void __launch_bounds__(256) k_inv_rows(const JxsInvRowJob* __restrict
             if (tid + 256u * j < nL)
                 vl[j] = gld(lf16 + tid + 256u * j); /* scaled by 2^Fq when stored */
     }
-    /* high band of level k: floor(n[k] / 2) <= JXS_MAX_ROW >> (k + 1) samples */
-    constexpr int NH0 = ((JXS_MAX_ROW >> 1) + 255) / 256, NH1 = ((JXS_MAX_ROW >> 2) + 255) / 256,
-                  NH2 = ((JXS_MAX_ROW >> 3) + 255) / 256, NH3 = ((JXS_MAX_ROW >> 4) + 255) / 256,
-                  NH4 = ((JXS_MAX_ROW >> 5) + 255) / 256;
+    /* high band of level k: floor(n[k] / 2) <= MAXW >> (k + 1) samples */
+    constexpr int NH0 = ((MAXW >> 1) + 255) / 256, NH1 = ((MAXW >> 2) + 255) / 256,
+                  NH2 = ((MAXW >> 3) + 255) / 256, NH3 = ((MAXW >> 4) + 255) / 256,
+                  NH4 = ((MAXW >> 5) + 255) / 256;
     int32_t h0[NH0], h1[NH1], h2[NH2], h3[NH3], h4[NH4]; /* full registers: no 16-bit packing waits */
     auto ldh = [&](auto& arr, auto cnt, uint32_t k) {
         constexpr int N = decltype(cnt)::value;
@@ -1510,10 +1513,20 @@ hipError_t jxs_launch_dec_slice(const JxsDecFrame* frames, int nfr, uint32_t sli
     return hipGetLastError();
 }
 
-hipError_t jxs_launch_inv_rows(const JxsInvRowJob* jobs, const uint32_t* bases, int njobs, uint32_t blocks, hipStream_t st) {
+hipError_t jxs_launch_inv_rows(const JxsInvRowJob* jobs, const uint32_t* bases, int njobs, uint32_t blocks, uint32_t maxw,
+                               hipStream_t st) {
     if (!njobs || !blocks)
         return hipSuccess;
-    hipLaunchKernelGGL(k_inv_rows, dim3(blocks), dim3(256), 0, st, jobs, bases, njobs);
+    if (maxw <= 1024)
+        hipLaunchKernelGGL(k_inv_rows<1024>, dim3(blocks), dim3(256), 0, st, jobs, bases, njobs);
+    else if (maxw <= 2048)
+        hipLaunchKernelGGL(k_inv_rows<2048>, dim3(blocks), dim3(256), 0, st, jobs, bases, njobs);
+    else if (maxw <= 4096)
+        hipLaunchKernelGGL(k_inv_rows<4096>, dim3(blocks), dim3(256), 0, st, jobs, bases, njobs);
+    else if (maxw <= JXS_MAX_ROW)
+        hipLaunchKernelGGL(k_inv_rows<JXS_MAX_ROW>, dim3(blocks), dim3(256), 0, st, jobs, bases, njobs);
+    else
+        return hipErrorInvalidValue;
     return hipGetLastError();
 }
 

@@ -27,7 +27,7 @@ hipError_t jxs_launch_dec_scan(const JxsDecFrame*, int, hipStream_t);
 hipError_t jxs_launch_dec_packets(const JxsDecFrame*, int, uint32_t, hipStream_t);
 hipError_t jxs_launch_dec_unpack(const JxsDecFrame*, int, uint32_t, uint32_t, hipStream_t);
 hipError_t jxs_launch_dec_slice(const JxsDecFrame*, int, uint32_t, hipStream_t);
-hipError_t jxs_launch_inv_rows(const JxsInvRowJob*, const uint32_t*, int, uint32_t, hipStream_t);
+hipError_t jxs_launch_inv_rows(const JxsInvRowJob*, const uint32_t*, int, uint32_t, uint32_t, hipStream_t);
 hipError_t jxs_launch_inv_vert(const JxsInvVertJob*, const uint32_t*, int, uint32_t, hipStream_t);
 hipError_t jxs_launch_inv_last(const JxsInvLastJob*, const uint32_t*, int, uint32_t, hipStream_t);
 hipError_t jxs_launch_zero(uint32_t*, uint32_t, hipStream_t);
@@ -198,6 +198,7 @@ struct svt_jpeg_xs_amd_decode_plan {
     uint32_t* d_vbase[2] = {nullptr, nullptr};
     int nrows[2] = {0, 0}, nvert[2] = {0, 0};
     uint32_t rblocks[2] = {0, 0}, vblocks[2] = {0, 0};
+    uint32_t rmaxw[2] = {0, 0}; /* widest row of each row launch */
     JxsInvLastJob* d_last = nullptr;
     uint32_t* d_lbase = nullptr;
     int nlast = 0;
@@ -310,6 +311,7 @@ static svt_jpeg_xs_amd_decode_plan_t* dplan_build(const std::vector<DecSpec>& sp
             r.base = blk;
             rb.push_back(blk);
             blk += r.rows;
+            p->rmaxw[k] = r.w > p->rmaxw[k] ? r.w : p->rmaxw[k];
         }
         p->rblocks[k] = blk;
         blk = 0;
@@ -358,11 +360,11 @@ static hipError_t dplan_launch(svt_jpeg_xs_amd_decode_plan_t* p, hipStream_t st)
         return e;
     if ((e = jxs_launch_dec_slice(p->d_frames, (int)p->nfr, p->total_slices, st)) != hipSuccess)
         return e;
-    if ((e = jxs_launch_inv_rows(p->d_rows[0], p->d_rbase[0], p->nrows[0], p->rblocks[0], st)) != hipSuccess)
+    if ((e = jxs_launch_inv_rows(p->d_rows[0], p->d_rbase[0], p->nrows[0], p->rblocks[0], p->rmaxw[0], st)) != hipSuccess)
         return e;
     if ((e = jxs_launch_inv_vert(p->d_vert[0], p->d_vbase[0], p->nvert[0], p->vblocks[0], st)) != hipSuccess)
         return e;
-    if ((e = jxs_launch_inv_rows(p->d_rows[1], p->d_rbase[1], p->nrows[1], p->rblocks[1], st)) != hipSuccess)
+    if ((e = jxs_launch_inv_rows(p->d_rows[1], p->d_rbase[1], p->nrows[1], p->rblocks[1], p->rmaxw[1], st)) != hipSuccess)
         return e;
     if ((e = jxs_launch_inv_vert(p->d_vert[1], p->d_vbase[1], p->nvert[1], p->vblocks[1], st)) != hipSuccess)
         return e;
