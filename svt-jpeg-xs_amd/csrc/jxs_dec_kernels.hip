@@ -110,19 +110,24 @@ __device__ __forceinline__ uint32_t gather4(uint64_t x) {
     return (uint32_t)((x | (x >> 24)) & 0xFFFF);
 }
 
+/* Exclusive block scan of v (every thread of the block, NW waves); *total
+ * the block's sum.  One barrier: tmp (NW words) is read after it, so two
+ * scans in a row use different tmp words (a scan's barrier orders the
+ * previous scan's reads before the next scan's writes). */
+template <uint32_t NW>
 __device__ __forceinline__ uint32_t dblock_scan(uint32_t v, uint32_t* tmp, uint32_t* total) {
-    const uint32_t lane = threadIdx.x & (WAVE - 1), wid = threadIdx.x / WAVE, nw = blockDim.x / WAVE;
+    const uint32_t lane = threadIdx.x & (WAVE - 1), wid = threadIdx.x / WAVE;
     const uint32_t x = wave_scan_dpp(v);
     if (lane == WAVE - 1)
         tmp[wid] = x;
     __syncthreads();
     uint32_t before = 0, tot = 0;
-    for (uint32_t w = 0; w < nw; w++) {
+#pragma unroll
+    for (uint32_t w = 0; w < NW; w++) {
         const uint32_t t = tmp[w];
         before += w < wid ? t : 0u;
         tot += t;
     }
-    __syncthreads();
     *total = tot;
     return before + x - v;
 }
@@ -214,7 +219,7 @@ struct UnpackCtx {
     PkBand band[PNB + 1]; /* + a sentinel whose g0 is the packet's group count */
     uint32_t nband, s_bit, g_bit, d_bit, g_end, d_end, raw, ok;
     uint32_t zpos[PMAXG]; /* aligned bit position of the zero ending code j of the pass */
-    uint32_t scan[PK_THREADS / WAVE];
+    uint32_t scan[2][PK_THREADS / WAVE]; /* alternating scan words */
     uint32_t stg[PK_STAGE + 1]; /* the packet's codestream words, byte-swapped, and a spare */
 };
 
@@ -506,8 +511,9 @@ __device__ __forceinline__ bool unpack_passes(UnpackCtx& C, const PkSrc& S, cons
                 ncoded += cd ? 1u : 0u;
             }
         }
-        uint32_t ncodes;
-        const uint32_t cbase = dblock_scan(ncoded, C.scan, &ncodes);
+        uint32_t ncodes, slot = 0; /* scan words: alternate between consecutive scans */
+        const uint32_t cbase = dblock_scan<PK_THREADS / WAVE>(ncoded, C.scan[slot], &ncodes);
+        slot ^= 1;
         ncodes = __builtin_amdgcn_readfirstlane(ncodes);
         /* GCLI codes */
         if (!raw) {
@@ -529,7 +535,8 @@ __device__ __forceinline__ bool unpack_passes(UnpackCtx& C, const PkSrc& S, cons
                 }
                 uint32_t z = ~word;
                 uint32_t t;
-                const uint32_t ex = dblock_scan((uint32_t)__popc(z), C.scan, &t);
+                const uint32_t ex = dblock_scan<PK_THREADS / WAVE>((uint32_t)__popc(z), C.scan[slot], &t);
+                slot ^= 1;
                 uint32_t k = found + ex;
                 while (z) {
                     const uint32_t lead = __clz(z);
@@ -570,7 +577,7 @@ __device__ __forceinline__ bool unpack_passes(UnpackCtx& C, const PkSrc& S, cons
             nbits += gc > gt ? 4 * (gc - gt + 1) : 0u;
         }
         uint32_t dtotal;
-        const uint32_t dbase = dblock_scan(nbits, C.scan, &dtotal);
+        const uint32_t dbase = dblock_scan<PK_THREADS / WAVE>(nbits, C.scan[slot], &dtotal);
         /* each group's data */
         uint32_t gbp = dp + dbase; /* bit position of the group's data */
         uint2 outs[PGPT];
