@@ -511,10 +511,29 @@ __device__ __forceinline__ bool unpack_passes(UnpackCtx& C, const PkSrc& S, cons
                 ncoded += cd ? 1u : 0u;
             }
         }
-        uint32_t ncodes, slot = 0; /* scan words: alternate between consecutive scans */
-        const uint32_t cbase = dblock_scan<PK_THREADS / WAVE>(ncoded, C.scan[slot], &ncodes);
+        /* the zero bits (code ends) of one block-wide stretch of GCLI words
+         * from word w on: this thread's word wi = w + tid, bits outside
+         * [gp, gend) set */
+        auto code_word = [&](uint32_t w) -> uint32_t {
+            const uint32_t wi = w + tid;
+            uint32_t word = 0xFFFFFFFFu;
+            if (!raw && (uint64_t)wi * 32 < gend) {
+                word = pkx_word<STAGED>(S, wi);
+                const uint32_t lo = wi * 32;
+                if (lo < gp)
+                    word |= ~(0xFFFFFFFFu >> (gp - lo)); /* bits before the pass's first code */
+                if (lo + 32 > gend)
+                    word |= (0xFFFFFFFFu >> (gend - lo)); /* bits after the sub-packet end */
+            }
+            return ~word;
+        };
+        /* one scan numbers the coded groups (low half) and the first
+         * stretch's zero bits (high half): both totals stay below 2^16 */
+        uint32_t z0 = code_word(gp >> 5);
+        uint32_t both, slot = 0; /* scan words: alternate between consecutive scans */
+        const uint32_t ex0 = dblock_scan<PK_THREADS / WAVE>(ncoded | (uint32_t)__popc(z0) << 16, C.scan[slot], &both);
         slot ^= 1;
-        ncodes = __builtin_amdgcn_readfirstlane(ncodes);
+        const uint32_t cbase = ex0 & 0xFFFFu, ncodes = __builtin_amdgcn_readfirstlane(both & 0xFFFFu);
         /* GCLI codes */
         if (!raw) {
             uint32_t found = 0;
@@ -524,19 +543,16 @@ __device__ __forceinline__ bool unpack_passes(UnpackCtx& C, const PkSrc& S, cons
                     break;
                 }
                 const uint32_t wi = w + tid;
-                uint32_t word = 0xFFFFFFFFu;
-                if ((uint64_t)wi * 32 < gend) {
-                    word = pkx_word<STAGED>(S, wi);
-                    const uint32_t lo = wi * 32;
-                    if (lo < gp)
-                        word |= ~(0xFFFFFFFFu >> (gp - lo)); /* bits before the pass's first code */
-                    if (lo + 32 > gend)
-                        word |= (0xFFFFFFFFu >> (gend - lo)); /* bits after the sub-packet end */
+                uint32_t z, t, ex;
+                if (w == (gp >> 5)) {
+                    z = z0;
+                    ex = ex0 >> 16;
+                    t = both >> 16;
+                } else {
+                    z = code_word(w);
+                    ex = dblock_scan<PK_THREADS / WAVE>((uint32_t)__popc(z), C.scan[slot], &t);
+                    slot ^= 1;
                 }
-                uint32_t z = ~word;
-                uint32_t t;
-                const uint32_t ex = dblock_scan<PK_THREADS / WAVE>((uint32_t)__popc(z), C.scan[slot], &t);
-                slot ^= 1;
                 uint32_t k = found + ex;
                 while (z) {
                     const uint32_t lead = __clz(z);
