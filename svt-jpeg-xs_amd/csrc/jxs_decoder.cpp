@@ -47,7 +47,7 @@ static bool dec_version_ok(uint64_t maj, uint64_t min) {
  * coarser horizontal levels of the first vertical level's low rows (into an
  * int32 plane at half width); last the first vertical level with the finest
  * horizontal level of its low and high rows. */
-#define INV_SEG 32 /* row pairs per block of the last stage */
+#define INV_SEG 64 /* row pairs per block of the last stage */
 struct InvJobs {
     std::vector<JxsInvRowJob> rows[2];
     std::vector<JxsInvVertJob> vert[2];
