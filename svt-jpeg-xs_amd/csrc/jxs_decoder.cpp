@@ -204,9 +204,20 @@ struct svt_jpeg_xs_amd_decode_plan {
     int nlast = 0;
     uint32_t lblocks = 0;
     uint32_t total_prec = 0, max_npk = 0, total_slices = 0;
+    bool own_status = true; /* false in a part: the status words belong to the whole plan */
+    /* A plan of several frames runs as parts on streams of its own, forked
+     * from and joined back to the caller's stream, so that one part's
+     * unpacking (instruction-bound) overlaps another's inverse transform
+     * (memory-bound). */
+    std::vector<svt_jpeg_xs_amd_decode_plan*> parts;
+    std::vector<hipStream_t> part_st;
+    std::vector<hipEvent_t> part_done;
+    hipEvent_t fork = nullptr;
 };
 
 namespace jxs {
+
+#define DPLAN_PARTS 4 /* parts a plan of several frames runs as, at most */
 
 template <typename T>
 static hipError_t up(const std::vector<T>& v, T** dst) {
@@ -222,8 +233,17 @@ static hipError_t up(const std::vector<T>& v, T** dst) {
 static void dplan_free(svt_jpeg_xs_amd_decode_plan_t* p) {
     if (!p)
         return;
+    for (auto* q : p->parts)
+        dplan_free(q);
+    for (hipStream_t s : p->part_st)
+        (void)hipStreamDestroy(s);
+    for (hipEvent_t e : p->part_done)
+        (void)hipEventDestroy(e);
+    if (p->fork)
+        (void)hipEventDestroy(p->fork);
     (void)hipFree(p->d_work);
-    (void)hipFree(p->d_status);
+    if (p->own_status)
+        (void)hipFree(p->d_status);
     (void)hipFree(p->d_frames);
     for (int s = 0; s < 2; s++) {
         (void)hipFree(p->d_rows[s]);
@@ -246,7 +266,9 @@ struct DecSpec {
     uint32_t prec_stride;
 };
 
-static svt_jpeg_xs_amd_decode_plan_t* dplan_build(const std::vector<DecSpec>& specs) {
+/* status: the frames' status words (a part's slice of its plan's), or NULL
+ * for words of its own */
+static svt_jpeg_xs_amd_decode_plan_t* dplan_build(const std::vector<DecSpec>& specs, uint32_t* status) {
     auto* p = new (std::nothrow) svt_jpeg_xs_amd_decode_plan;
     if (!p || specs.empty()) {
         delete p;
@@ -256,7 +278,11 @@ static svt_jpeg_xs_amd_decode_plan_t* dplan_build(const std::vector<DecSpec>& sp
     uint64_t work = 0;
     for (auto& s : specs)
         work += dec_work_bytes(s.geom->host.g);
-    if (hipMalloc(&p->d_work, work) != hipSuccess || hipMalloc(&p->d_status, p->nfr * 4) != hipSuccess) {
+    if (status) {
+        p->d_status = status;
+        p->own_status = false;
+    }
+    if (hipMalloc(&p->d_work, work) != hipSuccess || (!status && hipMalloc(&p->d_status, p->nfr * 4) != hipSuccess)) {
         dplan_free(p);
         return nullptr;
     }
@@ -348,7 +374,66 @@ static svt_jpeg_xs_amd_decode_plan_t* dplan_build(const std::vector<DecSpec>& sp
     return p;
 }
 
+/* A plan of several frames: DPLAN_PARTS parts of consecutive frames (as
+ * even as the count allows), each built as a plan of its own over the
+ * whole plan's status words. */
+static svt_jpeg_xs_amd_decode_plan_t* dplan_build_parts(const std::vector<DecSpec>& specs) {
+    const uint32_t n = (uint32_t)specs.size();
+    const uint32_t np = n < DPLAN_PARTS ? n : DPLAN_PARTS;
+    if (np <= 1)
+        return dplan_build(specs, nullptr);
+    auto* p = new (std::nothrow) svt_jpeg_xs_amd_decode_plan;
+    if (!p)
+        return nullptr;
+    p->nfr = n;
+    bool ok = hipMalloc(&p->d_status, n * 4) == hipSuccess &&
+        hipEventCreateWithFlags(&p->fork, hipEventDisableTiming) == hipSuccess;
+    uint32_t first = 0;
+    for (uint32_t i = 0; i < np && ok; i++) {
+        const uint32_t cnt = n / np + (i < n % np ? 1u : 0u);
+        std::vector<DecSpec> sub(specs.begin() + first, specs.begin() + first + cnt);
+        svt_jpeg_xs_amd_decode_plan_t* q = dplan_build(sub, p->d_status + first);
+        hipStream_t s = nullptr;
+        hipEvent_t e = nullptr;
+        ok = q != nullptr;
+        if (q)
+            p->parts.push_back(q);
+        if (ok && hipStreamCreateWithFlags(&s, hipStreamNonBlocking) == hipSuccess) /* default priority */
+            p->part_st.push_back(s);
+        else
+            ok = false;
+        if (ok && hipEventCreateWithFlags(&e, hipEventDisableTiming) == hipSuccess)
+            p->part_done.push_back(e);
+        else
+            ok = false;
+        first += cnt;
+    }
+    if (!ok) {
+        dplan_free(p);
+        return nullptr;
+    }
+    return p;
+}
+
+static hipError_t dplan_launch_one(svt_jpeg_xs_amd_decode_plan_t* p, hipStream_t st);
+
 static hipError_t dplan_launch(svt_jpeg_xs_amd_decode_plan_t* p, hipStream_t st) {
+    if (p->parts.empty())
+        return dplan_launch_one(p, st);
+    hipError_t e;
+    if ((e = hipEventRecord(p->fork, st)) != hipSuccess)
+        return e;
+    for (size_t i = 0; i < p->parts.size(); i++) {
+        if ((e = hipStreamWaitEvent(p->part_st[i], p->fork, 0)) != hipSuccess ||
+            (e = dplan_launch_one(p->parts[i], p->part_st[i])) != hipSuccess ||
+            (e = hipEventRecord(p->part_done[i], p->part_st[i])) != hipSuccess ||
+            (e = hipStreamWaitEvent(st, p->part_done[i], 0)) != hipSuccess)
+            return e;
+    }
+    return hipSuccess;
+}
+
+static hipError_t dplan_launch_one(svt_jpeg_xs_amd_decode_plan_t* p, hipStream_t st) {
     hipError_t e;
     if ((e = jxs_launch_zero(p->d_status, p->nfr, st)) != hipSuccess)
         return e;
@@ -560,7 +645,7 @@ static bool slot_bind(DecoderPriv* d, DecSlot& s, uint32_t cap) {
     }
     sp.prec_in = s.d_prec;
     sp.prec_stride = 1;
-    s.plan = dplan_build({sp});
+    s.plan = dplan_build({sp}, nullptr);
     if (!s.plan)
         return false;
     return hipMemcpy(s.h_frame, s.plan->d_frames, sizeof(JxsDecFrame), hipMemcpyDeviceToHost) == hipSuccess;
@@ -1093,7 +1178,7 @@ PREFIX_API svt_jpeg_xs_amd_decode_plan_t* svt_jpeg_xs_amd_decode_plan_create(con
         s.prec_in = f.d_precinct_offsets;
         s.prec_stride = f.precinct_offsets_stride;
     }
-    return dplan_build(specs);
+    return dplan_build_parts(specs);
 }
 
 PREFIX_API SvtJxsErrorType_t svt_jpeg_xs_amd_decode_plan_run(svt_jpeg_xs_amd_decode_plan_t* plan, void* stream) {
