@@ -100,6 +100,18 @@ __device__ __forceinline__ uint64_t cs_bits64(const CsView& v, uint64_t abp) {
     return sh ? (a << sh) | (uint64_t)(c >> (32 - sh)) : a;
 }
 
+/* The 64 bits from bit position bp of the codestream (counted from F.cs),
+ * with bits past its end read as 0, as cs_bits reads them. */
+__device__ __forceinline__ uint64_t cs_bits64z(const JxsDecFrame& F, const CsView& V, uint64_t bp) {
+    const uint64_t end = (uint64_t)F.cs_len * 8;
+    if (bp >= end)
+        return 0;
+    uint64_t w = cs_bits64(V, bp + V.sh);
+    if (end - bp < 64)
+        w &= ~0ull << (64 - (end - bp));
+    return w;
+}
+
 /* Every fourth bit of x, from bit 63 down (bits 63, 59, ..., 3), packed into a
  * 16-bit value with bit 63 as its MSB. */
 __device__ __forceinline__ uint32_t gather4(uint64_t x) {
@@ -269,20 +281,23 @@ __global__ void __launch_bounds__(64) k_dec_packets(const JxsDecFrame* __restric
     }
     const uint32_t type = row + 1 >= g.rows ? 1 : 0;
     const uint64_t off = F.prec_off[row];
-    const uint32_t lprc = cs_bits(F, off * 8, 24);
-    const uint32_t q = cs_byte(F, off + 3), r = cs_byte(F, off + 4);
+    const CsView V = cs_view(F);
+    /* the precinct header and its band modes, read as three 64-bit windows */
+    const uint64_t hw = cs_bits64z(F, V, off * 8), m0 = cs_bits64z(F, V, off * 8 + 40), m1 = cs_bits64z(F, V, off * 8 + 104);
+    const uint32_t lprc = (uint32_t)(hw >> 40);
+    const uint32_t q = (uint32_t)(hw >> 32) & 0xFFu, r = (uint32_t)(hw >> 24) & 0xFFu;
     uint16_t mode[JXS_MAXC * JXS_NBC];
     for (uint32_t k = 0; k < JXS_MAXC * JXS_NBC; k++)
         mode[k] = 0;
-    uint64_t bit = off * 8 + 40;
+    uint32_t nmode = 0; /* band modes read so far: 2 bits each from bit 40 of the header */
     bool bad = false;
     bool ordered = g.fs != 0; /* separate sign sub-packets */
     for (uint32_t gi = 0; gi < g.nb_all; gi++) {
         if (g.gb[gi] == 0xFF)
             continue;
         const uint32_t c = g.gc[gi], b = g.gb[gi];
-        const uint32_t d = cs_bits(F, bit, 2);
-        bit += 2;
+        const uint32_t d = (uint32_t)((nmode < 32 ? m0 >> (62 - 2 * nmode) : m1 >> (62 - 2 * (nmode - 32))) & 3u);
+        nmode++;
         mode[c * JXS_NBC + b] = (uint16_t)(d | dtrunc(g.c[c].b[b].gain, g.c[c].b[b].prio, q, r) << 8);
         if (d & 1)
             ordered = true; /* vertical prediction: needs the line above */
@@ -309,16 +324,16 @@ __global__ void __launch_bounds__(64) k_dec_packets(const JxsDecFrame* __restric
             continue;
         }
         uint32_t raw, ld, lg, ls;
+        const uint64_t ph = cs_bits64z(F, V, pos * 8); /* the packet header */
+        raw = (uint32_t)(ph >> 63);
         if (g.short_hdr) {
-            raw = cs_bits(F, pos * 8, 1);
-            ld = cs_bits(F, pos * 8 + 1, 15);
-            lg = cs_bits(F, pos * 8 + 16, 13);
-            ls = cs_bits(F, pos * 8 + 29, 11);
+            ld = (uint32_t)(ph >> 48) & 0x7FFFu;
+            lg = (uint32_t)(ph >> 35) & 0x1FFFu;
+            ls = (uint32_t)(ph >> 24) & 0x7FFu;
         } else {
-            raw = cs_bits(F, pos * 8, 1);
-            ld = cs_bits(F, pos * 8 + 1, 20);
-            lg = cs_bits(F, pos * 8 + 21, 20);
-            ls = cs_bits(F, pos * 8 + 41, 15);
+            ld = (uint32_t)(ph >> 43) & 0xFFFFFu;
+            lg = (uint32_t)(ph >> 23) & 0xFFFFFu;
+            ls = (uint32_t)(ph >> 8) & 0x7FFFu;
         }
         /* the three lengths must fit what is left of the precinct after the
          * header, the sign length counted even without a sign sub-packet
