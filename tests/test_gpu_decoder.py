@@ -22,6 +22,10 @@ CASES = [
     (96, 40, 8, 3, 3, 0, 2, 0, 1),
     (131, 37, 8, 4, 4, 0, 1, 0, 1),
     (250, 30, 12, 3, 3, 0, 4, 0, 1),
+    # one 7,680-sample line of three components per precinct at bpp 20: a
+    # packet of about 19 KB, longer than the unpack's LDS stage (12 KiB), so
+    # it is read from HBM
+    (7680, 6, 8, 4, 20, 0, 1, 0, 1),
 ]
 
 
