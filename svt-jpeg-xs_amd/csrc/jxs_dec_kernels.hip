@@ -7,21 +7,27 @@
  *   k_dec_packets one lane per precinct: band D-modes and GTLIs, and the
  *                 packet chain walked once into a table of sub-packet
  *                 offsets.
- *   k_dec_unpack  one block per (packet, precinct): the packet's band lines
- *                 decoded as one group sequence, 2048 groups per pass: the
- *                 significance flags, the unary bit-plane counts located all
- *                 at once (code j ends at the j-th zero bit: zero counts per
- *                 32-bit word + block prefix sum), then each group's data bits
- *                 at prefix-summed offsets, read as one 64-bit window,
- *                 dequantised and turned into two's complement
- *                 (Packing.c:513-868, Dequant.c:9-57, Precinct.c:9-29).
+ *   k_dec_unpack  one block per (packet, precinct): the packet staged in LDS,
+ *                 its band lines decoded as one group sequence, 1,024 groups
+ *                 per pass: the significance flags, the unary bit-plane
+ *                 counts located all at once (code j ends at the j-th zero
+ *                 bit: zero counts per 32-bit word + block prefix sum), then
+ *                 each group's data bits at prefix-summed offsets, read as
+ *                 one 64-bit window, dequantised and turned into two's
+ *                 complement (Packing.c:513-868, Dequant.c:9-57,
+ *                 Precinct.c:9-29).
+ *   k_dec_slice   one wave per slice coded with vertical prediction or
+ *                 separate signs: its precincts and packets in order.
  *   k_inv_rows    one block per row: every band sample of the row loaded at
  *                 once (one memory round trip), all inverse horizontal levels in
  *                 LDS (Idwt.c:11-53 semantics), into an int32 row or, for
  *                 components without vertical levels, straight to the output
  *                 plane with the linear output scaling (NltDec.c:242-265).
- *   k_inv_vert    inverse vertical 5/3 lifting, one lane per column, with the
- *                 output scaling fused when it produces the final plane.
+ *   k_inv_vert    inverse vertical 5/3 lifting of a second vertical level,
+ *                 one lane per column, into int32 rows.
+ *   k_inv_last    the last vertical level with the finest horizontal level
+ *                 of its low and high rows, one lane per output column pair
+ *                 down a strip of row pairs, the output scaling fused.
  *
  * Every codestream read is bounds-checked: a corrupt or truncated
  * codestream sets the frame's status word, and no read leaves the aligned
@@ -235,26 +241,13 @@ struct UnpackCtx {
     uint32_t stg[PK_STAGE + 1]; /* the packet's codestream words, byte-swapped, and a spare */
 };
 
-/* Codestream words read through the block's staged copy of its packet: words
- * lo .. lo + n - 1 come from LDS, any other word from HBM (n = 0 when the
- * packet did not fit). */
+/* A block's packet words: s[0 .. n - 1] is the LDS copy of codestream words
+ * lo .. lo + n - 1 (n = 0 when the packet did not fit, and V reads HBM). */
 struct PkSrc {
     CsView V;
     const uint32_t* s;
     uint32_t lo, n;
 };
-
-__device__ __forceinline__ uint32_t pk_word(const PkSrc& S, uint32_t wi) {
-    const uint32_t r = wi - S.lo;
-    return r < S.n ? S.s[r] : cs_word(S.V, wi);
-}
-
-__device__ __forceinline__ uint64_t pk_bits64(const PkSrc& S, uint64_t abp) {
-    const uint32_t wi = (uint32_t)(abp >> 5), sh = (uint32_t)(abp & 31);
-    const uint64_t a = ((uint64_t)pk_word(S, wi) << 32) | pk_word(S, wi + 1);
-    const uint32_t c = pk_word(S, wi + 2);
-    return sh ? (a << sh) | (uint64_t)(c >> (32 - sh)) : a;
-}
 
 /* One lane per precinct of the launch: parses the precinct header's band
  * D-modes, derives every band's GTLI from the precinct's quantisation and
