@@ -479,6 +479,9 @@ def serving(P, S, torch, dev, check, nstreams=32, seconds=3.0, budget_gib=16, ma
             "frame_errors": errs, "bit_exact_vs_alone": bool(exact),
             "hbm_budget_bytes": int(st["hbm_budget"]), "hbm_peak_bytes": int(st["hbm_peak"]),
             "max_batch": max_batch, "max_inflight": inflight,
+            "loop": "host-side: one worker thread waits at most 20 us for work, reaps finished launch slots and "
+                    "launches each batch's kernels on one of %d HIP streams; nothing runs persistently on the GPU"
+                    % inflight,
             "burst": {"frames": len(bc), "served_fps": round(len(bc) / ((b1 - b0) / 1e9), 1),
                       "p99_latency_ms": round(float(np.percentile(burst_lat, 99)), 4),
                       "largest_batch": int(st["largest_batch"]), "bit_exact_vs_alone": bool(burst_exact)},
