@@ -178,6 +178,12 @@ __device__ __forceinline__ int find_dec_frame_wave(const JxsDecFrame* fr, int nf
     return f - 1;
 }
 
+/* The codestream offset of precinct `row`: from the caller's precinct table
+ * when the frame has one, else as k_dec_scan found it. */
+__device__ __forceinline__ uint32_t prec_offset(const JxsDecFrame& F, uint32_t row) {
+    return F.prec_in ? F.prec_in[(size_t)row * F.prec_in_stride] : F.prec_off[row];
+}
+
 /* One block per codestream: copies a known precinct table, or walks the
  * slice headers and the precinct length chain (lane 0). */
 __global__ void k_dec_scan(const JxsDecFrame* __restrict__ frames, int nfr) {
@@ -273,7 +279,7 @@ __global__ void __launch_bounds__(64) k_dec_packets(const JxsDecFrame* __restric
         return;
     }
     const uint32_t type = row + 1 >= g.rows ? 1 : 0;
-    const uint64_t off = F.prec_off[row];
+    const uint64_t off = prec_offset(F, row);
     const CsView V = cs_view(F);
     /* the precinct header and its band modes, read as three 64-bit windows */
     const uint64_t hw = cs_bits64z(F, V, off * 8), m0 = cs_bits64z(F, V, off * 8 + 40), m1 = cs_bits64z(F, V, off * 8 + 104);
@@ -891,7 +897,7 @@ __global__ void __launch_bounds__(WAVE) k_dec_slice(const JxsDecFrame* __restric
         if (T[JXS_DPKT_BAD])
             break; /* already reported */
         const uint32_t type = row + 1 >= g.rows ? 1 : 0;
-        const uint64_t off = F.prec_off[row];
+        const uint64_t off = prec_offset(F, row);
         const uint64_t pend = off + g.prec_hdr_bytes + (uint64_t)cs_bits(F, off * 8, 24);
         uint64_t pos = off + g.prec_hdr_bytes;
         bool bad = false;

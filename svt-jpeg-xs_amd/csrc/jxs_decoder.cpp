@@ -204,6 +204,7 @@ struct svt_jpeg_xs_amd_decode_plan {
     int nlast = 0;
     uint32_t lblocks = 0;
     uint32_t total_prec = 0, max_npk = 0, total_slices = 0;
+    bool need_scan = false; /* a frame without the caller's precinct table */
     bool own_status = true; /* false in a part: the status words belong to the whole plan */
     /* A plan of several frames runs as parts on streams of its own, forked
      * from and joined back to the caller's stream, so that one part's
@@ -317,6 +318,7 @@ static svt_jpeg_xs_amd_decode_plan_t* dplan_build(const std::vector<DecSpec>& sp
         F.npk_dec = s.geom->npk_dec;
         F.prec_in = s.prec_in;
         F.prec_in_stride = s.prec_stride ? s.prec_stride : 1;
+        p->need_scan |= s.prec_in == nullptr;
         F.rows_valid = g.rows;
         base += g.rows;
         p->max_npk = s.geom->npk_dec > p->max_npk ? s.geom->npk_dec : p->max_npk;
@@ -437,7 +439,7 @@ static hipError_t dplan_launch_one(svt_jpeg_xs_amd_decode_plan_t* p, hipStream_t
     hipError_t e;
     if ((e = jxs_launch_zero(p->d_status, p->nfr, st)) != hipSuccess)
         return e;
-    if ((e = jxs_launch_dec_scan(p->d_frames, (int)p->nfr, st)) != hipSuccess)
+    if (p->need_scan && (e = jxs_launch_dec_scan(p->d_frames, (int)p->nfr, st)) != hipSuccess)
         return e;
     if ((e = jxs_launch_dec_packets(p->d_frames, (int)p->nfr, p->total_prec, st)) != hipSuccess)
         return e;
