@@ -53,6 +53,19 @@ __device__ __forceinline__ T gld(const T* p) {
     return *(const GAS T*)p;
 }
 
+/* base[idx] for a wave-uniform row base and a 32-bit per-lane index: the
+ * address is the scalar base plus a 32-bit byte offset (no 64-bit per-lane
+ * address arithmetic). */
+template <typename T>
+__device__ __forceinline__ T gldr(const T* base, uint32_t idx) {
+    return *(const GAS T*)((const GAS uint8_t*)base + idx * (uint32_t)sizeof(T));
+}
+
+template <typename T>
+__device__ __forceinline__ void gstr(void* base, uint32_t byte_off, T v) {
+    *(GAS T*)((GAS uint8_t*)base + byte_off) = v;
+}
+
 __device__ __forceinline__ uint32_t dtrunc(uint32_t gain, uint32_t prio, uint32_t q, uint32_t r) {
     const uint32_t pump = prio < r ? 1u : 0u;
     if (q < gain + pump)
@@ -1344,53 +1357,75 @@ __device__ __forceinline__ LvIn lv_load(const JxsInvLastJob& J, uint32_t i, uint
     LvIn v = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     const uint32_t w = J.w, n1 = (w + 1) / 2, nh = w / 2;
     const uint32_t jm = j > 0 ? j - 1 : 0, j1 = min(j + 1, n1 - 1), jh = min(j, nh - 1), jp = min(j + 1, nh - 1);
-    const int32_t fm = 1 << J.fq;
+    const uint32_t fq = J.fq; /* LSHIFT32(coefficient, Fq), Decoder.c:789 */
+    auto sc = [&](int16_t x) { return (int32_t)((uint32_t)(int32_t)x << fq); };
     if (low_row) {
         if (J.low32) {
             const int32_t* lr = J.low32 + (size_t)i * J.low_stride;
-            v.l0 = gld(lr + j);
-            v.l1 = gld(lr + j1);
+            v.l0 = gldr(lr, j);
+            v.l1 = gldr(lr, j1);
         } else {
             const int16_t* lr = J.low16 + (size_t)i * J.low_stride;
-            v.l0 = (int32_t)gld(lr + j) * fm;
-            v.l1 = (int32_t)gld(lr + j1) * fm;
+            v.l0 = sc(gldr(lr, j));
+            v.l1 = sc(gldr(lr, j1));
         }
         const int16_t* hr = J.hl + (size_t)i * J.hl_stride;
-        v.hm = (int32_t)gld(hr + jm) * fm;
-        v.h0 = (int32_t)gld(hr + jh) * fm;
-        v.hp = (int32_t)gld(hr + jp) * fm;
+        v.hm = sc(gldr(hr, jm));
+        v.h0 = sc(gldr(hr, jh));
+        v.hp = sc(gldr(hr, jp));
     }
     if (high_row) {
         const int16_t* lr = J.lh + (size_t)i * J.lh_stride;
-        v.m0 = (int32_t)gld(lr + j) * fm;
-        v.m1 = (int32_t)gld(lr + j1) * fm;
+        v.m0 = sc(gldr(lr, j));
+        v.m1 = sc(gldr(lr, j1));
         const int16_t* hr = J.hh + (size_t)i * J.hh_stride;
-        v.gm = (int32_t)gld(hr + jm) * fm;
-        v.g0 = (int32_t)gld(hr + jh) * fm;
-        v.gp = (int32_t)gld(hr + jp) * fm;
+        v.gm = sc(gldr(hr, jm));
+        v.g0 = sc(gldr(hr, jh));
+        v.gp = sc(gldr(hr, jp));
     }
     return v;
+}
+
+/* lv_h1 at an interior column pair (1 <= j, j + 1 < w / 2): no edge rule
+ * applies. */
+__device__ __forceinline__ void lv_h1i(int32_t l0, int32_t l1, int32_t hm, int32_t h0, int32_t hp, int32_t& ev,
+                                       int32_t& od) {
+    ev = l0 - ((hm + h0 + 2) >> 2);
+    od = h0 + ((ev + (l1 - ((h0 + hp + 2) >> 2))) >> 1);
 }
 
 /* Two output samples of row r at columns 2j, 2j + 1 (the second if inside). */
 __device__ __forceinline__ void lv_store2(const JxsInvLastJob& J, uint32_t r, uint32_t j, int32_t a, int32_t b) {
     const uint32_t x = 2 * j;
-    const size_t o = (size_t)r * J.out_stride + x;
-    const uint32_t oa = nlt_out(a, J.depth, J.bw), ob = nlt_out(b, J.depth, J.bw);
-    if (x + 1 < J.w) {
-        if (J.depth <= 8) {
-            if (((uintptr_t)((uint8_t*)J.out + o) & 1) == 0) {
-                *(GAS uint16_t*)((uint8_t*)J.out + o) = (uint16_t)(oa | ob << 8);
-                return;
-            }
-        } else if (((uintptr_t)((uint16_t*)J.out + o) & 3) == 0) {
-            *(GAS uint32_t*)((uint16_t*)J.out + o) = oa | ob << 16;
-            return;
-        }
-        store_out(J.out, o, a, J.depth, J.bw);
-        store_out(J.out, o + 1, b, J.depth, J.bw);
+    uint32_t oa, ob;
+    if (__builtin_expect(a > -(1 << 28) && a < (1 << 28) && b > -(1 << 28) && b < (1 << 28) && J.bw <= 24, 1)) {
+        /* nlt_out's 32-bit path, for both */
+        const int32_t dz = (int32_t)J.bw - (int32_t)J.depth;
+        const int32_t add = ((1 << J.bw) >> 1) + ((1 << dz) >> 1), m = (1 << J.depth) - 1;
+        oa = (uint32_t)min(max((a + add) >> dz, 0), m);
+        ob = (uint32_t)min(max((b + add) >> dz, 0), m);
     } else {
-        store_out(J.out, o, a, J.depth, J.bw);
+        oa = nlt_out(a, J.depth, J.bw);
+        ob = nlt_out(b, J.depth, J.bw);
+    }
+    if (J.depth <= 8) {
+        uint8_t* row = (uint8_t*)J.out + (size_t)r * J.out_stride;
+        if (x + 1 < J.w && ((uintptr_t)row & 1) == 0) {
+            gstr(row, x, (uint16_t)(oa | ob << 8));
+        } else {
+            gstr(row, x, (uint8_t)oa);
+            if (x + 1 < J.w)
+                gstr(row, x + 1, (uint8_t)ob);
+        }
+    } else {
+        uint16_t* row = (uint16_t*)J.out + (size_t)r * J.out_stride;
+        if (x + 1 < J.w && ((uintptr_t)row & 3) == 0) {
+            gstr(row, 2 * x, oa | ob << 16);
+        } else {
+            gstr(row, 2 * x, (uint16_t)oa);
+            if (x + 1 < J.w)
+                gstr(row, 2 * x + 2, (uint16_t)ob);
+        }
     }
 }
 
@@ -1415,6 +1450,7 @@ __global__ void __launch_bounds__(LV_NT) k_inv_last(const JxsInvLastJob* __restr
     const uint32_t i0 = ty * J.seg, i1 = min(i0 + J.seg, nl);
     if (i0 >= i1)
         return;
+    const bool interior = j >= 1 && j + 1 < w / 2; /* no horizontal edge rule at j */
     /* x(2 i0) and H(i0) */
     int32_t ea, eb, ha, hb; /* even row and high row at columns 2j, 2j + 1 */
     {
@@ -1453,9 +1489,15 @@ __global__ void __launch_bounds__(LV_NT) k_inv_last(const JxsInvLastJob* __restr
         int32_t oa = 0, ob = 0, na = 0, nb = 0;
         if (need_next) {
             int32_t la, lb, ga = 0, gb = 0;
-            lv_h1(j, w, cur.l0, cur.l1, cur.hm, cur.h0, cur.hp, la, lb);
-            if (has_hn)
-                lv_h1(j, w, cur.m0, cur.m1, cur.gm, cur.g0, cur.gp, ga, gb);
+            if (interior) {
+                lv_h1i(cur.l0, cur.l1, cur.hm, cur.h0, cur.hp, la, lb);
+                if (has_hn)
+                    lv_h1i(cur.m0, cur.m1, cur.gm, cur.g0, cur.gp, ga, gb);
+            } else {
+                lv_h1(j, w, cur.l0, cur.l1, cur.hm, cur.h0, cur.hp, la, lb);
+                if (has_hn)
+                    lv_h1(j, w, cur.m0, cur.m1, cur.gm, cur.g0, cur.gp, ga, gb);
+            }
             na = has_hn ? la - ((ha + ga + 2) >> 2) : la - ((ha + 1) >> 1);
             nb = has_hn ? lb - ((hb + gb + 2) >> 2) : lb - ((hb + 1) >> 1);
             oa = ha + ((ea + na) >> 1);
