@@ -1219,12 +1219,12 @@ __global__ void __launch_bounds__(256) k_inv_rows(const JxsInvRowJob* __restrict
 #pragma unroll
         for (int j = 0; j < NL; j++)
             if (tid + 256u * j < nL)
-                vl[j] = gld(lf32 + tid + 256u * j);
+                vl[j] = gldr(lf32, tid + 256u * j);
     } else {
 #pragma unroll
         for (int j = 0; j < NL; j++)
             if (tid + 256u * j < nL)
-                vl[j] = gld(lf16 + tid + 256u * j); /* scaled by 2^Fq when stored */
+                vl[j] = gldr(lf16, tid + 256u * j); /* scaled by 2^Fq when stored */
     }
     /* high band of level k: floor(n[k] / 2) <= MAXW >> (k + 1) samples */
     constexpr int NH0 = ((MAXW >> 1) + 255) / 256, NH1 = ((MAXW >> 2) + 255) / 256,
@@ -1240,10 +1240,10 @@ __global__ void __launch_bounds__(256) k_inv_rows(const JxsInvRowJob* __restrict
             const uint32_t i = tid + 256u * j;
             arr[j] = 0;
             if (i < nh)
-                arr[j] = (int32_t)gld(hp + i);
+                arr[j] = (int32_t)gldr(hp, i);
         }
     };
-    const int32_t fm = 1 << J.fq; /* LSHIFT32(coefficient, Fq), Decoder.c:789 */
+    const uint32_t fq = J.fq; /* LSHIFT32(coefficient, Fq), Decoder.c:789 */
     ldh(h0, std::integral_constant<int, NH0>{}, 0);
     ldh(h1, std::integral_constant<int, NH1>{}, 1);
     ldh(h2, std::integral_constant<int, NH2>{}, 2);
@@ -1252,7 +1252,7 @@ __global__ void __launch_bounds__(256) k_inv_rows(const JxsInvRowJob* __restrict
 #pragma unroll
     for (int j = 0; j < NL; j++)
         if (tid + 256u * j < nL)
-            buf[(tid + 256u * j) << L] = lf32 ? vl[j] : vl[j] * fm;
+            buf[(tid + 256u * j) << L] = lf32 ? vl[j] : (int32_t)((uint32_t)vl[j] << fq);
     auto sth = [&](const auto& arr, auto cnt, uint32_t k) {
         constexpr int N = decltype(cnt)::value;
         const uint32_t nh = k < L ? n[k] / 2 : 0u;
@@ -1260,7 +1260,7 @@ __global__ void __launch_bounds__(256) k_inv_rows(const JxsInvRowJob* __restrict
         for (int j = 0; j < N; j++) {
             const uint32_t i = tid + 256u * j;
             if (i < nh)
-                buf[(2 * i + 1) << k] = arr[j] * fm;
+                buf[(2 * i + 1) << k] = (int32_t)((uint32_t)arr[j] << fq);
         }
     };
     sth(h0, std::integral_constant<int, NH0>{}, 0);
@@ -1269,22 +1269,18 @@ __global__ void __launch_bounds__(256) k_inv_rows(const JxsInvRowJob* __restrict
     sth(h3, std::integral_constant<int, NH3>{}, 3);
     sth(h4, std::integral_constant<int, NH4>{}, 4);
     __syncthreads();
+    /* The edge rules are the interior steps with the missing neighbour
+     * mirrored: (2x + 2) >> 2 == (x + 1) >> 1 and (2x) >> 1 == x. */
     for (int k = (int)L - 1; k >= 0; k--) {
-        const uint32_t s = 1u << k, nk = n[k], nh = nk / 2, nl = nk - nh;
+        const uint32_t nk = n[k], nh = nk / 2, nl = nk - nh;
         for (uint32_t i = threadIdx.x; i < nl; i += blockDim.x) {
-            if (i == 0)
-                buf[0] -= (buf[s] + 1) >> 1;
-            else if (i < nh)
-                buf[2 * i * s] -= (buf[(2 * i - 1) * s] + buf[(2 * i + 1) * s] + 2) >> 2;
-            else
-                buf[2 * i * s] -= (buf[(2 * i - 1) * s] + 1) >> 1;
+            const uint32_t il = i == 0 ? 1u : 2 * i - 1, ir = i < nh ? 2 * i + 1 : 2 * i - 1;
+            buf[(2 * i) << k] -= (buf[il << k] + buf[ir << k] + 2) >> 2;
         }
         __syncthreads();
         for (uint32_t i = threadIdx.x; i < nh; i += blockDim.x) {
-            if (2 * i + 2 < nk)
-                buf[(2 * i + 1) * s] += (buf[2 * i * s] + buf[(2 * i + 2) * s]) >> 1;
-            else
-                buf[(2 * i + 1) * s] += buf[2 * i * s];
+            const uint32_t ir = 2 * i + 2 < nk ? 2 * i + 2 : 2 * i;
+            buf[(2 * i + 1) << k] += (buf[(2 * i) << k] + buf[ir << k]) >> 1;
         }
         __syncthreads();
     }
