@@ -373,3 +373,50 @@ def test_decode_plan_of_mixed_tool_codestreams():
     plan.destroy()
     for dec in decs:
         dec.close()
+
+
+@pytest.mark.gpu
+def test_decode_plan_parts_report_each_frames_status():
+    """A decode plan of six frames runs as four parts on streams of its own
+    (2 + 2 + 1 + 1 frames) over one array of status words.  Frame 3's
+    codestream is cut short: its precinct walk fails and only its status word
+    is set; every other frame equals the oracle's decode.  A second run
+    starts from cleared status words and reports the same."""
+    import torch
+    print("decode plan parts: statuses", flush=True)
+    P, O = load_package(), load_oracle()
+    dev = torch.device("cuda:0")
+    cfg = (256, 64, 10, 3, 3, 2, 5, 0, 1)
+    css = [_codestream(cfg, frame=i)[1] for i in range(6)]
+    dec = P.Decoder(css[0], verbose=0)
+    keep, specs, outs = [], [], []
+    for i, cs in enumerate(css):
+        d_cs = torch.from_numpy(np.frombuffer(cs, dtype=np.uint8).copy()).to(dev)
+        planes = [torch.zeros((dec.config.components[c].height, dec.config.components[c].width), dtype=torch.int16,
+                              device=dev) for c in range(dec.config.components_num)]
+        size = len(cs) // 2 if i == 3 else len(cs)
+        specs.append((dec, d_cs.data_ptr(), size, [t.data_ptr() for t in planes], [t.shape[1] for t in planes],
+                      None, 1))
+        keep.append(d_cs)
+        outs.append(planes)
+    plan = P.DecodePlan(specs)
+    try:
+        for _ in range(2):
+            for planes in outs:
+                for t in planes:
+                    t.zero_()
+            plan.run(torch.cuda.current_stream(dev).cuda_stream)
+            torch.cuda.synchronize()
+            rc, errs = plan.status()
+            assert rc != 0
+            assert [e != 0 for e in errs] == [False, False, False, True, False, False], errs
+            for i, cs in enumerate(css):
+                if i == 3:
+                    continue
+                e, ref = O.decode(cs)
+                assert e == 0
+                for t, r in zip(outs[i], ref):
+                    assert np.array_equal(t.cpu().numpy().view(np.uint16), r), "frame %d differs" % i
+    finally:
+        plan.destroy()
+        dec.close()
