@@ -1349,35 +1349,64 @@ struct LvIn {
     int32_t m0, m1, gm, g0, gp; /* high row */
 };
 
-__device__ __forceinline__ LvIn lv_load(const JxsInvLastJob& J, uint32_t i, uint32_t j, bool low_row, bool high_row) {
+/* The band rows of one row pair step: low row i's low samples (int32 rows
+ * or a band) and HL band, high row i's LH and HH bands; advanced a row at a
+ * time with adds (the row bases stay in scalar registers). */
+struct LvRows {
+    const int32_t* l32;
+    const int16_t* l16;
+    const int16_t* hl;
+    const int16_t* lh;
+    const int16_t* hh;
+};
+
+__device__ __forceinline__ LvRows lv_rows(const JxsInvLastJob& J, uint32_t i) {
+    LvRows r;
+    r.l32 = J.low32 ? J.low32 + (size_t)i * J.low_stride : nullptr;
+    r.l16 = J.low32 ? nullptr : J.low16 + (size_t)i * J.low_stride;
+    r.hl = J.hl + (size_t)i * J.hl_stride;
+    r.lh = J.lh + (size_t)i * J.lh_stride;
+    r.hh = J.hh + (size_t)i * J.hh_stride;
+    return r;
+}
+
+__device__ __forceinline__ void lv_next(LvRows& r, const JxsInvLastJob& J, bool l32) {
+    if (l32)
+        r.l32 += J.low_stride;
+    else
+        r.l16 += J.low_stride;
+    r.hl += J.hl_stride;
+    r.lh += J.lh_stride;
+    r.hh += J.hh_stride;
+}
+
+/* A thread's columns: low samples j and j1 = j + 1, high samples jm, jh, jp
+ * around j, each clamped to its band as lv_h1 expects. */
+struct LvCols {
+    uint32_t j, j1, jm, jh, jp, fq;
+};
+
+__device__ __forceinline__ LvIn lv_load(const LvRows& r, const LvCols& c, bool l32, bool low_row, bool high_row) {
     LvIn v = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-    const uint32_t w = J.w, n1 = (w + 1) / 2, nh = w / 2;
-    const uint32_t jm = j > 0 ? j - 1 : 0, j1 = min(j + 1, n1 - 1), jh = min(j, nh - 1), jp = min(j + 1, nh - 1);
-    const uint32_t fq = J.fq; /* LSHIFT32(coefficient, Fq), Decoder.c:789 */
-    auto sc = [&](int16_t x) { return (int32_t)((uint32_t)(int32_t)x << fq); };
+    auto sc = [&](int16_t x) { return (int32_t)((uint32_t)(int32_t)x << c.fq); }; /* LSHIFT32(coefficient, Fq), Decoder.c:789 */
     if (low_row) {
-        if (J.low32) {
-            const int32_t* lr = J.low32 + (size_t)i * J.low_stride;
-            v.l0 = gldr(lr, j);
-            v.l1 = gldr(lr, j1);
+        if (l32) {
+            v.l0 = gldr(r.l32, c.j);
+            v.l1 = gldr(r.l32, c.j1);
         } else {
-            const int16_t* lr = J.low16 + (size_t)i * J.low_stride;
-            v.l0 = sc(gldr(lr, j));
-            v.l1 = sc(gldr(lr, j1));
+            v.l0 = sc(gldr(r.l16, c.j));
+            v.l1 = sc(gldr(r.l16, c.j1));
         }
-        const int16_t* hr = J.hl + (size_t)i * J.hl_stride;
-        v.hm = sc(gldr(hr, jm));
-        v.h0 = sc(gldr(hr, jh));
-        v.hp = sc(gldr(hr, jp));
+        v.hm = sc(gldr(r.hl, c.jm));
+        v.h0 = sc(gldr(r.hl, c.jh));
+        v.hp = sc(gldr(r.hl, c.jp));
     }
     if (high_row) {
-        const int16_t* lr = J.lh + (size_t)i * J.lh_stride;
-        v.m0 = sc(gldr(lr, j));
-        v.m1 = sc(gldr(lr, j1));
-        const int16_t* hr = J.hh + (size_t)i * J.hh_stride;
-        v.gm = sc(gldr(hr, jm));
-        v.g0 = sc(gldr(hr, jh));
-        v.gp = sc(gldr(hr, jp));
+        v.m0 = sc(gldr(r.lh, c.j));
+        v.m1 = sc(gldr(r.lh, c.j1));
+        v.gm = sc(gldr(r.hh, c.jm));
+        v.g0 = sc(gldr(r.hh, c.jh));
+        v.gp = sc(gldr(r.hh, c.jp));
     }
     return v;
 }
@@ -1390,37 +1419,40 @@ __device__ __forceinline__ void lv_h1i(int32_t l0, int32_t l1, int32_t hm, int32
     od = h0 + ((ev + (l1 - ((h0 + hp + 2) >> 2))) >> 1);
 }
 
-/* Two output samples of row r at columns 2j, 2j + 1 (the second if inside). */
-__device__ __forceinline__ void lv_store2(const JxsInvLastJob& J, uint32_t r, uint32_t j, int32_t a, int32_t b) {
-    const uint32_t x = 2 * j;
+/* The output side of a thread: the scaling constants (nlt_out's 32-bit
+ * path), the sample size, whether both of its columns exist and whether a
+ * pair can go out as one store. */
+struct LvOut {
+    int32_t add, m;
+    uint32_t dz, depth, bw, x, row_bytes;
+    bool b8, two, pair, fast;
+};
+
+/* Two output samples at columns x, x + 1 of the row at `row` (bytes). */
+__device__ __forceinline__ void lv_store2(uint8_t* row, const LvOut& o, int32_t a, int32_t b) {
     uint32_t oa, ob;
-    if (__builtin_expect(a > -(1 << 28) && a < (1 << 28) && b > -(1 << 28) && b < (1 << 28) && J.bw <= 24, 1)) {
-        /* nlt_out's 32-bit path, for both */
-        const int32_t dz = (int32_t)J.bw - (int32_t)J.depth;
-        const int32_t add = ((1 << J.bw) >> 1) + ((1 << dz) >> 1), m = (1 << J.depth) - 1;
-        oa = (uint32_t)min(max((a + add) >> dz, 0), m);
-        ob = (uint32_t)min(max((b + add) >> dz, 0), m);
+    if (__builtin_expect(o.fast && a > -(1 << 28) && a < (1 << 28) && b > -(1 << 28) && b < (1 << 28), 1)) {
+        oa = (uint32_t)min(max((a + o.add) >> o.dz, 0), o.m);
+        ob = (uint32_t)min(max((b + o.add) >> o.dz, 0), o.m);
     } else {
-        oa = nlt_out(a, J.depth, J.bw);
-        ob = nlt_out(b, J.depth, J.bw);
+        oa = nlt_out(a, o.depth, o.bw);
+        ob = nlt_out(b, o.depth, o.bw);
     }
-    if (J.depth <= 8) {
-        uint8_t* row = (uint8_t*)J.out + (size_t)r * J.out_stride;
-        if (x + 1 < J.w && ((uintptr_t)row & 1) == 0) {
-            gstr(row, x, (uint16_t)(oa | ob << 8));
+    if (o.b8) {
+        if (o.pair) {
+            gstr(row, o.x, (uint16_t)(oa | ob << 8));
         } else {
-            gstr(row, x, (uint8_t)oa);
-            if (x + 1 < J.w)
-                gstr(row, x + 1, (uint8_t)ob);
+            gstr(row, o.x, (uint8_t)oa);
+            if (o.two)
+                gstr(row, o.x + 1, (uint8_t)ob);
         }
     } else {
-        uint16_t* row = (uint16_t*)J.out + (size_t)r * J.out_stride;
-        if (x + 1 < J.w && ((uintptr_t)row & 3) == 0) {
-            gstr(row, 2 * x, oa | ob << 16);
+        if (o.pair) {
+            gstr(row, 2 * o.x, oa | ob << 16);
         } else {
-            gstr(row, 2 * x, (uint16_t)oa);
-            if (x + 1 < J.w)
-                gstr(row, 2 * x + 2, (uint16_t)ob);
+            gstr(row, 2 * o.x, (uint16_t)oa);
+            if (o.two)
+                gstr(row, 2 * o.x + 2, (uint16_t)ob);
         }
     }
 }
@@ -1432,34 +1464,62 @@ __device__ __forceinline__ void lv_store2(const JxsInvLastJob& J, uint32_t r, ui
  * strip in registers: the step for output rows 2i, 2i + 1 carries x(2i) and
  * H(i) from the step before and makes L(i + 1), H(i + 1); the next step's
  * loads are issued before this step's arithmetic.  No barrier: neighbours'
- * samples are read again (from cache) instead of exchanged. */
+ * samples are read again (from cache) instead of exchanged.  Row bases
+ * advance by adds and every per-thread constant is set once, so a step is
+ * mostly vector work. */
 __global__ void __launch_bounds__(LV_NT) k_inv_last(const JxsInvLastJob* __restrict__ jobs,
                                                    const uint32_t* __restrict__ bases, int njobs) {
     const JxsInvLastJob& J = jobs[find_base(bases, njobs, blockIdx.x)];
     const uint32_t t = blockIdx.x - J.base;
     const uint32_t tx = t % J.tiles_x, ty = t / J.tiles_x;
     const uint32_t j = tx * LV_NT + threadIdx.x;
-    const uint32_t w = J.w, n1 = (w + 1) / 2;
+    const uint32_t w = J.w, n1 = (w + 1) / 2, nhw = w / 2;
     if (j >= n1)
         return;
     const uint32_t n = J.h, nh = n / 2, nl = n - nh;
     const uint32_t i0 = ty * J.seg, i1 = min(i0 + J.seg, nl);
     if (i0 >= i1)
         return;
-    const bool interior = j >= 1 && j + 1 < w / 2; /* no horizontal edge rule at j */
+    const bool interior = j >= 1 && j + 1 < nhw; /* no horizontal edge rule at j */
+    const bool l32 = J.low32 != nullptr;
+    LvCols c;
+    c.j = j;
+    c.j1 = min(j + 1, n1 - 1);
+    c.jm = j > 0 ? j - 1 : 0;
+    c.jh = min(j, nhw - 1);
+    c.jp = min(j + 1, nhw - 1);
+    c.fq = J.fq;
+    LvOut o;
+    o.depth = J.depth;
+    o.bw = J.bw;
+    o.dz = J.bw - J.depth;
+    o.add = ((1 << J.bw) >> 1) + ((1 << o.dz) >> 1);
+    o.m = (1 << J.depth) - 1;
+    o.fast = J.bw <= 24;
+    o.b8 = J.depth <= 8;
+    o.x = 2 * j;
+    o.two = 2 * j + 1 < w;
+    o.row_bytes = J.out_stride * (o.b8 ? 1u : 2u);
+    o.pair = o.two && (((uintptr_t)J.out | o.row_bytes) & (o.b8 ? 1u : 3u)) == 0;
+    auto hstep = [&](int32_t l0, int32_t l1, int32_t hm, int32_t h0, int32_t hp, int32_t& ev, int32_t& od) {
+        if (interior)
+            lv_h1i(l0, l1, hm, h0, hp, ev, od);
+        else
+            lv_h1(j, w, l0, l1, hm, h0, hp, ev, od);
+    };
     /* x(2 i0) and H(i0) */
     int32_t ea, eb, ha, hb; /* even row and high row at columns 2j, 2j + 1 */
     {
         const bool has_h0 = i0 < nh, has_hm = i0 > 0;
-        const LvIn a = lv_load(J, i0, j, true, has_h0);
-        const LvIn m = has_hm ? lv_load(J, i0 - 1, j, false, true) : LvIn{};
+        const LvIn a = lv_load(lv_rows(J, i0), c, l32, true, has_h0);
+        const LvIn m = has_hm ? lv_load(lv_rows(J, i0 - 1), c, l32, false, true) : LvIn{};
         int32_t la, lb, pa = 0, pb = 0;
-        lv_h1(j, w, a.l0, a.l1, a.hm, a.h0, a.hp, la, lb);
+        hstep(a.l0, a.l1, a.hm, a.h0, a.hp, la, lb);
         ha = hb = 0;
         if (has_h0)
-            lv_h1(j, w, a.m0, a.m1, a.gm, a.g0, a.gp, ha, hb);
+            hstep(a.m0, a.m1, a.gm, a.g0, a.gp, ha, hb);
         if (has_hm)
-            lv_h1(j, w, m.m0, m.m1, m.gm, m.g0, m.gp, pa, pb);
+            hstep(m.m0, m.m1, m.gm, m.g0, m.gp, pa, pb);
         if (i0 == 0) {
             ea = la - ((ha + 1) >> 1);
             eb = lb - ((hb + 1) >> 1);
@@ -1471,29 +1531,27 @@ __global__ void __launch_bounds__(LV_NT) k_inv_last(const JxsInvLastJob* __restr
             eb = lb - ((pb + 1) >> 1);
         }
     }
+    LvRows rn = lv_rows(J, i0 + 1); /* the rows the next loads read */
     LvIn nx = {};
     if (i0 < nh && 2 * i0 + 2 < n)
-        nx = lv_load(J, i0 + 1, j, true, i0 + 1 < nh);
+        nx = lv_load(rn, c, l32, true, i0 + 1 < nh);
+    lv_next(rn, J, l32);
+    uint8_t* orow = (uint8_t*)J.out + (size_t)(2 * i0) * o.row_bytes; /* output row 2i */
     for (uint32_t i = i0; i < i1; i++) {
         const bool has_odd = i < nh;
         const bool need_next = has_odd && 2 * i + 2 < n;
         const bool has_hn = i + 1 < nh;
         const LvIn cur = nx;
-        /* the loads of the step after this one, issued before its arithmetic */
+        /* the loads of the step after this one (rows i + 2), issued before its arithmetic */
         if (i + 1 < i1 && i + 1 < nh && 2 * i + 4 < n)
-            nx = lv_load(J, i + 2, j, true, i + 2 < nh);
+            nx = lv_load(rn, c, l32, true, i + 2 < nh);
+        lv_next(rn, J, l32);
         int32_t oa = 0, ob = 0, na = 0, nb = 0;
         if (need_next) {
             int32_t la, lb, ga = 0, gb = 0;
-            if (interior) {
-                lv_h1i(cur.l0, cur.l1, cur.hm, cur.h0, cur.hp, la, lb);
-                if (has_hn)
-                    lv_h1i(cur.m0, cur.m1, cur.gm, cur.g0, cur.gp, ga, gb);
-            } else {
-                lv_h1(j, w, cur.l0, cur.l1, cur.hm, cur.h0, cur.hp, la, lb);
-                if (has_hn)
-                    lv_h1(j, w, cur.m0, cur.m1, cur.gm, cur.g0, cur.gp, ga, gb);
-            }
+            hstep(cur.l0, cur.l1, cur.hm, cur.h0, cur.hp, la, lb);
+            if (has_hn)
+                hstep(cur.m0, cur.m1, cur.gm, cur.g0, cur.gp, ga, gb);
             na = has_hn ? la - ((ha + ga + 2) >> 2) : la - ((ha + 1) >> 1);
             nb = has_hn ? lb - ((hb + gb + 2) >> 2) : lb - ((hb + 1) >> 1);
             oa = ha + ((ea + na) >> 1);
@@ -1504,9 +1562,10 @@ __global__ void __launch_bounds__(LV_NT) k_inv_last(const JxsInvLastJob* __restr
             oa = ha + ea;
             ob = hb + eb;
         }
-        lv_store2(J, 2 * i, j, ea, eb);
+        lv_store2(orow, o, ea, eb);
         if (has_odd)
-            lv_store2(J, 2 * i + 1, j, oa, ob);
+            lv_store2(orow + o.row_bytes, o, oa, ob);
+        orow += 2 * (size_t)o.row_bytes;
         ea = na;
         eb = nb;
     }
