@@ -1657,6 +1657,8 @@ hipError_t jxs_launch_inv_rows(const JxsInvRowJob* jobs, const uint32_t* bases, 
         hipLaunchKernelGGL(k_inv_rows<2048>, dim3(blocks), dim3(256), 0, st, jobs, bases, njobs);
     else if (maxw <= 4096)
         hipLaunchKernelGGL(k_inv_rows<4096>, dim3(blocks), dim3(256), 0, st, jobs, bases, njobs);
+    else if (maxw <= 7680)
+        hipLaunchKernelGGL(k_inv_rows<7680>, dim3(blocks), dim3(256), 0, st, jobs, bases, njobs);
     else if (maxw <= JXS_MAX_ROW)
         hipLaunchKernelGGL(k_inv_rows<JXS_MAX_ROW>, dim3(blocks), dim3(256), 0, st, jobs, bases, njobs);
     else

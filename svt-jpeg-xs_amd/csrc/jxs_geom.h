@@ -23,7 +23,8 @@
 #define JXS_NR 32   /* refinement values 0..31 (<= bands - 1) */
 #define JXS_HDR_MAX 256
 #define JXS_NU 64  /* packing units (band lines) per precinct: one per lane of a wave */
-#define JXS_MAX_ROW 7680 /* widest component row the row kernels stage in LDS */
+#define JXS_MAX_ROW 8192 /* widest component row: the row kernels stage one in LDS, the transform covers
+                            one with 256 threads of 32 columns */
 #define JXS_SHADOW_UNITS 32768 /* packing units of a launch above which boundary words go through shadow words */
 
 /* Band-line histogram record (u16 units), written by the transform for

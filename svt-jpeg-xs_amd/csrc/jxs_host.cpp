@@ -588,7 +588,7 @@ int32_t build_encoder_geometry(const EncParams& p, HostGeom& out, int verbose) {
         return bad("band layout");
     for (uint32_t c = 0; c < nc; c++)
         if (g.c[c].w > JXS_MAX_ROW)
-            return bad("component wider than the row kernel supports (7680 samples)");
+            return bad("component wider than the row kernels support (8192 samples)");
     g.depth = p.depth;
     g.format = p.format;
     g.packed = p.format == F_PACKED444;

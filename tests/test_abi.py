@@ -125,6 +125,7 @@ BAD_CONFIGS = [
     dict(colour_format=2, ndecomp_v=0),
     dict(source_width=16, source_height=16, ndecomp_h=5, colour_format=3),
     dict(rate_control_mode=4),
+    dict(source_width=8200, colour_format=4),  # a component wider than 8,192 samples
     dict(coding_vertical_prediction_mode=3),
     dict(coding_signs_handling=3),
     dict(coding_signs_handling=2, rate_control_mode=2),

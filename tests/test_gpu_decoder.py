@@ -26,6 +26,11 @@ CASES = [
     # packet of about 19 KB, longer than the unpack's LDS stage (12 KiB), so
     # it is read from HBM
     (7680, 6, 8, 4, 20, 0, 1, 0, 1),
+    # the widest rows taken (8,192 samples): full-width rows through the
+    # row kernel sized for them, with and without vertical levels
+    (8192, 16, 10, 3, 3, 1, 5, 0, 1),
+    (8192, 8, 8, 4, 4, 0, 2, 0, 1),
+    (8192, 18, 12, 21, 2, 2, 5, 0, 1),
 ]
 
 

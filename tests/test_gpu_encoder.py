@@ -54,6 +54,25 @@ def test_small_configs_bit_exact(cfg):
         pytest.fail("codestream differs from byte %d of %d" % (first, len(ref)))
 
 
+# the widest rows taken (8,192 samples, DCI 8K): the transform's 256 threads
+# of 32 columns cover one row; 4:2:2 with a vertical level, 4:4:4 without,
+# packed RGB with two
+WIDEST = [
+    (8192, 16, 10, 3, 3, 1, 5, 0, 1),
+    (8192, 8, 8, 4, 4, 0, 2, 0, 1),
+    (8192, 18, 12, 21, 2, 2, 5, 0, 1),
+]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("cfg", WIDEST, ids=lambda c: "x".join(str(v) for v in c))
+def test_widest_rows_bit_exact(cfg):
+    print("case", cfg, flush=True)
+    e_ref, ref, rc, got = _encode_both(cfg)
+    assert e_ref == 0 and rc == 0
+    assert got == ref
+
+
 @pytest.mark.gpu
 def test_1080p_422_8bit_bpp5_bit_exact():
     print("case 1920x1080 422 8-bit bpp5", flush=True)
