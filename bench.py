@@ -26,7 +26,7 @@ W, H, DEPTH, FMT, BPP, DV, DH = 3840, 2160, 10, 3, 3, 2, 5
 NFRAMES = 64
 HBM_PEAK_GBS = 8000.0
 STAGES = ["dwt1", "dwt2", "rate", "pack"]
-KERNELS = {"dwt1": "k_dwt", "dwt2": "k_dwt", "rate": "k_rate", "pack": "k_pack"}
+KERNELS = {"dwt1": "k_dwt", "dwt2": "k_dwt", "rate": "k_rate_tab + k_rate_plan", "pack": "k_pack"}
 
 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "encode_3840x2160_422_10b_bpp3_traffic.json")
