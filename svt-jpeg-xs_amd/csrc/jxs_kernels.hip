@@ -1770,16 +1770,12 @@ struct WaveOut {
     uint32_t* ring;
     uint32_t mask;
     uint32_t* ob;
-    int64_t sd;  /* shadow words at ob + sd bytes (see flush) */
-    uint32_t sh; /* boundary words into the shadow (else into the codestream) */
     uint32_t lo, hi, pos, flushed, lane;
-    __device__ __forceinline__ void start(uint32_t* r, uint32_t m, uint32_t* o, int64_t sdelta, uint32_t shadow,
-                                          uint32_t first, uint32_t len, uint32_t ln) {
+    __device__ __forceinline__ void start(uint32_t* r, uint32_t m, uint32_t* o, uint32_t first, uint32_t len,
+                                          uint32_t ln) {
         ring = r;
         mask = m;
         ob = o;
-        sd = sdelta;
-        sh = shadow;
         lo = first;
         hi = first + len;
         pos = first;
@@ -1806,7 +1802,9 @@ struct WaveOut {
         if (w2)
             atomicOr(&ring[(wd + 2) & mask], w2);
     }
-    /* Writes ring words [from, to) out and clears them.  A word wholly inside
+    /* Writes ring words [from, to) out and clears them (sh: boundary words
+     * into the shadow words, sd bytes after their codestream words, else into
+     * the codestream).  A word wholly inside
      * the segment is stored; the segment's first and last words, which it
      * may share with its neighbours, are ORed in with device-scope atomics
      * (the neighbours may run on other XCDs).  A device-scope atomic on a
@@ -1816,7 +1814,7 @@ struct WaveOut {
      * later kernel ORs into the codestream, and keep the stores in L2; a
      * single frame's launch, where that kernel's latency would cost more,
      * writes the stores through to memory and ORs into the codestream. */
-    __device__ __forceinline__ void flush(uint32_t from, uint32_t to) {
+    __device__ __forceinline__ void flush(uint32_t from, uint32_t to, int64_t sd, uint32_t sh) {
         for (uint32_t w = from + lane; w < to; w += WAVE) {
             uint32_t* slot = &ring[w & mask];
             const uint32_t v = *slot;
@@ -1833,17 +1831,17 @@ struct WaveOut {
             }
         }
     }
-    __device__ __forceinline__ void maybe_flush() {
+    __device__ __forceinline__ void maybe_flush(int64_t sd, uint32_t sh) {
         if ((pos >> 5) - flushed >= RING_FLUSH) {
             wave_sync();
-            flush(flushed, pos >> 5);
+            flush(flushed, pos >> 5, sd, sh);
             flushed = pos >> 5;
             wave_sync();
         }
     }
-    __device__ __forceinline__ void finish() {
+    __device__ __forceinline__ void finish(int64_t sd, uint32_t sh) {
         wave_sync();
-        flush(flushed, (hi + 31) / 32);
+        flush(flushed, (hi + 31) / 32, sd, sh);
     }
 };
 
@@ -2042,7 +2040,30 @@ struct PackIn {
     uint32_t sf[IPL];
     GdIn<IPL> gd;
 };
-__device__ __forceinline__ PackIn pack_load(const JxsUnit& U, uint32_t lane) {
+/* The fields of a unit those loads need, read on their own (scalar loads)
+ * so that the next unit's record is not held in registers while this one is
+ * coded. */
+struct PackSrc {
+    const uint8_t* gl;
+    const uint8_t* sl;
+    const uint16_t* cl;
+    uint32_t len[3], gw, sw, ms;
+};
+__device__ __forceinline__ PackSrc pack_src(const JxsUnit* u) {
+    const CAS JxsUnit* c = (const CAS JxsUnit*)u;
+    PackSrc s;
+    s.gl = c->gl;
+    s.sl = c->sl;
+    s.cl = c->cl;
+    s.len[0] = c->len[0];
+    s.len[1] = c->len[1];
+    s.len[2] = c->len[2];
+    s.gw = c->gw;
+    s.sw = c->sw;
+    s.ms = c->ms;
+    return s;
+}
+__device__ __forceinline__ PackIn pack_load(const PackSrc& U, uint32_t lane) {
     PackIn in;
 #pragma unroll
     for (uint32_t k = 0; k < IPL; k++)
@@ -2067,7 +2088,7 @@ __device__ __forceinline__ void vm_wait_all() { __builtin_amdgcn_s_waitcnt(0x0F7
 
 /* The packet header (PackHeaders.c:98-127, sign size 0) of the first band
  * line of a packet, ORed into the zeroed output. */
-__device__ __forceinline__ void put_header(const JxsUnit& U, uint32_t lane, uint32_t shadow) {
+__device__ __forceinline__ void put_header(const CAS JxsUnit& U, uint32_t lane, uint32_t shadow) {
     if (U.hlen) {
         uint32_t w0, w1, w2;
         split_bits(U.hdr, U.hlen, U.hlo, w0, w1, w2);
@@ -2079,13 +2100,13 @@ __device__ __forceinline__ void put_header(const JxsUnit& U, uint32_t lane, uint
 
 /* The significance segment of a band line into ring A (written out later).
  * Returns whether its length differs from the plan's. */
-__device__ __forceinline__ bool code_sig(const JxsUnit& U, const PackIn& in, uint32_t lane, WaveOut& A,
-                                         uint32_t* ringa, uint32_t shadow) {
+__device__ __forceinline__ bool code_sig(const CAS JxsUnit& U, const PackIn& in, uint32_t lane, WaveOut& A,
+                                         uint32_t* ringa) {
     if (!U.len[0]) {
         A.clear();
         return false;
     }
-    A.start(ringa, RINGA - 1, U.ob[0], U.sdelta, shadow, U.lo[0], U.len[0], lane);
+    A.start(ringa, RINGA - 1, U.ob[0], U.lo[0], U.len[0], lane);
     wave_sync();
     sig_step(A, in.sf, U.sw, U.gt);
     return A.pos != A.hi;
@@ -2099,7 +2120,7 @@ __device__ __forceinline__ bool code_sig(const JxsUnit& U, const PackIn& in, uin
  * written out later.  Returns whether a segment's length differs from the
  * plan's. */
 template <bool UNIFORM>
-__device__ __forceinline__ bool code_gd(const JxsUnit& U, const GdIn<IPL>& first, uint32_t lane, WaveOut& B,
+__device__ __forceinline__ bool code_gd(const CAS JxsUnit& U, const GdIn<IPL>& first, uint32_t lane, WaveOut& B,
                                         WaveOut& C, uint32_t* ringb, uint32_t* ringc, uint32_t shadow) {
     if (!(U.len[1] | U.len[2])) {
         B.clear();
@@ -2110,16 +2131,17 @@ __device__ __forceinline__ bool code_gd(const JxsUnit& U, const GdIn<IPL>& first
     const uint8_t* sl = U.sl;
     const uint16_t* cl = U.cl;
     const uint32_t gw = U.gw, gt = U.gt, ms = U.ms, raw = U.raw;
-    B.start(ringb, RINGB - 1, U.ob[1], U.sdelta, shadow, U.lo[1], U.len[1], lane);
-    C.start(ringc, RINGC - 1, U.ob[2], U.sdelta, shadow, U.lo[2], U.len[2], lane);
+    B.start(ringb, RINGB - 1, U.ob[1], U.lo[1], U.len[1], lane);
+    C.start(ringc, RINGC - 1, U.ob[2], U.lo[2], U.len[2], lane);
+    const int64_t sd = U.sdelta;
     wave_sync();
     GdIn<IPL> cur = first;
     for (uint32_t base = 0; base < gw; base += IPL * WAVE) {
         const bool more = base + IPL * WAVE < gw;
         if (base) {
             vm_wait_all();
-            B.maybe_flush();
-            C.maybe_flush();
+            B.maybe_flush(sd, shadow);
+            C.maybe_flush(sd, shadow);
         }
         GdIn<IPL> nxt;
         if (more)
@@ -2155,42 +2177,43 @@ k_pack(const JxsUnit* __restrict__ units, uint32_t total_units, uint32_t shadow)
     if (gu >= total_units)
         return;
     WaveOut A, B, C; /* the previous unit's segments, not yet written out */
+    int64_t psd = 0; /* the previous unit's shadow offset */
     A.clear();
     B.clear();
     C.clear();
     A.lane = B.lane = C.lane = lane;
-    JxsUnit U = cld_words(units + gu);
-    PackIn in = pack_load(U, lane);
+    PackIn in = pack_load(pack_src(units + gu), lane);
     for (;;) {
+        const CAS JxsUnit& U = *(const CAS JxsUnit*)(units + gu);
         const uint32_t gn = gu + stride;
         const bool more = gn < total_units;
-        JxsUnit Un;
+        PackSrc Un;
         if (more)
-            Un = cld_words(units + gn);
+            Un = pack_src(units + gn);
         vm_wait_all();
-        A.finish();
-        B.finish();
-        C.finish();
+        A.finish(psd, shadow);
+        B.finish(psd, shadow);
+        C.finish(psd, shadow);
         wave_sync();
         put_header(U, lane, shadow);
         PackIn inn;
         if (more)
             inn = pack_load(Un, lane);
-        bool bad = code_sig(U, in, lane, A, ringa[wid], shadow);
+        bool bad = code_sig(U, in, lane, A, ringa[wid]);
         bad |= U.qpih ? code_gd<true>(U, in.gd, lane, B, C, ringb[wid], ringc[wid], shadow)
                       : code_gd<false>(U, in.gd, lane, B, C, ringb[wid], ringc[wid], shadow);
         /* the plan's lengths are the segments' exact bit counts */
         if (bad && lane == 0)
             gor(U.status, 2u);
+        psd = U.sdelta;
         if (!more)
             break;
         gu = gn;
-        U = Un;
         in = inn;
     }
-    A.finish();
-    B.finish();
-    C.finish();
+    A.finish(psd, shadow);
+    B.finish(psd, shadow);
+    C.finish(psd, shadow);
 }
 
 /* The codestream dwords packing units share (a segment's first and last
