@@ -412,32 +412,46 @@ __device__ __forceinline__ void vert_raw(const VertFlags& f, const RowRaw<KIND, 
  *   x(2i) += (x(2i-1) + x(2i+1) + 2) >> 2, a missing neighbour being
  * replaced by the other one.  The sample after the thread's last one (first
  * of the next thread) and the one before its first (the previous thread's
- * last, after the predict) come from the neighbours. */
+ * last, after the predict) come from the neighbours.
+ *
+ * The edge tests compare k with rem = n - t * M, the samples of the level
+ * from the thread's first on (may be <= 0 or > M).  With 16 or more
+ * columns per thread, rem is passed through an empty asm statement in each
+ * step, so that the per-sample lane masks are recomputed where they are used
+ * instead of being kept (and spilled out of the scalar registers) from one
+ * step to the next. */
+template <int CH>
+__device__ __forceinline__ int32_t lift_rem(uint32_t n, uint32_t first) {
+    int32_t rem = (int32_t)n - (int32_t)first;
+    if constexpr (CH >= 16)
+        asm volatile("" : "+v"(rem));
+    return rem;
+}
 template <int CH, int S>
 __device__ __forceinline__ void lift_predict(int32_t (&v)[CH], uint32_t t, uint32_t n, int32_t right_first) {
     constexpr int M = CH / S;
+    const int32_t rem = lift_rem<CH>(n, t * M);
 #pragma unroll
     for (int k = 1; k < M; k += 2) {
-        const uint32_t p = t * M + k;
         const int32_t xl = v[(k - 1) * S];
         const int32_t xr = (k + 1 < M) ? v[(k + 1) * S] : right_first;
-        const int32_t r = (p + 1 < n) ? xr : xl;
+        const int32_t r = (k + 1 < rem) ? xr : xl;
         const int32_t nv = v[k * S] - ((xl + r) >> 1);
-        v[k * S] = (p < n) ? nv : v[k * S];
+        v[k * S] = (k < rem) ? nv : v[k * S];
     }
 }
 template <int CH, int S>
 __device__ __forceinline__ void lift_update(int32_t (&v)[CH], uint32_t t, uint32_t n, int32_t left_last) {
     constexpr int M = CH / S;
+    const int32_t rem = lift_rem<CH>(n, t * M);
 #pragma unroll
     for (int k = 0; k < M; k += 2) {
-        const uint32_t p = t * M + k;
         const int32_t hr = v[(k + 1) * S];
         const int32_t hl0 = (k > 0) ? v[(k - 1) * S] : left_last;
-        const int32_t hl = (p > 0) ? hl0 : hr;
-        const int32_t hr2 = (p + 1 < n) ? hr : hl;
+        const int32_t hl = (k > 0 || t > 0) ? hl0 : hr;
+        const int32_t hr2 = (k + 1 < rem) ? hr : hl;
         const int32_t nv = v[k * S] + ((hl + hr2 + 2) >> 2);
-        v[k * S] = (p < n) ? nv : v[k * S];
+        v[k * S] = (k < rem) ? nv : v[k * S];
     }
 }
 
