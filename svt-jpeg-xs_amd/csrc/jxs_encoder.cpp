@@ -283,7 +283,14 @@ bool batch_place_into(const svt_jpeg_xs_amd_frame_spec_t* specs, uint32_t count,
         unit_base += F.ordered ? 0u : g.rows * g.n_units; /* k_enc_slice packs without units */
         FwdJobs jobs;
         fwd_jobs(g, specs[i].d_planes, specs[i].strides, F.coeff, F.gcli, F.smax, F.hist, scratch, jobs);
-        for (int s = 0; s < 2; s++)
+        for (int s = 0; s < 2; s++) {
+            /* the components of a packed plane read the same rows: their
+             * blocks are interleaved when the jobs are alike */
+            bool il = g.packed && s == 0 && jobs.n[s] > 1;
+            for (uint32_t ji = 1; il && ji < jobs.n[s]; ji++) {
+                const JxsDwtJob &a = jobs.dwt[s][0], &b = jobs.dwt[s][ji];
+                il = b.chunk == a.chunk && b.nblk == a.nblk && b.rows == a.rows && b.rpb == a.rpb;
+            }
             for (uint32_t ji = 0; ji < jobs.n[s]; ji++) {
                 JxsDwtJob j = jobs.dwt[s][ji];
                 const int c = j.chunk == 8 ? 0 : j.chunk == 16 ? 1 : j.chunk == 32 ? 2 : -1;
@@ -292,11 +299,14 @@ bool batch_place_into(const svt_jpeg_xs_amd_frame_spec_t* specs, uint32_t count,
                     j.rpb == 0 || fill[s][c] >= sz.njobs[s])
                     return false;
                 j.base = blk[s][c];
+                j.il = il ? jobs.n[s] : 0u;
+                j.il_first = il ? fill[s][c] - ji : 0u;
                 memcpy(img + job_off[s][c] + (uint64_t)fill[s][c] * sizeof(JxsDwtJob), &j, sizeof(j));
                 memcpy(img + base_off[s][c] + (uint64_t)fill[s][c] * 4, &blk[s][c], 4);
                 fill[s][c]++;
                 blk[s][c] += j.nblk;
             }
+        }
     }
     for (int s = 0; s < 2; s++)
         for (int c = 0; c < 3; c++) {

@@ -191,6 +191,11 @@ struct JxsDwtJob {
     uint32_t lf_is_band; /* low rows' low-pass output: band, or int32 plane */
     uint32_t lf_stride;
     uint32_t vec; /* 1: rows readable 16 bytes per load (step 1, plane and row pitch 16-byte aligned); 2 + c: packed 16-bit rows so readable, component c */
+    /* il > 1: jobs il_first .. il_first + il - 1 of the launch's list (the
+     * components of one packed plane) share their blocks interleaved, block
+     * b of the group working on job b % il, row b / il, so that the blocks
+     * reading the same packed rows run together on one XCD */
+    uint32_t il, il_first;
     int32_t* lf_i32;
     JxsBandOut lf, hf[5]; /* low rows: final low-pass and high-pass of each level */
     JxsBandOut lh, hh;    /* high rows: low-pass and high-pass of their level */

@@ -854,9 +854,15 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(CH == 3
                                                     uint32_t* status, uint32_t nstatus) {
     __shared__ DwLds L;
     const uint32_t blk = xcd_remap(blockIdx.x, nblocks);
-    const int j = find_job_u32(bases, njobs, blk);
-    const CAS JxsDwtJob& J = ((const CAS JxsDwtJob*)jobs)[j];
-    const uint32_t bi = blk - J.base;
+    const CAS JxsDwtJob* jl = (const CAS JxsDwtJob*)jobs;
+    const CAS JxsDwtJob* jp = jl + find_job_u32(bases, njobs, blk);
+    uint32_t bi = blk - jp->base;
+    if (jp->il > 1) {
+        const uint32_t il = jp->il, first = jp->il_first, rel = blk - jl[first].base;
+        jp = jl + first + rel % il;
+        bi = rel / il;
+    }
+    const CAS JxsDwtJob& J = *jp;
     if (status && blockIdx.x == 0)
         for (uint32_t k = threadIdx.x; k < nstatus; k += NT)
             gst(status + k, 0u);
