@@ -845,11 +845,15 @@ __device__ __forceinline__ void dwt_slots(const CAS JxsDwtJob& J, uint32_t bi, D
  * of the next vertical level), the high row through one level into the LH
  * and HH bands.  Without a vertical level, input row i through the
  * horizontal levels.  The first launch of a plan also clears the frames'
- * error flags (status, nstatus words).  The 32-column variant is held to
- * 168 VGPRs (3 waves per SIMD instead of 2, no spill): its 8K packed-RGB
- * launch ran 6 % faster that way. */
+ * error flags (status, nstatus words).  Register budgets, for occupancy:
+ * the 32-column variant is held to 168 VGPRs (3 waves per SIMD instead of
+ * 2, no spill: its 8K packed-RGB launch ran 6 % faster), the 16-column one
+ * to 128 (4 waves instead of 3, no spill: 4K second level 10.8 -> 9.3 us per
+ * batched frame), the 8-column one to 64 (8 waves, 4 blocks per CU instead
+ * of 3, at the cost of a few spills: 4K first level 29.9 -> 28.1 us per
+ * plain frame, 23.7 -> 21.5 batched). */
 template <int CH, int NT>
-__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(CH == 32 ? 3 : 1))) k_dwt(const JxsDwtJob* __restrict__ jobs,
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(CH == 32 ? 3 : CH == 16 ? 4 : 8))) k_dwt(const JxsDwtJob* __restrict__ jobs,
                                                     const uint32_t* __restrict__ bases, int njobs, uint32_t nblocks,
                                                     uint32_t* status, uint32_t nstatus) {
     __shared__ DwLds L;
