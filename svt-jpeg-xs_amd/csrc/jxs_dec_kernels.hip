@@ -465,12 +465,15 @@ __device__ __forceinline__ uint64_t pkx_bits64(const PkSrc& S, uint32_t abp) {
 }
 
 /* The passes of one packet (see k_dec_unpack); returns whether the GCLI
- * sub-packet ran out before every code was found. */
-template <bool STAGED>
+ * sub-packet ran out before every code was found.  UNIFORM: the frame's
+ * quantisation is uniform (Qpih 1), else deadzone; a template parameter so
+ * that its tests inside the data phase fold away instead of being kept as
+ * lane masks across the passes. */
+template <bool STAGED, bool UNIFORM>
 __device__ __forceinline__ bool unpack_passes(UnpackCtx& C, const PkSrc& S, const PkPass& P0, uint32_t tid) {
     const CsView& V = S.V;
     const uint32_t nband = P0.nband, total = P0.total, raw = P0.raw, s_bit = P0.s_bit, gend = P0.gend,
-                   d_end = P0.d_end, qpih = P0.qpih;
+                   d_end = P0.d_end, qpih = UNIFORM ? 1u : 0u;
     uint32_t gp = P0.gp, dp = P0.dp;
     uint32_t status = 0;
     bool err = false;
@@ -706,7 +709,7 @@ __device__ __forceinline__ bool unpack_passes(UnpackCtx& C, const PkSrc& S, cons
 /* The passes of a packet too long to stage (read from HBM): a call, so
  * that its registers do not weigh on the staged path. */
 __device__ __noinline__ bool unpack_passes_hbm(UnpackCtx& C, const PkSrc& S, const PkPass& P0, uint32_t tid) {
-    return unpack_passes<false>(C, S, P0, tid);
+    return P0.qpih ? unpack_passes<false, true>(C, S, P0, tid) : unpack_passes<false, false>(C, S, P0, tid);
 }
 
 /* One block per (packet, precinct).  Inside a packet the significance,
@@ -845,9 +848,11 @@ __global__ void __launch_bounds__(PK_THREADS) __attribute__((amdgpu_waves_per_eu
     P0.d_end = __builtin_amdgcn_readfirstlane(C.d_end);
     P0.gp = __builtin_amdgcn_readfirstlane(C.g_bit + V.sh);
     P0.dp = __builtin_amdgcn_readfirstlane(C.d_bit);
-    P0.qpih = g.qpih;
+    P0.qpih = __builtin_amdgcn_readfirstlane(g.qpih);
     const PkSrc S = {V, C.stg, st_lo, st_n};
-    const bool err = st_n ? unpack_passes<true>(C, S, P0, tid) : unpack_passes_hbm(C, S, P0, tid);
+    const bool err = !st_n ? unpack_passes_hbm(C, S, P0, tid)
+        : P0.qpih         ? unpack_passes<true, true>(C, S, P0, tid)
+                          : unpack_passes<true, false>(C, S, P0, tid);
     if (err && tid == 0)
         atomicOr(F.status, 1u);
 }
