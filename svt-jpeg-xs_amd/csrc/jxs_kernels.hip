@@ -2249,11 +2249,11 @@ k_pack(const JxsUnit* __restrict__ units, uint32_t total_units, uint32_t shadow)
  * 0..2: first dword of segment 0..2, 3..5: their last dwords, 6..8: the
  * packet header's dwords), so every thread reads its shadow and codestream
  * words together. */
-#define JOIN_SLOTS 16
+#define JOIN_SLOTS 9 /* 16 (a shift for the unit index) left 7 of every 16 threads idle and ran 8 % longer */
 __global__ void __launch_bounds__(256) k_join(const JxsUnit* __restrict__ units, uint32_t total_units) {
     const uint32_t gid = blockIdx.x * 256 + threadIdx.x;
     const uint32_t u = gid / JOIN_SLOTS, k = gid % JOIN_SLOTS;
-    if (u >= total_units || k >= 9)
+    if (u >= total_units)
         return;
     const GAS JxsUnit& U = ((const GAS JxsUnit*)units)[u];
     uint32_t* p = nullptr;
